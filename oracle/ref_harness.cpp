@@ -1,0 +1,341 @@
+// ref_harness.cpp -- test infrastructure only.
+//
+// A main() of our own, compiled against the reference's ch4/v3 sources where
+// they lie (see oracle/Makefile, target "ref").  It drives the reference's
+// own World / Species / PotentialSolver / MC_MEX_Ionization classes to
+//   golden <dir>   write small input/output vectors (raw float64 files plus
+//                  manifest.txt) that pin the CPU oracle, and
+//   bench ...      time the reference's multithreaded CPU path (threaded
+//                  electron push, Species.cpp:258-355) as the reported CPU
+//                  baseline.
+// Nothing here is reference source; it only calls the reference's API.
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <fstream>
+#include <string>
+#include <vector>
+
+#include "Config.h"
+#include "Interactions.h"
+#include "PotentialSolver.h"
+#include "Rnd.h"
+#include "Species.h"
+#include "World.h"
+
+static std::string g_dir;
+static std::ofstream g_manifest;
+
+static void dump(const std::string& name, const std::vector<double>& v) {
+    std::string path = g_dir + "/" + name + ".bin";
+    FILE* f = fopen(path.c_str(), "wb");
+    if (!f) {
+        fprintf(stderr, "cannot write %s\n", path.c_str());
+        exit(2);
+    }
+    if (!v.empty()) fwrite(v.data(), sizeof(double), v.size(), f);
+    fclose(f);
+    g_manifest << name << " " << v.size() << "\n";
+}
+
+// node field -> flat i-fastest vector
+static std::vector<double> flat(const Field<double>& f) {
+    std::vector<double> v((size_t)f.ni * f.nj * f.nk);
+    for (int i = 0; i < f.ni; i++)
+        for (int j = 0; j < f.nj; j++)
+            for (int k = 0; k < f.nk; k++) v[((size_t)k * f.nj + j) * f.ni + i] = f[i][j][k];
+    return v;
+}
+static std::vector<double> flat3(const Field<type_calc3>& f) {
+    std::vector<double> v((size_t)f.ni * f.nj * f.nk * 3);
+    for (int i = 0; i < f.ni; i++)
+        for (int j = 0; j < f.nj; j++)
+            for (int k = 0; k < f.nk; k++)
+                for (int c = 0; c < 3; c++)
+                    v[3 * (((size_t)k * f.nj + j) * f.ni + i) + c] = f[i][j][k][c];
+    return v;
+}
+static std::vector<double> parts(const Species& sp) {
+    const std::vector<Particle>& p = sp.getConstPartRef();
+    std::vector<double> v(p.size() * 7);
+    for (size_t n = 0; n < p.size(); n++) {
+        for (int c = 0; c < 3; c++) {
+            v[7 * n + c] = p[n].pos[c];
+            v[7 * n + 3 + c] = p[n].vel[c];
+        }
+        v[7 * n + 6] = p[n].macro_weight;
+    }
+    return v;
+}
+static void set_parts(Species& sp, const std::vector<double>& v) {
+    std::vector<Particle>& p = sp.getPartRef();
+    p.clear();
+    for (size_t n = 0; n < v.size() / 7; n++)
+        p.emplace_back(v[7 * n], v[7 * n + 1], v[7 * n + 2], v[7 * n + 3], v[7 * n + 4],
+                       v[7 * n + 5], v[7 * n + 6]);
+}
+
+// exposes the protected running maximum of W*sigma*v_rel
+struct MccProbe : public MC_MEX_Ionization {
+    using MC_MEX_Ionization::MC_MEX_Ionization;
+    double wsv() const { return W_sigma_v_rel_max; }
+};
+
+static void golden(const std::string& dir, const std::string& table_path) {
+    g_dir = dir;
+    g_manifest.open(dir + "/manifest.txt");
+    Config& cfg = Config::getInstance();
+    cfg.setMULTITHREADING(false);  // deterministic FP64 order
+    cfg.setSPUTTERING(false);
+
+    // ---- grid with distinct extents per axis (catches transposed axes) ----
+    const int NI = 9, NJ = 8, NK = 7;
+    type_calc3 x0(-0.01, 0.02, 0.0), xm(0.07, 0.104, 0.054);
+    World world(NI, NJ, NK, x0, xm);
+    world.setTime(2e-9, 10);
+    dump("grid", {(double)NI, (double)NJ, (double)NK, x0[0], x0[1], x0[2], xm[0], xm[1], xm[2],
+                  2e-9});
+    dump("node_vol", flat(world.node_vol));
+
+    // ---- analytic potential -> E (computeEF) ----
+    for (int i = 0; i < NI; i++)
+        for (int j = 0; j < NJ; j++)
+            for (int k = 0; k < NK; k++) {
+                type_calc3 x = world.LtoX(i, j, k);
+                world.phi[i][j][k] = 40.0 * std::sin(31.0 * x[0] + 0.3) * std::cos(23.0 * x[1]) +
+                                     900.0 * x[2] * x[2] - 150.0 * x[0] * x[1] + 12.0 * x[2];
+            }
+    dump("phi_analytic", flat(world.phi));
+    {
+        PotentialSolver ps(world, 10, 1e-3, GS);
+        ps.computeEF();
+    }
+    dump("ef_analytic", flat3(world.ef));
+
+    // ---- loader (mt19937 stream, seeded) ----
+    rnd = Rnd(12345u);
+    Species ele("e-", Const::m_e, -Const::q_e, world, 1.5e5);
+    Species ion("O+", 16 * Const::amu, Const::q_e, world, 2e5);
+    type_calc3 xc = world.getXc(), L = world.getL();
+    ele.loadParticleBoxThermal(xc, type_calc3(L[0] * 0.98, L[1] * 0.98, L[2] * 0.98), 1e12,
+                               2.0 * 11604.52);
+    dump("ele_loaded", parts(ele));
+    rnd = Rnd(777u);
+    ion.loadParticleBoxThermal(xc, type_calc3(L[0] * 0.9, L[1] * 0.9, L[2] * 0.9), 1e12, 0.1 * 11604.52);
+    dump("ion_loaded", parts(ion));
+
+    // ---- push (serial electron push, serial non-electron push) ----
+    double dt = 2e-9;
+    ele.advance_time_multi = 1e30;  // force the serial path
+    ele.advance_time_serial = 0;
+    ele.advanceElectrons(dt);
+    dump("ele_pushed", parts(ele));
+    ion.advanceNonElectron(ion, ion, dt);
+    dump("ion_pushed", parts(ion));
+    // a second, long electron step so that some electrons leave the box
+    std::vector<double> before = parts(ele);
+    dump("ele_pre_long", before);
+    ele.advanceElectrons(4e-8);
+    dump("ele_post_long", parts(ele));
+
+    // ---- density, moments, counts, rho ----
+    ele.computeNumberDensity();
+    ion.computeNumberDensity();
+    dump("ele_den", flat(ele.den));
+    dump("ion_den", flat(ion.den));
+    ele.sampleMoments();
+    ele.computeGasProperties();
+    dump("ele_vel", flat3(ele.vel));
+    dump("ele_T", flat(ele.T));
+    ele.computeMacroParticlesCount();
+    {
+        std::vector<double> mc((size_t)(NI - 1) * (NJ - 1) * (NK - 1));
+        for (int i = 0; i < NI - 1; i++)
+            for (int j = 0; j < NJ - 1; j++)
+                for (int k = 0; k < NK - 1; k++)
+                    mc[((size_t)k * (NJ - 1) + j) * (NI - 1) + i] = ele.macro_part_count[i][j][k];
+        dump("ele_macro_count", mc);
+    }
+    std::vector<Species> sps;
+    sps.push_back(std::move(ele));
+    sps.push_back(std::move(ion));
+    world.computeChargeDensity(sps);
+    dump("rho", flat(world.rho));
+
+    // ---- diagnostics (phi_analytic is still world.phi) ----
+    {
+        std::vector<double> d;
+        for (Species& sp : sps) {
+            type_calc3 m = sp.getMomentum();
+            d.push_back(sp.getMicroCount());
+            d.push_back(m[0]);
+            d.push_back(m[1]);
+            d.push_back(m[2]);
+            d.push_back(sp.getKE());
+            d.push_back(sp.getPE());
+        }
+        d.push_back(world.getPE());
+        dump("diag", d);
+    }
+
+    // ---- v3 GS-SOR with Neumann faces and one rectangle object ----
+    {
+        World w2(NI, NJ, NK, x0, xm);
+        w2.addObject<Rectangle>(type_calc3(0.03, 0.06, 0.027), 5.0, type_calc3(0.02, 0.02, 0.018));
+        w2.computeObjectID();
+        w2.rho = world.rho;
+        dump("gs_object_id", [&] {
+            std::vector<double> v((size_t)NI * NJ * NK);
+            for (int i = 0; i < NI; i++)
+                for (int j = 0; j < NJ; j++)
+                    for (int k = 0; k < NK; k++) v[((size_t)k * NJ + j) * NI + i] = w2.object_id[i][j][k];
+            return v;
+        }());
+        dump("gs_phi_in", flat(w2.phi));
+        PotentialSolver gs(w2, 4000, 1e-4, GS);
+        gs.setReferenceValues(0, 0, 1e20);
+        bool ok = gs.solve();
+        dump("gs_phi_out", flat(w2.phi));
+        dump("gs_ok", {(double)ok});
+    }
+    // ---- PCG (Newton) with Dirichlet walls, linear (n0 = 0) ----
+    {
+        World w3(NI, NJ, NK, x0, xm);
+        w3.dirichletBoundries();
+        w3.rho = world.rho;
+        PotentialSolver pcg(w3, 500, 1e-6, PCG);
+        pcg.setReferenceValues(0, 0, 1e20);
+        bool ok = pcg.solve();
+        dump("pcg_phi_out", flat(w3.phi));
+        dump("pcg_ok", {(double)ok});
+        pcg.computeEF();
+        dump("pcg_ef", flat3(w3.ef));
+    }
+    // ---- PCG (Newton) with Dirichlet walls and Boltzmann electrons ----
+    {
+        World w4(NI, NJ, NK, x0, xm);
+        w4.dirichletBoundries();
+        for (int i = 0; i < NI; i++)
+            for (int j = 0; j < NJ; j++)
+                for (int k = 0; k < NK; k++) w4.rho[i][j][k] = Const::q_e * 1.5e12;
+        PotentialSolver nr(w4, 500, 1e-8, PCG);
+        nr.setReferenceValues(0.0, 1.0e12, 3.0);
+        bool ok = nr.solve();
+        dump("nr_phi_out", flat(w4.phi));
+        dump("nr_ok", {(double)ok});
+    }
+
+    // ---- MC_MEX_Ionization on a small, dense box ----
+    {
+        type_calc3 y0(0.0, 0.0, 0.0), ym(8e-4, 7e-4, 6e-4);
+        World wd(9, 8, 7, y0, ym);
+        double ddt = 1e-10;
+        wd.setTime(ddt, 10);
+        double E_ion_O = 1313.9 * 1000 / Const::N_a;
+        Species neu("O", 16 * Const::amu, 0, wd, 5e11, E_ion_O);
+        Species oi("O+", 16 * Const::amu, Const::q_e, wd, 100);
+        Species el("e-", Const::m_e, -Const::q_e, wd, 100);
+        rnd = Rnd(4242u);
+        type_calc3 c = wd.getXc(), Ld = wd.getL();
+        // 336 cells; ~30 neutrals and ~10 electrons per cell
+        type_calc3 box = Ld * 0.999;
+        double volb = box[0] * box[1] * box[2];
+        neu.loadParticleBoxThermal(c, box, 336.0 * 30 * 5e11 / volb, 300.0);
+        el.loadParticleBoxThermal(c, box, 336.0 * 10 * 100 / volb, 60.0 * 11604.52);
+        dump("mcc_grid", {9, 8, 7, y0[0], y0[1], y0[2], ym[0], ym[1], ym[2], ddt});
+        dump("mcc_neu_in", parts(neu));
+        dump("mcc_ele_in", parts(el));
+        MccProbe mcc(neu, oi, el, wd, table_path);
+        dump("mcc_wsv0", {mcc.wsv()});
+        rnd = Rnd(99u);
+        for (int call = 0; call < 3; call++) {
+            mcc.apply(ddt);
+            std::string s = std::to_string(call);
+            dump("mcc_neu_" + s, parts(neu));
+            dump("mcc_ion_" + s, parts(oi));
+            dump("mcc_ele_" + s, parts(el));
+            dump("mcc_wsv_" + s, {mcc.wsv()});
+        }
+    }
+    g_manifest.close();
+}
+
+// reported CPU baseline: the reference's threaded electron push on a
+// uniform box (and, optionally, its PCG solve), medians over steps.
+static void bench(int n, long np, int steps, int threads, int do_solve) {
+    Config& cfg = Config::getInstance();
+    cfg.setMULTITHREADING(true);
+    cfg.setNUM_THREADS(threads);
+    type_calc3 x0(0, 0, 0);
+    double dx = 1e-2;
+    type_calc3 xm(dx * (n - 1), dx * (n - 1), dx * (n - 1));
+    World world(n, n, n, x0, xm);
+    double dt = 1e-10;
+    world.setTime(dt, steps);
+    world.dirichletBoundries();
+    Species ele("e-", Const::m_e, -Const::q_e, world, 1.0);
+    rnd = Rnd(2024u);
+    type_calc3 L = world.getL();
+    double vol = L[0] * L[1] * L[2];
+    ele.loadParticleBoxThermal(world.getXc(), L, (double)np / vol, 2.0 * 11604.52);
+    // a smooth field so the gather does real work
+    for (int i = 0; i < n; i++)
+        for (int j = 0; j < n; j++)
+            for (int k = 0; k < n; k++) {
+                world.ef[i][j][k][0] = 10.0 * std::sin(0.1 * i);
+                world.ef[i][j][k][1] = 5.0 * std::cos(0.07 * j);
+                world.ef[i][j][k][2] = 1.0;
+            }
+    std::vector<double> t_push, t_den, t_solve;
+    PotentialSolver* ps = nullptr;
+    if (do_solve) ps = new PotentialSolver(world, 20000, 1e-6, PCG);
+    for (int s = 0; s < steps; s++) {
+        size_t nbefore = ele.getNumParticles();
+        ele.advance_time_multi = -1;
+        ele.advance_time_serial = 1e30;
+        auto a = std::chrono::steady_clock::now();
+        ele.advanceElectrons(dt);
+        auto b = std::chrono::steady_clock::now();
+        ele.computeNumberDensity();
+        auto c = std::chrono::steady_clock::now();
+        t_push.push_back(std::chrono::duration<double>(b - a).count() / (double)nbefore);
+        t_den.push_back(std::chrono::duration<double>(c - b).count() / (double)nbefore);
+        if (ps) {
+            world.rho = ele.den;
+            world.rho *= -Const::q_e;
+            auto d = std::chrono::steady_clock::now();
+            ps->solve();
+            auto e = std::chrono::steady_clock::now();
+            t_solve.push_back(std::chrono::duration<double>(e - d).count());
+        }
+    }
+    auto med = [](std::vector<double> v) {
+        if (v.empty()) return 0.0;
+        std::sort(v.begin(), v.end());
+        return v[v.size() / 2];
+    };
+    printf("{\"n\": %d, \"particles\": %zu, \"steps\": %d, \"threads\": %d, "
+           "\"push_s_per_particle\": %.6e, \"deposit_s_per_particle\": %.6e, "
+           "\"pushes_per_s\": %.6e, \"solve_s\": %.6e}\n",
+           n, ele.getNumParticles(), steps, threads, med(t_push), med(t_den), 1.0 / med(t_push),
+           med(t_solve));
+    delete ps;
+}
+
+int main(int argc, char** argv) {
+    if (argc >= 4 && std::string(argv[1]) == "golden") {
+        golden(argv[2], argv[3]);
+        return 0;
+    }
+    if (argc >= 7 && std::string(argv[1]) == "bench") {
+        bench(atoi(argv[2]), atol(argv[3]), atoi(argv[4]), atoi(argv[5]), atoi(argv[6]));
+        return 0;
+    }
+    fprintf(stderr,
+            "usage: ref_harness golden <outdir> <sigma_el_table>\n"
+            "       ref_harness bench <nodes_per_axis> <particles> <steps> <threads> <solve0|1>\n");
+    return 1;
+}
