@@ -1,0 +1,265 @@
+#!/usr/bin/env python3
+"""PIC step throughput on one MI355X.
+
+Workload (the scaled plain-PIC configuration): 128^3 nodes, dx = 1e-2 m,
+Dirichlet phi = 0 walls, kinetic e- (Te = 2 eV) and O+ (16 amu, Ti = 0.1 eV),
+20M macro-particles each, uniform positions, Maxwellian velocities, uniform
+weight w = n0 L^3 / N with n0 = 1e12 m^-3, dt = 1e-10 s.  One step is
+  push e-, push O+  (CIC gather + kick + drift + absorb + sort histogram)
+  sort e-, sort O+  (counting sort by cell, every step)
+  deposit e-, O+    (number density), charge density,
+  Poisson solve to a 1e-6 relative residual (FP64 residual, FP32 inner PCG,
+  warm-started from the previous phi), E-field.
+value = particle pushes per second over the K timed steps, inputs resident
+in HBM.  Also reported: per-kernel HIP-event times, the roofline of the
+dominant kernel, cold-start Poisson time to 1e-6 for each solver, and the
+reference's own multithreaded CPU path timed on this host.
+"""
+import argparse
+import json
+import os
+import subprocess
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+PKG = os.path.join(ROOT, "engineering-degree-in-plasma-simulations_amd")
+sys.path.insert(0, PKG)
+
+QE = 1.602176565e-19
+ME = 9.10938215e-31
+AMU = 1.660538921e-27
+EV_K = 11604.52
+HBM_PEAK_GBS = 8000.0
+
+# algorithmic bytes per unit of work (per particle, or per node and iteration)
+ALG_BYTES = {
+    "push": 48.0,      # read + write x, y, z, vx, vy, vz (FP32)
+    "scatter": 64.0,   # counting-sort scatter: key r/w + 28 B SoA read + 28 B write
+    "deposit": 16.0,   # x, y, z, w
+    "pcg": 44.0,       # FP32 inner PCG: per node and iteration (88 B in FP64)
+    "sor": 24.0,       # FP64 phi r/w + rho read per node and sweep
+}
+KERNEL_NAMES = {
+    "push": "k_push", "scatter": "k_scatter", "deposit": "k_deposit<1>",
+    "pcg": "k_pcg1<float> + k_pcg2<float>", "sor": "k_sor",
+}
+
+
+def cpu_baseline(threads, particles, steps):
+    """The reference's threaded electron push + serial deposit on this host."""
+    harness = os.path.join(ROOT, "oracle", "_ref", "ref_harness")
+    sample = "%d electrons on the 128^3 grid, %d steps, threaded push + deposit" % (particles,
+                                                                                   steps)
+    if os.path.exists(harness):
+        t0 = time.time()
+        out = subprocess.run([harness, "bench", "128", str(particles), str(steps), str(threads),
+                              "0"], capture_output=True, text=True, timeout=600)
+        line = [ln for ln in out.stdout.splitlines() if ln.startswith("{")][-1]
+        r = json.loads(line)
+        per_particle = r["push_s_per_particle"] + r["deposit_s_per_particle"]
+        return {"value": 1.0 / per_particle, "unit": "particle pushes/s", "cores": threads,
+                "kind": "reference", "sample": sample,
+                "push_only_pushes_per_s": r["pushes_per_s"], "wall_s": round(time.time() - t0, 1)}
+    # the oracle's v3-structured threaded push (reference binary not built)
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+
+    n = 128
+    g = O.make_grid(n, n, n, [0, 0, 0], [1.27, 1.27, 1.27])
+    rng = np.random.default_rng(3)
+    p = np.zeros((particles, 7))
+    p[:, :3] = rng.random((particles, 3)) * 1.27 * 0.98 + 0.0127
+    p[:, 3:6] = rng.normal(0, 5e5, (particles, 3))
+    p[:, 6] = 1.0
+    p = p.reshape(-1)
+    ef = np.zeros(3 * O.nn(g))
+    t0 = time.time()
+    for _ in range(steps):
+        O.push_threaded(g, ef, p, -QE, ME, 1e-10, threads)
+    dt = time.time() - t0
+    return {"value": particles * steps / dt, "unit": "particle pushes/s", "cores": threads,
+            "kind": "port", "sample": sample.replace("push + deposit", "push")}
+
+
+def copy_calibration():
+    """A plain copy of known size, for scaling FETCH_SIZE / WRITE_SIZE counters."""
+    import torch
+
+    n = 256 * 1024 * 1024 // 4
+    a = torch.ones(n, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    for _ in range(5):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_bytes_read": 4 * n, "copy_bytes_written": 4 * n}))
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--particles", type=int, default=20_000_000, help="per species")
+    ap.add_argument("--nodes", type=int, default=128)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-poisson-compare", action="store_true")
+    ap.add_argument("--cpu-particles", type=int, default=8_000_000)
+    ap.add_argument("--cpu-steps", type=int, default=8)
+    ap.add_argument("--copy-calibration", action="store_true")
+    args = ap.parse_args()
+    if args.copy_calibration:
+        copy_calibration()
+        return 0
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py measures one GPU; this build has no multi-GPU path (--gpus %d refused)"
+              % args.gpus, file=sys.stderr)
+        return 2
+
+    cpu = None
+    if not args.no_cpu_baseline:  # before the GPU is initialised
+        threads = max(1, min(16, os.cpu_count() or 1))
+        cpu = cpu_baseline(threads, args.cpu_particles, args.cpu_steps)
+        print("cpu baseline: %.3e pushes/s (%s, %d threads)" % (cpu["value"], cpu["kind"],
+                                                                 threads), file=sys.stderr)
+
+    import pic_gpu as pg
+
+    n = args.nodes
+    dx = 1e-2
+    L = dx * (n - 1)
+    x0 = np.zeros(3)
+    xm = np.full(3, L)
+    N = args.particles
+    n0 = 1e12
+    w = n0 * L ** 3 / N
+    dt = 1e-10
+    ctx = pg.Context(n, n, n, x0, xm)
+    ctx.dirichlet_walls()
+    ctx.set_dt(dt)
+    cap = int(N * 1.02) + 1024
+    se = ctx.species_add(ME, -QE, w, cap)
+    si = ctx.species_add(16 * AMU, QE, w, cap)
+    centre = 0.5 * (x0 + xm)
+    ctx.load_box_thermal(se, centre, xm - x0, n0, 2.0 * EV_K, seed=11, mode=1)
+    ctx.load_box_thermal(si, centre, xm - x0, n0, 0.1 * EV_K, seed=12, mode=1)
+    species = (se, si)
+    for s in species:
+        ctx.sort(s)
+        ctx.deposit(s)
+    ctx.charge_density()
+    solve_kw = dict(method=pg.SOLVER_PCG_MIXED, tol=1e-6, tol_kind=pg.TOL_REL, max_it=20000)
+    ctx.poisson(**solve_kw)
+    ctx.efield()
+    print("loaded %d + %d particles" % (ctx.count(se), ctx.count(si)), file=sys.stderr)
+
+    poisson = None
+    if not args.no_poisson_compare:
+        phi_keep = ctx.get_field(pg.FIELD_PHI)
+        zeros = np.zeros(ctx.nn)
+        poisson = {"grid": "%d^3" % n, "start": "phi = 0", "tol_rel": 1e-6}
+        for name, kw in (("pcg_fp64", dict(method=pg.SOLVER_PCG)),
+                         ("pcg_mixed", dict(method=pg.SOLVER_PCG_MIXED)),
+                         ("sor_rb", dict(method=pg.SOLVER_SOR_RB, check_every=50,
+                                         omega=2.0 / (1.0 + np.sin(np.pi / (n - 1)))))):
+            ctx.set_field(pg.FIELD_PHI, zeros)
+            st = ctx.poisson(tol=1e-6, tol_kind=pg.TOL_REL, max_it=20000, **kw)
+            poisson[name] = {"seconds": round(st["seconds"], 5), "iterations": st["iterations"],
+                             "rel_residual": st["rel_residual"], "converged": st["converged"]}
+            print("poisson %s: %.4f s, %d it, rel %.2e" % (name, st["seconds"], st["iterations"],
+                                                           st["rel_residual"]), file=sys.stderr)
+        poisson["learned"] = None
+        ctx.set_field(pg.FIELD_PHI, phi_keep)
+
+    solve_iters = []
+
+    def step():
+        for s in species:
+            ctx.push(s, dt)
+        for s in species:
+            ctx.sort(s)
+        for s in species:
+            ctx.deposit(s)
+        ctx.charge_density()
+        st = ctx.poisson(**solve_kw)
+        if not st["converged"]:
+            raise RuntimeError("Poisson solve did not converge: %r" % st)
+        solve_iters.append(st["iterations"])
+        ctx.efield()
+
+    for _ in range(args.warmup):
+        step()
+    ctx.sync()
+    ctx.profile(True)
+    ctx.profile_reset()
+    solve_iters.clear()
+    ctx.sync()
+    t0 = time.perf_counter()
+    for k in range(args.steps):
+        step()
+    ctx.sync()
+    t1 = time.perf_counter()
+    prof = ctx.profile_read()
+    ctx.profile(False)
+    T = t1 - t0
+    K = args.steps
+    n_alive = ctx.count(se) + ctx.count(si)
+    value = 2 * N * K / T  # pushes issued (both species, every step)
+
+    stages = {k: {"ms_per_step": round(v[0] / K, 4), "launches": v[1]}
+              for k, v in prof.items() if v[1]}
+    # dominant kernel by total device time among the bandwidth kernels
+    cand = {k: v for k, v in prof.items() if k in ALG_BYTES and v[1]}
+    dom = max(cand, key=lambda k: cand[k][0])
+    tot_ms, launches = cand[dom]
+    avg_s = tot_ms * 1e-3 / launches
+    if dom == "pcg":
+        units = ctx.nn
+    elif dom == "sor":
+        units = ctx.nn / 2.0
+    else:
+        units = N
+    alg = ALG_BYTES[dom] * units
+    achieved = alg / avg_s / 1e9
+    traffic = None
+    tpath = os.path.join(ROOT, "profiles", "pmc_traffic.json")
+    if os.path.exists(tpath):
+        t = json.load(open(tpath))
+        traffic = t.get(dom, {}).get("bytes_per_launch")
+
+    line = {
+        "metric": "particle pushes/s + time-to-1e-6 Poisson residual, learned vs plain, 1 GPU",
+        "value": value,
+        "unit": "particle pushes/s",
+        "n_gpus": 1,
+        "steps": K,
+        "warmup": args.warmup,
+        "ms_per_step": T / K * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32 particles, f64 Poisson residual",
+        "data": "synthetic (uniform positions, Maxwellian velocities, seeded Philox)",
+        "config": {"workload": "scaled plain PIC, %d^3 nodes, 2 x %d particles, counting sort "
+                               "every step, Poisson to 1e-6 rel. residual every step" % (n, N),
+                   "grid_nodes": n ** 3, "particles_per_species": N, "dt": dt,
+                   "poisson_step_solver": "pcg_mixed (FP64 residual, FP32 inner PCG)"},
+        "particles_alive": n_alive,
+        "poisson_iterations_per_step": float(np.mean(solve_iters)) if solve_iters else None,
+        "stages": stages,
+        "roofline": {"bound": "hbm", "kernel": KERNEL_NAMES[dom], "achieved": achieved,
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": traffic, "alg_bytes_per_launch": alg,
+                     "avg_launch_ms": avg_s * 1e3},
+        "poisson": poisson,
+        "cpu_baseline": cpu,
+    }
+    ctx.close()
+    print(json.dumps(line))
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

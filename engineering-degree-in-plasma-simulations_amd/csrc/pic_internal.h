@@ -1,0 +1,234 @@
+// pic_internal.h -- shared definitions of the gfx950 PIC/DSMC library.
+//
+// Device data layout (HBM):
+//   node fields      flat, i-fastest, u = (k*nj + j)*ni + i
+//   phi, rho         double[nn]
+//   E                float4[nn] {Ex, Ey, Ez, 0}: one 16-byte load per CIC corner
+//   node type        uint8[nn]  (NT_* below; matrix-free operator rows)
+//   particles (SoA)  cell  uint32 packed (i | j<<11 | k<<22), DEAD when removed
+//                    fx, fy, fz float in [0,1): offset inside the cell, in cells
+//                    vx, vy, vz float [m/s], w float (macro weight)
+//   Storing the cell and the in-cell offset gives the same sub-cell precision
+//   anywhere in the box, makes the logical coordinate (World::XtoL,
+//   World.cpp:123-127) free, and the cell word is the counting-sort key.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <string>
+#include <vector>
+
+#include "../../include/pic_abi.h"
+
+namespace pic {
+
+constexpr double EPS0 = 8.85418782e-12;  // all.h:15
+constexpr double QE = 1.602176565e-19;   // all.h:16
+constexpr double ME = 9.10938215e-31;    // all.h:18
+constexpr double KB = 1.380648e-23;      // all.h:19
+constexpr double PI_REF = 3.141592653;   // all.h:20 (truncated, kept for parity)
+constexpr double TWO_PI = 6.283185307179586;
+
+constexpr uint32_t DEAD = 0xFFFFFFFFu;
+constexpr int MAX_RECT = 8;
+constexpr int MAX_SPECIES = 8;
+
+// node types of the matrix-free operator (PotentialSolver.cpp:414-471)
+enum : uint8_t {
+    NT_REG = 0,
+    NT_NEU_IM = 1,  // i == 0     -> neighbour +1
+    NT_NEU_IP = 2,  // i == ni-1  -> neighbour -1
+    NT_NEU_JM = 3,
+    NT_NEU_JP = 4,
+    NT_NEU_KM = 5,
+    NT_NEU_KP = 6,
+    NT_DIR = 7,
+};
+
+struct GridC {
+    int ni, nj, nk;   // nodes
+    int ci, cj, ck;   // cells
+    int ncells;
+    long long nn;
+    double x0[3], dx[3], inv_dx[3];
+    int nrect;
+    float rlo[MAX_RECT][3], rhi[MAX_RECT][3];  // rectangles in logical coordinates
+};
+
+__host__ __device__ inline uint32_t pack_cell(int i, int j, int k) {
+    return (uint32_t)i | ((uint32_t)j << 11) | ((uint32_t)k << 22);
+}
+__host__ __device__ inline void unpack_cell(uint32_t c, int& i, int& j, int& k) {
+    i = (int)(c & 0x7FFu);
+    j = (int)((c >> 11) & 0x7FFu);
+    k = (int)(c >> 22);
+}
+__host__ __device__ inline int cell_index(const GridC& g, int i, int j, int k) {
+    return (k * g.cj + j) * g.ci + i;
+}
+// 1 / node volume (World.cpp:353-367): half volume per boundary axis
+__host__ __device__ inline double inv_node_vol(const GridC& g, int i, int j, int k) {
+    double s = g.inv_dx[0] * g.inv_dx[1] * g.inv_dx[2];
+    if (i == 0 || i == g.ni - 1) s *= 2.0;
+    if (j == 0 || j == g.nj - 1) s *= 2.0;
+    if (k == 0 || k == g.nk - 1) s *= 2.0;
+    return s;
+}
+
+// Philox4x32-10 (Salmon, Moraes, Dror, Shaw, SC'11), counter-based
+__host__ __device__ inline void philox4x32(uint32_t c0, uint32_t c1, uint32_t c2, uint32_t c3,
+                                           uint32_t k0, uint32_t k1, uint32_t out[4]) {
+#pragma unroll
+    for (int r = 0; r < 10; r++) {
+        uint64_t p0 = (uint64_t)0xD2511F53u * c0;
+        uint64_t p1 = (uint64_t)0xCD9E8D57u * c2;
+        uint32_t n0 = (uint32_t)(p1 >> 32) ^ c1 ^ k0;
+        uint32_t n2 = (uint32_t)(p0 >> 32) ^ c3 ^ k1;
+        c1 = (uint32_t)p1;
+        c3 = (uint32_t)p0;
+        c0 = n0;
+        c2 = n2;
+        k0 += 0x9E3779B9u;
+        k1 += 0xBB67AE85u;
+    }
+    out[0] = c0;
+    out[1] = c1;
+    out[2] = c2;
+    out[3] = c3;
+}
+__host__ __device__ inline double u53(uint32_t a, uint32_t b) {
+    return ((double)(a >> 5) * 67108864.0 + (double)(b >> 6)) * (1.0 / 9007199254740992.0);
+}
+// draw `slot` of the stream addressed by (a, b): block slot/2, half slot%2
+struct Draws {
+    uint32_t a, b, k0, k1, tag;
+    uint32_t cache[4];
+    int cached_block;
+    __host__ __device__ void init(uint32_t a_, uint32_t b_, uint32_t k0_, uint32_t k1_,
+                                  uint32_t tag_) {
+        a = a_;
+        b = b_;
+        k0 = k0_;
+        k1 = k1_;
+        tag = tag_;
+        cached_block = -1;
+    }
+    __host__ __device__ double operator()(int slot) {
+        int blk = slot >> 1;
+        if (blk != cached_block) {
+            philox4x32(a, b, (uint32_t)blk, tag, k0, k1, cache);
+            cached_block = blk;
+        }
+        int h = slot & 1;
+        return u53(cache[2 * h], cache[2 * h + 1]);
+    }
+};
+
+constexpr uint32_t TAG_LOAD = 0x4C4F4144u;  // stream tags
+constexpr uint32_t TAG_MCC = 0x4D434301u;
+
+// one species: SoA particle arrays (two buffer sets for the sort) + grids
+struct SpeciesBuf {
+    uint32_t* cell;
+    float* f[3];
+    float* v[3];
+    float* w;
+};
+
+struct Species {
+    double mass = 0, charge = 0, mpw0 = 0;
+    size_t cap = 0;
+    SpeciesBuf buf[2] = {};
+    int cur = 0;
+    void* block[2] = {nullptr, nullptr};
+    uint32_t* rank = nullptr;    // rank of each particle inside its cell bucket
+    uint32_t* count = nullptr;   // ncells + 1 buckets (last = removed)
+    uint32_t* off = nullptr;     // ncells + 2 exclusive offsets
+    uint32_t* d_n = nullptr;     // [0] live count, [1] count the histogram covers, [2] overflow
+    size_t n_upper = 0;          // host upper bound of the live count
+    bool hist_valid = false;     // rank/count match the current arrays
+    bool sorted = false;         // arrays ordered by cell, off valid
+    double* den = nullptr;       // number density
+    double* mom = nullptr;       // n, nvx, nvy, nvz, nuu, nvv, nww (7 * nn)
+    uint32_t seq = 0;            // loader stream id
+};
+
+struct Mcc {
+    int sn, si, se;
+    pic_mcc_desc desc;
+    double* sig = nullptr;  // nsig energies followed by nsig values
+    int nsig = 0;
+    double* d_wsv = nullptr;  // [0] current max, [1] step max (as bits for atomicMax)
+    unsigned long long* d_stats = nullptr;  // collisions, ionizations, groups, overflow
+    double wsv_host = 0;
+};
+
+struct Prof {
+    bool on = false;
+    struct Rec {
+        int cat;
+        hipEvent_t a, b;
+    };
+    std::vector<Rec> recs;
+    std::vector<hipEvent_t> pool;
+    double ms[PIC_K_COUNT] = {};
+    long long n[PIC_K_COUNT] = {};
+};
+
+}  // namespace pic
+
+struct pic_ctx {
+    pic::GridC g;
+    hipStream_t stream = nullptr;
+    std::string err;
+    double dt = 1e-4;  // World.h:25 default
+    // host mirrors of the geometry flags (i-fastest)
+    std::vector<int32_t> object_id, node_type_in;
+    std::vector<double> phi_fixed;
+    struct RectH {
+        double c[3], half[3], phi;
+    };
+    std::vector<RectH> rects;
+    bool ntype_dirty = true;
+    // device fields
+    double* phi = nullptr;
+    double* rho = nullptr;
+    float4* ef = nullptr;
+    uint8_t* ntype = nullptr;
+    double* scratch = nullptr;   // reductions and solver scalars
+    uint32_t* scan_tmp = nullptr; // block sums of the offset scan
+    double* work = nullptr;      // solver vectors (allocated on demand)
+    size_t work_bytes = 0;
+    std::vector<pic::Species> sp;
+    std::vector<pic::Mcc> mcc;
+    pic::Prof prof;
+    void* unet = nullptr;  // learned preconditioner state
+};
+
+// ---- host helpers shared between translation units ----
+int pic_fail(pic_ctx* ctx, int code, const std::string& msg);
+int pic_hip_check(pic_ctx* ctx, hipError_t e, const char* what);
+#define PIC_HIP(ctx, call)                                        \
+    do {                                                          \
+        hipError_t e_ = (call);                                   \
+        if (e_ != hipSuccess) return pic_hip_check(ctx, e_, #call); \
+    } while (0)
+// profiling brackets around a launch
+void pic_prof_begin(pic_ctx* ctx, int cat, hipEvent_t* a, hipEvent_t* b);
+void pic_prof_end(pic_ctx* ctx, int cat, hipEvent_t a, hipEvent_t b);
+struct PicProfScope {
+    pic_ctx* ctx;
+    int cat;
+    hipEvent_t a = nullptr, b = nullptr;
+    PicProfScope(pic_ctx* c, int k) : ctx(c), cat(k) { pic_prof_begin(ctx, cat, &a, &b); }
+    ~PicProfScope() { pic_prof_end(ctx, cat, a, b); }
+};
+int pic_check_sid(pic_ctx* ctx, int sid);
+int pic_ensure_ntype(pic_ctx* ctx);
+int pic_ensure_work(pic_ctx* ctx, size_t bytes);
+// histogram + ranks of the current arrays (run-aggregated atomics)
+int pic_histogram(pic_ctx* ctx, int sid);
+inline unsigned grid_for(size_t n, unsigned block) {
+    size_t g = (n + block - 1) / block;
+    return (unsigned)(g ? g : 1);
+}

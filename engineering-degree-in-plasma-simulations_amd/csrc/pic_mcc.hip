@@ -1,0 +1,366 @@
+// pic_mcc.hip -- variable-weight electron-neutral Monte-Carlo collisions with
+// ionization (MC_MEX_Ionization, Interactions.cpp:476-980) on gfx950.
+//
+// One thread owns one cell and runs that cell's No-Time-Counter group loop
+// sequentially, exactly as apply_vector_indexes does (Interactions.cpp:
+// 627-748); cells are independent, so the grid of cells is the parallel
+// dimension.  Particles are read from the cell-sorted SoA arrays of the three
+// species; weights and velocities are updated in place; births (ions,
+// electrons, split neutrals) are appended at the tail of each species through
+// an atomic counter.  All collision arithmetic is FP64.
+//
+// Draws: Philox4x32-10, key (seed, step), counter (cell, group, slot/2, tag);
+// slots 0 neutral pick, 1 electron pick, 2 accept, 3 ionize test, 4.. the
+// scattering angles and the ejected energy (same order as the reference's
+// sequential rnd() calls, Interactions.cpp:656-678, :895-944).
+#include <algorithm>
+#include <cmath>
+
+#include "pic_internal.h"
+
+using namespace pic;
+
+namespace {
+
+constexpr int BLOCK = 128;
+
+struct MccArgs {
+    double m_n, m_e, sum_m, m_red;
+    double E_pre_rel;  // 0.5 m_red / q          (Interactions.cpp:501)
+    double E_pre_ele;  // m_e 0.5 / q            (Interactions.cpp:123)
+    double two_q_me;   // 2 q / m_e              (Interactions.cpp:124)
+    double E_ion_eV, c0, c1, c2, B_inc;
+    double dt_inv_dv;
+    uint32_t seed, step;
+    int nsig;
+};
+
+struct Sp {
+    SpeciesBuf b;
+    uint32_t* d_n;
+    uint32_t cap;
+};
+
+// Interactions.cpp:541-558: std::map::lower_bound + linear interpolation
+__device__ inline double sigma_el(const double* __restrict__ tE, const double* __restrict__ tV,
+                                  int n, double E) {
+    int lo = 0, hi = n;
+    while (lo < hi) {
+        int mid = (lo + hi) >> 1;
+        if (tE[mid] < E)
+            lo = mid + 1;
+        else
+            hi = mid;
+    }
+    if (lo == 0) return tV[0];
+    if (lo == n) return tV[n - 1];
+    double x1 = tE[lo - 1], x2 = tE[lo], y1 = tV[lo - 1], y2 = tV[lo];
+    return y1 + (E - x1) * (y2 - y1) / (x2 - x1);
+}
+// Interactions.cpp:559-566
+__device__ inline double sigma_iz(const MccArgs& a, double E) {
+    if (E <= a.E_ion_eV) return 0;
+    return a.c0 * log(E / a.c1) / E * exp(-a.c2 / E);
+}
+
+__device__ inline void cross3(const double* x, const double* y, double* o) {
+    o[0] = x[1] * y[2] - x[2] * y[1];
+    o[1] = x[2] * y[0] - x[0] * y[2];
+    o[2] = x[0] * y[1] - x[1] * y[0];
+}
+
+// Interactions.cpp:845-870 (the basis i x u, u x (i x u) is not normalised
+// in the reference; kept as is)
+__device__ inline void new_vel_electron(const MccArgs& a, double* vel, double E, const double* u,
+                                        double U1, double U2) {
+    double cos_ksi = (2 + E - 2 * pow(1 + E, U1)) / E;
+    double sin_ksi = sqrt(1 - cos_ksi * cos_ksi);
+    double phi = 2 * PI_REF * U2;
+    double v_mag = sqrt(E * a.two_q_me);
+    const double ix[3] = {1, 0, 0};
+    double ic[3], uic[3];
+    cross3(ix, u, ic);
+    cross3(u, ic, uic);
+    double sp = sin(phi), cp = cos(phi);
+#pragma unroll
+    for (int d = 0; d < 3; d++) vel[d] = (u[d] * cos_ksi + ic[d] * sin_ksi * sp + uic[d] * sin_ksi * cp) * v_mag;
+}
+
+// Interactions.cpp:885-947 (IONIZE_1).  vn is not modified.
+__device__ inline bool collide(const MccArgs& a, Draws& dr, const double* vn, double* ve,
+                               double* vnew, double sigma_coll) {
+    double vr[3] = {vn[0] - ve[0], vn[1] - ve[1], vn[2] - ve[2]};
+    double vrm = sqrt(vr[0] * vr[0] + vr[1] * vr[1] + vr[2] * vr[2]);
+    double E_rel_J = 0.5 * a.m_red * vrm * vrm;
+    double P = sigma_iz(a, E_rel_J / QE) / sigma_coll;
+    if (dr(3) <= P) {
+        double vem = sqrt(ve[0] * ve[0] + ve[1] * ve[1] + ve[2] * ve[2]);
+        double E_inc = vem * vem * a.E_pre_ele;
+        if (E_inc < a.E_ion_eV) return false;  // Interactions.cpp:901-905
+        double E_ej = 10.0 * tan(dr(4) * atan((E_inc - a.E_ion_eV) / (2 * a.B_inc)));
+        double E_sc = E_inc - a.E_ion_eV - E_ej;
+        if (E_sc < 0) E_sc = 0.000001;
+        double inv = 1.0 / vem;
+        double u[3] = {ve[0] * inv, ve[1] * inv, ve[2] * inv};
+        double U5 = dr(5), U6 = dr(6);
+        new_vel_electron(a, ve, E_sc, u, U5, U6);
+        double U7 = dr(7), U8 = dr(8);
+        new_vel_electron(a, vnew, E_ej, u, U7, U8);
+        return true;
+    }
+    double inv = 1.0 / a.sum_m;
+    double cm[3];
+#pragma unroll
+    for (int d = 0; d < 3; d++) cm[d] = (vn[d] * a.m_n + ve[d] * a.m_e) * inv;
+    double cos_ksi = 2 * dr(4) - 1;
+    double sin_ksi = sqrt(1 - cos_ksi * cos_ksi);
+    double eps = 2 * PI_REF * dr(5);
+    double v2[3] = {vrm * cos_ksi, vrm * sin_ksi * cos(eps), vrm * sin_ksi * sin(eps)};
+    double f = a.m_n / a.sum_m;
+#pragma unroll
+    for (int d = 0; d < 3; d++) ve[d] = cm[d] - v2[d] * f;
+    return false;
+}
+
+__device__ inline void append(Sp& s, uint32_t cell, float fx, float fy, float fz, const double* v,
+                              double w, unsigned long long* overflow) {
+    uint32_t idx = atomicAdd(&s.d_n[0], 1u);
+    if (idx >= s.cap) {
+        atomicAdd(overflow, 1ull);
+        return;
+    }
+    s.b.cell[idx] = cell;
+    s.b.f[0][idx] = fx;
+    s.b.f[1][idx] = fy;
+    s.b.f[2][idx] = fz;
+    s.b.v[0][idx] = (float)v[0];
+    s.b.v[1][idx] = (float)v[1];
+    s.b.v[2][idx] = (float)v[2];
+    s.b.w[idx] = (float)w;
+}
+
+__global__ void __launch_bounds__(BLOCK)
+    k_mcc(GridC g, MccArgs a, Sp N, const uint32_t* __restrict__ offN, Sp I, Sp E,
+          const uint32_t* __restrict__ offE, const double* __restrict__ table,
+          double* __restrict__ wsv, unsigned long long* __restrict__ stats) {
+    int c = blockIdx.x * BLOCK + threadIdx.x;
+    double wsv_tmp = 0;
+    unsigned long long ncoll = 0, niz = 0, ngr = 0;
+    if (c < g.ncells) {
+        uint32_t e0 = offE[c], e1 = offE[c + 1];
+        uint32_t n0 = offN[c], n1 = offN[c + 1];
+        int Ne = (int)(e1 - e0), Nn = (int)(n1 - n0);
+        if (Ne > 0 && Nn > 0) {
+            const double wmax = wsv[0];
+            double frac = (double)(Nn * Ne) * wmax * a.dt_inv_dv;  // Interactions.cpp:646
+            int n_groups = (int)(frac + 0.5);
+            if (n_groups > Nn) n_groups = Nn - 1;
+            const double* tE = table;
+            const double* tV = table + a.nsig;
+            for (int gi = 0; gi < n_groups; gi++) {
+                Draws dr;
+                dr.init((uint32_t)c, (uint32_t)gi, a.seed, a.step, TAG_MCC);
+                ngr++;
+                int ns = (int)(dr(0) * (double)Nn);
+                int es = (int)(dr(1) * (double)Ne);
+                uint32_t pn = n0 + (uint32_t)ns, pe = e0 + (uint32_t)es;
+                double wn = N.b.w[pn], we = E.b.w[pe];
+                if (wn == 0.0) continue;  // neutral removed earlier in this cell
+                double vn[3] = {N.b.v[0][pn], N.b.v[1][pn], N.b.v[2][pn]};
+                double ve[3] = {E.b.v[0][pe], E.b.v[1][pe], E.b.v[2][pe]};
+                double vr[3] = {vn[0] - ve[0], vn[1] - ve[1], vn[2] - ve[2]};
+                double v_rel = sqrt(vr[0] * vr[0] + vr[1] * vr[1] + vr[2] * vr[2]);
+                double E_rel = a.E_pre_rel * v_rel * v_rel;
+                double sigma = sigma_el(tE, tV, a.nsig, E_rel);
+                double Wg = wn > we ? wn : we;
+                double wsv_g = Wg * sigma * v_rel;
+                if (wsv_g > wsv_tmp) wsv_tmp = wsv_g;
+                double P = wsv_g / wmax;
+                if (!(dr(2) < P)) continue;
+                ncoll++;
+                uint32_t ncell = N.b.cell[pn];
+                float nfx = N.b.f[0][pn], nfy = N.b.f[1][pn], nfz = N.b.f[2][pn];
+                double vnew[3] = {0, 0, 0};
+                if (wn > we) {  // Interactions.cpp:684-703
+                    N.b.w[pn] = (float)(wn - we);
+                    bool iz = collide(a, dr, vn, ve, vnew, sigma);
+                    if (iz) {
+                        niz++;
+                        append(I, ncell, nfx, nfy, nfz, vn, we, &stats[3]);
+                        append(E, ncell, nfx, nfy, nfz, vnew, we, &stats[3]);
+                    } else {
+                        append(N, ncell, nfx, nfy, nfz, vn, we, &stats[3]);
+                    }
+                } else if (wn < we) {  // Interactions.cpp:705-726, defined semantics
+                    E.b.w[pe] = (float)wn;
+                    append(E, E.b.cell[pe], E.b.f[0][pe], E.b.f[1][pe], E.b.f[2][pe], ve, we - wn,
+                           &stats[3]);
+                    bool iz = collide(a, dr, vn, ve, vnew, sigma);
+                    if (iz) {
+                        niz++;
+                        append(I, ncell, nfx, nfy, nfz, vn, wn, &stats[3]);
+                        append(E, ncell, nfx, nfy, nfz, vnew, wn, &stats[3]);
+                        N.b.w[pn] = 0.0f;
+                        N.b.cell[pn] = DEAD;
+                    }
+                }
+                E.b.v[0][pe] = (float)ve[0];
+                E.b.v[1][pe] = (float)ve[1];
+                E.b.v[2][pe] = (float)ve[2];
+            }
+        }
+    }
+    // block reduction of the counters and the step maximum
+    __shared__ double smax[BLOCK / 64];
+    __shared__ unsigned long long scnt[BLOCK / 64][3];
+    for (int o = 32; o > 0; o >>= 1) {
+        wsv_tmp = fmax(wsv_tmp, __shfl_xor(wsv_tmp, o, 64));
+        ncoll += __shfl_xor(ncoll, o, 64);
+        niz += __shfl_xor(niz, o, 64);
+        ngr += __shfl_xor(ngr, o, 64);
+    }
+    int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    if (lane == 0) {
+        smax[wv] = wsv_tmp;
+        scnt[wv][0] = ncoll;
+        scnt[wv][1] = niz;
+        scnt[wv][2] = ngr;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double m = 0;
+        unsigned long long t0 = 0, t1 = 0, t2 = 0;
+        for (int w = 0; w < BLOCK / 64; w++) {
+            m = fmax(m, smax[w]);
+            t0 += scnt[w][0];
+            t1 += scnt[w][1];
+            t2 += scnt[w][2];
+        }
+        if (t2) {
+            atomicMax((unsigned long long*)&wsv[1], (unsigned long long)__double_as_longlong(m));
+            atomicAdd(&stats[0], t0);
+            atomicAdd(&stats[1], t1);
+            atomicAdd(&stats[2], t2);
+        }
+    }
+}
+
+// W_sigma_v_rel_max <- step maximum when any collision happened
+// (Interactions.cpp:751-756)
+__global__ void k_mcc_finish(double* wsv, const unsigned long long* stats, uint32_t* caps_n,
+                             uint32_t cap_n, uint32_t* caps_i, uint32_t cap_i, uint32_t* caps_e,
+                             uint32_t cap_e) {
+    if (stats[0] > 0) wsv[0] = wsv[1];
+    if (caps_n[0] > cap_n) caps_n[0] = cap_n;
+    if (caps_i[0] > cap_i) caps_i[0] = cap_i;
+    if (caps_e[0] > cap_e) caps_e[0] = cap_e;
+}
+
+}  // namespace
+
+extern "C" int pic_mcc_create(pic_ctx* ctx, int sn, int si, int se, const pic_mcc_desc* desc,
+                              const double* sig_E, const double* sig_v, int nsig, int* id) {
+    if (!ctx || !desc || !sig_E || !sig_v || nsig < 1 || !id) return PIC_ERR_ARG;
+    int rc = pic_check_sid(ctx, sn);
+    if (!rc) rc = pic_check_sid(ctx, si);
+    if (!rc) rc = pic_check_sid(ctx, se);
+    if (rc) return rc;
+    const Species &N = ctx->sp[sn], &I = ctx->sp[si], &E = ctx->sp[se];
+    // constructor guards (Interactions.cpp:479-489)
+    if (N.mpw0 < 1e2 * E.mpw0)
+        return pic_fail(ctx, PIC_ERR_ARG, "neutrals mpw0 must be >= 100 x electrons mpw0");
+    if (!(desc->E_ion_J > 0)) return pic_fail(ctx, PIC_ERR_ARG, "neutral E_ion must be positive");
+    if (N.mpw0 < I.mpw0) return pic_fail(ctx, PIC_ERR_ARG, "neutrals mpw0 must be >= ions mpw0");
+    for (int q = 1; q < nsig; q++)
+        if (!(sig_E[q] > sig_E[q - 1]))
+            return pic_fail(ctx, PIC_ERR_ARG, "cross-section energies must be ascending");
+    Mcc m;
+    m.sn = sn;
+    m.si = si;
+    m.se = se;
+    m.desc = *desc;
+    m.nsig = nsig;
+    PIC_HIP(ctx, hipMalloc(&m.sig, 2 * (size_t)nsig * sizeof(double)));
+    PIC_HIP(ctx, hipMalloc(&m.d_wsv, 2 * sizeof(double)));
+    PIC_HIP(ctx, hipMalloc(&m.d_stats, 4 * sizeof(unsigned long long)));
+    PIC_HIP(ctx, hipMemcpyAsync(m.sig, sig_E, nsig * sizeof(double), hipMemcpyHostToDevice,
+                                ctx->stream));
+    PIC_HIP(ctx, hipMemcpyAsync(m.sig + nsig, sig_v, nsig * sizeof(double), hipMemcpyHostToDevice,
+                                ctx->stream));
+    // W_sigma_v_rel_max = 1e-14 * max(mpw0) unless given (Interactions.cpp:129, :534)
+    double w0 = desc->wsv_max_init > 0 ? desc->wsv_max_init : 1e-14 * std::max(E.mpw0, N.mpw0);
+    double hw[2] = {w0, 0.0};
+    PIC_HIP(ctx, hipMemcpyAsync(m.d_wsv, hw, sizeof(hw), hipMemcpyHostToDevice, ctx->stream));
+    PIC_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    m.wsv_host = w0;
+    ctx->mcc.push_back(m);
+    *id = (int)ctx->mcc.size() - 1;
+    return PIC_OK;
+}
+
+extern "C" int pic_mcc_apply(pic_ctx* ctx, int id, double dt, uint32_t seed, uint32_t step,
+                             pic_mcc_stats* stats) {
+    if (!ctx || id < 0 || id >= (int)ctx->mcc.size()) return PIC_ERR_ARG;
+    Mcc& m = ctx->mcc[id];
+    Species &N = ctx->sp[m.sn], &I = ctx->sp[m.si], &E = ctx->sp[m.se];
+    int rc = 0;
+    if (!N.sorted) rc = pic_sort(ctx, m.sn);
+    if (!rc && !E.sorted) rc = pic_sort(ctx, m.se);
+    if (rc) return rc;
+    const GridC& g = ctx->g;
+    MccArgs a;
+    a.m_n = N.mass;
+    a.m_e = E.mass;
+    a.sum_m = N.mass + E.mass;
+    a.m_red = N.mass * E.mass / (N.mass + E.mass);
+    a.E_pre_rel = 0.5 * a.m_red / QE;
+    a.E_pre_ele = ME * 0.5 / QE;
+    a.two_q_me = 2 * QE / ME;
+    a.E_ion_eV = m.desc.E_ion_J / QE;
+    a.c0 = m.desc.c0;
+    a.c1 = m.desc.c1;
+    a.c2 = m.desc.c2;
+    a.B_inc = m.desc.B_inc;
+    a.dt_inv_dv = dt * (1 / (g.dx[0] * g.dx[1] * g.dx[2]));
+    a.seed = seed;
+    a.step = step;
+    a.nsig = m.nsig;
+    Sp sn{N.buf[N.cur], N.d_n, (uint32_t)N.cap};
+    Sp si{I.buf[I.cur], I.d_n, (uint32_t)I.cap};
+    Sp se{E.buf[E.cur], E.d_n, (uint32_t)E.cap};
+    PIC_HIP(ctx, hipMemsetAsync(m.d_stats, 0, 4 * sizeof(unsigned long long), ctx->stream));
+    PIC_HIP(ctx, hipMemsetAsync(m.d_wsv + 1, 0, sizeof(double), ctx->stream));
+    {
+        PicProfScope ps(ctx, PIC_K_MCC);
+        k_mcc<<<grid_for((size_t)g.ncells, BLOCK), BLOCK, 0, ctx->stream>>>(
+            g, a, sn, N.off, si, se, E.off, m.sig, m.d_wsv, m.d_stats);
+        k_mcc_finish<<<1, 1, 0, ctx->stream>>>(m.d_wsv, m.d_stats, N.d_n, (uint32_t)N.cap, I.d_n,
+                                               (uint32_t)I.cap, E.d_n, (uint32_t)E.cap);
+    }
+    PIC_HIP(ctx, hipGetLastError());
+    unsigned long long hs[4];
+    double hw[2];
+    uint32_t cn[3];
+    PIC_HIP(ctx, hipMemcpyAsync(hs, m.d_stats, sizeof(hs), hipMemcpyDeviceToHost, ctx->stream));
+    PIC_HIP(ctx, hipMemcpyAsync(hw, m.d_wsv, sizeof(hw), hipMemcpyDeviceToHost, ctx->stream));
+    PIC_HIP(ctx, hipMemcpyAsync(&cn[0], N.d_n, 4, hipMemcpyDeviceToHost, ctx->stream));
+    PIC_HIP(ctx, hipMemcpyAsync(&cn[1], I.d_n, 4, hipMemcpyDeviceToHost, ctx->stream));
+    PIC_HIP(ctx, hipMemcpyAsync(&cn[2], E.d_n, 4, hipMemcpyDeviceToHost, ctx->stream));
+    PIC_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    N.n_upper = cn[0];
+    I.n_upper = cn[1];
+    E.n_upper = cn[2];
+    m.wsv_host = hw[0];
+    if (hs[0] || hs[1]) {
+        N.sorted = I.sorted = E.sorted = false;
+        N.hist_valid = I.hist_valid = E.hist_valid = false;
+    }
+    if (stats) {
+        stats->collisions = (long long)hs[0];
+        stats->ionizations = (long long)hs[1];
+        stats->groups = (long long)hs[2];
+        stats->wsv_max = hw[0];
+    }
+    if (hs[3]) return pic_fail(ctx, PIC_ERR_CAPACITY, "species capacity exceeded by DSMC births");
+    return PIC_OK;
+}

@@ -1,0 +1,599 @@
+// pic_particles.hip -- particle kernels of the gfx950 PIC library:
+//   k_push     fused CIC gather + kick + drift + absorb + counting-sort
+//              histogram (Species.cpp:356-399, Field.h:201-232)
+//   k_hist     histogram of unchanged particles (Species.cpp:905-915)
+//   k_scan_*   exclusive scan of the per-cell counts -> cell offsets
+//   k_scatter  stable-per-wave scatter into the second buffer (compacts)
+//   k_deposit  one workgroup per cell row, LDS tile of the row's 4 node
+//              lines, flushed with FP64 global atomics (Species.cpp:401-416,
+//              :767-776, Field.h:157-199)
+//   k_rho      charge density (World.cpp:193-200)
+//   k_load     Philox loader (Species.cpp:564-599, :420-434, :855-869)
+#include <algorithm>
+#include <cmath>
+
+#include "pic_internal.h"
+
+using namespace pic;
+
+namespace {
+
+constexpr int BLOCK = 256;
+
+// Wave-aggregated counting-sort histogram.  Lanes of a wave hold
+// consecutive particles; on cell-sorted input they form long runs of equal
+// keys, so only the head of each run issues the atomic.  Every lane gets its
+// rank inside the bucket.  All lanes of the wave must call this.
+__device__ inline uint32_t run_histogram(uint32_t key, bool valid, uint32_t* __restrict__ count) {
+    const int lane = threadIdx.x & 63;
+    uint32_t prev = __shfl_up(key, 1, 64);
+    bool head = (lane == 0) || (key != prev);
+    unsigned long long heads = __ballot(head);
+    unsigned long long upto = (lane == 63) ? ~0ull : ((2ull << lane) - 1ull);
+    int my_head = 63 - __clzll(heads & upto);
+    unsigned long long above = heads & ~upto;
+    int next_head = above ? (__ffsll((unsigned long long)above) - 1) : 64;
+    uint32_t base = 0;
+    if (head && valid) base = atomicAdd(&count[key], (uint32_t)(next_head - lane));
+    base = __shfl(base, my_head, 64);
+    return base + (uint32_t)(lane - my_head);
+}
+
+struct PushArgs {
+    float qmdt;    // dt * q / m   (Species.cpp:372)
+    float dtx[3];  // dt / dx      (drift in cells)
+    int kick;
+};
+
+__device__ inline float3 gather_e(const GridC& g, const float4* __restrict__ E, int i, int j, int k,
+                                  float fx, float fy, float fz) {
+    long long u = ((long long)k * g.nj + j) * g.ni + i;
+    long long sj = g.ni, sk = (long long)g.ni * g.nj;
+    float4 e000 = E[u], e001 = E[u + sk], e010 = E[u + sj], e011 = E[u + sj + sk];
+    float4 e100 = E[u + 1], e101 = E[u + 1 + sk], e110 = E[u + 1 + sj], e111 = E[u + 1 + sj + sk];
+    float gx = 1.f - fx, gy = 1.f - fy, gz = 1.f - fz;
+    float a = gx * gy, b = gx * fy, c = fx * gy, d = fx * fy;
+    float w000 = a * gz, w001 = a * fz, w010 = b * gz, w011 = b * fz;
+    float w100 = c * gz, w101 = c * fz, w110 = d * gz, w111 = d * fz;
+    float3 r;
+    r.x = e000.x * w000 + e001.x * w001 + e010.x * w010 + e011.x * w011 + e100.x * w100 +
+          e101.x * w101 + e110.x * w110 + e111.x * w111;
+    r.y = e000.y * w000 + e001.y * w001 + e010.y * w010 + e011.y * w011 + e100.y * w100 +
+          e101.y * w101 + e110.y * w110 + e111.y * w111;
+    r.z = e000.z * w000 + e001.z * w001 + e010.z * w010 + e011.z * w011 + e100.z * w100 +
+          e101.z * w101 + e110.z * w110 + e111.z * w111;
+    return r;
+}
+
+// move the in-cell offset by dcell cells; returns the new cell coordinate
+__device__ inline int relocate(float& f, int i) {
+    float o = floorf(f);
+    f -= o;
+    if (f >= 1.0f) {  // f was a tiny negative number
+        f = 0.0f;
+        o += 1.0f;
+    }
+    o = fminf(fmaxf(o, -4096.0f), 4096.0f);
+    return i + (int)o;
+}
+
+__device__ inline bool in_rect(const GridC& g, int i, int j, int k, float fx, float fy, float fz) {
+    float l[3] = {(float)i + fx, (float)j + fy, (float)k + fz};
+    for (int r = 0; r < g.nrect; r++) {
+        bool in = true;
+#pragma unroll
+        for (int d = 0; d < 3; d++)
+            if (l[d] < g.rlo[r][d] || l[d] > g.rhi[r][d]) in = false;
+        if (in) return true;
+    }
+    return false;
+}
+
+__global__ void __launch_bounds__(BLOCK)
+    k_push(GridC g, SpeciesBuf b, const float4* __restrict__ E, PushArgs a,
+           uint32_t* __restrict__ d_n, uint32_t* __restrict__ count, uint32_t* __restrict__ rank) {
+    size_t p = (size_t)blockIdx.x * BLOCK + threadIdx.x;
+    uint32_t n = d_n[0];
+    uint32_t covered = gridDim.x * BLOCK;
+    if (n > covered) n = covered;
+    if (p == 0) d_n[1] = n;
+    bool valid = p < n;
+    uint32_t key = 0xFFFFFFFFu;
+    if (valid) {
+        uint32_t c = b.cell[p];
+        key = (uint32_t)g.ncells;
+        if (c != DEAD) {
+            int i, j, k;
+            unpack_cell(c, i, j, k);
+            float fx = b.f[0][p], fy = b.f[1][p], fz = b.f[2][p];
+            float vx = b.v[0][p], vy = b.v[1][p], vz = b.v[2][p];
+            if (a.kick) {
+                float3 e = gather_e(g, E, i, j, k, fx, fy, fz);
+                vx += e.x * a.qmdt;
+                vy += e.y * a.qmdt;
+                vz += e.z * a.qmdt;
+            }
+            fx += vx * a.dtx[0];
+            fy += vy * a.dtx[1];
+            fz += vz * a.dtx[2];
+            bool ok = (fx == fx) && (fy == fy) && (fz == fz);
+            if (ok) {
+                i = relocate(fx, i);
+                j = relocate(fy, j);
+                k = relocate(fz, k);
+                // World::inBounds (World.cpp:201-205): 0 <= lc < n-1 per axis
+                ok = i >= 0 && i < g.ci && j >= 0 && j < g.cj && k >= 0 && k < g.ck;
+                if (ok && g.nrect) ok = !in_rect(g, i, j, k, fx, fy, fz);
+            }
+            if (ok) {
+                b.cell[p] = pack_cell(i, j, k);
+                key = (uint32_t)cell_index(g, i, j, k);
+            } else {
+                b.cell[p] = DEAD;
+            }
+            b.f[0][p] = fx;
+            b.f[1][p] = fy;
+            b.f[2][p] = fz;
+            b.v[0][p] = vx;
+            b.v[1][p] = vy;
+            b.v[2][p] = vz;
+        }
+    }
+    uint32_t r = run_histogram(key, valid, count);
+    if (valid) rank[p] = r;
+}
+
+__global__ void __launch_bounds__(BLOCK)
+    k_hist(GridC g, SpeciesBuf b, uint32_t* __restrict__ d_n, uint32_t* __restrict__ count,
+           uint32_t* __restrict__ rank) {
+    size_t p = (size_t)blockIdx.x * BLOCK + threadIdx.x;
+    uint32_t n = d_n[0];
+    uint32_t covered = gridDim.x * BLOCK;
+    if (n > covered) n = covered;
+    if (p == 0) d_n[1] = n;
+    bool valid = p < n;
+    uint32_t key = 0xFFFFFFFFu;
+    if (valid) {
+        uint32_t c = b.cell[p];
+        key = (uint32_t)g.ncells;
+        if (c != DEAD) {
+            int i, j, k;
+            unpack_cell(c, i, j, k);
+            key = (uint32_t)cell_index(g, i, j, k);
+        }
+    }
+    uint32_t r = run_histogram(key, valid, count);
+    if (valid) rank[p] = r;
+}
+
+// ---- exclusive scan of count[0..M) -> off[0..M], M = ncells + 1 ----
+constexpr int SCAN_ITEMS = 16;
+constexpr int SCAN_TILE = BLOCK * SCAN_ITEMS;
+
+__device__ inline uint32_t block_exclusive_scan(uint32_t v, uint32_t* total) {
+    __shared__ uint32_t wsum[BLOCK / 64];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    uint32_t incl = v;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        uint32_t t = __shfl_up(incl, o, 64);
+        if (lane >= o) incl += t;
+    }
+    if (lane == 63) wsum[wv] = incl;
+    __syncthreads();
+    uint32_t before = 0, all = 0;
+#pragma unroll
+    for (int w = 0; w < BLOCK / 64; w++) {
+        if (w < wv) before += wsum[w];
+        all += wsum[w];
+    }
+    __syncthreads();
+    if (total) *total = all;
+    return before + incl - v;
+}
+
+__global__ void __launch_bounds__(BLOCK) k_scan_reduce(const uint32_t* __restrict__ in, int M,
+                                                       uint32_t* __restrict__ bsum) {
+    size_t base = (size_t)blockIdx.x * SCAN_TILE + (size_t)threadIdx.x * SCAN_ITEMS;
+    uint32_t s = 0;
+#pragma unroll
+    for (int q = 0; q < SCAN_ITEMS; q++)
+        if (base + q < (size_t)M) s += in[base + q];
+    uint32_t tot;
+    block_exclusive_scan(s, &tot);
+    if (threadIdx.x == 0) bsum[blockIdx.x] = tot;
+}
+
+// single workgroup: exclusive scan of the tile sums in place
+__global__ void __launch_bounds__(BLOCK) k_scan_top(uint32_t* __restrict__ bsum, int nb) {
+    int per = (nb + BLOCK - 1) / BLOCK;
+    int beg = threadIdx.x * per;
+    uint32_t s = 0;
+    for (int q = 0; q < per; q++)
+        if (beg + q < nb) s += bsum[beg + q];
+    uint32_t ex = block_exclusive_scan(s, nullptr);
+    for (int q = 0; q < per; q++)
+        if (beg + q < nb) {
+            uint32_t t = bsum[beg + q];
+            bsum[beg + q] = ex;
+            ex += t;
+        }
+}
+
+__global__ void __launch_bounds__(BLOCK)
+    k_scan_down(const uint32_t* __restrict__ in, int M, const uint32_t* __restrict__ bsum,
+                uint32_t* __restrict__ out, int ncells, uint32_t* __restrict__ d_n) {
+    size_t base = (size_t)blockIdx.x * SCAN_TILE + (size_t)threadIdx.x * SCAN_ITEMS;
+    uint32_t v[SCAN_ITEMS];
+    uint32_t s = 0;
+#pragma unroll
+    for (int q = 0; q < SCAN_ITEMS; q++) {
+        v[q] = (base + q < (size_t)M) ? in[base + q] : 0u;
+        s += v[q];
+    }
+    uint32_t ex = block_exclusive_scan(s, nullptr) + bsum[blockIdx.x];
+#pragma unroll
+    for (int q = 0; q < SCAN_ITEMS; q++) {
+        size_t idx = base + q;
+        if (idx < (size_t)M) {
+            out[idx] = ex;
+            if (idx == (size_t)ncells) d_n[0] = ex;  // particles alive after compaction
+            if (idx == (size_t)M - 1) out[M] = ex + v[q];
+        }
+        ex += v[q];
+    }
+}
+
+__global__ void __launch_bounds__(BLOCK)
+    k_scatter(GridC g, SpeciesBuf src, SpeciesBuf dst, const uint32_t* __restrict__ rank,
+              const uint32_t* __restrict__ off, const uint32_t* __restrict__ d_n, uint32_t cap) {
+    size_t p = (size_t)blockIdx.x * BLOCK + threadIdx.x;
+    if (p >= d_n[1]) return;
+    uint32_t c = src.cell[p];
+    if (c == DEAD) return;
+    int i, j, k;
+    unpack_cell(c, i, j, k);
+    uint32_t dst_i = off[cell_index(g, i, j, k)] + rank[p];
+    if (dst_i >= cap) return;  // cannot happen with a current histogram
+    dst.cell[dst_i] = c;
+#pragma unroll
+    for (int d = 0; d < 3; d++) {
+        dst.f[d][dst_i] = src.f[d][p];
+        dst.v[d][dst_i] = src.v[d][p];
+    }
+    dst.w[dst_i] = src.w[p];
+}
+
+// ---- deposit: one workgroup per cell row (j, k); the row's particles are
+// contiguous after the sort and touch node lines (j..j+1, k..k+1). ----
+template <int NCH>
+__global__ void __launch_bounds__(BLOCK)
+    k_deposit(GridC g, SpeciesBuf b, const uint32_t* __restrict__ off, double* __restrict__ den,
+              double* __restrict__ mom, int want_den) {
+    extern __shared__ float tile[];  // NCH channels x 4 node lines x ni
+    const int r = blockIdx.x;
+    const int j = r % g.cj, k = r / g.cj;
+    const uint32_t p0 = off[(size_t)r * g.ci], p1 = off[(size_t)(r + 1) * g.ci];
+    if (p0 == p1) return;  // uniform for the whole workgroup
+    const int ni = g.ni;
+    const int line = 4 * ni;
+    for (int t = threadIdx.x; t < NCH * line; t += BLOCK) tile[t] = 0.f;
+    __syncthreads();
+    for (uint32_t p = p0 + threadIdx.x; p < p1; p += BLOCK) {
+        int i = (int)(b.cell[p] & 0x7FFu);
+        float fx = b.f[0][p], fy = b.f[1][p], fz = b.f[2][p];
+        float w = b.w[p];
+        float gx = 1.f - fx, gy = 1.f - fy, gz = 1.f - fz;
+        float wt[8];  // index (dk*2 + dj)*2 + di
+        wt[0] = gx * gy * gz;
+        wt[1] = fx * gy * gz;
+        wt[2] = gx * fy * gz;
+        wt[3] = fx * fy * gz;
+        wt[4] = gx * gy * fz;
+        wt[5] = fx * gy * fz;
+        wt[6] = gx * fy * fz;
+        wt[7] = fx * fy * fz;
+        float val[NCH];
+        val[0] = w;
+        if (NCH > 1) {
+            float vx = b.v[0][p], vy = b.v[1][p], vz = b.v[2][p];
+            val[1 % NCH] = w * vx;
+            val[2 % NCH] = w * vy;
+            val[3 % NCH] = w * vz;
+            val[4 % NCH] = w * vx * vx;
+            val[5 % NCH] = w * vy * vy;
+            val[6 % NCH] = w * vz * vz;
+        }
+#pragma unroll
+        for (int q = 0; q < 8; q++) {
+            int plane = q >> 1, di = q & 1;
+            int slot = plane * ni + i + di;
+#pragma unroll
+            for (int ch = 0; ch < NCH; ch++) atomicAdd(&tile[ch * line + slot], val[ch] * wt[q]);
+        }
+    }
+    __syncthreads();
+    const long long nn = g.nn;
+    for (int t = threadIdx.x; t < line; t += BLOCK) {
+        int plane = t / ni, ii = t - plane * ni;
+        int jj = j + (plane & 1), kk = k + (plane >> 1);
+        long long u = ((long long)kk * g.nj + jj) * ni + ii;
+        float v0 = tile[t];
+        if (v0 != 0.f) {
+            if (want_den) atomicAdd(&den[u], (double)v0 * inv_node_vol(g, ii, jj, kk));
+            if (NCH > 1) atomicAdd(&mom[u], (double)v0);
+        }
+        if (NCH > 1) {
+#pragma unroll
+            for (int ch = 1; ch < NCH; ch++) {
+                float vc = tile[ch * line + t];
+                if (vc != 0.f) atomicAdd(&mom[(long long)ch * nn + u], (double)vc);
+            }
+        }
+    }
+}
+
+struct RhoArgs {
+    const double* den[MAX_SPECIES];
+    double q[MAX_SPECIES];
+    int ns;
+};
+
+__global__ void __launch_bounds__(BLOCK) k_rho(long long nn, RhoArgs a, double* __restrict__ rho) {
+    long long u = (long long)blockIdx.x * BLOCK + threadIdx.x;
+    if (u >= nn) return;
+    double r = 0;
+    for (int s = 0; s < a.ns; s++) r += a.q[s] * a.den[s][u];
+    rho[u] = r;
+}
+
+struct LoadArgs {
+    double xmin[3], xmax[3];
+    double vth;     // sqrt(2kT/m) (mode 0) or sqrt(kT/m) (mode 1)
+    double rewind;  // (q/m) * 0.5 dt
+    uint32_t seed, seq;
+    int mode;
+    float w;
+};
+
+__global__ void __launch_bounds__(BLOCK)
+    k_load(GridC g, SpeciesBuf b, const float4* __restrict__ E, LoadArgs a, size_t base,
+           size_t count) {
+    size_t i = (size_t)blockIdx.x * BLOCK + threadIdx.x;
+    if (i >= count) return;
+    Draws dr;
+    dr.init((uint32_t)i, (uint32_t)(i >> 32), a.seed, a.seq, TAG_LOAD);
+    double pos[3], vel[3];
+    for (int d = 0; d < 3; d++) pos[d] = a.xmin[d] + dr(d) * (a.xmax[d] - a.xmin[d]);
+    if (a.mode == 0) {  // Species.cpp:855-869
+        double v[3];
+        for (int c = 0; c < 3; c++) {
+            double u1 = dr(3 + 3 * c), u2 = dr(4 + 3 * c), u3 = dr(5 + 3 * c);
+            v[c] = a.vth * (u1 + u2 + u3 - 1.5);
+        }
+        double sp = sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
+        double theta = 2 * PI_REF * dr(12);
+        double r = -1.0 + 2 * dr(13);
+        double s = sqrt(1 - r * r);
+        vel[0] = sp * r;
+        vel[1] = sp * (cos(theta) * s);
+        vel[2] = sp * (sin(theta) * s);
+    } else {  // Box-Muller Maxwellian
+        double u1 = dr(3), u2 = dr(4), u3 = dr(5), u4 = dr(6);
+        double r1 = sqrt(-2.0 * log(1.0 - u1)), r2 = sqrt(-2.0 * log(1.0 - u3));
+        vel[0] = a.vth * r1 * cos(TWO_PI * u2);
+        vel[1] = a.vth * r1 * sin(TWO_PI * u2);
+        vel[2] = a.vth * r2 * cos(TWO_PI * u4);
+    }
+    // addParticle (Species.cpp:420-434): reject, then rewind v by half a step
+    double lc[3];
+    int ijk[3];
+    int lim[3] = {g.ci, g.cj, g.ck};
+    float f[3] = {0, 0, 0};
+    uint32_t c = 0;
+    for (int d = 0; d < 3; d++) {
+        lc[d] = (pos[d] - g.x0[d]) * g.inv_dx[d];
+        if (!(lc[d] >= 0.0)) c = DEAD;
+        ijk[d] = (int)lc[d];
+        if (ijk[d] >= lim[d]) c = DEAD;
+        float fr = (float)(lc[d] - ijk[d]);
+        f[d] = fr < 1.0f ? fr : 0.99999994f;
+    }
+    if (c != DEAD && g.nrect && in_rect(g, ijk[0], ijk[1], ijk[2], f[0], f[1], f[2])) c = DEAD;
+    if (c != DEAD) {
+        c = pack_cell(ijk[0], ijk[1], ijk[2]);
+        if (a.rewind != 0.0) {
+            float3 e = gather_e(g, E, ijk[0], ijk[1], ijk[2], f[0], f[1], f[2]);
+            vel[0] -= (double)e.x * a.rewind;
+            vel[1] -= (double)e.y * a.rewind;
+            vel[2] -= (double)e.z * a.rewind;
+        }
+    }
+    size_t o = base + i;
+    b.cell[o] = c;
+    for (int d = 0; d < 3; d++) {
+        b.f[d][o] = f[d];
+        b.v[d][o] = (float)vel[d];
+    }
+    b.w[o] = a.w;
+}
+
+}  // namespace
+
+// ------------------------------------------------------------------------
+// host entry points
+// ------------------------------------------------------------------------
+extern "C" int pic_push(pic_ctx* ctx, int sid, double dt) {
+    int rc = pic_check_sid(ctx, sid);
+    if (rc) return rc;
+    Species& s = ctx->sp[sid];
+    const GridC& g = ctx->g;
+    PushArgs a;
+    a.qmdt = (float)(dt * s.charge / s.mass);
+    for (int d = 0; d < 3; d++) a.dtx[d] = (float)(dt * g.inv_dx[d]);
+    a.kick = s.charge != 0.0;
+    PIC_HIP(ctx, hipMemsetAsync(s.count, 0, ((size_t)g.ncells + 1) * 4, ctx->stream));
+    {
+        PicProfScope ps(ctx, PIC_K_PUSH);
+        size_t n = s.n_upper ? s.n_upper : 1;
+        k_push<<<grid_for(n, BLOCK), BLOCK, 0, ctx->stream>>>(g, s.buf[s.cur], ctx->ef, a, s.d_n,
+                                                              s.count, s.rank);
+    }
+    PIC_HIP(ctx, hipGetLastError());
+    s.hist_valid = true;
+    s.sorted = false;
+    return PIC_OK;
+}
+
+int pic_histogram(pic_ctx* ctx, int sid) {
+    Species& s = ctx->sp[sid];
+    const GridC& g = ctx->g;
+    PIC_HIP(ctx, hipMemsetAsync(s.count, 0, ((size_t)g.ncells + 1) * 4, ctx->stream));
+    {
+        PicProfScope ps(ctx, PIC_K_SORT);
+        size_t n = s.n_upper ? s.n_upper : 1;
+        k_hist<<<grid_for(n, BLOCK), BLOCK, 0, ctx->stream>>>(g, s.buf[s.cur], s.d_n, s.count,
+                                                              s.rank);
+    }
+    PIC_HIP(ctx, hipGetLastError());
+    s.hist_valid = true;
+    return PIC_OK;
+}
+
+extern "C" int pic_sort(pic_ctx* ctx, int sid) {
+    int rc = pic_check_sid(ctx, sid);
+    if (rc) return rc;
+    Species& s = ctx->sp[sid];
+    const GridC& g = ctx->g;
+    if (!s.hist_valid) {
+        rc = pic_histogram(ctx, sid);
+        if (rc) return rc;
+    }
+    int M = g.ncells + 1;
+    int nb = (M + SCAN_TILE - 1) / SCAN_TILE;
+    {
+        PicProfScope ps(ctx, PIC_K_SORT);
+        k_scan_reduce<<<nb, BLOCK, 0, ctx->stream>>>(s.count, M, ctx->scan_tmp);
+        k_scan_top<<<1, BLOCK, 0, ctx->stream>>>(ctx->scan_tmp, nb);
+        k_scan_down<<<nb, BLOCK, 0, ctx->stream>>>(s.count, M, ctx->scan_tmp, s.off, g.ncells,
+                                                   s.d_n);
+    }
+    {
+        PicProfScope ps(ctx, PIC_K_SCATTER);
+        size_t n = s.n_upper ? s.n_upper : 1;
+        k_scatter<<<grid_for(n, BLOCK), BLOCK, 0, ctx->stream>>>(
+            g, s.buf[s.cur], s.buf[s.cur ^ 1], s.rank, s.off, s.d_n, (uint32_t)s.cap);
+    }
+    PIC_HIP(ctx, hipGetLastError());
+    s.cur ^= 1;
+    s.sorted = true;
+    s.hist_valid = false;
+    return PIC_OK;
+}
+
+extern "C" int pic_deposit(pic_ctx* ctx, int sid, int flags) {
+    int rc = pic_check_sid(ctx, sid);
+    if (rc) return rc;
+    Species& s = ctx->sp[sid];
+    const GridC& g = ctx->g;
+    if (!(flags & (PIC_DEPOSIT_DENSITY | PIC_DEPOSIT_MOMENTS))) return PIC_ERR_ARG;
+    if (!s.sorted) {
+        rc = pic_sort(ctx, sid);
+        if (rc) return rc;
+    }
+    size_t nn = (size_t)g.nn;
+    int want_den = (flags & PIC_DEPOSIT_DENSITY) != 0;
+    if (want_den) PIC_HIP(ctx, hipMemsetAsync(s.den, 0, nn * 8, ctx->stream));
+    bool mom = (flags & PIC_DEPOSIT_MOMENTS) != 0;
+    if (mom && !s.mom) {
+        PIC_HIP(ctx, hipMalloc(&s.mom, 7 * nn * 8));
+        PIC_HIP(ctx, hipMemsetAsync(s.mom, 0, 7 * nn * 8, ctx->stream));
+    }
+    int rows = g.cj * g.ck;
+    {
+        PicProfScope ps(ctx, PIC_K_DEPOSIT);
+        if (mom) {
+            size_t lds = 7 * 4 * (size_t)g.ni * sizeof(float);
+            k_deposit<7><<<rows, BLOCK, lds, ctx->stream>>>(g, s.buf[s.cur], s.off, s.den, s.mom,
+                                                            want_den);
+        } else {
+            size_t lds = 4 * (size_t)g.ni * sizeof(float);
+            k_deposit<1><<<rows, BLOCK, lds, ctx->stream>>>(g, s.buf[s.cur], s.off, s.den,
+                                                            nullptr, want_den);
+        }
+    }
+    PIC_HIP(ctx, hipGetLastError());
+    return PIC_OK;
+}
+
+extern "C" int pic_clear_samples(pic_ctx* ctx, int sid) {
+    int rc = pic_check_sid(ctx, sid);
+    if (rc) return rc;
+    Species& s = ctx->sp[sid];
+    if (s.mom) PIC_HIP(ctx, hipMemsetAsync(s.mom, 0, 7 * (size_t)ctx->g.nn * 8, ctx->stream));
+    return PIC_OK;
+}
+
+extern "C" int pic_charge_density(pic_ctx* ctx) {
+    if (!ctx) return PIC_ERR_ARG;
+    RhoArgs a;
+    a.ns = 0;
+    for (auto& s : ctx->sp) {
+        if (s.charge == 0) continue;  // World.cpp:197
+        a.den[a.ns] = s.den;
+        a.q[a.ns] = s.charge;
+        a.ns++;
+    }
+    {
+        PicProfScope ps(ctx, PIC_K_RHO);
+        k_rho<<<grid_for((size_t)ctx->g.nn, BLOCK), BLOCK, 0, ctx->stream>>>(ctx->g.nn, a,
+                                                                            ctx->rho);
+    }
+    PIC_HIP(ctx, hipGetLastError());
+    return PIC_OK;
+}
+
+extern "C" int pic_load_box_thermal(pic_ctx* ctx, int sid, const double centre[3],
+                                    const double sides[3], double num_den, double T,
+                                    uint32_t seed, int mode, size_t* n_loaded) {
+    int rc = pic_check_sid(ctx, sid);
+    if (rc) return rc;
+    if (!centre || !sides || (mode != 0 && mode != 1)) return PIC_ERR_ARG;
+    Species& s = ctx->sp[sid];
+    double box_vol = sides[0] * sides[1] * sides[2];
+    double num_micro = num_den * box_vol;
+    size_t num_macro = (size_t)(num_micro / s.mpw0);  // Species.cpp:565-567
+    if (num_macro < 1)
+        return pic_fail(ctx, PIC_ERR_ARG, "number of macroparticles less than 1");
+    uint32_t hn[4];
+    PIC_HIP(ctx, hipMemcpyAsync(hn, s.d_n, sizeof(hn), hipMemcpyDeviceToHost, ctx->stream));
+    PIC_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    size_t base = hn[0];
+    if (base + num_macro > s.cap)
+        return pic_fail(ctx, PIC_ERR_CAPACITY, "loader exceeds species capacity");
+    LoadArgs a;
+    for (int d = 0; d < 3; d++) {
+        a.xmin[d] = centre[d] - sides[d] * 0.5;
+        a.xmax[d] = centre[d] + sides[d] * 0.5;
+    }
+    a.vth = mode == 0 ? sqrt(2 * KB * T / s.mass) : sqrt(KB * T / s.mass);
+    a.rewind = s.charge / s.mass * (0.5 * ctx->dt);
+    a.seed = seed;
+    a.seq = s.seq;
+    a.mode = mode;
+    a.w = (float)s.mpw0;
+    {
+        PicProfScope ps(ctx, PIC_K_LOAD);
+        k_load<<<grid_for(num_macro, BLOCK), BLOCK, 0, ctx->stream>>>(ctx->g, s.buf[s.cur],
+                                                                      ctx->ef, a, base, num_macro);
+    }
+    PIC_HIP(ctx, hipGetLastError());
+    hn[0] = (uint32_t)(base + num_macro);
+    PIC_HIP(ctx, hipMemcpyAsync(s.d_n, hn, sizeof(uint32_t), hipMemcpyHostToDevice, ctx->stream));
+    PIC_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    s.n_upper = base + num_macro;
+    s.hist_valid = false;
+    s.sorted = false;
+    if (n_loaded) *n_loaded = num_macro;
+    return PIC_OK;
+}

@@ -1,0 +1,714 @@
+// pic_solver.hip -- field solve of the gfx950 PIC library.
+//
+//   k_efield     E = -grad(phi), FP64 math, FP32 float4 output
+//                (PotentialSolver.cpp:354-408)
+//   k_sor        red-black SOR with the Boltzmann electron term
+//                (PotentialSolver.cpp:69-166; ch2/v1/PotentialSolver.cpp:8-87)
+//   k_neumann    Neumann face copies (PotentialSolver.cpp:96-107)
+//   k_F          Newton residual F = A phi - b - q n0 e^{(phi-phi0)/Te0}/eps0
+//                and the Jacobian diagonal shift (PotentialSolver.cpp:189-217)
+//   k_pcg1/2     matrix-free Jacobi PCG (PotentialSolver.cpp:243-313), two
+//                fused kernels per iteration, scalars kept on the device
+//
+// The operator rows follow PotentialSolver::buildMatrix
+// (PotentialSolver.cpp:414-471): Dirichlet rows are the identity, Neumann face
+// rows are one-sided (1/dx)(phi_u - phi_nbr), regular rows are the 7-point
+// Laplacian.  Jacobi M = 1/diag(A) of that operator (PotentialSolver.cpp:470).
+//
+// All stencil kernels walk the grid in contiguous chunks per workgroup and
+// map consecutive chunks to the same XCD (workgroups are dealt round-robin
+// over the 8 XCDs), so k-neighbour planes are reused from that XCD's L2.
+#include <algorithm>
+#include <cmath>
+#include <vector>
+
+#include "pic_internal.h"
+
+using namespace pic;
+
+namespace {
+
+constexpr int BLOCK = 256;
+constexpr int NXCD = 8;
+
+struct Stencil {
+    double cx, cy, cz, twos;  // +-1, +-ni, +-ni*nj coefficients; 2(sum 1/dx^2)
+    double idx[3];            // 1/dx per axis (Neumann rows)
+    long long sj, sk, nn;
+    int ni, nj, nk;
+};
+
+// gridDim.x is a multiple of NXCD (stencil_blocks), so lb is a bijection
+__device__ inline long long chunk_begin(long long nn, long long& end) {
+    int nb = gridDim.x;
+    int b = blockIdx.x;
+    int per_xcd = nb / NXCD;
+    int lb = (b % NXCD) * per_xcd + b / NXCD;  // logical, XCD-contiguous
+    long long chunk = ((nn + nb - 1) / nb + BLOCK - 1) / BLOCK * BLOCK;
+    long long beg = (long long)lb * chunk;
+    end = beg + chunk < nn ? beg + chunk : nn;
+    return beg;
+}
+
+__device__ inline double block_sum(double v) {
+    __shared__ double red[BLOCK / 64];
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+    __syncthreads();
+    double t = 0;
+    if (threadIdx.x == 0)
+        for (int w = 0; w < BLOCK / 64; w++) t += red[w];
+    __syncthreads();
+    return t;  // valid in thread 0
+}
+
+template <typename T>
+__device__ inline T nbr_of(uint8_t t, const T* v, long long u, const Stencil& s) {
+    switch (t) {
+        case NT_NEU_IM: return v[u + 1];
+        case NT_NEU_IP: return v[u - 1];
+        case NT_NEU_JM: return v[u + s.sj];
+        case NT_NEU_JP: return v[u - s.sj];
+        case NT_NEU_KM: return v[u + s.sk];
+        default: return v[u - s.sk];
+    }
+}
+__device__ inline double neu_coef(uint8_t t, const Stencil& s) {
+    return t <= NT_NEU_IP ? s.idx[0] : (t <= NT_NEU_JP ? s.idx[1] : s.idx[2]);
+}
+// diag(A) for the Jacobi preconditioner
+__device__ inline double diagA(uint8_t t, const Stencil& s) {
+    if (t == NT_REG) return -s.twos;
+    if (t == NT_DIR) return 1.0;
+    return neu_coef(t, s);
+}
+
+// (A v)_u for a vector given as a functor of the node index
+template <typename T, typename F>
+__device__ inline T apply_A(uint8_t t, long long u, const Stencil& s, F v) {
+    T vu = v(u);
+    if (t == NT_REG)
+        return (T)s.cz * (v(u - s.sk) + v(u + s.sk)) + (T)s.cy * (v(u - s.sj) + v(u + s.sj)) +
+               (T)s.cx * (v(u - 1) + v(u + 1)) - (T)s.twos * vu;
+    if (t == NT_DIR) return vu;
+    long long o;
+    switch (t) {
+        case NT_NEU_IM: o = 1; break;
+        case NT_NEU_IP: o = -1; break;
+        case NT_NEU_JM: o = s.sj; break;
+        case NT_NEU_JP: o = -s.sj; break;
+        case NT_NEU_KM: o = s.sk; break;
+        default: o = -s.sk; break;
+    }
+    return (T)neu_coef(t, s) * (vu - v(u + o));
+}
+
+// ------------------------------------------------------------------------
+// E-field
+// ------------------------------------------------------------------------
+__global__ void __launch_bounds__(BLOCK)
+    k_efield(GridC g, const double* __restrict__ phi, float4* __restrict__ ef) {
+    long long u = (long long)blockIdx.x * BLOCK + threadIdx.x;
+    if (u >= g.nn) return;
+    int i = (int)(u % g.ni);
+    long long r = u / g.ni;
+    int j = (int)(r % g.nj), k = (int)(r / g.nj);
+    int ijk[3] = {i, j, k};
+    int n[3] = {g.ni, g.nj, g.nk};
+    long long st[3] = {1, g.ni, (long long)g.ni * g.nj};
+    float e[3];
+#pragma unroll
+    for (int d = 0; d < 3; d++) {
+        double inv2 = 1.0 / (2 * g.dx[d]);
+        long long s = st[d];
+        double v;
+        if (ijk[d] == 0)
+            v = (3 * phi[u] - 4 * phi[u + s] + phi[u + 2 * s]) * inv2;
+        else if (ijk[d] == n[d] - 1)
+            v = (-phi[u - 2 * s] + 4 * phi[u - s] - 3 * phi[u]) * inv2;
+        else
+            v = (phi[u - s] - phi[u + s]) * inv2;
+        e[d] = (float)v;
+    }
+    ef[u] = make_float4(e[0], e[1], e[2], 0.f);
+}
+
+// ------------------------------------------------------------------------
+// red-black SOR
+// ------------------------------------------------------------------------
+struct SorArgs {
+    double omega, ieps, itwos, phi0, n0, inv_te0;
+    int color;
+};
+
+__global__ void __launch_bounds__(BLOCK)
+    k_sor(Stencil s, SorArgs a, const uint8_t* __restrict__ nt, double* __restrict__ phi,
+          const double* __restrict__ rho) {
+    long long half = (s.ni + 1) / 2;
+    long long items = half * s.nj * s.nk;
+    long long end;
+    for (long long m = chunk_begin(items, end) + threadIdx.x; m < end; m += BLOCK) {
+        long long row = m / half;
+        int h = (int)(m - row * half);
+        int j = (int)(row % s.nj), k = (int)(row / s.nj);
+        int i = 2 * h + ((j + k + a.color) & 1);
+        if (i >= s.ni) continue;
+        long long u = row * s.ni + i;
+        if (nt[u] != NT_REG) continue;
+        double p = phi[u];
+        double q = rho[u];
+        if (a.n0 != 0.0) q -= QE * (a.n0 * exp((p - a.phi0) * a.inv_te0));
+        double nw = (q * a.ieps + (phi[u - 1] + phi[u + 1]) * s.cx +
+                     (phi[u - s.sj] + phi[u + s.sj]) * s.cy + (phi[u - s.sk] + phi[u + s.sk]) * s.cz) *
+                    a.itwos;
+        phi[u] = p + a.omega * (nw - p);
+    }
+}
+
+// Neumann copies; phase 0 = k faces, 1 = j faces, 2 = i faces, so edge and
+// corner chains (an i face reads a j face reads a k face) resolve in order.
+__global__ void __launch_bounds__(BLOCK)
+    k_neumann(Stencil s, const uint8_t* __restrict__ nt, double* __restrict__ phi, int phase) {
+    long long u = (long long)blockIdx.x * BLOCK + threadIdx.x;
+    if (u >= s.nn) return;
+    uint8_t t = nt[u];
+    if (t == NT_REG || t == NT_DIR) return;
+    int ph = t <= NT_NEU_IP ? 2 : (t <= NT_NEU_JP ? 1 : 0);
+    if (ph != phase) return;
+    phi[u] = nbr_of<double>(t, phi, u, s);
+}
+
+// residual of the (Boltzmann) Poisson problem: out[0] += sum R^2 over all
+// non-Dirichlet rows (reference RMS numerator), out[1] += sum R^2 over
+// regular rows, out[2] += sum b^2 over regular rows.
+__global__ void __launch_bounds__(BLOCK)
+    k_resid(Stencil s, SorArgs a, const uint8_t* __restrict__ nt, const double* __restrict__ phi,
+            const double* __restrict__ rho, double* __restrict__ out) {
+    double r_all = 0, r_reg = 0, b_reg = 0;
+    long long end;
+    for (long long u = chunk_begin(s.nn, end) + threadIdx.x; u < end; u += BLOCK) {
+        uint8_t t = nt[u];
+        if (t == NT_DIR) continue;
+        double p = phi[u];
+        if (t != NT_REG) {
+            double R = p - nbr_of<double>(t, phi, u, s);
+            r_all += R * R;
+            continue;
+        }
+        double q = rho[u];
+        if (a.n0 != 0.0) q -= QE * (a.n0 * exp((p - a.phi0) * a.inv_te0));
+        double R = -p * s.twos + q * a.ieps + (phi[u - 1] + phi[u + 1]) * s.cx +
+                   (phi[u - s.sj] + phi[u + s.sj]) * s.cy + (phi[u - s.sk] + phi[u + s.sk]) * s.cz;
+        r_all += R * R;
+        r_reg += R * R;
+        double b = rho[u] * a.ieps;
+        b_reg += b * b;
+    }
+    double t0 = block_sum(r_all), t1 = block_sum(r_reg), t2 = block_sum(b_reg);
+    if (threadIdx.x == 0) {
+        atomicAdd(&out[0], t0);
+        atomicAdd(&out[1], t1);
+        atomicAdd(&out[2], t2);
+    }
+}
+
+// ------------------------------------------------------------------------
+// Newton residual and Jacobian shift (PotentialSolver.cpp:189-217)
+//   b_u = 0 (Neumann), phi_u (Dirichlet), -rho_u/eps0 (regular)
+//   F = A phi - b - [regular] q n0 exp((phi-phi0)/Te0)/eps0
+//   P = [regular] q n0/(eps0 Te0) exp((phi-phi0)/Te0)
+// out[0] += F.F (all rows), out[1] += F.F (regular), out[2] += b.b (regular)
+// ------------------------------------------------------------------------
+struct NewtonArgs {
+    double ieps, phi0, n0, inv_te0;
+};
+
+template <typename TF>
+__global__ void __launch_bounds__(BLOCK)
+    k_F(Stencil s, NewtonArgs a, const uint8_t* __restrict__ nt, const double* __restrict__ phi,
+        const double* __restrict__ rho, TF* __restrict__ F, double* __restrict__ P,
+        double* __restrict__ out) {
+    double f_all = 0, f_reg = 0, b_reg = 0;
+    long long end;
+    for (long long u = chunk_begin(s.nn, end) + threadIdx.x; u < end; u += BLOCK) {
+        uint8_t t = nt[u];
+        double Ap = apply_A<double>(t, u, s, [&](long long v) { return phi[v]; });
+        double b;
+        if (t == NT_REG)
+            b = -rho[u] * a.ieps;
+        else if (t == NT_DIR)
+            b = phi[u];
+        else
+            b = 0.0;
+        double f = Ap - b;
+        if (t == NT_REG && a.n0 != 0.0) {
+            double ex = exp((phi[u] - a.phi0) * a.inv_te0);
+            f -= QE * a.n0 * ex * a.ieps;
+            P[u] = QE * a.n0 * a.ieps * a.inv_te0 * ex;
+        } else if (P) {
+            P[u] = 0.0;
+        }
+        F[u] = (TF)f;
+        f_all += f * f;
+        if (t == NT_REG) {
+            f_reg += f * f;
+            b_reg += b * b;
+        }
+    }
+    double t0 = block_sum(f_all), t1 = block_sum(f_reg), t2 = block_sum(b_reg);
+    if (threadIdx.x == 0) {
+        atomicAdd(&out[0], t0);
+        atomicAdd(&out[1], t1);
+        atomicAdd(&out[2], t2);
+    }
+}
+
+// ------------------------------------------------------------------------
+// PCG on J y = F, J = A - diag(P), M = 1/diag(A).  History arrays hold
+// dz[t] = d_t.z_t, gs[t] = g_t.s_t, gg[t] = g_t.g_t.
+// ------------------------------------------------------------------------
+struct PcgHist {
+    double* dz;
+    double* gs;
+    double* gg;
+};
+
+template <typename T>
+struct PcgVec {
+    T *y, *g, *s, *d0, *d1, *z;
+    const T* F;
+    const double* P;  // may be null
+};
+
+// g = -F (y = 0), s = M g, d = -s, gs[0], gg[0]
+template <typename T>
+__global__ void __launch_bounds__(BLOCK)
+    k_pcg0(Stencil s, const uint8_t* __restrict__ nt, PcgVec<T> v, PcgHist h) {
+    double gs = 0, gg = 0;
+    long long end;
+    for (long long u = chunk_begin(s.nn, end) + threadIdx.x; u < end; u += BLOCK) {
+        T g = -v.F[u];
+        T m = (T)(1.0 / diagA(nt[u], s));
+        T sv = m * g;
+        v.y[u] = 0;
+        v.g[u] = g;
+        v.s[u] = sv;
+        v.d0[u] = 0;
+        gs += (double)g * (double)sv;
+        gg += (double)g * (double)g;
+    }
+    double a = block_sum(gs), b = block_sum(gg);
+    if (threadIdx.x == 0) {
+        atomicAdd(&h.gs[0], a);
+        atomicAdd(&h.gg[0], b);
+    }
+}
+
+// d_new = beta d_old - s (beta = gs[t]/gs[t-1], 0 at t = 0); z = J d_new
+template <typename T>
+__global__ void __launch_bounds__(BLOCK)
+    k_pcg1(Stencil s, const uint8_t* __restrict__ nt, const T* __restrict__ dold,
+           T* __restrict__ dnew, const T* __restrict__ sv, T* __restrict__ z,
+           const double* __restrict__ P, PcgHist h, int t) {
+    double beta = 0.0;
+    if (t > 0) {
+        double den = h.gs[t - 1];
+        beta = den != 0.0 ? h.gs[t] / den : 0.0;
+    }
+    T bt = (T)beta;
+    double acc = 0;
+    long long end;
+    for (long long u = chunk_begin(s.nn, end) + threadIdx.x; u < end; u += BLOCK) {
+        auto dn = [&](long long q) { return bt * dold[q] - sv[q]; };
+        uint8_t ty = nt[u];
+        T du = dn(u);
+        T zu = apply_A<T>(ty, u, s, dn);
+        if (P) zu -= (T)P[u] * du;
+        dnew[u] = du;
+        z[u] = zu;
+        acc += (double)du * (double)zu;
+    }
+    double a = block_sum(acc);
+    if (threadIdx.x == 0) atomicAdd(&h.dz[t], a);
+}
+
+// alpha = gs[t]/dz[t]; y += alpha d; g += alpha z; s = M g; gs[t+1], gg[t+1]
+template <typename T>
+__global__ void __launch_bounds__(BLOCK)
+    k_pcg2(Stencil s, const uint8_t* __restrict__ nt, T* __restrict__ y, T* __restrict__ g,
+           T* __restrict__ sv, const T* __restrict__ d, const T* __restrict__ z, PcgHist h,
+           int t) {
+    double den = h.dz[t];
+    T alpha = (T)(den != 0.0 ? h.gs[t] / den : 0.0);
+    double gs = 0, gg = 0;
+    long long end;
+    for (long long u = chunk_begin(s.nn, end) + threadIdx.x; u < end; u += BLOCK) {
+        T gu = g[u] + alpha * z[u];
+        y[u] += alpha * d[u];
+        T m = (T)(1.0 / diagA(nt[u], s));
+        T su = m * gu;
+        g[u] = gu;
+        sv[u] = su;
+        gs += (double)gu * (double)su;
+        gg += (double)gu * (double)gu;
+    }
+    double a = block_sum(gs), b = block_sum(gg);
+    if (threadIdx.x == 0) {
+        atomicAdd(&h.gs[t + 1], a);
+        atomicAdd(&h.gg[t + 1], b);
+    }
+}
+
+// phi -= y on non-Dirichlet rows (PotentialSolver.cpp:225-228)
+template <typename T>
+__global__ void __launch_bounds__(BLOCK)
+    k_update_phi(Stencil s, const uint8_t* __restrict__ nt, double* __restrict__ phi,
+                 const T* __restrict__ y, double* __restrict__ out) {
+    double yy = 0;
+    long long end;
+    for (long long u = chunk_begin(s.nn, end) + threadIdx.x; u < end; u += BLOCK) {
+        if (nt[u] == NT_DIR) continue;
+        double yv = (double)y[u];
+        phi[u] -= yv;
+        yy += yv * yv;
+    }
+    double a = block_sum(yy);
+    if (threadIdx.x == 0) atomicAdd(out, a);
+}
+
+Stencil make_stencil(const GridC& g, bool axis_swap) {
+    Stencil s;
+    double d2x = 1.0 / (g.dx[0] * g.dx[0]), d2y = 1.0 / (g.dx[1] * g.dx[1]),
+           d2z = 1.0 / (g.dx[2] * g.dx[2]);
+    s.cx = axis_swap ? d2z : d2x;
+    s.cy = d2y;
+    s.cz = axis_swap ? d2x : d2z;
+    s.twos = 2.0 * (d2x + d2y + d2z);
+    for (int d = 0; d < 3; d++) s.idx[d] = 1.0 / g.dx[d];
+    s.sj = g.ni;
+    s.sk = (long long)g.ni * g.nj;
+    s.nn = g.nn;
+    s.ni = g.ni;
+    s.nj = g.nj;
+    s.nk = g.nk;
+    return s;
+}
+
+unsigned stencil_blocks(long long items) {
+    long long b = (items + BLOCK * 8 - 1) / (BLOCK * 8);
+    if (b > 2048) b = 2048;
+    b = (b + NXCD - 1) / NXCD * NXCD;  // multiple of NXCD, at least NXCD
+    return (unsigned)b;
+}
+
+}  // namespace
+
+// ------------------------------------------------------------------------
+extern "C" int pic_efield(pic_ctx* ctx) {
+    if (!ctx) return PIC_ERR_ARG;
+    {
+        PicProfScope ps(ctx, PIC_K_EFIELD);
+        k_efield<<<grid_for((size_t)ctx->g.nn, BLOCK), BLOCK, 0, ctx->stream>>>(ctx->g, ctx->phi,
+                                                                                ctx->ef);
+    }
+    PIC_HIP(ctx, hipGetLastError());
+    return PIC_OK;
+}
+
+static bool has_neumann(pic_ctx* ctx) {
+    const GridC& g = ctx->g;
+    // a face node that is neither an object node nor flagged Dirichlet
+    for (int k = 0; k < g.nk; k++)
+        for (int j = 0; j < g.nj; j++)
+            for (int i = 0; i < g.ni; i++) {
+                bool face = i == 0 || i == g.ni - 1 || j == 0 || j == g.nj - 1 || k == 0 ||
+                            k == g.nk - 1;
+                if (!face) continue;
+                size_t u = ((size_t)k * g.nj + j) * g.ni + i;
+                if (ctx->object_id[u] == 0 && ctx->node_type_in[u] != 2) return true;
+            }
+    return false;
+}
+
+static int residual_norms(pic_ctx* ctx, const Stencil& st, const SorArgs& a, double out[3]) {
+    PIC_HIP(ctx, hipMemsetAsync(ctx->scratch, 0, 3 * sizeof(double), ctx->stream));
+    k_resid<<<stencil_blocks(st.nn), BLOCK, 0, ctx->stream>>>(st, a, ctx->ntype, ctx->phi, ctx->rho,
+                                                              ctx->scratch);
+    PIC_HIP(ctx, hipGetLastError());
+    PIC_HIP(ctx, hipMemcpyAsync(out, ctx->scratch, 3 * sizeof(double), hipMemcpyDeviceToHost,
+                                ctx->stream));
+    PIC_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    return PIC_OK;
+}
+
+static double crit_value(const pic_solver_params* p, const double r[3], double nn) {
+    if (p->tol_kind == PIC_TOL_REF_RMS) return std::sqrt(r[0] / nn);
+    return r[2] > 0 ? std::sqrt(r[1] / r[2]) : std::sqrt(r[1]);
+}
+
+static int solve_sor(pic_ctx* ctx, const pic_solver_params* p, pic_solver_stats* st) {
+    const GridC& g = ctx->g;
+    Stencil s = make_stencil(g, false);
+    SorArgs a;
+    a.omega = p->omega > 0 ? p->omega : 1.4;
+    a.ieps = 1.0 / EPS0;
+    a.itwos = 1.0 / s.twos;
+    a.phi0 = p->phi0;
+    a.n0 = p->n0;
+    a.inv_te0 = 1.0 / (p->Te0 != 0 ? p->Te0 : 1.0);
+    bool neu = has_neumann(ctx);
+    int check = p->check_every > 0 ? p->check_every : 25;
+    long long half_items = (long long)((g.ni + 1) / 2) * g.nj * g.nk;
+    double r[3] = {0, 0, 0};
+    int it;
+    bool conv = false;
+    double crit = 0;
+    for (it = 0; it < p->max_it; it++) {
+        {
+            PicProfScope ps(ctx, PIC_K_SOR);
+            for (int c = 0; c < 2; c++) {
+                a.color = c;
+                k_sor<<<stencil_blocks(half_items), BLOCK, 0, ctx->stream>>>(s, a, ctx->ntype,
+                                                                            ctx->phi, ctx->rho);
+            }
+            if (neu)
+                for (int ph = 0; ph < 3; ph++)
+                    k_neumann<<<grid_for((size_t)g.nn, BLOCK), BLOCK, 0, ctx->stream>>>(
+                        s, ctx->ntype, ctx->phi, ph);
+        }
+        PIC_HIP(ctx, hipGetLastError());
+        if (it % check == 0) {  // PotentialSolver.cpp:124
+            int rc = residual_norms(ctx, s, a, r);
+            if (rc) return rc;
+            crit = crit_value(p, r, (double)g.nn);
+            if (crit < p->tol) {
+                conv = true;
+                break;
+            }
+        }
+    }
+    int rc = residual_norms(ctx, s, a, r);
+    if (rc) return rc;
+    st->converged = conv;
+    st->iterations = conv ? it + 1 : it;
+    st->newton_iterations = 0;
+    st->ref_l2 = std::sqrt(r[0] / (double)g.nn);
+    st->rel_residual = r[2] > 0 ? std::sqrt(r[1] / r[2]) : std::sqrt(r[1]);
+    return conv ? PIC_OK : PIC_NOT_CONVERGED;
+}
+
+// inner PCG on J y = F (vectors of type T); stops when ||g|| <= target or at
+// max_it; returns iterations used.
+template <typename T>
+static int pcg_inner(pic_ctx* ctx, const Stencil& s, T* base, const double* P, double target2,
+                     int max_it, int check, int* iters, double* gg_last) {
+    size_t nn = (size_t)s.nn;
+    PcgVec<T> v;
+    v.F = base;
+    v.y = base + nn;
+    v.g = base + 2 * nn;
+    v.s = base + 3 * nn;
+    v.d0 = base + 4 * nn;
+    v.d1 = base + 5 * nn;
+    v.z = base + 6 * nn;
+    v.P = P;
+    double* hist = (double*)(base + 7 * nn);
+    hist = (double*)(((uintptr_t)hist + 255) & ~(uintptr_t)255);
+    PcgHist h;
+    h.dz = hist;
+    h.gs = hist + (max_it + 2);
+    h.gg = hist + 2 * (max_it + 2);
+    PIC_HIP(ctx, hipMemsetAsync(hist, 0, 3 * (size_t)(max_it + 2) * sizeof(double), ctx->stream));
+    unsigned nb = stencil_blocks(s.nn);
+    {
+        // set-up launches are not counted as PCG iterations
+        k_pcg0<T><<<nb, BLOCK, 0, ctx->stream>>>(s, ctx->ntype, v, h);
+    }
+    PIC_HIP(ctx, hipGetLastError());
+    std::vector<double> hg(max_it + 2);
+    PIC_HIP(ctx, hipMemcpyAsync(hg.data(), h.gg, sizeof(double), hipMemcpyDeviceToHost,
+                                ctx->stream));
+    PIC_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    *gg_last = hg[0];
+    if (hg[0] <= target2) {  // already converged (zero RHS included)
+        *iters = 0;
+        return PIC_OK;
+    }
+    T* dold = v.d0;
+    T* dnew = v.d1;
+    int t = 0, last_checked = 0;
+    while (t < max_it) {
+        int upto = std::min(max_it, t + check);
+        for (; t < upto; t++) {  // one profiled "launch" = one PCG iteration (2 kernels)
+            PicProfScope ps(ctx, PIC_K_PCG);
+            k_pcg1<T><<<nb, BLOCK, 0, ctx->stream>>>(s, ctx->ntype, dold, dnew, v.s, v.z, P, h,
+                                                     t);
+            k_pcg2<T><<<nb, BLOCK, 0, ctx->stream>>>(s, ctx->ntype, v.y, v.g, v.s, dnew, v.z, h,
+                                                     t);
+            std::swap(dold, dnew);
+        }
+        PIC_HIP(ctx, hipGetLastError());
+        PIC_HIP(ctx, hipMemcpyAsync(hg.data() + last_checked + 1, h.gg + last_checked + 1,
+                                    (size_t)(t - last_checked) * sizeof(double),
+                                    hipMemcpyDeviceToHost, ctx->stream));
+        PIC_HIP(ctx, hipStreamSynchronize(ctx->stream));
+        for (int q = last_checked + 1; q <= t; q++) {
+            if (!(hg[q] == hg[q])) return pic_fail(ctx, PIC_ERR_NUMERIC, "PCG produced NaN");
+            if (hg[q] <= target2) {
+                *iters = t;
+                *gg_last = hg[q];
+                return PIC_OK;
+            }
+        }
+        last_checked = t;
+        *gg_last = hg[t];
+    }
+    *iters = t;
+    return PIC_NOT_CONVERGED;
+}
+
+// Newton + PCG (PotentialSolver.cpp:178-241).  The linear case (n0 == 0)
+// needs a single Newton step; each Newton step restarts PCG from y = 0.
+static int solve_pcg(pic_ctx* ctx, const pic_solver_params* p, pic_solver_stats* st, bool mixed) {
+    const GridC& g = ctx->g;
+    size_t nn = (size_t)g.nn;
+    Stencil s = make_stencil(g, (p->flags & PIC_SOLVE_AXIS_SWAP) != 0);
+    bool nonlinear = p->n0 != 0.0;
+    int max_it = p->max_it > 0 ? p->max_it : 10000;
+    int check = p->check_every > 0 ? p->check_every : 8;
+    size_t tsz = mixed ? sizeof(float) : sizeof(double);
+    size_t bytes = 7 * nn * tsz + 256 + 3 * (size_t)(max_it + 2) * sizeof(double);
+    size_t pbytes = nonlinear ? nn * sizeof(double) : 0;
+    int rc = pic_ensure_work(ctx, bytes + pbytes + 256);
+    if (rc) return rc;
+    char* wbase = (char*)ctx->work;
+    double* P = nonlinear ? (double*)(wbase + ((bytes + 255) & ~(size_t)255)) : nullptr;
+    NewtonArgs na;
+    na.ieps = 1.0 / EPS0;
+    na.phi0 = p->phi0;
+    na.n0 = p->n0;
+    na.inv_te0 = 1.0 / (p->Te0 != 0 ? p->Te0 : 1.0);
+    unsigned nb = stencil_blocks(s.nn);
+    int total_it = 0, newton = 0;
+    bool conv = false;
+    double fr[3];
+    const int newton_max = nonlinear ? 20 : 1000;  // PotentialSolver.cpp:179
+    for (newton = 0; newton < newton_max; newton++) {
+        PIC_HIP(ctx, hipMemsetAsync(ctx->scratch, 0, 4 * sizeof(double), ctx->stream));
+        {
+            // set-up launches are not counted as PCG iterations
+            if (mixed)
+                k_F<float><<<nb, BLOCK, 0, ctx->stream>>>(s, na, ctx->ntype, ctx->phi, ctx->rho,
+                                                          (float*)wbase, P, ctx->scratch);
+            else
+                k_F<double><<<nb, BLOCK, 0, ctx->stream>>>(s, na, ctx->ntype, ctx->phi, ctx->rho,
+                                                           (double*)wbase, P, ctx->scratch);
+        }
+        PIC_HIP(ctx, hipGetLastError());
+        PIC_HIP(ctx, hipMemcpyAsync(fr, ctx->scratch, sizeof(fr), hipMemcpyDeviceToHost,
+                                    ctx->stream));
+        PIC_HIP(ctx, hipStreamSynchronize(ctx->stream));
+        // target on the inner residual g (g = -F at y = 0)
+        double target2;
+        if (p->tol_kind == PIC_TOL_REF_RMS)
+            target2 = p->tol * p->tol * (double)nn;
+        else
+            target2 = p->tol * p->tol * (fr[2] > 0 ? fr[2] : 1.0);
+        if (!nonlinear && fr[0] <= target2) {
+            conv = true;
+            break;
+        }
+        double inner_target2 = target2;
+        if (mixed) {  // FP32 inner solve: reduce the residual by ~1e-3 per pass
+            double floor2 = fr[0] * 1e-6;
+            inner_target2 = std::max(target2 * 0.25, floor2);
+        }
+        int its = 0;
+        double gg = 0;
+        if (mixed)
+            rc = pcg_inner<float>(ctx, s, (float*)wbase, P, inner_target2, max_it - total_it, check,
+                                  &its, &gg);
+        else
+            rc = pcg_inner<double>(ctx, s, (double*)wbase, P, inner_target2, max_it - total_it,
+                                   check, &its, &gg);
+        total_it += its;
+        if (rc < 0) return rc;
+        PIC_HIP(ctx, hipMemsetAsync(ctx->scratch, 0, sizeof(double), ctx->stream));
+        {
+            // set-up launches are not counted as PCG iterations
+            if (mixed)
+                k_update_phi<float><<<nb, BLOCK, 0, ctx->stream>>>(
+                    s, ctx->ntype, ctx->phi, (const float*)wbase + nn, ctx->scratch);
+            else
+                k_update_phi<double><<<nb, BLOCK, 0, ctx->stream>>>(
+                    s, ctx->ntype, ctx->phi, (const double*)wbase + nn, ctx->scratch);
+        }
+        PIC_HIP(ctx, hipGetLastError());
+        double yy;
+        PIC_HIP(ctx, hipMemcpyAsync(&yy, ctx->scratch, sizeof(double), hipMemcpyDeviceToHost,
+                                    ctx->stream));
+        PIC_HIP(ctx, hipStreamSynchronize(ctx->stream));
+        if (nonlinear) {
+            if (std::sqrt(yy / (double)nn) < 1e-3) {  // NR_TOL, PotentialSolver.cpp:180, :229-233
+                conv = true;
+                newton++;
+                break;
+            }
+        }
+        if (total_it >= max_it) break;
+    }
+    SorArgs a;
+    a.omega = 1;
+    a.ieps = 1.0 / EPS0;
+    a.itwos = 1.0 / s.twos;
+    a.phi0 = p->phi0;
+    a.n0 = p->n0;
+    a.inv_te0 = na.inv_te0;
+    double r[3];
+    Stencil sphys = make_stencil(g, (p->flags & PIC_SOLVE_AXIS_SWAP) != 0);
+    rc = residual_norms(ctx, sphys, a, r);
+    if (rc) return rc;
+    st->converged = conv;
+    st->iterations = total_it;
+    st->newton_iterations = newton;
+    st->ref_l2 = std::sqrt(r[0] / (double)nn);
+    st->rel_residual = r[2] > 0 ? std::sqrt(r[1] / r[2]) : std::sqrt(r[1]);
+    return conv ? PIC_OK : PIC_NOT_CONVERGED;
+}
+
+int pic_solve_learned(pic_ctx* ctx, const pic_solver_params* p, pic_solver_stats* st);
+
+extern "C" int pic_poisson(pic_ctx* ctx, const pic_solver_params* p, pic_solver_stats* st) {
+    if (!ctx || !p) return PIC_ERR_ARG;
+    if (!(p->tol > 0) || p->max_it <= 0) return pic_fail(ctx, PIC_ERR_ARG, "bad solver params");
+    int rc = pic_ensure_ntype(ctx);
+    if (rc) return rc;
+    pic_solver_stats local;
+    if (!st) st = &local;
+    *st = pic_solver_stats{};
+    hipEvent_t e0, e1;
+    PIC_HIP(ctx, hipEventCreate(&e0));
+    PIC_HIP(ctx, hipEventCreate(&e1));
+    PIC_HIP(ctx, hipEventRecord(e0, ctx->stream));
+    switch (p->method) {
+        case PIC_SOLVER_SOR_RB: rc = solve_sor(ctx, p, st); break;
+        case PIC_SOLVER_PCG: rc = solve_pcg(ctx, p, st, false); break;
+        case PIC_SOLVER_PCG_MIXED:
+            if (p->n0 != 0.0) {
+                rc = pic_fail(ctx, PIC_ERR_ARG, "mixed precision PCG is linear only");
+                break;
+            }
+            rc = solve_pcg(ctx, p, st, true);
+            break;
+        case PIC_SOLVER_LEARNED: rc = pic_solve_learned(ctx, p, st); break;
+        default: rc = pic_fail(ctx, PIC_ERR_ARG, "unknown solver method");
+    }
+    (void)hipEventRecord(e1, ctx->stream);
+    (void)hipEventSynchronize(e1);
+    float ms = 0;
+    (void)hipEventElapsedTime(&ms, e0, e1);
+    st->seconds = ms * 1e-3;
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    return rc;
+}

@@ -1,0 +1,210 @@
+/*
+ * pic_abi.h -- C ABI of the MI355X (gfx950) PIC/DSMC hot-path library.
+ *
+ * The reference (Misquic/Engineering-degree-in-Plasma-Simulations, ch4/v3)
+ * has no FFI: its boundary is the concrete C++ class API called from
+ * main.cpp.  Each entry point below replaces one of those methods; the
+ * comment names the reference declaration it stands in for.  The C++ host
+ * classes in engineering-degree-in-plasma-simulations_amd/host/ keep the
+ * reference's class names and signatures and call these functions.
+ *
+ * Conventions
+ *   - every function returns an int status: PIC_OK (0), a negative PIC_ERR_*
+ *     code, or PIC_NOT_CONVERGED (1) from the field solvers; no C++
+ *     exception crosses this boundary; pic_last_error() explains failures.
+ *   - host memory is caller-owned; device memory is library-owned.
+ *   - all work of one context is ordered on one HIP stream owned by the
+ *     context; downloads and *_stats outputs are synchronous.
+ *   - node fields on the host are flat, i-fastest: u = (k*nj + j)*ni + i
+ *     (the reference's solver index, PotentialSolver.cpp:421); vector fields
+ *     are interleaved xyz per node.  Cell index c = (k*(nj-1)+j)*(ni-1)+i
+ *     (World.cpp:134-137).
+ *   - particles cross the boundary as 7 doubles {x,y,z,vx,vy,vz,w}, the
+ *     layout of the reference's Particle (Species.h:12-29), so
+ *     std::vector<Particle>::data() can be passed directly.
+ *   - one context per thread; calls on one context must not overlap.
+ */
+#ifndef PIC_ABI_H
+#define PIC_ABI_H
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define PIC_OK 0
+#define PIC_NOT_CONVERGED 1
+#define PIC_ERR_ARG (-1)
+#define PIC_ERR_HIP (-2)
+#define PIC_ERR_CAPACITY (-3)
+#define PIC_ERR_STATE (-4)
+#define PIC_ERR_NOMEM (-5)
+#define PIC_ERR_NUMERIC (-6)
+
+typedef struct pic_ctx pic_ctx;
+
+/* World(ni,nj,nk, x1,y1,z1, x2,y2,z2): World.cpp:16-21, setExtents :63-78 */
+typedef struct {
+    int ni, nj, nk;
+    double x0[3];
+    double xm[3];
+} pic_grid_desc;
+
+/* ---- context ---- */
+/* World::World (World.cpp:16-21) + computeNodeVolumes (:353-367) */
+int pic_create(const pic_grid_desc* grid, pic_ctx** out);
+void pic_destroy(pic_ctx* ctx);
+const char* pic_last_error(const pic_ctx* ctx);
+/* World::setTime (World.cpp:321-324) */
+int pic_set_dt(pic_ctx* ctx, double dt);
+/* blocks until all queued work of the context is done */
+int pic_sync(pic_ctx* ctx);
+
+/* ---- geometry ---- */
+/* World::addObject<Rectangle>(centre, phi, sides) + computeObjectID
+ * (World.h:110-121, World.cpp:276-292, Object.cpp:166-172, :231-238).
+ * Rectangles are absorbing for particles and Dirichlet for the solver. */
+int pic_add_rectangle(pic_ctx* ctx, const double centre[3], const double sides[3], double phi);
+/* World::dirichletBoundries (World.cpp:263-273) */
+int pic_dirichlet_walls(pic_ctx* ctx);
+/* World::node_type / object_id public fields (World.h:54-57) as i-fastest
+ * int32 arrays (node_type: 0 REGULAR, 1 NEUMANN, 2 DIRICHLET). */
+int pic_get_node_flags(pic_ctx* ctx, int32_t* object_id, int32_t* node_type);
+
+/* ---- species ---- */
+/* Species(name, mass, charge, world, mpw0) (Species.cpp:30-34); capacity
+ * bounds the particle count (births included). */
+int pic_species_add(pic_ctx* ctx, double mass, double charge, double mpw0, size_t capacity,
+                    int* sid);
+/* std::vector<Particle>& getPartRef() (Species.cpp:823-825): replace the
+ * particle set (positions outside the box or in an object are dropped at
+ * the next sort, as addParticle would, Species.cpp:420-426). */
+int pic_particles_upload(pic_ctx* ctx, int sid, const double* aos7, size_t n);
+/* const std::vector<Particle>& getConstPartRef() (Species.cpp:820-822) */
+int pic_particles_download(pic_ctx* ctx, int sid, double* aos7, size_t capacity, size_t* n);
+/* size_t getNumParticles() (Species.cpp:43-45) */
+int pic_particles_count(pic_ctx* ctx, int sid, size_t* n);
+/* loadParticleBoxThermal(x0, sides, num_den, T) (Species.cpp:564-599) with
+ * addParticle's half-step rewind (Species.cpp:427-433).  Draws come from a
+ * counter-based Philox4x32-10 stream keyed by seed.  mode 0 reproduces the
+ * reference speed sampler (Species.cpp:855-869); mode 1 draws a Maxwellian. */
+int pic_load_box_thermal(pic_ctx* ctx, int sid, const double centre[3], const double sides[3],
+                         double num_den, double T, uint32_t seed, int mode, size_t* n_loaded);
+
+/* advanceElectrons(dt) / advanceNonElectron(..., dt) (Species.cpp:258-399,
+ * :47-77, :170-256): gather E (CIC), kick, drift, remove particles leaving
+ * the box or entering an object.  Removed particles disappear at the next
+ * pic_sort.  Object hits are absorbing for every species. */
+int pic_push(pic_ctx* ctx, int sid, double dt);
+/* sortIndexes() (Species.cpp:905-929): counting sort of the particle arrays
+ * by cell; compacts removed particles; cell offsets stay on the device. */
+int pic_sort(pic_ctx* ctx, int sid);
+/* computeNumberDensity() (Species.cpp:401-416); with PIC_DEPOSIT_MOMENTS
+ * also sampleMoments() (Species.cpp:767-776), accumulated until
+ * pic_clear_samples (Species.cpp:805-812).  Particles must be sorted. */
+#define PIC_DEPOSIT_DENSITY 1
+#define PIC_DEPOSIT_MOMENTS 2
+int pic_deposit(pic_ctx* ctx, int sid, int flags);
+int pic_clear_samples(pic_ctx* ctx, int sid);
+/* World::computeChargeDensity (World.cpp:193-200) over all species */
+int pic_charge_density(pic_ctx* ctx);
+
+/* ---- field solve ---- */
+/* PotentialSolver::solve() (PotentialSolver.cpp:55-67): GS-SOR (:69-166),
+ * Newton + PCG (:178-313), plus the learned-preconditioned variant. */
+#define PIC_SOLVER_SOR_RB 0  /* red-black SOR, Boltzmann term, Neumann faces */
+#define PIC_SOLVER_PCG 1     /* Newton outer loop + Jacobi PCG, FP64 */
+#define PIC_SOLVER_PCG_MIXED 2 /* FP64 residual + FP32 inner PCG (refinement) */
+#define PIC_SOLVER_LEARNED 3 /* U-Net warm start + preconditioner, flexible PCG */
+#define PIC_TOL_REF_RMS 0    /* reference criterion: sqrt(sum R^2 / N) < tol */
+#define PIC_TOL_REL 1        /* ||b - A phi||_2 / ||b||_2 < tol (regular nodes) */
+#define PIC_SOLVE_AXIS_SWAP 1 /* PCG coefficients placed as in PotentialSolver.cpp:457-463 */
+typedef struct {
+    int method;
+    int tol_kind;
+    double tol;
+    int max_it;        /* SOR sweeps / PCG iterations per linear solve */
+    double omega;      /* SOR weight (reference 1.4, PotentialSolver.h:38) */
+    int check_every;   /* SOR/PCG convergence check interval (reference 25 for SOR) */
+    double phi0, n0, Te0; /* setReferenceValues (PotentialSolver.cpp:409-413) */
+    int flags;
+} pic_solver_params;
+typedef struct {
+    int converged;
+    int iterations;      /* SOR sweeps or total inner PCG iterations */
+    int newton_iterations;
+    double rel_residual; /* ||b - A phi|| / ||b|| over regular nodes, FP64 */
+    double ref_l2;       /* the reference's RMS criterion value */
+    double seconds;      /* device time of the solve */
+} pic_solver_stats;
+int pic_poisson(pic_ctx* ctx, const pic_solver_params* params, pic_solver_stats* stats);
+/* PotentialSolver::computeEF (PotentialSolver.cpp:354-408) */
+int pic_efield(pic_ctx* ctx);
+
+/* ---- field transfer ---- */
+#define PIC_FIELD_PHI 0
+#define PIC_FIELD_RHO 1
+#define PIC_FIELD_EF 2       /* 3 doubles per node */
+#define PIC_FIELD_NODE_VOL 3 /* download only */
+int pic_field_upload(pic_ctx* ctx, int which, const double* src);
+int pic_field_download(pic_ctx* ctx, int which, double* dst);
+#define PIC_SFIELD_DEN 0     /* Species::den */
+#define PIC_SFIELD_NSUM 1    /* n_sum */
+#define PIC_SFIELD_NVSUM 2   /* nv_sum, 3 per node */
+#define PIC_SFIELD_NUU 3
+#define PIC_SFIELD_NVV 4
+#define PIC_SFIELD_NWW 5
+#define PIC_SFIELD_MACRO_COUNT 6 /* per cell (computeMacroParticlesCount, Species.cpp:813-819) */
+int pic_species_field_download(pic_ctx* ctx, int sid, int which, double* dst);
+
+/* ---- diagnostics ---- */
+/* {getMicroCount, getMomentum x,y,z, getKE, getPE} (Species.cpp:732-763),
+ * FP64 device reductions */
+int pic_diag_sums(pic_ctx* ctx, int sid, double out[6]);
+/* World::getPE (World.cpp:108-118) */
+int pic_field_energy(pic_ctx* ctx, double* out);
+
+/* ---- DSMC: MC_MEX_Ionization (Interactions.h:99-143) ---- */
+typedef struct {
+    double E_ion_J;        /* neutral ionization energy [J] */
+    double c0, c1, c2;     /* sigma_iz fit (Interactions.cpp:506-508) */
+    double B_inc;          /* eV (Interactions.cpp:511) */
+    double wsv_max_init;   /* initial W*sigma*v_rel max (Interactions.cpp:534) */
+} pic_mcc_desc;
+typedef struct {
+    long long collisions, ionizations, groups;
+    double wsv_max; /* W_sigma_v_rel_max after the call */
+} pic_mcc_stats;
+/* MC_MEX_Ionization(neutrals, ions, electrons, world, table_path) ctor
+ * (Interactions.cpp:476-539); the elastic table is passed as arrays. */
+int pic_mcc_create(pic_ctx* ctx, int sid_neutral, int sid_ion, int sid_electron,
+                   const pic_mcc_desc* desc, const double* sig_E, const double* sig_v, int nsig,
+                   int* mcc_id);
+/* Interaction::apply(dt) (Interactions.cpp:567-598 -> :600-762).  Needs all
+ * three species sorted.  Draws: Philox4x32-10 keyed by (seed, step). */
+int pic_mcc_apply(pic_ctx* ctx, int mcc_id, double dt, uint32_t seed, uint32_t step,
+                  pic_mcc_stats* stats);
+
+/* ---- profiling: HIP events around every launch of the context ---- */
+#define PIC_K_PUSH 0
+#define PIC_K_SORT 1
+#define PIC_K_DEPOSIT 2
+#define PIC_K_RHO 3
+#define PIC_K_EFIELD 4
+#define PIC_K_SOR 5
+#define PIC_K_PCG 6
+#define PIC_K_MCC 7
+#define PIC_K_LOAD 8
+#define PIC_K_UNET 9
+#define PIC_K_SCATTER 10 /* the sort's scatter pass (PIC_K_SORT: histogram + scan) */
+#define PIC_K_COUNT 11
+int pic_profile_enable(pic_ctx* ctx, int on);
+/* total milliseconds and launch count per category since the last reset */
+int pic_profile_read(pic_ctx* ctx, double ms[PIC_K_COUNT], long long launches[PIC_K_COUNT]);
+int pic_profile_reset(pic_ctx* ctx);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -280,7 +280,10 @@ static void bench(int n, long np, int steps, int threads, int do_solve) {
     rnd = Rnd(2024u);
     type_calc3 L = world.getL();
     double vol = L[0] * L[1] * L[2];
-    ele.loadParticleBoxThermal(world.getXc(), L, (double)np / vol, 2.0 * 11604.52);
+    // 0.98 L: a particle within rounding of xm gets lc == n-1 and the
+    // reference gather (Field.h:204-221) would read past the last node
+    ele.loadParticleBoxThermal(world.getXc(), L * 0.98, (double)np / (vol * 0.98 * 0.98 * 0.98),
+                               2.0 * 11604.52);
     // a smooth field so the gather does real work
     for (int i = 0; i < n; i++)
         for (int j = 0; j < n; j++)
