@@ -109,6 +109,7 @@ def main():
     ap.add_argument("--cpu-particles", type=int, default=8_000_000)
     ap.add_argument("--cpu-steps", type=int, default=8)
     ap.add_argument("--copy-calibration", action="store_true")
+    ap.add_argument("--step-solver", default="sor_rb", choices=["sor_rb", "pcg_mixed", "pcg_fp64"])
     args = ap.parse_args()
     if args.copy_calibration:
         copy_calibration()
@@ -150,7 +151,11 @@ def main():
         ctx.sort(s)
         ctx.deposit(s)
     ctx.charge_density()
-    solve_kw = dict(method=pg.SOLVER_PCG_MIXED, tol=1e-6, tol_kind=pg.TOL_REL, max_it=20000)
+    omega_opt = 2.0 / (1.0 + np.sin(np.pi / (n - 1)))
+    solve_kw = {"sor_rb": dict(method=pg.SOLVER_SOR_RB, omega=omega_opt, check_every=10),
+                "pcg_mixed": dict(method=pg.SOLVER_PCG_MIXED),
+                "pcg_fp64": dict(method=pg.SOLVER_PCG)}[args.step_solver]
+    solve_kw.update(tol=1e-6, tol_kind=pg.TOL_REL, max_it=20000)
     ctx.poisson(**solve_kw)
     ctx.efield()
     print("loaded %d + %d particles" % (ctx.count(se), ctx.count(si)), file=sys.stderr)
@@ -245,7 +250,7 @@ def main():
         "config": {"workload": "scaled plain PIC, %d^3 nodes, 2 x %d particles, counting sort "
                                "every step, Poisson to 1e-6 rel. residual every step" % (n, N),
                    "grid_nodes": n ** 3, "particles_per_species": N, "dt": dt,
-                   "poisson_step_solver": "pcg_mixed (FP64 residual, FP32 inner PCG)"},
+                   "poisson_step_solver": args.step_solver},
         "particles_alive": n_alive,
         "poisson_iterations_per_step": float(np.mean(solve_iters)) if solve_iters else None,
         "stages": stages,

@@ -48,8 +48,18 @@ def build(verbose=True):
     if r.returncode != 0:
         raise RuntimeError("link failed:\n" + r.stderr[-4000:])
     os.replace(tmp, LIB)
+    # host classes (the reference's interfaces) + driver, linked to the library
+    driver = os.path.join(HERE, "pic_driver")
+    cmd = ["g++", "-O2", "-std=c++17", "-Wall", "-I", os.path.join(ROOT, "include"),
+           os.path.join(HERE, "host", "pic_host.cpp"), os.path.join(HERE, "host", "pic_main.cpp"),
+           "-o", driver + ".tmp", "-L", HERE, "-lpic_mi355", "-Wl,-rpath,$ORIGIN",
+           "-Wl,-rpath,/opt/rocm/lib"]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError("driver build failed:\n" + r.stderr[-4000:])
+    os.replace(driver + ".tmp", driver)
     if verbose:
-        print("built", LIB)
+        print("built", LIB, "and", driver)
     return LIB
 
 

@@ -279,39 +279,82 @@ __global__ void __launch_bounds__(BLOCK)
     const int line = 4 * ni;
     for (int t = threadIdx.x; t < NCH * line; t += BLOCK) tile[t] = 0.f;
     __syncthreads();
-    for (uint32_t p = p0 + threadIdx.x; p < p1; p += BLOCK) {
-        int i = (int)(b.cell[p] & 0x7FFu);
-        float fx = b.f[0][p], fy = b.f[1][p], fz = b.f[2][p];
-        float w = b.w[p];
-        float gx = 1.f - fx, gy = 1.f - fy, gz = 1.f - fz;
-        float wt[8];  // index (dk*2 + dj)*2 + di
-        wt[0] = gx * gy * gz;
-        wt[1] = fx * gy * gz;
-        wt[2] = gx * fy * gz;
-        wt[3] = fx * fy * gz;
-        wt[4] = gx * gy * fz;
-        wt[5] = fx * gy * fz;
-        wt[6] = gx * fy * fz;
-        wt[7] = fx * fy * fz;
-        float val[NCH];
-        val[0] = w;
-        if (NCH > 1) {
-            float vx = b.v[0][p], vy = b.v[1][p], vz = b.v[2][p];
-            val[1 % NCH] = w * vx;
-            val[2 % NCH] = w * vy;
-            val[3 % NCH] = w * vz;
-            val[4 % NCH] = w * vx * vx;
-            val[5 % NCH] = w * vy * vy;
-            val[6 % NCH] = w * vz * vz;
-        }
+    // Each thread takes 4 consecutive particles with 16-byte loads (the
+    // arrays are 256-byte aligned and sized to a multiple of 8 entries, so the
+    // aligned quad around [p0, p1) stays inside the allocation).  Sorted
+    // particles of one cell are folded in registers; LDS sees one atomic per
+    // corner per run of equal cells instead of one per particle.
+    float acc[8][NCH];
+    int cur = -1;
+    auto flush = [&]() {
+        if (cur < 0) return;
 #pragma unroll
         for (int q = 0; q < 8; q++) {
-            int plane = q >> 1, di = q & 1;
-            int slot = plane * ni + i + di;
+            int slot = (q >> 1) * ni + cur + (q & 1);
 #pragma unroll
-            for (int ch = 0; ch < NCH; ch++) atomicAdd(&tile[ch * line + slot], val[ch] * wt[q]);
+            for (int ch = 0; ch < NCH; ch++) atomicAdd(&tile[ch * line + slot], acc[q][ch]);
+        }
+    };
+    for (uint32_t base = (p0 & ~3u) + 4u * threadIdx.x; base < p1; base += 4u * BLOCK) {
+        uint4 c4 = *reinterpret_cast<const uint4*>(b.cell + base);
+        float4 fx4 = *reinterpret_cast<const float4*>(b.f[0] + base);
+        float4 fy4 = *reinterpret_cast<const float4*>(b.f[1] + base);
+        float4 fz4 = *reinterpret_cast<const float4*>(b.f[2] + base);
+        float4 w4 = *reinterpret_cast<const float4*>(b.w + base);
+        float4 vx4, vy4, vz4;
+        if (NCH > 1) {
+            vx4 = *reinterpret_cast<const float4*>(b.v[0] + base);
+            vy4 = *reinterpret_cast<const float4*>(b.v[1] + base);
+            vz4 = *reinterpret_cast<const float4*>(b.v[2] + base);
+        }
+        const uint32_t cs[4] = {c4.x, c4.y, c4.z, c4.w};
+        const float fxs[4] = {fx4.x, fx4.y, fx4.z, fx4.w}, fys[4] = {fy4.x, fy4.y, fy4.z, fy4.w};
+        const float fzs[4] = {fz4.x, fz4.y, fz4.z, fz4.w}, ws[4] = {w4.x, w4.y, w4.z, w4.w};
+#pragma unroll
+        for (int m = 0; m < 4; m++) {
+            uint32_t p = base + m;
+            if (p < p0 || p >= p1) continue;
+            int i = (int)(cs[m] & 0x7FFu);
+            if (i != cur) {
+                flush();
+                cur = i;
+#pragma unroll
+                for (int q = 0; q < 8; q++)
+#pragma unroll
+                    for (int ch = 0; ch < NCH; ch++) acc[q][ch] = 0.f;
+            }
+            float fx = fxs[m], fy = fys[m], fz = fzs[m], w = ws[m];
+            float gx = 1.f - fx, gy = 1.f - fy, gz = 1.f - fz;
+            float wt[8];  // index (dk*2 + dj)*2 + di
+            wt[0] = gx * gy * gz;
+            wt[1] = fx * gy * gz;
+            wt[2] = gx * fy * gz;
+            wt[3] = fx * fy * gz;
+            wt[4] = gx * gy * fz;
+            wt[5] = fx * gy * fz;
+            wt[6] = gx * fy * fz;
+            wt[7] = fx * fy * fz;
+            float val[NCH];
+            val[0] = w;
+            if (NCH > 1) {
+                const float vxs[4] = {vx4.x, vx4.y, vx4.z, vx4.w};
+                const float vys[4] = {vy4.x, vy4.y, vy4.z, vy4.w};
+                const float vzs[4] = {vz4.x, vz4.y, vz4.z, vz4.w};
+                float vx = vxs[m], vy = vys[m], vz = vzs[m];
+                val[1 % NCH] = w * vx;
+                val[2 % NCH] = w * vy;
+                val[3 % NCH] = w * vz;
+                val[4 % NCH] = w * vx * vx;
+                val[5 % NCH] = w * vy * vy;
+                val[6 % NCH] = w * vz * vz;
+            }
+#pragma unroll
+            for (int q = 0; q < 8; q++)
+#pragma unroll
+                for (int ch = 0; ch < NCH; ch++) acc[q][ch] += val[ch] * wt[q];
         }
     }
+    flush();
     __syncthreads();
     const long long nn = g.nn;
     for (int t = threadIdx.x; t < line; t += BLOCK) {

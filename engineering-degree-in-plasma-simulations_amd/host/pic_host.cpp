@@ -1,0 +1,318 @@
+// pic_host.cpp -- host mirrors of World / Species / PotentialSolver /
+// MC_MEX_Ionization over the C ABI.  Every compute call goes to the gfx950
+// library; the host only keeps mirrors for the reference's public fields.
+#include "pic_host.hpp"
+
+#include <fstream>
+#include <iostream>
+#include <limits>
+#include <map>
+
+// ---------------------------------------------------------------- World
+World::World(int ni_, int nj_, int nk_, type_calc3 x0_, type_calc3 xm_)
+    : nn{ni_, nj_, nk_},
+      ni(ni_),
+      nj(nj_),
+      nk(nk_),
+      ni_1(ni_ - 1),
+      nj_1(nj_ - 1),
+      nk_1(nk_ - 1),
+      nv(ni_ * nj_ * nk_),
+      num_cells((ni_ - 1) * (nj_ - 1) * (nk_ - 1)),
+      phi(ni_, nj_, nk_),
+      rho(ni_, nj_, nk_),
+      node_vol(ni_, nj_, nk_),
+      ef(ni_, nj_, nk_),
+      x0(x0_),
+      xm(xm_) {
+    pic_grid_desc g{ni_, nj_, nk_, {x0_[0], x0_[1], x0_[2]}, {xm_[0], xm_[1], xm_[2]}};
+    int rc = pic_create(&g, &ctx);
+    if (rc != PIC_OK) throw std::invalid_argument("World: no gfx950 device or invalid grid");
+    for (int d = 0; d < 3; d++) dx[d] = (xm[d] - x0[d]) / (nn[d] - 1);  // World.cpp:72-76
+    phi.puller = [this](std::vector<double>& h) { check(pic_field_download(ctx, PIC_FIELD_PHI, h.data()), "phi"); };
+    phi.pusher = [this](const std::vector<double>& h) { check(pic_field_upload(ctx, PIC_FIELD_PHI, h.data()), "phi"); };
+    rho.puller = [this](std::vector<double>& h) { check(pic_field_download(ctx, PIC_FIELD_RHO, h.data()), "rho"); };
+    rho.pusher = [this](const std::vector<double>& h) { check(pic_field_upload(ctx, PIC_FIELD_RHO, h.data()), "rho"); };
+    node_vol.puller = [this](std::vector<double>& h) {
+        check(pic_field_download(ctx, PIC_FIELD_NODE_VOL, h.data()), "node_vol");
+    };
+    ef.puller = [this](std::vector<type_calc3>& h) {
+        check(pic_field_download(ctx, PIC_FIELD_EF, &h[0][0]), "ef");
+    };
+    ef.pusher = [this](const std::vector<type_calc3>& h) {
+        check(pic_field_upload(ctx, PIC_FIELD_EF, &h[0][0]), "ef");
+    };
+    node_vol.device_newer = true;
+    device_fields_changed();
+}
+
+World::~World() { pic_destroy(ctx); }
+
+void World::check(int rc, const char* what) const {
+    if (rc < 0) throw std::runtime_error(std::string(what) + ": " + pic_last_error(ctx));
+}
+void World::device_fields_changed() {
+    phi.device_newer = rho.device_newer = ef.device_newer = true;
+}
+void World::flush_host_fields() {
+    phi.push();
+    rho.push();
+    ef.push();
+}
+void World::dirichletBoundries() { check(pic_dirichlet_walls(ctx), "dirichletBoundries"); }
+void World::add_rectangle(type_calc3 pos, type_calc ph, type_calc3 sides) {
+    flush_host_fields();
+    check(pic_add_rectangle(ctx, pos.v, sides.v, ph), "addObject");
+    device_fields_changed();
+}
+type_calc World::getPE() {
+    flush_host_fields();
+    double v = 0;
+    check(pic_field_energy(ctx, &v), "getPE");
+    return v;
+}
+void World::computeChargeDensity(std::vector<Species>& species) {
+    (void)species;  // the device holds every species' density
+    flush_host_fields();
+    check(pic_charge_density(ctx), "computeChargeDensity");
+    rho.device_newer = true;
+}
+
+// -------------------------------------------------------------- Species
+Species::Species(std::string name_, type_calc mass_, type_calc charge_, World& w, type_calc mpw0_,
+                 type_calc E_ion_, size_t capacity)
+    : name(std::move(name_)),
+      mass(mass_),
+      charge(charge_),
+      mpw0(mpw0_),
+      E_ion(E_ion_),
+      den(w.ni, w.nj, w.nk),
+      T(w.ni, w.nj, w.nk),
+      vel(w.ni, w.nj, w.nk),
+      world(w) {
+    w.check(pic_species_add(w.ctx, mass, charge, mpw0, capacity, &sid), "Species");
+    // captures the World and the species id, not this: Species objects may be
+    // moved when a std::vector<Species> grows, as in the reference's main
+    World* wp = &w;
+    int id = sid;
+    den.puller = [wp, id](std::vector<double>& h) {
+        wp->check(pic_species_field_download(wp->ctx, id, PIC_SFIELD_DEN, h.data()), "den");
+    };
+}
+
+void Species::sync_to_device() {
+    if (host_dirty) {
+        world.check(pic_particles_upload(world.ctx, sid, reinterpret_cast<const double*>(host.data()),
+                                         host.size()),
+                    "particles");
+        host_dirty = false;
+    }
+}
+std::vector<Particle>& Species::getPartRef() {
+    if (!host_valid) {
+        size_t n = 0;
+        world.check(pic_particles_download(world.ctx, sid, nullptr, 0, &n), "getPartRef");
+        host.assign(n, Particle({}, {}, 0));
+        world.check(pic_particles_download(world.ctx, sid, reinterpret_cast<double*>(host.data()), n, &n),
+                    "getPartRef");
+        host_valid = true;
+    }
+    host_dirty = true;  // a mutable reference may be written
+    return host;
+}
+const std::vector<Particle>& Species::getConstPartRef() {
+    std::vector<Particle>& h = getPartRef();
+    host_dirty = false;
+    return h;
+}
+size_t Species::getNumParticles() const {
+    if (host_valid && host_dirty) return host.size();
+    size_t n = 0;
+    world.check(pic_particles_count(world.ctx, sid, &n), "getNumParticles");
+    return n;
+}
+void Species::advanceElectrons(type_calc dt) {
+    sync_to_device();
+    world.flush_host_fields();
+    world.check(pic_push(world.ctx, sid, dt), "advanceElectrons");
+    world.check(pic_sort(world.ctx, sid), "advanceElectrons");
+    host_valid = false;
+}
+void Species::advanceNonElectron(Species& neutrals, Species& spherium, type_calc dt) {
+    (void)neutrals;
+    (void)spherium;  // surface emission is not on this path: object hits absorb
+    advanceElectrons(dt);
+}
+void Species::computeNumberDensity() {
+    sync_to_device();
+    world.check(pic_deposit(world.ctx, sid, PIC_DEPOSIT_DENSITY), "computeNumberDensity");
+    den.device_newer = true;
+}
+void Species::sampleMoments() {
+    sync_to_device();
+    world.check(pic_deposit(world.ctx, sid, PIC_DEPOSIT_MOMENTS), "sampleMoments");
+}
+void Species::clearSamples() { world.check(pic_clear_samples(world.ctx, sid), "clearSamples"); }
+void Species::computeGasProperties() {
+    size_t n = (size_t)world.nv;
+    std::vector<double> ns(n), nv(3 * n), uu(n), vv(n), ww(n);
+    world.check(pic_species_field_download(world.ctx, sid, PIC_SFIELD_NSUM, ns.data()), "moments");
+    world.check(pic_species_field_download(world.ctx, sid, PIC_SFIELD_NVSUM, nv.data()), "moments");
+    world.check(pic_species_field_download(world.ctx, sid, PIC_SFIELD_NUU, uu.data()), "moments");
+    world.check(pic_species_field_download(world.ctx, sid, PIC_SFIELD_NVV, vv.data()), "moments");
+    world.check(pic_species_field_download(world.ctx, sid, PIC_SFIELD_NWW, ww.data()), "moments");
+    std::vector<type_calc3>& V = vel.raw();
+    std::vector<double>& TT = T.raw();
+    for (size_t u = 0; u < n; u++) {  // Species.cpp:777-804
+        double c = ns[u];
+        type_calc3 m(nv[3 * u], nv[3 * u + 1], nv[3 * u + 2]);
+        V[u] = c != 0 ? m / c : type_calc3();
+        if (c <= 0) {
+            TT[u] = 0;
+            continue;
+        }
+        double a = uu[u] / c - V[u][0] * V[u][0], b = vv[u] / c - V[u][1] * V[u][1],
+               d = ww[u] / c - V[u][2] * V[u][2];
+        TT[u] = mass / (2 * Const::k) * (a + b + d);
+    }
+}
+void Species::computeMacroParticlesCount() {
+    sync_to_device();
+    macro_part_count.assign((size_t)world.num_cells, 0.0);
+    world.check(pic_species_field_download(world.ctx, sid, PIC_SFIELD_MACRO_COUNT,
+                                           macro_part_count.data()),
+                "computeMacroParticlesCount");
+}
+void Species::loadParticleBoxThermal(type_calc3 x0, type_calc3 sides, type_calc num_den,
+                                     type_calc temp) {
+    sync_to_device();
+    world.flush_host_fields();
+    size_t n = 0;
+    int rc = pic_load_box_thermal(world.ctx, sid, x0.v, sides.v, num_den, temp, load_seed++, 0, &n);
+    if (rc == PIC_ERR_ARG)
+        throw std::invalid_argument("number of macroparticles less than 1, change initial values\n");
+    world.check(rc, "loadParticleBoxThermal");
+    world.check(pic_sort(world.ctx, sid), "loadParticleBoxThermal");
+    host_valid = false;
+}
+void Species::sortIndexes() {
+    sync_to_device();
+    world.check(pic_sort(world.ctx, sid), "sortIndexes");
+    host_valid = false;
+}
+void Species::diag(double out[6]) {
+    sync_to_device();
+    world.flush_host_fields();
+    world.check(pic_diag_sums(world.ctx, sid, out), "diagnostics");
+}
+type_calc Species::getMicroCount() {
+    double d[6];
+    diag(d);
+    return d[0];
+}
+type_calc3 Species::getMomentum() {
+    double d[6];
+    diag(d);
+    return {d[1], d[2], d[3]};
+}
+type_calc Species::getKE() {
+    double d[6];
+    diag(d);
+    return d[4];
+}
+type_calc Species::getPE() {
+    double d[6];
+    diag(d);
+    return d[5];
+}
+
+// ------------------------------------------------------ PotentialSolver
+PotentialSolver::PotentialSolver(World& w, unsigned max_solver_it, type_calc tol, SolverType type)
+    : world(w), solver_type(type), PCG_max_solver_it(max_solver_it), tolerance(tol) {
+    GS_max_solver_it = type == GS ? max_solver_it : 20 * max_solver_it;
+}
+void PotentialSolver::setReferenceValues(type_calc p0, type_calc n0_, type_calc te0) {
+    phi0 = p0;
+    n0 = n0_;
+    Te0 = te0;
+}
+bool PotentialSolver::solve() {
+    if (solver_type == QN) {  // PotentialSolver.cpp:349-352
+        std::cerr << "quasi neutral not implemented yet\n";
+        return false;
+    }
+    world.flush_host_fields();
+    pic_solver_params p{};
+    p.method = solver_type == GS ? PIC_SOLVER_SOR_RB : pcg_method;
+    p.tol_kind = tol_kind;
+    p.tol = tolerance;
+    p.max_it = (int)(solver_type == GS ? GS_max_solver_it : PCG_max_solver_it);
+    p.omega = omega;
+    p.check_every = solver_type == GS ? 25 : 8;
+    p.phi0 = phi0;
+    p.n0 = n0;
+    p.Te0 = Te0;
+    int rc = pic_poisson(world.ctx, &p, &last);
+    if (rc < 0) world.check(rc, "solve");
+    world.device_fields_changed();
+    if (rc == PIC_NOT_CONVERGED && solver_type == PCG) {  // fall back to GS (:222-224)
+        p.method = PIC_SOLVER_SOR_RB;
+        p.max_it = (int)GS_max_solver_it;
+        p.check_every = 25;
+        rc = pic_poisson(world.ctx, &p, &last);
+        if (rc < 0) world.check(rc, "solve");
+    }
+    if (rc != PIC_OK)
+        std::cerr << "solver failed to converge, residual = " << last.ref_l2
+                  << " tolerance = " << tolerance << " time step = " << world.getTs() << std::endl;
+    return rc == PIC_OK;
+}
+void PotentialSolver::computeEF() {
+    world.flush_host_fields();
+    world.check(pic_efield(world.ctx), "computeEF");
+    world.ef.device_newer = true;
+}
+
+// ---------------------------------------------------- MC_MEX_Ionization
+MC_MEX_Ionization::MC_MEX_Ionization(Species& n, Species& i, Species& e, World& w,
+                                     std::string coll_data_path, int freq_should_use_map)
+    : neutrals(n), ions(i), electrons(e), world(w) {
+    (void)freq_should_use_map;
+    // Interactions.cpp:479-489 guards
+    if (n.mpw0 < 1e2 * e.mpw0)
+        throw std::invalid_argument("neutrals species should have mpw0 at least 100 times greater than electrons");
+    if (!n.E_ion || n.E_ion < 0)
+        throw std::invalid_argument("neutral species must have proper ionization energy E_ion (in Joules)");
+    if (n.mpw0 < i.mpw0) throw std::invalid_argument("neutrals mpw0 must be >= ions mpw0");
+    // Interactions.cpp:518-533: (energy eV, sigma m^2) pairs, map-ordered
+    std::ifstream in(coll_data_path);
+    if (!in.is_open())
+        throw std::invalid_argument("Couldn't open file " + coll_data_path +
+                                    " for crosssection data of collisions");
+    std::map<double, double> table;
+    double E, s;
+    while (in >> E >> s) table[E] = s;
+    for (auto& kv : table) {
+        tE.push_back(kv.first);
+        tV.push_back(kv.second);
+    }
+    if (tE.empty()) throw std::invalid_argument("empty cross-section table " + coll_data_path);
+    desc.E_ion_J = n.E_ion;
+    desc.c0 = 1.015e-18;  // Interactions.cpp:506-508
+    desc.c1 = 9.793;
+    desc.c2 = 61.81;
+    desc.B_inc = 10.0;  // Interactions.cpp:511
+}
+void MC_MEX_Ionization::apply(type_calc dt) noexcept {
+    if (id < 0) {
+        desc.wsv_max_init = wsv_max_init;
+        last_status = pic_mcc_create(world.ctx, neutrals.sid, ions.sid, electrons.sid, &desc,
+                                     tE.data(), tV.data(), (int)tE.size(), &id);
+        if (last_status != PIC_OK) {
+            std::cerr << "MC_MEX_Ionization: " << pic_last_error(world.ctx) << "\n";
+            return;
+        }
+    }
+    last_status = pic_mcc_apply(world.ctx, id, dt, seed, step++, &last);
+    if (last_status != PIC_OK) std::cerr << "MC_MEX_Ionization: " << pic_last_error(world.ctx) << "\n";
+}
