@@ -676,7 +676,308 @@ static int solve_pcg(pic_ctx* ctx, const pic_solver_params* p, pic_solver_stats*
     return conv ? PIC_OK : PIC_NOT_CONVERGED;
 }
 
-int pic_solve_learned(pic_ctx* ctx, const pic_solver_params* p, pic_solver_stats* st);
+// ------------------------------------------------------------------------
+// flexible PCG with the learned preconditioner (Notay 2000: beta from the
+// Polak-Ribiere form, so a nonlinear preconditioner keeps CG's descent).
+// The network also supplies the initial guess on a cold start; plain PCG is
+// the fallback and the final residual is always checked in FP64.
+// ------------------------------------------------------------------------
+bool pic_unet_ready(pic_ctx* ctx);
+int pic_unet_apply(pic_ctx* ctx, const double* r, double* z);
+
+namespace {
+
+__global__ void __launch_bounds__(BLOCK)
+    k_fcg_resid(Stencil s, const uint8_t* __restrict__ nt, const double* __restrict__ phi,
+                const double* __restrict__ rho, double ieps, double* __restrict__ r,
+                double* __restrict__ out) {
+    double rr = 0, bb = 0, cold = 0;
+    long long end;
+    for (long long u = chunk_begin(s.nn, end) + threadIdx.x; u < end; u += BLOCK) {
+        uint8_t t = nt[u];
+        double rv = 0;
+        if (t == NT_REG) {
+            double b = -rho[u] * ieps;
+            double Ap = apply_A<double>(t, u, s, [&](long long q) { return phi[q]; });
+            rv = b - Ap;
+            bb += b * b;
+            cold += phi[u] != 0.0;
+        }
+        r[u] = rv;
+        rr += rv * rv;
+    }
+    double a = block_sum(rr), b2 = block_sum(bb), c = block_sum(cold);
+    if (threadIdx.x == 0) {
+        atomicAdd(&out[0], a);
+        atomicAdd(&out[1], b2);
+        atomicAdd(&out[2], c);
+    }
+}
+
+__global__ void __launch_bounds__(BLOCK)
+    k_fcg_q(Stencil s, const uint8_t* __restrict__ nt, const double* __restrict__ p,
+            double* __restrict__ q, double* __restrict__ out) {
+    double pq = 0;
+    long long end;
+    for (long long u = chunk_begin(s.nn, end) + threadIdx.x; u < end; u += BLOCK) {
+        uint8_t t = nt[u];
+        double qv = t == NT_REG ? apply_A<double>(t, u, s, [&](long long w) { return p[w]; }) : 0.0;
+        q[u] = qv;
+        pq += p[u] * qv;
+    }
+    double a = block_sum(pq);
+    if (threadIdx.x == 0) atomicAdd(out, a);
+}
+
+__global__ void __launch_bounds__(BLOCK)
+    k_fcg_update(long long n, const double* __restrict__ sc, double* __restrict__ x,
+                 const double* __restrict__ p, double* __restrict__ r, const double* __restrict__ q,
+                 double* __restrict__ rold, double* __restrict__ out) {
+    double alpha = sc[1] != 0.0 ? sc[0] / sc[1] : 0.0;  // (r.z) / (p.q)
+    double rr = 0;
+    long long end;
+    for (long long u = chunk_begin(n, end) + threadIdx.x; u < end; u += BLOCK) {
+        double rv = r[u];
+        rold[u] = rv;
+        x[u] += alpha * p[u];
+        rv -= alpha * q[u];
+        r[u] = rv;
+        rr += rv * rv;
+    }
+    double a = block_sum(rr);
+    if (threadIdx.x == 0) atomicAdd(out, a);
+}
+
+__global__ void __launch_bounds__(BLOCK)
+    k_fcg_dots(long long n, const double* __restrict__ z, const double* __restrict__ r,
+               const double* __restrict__ rold, double* __restrict__ out) {
+    double a = 0, b = 0;
+    long long end;
+    for (long long u = chunk_begin(n, end) + threadIdx.x; u < end; u += BLOCK) {
+        double zv = z[u], rv = r[u];
+        a += zv * (rv - rold[u]);
+        b += zv * rv;
+    }
+    double ta = block_sum(a), tb = block_sum(b);
+    if (threadIdx.x == 0) {
+        atomicAdd(&out[0], ta);
+        atomicAdd(&out[1], tb);
+    }
+}
+
+// p = z + beta p with beta = z.(r - r_old) / (r_old . z_old)
+__global__ void __launch_bounds__(BLOCK)
+    k_fcg_p(long long n, double beta, const double* __restrict__ z, double* __restrict__ p) {
+    long long end;
+    for (long long u = chunk_begin(n, end) + threadIdx.x; u < end; u += BLOCK) p[u] = z[u] + beta * p[u];
+}
+
+__global__ void __launch_bounds__(BLOCK)
+    k_dot(long long n, const double* __restrict__ a, const double* __restrict__ b,
+          double* __restrict__ out) {
+    double s = 0;
+    long long end;
+    for (long long u = chunk_begin(n, end) + threadIdx.x; u < end; u += BLOCK) s += a[u] * b[u];
+    double t = block_sum(s);
+    if (threadIdx.x == 0) atomicAdd(out, t);
+}
+
+// damped Jacobi sweep on A z = r (regular rows): first sweep from z = 0
+__global__ void __launch_bounds__(BLOCK)
+    k_jac(Stencil s, const uint8_t* __restrict__ nt, const double* __restrict__ r,
+          const double* __restrict__ zin, double* __restrict__ zout, double w, int first) {
+    long long end;
+    for (long long u = chunk_begin(s.nn, end) + threadIdx.x; u < end; u += BLOCK) {
+        uint8_t t = nt[u];
+        double v = 0.0;
+        if (t == NT_REG) {
+            if (first) {
+                v = w * r[u] / (-s.twos);
+            } else {
+                double Az = apply_A<double>(t, u, s, [&](long long q) { return zin[q]; });
+                v = zin[u] + w * (r[u] - Az) / (-s.twos);
+            }
+        }
+        zout[u] = v;
+    }
+}
+
+// t = r - A z on regular rows
+__global__ void __launch_bounds__(BLOCK)
+    k_res(Stencil s, const uint8_t* __restrict__ nt, const double* __restrict__ r,
+          const double* __restrict__ z, double* __restrict__ t) {
+    long long end;
+    for (long long u = chunk_begin(s.nn, end) + threadIdx.x; u < end; u += BLOCK) {
+        uint8_t ty = nt[u];
+        t[u] = ty == NT_REG ? r[u] - apply_A<double>(ty, u, s, [&](long long q) { return z[q]; }) : 0.0;
+    }
+}
+
+__global__ void __launch_bounds__(BLOCK)
+    k_add(long long n, double* __restrict__ z, const double* __restrict__ zn) {
+    long long end;
+    for (long long u = chunk_begin(n, end) + threadIdx.x; u < end; u += BLOCK) z[u] += zn[u];
+}
+
+__global__ void __launch_bounds__(BLOCK)
+    k_axpy_phi(long long n, const uint8_t* __restrict__ nt, double* __restrict__ x,
+               const double* __restrict__ z) {
+    long long end;
+    for (long long u = chunk_begin(n, end) + threadIdx.x; u < end; u += BLOCK)
+        if (nt[u] == NT_REG) x[u] += z[u];
+}
+
+}  // namespace
+
+static int solve_learned(pic_ctx* ctx, const pic_solver_params* p, pic_solver_stats* st) {
+    if (!pic_unet_ready(ctx))
+        return pic_fail(ctx, PIC_ERR_STATE, "learned solver: no trained network for this grid");
+    if (has_neumann(ctx))
+        return pic_fail(ctx, PIC_ERR_ARG, "learned solver: Dirichlet-walled problems only");
+    if (p->n0 != 0.0) return pic_fail(ctx, PIC_ERR_ARG, "learned solver: linear problems only");
+    const GridC& g = ctx->g;
+    size_t nn = (size_t)g.nn;
+    Stencil s = make_stencil(g, false);
+    int rc = pic_ensure_work(ctx, 8 * nn * sizeof(double));
+    if (rc) return rc;
+    double* r = ctx->work;
+    double* z = r + nn;
+    double* pv = z + nn;
+    double* q = pv + nn;
+    double* rold = q + nn;
+    double* t1 = rold + nn;
+    double* t2 = t1 + nn;
+    double* zn = t2 + nn;
+    double* sc = ctx->scratch;  // [0] r.z [1] p.q [2..] reductions
+    unsigned nb = stencil_blocks(s.nn);
+    // two-level preconditioner: nu damped-Jacobi sweeps, the network on the
+    // remaining residual (it carries the smooth error Jacobi cannot reach),
+    // nu damped-Jacobi sweeps
+    const int nu = 2;
+    const double wj = 0.8;
+    auto jacobi = [&](const double* rin, double* zio, bool from_zero) {
+        double* a = zio;
+        double* b = t2;
+        for (int k = 0; k < nu; k++) {
+            bool first = from_zero && k == 0;
+            k_jac<<<nb, BLOCK, 0, ctx->stream>>>(s, ctx->ntype, rin, a, b, wj, first);
+            std::swap(a, b);
+        }
+        if (a != zio)
+            (void)hipMemcpyAsync(zio, a, nn * 8, hipMemcpyDeviceToDevice, ctx->stream);
+    };
+    auto precond = [&](const double* rin, double* zout) -> int {
+        jacobi(rin, zout, true);
+        k_res<<<nb, BLOCK, 0, ctx->stream>>>(s, ctx->ntype, rin, zout, t1);
+        int e = pic_unet_apply(ctx, t1, zn);
+        if (e) return e;
+        k_add<<<nb, BLOCK, 0, ctx->stream>>>(s.nn, zout, zn);
+        jacobi(rin, zout, false);
+        return PIC_OK;
+    };
+    auto read = [&](double* h, int cnt) -> int {
+        PIC_HIP(ctx, hipMemcpyAsync(h, sc + 8, cnt * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+        PIC_HIP(ctx, hipStreamSynchronize(ctx->stream));
+        return PIC_OK;
+    };
+    auto resid = [&](double h[3]) -> int {
+        PIC_HIP(ctx, hipMemsetAsync(sc + 8, 0, 3 * sizeof(double), ctx->stream));
+        k_fcg_resid<<<nb, BLOCK, 0, ctx->stream>>>(s, ctx->ntype, ctx->phi, ctx->rho, 1.0 / EPS0, r,
+                                                   sc + 8);
+        return read(h, 3);
+    };
+    double h[3];
+    rc = resid(h);
+    if (rc) return rc;
+    double bnorm2 = h[1] > 0 ? h[1] : 1.0;
+    double target2 = (p->tol_kind == PIC_TOL_REF_RMS) ? p->tol * p->tol * (double)nn
+                                                      : p->tol * p->tol * bnorm2;
+    int it = 0;
+    bool conv = h[0] <= target2;
+    if (!conv && h[2] == 0.0) {  // cold start: the network's prediction is the initial guess
+        PicProfScope ps(ctx, PIC_K_PCG);
+        rc = precond(r, z);
+        if (rc) return rc;
+        k_axpy_phi<<<nb, BLOCK, 0, ctx->stream>>>(s.nn, ctx->ntype, ctx->phi, z);
+        rc = resid(h);
+        if (rc) return rc;
+        conv = h[0] <= target2;
+    }
+    int max_fcg = std::min(p->max_it, 400);
+    if (!conv) {
+        rc = precond(r, z);
+        if (rc) return rc;
+        PIC_HIP(ctx, hipMemcpyAsync(pv, z, nn * 8, hipMemcpyDeviceToDevice, ctx->stream));
+        PIC_HIP(ctx, hipMemsetAsync(sc, 0, 16 * sizeof(double), ctx->stream));
+        k_dot<<<nb, BLOCK, 0, ctx->stream>>>(s.nn, r, z, sc);  // sc[0] = r.z
+        for (it = 0; it < max_fcg; it++) {
+            PicProfScope ps(ctx, PIC_K_PCG);
+            PIC_HIP(ctx, hipMemsetAsync(sc + 1, 0, 15 * sizeof(double), ctx->stream));
+            k_fcg_q<<<nb, BLOCK, 0, ctx->stream>>>(s, ctx->ntype, pv, q, sc + 1);
+            k_fcg_update<<<nb, BLOCK, 0, ctx->stream>>>(s.nn, sc, ctx->phi, pv, r, q, rold, sc + 8);
+            double rr;
+            rc = read(&rr, 1);
+            if (rc) return rc;
+            if (!(rr == rr)) return pic_fail(ctx, PIC_ERR_NUMERIC, "learned PCG produced NaN");
+            if (rr <= target2) {
+                conv = true;
+                it++;
+                break;
+            }
+            rc = precond(r, z);
+            if (rc) return rc;
+            PIC_HIP(ctx, hipMemsetAsync(sc + 8, 0, 2 * sizeof(double), ctx->stream));
+            k_fcg_dots<<<nb, BLOCK, 0, ctx->stream>>>(s.nn, z, r, rold, sc + 8);
+            double d2[2], rz_old;
+            PIC_HIP(ctx, hipMemcpyAsync(&rz_old, sc, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+            rc = read(d2, 2);
+            if (rc) return rc;
+            double beta = rz_old != 0.0 ? d2[0] / rz_old : 0.0;
+            k_fcg_p<<<nb, BLOCK, 0, ctx->stream>>>(s.nn, beta, z, pv);
+            PIC_HIP(ctx, hipMemcpyAsync(sc, &d2[1], sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+        }
+    }
+    if (!conv) {  // plain PCG is the fallback
+        pic_solver_params q2 = *p;
+        q2.method = PIC_SOLVER_PCG;
+        pic_solver_stats s2;
+        rc = solve_pcg(ctx, &q2, &s2, false);
+        *st = s2;
+        st->iterations += it;
+        return rc;
+    }
+    SorArgs a;
+    a.omega = 1;
+    a.ieps = 1.0 / EPS0;
+    a.itwos = 1.0 / s.twos;
+    a.phi0 = 0;
+    a.n0 = 0;
+    a.inv_te0 = 1;
+    double rn[3];
+    rc = residual_norms(ctx, s, a, rn);
+    if (rc) return rc;
+    st->converged = 1;
+    st->iterations = it;
+    st->newton_iterations = 0;
+    st->ref_l2 = std::sqrt(rn[0] / (double)nn);
+    st->rel_residual = rn[2] > 0 ? std::sqrt(rn[1] / rn[2]) : std::sqrt(rn[1]);
+    return PIC_OK;
+}
+
+// plain red-black SOR to a relative tolerance (used to make training pairs)
+int pic_poisson_internal_sor(pic_ctx* ctx, double tol, int max_it, pic_solver_stats* st) {
+    pic_solver_params p{};
+    p.method = PIC_SOLVER_SOR_RB;
+    p.tol_kind = PIC_TOL_REL;
+    p.tol = tol;
+    p.max_it = max_it;
+    int nmax = std::max(ctx->g.ni, std::max(ctx->g.nj, ctx->g.nk));
+    p.omega = 2.0 / (1.0 + std::sin(3.141592653589793 / (nmax - 1)));
+    p.check_every = 25;
+    int rc = pic_ensure_ntype(ctx);
+    if (rc) return rc;
+    return solve_sor(ctx, &p, st);
+}
 
 extern "C" int pic_poisson(pic_ctx* ctx, const pic_solver_params* p, pic_solver_stats* st) {
     if (!ctx || !p) return PIC_ERR_ARG;
@@ -700,7 +1001,7 @@ extern "C" int pic_poisson(pic_ctx* ctx, const pic_solver_params* p, pic_solver_
             }
             rc = solve_pcg(ctx, p, st, true);
             break;
-        case PIC_SOLVER_LEARNED: rc = pic_solve_learned(ctx, p, st); break;
+        case PIC_SOLVER_LEARNED: rc = solve_learned(ctx, p, st); break;
         default: rc = pic_fail(ctx, PIC_ERR_ARG, "unknown solver method");
     }
     (void)hipEventRecord(e1, ctx->stream);

@@ -38,8 +38,25 @@ ABI_FUNCTIONS = [
     "pic_efield", "pic_field_upload", "pic_field_download",
     "pic_species_field_download", "pic_diag_sums", "pic_field_energy",
     "pic_mcc_create", "pic_mcc_apply", "pic_profile_enable", "pic_profile_read",
-    "pic_profile_reset",
+    "pic_profile_reset", "pic_unet_create", "pic_unet_param_count", "pic_unet_get_params",
+    "pic_unet_set_params", "pic_unet_forward", "pic_unet_loss_grad", "pic_unet_adam_step",
+    "pic_unet_train",
 ]
+
+
+class UNetDesc(C.Structure):
+    _fields_ = [("c1", C.c_int), ("c2", C.c_int), ("c3", C.c_int), ("seed", C.c_uint32)]
+
+
+class UNetTrainDesc(C.Structure):
+    _fields_ = [("samples", C.c_int), ("epochs", C.c_int), ("batch", C.c_int), ("lr", C.c_double),
+                ("seed", C.c_uint32), ("blobs_min", C.c_int), ("blobs_max", C.c_int),
+                ("sigma_min", C.c_double), ("sigma_max", C.c_double), ("noise", C.c_double)]
+
+
+class UNetTrainStats(C.Structure):
+    _fields_ = [("loss_first", C.c_double), ("loss_last", C.c_double),
+                ("seconds_data", C.c_double), ("seconds_train", C.c_double), ("steps", C.c_int)]
 
 
 class GridDesc(C.Structure):
@@ -122,6 +139,15 @@ def load_library():
     L.pic_profile_enable.argtypes = [P, C.c_int]
     L.pic_profile_read.argtypes = [P, _dp, np.ctypeslib.ndpointer(dtype=np.int64)]
     L.pic_profile_reset.argtypes = [P]
+    _fp = np.ctypeslib.ndpointer(dtype=np.float32, flags="C_CONTIGUOUS")
+    L.pic_unet_create.argtypes = [P, C.POINTER(UNetDesc)]
+    L.pic_unet_param_count.argtypes = [P, C.POINTER(C.c_size_t)]
+    L.pic_unet_get_params.argtypes = [P, _fp, C.c_size_t]
+    L.pic_unet_set_params.argtypes = [P, _fp, C.c_size_t]
+    L.pic_unet_forward.argtypes = [P, _fp, _fp]
+    L.pic_unet_loss_grad.argtypes = [P, _fp, _fp, C.POINTER(C.c_double), _fp]
+    L.pic_unet_adam_step.argtypes = [P, _fp, C.c_float]
+    L.pic_unet_train.argtypes = [P, C.POINTER(UNetTrainDesc), C.POINTER(UNetTrainStats)]
     _lib = L
     return L
 
@@ -299,6 +325,51 @@ class Context:
         self._ck(self.L.pic_mcc_apply(self.h, mid, dt, seed, step, C.byref(st)))
         return {"collisions": st.collisions, "ionizations": st.ionizations,
                 "groups": st.groups, "wsv_max": st.wsv_max}
+
+    # learned preconditioner
+    def unet_create(self, c1=16, c2=32, c3=64, seed=1):
+        d = UNetDesc(c1, c2, c3, seed)
+        self._ck(self.L.pic_unet_create(self.h, C.byref(d)))
+        n = C.c_size_t()
+        self._ck(self.L.pic_unet_param_count(self.h, C.byref(n)))
+        self.unet_nparam = n.value
+        return n.value
+
+    def unet_get_params(self):
+        p = np.zeros(self.unet_nparam, np.float32)
+        self._ck(self.L.pic_unet_get_params(self.h, p, p.size))
+        return p
+
+    def unet_set_params(self, p):
+        p = np.ascontiguousarray(p, np.float32)
+        self._ck(self.L.pic_unet_set_params(self.h, p, p.size))
+
+    def unet_forward(self, x2):
+        x2 = np.ascontiguousarray(x2, np.float32).reshape(-1)
+        out = np.zeros(self.nn, np.float32)
+        self._ck(self.L.pic_unet_forward(self.h, x2, out))
+        return out
+
+    def unet_loss_grad(self, x2, target):
+        x2 = np.ascontiguousarray(x2, np.float32).reshape(-1)
+        t = np.ascontiguousarray(target, np.float32).reshape(-1)
+        loss = C.c_double()
+        g = np.zeros(self.unet_nparam, np.float32)
+        self._ck(self.L.pic_unet_loss_grad(self.h, x2, t, C.byref(loss), g))
+        return loss.value, g
+
+    def unet_adam_step(self, grad, lr):
+        self._ck(self.L.pic_unet_adam_step(self.h, np.ascontiguousarray(grad, np.float32), lr))
+
+    def unet_train(self, samples, epochs, batch=4, lr=1e-3, seed=1, blobs=(1, 8),
+                   sigma=(4.0, 16.0), noise=0.0):
+        d = UNetTrainDesc(samples, epochs, batch, lr, seed, blobs[0], blobs[1], sigma[0],
+                          sigma[1], noise)
+        st = UNetTrainStats()
+        self._ck(self.L.pic_unet_train(self.h, C.byref(d), C.byref(st)))
+        return {"loss_first": st.loss_first, "loss_last": st.loss_last,
+                "seconds_data": st.seconds_data, "seconds_train": st.seconds_train,
+                "steps": st.steps}
 
     # profiling
     def profile(self, on=True):

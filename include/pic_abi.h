@@ -186,6 +186,44 @@ int pic_mcc_create(pic_ctx* ctx, int sid_neutral, int sid_ion, int sid_electron,
 int pic_mcc_apply(pic_ctx* ctx, int mcc_id, double dt, uint32_t seed, uint32_t step,
                   pic_mcc_stats* stats);
 
+/* ---- learned warm start / preconditioner (no counterpart in the reference) ----
+ * A 3-level 3-D conv U-Net on the context's node grid (node counts divisible
+ * by 4), BF16 weights on MFMA, FP32 accumulation.  Input channels per node:
+ * {rho in grid units, Dirichlet mask}; output: the potential of the
+ * unit-spacing problem laplacian(phi) = -rho with phi = 0 on Dirichlet nodes.
+ * Host arrays are channels-last, node order i-fastest. */
+typedef struct {
+    int c1, c2, c3;   /* channels per level: multiples of 16, c1 <= 32, c2, c3 <= 64 */
+    uint32_t seed;    /* weight initialisation */
+} pic_unet_desc;
+typedef struct {
+    int samples;          /* training pairs generated by the plain solver */
+    int epochs;
+    int batch;            /* samples per Adam step */
+    double lr;
+    uint32_t seed;
+    int blobs_min, blobs_max;     /* Gaussian blobs per density */
+    double sigma_min, sigma_max;  /* blob widths in cells */
+    double noise;                 /* white-noise RMS added to each density (particle noise) */
+} pic_unet_train_desc;
+typedef struct {
+    double loss_first, loss_last;  /* mean loss of the first and last epoch */
+    double seconds_data, seconds_train;
+    int steps;
+} pic_unet_train_stats;
+int pic_unet_create(pic_ctx* ctx, const pic_unet_desc* desc);
+int pic_unet_param_count(pic_ctx* ctx, size_t* n);
+int pic_unet_get_params(pic_ctx* ctx, float* dst, size_t n);
+int pic_unet_set_params(pic_ctx* ctx, const float* src, size_t n);
+/* in2: 2 floats per node; out: 1 float per node */
+int pic_unet_forward(pic_ctx* ctx, const float* in2, float* out);
+/* loss = 0.5 mean over non-Dirichlet nodes of (y - target)^2 and its gradient */
+int pic_unet_loss_grad(pic_ctx* ctx, const float* in2, const float* target, double* loss,
+                       float* grad);
+int pic_unet_adam_step(pic_ctx* ctx, const float* grad, float lr);
+/* generate pairs with the plain solver on the device and train (Adam) */
+int pic_unet_train(pic_ctx* ctx, const pic_unet_train_desc* desc, pic_unet_train_stats* stats);
+
 /* ---- profiling: HIP events around every launch of the context ---- */
 #define PIC_K_PUSH 0
 #define PIC_K_SORT 1
