@@ -223,7 +223,7 @@ def main():
     if dom == "pcg":
         units = ctx.nn
     elif dom == "sor":
-        units = ctx.nn / 2.0
+        units = ctx.nn  # one profiled launch = one full red-black sweep
     else:
         units = N
     alg = ALG_BYTES[dom] * units

@@ -677,16 +677,33 @@ static int solve_pcg(pic_ctx* ctx, const pic_solver_params* p, pic_solver_stats*
 }
 
 // ------------------------------------------------------------------------
-// flexible PCG with the learned preconditioner (Notay 2000: beta from the
-// Polak-Ribiere form, so a nonlinear preconditioner keeps CG's descent).
-// The network also supplies the initial guess on a cold start; plain PCG is
-// the fallback and the final residual is always checked in FP64.
+// Flexible CG with the learned two-level preconditioner.  FCG(1) (Notay
+// 2000): p_k = z_k - (z_k.A p_{k-1}) / (p_{k-1}.A p_{k-1}) p_{k-1}, which keeps
+// CG's descent when the preconditioner varies from step to step.  On a cold
+// start (phi = 0) the first step's correction is the preconditioned residual
+// -- the network's prediction -- scaled by the optimal step length, so the
+// network supplies the initial guess.  Scalars stay on the device in
+// per-iteration history slots; the host reads the residual history every
+// few iterations.  Plain PCG is the fallback and the final residual is
+// recomputed in FP64.
+//
+// Preconditioner M(r): one red-black Gauss-Seidel sweep on A z = r from
+// z = 0 (red, black), the network's coarse correction of the remaining
+// residual, one sweep in the reverse colour order (black, red).
 // ------------------------------------------------------------------------
 bool pic_unet_ready(pic_ctx* ctx);
-int pic_unet_apply(pic_ctx* ctx, const double* r, double* z);
+int pic_unet_coarse_correct(pic_ctx* ctx, const double* t, double* z, double* c);
 
 namespace {
 
+struct FcgHist {
+    double* rz;  // r_k . z_k
+    double* pq;  // p_k . A p_k
+    double* zq;  // z_{k+1} . A p_k
+    double* rr;  // r_{k+1} . r_{k+1}
+};
+
+// r = b - A phi on regular rows (0 elsewhere); out += {r.r, b.b, #(phi != 0)}
 __global__ void __launch_bounds__(BLOCK)
     k_fcg_resid(Stencil s, const uint8_t* __restrict__ nt, const double* __restrict__ phi,
                 const double* __restrict__ rho, double ieps, double* __restrict__ r,
@@ -714,62 +731,67 @@ __global__ void __launch_bounds__(BLOCK)
     }
 }
 
+// p_k = z_k + beta p_{k-1} (beta = -zq[k-1] / pq[k-1]; p_0 = z_0), q = A p_k on
+// regular rows; pq[k] += p_k.q.  z and p vanish on non-regular nodes.
 __global__ void __launch_bounds__(BLOCK)
-    k_fcg_q(Stencil s, const uint8_t* __restrict__ nt, const double* __restrict__ p,
-            double* __restrict__ q, double* __restrict__ out) {
-    double pq = 0;
+    k_fcg_pq(Stencil s, const uint8_t* __restrict__ nt, const double* __restrict__ z,
+             const double* __restrict__ pold, double* __restrict__ pnew, double* __restrict__ q,
+             FcgHist h, int k) {
+    double beta = 0.0;
+    if (k > 0) {
+        double den = h.pq[k - 1];
+        beta = den != 0.0 ? -h.zq[k - 1] / den : 0.0;
+    }
+    const bool first = k == 0;
+    double acc = 0;
     long long end;
     for (long long u = chunk_begin(s.nn, end) + threadIdx.x; u < end; u += BLOCK) {
+        auto pn = [&](long long w) { return first ? z[w] : z[w] + beta * pold[w]; };
         uint8_t t = nt[u];
-        double qv = t == NT_REG ? apply_A<double>(t, u, s, [&](long long w) { return p[w]; }) : 0.0;
+        double pu = pn(u);
+        double qv = t == NT_REG ? apply_A<double>(t, u, s, pn) : 0.0;
+        pnew[u] = pu;
         q[u] = qv;
-        pq += p[u] * qv;
+        acc += pu * qv;
     }
-    double a = block_sum(pq);
-    if (threadIdx.x == 0) atomicAdd(out, a);
+    double a = block_sum(acc);
+    if (threadIdx.x == 0) atomicAdd(&h.pq[k], a);
 }
 
+// alpha = rz[k] / pq[k]; x += alpha p; r -= alpha q; rr[k] += r.r
 __global__ void __launch_bounds__(BLOCK)
-    k_fcg_update(long long n, const double* __restrict__ sc, double* __restrict__ x,
-                 const double* __restrict__ p, double* __restrict__ r, const double* __restrict__ q,
-                 double* __restrict__ rold, double* __restrict__ out) {
-    double alpha = sc[1] != 0.0 ? sc[0] / sc[1] : 0.0;  // (r.z) / (p.q)
+    k_fcg_update(long long n, FcgHist h, int k, double* __restrict__ x, const double* __restrict__ p,
+                 double* __restrict__ r, const double* __restrict__ q) {
+    double den = h.pq[k];
+    double alpha = den != 0.0 ? h.rz[k] / den : 0.0;
     double rr = 0;
     long long end;
     for (long long u = chunk_begin(n, end) + threadIdx.x; u < end; u += BLOCK) {
-        double rv = r[u];
-        rold[u] = rv;
         x[u] += alpha * p[u];
-        rv -= alpha * q[u];
+        double rv = r[u] - alpha * q[u];
         r[u] = rv;
         rr += rv * rv;
     }
     double a = block_sum(rr);
-    if (threadIdx.x == 0) atomicAdd(out, a);
+    if (threadIdx.x == 0) atomicAdd(&h.rr[k], a);
 }
 
+// zq[k] += z.q, rz[k+1] += z.r
 __global__ void __launch_bounds__(BLOCK)
-    k_fcg_dots(long long n, const double* __restrict__ z, const double* __restrict__ r,
-               const double* __restrict__ rold, double* __restrict__ out) {
+    k_fcg_dots(long long n, FcgHist h, int k, const double* __restrict__ z,
+               const double* __restrict__ r, const double* __restrict__ q) {
     double a = 0, b = 0;
     long long end;
     for (long long u = chunk_begin(n, end) + threadIdx.x; u < end; u += BLOCK) {
-        double zv = z[u], rv = r[u];
-        a += zv * (rv - rold[u]);
-        b += zv * rv;
+        double zv = z[u];
+        a += zv * q[u];
+        b += zv * r[u];
     }
     double ta = block_sum(a), tb = block_sum(b);
     if (threadIdx.x == 0) {
-        atomicAdd(&out[0], ta);
-        atomicAdd(&out[1], tb);
+        atomicAdd(&h.zq[k], ta);
+        atomicAdd(&h.rz[k + 1], tb);
     }
-}
-
-// p = z + beta p with beta = z.(r - r_old) / (r_old . z_old)
-__global__ void __launch_bounds__(BLOCK)
-    k_fcg_p(long long n, double beta, const double* __restrict__ z, double* __restrict__ p) {
-    long long end;
-    for (long long u = chunk_begin(n, end) + threadIdx.x; u < end; u += BLOCK) p[u] = z[u] + beta * p[u];
 }
 
 __global__ void __launch_bounds__(BLOCK)
@@ -782,159 +804,159 @@ __global__ void __launch_bounds__(BLOCK)
     if (threadIdx.x == 0) atomicAdd(out, t);
 }
 
-// damped Jacobi sweep on A z = r (regular rows): first sweep from z = 0
+// first red half-sweep on A z = r from z = 0: z = -r / (2 sum 1/dx^2) on red
+// regular nodes ((i + j + k) even), 0 everywhere else
 __global__ void __launch_bounds__(BLOCK)
-    k_jac(Stencil s, const uint8_t* __restrict__ nt, const double* __restrict__ r,
-          const double* __restrict__ zin, double* __restrict__ zout, double w, int first) {
+    k_gs_first(Stencil s, const uint8_t* __restrict__ nt, const double* __restrict__ r,
+               double* __restrict__ z) {
     long long end;
     for (long long u = chunk_begin(s.nn, end) + threadIdx.x; u < end; u += BLOCK) {
-        uint8_t t = nt[u];
-        double v = 0.0;
-        if (t == NT_REG) {
-            if (first) {
-                v = w * r[u] / (-s.twos);
-            } else {
-                double Az = apply_A<double>(t, u, s, [&](long long q) { return zin[q]; });
-                v = zin[u] + w * (r[u] - Az) / (-s.twos);
-            }
-        }
-        zout[u] = v;
+        int i = (int)(u % s.ni);
+        long long row = u / s.ni;
+        int j = (int)(row % s.nj), k = (int)(row / s.nj);
+        bool red = ((i + j + k) & 1) == 0;
+        z[u] = (red && nt[u] == NT_REG) ? -r[u] / s.twos : 0.0;
     }
 }
 
-// t = r - A z on regular rows
+// Gauss-Seidel half-sweep on A z = r over regular nodes of one colour
 __global__ void __launch_bounds__(BLOCK)
-    k_res(Stencil s, const uint8_t* __restrict__ nt, const double* __restrict__ r,
-          const double* __restrict__ z, double* __restrict__ t) {
+    k_gs(Stencil s, const uint8_t* __restrict__ nt, const double* __restrict__ r,
+         double* __restrict__ z, int color) {
+    long long half = (s.ni + 1) / 2;
+    long long items = half * s.nj * s.nk;
     long long end;
-    for (long long u = chunk_begin(s.nn, end) + threadIdx.x; u < end; u += BLOCK) {
-        uint8_t ty = nt[u];
-        t[u] = ty == NT_REG ? r[u] - apply_A<double>(ty, u, s, [&](long long q) { return z[q]; }) : 0.0;
+    for (long long m = chunk_begin(items, end) + threadIdx.x; m < end; m += BLOCK) {
+        long long row = m / half;
+        int hh = (int)(m - row * half);
+        int j = (int)(row % s.nj), k = (int)(row / s.nj);
+        int i = 2 * hh + ((j + k + color) & 1);
+        if (i >= s.ni) continue;
+        long long u = row * s.ni + i;
+        if (nt[u] != NT_REG) continue;
+        z[u] = ((z[u - 1] + z[u + 1]) * s.cx + (z[u - s.sj] + z[u + s.sj]) * s.cy +
+                (z[u - s.sk] + z[u + s.sk]) * s.cz - r[u]) /
+               s.twos;
     }
-}
-
-__global__ void __launch_bounds__(BLOCK)
-    k_add(long long n, double* __restrict__ z, const double* __restrict__ zn) {
-    long long end;
-    for (long long u = chunk_begin(n, end) + threadIdx.x; u < end; u += BLOCK) z[u] += zn[u];
-}
-
-__global__ void __launch_bounds__(BLOCK)
-    k_axpy_phi(long long n, const uint8_t* __restrict__ nt, double* __restrict__ x,
-               const double* __restrict__ z) {
-    long long end;
-    for (long long u = chunk_begin(n, end) + threadIdx.x; u < end; u += BLOCK)
-        if (nt[u] == NT_REG) x[u] += z[u];
 }
 
 }  // namespace
 
-static int solve_learned(pic_ctx* ctx, const pic_solver_params* p, pic_solver_stats* st) {
+static int learned_precond(pic_ctx* ctx, const Stencil& s, const double* rin, double* zout,
+                           double* tmp) {
+    unsigned nb = stencil_blocks(s.nn);
+    long long half_items = (long long)((s.ni + 1) / 2) * s.nj * s.nk;
+    unsigned nbh = stencil_blocks(half_items);
+    k_gs_first<<<nb, BLOCK, 0, ctx->stream>>>(s, ctx->ntype, rin, zout);
+    k_gs<<<nbh, BLOCK, 0, ctx->stream>>>(s, ctx->ntype, rin, zout, 1);
+    int e = pic_unet_coarse_correct(ctx, rin, zout, tmp);
+    if (e) return e;
+    k_gs<<<nbh, BLOCK, 0, ctx->stream>>>(s, ctx->ntype, rin, zout, 1);
+    k_gs<<<nbh, BLOCK, 0, ctx->stream>>>(s, ctx->ntype, rin, zout, 0);
+    PIC_HIP(ctx, hipGetLastError());
+    return PIC_OK;
+}
+
+static int learned_check(pic_ctx* ctx) {
     if (!pic_unet_ready(ctx))
         return pic_fail(ctx, PIC_ERR_STATE, "learned solver: no trained network for this grid");
     if (has_neumann(ctx))
         return pic_fail(ctx, PIC_ERR_ARG, "learned solver: Dirichlet-walled problems only");
+    return PIC_OK;
+}
+
+// z = M(r) for host vectors (node order), the preconditioner alone
+extern "C" int pic_learned_precondition(pic_ctx* ctx, const double* r, double* z) {
+    if (!ctx || !r || !z) return PIC_ERR_ARG;
+    int rc = pic_ensure_ntype(ctx);
+    if (rc) return rc;
+    rc = learned_check(ctx);
+    if (rc) return rc;
+    size_t nn = (size_t)ctx->g.nn;
+    rc = pic_ensure_work(ctx, 3 * nn * sizeof(double));
+    if (rc) return rc;
+    double* rd = ctx->work;
+    double* zd = rd + nn;
+    PIC_HIP(ctx, hipMemcpyAsync(rd, r, nn * 8, hipMemcpyHostToDevice, ctx->stream));
+    rc = learned_precond(ctx, make_stencil(ctx->g, false), rd, zd, zd + nn);
+    if (rc) return rc;
+    PIC_HIP(ctx, hipMemcpyAsync(z, zd, nn * 8, hipMemcpyDeviceToHost, ctx->stream));
+    PIC_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    return PIC_OK;
+}
+
+static int solve_learned(pic_ctx* ctx, const pic_solver_params* p, pic_solver_stats* st) {
+    int rc = learned_check(ctx);
+    if (rc) return rc;
     if (p->n0 != 0.0) return pic_fail(ctx, PIC_ERR_ARG, "learned solver: linear problems only");
     const GridC& g = ctx->g;
     size_t nn = (size_t)g.nn;
     Stencil s = make_stencil(g, false);
-    int rc = pic_ensure_work(ctx, 8 * nn * sizeof(double));
+    rc = pic_ensure_work(ctx, 6 * nn * sizeof(double));
     if (rc) return rc;
     double* r = ctx->work;
     double* z = r + nn;
-    double* pv = z + nn;
-    double* q = pv + nn;
-    double* rold = q + nn;
-    double* t1 = rold + nn;
-    double* t2 = t1 + nn;
-    double* zn = t2 + nn;
-    double* sc = ctx->scratch;  // [0] r.z [1] p.q [2..] reductions
+    double* p0 = z + nn;
+    double* p1 = p0 + nn;
+    double* q = p1 + nn;
+    double* ctmp = q + nn;
+    const int max_fcg = std::min(p->max_it, 500);
+    const int check = p->check_every > 0 ? p->check_every : 4;
+    // history slots: scratch[1024 ..) (the network's normalisation uses scratch[64..65])
+    FcgHist h;
+    h.rz = ctx->scratch + 1024;
+    h.pq = h.rz + (max_fcg + 2);
+    h.zq = h.pq + (max_fcg + 2);
+    h.rr = h.zq + (max_fcg + 2);
+    double* sc = ctx->scratch + 8;
     unsigned nb = stencil_blocks(s.nn);
-    // two-level preconditioner: nu damped-Jacobi sweeps, the network on the
-    // remaining residual (it carries the smooth error Jacobi cannot reach),
-    // nu damped-Jacobi sweeps
-    const int nu = 2;
-    const double wj = 0.8;
-    auto jacobi = [&](const double* rin, double* zio, bool from_zero) {
-        double* a = zio;
-        double* b = t2;
-        for (int k = 0; k < nu; k++) {
-            bool first = from_zero && k == 0;
-            k_jac<<<nb, BLOCK, 0, ctx->stream>>>(s, ctx->ntype, rin, a, b, wj, first);
-            std::swap(a, b);
-        }
-        if (a != zio)
-            (void)hipMemcpyAsync(zio, a, nn * 8, hipMemcpyDeviceToDevice, ctx->stream);
-    };
-    auto precond = [&](const double* rin, double* zout) -> int {
-        jacobi(rin, zout, true);
-        k_res<<<nb, BLOCK, 0, ctx->stream>>>(s, ctx->ntype, rin, zout, t1);
-        int e = pic_unet_apply(ctx, t1, zn);
-        if (e) return e;
-        k_add<<<nb, BLOCK, 0, ctx->stream>>>(s.nn, zout, zn);
-        jacobi(rin, zout, false);
-        return PIC_OK;
-    };
-    auto read = [&](double* h, int cnt) -> int {
-        PIC_HIP(ctx, hipMemcpyAsync(h, sc + 8, cnt * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
-        PIC_HIP(ctx, hipStreamSynchronize(ctx->stream));
-        return PIC_OK;
-    };
-    auto resid = [&](double h[3]) -> int {
-        PIC_HIP(ctx, hipMemsetAsync(sc + 8, 0, 3 * sizeof(double), ctx->stream));
-        k_fcg_resid<<<nb, BLOCK, 0, ctx->stream>>>(s, ctx->ntype, ctx->phi, ctx->rho, 1.0 / EPS0, r,
-                                                   sc + 8);
-        return read(h, 3);
-    };
-    double h[3];
-    rc = resid(h);
-    if (rc) return rc;
-    double bnorm2 = h[1] > 0 ? h[1] : 1.0;
+    auto precond = [&](const double* rin, double* zout) { return learned_precond(ctx, s, rin, zout, ctmp); };
+    double h0[3];
+    PIC_HIP(ctx, hipMemsetAsync(sc, 0, 3 * sizeof(double), ctx->stream));
+    k_fcg_resid<<<nb, BLOCK, 0, ctx->stream>>>(s, ctx->ntype, ctx->phi, ctx->rho, 1.0 / EPS0, r, sc);
+    PIC_HIP(ctx, hipMemcpyAsync(h0, sc, sizeof(h0), hipMemcpyDeviceToHost, ctx->stream));
+    PIC_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    double bnorm2 = h0[1] > 0 ? h0[1] : 1.0;
     double target2 = (p->tol_kind == PIC_TOL_REF_RMS) ? p->tol * p->tol * (double)nn
                                                       : p->tol * p->tol * bnorm2;
     int it = 0;
-    bool conv = h[0] <= target2;
-    if (!conv && h[2] == 0.0) {  // cold start: the network's prediction is the initial guess
-        PicProfScope ps(ctx, PIC_K_PCG);
-        rc = precond(r, z);
-        if (rc) return rc;
-        k_axpy_phi<<<nb, BLOCK, 0, ctx->stream>>>(s.nn, ctx->ntype, ctx->phi, z);
-        rc = resid(h);
-        if (rc) return rc;
-        conv = h[0] <= target2;
-    }
-    int max_fcg = std::min(p->max_it, 400);
+    bool conv = h0[0] <= target2;
     if (!conv) {
+        PIC_HIP(ctx, hipMemsetAsync(h.rz, 0, 4 * (size_t)(max_fcg + 2) * sizeof(double), ctx->stream));
         rc = precond(r, z);
         if (rc) return rc;
-        PIC_HIP(ctx, hipMemcpyAsync(pv, z, nn * 8, hipMemcpyDeviceToDevice, ctx->stream));
-        PIC_HIP(ctx, hipMemsetAsync(sc, 0, 16 * sizeof(double), ctx->stream));
-        k_dot<<<nb, BLOCK, 0, ctx->stream>>>(s.nn, r, z, sc);  // sc[0] = r.z
-        for (it = 0; it < max_fcg; it++) {
-            PicProfScope ps(ctx, PIC_K_PCG);
-            PIC_HIP(ctx, hipMemsetAsync(sc + 1, 0, 15 * sizeof(double), ctx->stream));
-            k_fcg_q<<<nb, BLOCK, 0, ctx->stream>>>(s, ctx->ntype, pv, q, sc + 1);
-            k_fcg_update<<<nb, BLOCK, 0, ctx->stream>>>(s.nn, sc, ctx->phi, pv, r, q, rold, sc + 8);
-            double rr;
-            rc = read(&rr, 1);
-            if (rc) return rc;
-            if (!(rr == rr)) return pic_fail(ctx, PIC_ERR_NUMERIC, "learned PCG produced NaN");
-            if (rr <= target2) {
-                conv = true;
-                it++;
-                break;
+        k_dot<<<nb, BLOCK, 0, ctx->stream>>>(s.nn, r, z, h.rz);
+        std::vector<double> rr(max_fcg + 2);
+        double* pold = p0;
+        double* pnew = p1;
+        int last = 0;
+        for (int k = 0; k < max_fcg; k++) {
+            {
+                PicProfScope ps(ctx, PIC_K_PCG);
+                k_fcg_pq<<<nb, BLOCK, 0, ctx->stream>>>(s, ctx->ntype, z, pold, pnew, q, h, k);
+                k_fcg_update<<<nb, BLOCK, 0, ctx->stream>>>(s.nn, h, k, ctx->phi, pnew, r, q);
             }
+            PIC_HIP(ctx, hipGetLastError());
+            it = k + 1;
+            if (it % check == 0 || it == max_fcg) {
+                PIC_HIP(ctx, hipMemcpyAsync(rr.data() + last, h.rr + last, (size_t)(it - last) * sizeof(double),
+                                            hipMemcpyDeviceToHost, ctx->stream));
+                PIC_HIP(ctx, hipStreamSynchronize(ctx->stream));
+                for (int j = last; j < it; j++) {
+                    if (!(rr[j] == rr[j])) return pic_fail(ctx, PIC_ERR_NUMERIC, "learned PCG produced NaN");
+                    if (rr[j] <= target2) conv = true;
+                }
+                last = it;
+                if (conv) break;
+            }
+            if (it == max_fcg) break;
             rc = precond(r, z);
             if (rc) return rc;
-            PIC_HIP(ctx, hipMemsetAsync(sc + 8, 0, 2 * sizeof(double), ctx->stream));
-            k_fcg_dots<<<nb, BLOCK, 0, ctx->stream>>>(s.nn, z, r, rold, sc + 8);
-            double d2[2], rz_old;
-            PIC_HIP(ctx, hipMemcpyAsync(&rz_old, sc, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
-            rc = read(d2, 2);
-            if (rc) return rc;
-            double beta = rz_old != 0.0 ? d2[0] / rz_old : 0.0;
-            k_fcg_p<<<nb, BLOCK, 0, ctx->stream>>>(s.nn, beta, z, pv);
-            PIC_HIP(ctx, hipMemcpyAsync(sc, &d2[1], sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+            {
+                PicProfScope ps(ctx, PIC_K_PCG);
+                k_fcg_dots<<<nb, BLOCK, 0, ctx->stream>>>(s.nn, h, k, z, r, q);
+            }
+            std::swap(pold, pnew);
         }
     }
     if (!conv) {  // plain PCG is the fallback

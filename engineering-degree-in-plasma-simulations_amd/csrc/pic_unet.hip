@@ -1,25 +1,34 @@
 // pic_unet.hip -- learned warm start and preconditioner for the Poisson solve
 // (no counterpart in the reference).
 //
-// Network: a 3-level 3-D convolutional U-Net on the node grid.  Activations
-// are channels-last BF16 ([voxel][C]); weights are BF16 for the MFMA layers
-// with FP32 master copies; every accumulation is FP32.
+// Network: a D-level (2..5) 3-D convolutional U-Net.  Activations are
+// channels-last BF16 ([voxel][C]); weights are BF16 for the MFMA layers with
+// FP32 master copies; every accumulation is FP32.  With C_l channels on level
+// l (grid halved per level):
 //
-//   L1 (n^3)   a1a = relu(conv3(x, 2->C1))            direct kernel (FP32 input)
-//              s1  = relu(conv3(a1a, C1->C1))         MFMA
-//   L2 (n/2)   a2a = relu(conv3(pool(s1), C1->C2))    MFMA
-//              s2  = relu(conv3(a2a, C2->C2))         MFMA
-//   L3 (n/4)   a3a = relu(conv3(pool(s2), C2->C3))    MFMA
-//              a3b = relu(conv3(a3a, C3->C3))         MFMA
-//   L2         a2  = up(conv1(a3b, C3->C2)) + s2      MFMA (1x1) + nearest upsample
-//              c2c = relu(conv3(a2, C2->C2))          MFMA
-//   L1         a1  = up(conv1(c2c, C2->C1)) + s1
-//              c1c = relu(conv3(a1, C1->C1))          MFMA
-//              y   = conv3(c1c, C1->1)                direct kernel (FP32 output)
+//   level 0     a0 = relu(conv3(x, 2->C0))            direct kernel (FP32 input)
+//               s0 = relu(conv3(a0, C0->C0))          MFMA
+//   level l>0   p_l = pool(s_{l-1})
+//               a_l = relu(conv3(p_l, C_{l-1}->C_l))  MFMA
+//               s_l = relu(conv3(a_l, C_l->C_l))      MFMA
+//   decoder     u_l = conv1(h_{l+1} or s_{D-1}, C_{l+1}->C_l)   MFMA (1x1)
+//   l = D-2..0  m_l = up(u_l) + s_l                   nearest upsample + skip
+//               h_l = relu(conv3(m_l, C_l->C_l))      MFMA
+//   output      y = conv3(h_0, C0->1)                 direct kernel (FP32 output)
 //
-// Input channels: the right-hand side in grid units (normalised) and the
-// Dirichlet mask.  Target: the potential of the unit-spacing Poisson problem
-// (laplacian(phi) = -rho, phi = 0 on Dirichlet nodes).
+// Parameter order: in, B0, A1, B1, ..., A_{D-1}, B_{D-1}, U_{D-2}, Dec_{D-2},
+// ..., U_0, Dec_0, out; each layer [cout][taps * cin] (k = tap * cin + ci)
+// then [cout] biases.
+//
+// The network runs on the node grid coarsened by 1 or 2 (2x2x2 aggregates).
+// Input channels: a charge density in grid units (normalised to unit RMS)
+// and the fraction of non-regular (Dirichlet) nodes in the aggregate.
+// Target: the mean over the aggregate's regular nodes of the potential of
+// the unit-spacing problem laplacian(phi) = -rho, phi = 0 on Dirichlet nodes,
+// divided by the same normalisation.  As a preconditioner the network is the
+// coarse solve of a two-level cycle: the fine residual after smoothing is
+// restricted to the aggregates, the network's correction is interpolated
+// back trilinearly (cell-centred weights 3/4, 1/4).
 //
 // MFMA convolution (implicit GEMM): a workgroup owns 16 x 4 x 4 output
 // voxels (16 M-blocks of 16 voxels along x); the input tile plus a one-voxel
@@ -58,22 +67,33 @@ __device__ __host__ inline bf16 f2bf(float f) {  // round to nearest even
     u += 0x7FFFu + ((u >> 16) & 1u);
     return (bf16)(u >> 16);
 }
+__device__ inline uint32_t pack2(float a, float b) {
+    return (uint32_t)f2bf(a) | ((uint32_t)f2bf(b) << 16);
+}
+__device__ inline void unpack8(const uint4& q, float* f) {
+    const uint32_t w[4] = {q.x, q.y, q.z, q.w};
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        f[2 * k] = bf2f((bf16)(w[k] & 0xFFFFu));
+        f[2 * k + 1] = bf2f((bf16)(w[k] >> 16));
+    }
+}
 
 struct Dims {
     int nx, ny, nz;
     long long v;
 };
-Dims half_dims(const Dims& d) {
-    Dims h{d.nx / 2, d.ny / 2, d.nz / 2, 0};
+Dims scaled_dims(const Dims& d, int f) {
+    Dims h{d.nx / f, d.ny / f, d.nz / f, 0};
     h.v = (long long)h.nx * h.ny * h.nz;
     return h;
 }
 
 __device__ inline void vox_xyz(const Dims& d, long long v, int& x, int& y, int& z) {
-    x = (int)(v % d.nx);
-    long long r = v / d.nx;
-    y = (int)(r % d.ny);
-    z = (int)(r / d.ny);
+    int r = (int)(v / d.nx);
+    x = (int)(v - (long long)r * d.nx);
+    z = r / d.ny;
+    y = r - z * d.ny;
 }
 
 // ---------------------------------------------------------------- MFMA conv
@@ -177,33 +197,54 @@ __global__ void __launch_bounds__(256) k_conv_mfma(ConvArgs a) {
 }
 
 // ------------------------------------------------------- direct conv layers
-// input layer: FP32 [V][2] -> BF16 [V][C] with bias and ReLU
+// input layer: FP32 [V][2] -> BF16 [V][C] with bias and ReLU.  nrm (optional,
+// device {sum x^2, count}) scales channel 0 to unit RMS.
+template <int C>
 __global__ void __launch_bounds__(256)
-    k_conv_in(Dims d, const float* __restrict__ x, const float* __restrict__ w,
-              const float* __restrict__ bias, bf16* __restrict__ out, int C) {
-    __shared__ float ws[32 * 54];
-    for (int t = threadIdx.x; t < C * 54; t += 256) ws[t] = w[t];
+    k_conv_in(Dims d, const float* __restrict__ x, const double* __restrict__ nrm,
+              const float* __restrict__ w, const float* __restrict__ bias, bf16* __restrict__ out) {
+    __shared__ float ws[27][2][C];  // transposed: [tap][input channel][output channel]
+    __shared__ float bs[C];
+    for (int e = threadIdx.x; e < C * 54; e += 256) {
+        int c = e / 54, k = e - c * 54;
+        ws[k >> 1][k & 1][c] = w[e];
+    }
+    if (threadIdx.x < C) bs[threadIdx.x] = bias[threadIdx.x];
     __syncthreads();
+    float sc = 1.f;
+    if (nrm && nrm[0] > 0 && nrm[1] > 0) sc = (float)(1.0 / sqrt(nrm[0] / nrm[1]));
     long long v = (long long)blockIdx.x * 256 + threadIdx.x;
     if (v >= d.v) return;
     int xi, yi, zi;
     vox_xyz(d, v, xi, yi, zi);
-    float acc[32];
-    for (int c = 0; c < C; c++) acc[c] = bias[c];
+    float acc[C];
+#pragma unroll
+    for (int c = 0; c < C; c++) acc[c] = bs[c];
     for (int t = 0; t < 27; t++) {
         int gx = xi + t % 3 - 1, gy = yi + (t / 3) % 3 - 1, gz = zi + t / 9 - 1;
         if (gx < 0 || gx >= d.nx || gy < 0 || gy >= d.ny || gz < 0 || gz >= d.nz) continue;
         float2 in = *reinterpret_cast<const float2*>(x + 2 * (((long long)gz * d.ny + gy) * d.nx + gx));
-        for (int c = 0; c < C; c++) acc[c] += in.x * ws[c * 54 + 2 * t] + in.y * ws[c * 54 + 2 * t + 1];
+        in.x *= sc;
+#pragma unroll
+        for (int c = 0; c < C; c++) acc[c] += in.x * ws[t][0][c] + in.y * ws[t][1][c];
     }
-    for (int c = 0; c < C; c++) out[v * C + c] = f2bf(fmaxf(acc[c], 0.f));
+    uint4* o = reinterpret_cast<uint4*>(out + v * C);
+#pragma unroll
+    for (int q = 0; q < C / 8; q++) {
+        const float* a8 = acc + 8 * q;
+        o[q] = make_uint4(pack2(fmaxf(a8[0], 0.f), fmaxf(a8[1], 0.f)),
+                          pack2(fmaxf(a8[2], 0.f), fmaxf(a8[3], 0.f)),
+                          pack2(fmaxf(a8[4], 0.f), fmaxf(a8[5], 0.f)),
+                          pack2(fmaxf(a8[6], 0.f), fmaxf(a8[7], 0.f)));
+    }
 }
 
 // output layer: BF16 [V][C] -> FP32 [V], bias, no activation; w is [27][C]
+template <int C>
 __global__ void __launch_bounds__(256)
-    k_conv_out(Dims d, const bf16* __restrict__ x, int C, const float* __restrict__ w,
+    k_conv_out(Dims d, const bf16* __restrict__ x, const float* __restrict__ w,
                const float* __restrict__ bias, float* __restrict__ y) {
-    __shared__ float ws[27 * 32];
+    __shared__ float ws[27 * C];
     for (int t = threadIdx.x; t < 27 * C; t += 256) ws[t] = w[t];
     __syncthreads();
     long long v = (long long)blockIdx.x * 256 + threadIdx.x;
@@ -214,103 +255,146 @@ __global__ void __launch_bounds__(256)
     for (int t = 0; t < 27; t++) {
         int gx = xi + t % 3 - 1, gy = yi + (t / 3) % 3 - 1, gz = zi + t / 9 - 1;
         if (gx < 0 || gx >= d.nx || gy < 0 || gy >= d.ny || gz < 0 || gz >= d.nz) continue;
-        const bf16* row = x + (((long long)gz * d.ny + gy) * d.nx + gx) * C;
-        for (int c = 0; c < C; c++) acc += bf2f(row[c]) * ws[t * C + c];
+        const uint4* row = reinterpret_cast<const uint4*>(x + (((long long)gz * d.ny + gy) * d.nx + gx) * C);
+#pragma unroll
+        for (int q = 0; q < C / 8; q++) {
+            float f[8];
+            unpack8(row[q], f);
+#pragma unroll
+            for (int k = 0; k < 8; k++) acc += f[k] * ws[t * C + 8 * q + k];
+        }
     }
     y[v] = acc;
 }
 
 // d(input of the output layer) = sum_t dy[v - off t] w[t][c], times the ReLU
 // mask of that input
+template <int C>
 __global__ void __launch_bounds__(256)
-    k_conv_out_bwd(Dims d, const float* __restrict__ dy, int C, const float* __restrict__ w,
+    k_conv_out_bwd(Dims d, const float* __restrict__ dy, const float* __restrict__ w,
                    const bf16* __restrict__ act, bf16* __restrict__ dx) {
-    __shared__ float ws[27 * 32];
+    __shared__ float ws[27 * C];
     for (int t = threadIdx.x; t < 27 * C; t += 256) ws[t] = w[t];
     __syncthreads();
     long long v = (long long)blockIdx.x * 256 + threadIdx.x;
     if (v >= d.v) return;
     int xi, yi, zi;
     vox_xyz(d, v, xi, yi, zi);
-    float acc[32];
+    float acc[C];
+#pragma unroll
     for (int c = 0; c < C; c++) acc[c] = 0.f;
     for (int t = 0; t < 27; t++) {
         int gx = xi - (t % 3 - 1), gy = yi - ((t / 3) % 3 - 1), gz = zi - (t / 9 - 1);
         if (gx < 0 || gx >= d.nx || gy < 0 || gy >= d.ny || gz < 0 || gz >= d.nz) continue;
         float g = dy[((long long)gz * d.ny + gy) * d.nx + gx];
+#pragma unroll
         for (int c = 0; c < C; c++) acc[c] += g * ws[t * C + c];
     }
-    for (int c = 0; c < C; c++) {
-        bool on = bf2f(act[v * C + c]) > 0.f;
-        dx[v * C + c] = f2bf(on ? acc[c] : 0.f);
+    const uint4* ar = reinterpret_cast<const uint4*>(act + v * C);
+    uint4* o = reinterpret_cast<uint4*>(dx + v * C);
+#pragma unroll
+    for (int q = 0; q < C / 8; q++) {
+        float f[8];
+        unpack8(ar[q], f);
+        float* a8 = acc + 8 * q;
+#pragma unroll
+        for (int k = 0; k < 8; k++) a8[k] = f[k] > 0.f ? a8[k] : 0.f;
+        o[q] = make_uint4(pack2(a8[0], a8[1]), pack2(a8[2], a8[3]), pack2(a8[4], a8[5]),
+                          pack2(a8[6], a8[7]));
     }
 }
 
 // ------------------------------------------------------ pooling / upsampling
+// one thread per (voxel, group of 8 channels); C is a multiple of 16
+__device__ inline long long child(const Dims& dc, int xi, int yi, int zi, int q) {
+    int fx = 2 * dc.nx, fy = 2 * dc.ny;
+    return ((long long)(2 * zi + (q >> 2)) * fy + (2 * yi + ((q >> 1) & 1))) * fx + (2 * xi + (q & 1));
+}
+__device__ inline long long parent(const Dims& df, int xi, int yi, int zi) {
+    return ((long long)(zi >> 1) * (df.ny >> 1) + (yi >> 1)) * (df.nx >> 1) + (xi >> 1);
+}
+
 __global__ void __launch_bounds__(256)
     k_pool(Dims dc, const bf16* __restrict__ fine, bf16* __restrict__ coarse, int C) {
+    const int G = C / 8;
     long long e = (long long)blockIdx.x * 256 + threadIdx.x;
-    if (e >= dc.v * C) return;
-    long long v = e / C;
-    int c = (int)(e - v * C);
+    if (e >= dc.v * G) return;
+    long long v = e / G;
+    int c8 = (int)(e - v * G);
     int xi, yi, zi;
     vox_xyz(dc, v, xi, yi, zi);
-    int fx = 2 * dc.nx, fy = 2 * dc.ny;
-    float s = 0.f;
+    float s[8] = {0, 0, 0, 0, 0, 0, 0, 0};
     for (int q = 0; q < 8; q++) {
-        long long fv = ((long long)(2 * zi + (q >> 2)) * fy + (2 * yi + ((q >> 1) & 1))) * fx +
-                       (2 * xi + (q & 1));
-        s += bf2f(fine[fv * C + c]);
+        float f[8];
+        unpack8(reinterpret_cast<const uint4*>(fine + child(dc, xi, yi, zi, q) * C)[c8], f);
+#pragma unroll
+        for (int k = 0; k < 8; k++) s[k] += f[k];
     }
-    coarse[e] = f2bf(0.125f * s);
+    reinterpret_cast<uint4*>(coarse + v * C)[c8] =
+        make_uint4(pack2(0.125f * s[0], 0.125f * s[1]), pack2(0.125f * s[2], 0.125f * s[3]),
+                   pack2(0.125f * s[4], 0.125f * s[5]), pack2(0.125f * s[6], 0.125f * s[7]));
 }
 
 // d_fine = (d_fine + d_coarse[parent] / 8) * (act > 0)
 __global__ void __launch_bounds__(256)
     k_pool_bwd(Dims df, const bf16* __restrict__ dcoarse, bf16* __restrict__ dfine,
                const bf16* __restrict__ act, int C) {
+    const int G = C / 8;
     long long e = (long long)blockIdx.x * 256 + threadIdx.x;
-    if (e >= df.v * C) return;
-    long long v = e / C;
-    int c = (int)(e - v * C);
+    if (e >= df.v * G) return;
+    long long v = e / G;
+    int c8 = (int)(e - v * G);
     int xi, yi, zi;
     vox_xyz(df, v, xi, yi, zi);
-    long long pv = ((long long)(zi >> 1) * (df.ny >> 1) + (yi >> 1)) * (df.nx >> 1) + (xi >> 1);
-    float g = bf2f(dfine[e]) + 0.125f * bf2f(dcoarse[pv * C + c]);
-    dfine[e] = f2bf(bf2f(act[e]) > 0.f ? g : 0.f);
+    float a[8], b[8], m[8], r[8];
+    unpack8(reinterpret_cast<const uint4*>(dfine + v * C)[c8], a);
+    unpack8(reinterpret_cast<const uint4*>(dcoarse + parent(df, xi, yi, zi) * C)[c8], b);
+    unpack8(reinterpret_cast<const uint4*>(act + v * C)[c8], m);
+#pragma unroll
+    for (int k = 0; k < 8; k++) r[k] = m[k] > 0.f ? a[k] + 0.125f * b[k] : 0.f;
+    reinterpret_cast<uint4*>(dfine + v * C)[c8] =
+        make_uint4(pack2(r[0], r[1]), pack2(r[2], r[3]), pack2(r[4], r[5]), pack2(r[6], r[7]));
 }
 
 // fine = up(coarse) + skip
 __global__ void __launch_bounds__(256)
     k_up_add(Dims df, const bf16* __restrict__ coarse, const bf16* __restrict__ skip,
              bf16* __restrict__ fine, int C) {
+    const int G = C / 8;
     long long e = (long long)blockIdx.x * 256 + threadIdx.x;
-    if (e >= df.v * C) return;
-    long long v = e / C;
-    int c = (int)(e - v * C);
+    if (e >= df.v * G) return;
+    long long v = e / G;
+    int c8 = (int)(e - v * G);
     int xi, yi, zi;
     vox_xyz(df, v, xi, yi, zi);
-    long long pv = ((long long)(zi >> 1) * (df.ny >> 1) + (yi >> 1)) * (df.nx >> 1) + (xi >> 1);
-    fine[e] = f2bf(bf2f(coarse[pv * C + c]) + bf2f(skip[e]));
+    float a[8], b[8], r[8];
+    unpack8(reinterpret_cast<const uint4*>(coarse + parent(df, xi, yi, zi) * C)[c8], a);
+    unpack8(reinterpret_cast<const uint4*>(skip + v * C)[c8], b);
+#pragma unroll
+    for (int k = 0; k < 8; k++) r[k] = a[k] + b[k];
+    reinterpret_cast<uint4*>(fine + v * C)[c8] =
+        make_uint4(pack2(r[0], r[1]), pack2(r[2], r[3]), pack2(r[4], r[5]), pack2(r[6], r[7]));
 }
 
 // d_coarse = sum over the 8 children of d_fine
 __global__ void __launch_bounds__(256)
     k_up_bwd(Dims dc, const bf16* __restrict__ dfine, bf16* __restrict__ dcoarse, int C) {
+    const int G = C / 8;
     long long e = (long long)blockIdx.x * 256 + threadIdx.x;
-    if (e >= dc.v * C) return;
-    long long v = e / C;
-    int c = (int)(e - v * C);
+    if (e >= dc.v * G) return;
+    long long v = e / G;
+    int c8 = (int)(e - v * G);
     int xi, yi, zi;
     vox_xyz(dc, v, xi, yi, zi);
-    int fx = 2 * dc.nx, fy = 2 * dc.ny;
-    float s = 0.f;
+    float s[8] = {0, 0, 0, 0, 0, 0, 0, 0};
     for (int q = 0; q < 8; q++) {
-        long long fv = ((long long)(2 * zi + (q >> 2)) * fy + (2 * yi + ((q >> 1) & 1))) * fx +
-                       (2 * xi + (q & 1));
-        s += bf2f(dfine[fv * C + c]);
+        float f[8];
+        unpack8(reinterpret_cast<const uint4*>(dfine + child(dc, xi, yi, zi, q) * C)[c8], f);
+#pragma unroll
+        for (int k = 0; k < 8; k++) s[k] += f[k];
     }
-    dcoarse[e] = f2bf(s);
+    reinterpret_cast<uint4*>(dcoarse + v * C)[c8] =
+        make_uint4(pack2(s[0], s[1]), pack2(s[2], s[3]), pack2(s[4], s[5]), pack2(s[6], s[7]));
 }
 
 // ------------------------------------------------------- weight gradients
@@ -340,28 +424,29 @@ __global__ void __launch_bounds__(256)
 #pragma unroll
         for (int q = 0; q < 16; q++) acc[r][q] = 0.f;
     }
+    const int cin8 = cin / 8, cout8 = cout / 8;
     for (int tt = 0; tt < tiles_per_block; tt++) {
         int tile = blockIdx.x * tiles_per_block + tt;
         if (tile >= ntiles) break;
         int x0 = (tile % ntx) * WX, y0 = ((tile / ntx) % nty) * WY, z0 = (tile / (ntx * nty)) * WZ;
         __syncthreads();
-        for (int s = threadIdx.x; s < SX * SY * SZ * cin; s += 256) {
-            int vox = s / cin, c = s - vox * cin;
+        for (int s = threadIdx.x; s < SX * SY * SZ * cin8; s += 256) {
+            int vox = s / cin8, c8 = s - vox * cin8;
             int sx = vox % SX, sy = (vox / SX) % SY, sz = vox / (SX * SY);
             int gx = x0 + sx - H, gy = y0 + sy - H, gz = z0 + sz - H;
-            bf16 val = 0;
+            uint4 val = make_uint4(0, 0, 0, 0);
             if (gx >= 0 && gx < d.nx && gy >= 0 && gy < d.ny && gz >= 0 && gz < d.nz)
-                val = X[(((long long)gz * d.ny + gy) * d.nx + gx) * cin + c];
-            xs[s] = val;
+                val = reinterpret_cast<const uint4*>(X + (((long long)gz * d.ny + gy) * d.nx + gx) * cin)[c8];
+            reinterpret_cast<uint4*>(xs)[s] = val;
         }
-        for (int s = threadIdx.x; s < WX * WY * WZ * cout; s += 256) {
-            int vox = s / cout, c = s - vox * cout;
+        for (int s = threadIdx.x; s < WX * WY * WZ * cout8; s += 256) {
+            int vox = s / cout8, c8 = s - vox * cout8;
             int sx = vox % WX, sy = (vox / WX) % WY, sz = vox / (WX * WY);
             int gx = x0 + sx, gy = y0 + sy, gz = z0 + sz;
-            bf16 val = 0;
+            uint4 val = make_uint4(0, 0, 0, 0);
             if (gx < d.nx && gy < d.ny && gz < d.nz)
-                val = dY[(((long long)gz * d.ny + gy) * d.nx + gx) * cout + c];
-            ys[s] = val;
+                val = reinterpret_cast<const uint4*>(dY + (((long long)gz * d.ny + gy) * d.nx + gx) * cout)[c8];
+            reinterpret_cast<uint4*>(ys)[s] = val;
         }
         __syncthreads();
 #pragma unroll
@@ -377,12 +462,12 @@ __global__ void __launch_bounds__(256)
                 int vx = v % WX, vy = (v / WX) % WY, vz = v / (WX * WY);
                 const bf16* xr = xs + (((vz + oz) * SY + (vy + oy)) * SX + (vx + ox)) * cin + 4 * cib;
                 const bf16* yr = ys + v * cout + 4 * cob;
-                float xv[4], yv[4];
-#pragma unroll
-                for (int q = 0; q < 4; q++) {
-                    xv[q] = bf2f(xr[q]);
-                    yv[q] = bf2f(yr[q]);
-                }
+                uint2 xq = *reinterpret_cast<const uint2*>(xr);
+                uint2 yq = *reinterpret_cast<const uint2*>(yr);
+                float xv[4] = {bf2f(xq.x & 0xFFFFu), bf2f(xq.x >> 16), bf2f(xq.y & 0xFFFFu),
+                               bf2f(xq.y >> 16)};
+                float yv[4] = {bf2f(yq.x & 0xFFFFu), bf2f(yq.x >> 16), bf2f(yq.y & 0xFFFFu),
+                               bf2f(yq.y >> 16)};
 #pragma unroll
                 for (int p = 0; p < 4; p++)
 #pragma unroll
@@ -406,72 +491,111 @@ __global__ void __launch_bounds__(256)
     }
 }
 
-// direct layers' weight gradients: one thread per weight, a chunk of voxels
-// per workgroup.  in layer: dW[c][2t + i] = sum dY[v][c] x[v + off t][i];
-// out layer: dW[t][c] = sum dy[v] X[v + off t][c].
-constexpr int DCHUNK = 4096;
+template <typename T>
+__device__ inline float to_f(T v);
+template <>
+__device__ inline float to_f<float>(float v) {
+    return v;
+}
+template <>
+__device__ inline float to_f<bf16>(bf16 v) {
+    return bf2f(v);
+}
+
+// direct layers: dW[o][t*XC + i] += sum_v G[v][o] X[v + off t][i] and
+// db[o] += sum_v G[v][o] (in layer: G = dY (C ch), X = x (2 ch); out layer:
+// G = dy (1 ch), X = h0 (C ch)).  Same tiling as k_wgrad, one weight per
+// thread slot, RPT slots per thread.
+template <int GC, int XC, typename GT, typename XT>
 __global__ void __launch_bounds__(256)
-    k_wgrad_in(Dims d, const float* __restrict__ x, const bf16* __restrict__ dY, int C,
-               float* __restrict__ dW, float* __restrict__ db) {
-    long long v0 = (long long)blockIdx.x * DCHUNK;
-    long long v1 = min(v0 + DCHUNK, d.v);
-    for (int wi = threadIdx.x; wi < C * 55; wi += 256) {
-        int c = wi / 55, k = wi - c * 55;
-        float acc = 0.f;
-        if (k == 54) {
-            for (long long v = v0; v < v1; v++) acc += bf2f(dY[v * C + c]);
-            atomicAdd(&db[c], acc);
-            continue;
+    k_wgrad_direct(Dims d, const XT* __restrict__ X, const GT* __restrict__ G,
+                   float* __restrict__ dW, float* __restrict__ db, int tiles_per_block) {
+    constexpr int SX = WX + 2, SY = WY + 2, SZ = WZ + 2;
+    constexpr int PER_O = 27 * XC + 1;
+    constexpr int NW = GC * PER_O;
+    constexpr int RPT = (NW + 255) / 256;
+    __shared__ float xs[SX * SY * SZ * XC];
+    __shared__ float gs[WX * WY * WZ * GC];
+    const int ntx = (d.nx + WX - 1) / WX, nty = (d.ny + WY - 1) / WY, ntz = (d.nz + WZ - 1) / WZ;
+    const int ntiles = ntx * nty * ntz;
+    float acc[RPT];
+#pragma unroll
+    for (int r = 0; r < RPT; r++) acc[r] = 0.f;
+    for (int tt = 0; tt < tiles_per_block; tt++) {
+        int tile = blockIdx.x * tiles_per_block + tt;
+        if (tile >= ntiles) break;
+        int x0 = (tile % ntx) * WX, y0 = ((tile / ntx) % nty) * WY, z0 = (tile / (ntx * nty)) * WZ;
+        __syncthreads();
+        for (int s = threadIdx.x; s < SX * SY * SZ * XC; s += 256) {
+            int vox = s / XC, c = s - vox * XC;
+            int sx = vox % SX, sy = (vox / SX) % SY, sz = vox / (SX * SY);
+            int gx = x0 + sx - 1, gy = y0 + sy - 1, gz = z0 + sz - 1;
+            float val = 0.f;
+            if (gx >= 0 && gx < d.nx && gy >= 0 && gy < d.ny && gz >= 0 && gz < d.nz)
+                val = to_f<XT>(X[(((long long)gz * d.ny + gy) * d.nx + gx) * XC + c]);
+            xs[s] = val;
         }
-        int t = k >> 1, ch = k & 1;
-        int ox = t % 3 - 1, oy = (t / 3) % 3 - 1, oz = t / 9 - 1;
-        for (long long v = v0; v < v1; v++) {
-            int xi, yi, zi;
-            vox_xyz(d, v, xi, yi, zi);
-            int gx = xi + ox, gy = yi + oy, gz = zi + oz;
-            if (gx < 0 || gx >= d.nx || gy < 0 || gy >= d.ny || gz < 0 || gz >= d.nz) continue;
-            acc += bf2f(dY[v * C + c]) * x[2 * (((long long)gz * d.ny + gy) * d.nx + gx) + ch];
+        for (int s = threadIdx.x; s < WX * WY * WZ * GC; s += 256) {
+            int vox = s / GC, c = s - vox * GC;
+            int sx = vox % WX, sy = (vox / WX) % WY, sz = vox / (WX * WY);
+            int gx = x0 + sx, gy = y0 + sy, gz = z0 + sz;
+            float val = 0.f;
+            if (gx < d.nx && gy < d.ny && gz < d.nz)
+                val = to_f<GT>(G[(((long long)gz * d.ny + gy) * d.nx + gx) * GC + c]);
+            gs[s] = val;
         }
-        atomicAdd(&dW[c * 54 + k], acc);
+        __syncthreads();
+#pragma unroll
+        for (int r = 0; r < RPT; r++) {
+            int wi = threadIdx.x + 256 * r;
+            if (wi >= NW) continue;
+            int o = wi / PER_O, k = wi - o * PER_O;
+            float a = 0.f;
+            if (k == 27 * XC) {
+                for (int v = 0; v < WX * WY * WZ; v++) a += gs[v * GC + o];
+            } else {
+                int t = k / XC, i = k - t * XC;
+                int ox = t % 3, oy = (t / 3) % 3, oz = t / 9;
+                for (int v = 0; v < WX * WY * WZ; v++) {
+                    int vx = v % WX, vy = (v / WX) % WY, vz = v / (WX * WY);
+                    a += gs[v * GC + o] * xs[(((vz + oz) * SY + (vy + oy)) * SX + (vx + ox)) * XC + i];
+                }
+            }
+            acc[r] += a;
+        }
+    }
+#pragma unroll
+    for (int r = 0; r < RPT; r++) {
+        int wi = threadIdx.x + 256 * r;
+        if (wi >= NW) continue;
+        int o = wi / PER_O, k = wi - o * PER_O;
+        if (k == 27 * XC)
+            atomicAdd(&db[o], acc[r]);
+        else
+            atomicAdd(&dW[o * 27 * XC + k], acc[r]);
     }
 }
 
-__global__ void __launch_bounds__(256)
-    k_wgrad_out(Dims d, const bf16* __restrict__ X, int C, const float* __restrict__ dy,
-                float* __restrict__ dW, float* __restrict__ db) {
-    long long v0 = (long long)blockIdx.x * DCHUNK;
-    long long v1 = min(v0 + DCHUNK, d.v);
-    for (int wi = threadIdx.x; wi <= 27 * C; wi += 256) {
-        float acc = 0.f;
-        if (wi == 27 * C) {
-            for (long long v = v0; v < v1; v++) acc += dy[v];
-            atomicAdd(db, acc);
-            continue;
-        }
-        int t = wi / C, c = wi - t * C;
-        int ox = t % 3 - 1, oy = (t / 3) % 3 - 1, oz = t / 9 - 1;
-        for (long long v = v0; v < v1; v++) {
-            int xi, yi, zi;
-            vox_xyz(d, v, xi, yi, zi);
-            int gx = xi + ox, gy = yi + oy, gz = zi + oz;
-            if (gx < 0 || gx >= d.nx || gy < 0 || gy >= d.ny || gz < 0 || gz >= d.nz) continue;
-            acc += dy[v] * bf2f(X[(((long long)gz * d.ny + gy) * d.nx + gx) * C + c]);
-        }
-        atomicAdd(&dW[wi], acc);
-    }
-}
-
-// bias gradient of an MFMA layer: db[c] += sum_v dY[v][c]
+// bias gradient of an MFMA layer: db[c] += sum_v dY[v][c]; partial sums of a
+// workgroup are combined in LDS, one atomic per channel and workgroup
 __global__ void __launch_bounds__(256)
     k_bias_grad(long long V, const bf16* __restrict__ dY, int C, float* __restrict__ db) {
+    __shared__ float red[256];
     int c = threadIdx.x % C;
-    int lanes = 256 / C * C;
-    if (threadIdx.x >= lanes) return;
+    int per = 256 / C;  // voxel lanes
+    int lane = threadIdx.x / C;
     float acc = 0.f;
-    long long step = (long long)gridDim.x * (lanes / C);
-    for (long long v = (long long)blockIdx.x * (lanes / C) + threadIdx.x / C; v < V; v += step)
-        acc += bf2f(dY[v * C + c]);
-    atomicAdd(&db[c], acc);
+    if (lane < per) {
+        long long step = (long long)gridDim.x * per;
+        for (long long v = (long long)blockIdx.x * per + lane; v < V; v += step) acc += bf2f(dY[v * C + c]);
+    }
+    red[threadIdx.x] = acc;
+    __syncthreads();
+    if (threadIdx.x < C) {
+        float s = 0.f;
+        for (int l = 0; l < per; l++) s += red[l * C + threadIdx.x];
+        atomicAdd(&db[threadIdx.x], s);
+    }
 }
 
 // ---------------------------------------------------------- misc kernels
@@ -490,7 +614,8 @@ __global__ void k_transpose_w(const float* __restrict__ w, int cout, int cin, in
     wt[(long long)ci * taps * cout + (long long)(taps - 1 - t) * cout + co] = v;
 }
 
-// loss over non-Dirichlet voxels: dy = (y - t) / Nvalid, out[0] += 0.5 sum (y-t)^2 / Nvalid
+// loss over voxels holding a regular node (mask channel < 1):
+// dy = (y - t) / Nvalid, out[0] += 0.5 sum (y-t)^2 / Nvalid
 __global__ void __launch_bounds__(256)
     k_loss(long long V, const float* __restrict__ y, const float* __restrict__ t,
            const float* __restrict__ x2, float inv_n, float* __restrict__ dy,
@@ -498,7 +623,7 @@ __global__ void __launch_bounds__(256)
     long long v = (long long)blockIdx.x * 256 + threadIdx.x;
     double l = 0;
     if (v < V) {
-        float valid = x2[2 * v + 1] > 0.5f ? 0.f : 1.f;
+        float valid = x2[2 * v + 1] < 0.999f ? 1.f : 0.f;
         float r = (y[v] - t[v]) * valid;
         dy[v] = r * inv_n;
         l = 0.5 * (double)r * r * inv_n;
@@ -521,11 +646,217 @@ __global__ void __launch_bounds__(256)
     p[e] -= lr * (mm / c1) / (sqrtf(vv / c2) + eps);
 }
 
+template <int N>
+__device__ inline void block_sums(double (&v)[N]) {
+    __shared__ double red[N][4];
+#pragma unroll
+    for (int q = 0; q < N; q++) {
+        for (int o = 32; o > 0; o >>= 1) v[q] += __shfl_xor(v[q], o, 64);
+        if ((threadIdx.x & 63) == 0) red[q][threadIdx.x >> 6] = v[q];
+    }
+    __syncthreads();
+    if (threadIdx.x == 0)
+#pragma unroll
+        for (int q = 0; q < N; q++) v[q] = red[q][0] + red[q][1] + red[q][2] + red[q][3];
+}
+
+__device__ inline void block_sum2(double& a, double& b) {
+    __shared__ double red[2][4];
+    for (int o = 32; o > 0; o >>= 1) {
+        a += __shfl_xor(a, o, 64);
+        b += __shfl_xor(b, o, 64);
+    }
+    if ((threadIdx.x & 63) == 0) {
+        red[0][threadIdx.x >> 6] = a;
+        red[1][threadIdx.x >> 6] = b;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        a = red[0][0] + red[0][1] + red[0][2] + red[0][3];
+        b = red[1][0] + red[1][1] + red[1][2] + red[1][3];
+    }
+}
+
+// ------------------------------------------- fine grid <-> network grid
+// The node (i, j, k) belongs to aggregate (i/CF, j/CF, k/CF).
+
+// training pair: x2[v] = {mean over regular nodes of a, fraction of other
+// nodes}, tgt[v] = mean over regular nodes of phi; sums += {x^2, 1, tgt^2}
+// over aggregates holding a regular node
+template <int CF>
+__global__ void __launch_bounds__(256)
+    k_restrict_pair(Dims df, Dims dc, const uint8_t* __restrict__ nt, const float* __restrict__ a,
+                    const double* __restrict__ phi, float* __restrict__ x2, float* __restrict__ tgt,
+                    double* __restrict__ sums) {
+    long long v = (long long)blockIdx.x * 256 + threadIdx.x;
+    double acc[3] = {0, 0, 0};
+    if (v < dc.v) {
+        int cx, cy, cz;
+        vox_xyz(dc, v, cx, cy, cz);
+        double sa = 0, sp = 0;
+        int nreg = 0;
+#pragma unroll
+        for (int q = 0; q < CF * CF * CF; q++) {
+            int fx = CF * cx + q % CF, fy = CF * cy + (q / CF) % CF, fz = CF * cz + q / (CF * CF);
+            long long u = ((long long)fz * df.ny + fy) * df.nx + fx;
+            if (nt[u] == NT_REG) {
+                sa += a[u];
+                sp += phi[u];
+                nreg++;
+            }
+        }
+        float xv = nreg ? (float)(sa / nreg) : 0.f;
+        x2[2 * v] = xv;
+        x2[2 * v + 1] = 1.f - (float)nreg / (float)(CF * CF * CF);
+        float tv = nreg ? (float)(sp / nreg) : 0.f;
+        tgt[v] = tv;
+        if (nreg) {
+            acc[0] = (double)xv * xv;
+            acc[1] = 1;
+            acc[2] = (double)tv * tv;
+        }
+    }
+    block_sums<3>(acc);
+    if (threadIdx.x == 0)
+        for (int q = 0; q < 3; q++) atomicAdd(&sums[q], acc[q]);
+}
+
+// input and target to unit RMS over the valid aggregates (the preconditioner
+// fixes the correction's scale by a line search, so only its shape is learnt)
+__global__ void __launch_bounds__(256)
+    k_scale_pair(long long n, const double* __restrict__ sums, float* __restrict__ x2,
+                 float* __restrict__ tgt) {
+    long long v = (long long)blockIdx.x * 256 + threadIdx.x;
+    if (v >= n) return;
+    double sx = sums[0] > 0 && sums[1] > 0 ? sqrt(sums[0] / sums[1]) : 1.0;
+    double st = sums[2] > 0 && sums[1] > 0 ? sqrt(sums[2] / sums[1]) : 1.0;
+    x2[2 * v] *= (float)(1.0 / sx);
+    tgt[v] *= (float)(1.0 / st);
+}
+
+// preconditioner input from the fine residual of A e = t after smoothing z:
+// x2[v] = {-h^2 mean over regular nodes of (t - A z), fraction of other nodes};
+// sums += {x^2, 1}.  Regular nodes are interior (Dirichlet walls), so all six
+// neighbours exist; z is zero on non-regular nodes.
+template <int CF>
+__global__ void __launch_bounds__(256)
+    k_res_restrict(Dims df, Dims dc, double h2, const uint8_t* __restrict__ nt,
+                   const double* __restrict__ t, const double* __restrict__ z, float* __restrict__ x2,
+                   double* __restrict__ sums) {
+    long long v = (long long)blockIdx.x * 256 + threadIdx.x;
+    double s2 = 0, cnt = 0;
+    const long long sj = df.nx, sk = (long long)df.nx * df.ny;
+    const double ih2 = 1.0 / h2;
+    if (v < dc.v) {
+        int cx, cy, cz;
+        vox_xyz(dc, v, cx, cy, cz);
+        double acc = 0;
+        int nreg = 0;
+#pragma unroll
+        for (int q = 0; q < CF * CF * CF; q++) {
+            int fx = CF * cx + q % CF, fy = CF * cy + (q / CF) % CF, fz = CF * cz + q / (CF * CF);
+            long long u = ((long long)fz * df.ny + fy) * df.nx + fx;
+            if (nt[u] == NT_REG) {
+                double az = (z[u - 1] + z[u + 1] + z[u - sj] + z[u + sj] + z[u - sk] + z[u + sk] -
+                             6.0 * z[u]) * ih2;
+                acc += t[u] - az;
+                nreg++;
+            }
+        }
+        float xv = nreg ? (float)(-h2 * acc / nreg) : 0.f;
+        x2[2 * v] = xv;
+        x2[2 * v + 1] = 1.f - (float)nreg / (float)(CF * CF * CF);
+        if (nreg) {
+            s2 = (double)xv * xv;
+            cnt = 1;
+        }
+    }
+    block_sum2(s2, cnt);
+    if (threadIdx.x == 0) {
+        atomicAdd(&sums[0], s2);
+        atomicAdd(&sums[1], cnt);
+    }
+}
+
+// c = P y on regular nodes (0 elsewhere); P = identity (CF 1) or trilinear
+// from aggregate centres (CF 2: weights 3/4 on the own aggregate, 1/4 on the
+// neighbour on the node's side, per axis)
+template <int CF>
+__global__ void __launch_bounds__(256)
+    k_prolong(Dims df, Dims dc, const uint8_t* __restrict__ nt, const float* __restrict__ y,
+              double* __restrict__ c) {
+    long long u = (long long)blockIdx.x * 256 + threadIdx.x;
+    if (u >= df.v) return;
+    if (nt[u] != NT_REG) {
+        c[u] = 0.0;
+        return;
+    }
+    int i, j, k;
+    vox_xyz(df, u, i, j, k);
+    float val;
+    if (CF == 1) {
+        val = y[u];
+    } else {
+        int c[3] = {i >> 1, j >> 1, k >> 1};
+        int o[3] = {(i & 1) ? 1 : -1, (j & 1) ? 1 : -1, (k & 1) ? 1 : -1};
+        int n[3] = {dc.nx, dc.ny, dc.nz};
+        int cn[3];
+#pragma unroll
+        for (int a = 0; a < 3; a++) {
+            cn[a] = c[a] + o[a];
+            if (cn[a] < 0 || cn[a] >= n[a]) cn[a] = c[a];
+        }
+        val = 0.f;
+#pragma unroll
+        for (int q = 0; q < 8; q++) {
+            int xx = (q & 1) ? cn[0] : c[0], yy = (q & 2) ? cn[1] : c[1], zz = (q & 4) ? cn[2] : c[2];
+            float w = ((q & 1) ? 0.25f : 0.75f) * ((q & 2) ? 0.25f : 0.75f) * ((q & 4) ? 0.25f : 0.75f);
+            val += w * y[((long long)zz * dc.ny + yy) * dc.nx + xx];
+        }
+    }
+    c[u] = (double)val;
+}
+
+// line search of the coarse correction c: with res = t - A z,
+// out += {c.res, c.A c} over regular nodes (interior: all neighbours exist)
+__global__ void __launch_bounds__(256)
+    k_cdots(Dims df, double ih2, const uint8_t* __restrict__ nt, const double* __restrict__ t,
+            const double* __restrict__ z, const double* __restrict__ c, double* __restrict__ out) {
+    const long long sj = df.nx, sk = (long long)df.nx * df.ny;
+    double acc[2] = {0, 0};
+    for (long long u = (long long)blockIdx.x * 256 + threadIdx.x; u < df.v;
+         u += (long long)gridDim.x * 256) {
+        if (nt[u] != NT_REG) continue;
+        double az = (z[u - 1] + z[u + 1] + z[u - sj] + z[u + sj] + z[u - sk] + z[u + sk] - 6.0 * z[u]) * ih2;
+        double ac = (c[u - 1] + c[u + 1] + c[u - sj] + c[u + sj] + c[u - sk] + c[u + sk] - 6.0 * c[u]) * ih2;
+        double cu = c[u];
+        acc[0] += cu * (t[u] - az);
+        acc[1] += cu * ac;
+    }
+    block_sums<2>(acc);
+    if (threadIdx.x == 0) {
+        atomicAdd(&out[0], acc[0]);
+        atomicAdd(&out[1], acc[1]);
+    }
+}
+
+// z += omega c, omega = (c.res) / (c.A c) minimises the A-norm error of z
+__global__ void __launch_bounds__(256)
+    k_cupdate(long long n, const double* __restrict__ d, const double* __restrict__ c,
+              double* __restrict__ z) {
+    long long u = (long long)blockIdx.x * 256 + threadIdx.x;
+    if (u >= n) return;
+    double om = d[1] != 0.0 ? d[0] / d[1] : 0.0;
+    z[u] += om * c[u];
+}
+
 }  // namespace
 
 // =========================================================================
 // host side: network state
 // =========================================================================
+constexpr int MAXLV = PIC_UNET_MAX_LEVELS;
+
 struct UNetLayer {
     int cin, cout, taps;  // taps 27 or 1; direct layers: in (cin 2) / out (cout 1)
     size_t woff, boff;    // offsets into the FP32 parameter vector
@@ -533,24 +864,28 @@ struct UNetLayer {
     bf16* wt = nullptr;   // BF16 transposed, tap-flipped
 };
 
+struct ULevel {
+    Dims d;
+    int C;
+    bf16 *a = nullptr, *s = nullptr, *p = nullptr;  // p: pooled input (l >= 1)
+    bf16 *u = nullptr, *m = nullptr, *h = nullptr;  // decoder (l <= D-2); u on level l+1's grid
+    bf16 *ga = nullptr, *gs = nullptr, *gp = nullptr, *gu = nullptr, *gm = nullptr, *gh = nullptr;
+};
+
 struct UNet {
-    int C1, C2, C3;
-    Dims d1, d2, d3;
-    std::vector<UNetLayer> L;  // order: in, c1b, c2a, c2b, c3a, c3b, u2, c2c, u1, c1c, out
+    int D = 3, coarsen = 1;
+    ULevel lv[MAXLV];
+    std::vector<UNetLayer> L;
+    int l_in = 0, l_out = 0, l_a[MAXLV] = {}, l_b[MAXLV] = {}, l_u[MAXLV] = {}, l_dec[MAXLV] = {};
     size_t nparam = 0;
     float *p = nullptr, *g = nullptr, *m = nullptr, *v = nullptr;
-    // activations (BF16 channels-last) and their gradients
-    float* x = nullptr;   // [V1][2]
-    float* y = nullptr;   // [V1]
-    float* dy = nullptr;  // [V1]
-    bf16 *a1a, *s1, *p1, *a2a, *s2, *p2, *a3a, *a3b, *u2, *a2, *c2c, *u1, *a1, *c1c;
-    bf16 *g1a, *gs1, *gp1, *g2a, *gs2, *gp2, *g3a, *g3b, *gu2, *g2, *gc2c, *gu1, *g1, *gc1c;
+    float* x = nullptr;   // network input [V0][2]
+    float* y = nullptr;   // [V0]
+    float* dy = nullptr;  // [V0]
     long long adam_t = 0;
     std::vector<void*> allocs;
     bool trained = false;
 };
-
-enum { L_IN = 0, L_C1B, L_C2A, L_C2B, L_C3A, L_C3B, L_U2, L_C2C, L_U1, L_C1C, L_OUT, L_N };
 
 void pic_unet_free(pic_ctx* ctx) {
     UNet* u = (UNet*)ctx->unet;
@@ -577,43 +912,66 @@ static void init_params(UNet* u, uint32_t seed, std::vector<float>& h) {
         s ^= s << 5;
         return (float)((s >> 8) * (1.0 / 8388608.0) - 1.0);
     };
-    for (auto& l : u->L) {
-        size_t nw = (size_t)l.cout * l.taps * l.cin;
-        float scale = std::sqrt(6.0f / (float)(l.taps * l.cin));
-        if (&l == &u->L[L_OUT]) scale *= 0.1f;
-        for (size_t q = 0; q < nw; q++) h[l.woff + q] = scale * rnd();
+    for (size_t l = 0; l < u->L.size(); l++) {
+        const UNetLayer& ly = u->L[l];
+        size_t nw = (size_t)ly.cout * ly.taps * ly.cin;
+        float scale = std::sqrt(6.0f / (float)(ly.taps * ly.cin));
+        if ((int)l == u->l_out) scale *= 0.1f;
+        for (size_t q = 0; q < nw; q++) h[ly.woff + q] = scale * rnd();
     }
 }
 
-static int unet_build(pic_ctx* ctx, int C1, int C2, int C3, uint32_t seed) {
+static int unet_build(pic_ctx* ctx, const pic_unet_desc* desc) {
     pic_unet_free(ctx);
     const GridC& g = ctx->g;
-    if (g.ni % 4 || g.nj % 4 || g.nk % 4)
-        return pic_fail(ctx, PIC_ERR_ARG, "U-Net needs node counts divisible by 4");
-    if (C1 % 16 || C2 % 16 || C3 % 16 || C1 > 32 || C2 > 64 || C3 > 64)
-        return pic_fail(ctx, PIC_ERR_ARG, "U-Net channels must be multiples of 16 (C1<=32, C2,C3<=64)");
+    const int D = desc->levels, cf = desc->coarsen;
+    if (D < 2 || D > MAXLV) return pic_fail(ctx, PIC_ERR_ARG, "U-Net levels must be 2..5");
+    if (cf != 1 && cf != 2) return pic_fail(ctx, PIC_ERR_ARG, "U-Net coarsen must be 1 or 2");
+    for (int l = 0; l < D; l++) {
+        int c = desc->channels[l];
+        if (c < 16 || c % 16 || c > (l == 0 ? 32 : 64))
+            return pic_fail(ctx, PIC_ERR_ARG,
+                            "U-Net channels must be multiples of 16 (level 0 <= 32, others <= 64)");
+    }
+    int div = cf << (D - 1);
+    if (g.ni % div || g.nj % div || g.nk % div)
+        return pic_fail(ctx, PIC_ERR_ARG, "U-Net: node counts must be divisible by coarsen * 2^(levels-1)");
     UNet* u = new UNet();
     ctx->unet = u;
-    u->C1 = C1;
-    u->C2 = C2;
-    u->C3 = C3;
-    u->d1 = Dims{g.ni, g.nj, g.nk, g.nn};
-    u->d2 = half_dims(u->d1);
-    u->d3 = half_dims(u->d2);
-    int spec[L_N][3] = {{2, C1, 27},  {C1, C1, 27}, {C1, C2, 27}, {C2, C2, 27},
-                        {C2, C3, 27}, {C3, C3, 27}, {C3, C2, 1},  {C2, C2, 27},
-                        {C2, C1, 1},  {C1, C1, 27}, {C1, 1, 27}};
-    size_t off = 0;
-    for (int l = 0; l < L_N; l++) {
+    u->D = D;
+    u->coarsen = cf;
+    Dims d{g.ni / cf, g.nj / cf, g.nk / cf, 0};
+    d.v = (long long)d.nx * d.ny * d.nz;
+    for (int l = 0; l < D; l++) {
+        u->lv[l].d = l == 0 ? d : scaled_dims(u->lv[l - 1].d, 2);
+        u->lv[l].C = desc->channels[l];
+    }
+    auto add = [&](int cin, int cout, int taps) {
         UNetLayer ly;
-        ly.cin = spec[l][0];
-        ly.cout = spec[l][1];
-        ly.taps = spec[l][2];
+        ly.cin = cin;
+        ly.cout = cout;
+        ly.taps = taps;
+        u->L.push_back(ly);
+        return (int)u->L.size() - 1;
+    };
+    const int* C = desc->channels;
+    u->l_in = add(2, C[0], 27);
+    u->l_b[0] = add(C[0], C[0], 27);
+    for (int l = 1; l < D; l++) {
+        u->l_a[l] = add(C[l - 1], C[l], 27);
+        u->l_b[l] = add(C[l], C[l], 27);
+    }
+    for (int l = D - 2; l >= 0; l--) {
+        u->l_u[l] = add(C[l + 1], C[l], 1);
+        u->l_dec[l] = add(C[l], C[l], 27);
+    }
+    u->l_out = add(C[0], 1, 27);
+    size_t off = 0;
+    for (auto& ly : u->L) {
         ly.woff = off;
         off += (size_t)ly.cin * ly.cout * ly.taps;
         ly.boff = off;
         off += ly.cout;
-        u->L.push_back(ly);
     }
     u->nparam = off;
     int rc = 0;
@@ -622,32 +980,44 @@ static int unet_build(pic_ctx* ctx, int C1, int C2, int C3, uint32_t seed) {
     rc |= ualloc(ctx, u, (void**)&u->g, pb);
     rc |= ualloc(ctx, u, (void**)&u->m, pb);
     rc |= ualloc(ctx, u, (void**)&u->v, pb);
-    for (int l = L_C1B; l < L_OUT; l++) {
+    for (size_t l = 0; l < u->L.size(); l++) {
+        if ((int)l == u->l_in || (int)l == u->l_out) continue;
         size_t nw = (size_t)u->L[l].cin * u->L[l].cout * u->L[l].taps * sizeof(bf16);
         rc |= ualloc(ctx, u, (void**)&u->L[l].wb, nw);
         rc |= ualloc(ctx, u, (void**)&u->L[l].wt, nw);
     }
-    long long V1 = u->d1.v, V2 = u->d2.v, V3 = u->d3.v;
-    rc |= ualloc(ctx, u, (void**)&u->x, V1 * 2 * sizeof(float));
-    rc |= ualloc(ctx, u, (void**)&u->y, V1 * sizeof(float));
-    rc |= ualloc(ctx, u, (void**)&u->dy, V1 * sizeof(float));
-    struct A {
-        bf16** p;
-        long long n;
-    } acts[] = {{&u->a1a, V1 * C1}, {&u->s1, V1 * C1},  {&u->p1, V2 * C1},  {&u->a2a, V2 * C2},
-                {&u->s2, V2 * C2},  {&u->p2, V3 * C2},  {&u->a3a, V3 * C3}, {&u->a3b, V3 * C3},
-                {&u->u2, V3 * C2},  {&u->a2, V2 * C2},  {&u->c2c, V2 * C2}, {&u->u1, V2 * C1},
-                {&u->a1, V1 * C1},  {&u->c1c, V1 * C1}, {&u->g1a, V1 * C1}, {&u->gs1, V1 * C1},
-                {&u->gp1, V2 * C1}, {&u->g2a, V2 * C2}, {&u->gs2, V2 * C2}, {&u->gp2, V3 * C2},
-                {&u->g3a, V3 * C3}, {&u->g3b, V3 * C3}, {&u->gu2, V3 * C2}, {&u->g2, V2 * C2},
-                {&u->gc2c, V2 * C2}, {&u->gu1, V2 * C1}, {&u->g1, V1 * C1}, {&u->gc1c, V1 * C1}};
-    for (auto& a : acts) rc |= ualloc(ctx, u, (void**)a.p, a.n * sizeof(bf16));
+    long long V0 = d.v;
+    rc |= ualloc(ctx, u, (void**)&u->x, V0 * 2 * sizeof(float));
+    rc |= ualloc(ctx, u, (void**)&u->y, V0 * sizeof(float));
+    rc |= ualloc(ctx, u, (void**)&u->dy, V0 * sizeof(float));
+    for (int l = 0; l < D && !rc; l++) {
+        ULevel& L = u->lv[l];
+        long long V = L.d.v, n = V * L.C;
+        rc |= ualloc(ctx, u, (void**)&L.a, n * 2);
+        rc |= ualloc(ctx, u, (void**)&L.s, n * 2);
+        rc |= ualloc(ctx, u, (void**)&L.ga, n * 2);
+        if (l >= 1) {
+            long long np = V * u->lv[l - 1].C;
+            rc |= ualloc(ctx, u, (void**)&L.p, np * 2);
+            rc |= ualloc(ctx, u, (void**)&L.gp, np * 2);
+        }
+        if (l == D - 1) rc |= ualloc(ctx, u, (void**)&L.gs, n * 2);
+        if (l <= D - 2) {
+            long long nu = u->lv[l + 1].d.v * L.C;
+            rc |= ualloc(ctx, u, (void**)&L.u, nu * 2);
+            rc |= ualloc(ctx, u, (void**)&L.gu, nu * 2);
+            rc |= ualloc(ctx, u, (void**)&L.m, n * 2);
+            rc |= ualloc(ctx, u, (void**)&L.h, n * 2);
+            rc |= ualloc(ctx, u, (void**)&L.gm, n * 2);
+            rc |= ualloc(ctx, u, (void**)&L.gh, n * 2);
+        }
+    }
     if (rc) {
         pic_unet_free(ctx);
         return PIC_ERR_HIP;
     }
     std::vector<float> h;
-    init_params(u, seed, h);
+    init_params(u, desc->seed, h);
     PIC_HIP(ctx, hipMemcpyAsync(u->p, h.data(), pb, hipMemcpyHostToDevice, ctx->stream));
     PIC_HIP(ctx, hipMemsetAsync(u->m, 0, pb, ctx->stream));
     PIC_HIP(ctx, hipMemsetAsync(u->v, 0, pb, ctx->stream));
@@ -657,8 +1027,9 @@ static int unet_build(pic_ctx* ctx, int C1, int C2, int C3, uint32_t seed) {
 
 // refresh the BF16 weight copies from the FP32 master parameters
 static int unet_sync_weights(pic_ctx* ctx, UNet* u) {
-    for (int l = L_C1B; l < L_OUT; l++) {
+    for (size_t l = 0; l < u->L.size(); l++) {
         UNetLayer& ly = u->L[l];
+        if (!ly.wb) continue;
         long long n = (long long)ly.cout * ly.taps * ly.cin;
         k_transpose_w<<<grid_for(n, 256), 256, 0, ctx->stream>>>(u->p + ly.woff, ly.cout, ly.cin,
                                                                 ly.taps, ly.wb, ly.wt);
@@ -668,9 +1039,8 @@ static int unet_sync_weights(pic_ctx* ctx, UNet* u) {
 }
 
 static void conv(pic_ctx* ctx, const Dims& d, const bf16* in, int cin, const bf16* w,
-                 const float* bias, bf16* out, int cout, int taps, int relu, const bf16* mask,
-                 int accumulate) {
-    ConvArgs a{in, cin, w, bias, out, cout, mask, relu, accumulate, d};
+                 const float* bias, bf16* out, int cout, int taps, int relu, const bf16* mask) {
+    ConvArgs a{in, cin, w, bias, out, cout, mask, relu, 0, d};
     int ntx = (d.nx + TX - 1) / TX, nty = (d.ny + TY - 1) / TY, ntz = (d.nz + TZ - 1) / TZ;
     dim3 grid(ntx * nty * ntz, (cout / 16 + NBB - 1) / NBB);
     if (taps == 27)
@@ -679,13 +1049,19 @@ static void conv(pic_ctx* ctx, const Dims& d, const bf16* in, int cin, const bf1
         k_conv_mfma<1><<<grid, 256, 0, ctx->stream>>>(a);
 }
 
-static void wgrad(pic_ctx* ctx, const Dims& d, const bf16* X, int cin, const bf16* dY, int cout,
-                  int taps, float* dW, float* db) {
+static int wgrad_tiles(const Dims& d, int* tpb) {
     int ntx = (d.nx + WX - 1) / WX, nty = (d.ny + WY - 1) / WY, ntz = (d.nz + WZ - 1) / WZ;
     int ntiles = ntx * nty * ntz;
-    int tpb = std::max(1, ntiles / 512);
+    *tpb = std::max(1, ntiles / 512);
+    return (ntiles + *tpb - 1) / *tpb;
+}
+
+static void wgrad(pic_ctx* ctx, const Dims& d, const bf16* X, int cin, const bf16* dY, int cout,
+                  int taps, float* dW, float* db) {
+    int tpb;
+    int nblk = wgrad_tiles(d, &tpb);
     int combos = taps * (cout / 4) * (cin / 4);
-    dim3 grid((ntiles + tpb - 1) / tpb, (combos + 1023) / 1024);
+    dim3 grid(nblk, (combos + 1023) / 1024);
     int H = taps == 27 ? 1 : 0;
     size_t lds = ((size_t)(WX + 2 * H) * (WY + 2 * H) * (WZ + 2 * H) * cin +
                   (size_t)WX * WY * WZ * cout) * sizeof(bf16);
@@ -699,92 +1075,142 @@ static void wgrad(pic_ctx* ctx, const Dims& d, const bf16* X, int cin, const bf1
 
 static unsigned ew(long long n) { return grid_for((size_t)n, 256); }
 
-// forward pass: u->x (FP32 [V1][2]) -> u->y (FP32 [V1])
-static int unet_forward(pic_ctx* ctx, UNet* u) {
-    const Dims &d1 = u->d1, &d2 = u->d2, &d3 = u->d3;
-    const int C1 = u->C1, C2 = u->C2, C3 = u->C3;
+// forward pass: x (FP32 [V0][2], channel 0 scaled by nrm if given) -> u->y
+static int unet_forward(pic_ctx* ctx, UNet* u, const float* x, const double* nrm) {
+    const int D = u->D;
+    ULevel* lv = u->lv;
     auto& L = u->L;
     float* P = u->p;
     PicProfScope ps(ctx, PIC_K_UNET);
-    k_conv_in<<<ew(d1.v), 256, 0, ctx->stream>>>(d1, u->x, P + L[L_IN].woff, P + L[L_IN].boff,
-                                                 u->a1a, C1);
-    conv(ctx, d1, u->a1a, C1, L[L_C1B].wb, P + L[L_C1B].boff, u->s1, C1, 27, 1, nullptr, 0);
-    k_pool<<<ew(d2.v * C1), 256, 0, ctx->stream>>>(d2, u->s1, u->p1, C1);
-    conv(ctx, d2, u->p1, C1, L[L_C2A].wb, P + L[L_C2A].boff, u->a2a, C2, 27, 1, nullptr, 0);
-    conv(ctx, d2, u->a2a, C2, L[L_C2B].wb, P + L[L_C2B].boff, u->s2, C2, 27, 1, nullptr, 0);
-    k_pool<<<ew(d3.v * C2), 256, 0, ctx->stream>>>(d3, u->s2, u->p2, C2);
-    conv(ctx, d3, u->p2, C2, L[L_C3A].wb, P + L[L_C3A].boff, u->a3a, C3, 27, 1, nullptr, 0);
-    conv(ctx, d3, u->a3a, C3, L[L_C3B].wb, P + L[L_C3B].boff, u->a3b, C3, 27, 1, nullptr, 0);
-    conv(ctx, d3, u->a3b, C3, L[L_U2].wb, P + L[L_U2].boff, u->u2, C2, 1, 0, nullptr, 0);
-    k_up_add<<<ew(d2.v * C2), 256, 0, ctx->stream>>>(d2, u->u2, u->s2, u->a2, C2);
-    conv(ctx, d2, u->a2, C2, L[L_C2C].wb, P + L[L_C2C].boff, u->c2c, C2, 27, 1, nullptr, 0);
-    conv(ctx, d2, u->c2c, C2, L[L_U1].wb, P + L[L_U1].boff, u->u1, C1, 1, 0, nullptr, 0);
-    k_up_add<<<ew(d1.v * C1), 256, 0, ctx->stream>>>(d1, u->u1, u->s1, u->a1, C1);
-    conv(ctx, d1, u->a1, C1, L[L_C1C].wb, P + L[L_C1C].boff, u->c1c, C1, 27, 1, nullptr, 0);
-    k_conv_out<<<ew(d1.v), 256, 0, ctx->stream>>>(d1, u->c1c, C1, P + L[L_OUT].woff,
-                                                  P + L[L_OUT].boff, u->y);
+    const Dims& d0 = lv[0].d;
+    const int C0 = lv[0].C;
+    const UNetLayer& li = L[u->l_in];
+    if (C0 == 16)
+        k_conv_in<16><<<ew(d0.v), 256, 0, ctx->stream>>>(d0, x, nrm, P + li.woff, P + li.boff, lv[0].a);
+    else
+        k_conv_in<32><<<ew(d0.v), 256, 0, ctx->stream>>>(d0, x, nrm, P + li.woff, P + li.boff, lv[0].a);
+    conv(ctx, d0, lv[0].a, C0, L[u->l_b[0]].wb, P + L[u->l_b[0]].boff, lv[0].s, C0, 27, 1, nullptr);
+    for (int l = 1; l < D; l++) {
+        const Dims& d = lv[l].d;
+        int Cp = lv[l - 1].C, Cl = lv[l].C;
+        k_pool<<<ew(d.v * Cp / 8), 256, 0, ctx->stream>>>(d, lv[l - 1].s, lv[l].p, Cp);
+        conv(ctx, d, lv[l].p, Cp, L[u->l_a[l]].wb, P + L[u->l_a[l]].boff, lv[l].a, Cl, 27, 1, nullptr);
+        conv(ctx, d, lv[l].a, Cl, L[u->l_b[l]].wb, P + L[u->l_b[l]].boff, lv[l].s, Cl, 27, 1, nullptr);
+    }
+    for (int l = D - 2; l >= 0; l--) {
+        const Dims& d = lv[l].d;
+        int Cl = lv[l].C, Cn = lv[l + 1].C;
+        const bf16* src = l == D - 2 ? lv[D - 1].s : lv[l + 1].h;
+        conv(ctx, lv[l + 1].d, src, Cn, L[u->l_u[l]].wb, P + L[u->l_u[l]].boff, lv[l].u, Cl, 1, 0,
+             nullptr);
+        k_up_add<<<ew(d.v * Cl / 8), 256, 0, ctx->stream>>>(d, lv[l].u, lv[l].s, lv[l].m, Cl);
+        conv(ctx, d, lv[l].m, Cl, L[u->l_dec[l]].wb, P + L[u->l_dec[l]].boff, lv[l].h, Cl, 27, 1,
+             nullptr);
+    }
+    const UNetLayer& lo = L[u->l_out];
+    if (C0 == 16)
+        k_conv_out<16><<<ew(d0.v), 256, 0, ctx->stream>>>(d0, lv[0].h, P + lo.woff, P + lo.boff, u->y);
+    else
+        k_conv_out<32><<<ew(d0.v), 256, 0, ctx->stream>>>(d0, lv[0].h, P + lo.woff, P + lo.boff, u->y);
     PIC_HIP(ctx, hipGetLastError());
     return PIC_OK;
 }
 
-// backward pass from u->dy (FP32 [V1]); accumulates into u->g
-static int unet_backward(pic_ctx* ctx, UNet* u) {
-    const Dims &d1 = u->d1, &d2 = u->d2, &d3 = u->d3;
-    const int C1 = u->C1, C2 = u->C2, C3 = u->C3;
+template <int C>
+static void output_grads(pic_ctx* ctx, UNet* u) {
+    const Dims& d0 = u->lv[0].d;
+    int tpb;
+    int nblk = wgrad_tiles(d0, &tpb);
+    const UNetLayer& lo = u->L[u->l_out];
+    k_wgrad_direct<1, C, float, bf16><<<nblk, 256, 0, ctx->stream>>>(
+        d0, u->lv[0].h, u->dy, u->g + lo.woff, u->g + lo.boff, tpb);
+}
+
+template <int C>
+static void input_grads(pic_ctx* ctx, UNet* u, const float* x) {
+    const Dims& d0 = u->lv[0].d;
+    int tpb;
+    int nblk = wgrad_tiles(d0, &tpb);
+    const UNetLayer& li = u->L[u->l_in];
+    k_wgrad_direct<C, 2, bf16, float><<<nblk, 256, 0, ctx->stream>>>(
+        d0, x, u->lv[0].ga, u->g + li.woff, u->g + li.boff, tpb);
+}
+
+// backward pass from u->dy (FP32 [V0]); accumulates into u->g
+static int unet_backward(pic_ctx* ctx, UNet* u, const float* x) {
+    const int D = u->D;
+    ULevel* lv = u->lv;
     auto& L = u->L;
     float* G = u->g;
     PicProfScope ps(ctx, PIC_K_UNET);
+    const Dims& d0 = lv[0].d;
+    const int C0 = lv[0].C;
     // output layer
-    k_wgrad_out<<<grid_for(d1.v, DCHUNK), 256, 0, ctx->stream>>>(d1, u->c1c, C1, u->dy,
-                                                                 G + L[L_OUT].woff, G + L[L_OUT].boff);
-    k_conv_out_bwd<<<ew(d1.v), 256, 0, ctx->stream>>>(d1, u->dy, C1, u->p + L[L_OUT].woff, u->c1c,
-                                                      u->gc1c);
-    // c1c (input a1, no ReLU before it)
-    wgrad(ctx, d1, u->a1, C1, u->gc1c, C1, 27, G + L[L_C1C].woff, G + L[L_C1C].boff);
-    conv(ctx, d1, u->gc1c, C1, L[L_C1C].wt, nullptr, u->g1, C1, 27, 0, nullptr, 0);
-    // a1 = up(u1) + s1: g(s1) starts as g1; g(u1) = sum over children
-    k_up_bwd<<<ew(d2.v * C1), 256, 0, ctx->stream>>>(d2, u->g1, u->gu1, C1);
-    // u1 (1x1, input c2c which is post-ReLU)
-    wgrad(ctx, d2, u->c2c, C2, u->gu1, C1, 1, G + L[L_U1].woff, G + L[L_U1].boff);
-    conv(ctx, d2, u->gu1, C1, L[L_U1].wt, nullptr, u->gc2c, C2, 1, 0, u->c2c, 0);
-    // c2c (input a2)
-    wgrad(ctx, d2, u->a2, C2, u->gc2c, C2, 27, G + L[L_C2C].woff, G + L[L_C2C].boff);
-    conv(ctx, d2, u->gc2c, C2, L[L_C2C].wt, nullptr, u->g2, C2, 27, 0, nullptr, 0);
-    // a2 = up(u2) + s2
-    k_up_bwd<<<ew(d3.v * C2), 256, 0, ctx->stream>>>(d3, u->g2, u->gu2, C2);
-    wgrad(ctx, d3, u->a3b, C3, u->gu2, C2, 1, G + L[L_U2].woff, G + L[L_U2].boff);
-    conv(ctx, d3, u->gu2, C2, L[L_U2].wt, nullptr, u->g3b, C3, 1, 0, u->a3b, 0);
-    // c3b (input a3a)
-    wgrad(ctx, d3, u->a3a, C3, u->g3b, C3, 27, G + L[L_C3B].woff, G + L[L_C3B].boff);
-    conv(ctx, d3, u->g3b, C3, L[L_C3B].wt, nullptr, u->g3a, C3, 27, 0, u->a3a, 0);
-    // c3a (input p2 = pool(s2))
-    wgrad(ctx, d3, u->p2, C2, u->g3a, C3, 27, G + L[L_C3A].woff, G + L[L_C3A].boff);
-    conv(ctx, d3, u->g3a, C3, L[L_C3A].wt, nullptr, u->gp2, C2, 27, 0, nullptr, 0);
-    // g(s2) = (g2 + pool_bwd(gp2)) * relu'(s2)   (g2 buffer reused in place)
-    k_pool_bwd<<<ew(d2.v * C2), 256, 0, ctx->stream>>>(d2, u->gp2, u->g2, u->s2, C2);
-    // c2b (input a2a)
-    wgrad(ctx, d2, u->a2a, C2, u->g2, C2, 27, G + L[L_C2B].woff, G + L[L_C2B].boff);
-    conv(ctx, d2, u->g2, C2, L[L_C2B].wt, nullptr, u->g2a, C2, 27, 0, u->a2a, 0);
-    // c2a (input p1 = pool(s1))
-    wgrad(ctx, d2, u->p1, C1, u->g2a, C2, 27, G + L[L_C2A].woff, G + L[L_C2A].boff);
-    conv(ctx, d2, u->g2a, C2, L[L_C2A].wt, nullptr, u->gp1, C1, 27, 0, nullptr, 0);
-    k_pool_bwd<<<ew(d1.v * C1), 256, 0, ctx->stream>>>(d1, u->gp1, u->g1, u->s1, C1);
-    // c1b (input a1a)
-    wgrad(ctx, d1, u->a1a, C1, u->g1, C1, 27, G + L[L_C1B].woff, G + L[L_C1B].boff);
-    conv(ctx, d1, u->g1, C1, L[L_C1B].wt, nullptr, u->g1a, C1, 27, 0, u->a1a, 0);
-    // input layer
-    k_wgrad_in<<<grid_for(d1.v, DCHUNK), 256, 0, ctx->stream>>>(d1, u->x, u->g1a, C1,
-                                                                G + L[L_IN].woff, G + L[L_IN].boff);
+    if (C0 == 16) {
+        output_grads<16>(ctx, u);
+        k_conv_out_bwd<16><<<ew(d0.v), 256, 0, ctx->stream>>>(d0, u->dy, u->p + L[u->l_out].woff,
+                                                              lv[0].h, lv[0].gh);
+    } else {
+        output_grads<32>(ctx, u);
+        k_conv_out_bwd<32><<<ew(d0.v), 256, 0, ctx->stream>>>(d0, u->dy, u->p + L[u->l_out].woff,
+                                                              lv[0].h, lv[0].gh);
+    }
+    // decoder, top down: gh_l is the gradient at h_l's pre-activation
+    for (int l = 0; l <= D - 2; l++) {
+        const Dims& d = lv[l].d;
+        int Cl = lv[l].C, Cn = lv[l + 1].C;
+        const UNetLayer& dec = L[u->l_dec[l]];
+        wgrad(ctx, d, lv[l].m, Cl, lv[l].gh, Cl, 27, G + dec.woff, G + dec.boff);
+        conv(ctx, d, lv[l].gh, Cl, dec.wt, nullptr, lv[l].gm, Cl, 27, 0, nullptr);
+        k_up_bwd<<<ew(lv[l + 1].d.v * Cl / 8), 256, 0, ctx->stream>>>(lv[l + 1].d, lv[l].gm, lv[l].gu, Cl);
+        const bf16* src = l == D - 2 ? lv[D - 1].s : lv[l + 1].h;
+        bf16* gsrc = l == D - 2 ? lv[D - 1].gs : lv[l + 1].gh;
+        const UNetLayer& up = L[u->l_u[l]];
+        wgrad(ctx, lv[l + 1].d, src, Cn, lv[l].gu, Cl, 1, G + up.woff, G + up.boff);
+        conv(ctx, lv[l + 1].d, lv[l].gu, Cl, up.wt, nullptr, gsrc, Cn, 1, 0, src);
+    }
+    // encoder, bottom up
+    for (int l = D - 1; l >= 1; l--) {
+        const Dims& d = lv[l].d;
+        int Cl = lv[l].C, Cp = lv[l - 1].C;
+        bf16* gsl;  // gradient at s_l's pre-activation
+        if (l == D - 1) {
+            gsl = lv[l].gs;
+        } else {
+            k_pool_bwd<<<ew(d.v * Cl / 8), 256, 0, ctx->stream>>>(d, lv[l + 1].gp, lv[l].gm, lv[l].s, Cl);
+            gsl = lv[l].gm;
+        }
+        const UNetLayer &B = L[u->l_b[l]], &A = L[u->l_a[l]];
+        wgrad(ctx, d, lv[l].a, Cl, gsl, Cl, 27, G + B.woff, G + B.boff);
+        conv(ctx, d, gsl, Cl, B.wt, nullptr, lv[l].ga, Cl, 27, 0, lv[l].a);
+        wgrad(ctx, d, lv[l].p, Cp, lv[l].ga, Cl, 27, G + A.woff, G + A.boff);
+        conv(ctx, d, lv[l].ga, Cl, A.wt, nullptr, lv[l].gp, Cp, 27, 0, nullptr);
+    }
+    // level 0
+    k_pool_bwd<<<ew(d0.v * C0 / 8), 256, 0, ctx->stream>>>(d0, lv[1].gp, lv[0].gm, lv[0].s, C0);
+    const UNetLayer& B0 = L[u->l_b[0]];
+    wgrad(ctx, d0, lv[0].a, C0, lv[0].gm, C0, 27, G + B0.woff, G + B0.boff);
+    conv(ctx, d0, lv[0].gm, C0, B0.wt, nullptr, lv[0].ga, C0, 27, 0, lv[0].a);
+    if (C0 == 16)
+        input_grads<16>(ctx, u, x);
+    else
+        input_grads<32>(ctx, u, x);
     PIC_HIP(ctx, hipGetLastError());
     return PIC_OK;
 }
 
 static UNet* get_unet(pic_ctx* ctx) { return ctx ? (UNet*)ctx->unet : nullptr; }
 
+static double count_valid(const float* x2, size_t V) {
+    size_t n = 0;
+    for (size_t q = 0; q < V; q++) n += x2[2 * q + 1] < 0.999f;
+    return (double)std::max<size_t>(1, n);
+}
+
 // ---------------------------------------------------------------- ABI
 extern "C" int pic_unet_create(pic_ctx* ctx, const pic_unet_desc* desc) {
     if (!ctx || !desc) return PIC_ERR_ARG;
-    int rc = unet_build(ctx, desc->c1, desc->c2, desc->c3, desc->seed);
+    int rc = unet_build(ctx, desc);
     if (rc) return rc;
     return unet_sync_weights(ctx, get_unet(ctx));
 }
@@ -818,9 +1244,9 @@ extern "C" int pic_unet_set_params(pic_ctx* ctx, const float* src, size_t n) {
 extern "C" int pic_unet_forward(pic_ctx* ctx, const float* in2, float* out) {
     UNet* u = get_unet(ctx);
     if (!u || !in2 || !out) return PIC_ERR_ARG;
-    size_t V = (size_t)u->d1.v;
+    size_t V = (size_t)u->lv[0].d.v;
     PIC_HIP(ctx, hipMemcpyAsync(u->x, in2, 2 * V * sizeof(float), hipMemcpyHostToDevice, ctx->stream));
-    int rc = unet_forward(ctx, u);
+    int rc = unet_forward(ctx, u, u->x, nullptr);
     if (rc) return rc;
     PIC_HIP(ctx, hipMemcpyAsync(out, u->y, V * sizeof(float), hipMemcpyDeviceToHost, ctx->stream));
     PIC_HIP(ctx, hipStreamSynchronize(ctx->stream));
@@ -831,7 +1257,7 @@ extern "C" int pic_unet_loss_grad(pic_ctx* ctx, const float* in2, const float* t
                                   double* loss, float* grad) {
     UNet* u = get_unet(ctx);
     if (!u || !in2 || !target) return PIC_ERR_ARG;
-    size_t V = (size_t)u->d1.v;
+    size_t V = (size_t)u->lv[0].d.v;
     float* tdev = nullptr;
     PIC_HIP(ctx, hipMalloc(&tdev, V * sizeof(float)));
     hipError_t e = hipMemcpyAsync(tdev, target, V * sizeof(float), hipMemcpyHostToDevice, ctx->stream);
@@ -841,16 +1267,14 @@ extern "C" int pic_unet_loss_grad(pic_ctx* ctx, const float* in2, const float* t
         (void)hipFree(tdev);
         return pic_hip_check(ctx, e, "U-Net inputs");
     }
-    size_t nvalid = 0;
-    for (size_t q = 0; q < V; q++) nvalid += in2[2 * q + 1] <= 0.5f;
-    int rc = unet_forward(ctx, u);
+    double nvalid = count_valid(in2, V);
+    int rc = unet_forward(ctx, u, u->x, nullptr);
     if (!rc) {
         (void)hipMemsetAsync(ctx->scratch, 0, sizeof(double), ctx->stream);
         (void)hipMemsetAsync(u->g, 0, u->nparam * sizeof(float), ctx->stream);
-        k_loss<<<ew(u->d1.v), 256, 0, ctx->stream>>>(u->d1.v, u->y, tdev, u->x,
-                                                     1.0f / (float)std::max<size_t>(1, nvalid),
-                                                     u->dy, ctx->scratch);
-        rc = unet_backward(ctx, u);
+        k_loss<<<ew(V), 256, 0, ctx->stream>>>((long long)V, u->y, tdev, u->x, (float)(1.0 / nvalid),
+                                               u->dy, ctx->scratch);
+        rc = unet_backward(ctx, u, u->x);
     }
     double l = 0;
     if (!rc) {
@@ -945,59 +1369,6 @@ __global__ void __launch_bounds__(256)
     rho[e] = rho_fac * v;
 }
 
-__global__ void __launch_bounds__(256)
-    k_d2f(long long n, const double* __restrict__ a, float* __restrict__ b) {
-    long long e = (long long)blockIdx.x * 256 + threadIdx.x;
-    if (e < n) b[e] = (float)a[e];
-}
-
-__global__ void __launch_bounds__(256)
-    k_make_input(long long n, const float* __restrict__ rt, const uint8_t* __restrict__ nt,
-                 float* __restrict__ x2) {
-    long long e = (long long)blockIdx.x * 256 + threadIdx.x;
-    if (e >= n) return;
-    x2[2 * e] = rt[e];
-    x2[2 * e + 1] = nt[e] == NT_DIR ? 1.f : 0.f;
-}
-
-// preconditioner input: rho_tilde = -r dx^2 / s (unit RMS), s = rms(r dx^2)
-__global__ void __launch_bounds__(256)
-    k_input_from_residual(long long n, const double* __restrict__ r, const uint8_t* __restrict__ nt,
-                          const double* __restrict__ sumsq, double dx2, double count,
-                          float* __restrict__ x2) {
-    long long e = (long long)blockIdx.x * 256 + threadIdx.x;
-    if (e >= n) return;
-    double s = sumsq[0] > 0 ? sqrt(sumsq[0] / count) : 1.0;  // rms of r (regular rows)
-    bool dir = nt[e] == NT_DIR;
-    x2[2 * e] = dir ? 0.f : (float)(-r[e] / s);
-    x2[2 * e + 1] = dir ? 1.f : 0.f;
-}
-
-// A = L / dx^2, so A^-1 r = dx^2 L^-1 r; with rho_tilde = -r dx^2 / (s dx^2)
-// the network returns phi_tilde = L^-1 (r / s), hence z = s dx^2 phi_tilde.
-// (s is the RMS of r itself, so the unit-RMS input matches the training data.)
-__global__ void __launch_bounds__(256)
-    k_output_to_z(long long n, const float* __restrict__ y, const uint8_t* __restrict__ nt,
-                  const double* __restrict__ sumsq, double dx2, double count, double* __restrict__ z) {
-    long long e = (long long)blockIdx.x * 256 + threadIdx.x;
-    if (e >= n) return;
-    double s = sumsq[0] > 0 ? sqrt(sumsq[0] / count) : 1.0;
-    z[e] = nt[e] == NT_DIR ? 0.0 : s * dx2 * (double)y[e];
-}
-
-__global__ void __launch_bounds__(256)
-    k_sumsq_regular(long long n, const double* __restrict__ r, const uint8_t* __restrict__ nt,
-                    double* __restrict__ out) {
-    double s = 0;
-    for (long long e = (long long)blockIdx.x * 256 + threadIdx.x; e < n; e += (long long)gridDim.x * 256)
-        if (nt[e] == NT_REG) s += r[e] * r[e];
-    for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
-    __shared__ double red[4];
-    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s;
-    __syncthreads();
-    if (threadIdx.x == 0) atomicAdd(out, red[0] + red[1] + red[2] + red[3]);
-}
-
 }  // namespace
 
 static bool uniform_dx(const GridC& g) {
@@ -1010,22 +1381,34 @@ bool pic_unet_ready(pic_ctx* ctx) {
     return u && u->trained && uniform_dx(ctx->g);
 }
 
-// z = M(r): the network applied to the residual (device FP64 vectors)
-int pic_unet_apply(pic_ctx* ctx, const double* r, double* z) {
+// Coarse correction of the two-level preconditioner: the residual t - A z
+// (t: the fine right-hand side, z: the smoothed approximation, both device
+// FP64) is restricted to the network grid, the network is applied, its
+// output interpolated to c (device FP64 scratch vector) and z += omega c with
+// the A-norm optimal omega.  Only for Dirichlet-walled uniform grids (every
+// regular node is interior).
+int pic_unet_coarse_correct(pic_ctx* ctx, const double* t, double* z, double* c) {
     UNet* u = get_unet(ctx);
     if (!u) return pic_fail(ctx, PIC_ERR_STATE, "no U-Net");
     const GridC& g = ctx->g;
-    long long n = g.nn;
-    double dx2 = g.dx[0] * g.dx[0];
-    double* ss = ctx->scratch + 64;
-    PIC_HIP(ctx, hipMemsetAsync(ss, 0, sizeof(double), ctx->stream));
-    unsigned nb = std::min<unsigned>(1024, ew(n));
-    k_sumsq_regular<<<nb, 256, 0, ctx->stream>>>(n, r, ctx->ntype, ss);
-    double count = (double)(g.ni - 2) * (g.nj - 2) * (g.nk - 2);
-    k_input_from_residual<<<ew(n), 256, 0, ctx->stream>>>(n, r, ctx->ntype, ss, dx2, count, u->x);
-    int rc = unet_forward(ctx, u);
+    Dims df{g.ni, g.nj, g.nk, g.nn};
+    const Dims& dc = u->lv[0].d;
+    double h2 = g.dx[0] * g.dx[0];
+    double* sums = ctx->scratch + 64;  // [0..1] input normalisation, [2..3] line search
+    PIC_HIP(ctx, hipMemsetAsync(sums, 0, 4 * sizeof(double), ctx->stream));
+    if (u->coarsen == 2)
+        k_res_restrict<2><<<ew(dc.v), 256, 0, ctx->stream>>>(df, dc, h2, ctx->ntype, t, z, u->x, sums);
+    else
+        k_res_restrict<1><<<ew(dc.v), 256, 0, ctx->stream>>>(df, dc, h2, ctx->ntype, t, z, u->x, sums);
+    int rc = unet_forward(ctx, u, u->x, sums);
     if (rc) return rc;
-    k_output_to_z<<<ew(n), 256, 0, ctx->stream>>>(n, u->y, ctx->ntype, ss, dx2, count, z);
+    if (u->coarsen == 2)
+        k_prolong<2><<<ew(df.v), 256, 0, ctx->stream>>>(df, dc, ctx->ntype, u->y, c);
+    else
+        k_prolong<1><<<ew(df.v), 256, 0, ctx->stream>>>(df, dc, ctx->ntype, u->y, c);
+    unsigned nb = std::min<unsigned>(2048, ew(df.v));
+    k_cdots<<<nb, 256, 0, ctx->stream>>>(df, 1.0 / h2, ctx->ntype, t, z, c, sums + 2);
+    k_cupdate<<<ew(df.v), 256, 0, ctx->stream>>>(df.v, sums + 2, c, z);
     PIC_HIP(ctx, hipGetLastError());
     return PIC_OK;
 }
@@ -1043,19 +1426,24 @@ extern "C" int pic_unet_train(pic_ctx* ctx, const pic_unet_train_desc* td,
     int rc = pic_ensure_ntype(ctx);
     if (rc) return rc;
     long long V = g.nn;
+    Dims df{g.ni, g.nj, g.nk, g.nn};
+    const Dims& dc = u->lv[0].d;
+    long long Vc = dc.v;
     size_t S = (size_t)td->samples;
-    float *dens = nullptr, *targ = nullptr;
+    float *dens = nullptr, *xs = nullptr, *ts = nullptr;
     double *phi_keep = nullptr, *rho_keep = nullptr;
     Blob* dblobs = nullptr;
     auto cleanup = [&]() {
         (void)hipFree(dens);
-        (void)hipFree(targ);
+        (void)hipFree(xs);
+        (void)hipFree(ts);
         (void)hipFree(phi_keep);
         (void)hipFree(rho_keep);
         (void)hipFree(dblobs);
     };
-    hipError_t e = hipMalloc(&dens, S * V * sizeof(float));
-    if (e == hipSuccess) e = hipMalloc(&targ, S * V * sizeof(float));
+    hipError_t e = hipMalloc(&dens, V * sizeof(float));
+    if (e == hipSuccess) e = hipMalloc(&xs, S * Vc * 2 * sizeof(float));
+    if (e == hipSuccess) e = hipMalloc(&ts, S * Vc * sizeof(float));
     if (e == hipSuccess) e = hipMalloc(&phi_keep, V * sizeof(double));
     if (e == hipSuccess) e = hipMalloc(&rho_keep, V * sizeof(double));
     if (e == hipSuccess) e = hipMalloc(&dblobs, 64 * sizeof(Blob));
@@ -1080,6 +1468,7 @@ extern "C" int pic_unet_train(pic_ctx* ctx, const pic_unet_train_desc* td,
     double dx2 = g.dx[0] * g.dx[0];
     double count = (double)(g.ni - 2) * (g.nj - 2) * (g.nk - 2);
     double* ss = ctx->scratch + 64;
+    std::vector<double> nvalid(S, 1.0);
     for (size_t s = 0; s < S && !rc; s++) {
         int nb = td->blobs_min + (int)(urand() * (td->blobs_max - td->blobs_min + 1));
         nb = std::min(nb, 64);
@@ -1092,17 +1481,16 @@ extern "C" int pic_unet_train(pic_ctx* ctx, const pic_unet_train_desc* td,
             hb[b].inv2s2 = (float)(1.0 / (2 * sig * sig));
             hb[b].amp = (float)((urand() < 0.5 ? -1 : 1) * (0.3 + urand()));
         }
-        float* dst = dens + s * V;
         e = hipMemcpyAsync(dblobs, hb.data(), hb.size() * sizeof(Blob), hipMemcpyHostToDevice, ctx->stream);
         if (e != hipSuccess) {
             rc = pic_hip_check(ctx, e, "blobs");
             break;
         }
-        k_blobs<<<ew(V), 256, 0, ctx->stream>>>(u->d1, dblobs, nb, (float)td->noise, td->seed,
-                                               (uint32_t)s, ctx->ntype, dst);
-        (void)hipMemsetAsync(ss, 0, sizeof(double), ctx->stream);
-        k_sumsq_f<<<std::min<unsigned>(1024, ew(V)), 256, 0, ctx->stream>>>(V, dst, ss);
-        k_norm_rho<<<ew(V), 256, 0, ctx->stream>>>(V, dst, ss, count, EPS0 / dx2, ctx->rho);
+        k_blobs<<<ew(V), 256, 0, ctx->stream>>>(df, dblobs, nb, (float)td->noise, td->seed,
+                                               (uint32_t)s, ctx->ntype, dens);
+        (void)hipMemsetAsync(ss, 0, 2 * sizeof(double), ctx->stream);
+        k_sumsq_f<<<std::min<unsigned>(1024, ew(V)), 256, 0, ctx->stream>>>(V, dens, ss);
+        k_norm_rho<<<ew(V), 256, 0, ctx->stream>>>(V, dens, ss, count, EPS0 / dx2, ctx->rho);
         (void)hipMemsetAsync(ctx->phi, 0, V * 8, ctx->stream);
         pic_solver_stats st;
         int r = pic_poisson_internal_sor(ctx, 1e-6, 20000, &st);
@@ -1110,7 +1498,22 @@ extern "C" int pic_unet_train(pic_ctx* ctx, const pic_unet_train_desc* td,
             rc = r;
             break;
         }
-        k_d2f<<<ew(V), 256, 0, ctx->stream>>>(V, ctx->phi, targ + s * V);
+        float* xd = xs + s * Vc * 2;
+        float* td_ = ts + s * Vc;
+        (void)hipMemsetAsync(ss, 0, 3 * sizeof(double), ctx->stream);
+        if (u->coarsen == 2)
+            k_restrict_pair<2><<<ew(Vc), 256, 0, ctx->stream>>>(df, dc, ctx->ntype, dens, ctx->phi, xd, td_, ss);
+        else
+            k_restrict_pair<1><<<ew(Vc), 256, 0, ctx->stream>>>(df, dc, ctx->ntype, dens, ctx->phi, xd, td_, ss);
+        k_scale_pair<<<ew(Vc), 256, 0, ctx->stream>>>(Vc, ss, xd, td_);
+        double h2[2];
+        (void)hipMemcpyAsync(h2, ss, sizeof(h2), hipMemcpyDeviceToHost, ctx->stream);
+        e = hipStreamSynchronize(ctx->stream);
+        if (e != hipSuccess) {
+            rc = pic_hip_check(ctx, e, "training data");
+            break;
+        }
+        nvalid[s] = std::max(1.0, h2[1]);
     }
     (void)hipMemcpyAsync(ctx->phi, phi_keep, V * 8, hipMemcpyDeviceToDevice, ctx->stream);
     (void)hipMemcpyAsync(ctx->rho, rho_keep, V * 8, hipMemcpyDeviceToDevice, ctx->stream);
@@ -1130,12 +1533,12 @@ extern "C" int pic_unet_train(pic_ctx* ctx, const pic_unet_train_desc* td,
             size_t b1 = std::min(S, b0 + (size_t)td->batch);
             for (size_t q = b0; q < b1 && !rc; q++) {
                 size_t s = order[q];
-                k_make_input<<<ew(V), 256, 0, ctx->stream>>>(V, dens + s * V, ctx->ntype, u->x);
-                rc = unet_forward(ctx, u);
+                const float* xd = xs + s * Vc * 2;
+                rc = unet_forward(ctx, u, xd, nullptr);
                 if (rc) break;
-                k_loss<<<ew(V), 256, 0, ctx->stream>>>(V, u->y, targ + s * V, u->x, (float)(1.0 / count),
-                                                       u->dy, ctx->scratch);
-                rc = unet_backward(ctx, u);
+                k_loss<<<ew(Vc), 256, 0, ctx->stream>>>(Vc, u->y, ts + s * Vc, xd,
+                                                        (float)(1.0 / nvalid[s]), u->dy, ctx->scratch);
+                rc = unet_backward(ctx, u, xd);
             }
             if (!rc) rc = adam_step(ctx, u, (float)td->lr, 1.0f / (float)(b1 - b0));
             steps++;

@@ -40,12 +40,16 @@ ABI_FUNCTIONS = [
     "pic_mcc_create", "pic_mcc_apply", "pic_profile_enable", "pic_profile_read",
     "pic_profile_reset", "pic_unet_create", "pic_unet_param_count", "pic_unet_get_params",
     "pic_unet_set_params", "pic_unet_forward", "pic_unet_loss_grad", "pic_unet_adam_step",
-    "pic_unet_train",
+    "pic_unet_train", "pic_learned_precondition",
 ]
 
 
+UNET_MAX_LEVELS = 5
+
+
 class UNetDesc(C.Structure):
-    _fields_ = [("c1", C.c_int), ("c2", C.c_int), ("c3", C.c_int), ("seed", C.c_uint32)]
+    _fields_ = [("levels", C.c_int), ("channels", C.c_int * UNET_MAX_LEVELS), ("coarsen", C.c_int),
+                ("seed", C.c_uint32)]
 
 
 class UNetTrainDesc(C.Structure):
@@ -148,6 +152,7 @@ def load_library():
     L.pic_unet_loss_grad.argtypes = [P, _fp, _fp, C.POINTER(C.c_double), _fp]
     L.pic_unet_adam_step.argtypes = [P, _fp, C.c_float]
     L.pic_unet_train.argtypes = [P, C.POINTER(UNetTrainDesc), C.POINTER(UNetTrainStats)]
+    L.pic_learned_precondition.argtypes = [P, _dp, _dp]
     _lib = L
     return L
 
@@ -327,12 +332,17 @@ class Context:
                 "groups": st.groups, "wsv_max": st.wsv_max}
 
     # learned preconditioner
-    def unet_create(self, c1=16, c2=32, c3=64, seed=1):
-        d = UNetDesc(c1, c2, c3, seed)
+    def unet_create(self, channels=(16, 32, 64), seed=1, coarsen=1):
+        """channels: one entry per level (2..5 levels); the network runs on the
+        node grid divided by coarsen (1 or 2)"""
+        ch = (C.c_int * UNET_MAX_LEVELS)(*([int(c) for c in channels] +
+                                           [0] * (UNET_MAX_LEVELS - len(channels))))
+        d = UNetDesc(len(channels), ch, int(coarsen), seed)
         self._ck(self.L.pic_unet_create(self.h, C.byref(d)))
         n = C.c_size_t()
         self._ck(self.L.pic_unet_param_count(self.h, C.byref(n)))
         self.unet_nparam = n.value
+        self.unet_voxels = (self.ni // coarsen) * (self.nj // coarsen) * (self.nk // coarsen)
         return n.value
 
     def unet_get_params(self):
@@ -346,7 +356,7 @@ class Context:
 
     def unet_forward(self, x2):
         x2 = np.ascontiguousarray(x2, np.float32).reshape(-1)
-        out = np.zeros(self.nn, np.float32)
+        out = np.zeros(self.unet_voxels, np.float32)
         self._ck(self.L.pic_unet_forward(self.h, x2, out))
         return out
 
@@ -370,6 +380,13 @@ class Context:
         return {"loss_first": st.loss_first, "loss_last": st.loss_last,
                 "seconds_data": st.seconds_data, "seconds_train": st.seconds_train,
                 "steps": st.steps}
+
+    def learned_precondition(self, r):
+        """z = M(r), the learned solver's preconditioner on one vector"""
+        r = np.ascontiguousarray(r, np.float64)
+        z = np.zeros(self.nn)
+        self._ck(self.L.pic_learned_precondition(self.h, r, z))
+        return z
 
     # profiling
     def profile(self, on=True):

@@ -187,14 +187,21 @@ int pic_mcc_apply(pic_ctx* ctx, int mcc_id, double dt, uint32_t seed, uint32_t s
                   pic_mcc_stats* stats);
 
 /* ---- learned warm start / preconditioner (no counterpart in the reference) ----
- * A 3-level 3-D conv U-Net on the context's node grid (node counts divisible
- * by 4), BF16 weights on MFMA, FP32 accumulation.  Input channels per node:
- * {rho in grid units, Dirichlet mask}; output: the potential of the
- * unit-spacing problem laplacian(phi) = -rho with phi = 0 on Dirichlet nodes.
- * Host arrays are channels-last, node order i-fastest. */
+ * A 2..5-level 3-D conv U-Net, BF16 weights on MFMA, FP32 accumulation, on
+ * the node grid coarsened by `coarsen` (1: one voxel per node; 2: one voxel
+ * per 2x2x2 aggregate of nodes).  Node counts must be divisible by
+ * coarsen * 2^(levels-1).  Input channels per voxel: {charge density in grid
+ * units, fraction of non-regular (Dirichlet) nodes}; output: the potential
+ * of the unit-spacing problem laplacian(phi) = -rho, phi = 0 on Dirichlet
+ * nodes, averaged over the aggregate's regular nodes.  With coarsen 2 the
+ * network is the coarse solve of the learned solver's two-level
+ * preconditioner.  Host arrays are channels-last, voxel order x-fastest. */
+#define PIC_UNET_MAX_LEVELS 5
 typedef struct {
-    int c1, c2, c3;   /* channels per level: multiples of 16, c1 <= 32, c2, c3 <= 64 */
-    uint32_t seed;    /* weight initialisation */
+    int levels;                         /* 2..5 */
+    int channels[PIC_UNET_MAX_LEVELS];  /* multiples of 16: level 0 <= 32, others <= 64 */
+    int coarsen;                        /* 1 or 2 */
+    uint32_t seed;                      /* weight initialisation */
 } pic_unet_desc;
 typedef struct {
     int samples;          /* training pairs generated by the plain solver */
@@ -215,14 +222,19 @@ int pic_unet_create(pic_ctx* ctx, const pic_unet_desc* desc);
 int pic_unet_param_count(pic_ctx* ctx, size_t* n);
 int pic_unet_get_params(pic_ctx* ctx, float* dst, size_t n);
 int pic_unet_set_params(pic_ctx* ctx, const float* src, size_t n);
-/* in2: 2 floats per node; out: 1 float per node */
+/* in2: 2 floats per network voxel; out: 1 float per network voxel */
 int pic_unet_forward(pic_ctx* ctx, const float* in2, float* out);
-/* loss = 0.5 mean over non-Dirichlet nodes of (y - target)^2 and its gradient */
+/* loss = 0.5 mean over voxels holding a regular node (mask channel < 1) of
+ * (y - target)^2, and its gradient */
 int pic_unet_loss_grad(pic_ctx* ctx, const float* in2, const float* target, double* loss,
                        float* grad);
 int pic_unet_adam_step(pic_ctx* ctx, const float* grad, float lr);
 /* generate pairs with the plain solver on the device and train (Adam) */
 int pic_unet_train(pic_ctx* ctx, const pic_unet_train_desc* desc, pic_unet_train_stats* stats);
+/* z = M(r): the learned solver's preconditioner alone on host vectors of one
+ * double per node (a red-black Gauss-Seidel sweep from z = 0, the network's
+ * coarse correction of the remaining residual, a reverse-order sweep). */
+int pic_learned_precondition(pic_ctx* ctx, const double* r, double* z);
 
 /* ---- profiling: HIP events around every launch of the context ---- */
 #define PIC_K_PUSH 0

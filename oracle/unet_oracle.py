@@ -2,23 +2,34 @@
 backward), the checker for the GPU training and inference kernels.
 
 TEST INFRASTRUCTURE ONLY.  Layout and parameter order match the device
-library: activations channels-last with voxel order (z, y, x) (= node order
-i-fastest); each layer's weights are [cout][taps * cin] with k = tap * cin +
-ci and tap = (dz+1)*9 + (dy+1)*3 + (dx+1), followed by [cout] biases; layers
-in the order in, c1b, c2a, c2b, c3a, c3b, u2, c2c, u1, c1c, out.  The
-backward pass is pinned by central finite differences in the tests.
+library: activations channels-last with voxel order (z, y, x) (x fastest);
+each layer's weights are [cout][taps * cin] with k = tap * cin + ci and
+tap = (dz+1)*9 + (dy+1)*3 + (dx+1), followed by [cout] biases.  With channels
+C_0..C_{D-1} the layers are
+    in (2->C0), B0 (C0->C0),
+    A_l (C_{l-1}->C_l), B_l (C_l->C_l)          for l = 1..D-1,
+    U_l (1x1, C_{l+1}->C_l), Dec_l (C_l->C_l)   for l = D-2..0,
+    out (C0->1).
+The backward pass is pinned by central finite differences in the tests.
 """
 import numpy as np
 
 
-def layer_specs(c1, c2, c3):
-    return [(2, c1, 27), (c1, c1, 27), (c1, c2, 27), (c2, c2, 27), (c2, c3, 27), (c3, c3, 27),
-            (c3, c2, 1), (c2, c2, 27), (c2, c1, 1), (c1, c1, 27), (c1, 1, 27)]
+def layer_specs(channels):
+    C = list(channels)
+    D = len(C)
+    s = [(2, C[0], 27), (C[0], C[0], 27)]
+    for l in range(1, D):
+        s += [(C[l - 1], C[l], 27), (C[l], C[l], 27)]
+    for l in range(D - 2, -1, -1):
+        s += [(C[l + 1], C[l], 1), (C[l], C[l], 27)]
+    s.append((C[0], 1, 27))
+    return s
 
 
-def unpack(params, c1, c2, c3):
+def unpack(params, channels):
     out, off = [], 0
-    for cin, cout, taps in layer_specs(c1, c2, c3):
+    for cin, cout, taps in layer_specs(channels):
         nw = cout * taps * cin
         W = params[off:off + nw].reshape(cout, taps * cin)
         off += nw
@@ -29,8 +40,8 @@ def unpack(params, c1, c2, c3):
     return out
 
 
-def n_params(c1, c2, c3):
-    return sum(co * t * ci + co for ci, co, t in layer_specs(c1, c2, c3))
+def n_params(channels):
+    return sum(co * t * ci + co for ci, co, t in layer_specs(channels))
 
 
 def conv(x, W, b, taps):
@@ -83,82 +94,163 @@ def up_bwd(df):
 relu = lambda a: np.maximum(a, 0.0)
 
 
-def forward(params, x, c1, c2, c3, keep=False):
+def _index(channels):
+    """layer indices: in, B[l], A[l], U[l], Dec[l], out"""
+    D = len(channels)
+    idx = {"in": 0, "B": {0: 1}, "A": {}, "U": {}, "Dec": {}}
+    k = 2
+    for l in range(1, D):
+        idx["A"][l], idx["B"][l] = k, k + 1
+        k += 2
+    for l in range(D - 2, -1, -1):
+        idx["U"][l], idx["Dec"][l] = k, k + 1
+        k += 2
+    idx["out"] = k
+    return idx
+
+
+def forward(params, x, channels, keep=False):
     """x: (nz, ny, nx, 2) -> y (nz, ny, nx)"""
-    L = unpack(params, c1, c2, c3)
-    a = {}
-    a["x"] = x
-    a["a1a"] = relu(conv(x, *L[0]))
-    a["s1"] = relu(conv(a["a1a"], *L[1]))
-    a["p1"] = pool(a["s1"])
-    a["a2a"] = relu(conv(a["p1"], *L[2]))
-    a["s2"] = relu(conv(a["a2a"], *L[3]))
-    a["p2"] = pool(a["s2"])
-    a["a3a"] = relu(conv(a["p2"], *L[4]))
-    a["a3b"] = relu(conv(a["a3a"], *L[5]))
-    a["u2"] = conv(a["a3b"], *L[6])
-    a["a2"] = up(a["u2"]) + a["s2"]
-    a["c2c"] = relu(conv(a["a2"], *L[7]))
-    a["u1"] = conv(a["c2c"], *L[8])
-    a["a1"] = up(a["u1"]) + a["s1"]
-    a["c1c"] = relu(conv(a["a1"], *L[9]))
-    y = conv(a["c1c"], *L[10])[..., 0]
+    L = unpack(params, channels)
+    I = _index(channels)
+    D = len(channels)
+    a = {"x": x}
+    a["a0"] = relu(conv(x, *L[I["in"]]))
+    a["s0"] = relu(conv(a["a0"], *L[I["B"][0]]))
+    for l in range(1, D):
+        a["p%d" % l] = pool(a["s%d" % (l - 1)])
+        a["a%d" % l] = relu(conv(a["p%d" % l], *L[I["A"][l]]))
+        a["s%d" % l] = relu(conv(a["a%d" % l], *L[I["B"][l]]))
+    for l in range(D - 2, -1, -1):
+        src = a["s%d" % (D - 1)] if l == D - 2 else a["h%d" % (l + 1)]
+        a["u%d" % l] = conv(src, *L[I["U"][l]])
+        a["m%d" % l] = up(a["u%d" % l]) + a["s%d" % l]
+        a["h%d" % l] = relu(conv(a["m%d" % l], *L[I["Dec"][l]]))
+    y = conv(a["h0"], *L[I["out"]])[..., 0]
     return (y, a) if keep else y
 
 
-def loss_grad(params, x, target, c1, c2, c3):
-    """0.5 mean over non-Dirichlet nodes (mask channel <= 0.5) of (y - t)^2"""
-    L = unpack(params, c1, c2, c3)
-    y, a = forward(params, x, c1, c2, c3, keep=True)
-    valid = (x[..., 1] <= 0.5).astype(np.float64)
+def loss_grad(params, x, target, channels):
+    """0.5 mean over voxels holding a regular node (mask channel < 1) of (y - t)^2"""
+    L = unpack(params, channels)
+    I = _index(channels)
+    D = len(channels)
+    y, a = forward(params, x, channels, keep=True)
+    valid = (x[..., 1] < 0.999).astype(np.float64)
     nv = max(valid.sum(), 1.0)
     r = (y - target) * valid
     loss = 0.5 * np.sum(r * r) / nv
     grads = [None] * len(L)
     dy = (r / nv)[..., None]
-    d, gw, gb = conv_bwd(a["c1c"], L[10][0], 27, dy)
-    grads[10] = (gw, gb)
-    d = d * (a["c1c"] > 0)
-    d, gw, gb = conv_bwd(a["a1"], L[9][0], 27, d)
-    grads[9] = (gw, gb)
-    g_s1 = d.copy()
-    d_u1 = up_bwd(d)
-    d, gw, gb = conv_bwd(a["c2c"], L[8][0], 1, d_u1)
-    grads[8] = (gw, gb)
-    d = d * (a["c2c"] > 0)
-    d, gw, gb = conv_bwd(a["a2"], L[7][0], 27, d)
-    grads[7] = (gw, gb)
-    g_s2 = d.copy()
-    d_u2 = up_bwd(d)
-    d, gw, gb = conv_bwd(a["a3b"], L[6][0], 1, d_u2)
-    grads[6] = (gw, gb)
-    d = d * (a["a3b"] > 0)
-    d, gw, gb = conv_bwd(a["a3a"], L[5][0], 27, d)
-    grads[5] = (gw, gb)
-    d = d * (a["a3a"] > 0)
-    d, gw, gb = conv_bwd(a["p2"], L[4][0], 27, d)
-    grads[4] = (gw, gb)
-    g_s2 = (g_s2 + pool_bwd(d)) * (a["s2"] > 0)
-    d, gw, gb = conv_bwd(a["a2a"], L[3][0], 27, g_s2)
-    grads[3] = (gw, gb)
-    d = d * (a["a2a"] > 0)
-    d, gw, gb = conv_bwd(a["p1"], L[2][0], 27, d)
-    grads[2] = (gw, gb)
-    g_s1 = (g_s1 + pool_bwd(d)) * (a["s1"] > 0)
-    d, gw, gb = conv_bwd(a["a1a"], L[1][0], 27, g_s1)
-    grads[1] = (gw, gb)
-    d = d * (a["a1a"] > 0)
-    _, gw, gb = conv_bwd(a["x"], L[0][0], 27, d)
-    grads[0] = (gw, gb)
+    d, gw, gb = conv_bwd(a["h0"], L[I["out"]][0], 27, dy)
+    grads[I["out"]] = (gw, gb)
+    gh = d * (a["h0"] > 0)  # at h0's pre-activation
+    gm = {}
+    for l in range(0, D - 1):
+        d, gw, gb = conv_bwd(a["m%d" % l], L[I["Dec"][l]][0], 27, gh)
+        grads[I["Dec"][l]] = (gw, gb)
+        gm[l] = d
+        gu = up_bwd(d)
+        src = a["s%d" % (D - 1)] if l == D - 2 else a["h%d" % (l + 1)]
+        d, gw, gb = conv_bwd(src, L[I["U"][l]][0], 1, gu)
+        grads[I["U"][l]] = (gw, gb)
+        gh = d * (src > 0)  # at src's pre-activation
+    gs = gh  # at s_{D-1}'s pre-activation
+    gp = None
+    for l in range(D - 1, 0, -1):
+        if l < D - 1:
+            gs = (gm[l] + pool_bwd(gp)) * (a["s%d" % l] > 0)
+        d, gw, gb = conv_bwd(a["a%d" % l], L[I["B"][l]][0], 27, gs)
+        grads[I["B"][l]] = (gw, gb)
+        ga = d * (a["a%d" % l] > 0)
+        gp, gw, gb = conv_bwd(a["p%d" % l], L[I["A"][l]][0], 27, ga)
+        grads[I["A"][l]] = (gw, gb)
+    gs = (gm[0] + pool_bwd(gp)) * (a["s0"] > 0)
+    d, gw, gb = conv_bwd(a["a0"], L[I["B"][0]][0], 27, gs)
+    grads[I["B"][0]] = (gw, gb)
+    ga = d * (a["a0"] > 0)
+    _, gw, gb = conv_bwd(a["x"], L[I["in"]][0], 27, ga)
+    grads[I["in"]] = (gw, gb)
     flat = np.concatenate([np.concatenate([gw.ravel(), gb.ravel()]) for gw, gb in grads])
     return loss, flat
 
 
-def init_params(c1, c2, c3, seed):
+def init_params(channels, seed):
     rng = np.random.default_rng(seed)
     parts = []
-    for cin, cout, taps in layer_specs(c1, c2, c3):
+    for cin, cout, taps in layer_specs(channels):
         s = np.sqrt(6.0 / (taps * cin))
         parts.append(rng.uniform(-s, s, cout * taps * cin))
         parts.append(rng.uniform(-0.1, 0.1, cout))
     return np.concatenate(parts)
+
+
+# ---- the two-level preconditioner's grid transfers (aggregates of 2x2x2 nodes)
+
+def restrict_mean(f, regular):
+    """mean over each aggregate's regular nodes; arrays (nz, ny, nx)"""
+    nz, ny, nx = f.shape
+    r = regular.astype(np.float64)
+    s = (f * r).reshape(nz // 2, 2, ny // 2, 2, nx // 2, 2).sum(axis=(1, 3, 5))
+    n = r.reshape(nz // 2, 2, ny // 2, 2, nx // 2, 2).sum(axis=(1, 3, 5))
+    return np.where(n > 0, s / np.maximum(n, 1), 0.0), 1.0 - n / 8.0
+
+
+def prolong_trilinear(y):
+    """cell-centred trilinear interpolation from aggregate centres: per axis,
+    weight 3/4 on the node's aggregate and 1/4 on the neighbour on its side
+    (the aggregate itself at the grid's ends)"""
+    out = y
+    for ax in range(3):
+        n = out.shape[ax]
+        idx = np.arange(2 * n)
+        c = idx // 2
+        nb = np.clip(c + np.where(idx % 2 == 1, 1, -1), 0, n - 1)
+        out = 0.75 * np.take(out, c, axis=ax) + 0.25 * np.take(out, nb, axis=ax)
+    return out
+
+
+def precondition(r, regular, h2, params, channels, coarsen):
+    """z = M(r) of the learned solver on a Dirichlet-walled uniform grid;
+    arrays (nz, ny, nx).  One red-black Gauss-Seidel sweep on A z = r from 0
+    (A = 7-point Laplacian / h2 on regular nodes; red = (i+j+k) even), the
+    network's coarse correction c of the remaining residual res = r - A z,
+    added as z += omega c with the A-norm optimal omega = (c.res)/(c.A c),
+    one sweep in the reverse colour order.  Returns (z, omega c)."""
+    nz, ny, nx = r.shape
+    k, j, i = np.meshgrid(np.arange(nz), np.arange(ny), np.arange(nx), indexing="ij")
+    par = (i + j + k) % 2
+    z = np.zeros_like(r, dtype=np.float64)
+
+    def nbr_sum(v):
+        s = np.zeros_like(v)
+        s[1:-1, 1:-1, 1:-1] = (v[:-2, 1:-1, 1:-1] + v[2:, 1:-1, 1:-1] + v[1:-1, :-2, 1:-1] +
+                               v[1:-1, 2:, 1:-1] + v[1:-1, 1:-1, :-2] + v[1:-1, 1:-1, 2:])
+        return s
+
+    def sweep(color):
+        m = regular & (par == color)
+        z[m] = (nbr_sum(z)[m] - h2 * r[m]) / 6.0
+
+    sweep(0)
+    sweep(1)
+    Az = (nbr_sum(z) - 6.0 * z) / h2
+    res = np.where(regular, r - Az, 0.0)
+    if coarsen == 2:
+        xin, mask = restrict_mean(-h2 * res, regular)
+    else:
+        xin, mask = np.where(regular, -h2 * res, 0.0), 1.0 - regular
+    valid = mask < 0.999
+    s = np.sqrt(np.mean(xin[valid] ** 2))
+    x = np.stack([xin / s, mask], axis=-1)
+    y = forward(params, x, channels)
+    c = prolong_trilinear(y) if coarsen == 2 else y
+    c = np.where(regular, c, 0.0)
+    Ac = np.where(regular, (nbr_sum(c) - 6.0 * c) / h2, 0.0)
+    den = np.sum(c * Ac)
+    omega = np.sum(c * res) / den if den != 0 else 0.0
+    corr = omega * c
+    z += corr
+    sweep(1)
+    sweep(0)
+    return z, corr

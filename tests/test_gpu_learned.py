@@ -1,6 +1,8 @@
 """The learned preconditioner end to end on the device: train on pairs made by
-the plain solver, then solve a new problem with flexible PCG and compare
-with the plain method.  The final residual is checked by the FP64 oracle."""
+the plain solver, then solve a new problem with flexible CG and compare with
+the same two-level cycle without a network (a zero network: a symmetric
+Gauss-Seidel preconditioner) and with plain PCG.  The final residual is
+checked by the FP64 oracle."""
 import numpy as np
 import pytest
 
@@ -23,34 +25,43 @@ def _blob_density(n, seed):
     return 1e-8 * rho
 
 
-def test_learned_preconditioner_trains_and_beats_plain_pcg():
+def _solve(ctx, pg, method, rho, g):
+    ctx.set_field(pg.FIELD_RHO, rho)
+    ctx.set_field(pg.FIELD_PHI, np.zeros(ctx.nn))
+    s = ctx.poisson(method=method, tol=1e-6, tol_kind=pg.TOL_REL, max_it=5000)
+    assert s["converged"] and s["rel_residual"] <= 1e-6, s
+    phi = ctx.get_field(pg.FIELD_PHI)
+    r2, b2 = O.residual_dirichlet(g, phi, rho)
+    assert r2 / b2 <= 1.05e-6
+    return s, phi
+
+
+@pytest.mark.parametrize("coarsen", [2, 1])
+def test_learned_preconditioner_trains_and_beats_untrained_cycle(coarsen):
     import pic_gpu as pg
 
     n = 32
     L = 0.01 * (n - 1)
+    print("learned solver, coarsen", coarsen, flush=True)
     ctx = pg.Context(n, n, n, [0, 0, 0], [L, L, L])
     ctx.dirichlet_walls()
-    ctx.unet_create(16, 16, 32, seed=3)
+    g = O.make_grid(n, n, n, [0, 0, 0], [L, L, L])
+    rho = _blob_density(n, 42)
+    ctx.unet_create((16, 16, 16), seed=3, coarsen=coarsen)
     st = ctx.unet_train(samples=128, epochs=20, batch=4, lr=2e-3, seed=5, blobs=(1, 6),
                         sigma=(2.0, 8.0), noise=0.05)
     assert st["loss_last"] < 0.2 * st["loss_first"], st
-    rho = _blob_density(n, 42)
-    ctx.set_field(pg.FIELD_RHO, rho)
-    g = O.make_grid(n, n, n, [0, 0, 0], [L, L, L])
-    results = {}
-    for name, method in (("plain", pg.SOLVER_PCG), ("learned", pg.SOLVER_LEARNED)):
-        ctx.set_field(pg.FIELD_PHI, np.zeros(ctx.nn))
-        s = ctx.poisson(method=method, tol=1e-6, tol_kind=pg.TOL_REL, max_it=5000)
-        assert s["converged"] and s["rel_residual"] <= 1e-6, (name, s)
-        phi = ctx.get_field(pg.FIELD_PHI)
-        r2, b2 = O.residual_dirichlet(g, phi, rho)
-        assert r2 / b2 <= 1.05e-6, name
-        results[name] = (s, phi)
-    it_plain = results["plain"][0]["iterations"]
-    it_learned = results["learned"][0]["iterations"]
-    assert it_learned < 0.5 * it_plain, (it_learned, it_plain)
-    a, b = results["plain"][1], results["learned"][1]
-    assert np.max(np.abs(a - b)) <= 1e-4 * np.max(np.abs(a))
+    s_learned, phi_l = _solve(ctx, pg, pg.SOLVER_LEARNED, rho, g)
+    trained = ctx.unet_get_params()
+    ctx.unet_set_params(np.zeros_like(trained))
+    s_zero, _ = _solve(ctx, pg, pg.SOLVER_LEARNED, rho, g)
+    ctx.unet_set_params(trained)
+    s_plain, phi_p = _solve(ctx, pg, pg.SOLVER_PCG, rho, g)
+    print("iterations: plain PCG %d, zero network %d, trained %d" % (
+        s_plain["iterations"], s_zero["iterations"], s_learned["iterations"]), flush=True)
+    assert s_learned["iterations"] < 0.7 * s_zero["iterations"]
+    assert s_learned["iterations"] < 0.3 * s_plain["iterations"]
+    assert np.max(np.abs(phi_l - phi_p)) <= 1e-4 * np.max(np.abs(phi_p))
 
 
 def test_learned_solver_refuses_without_network():
