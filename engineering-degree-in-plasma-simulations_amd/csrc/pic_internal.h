@@ -232,3 +232,24 @@ inline unsigned grid_for(size_t n, unsigned block) {
     size_t g = (n + block - 1) / block;
     return (unsigned)(g ? g : 1);
 }
+
+// XCD-contiguous chunks for grid sweeps: workgroups are dealt round-robin
+// over the 8 XCDs, so workgroup b takes logical chunk (b % 8) * (nb / 8) + b / 8
+// and each XCD walks one contiguous slab (its L2 then serves the +-k
+// neighbour planes).  The grid size must be a multiple of 8 (xcd_blocks).
+constexpr int XCD_COUNT = 8;
+inline unsigned xcd_blocks(long long items, int block) {
+    long long b = (items + block * 8 - 1) / (block * 8);
+    if (b > 2048) b = 2048;
+    b = (b + XCD_COUNT - 1) / XCD_COUNT * XCD_COUNT;
+    return (unsigned)b;
+}
+__device__ inline long long xcd_chunk(long long n, int block, long long& end) {
+    int nb = gridDim.x;
+    int b = blockIdx.x;
+    int lb = (b % XCD_COUNT) * (nb / XCD_COUNT) + b / XCD_COUNT;
+    long long chunk = ((n + nb - 1) / nb + block - 1) / block * block;
+    long long beg = (long long)lb * chunk;
+    end = beg + chunk < n ? beg + chunk : n;
+    return beg;
+}

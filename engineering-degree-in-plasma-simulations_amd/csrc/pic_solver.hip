@@ -692,7 +692,7 @@ static int solve_pcg(pic_ctx* ctx, const pic_solver_params* p, pic_solver_stats*
 // residual, one sweep in the reverse colour order (black, red).
 // ------------------------------------------------------------------------
 bool pic_unet_ready(pic_ctx* ctx);
-int pic_unet_coarse_correct(pic_ctx* ctx, const double* t, double* z, double* c);
+int pic_unet_coarse_correct(pic_ctx* ctx, const double* t, const double* z, double* c, double* dots);
 
 namespace {
 
@@ -840,18 +840,64 @@ __global__ void __launch_bounds__(BLOCK)
     }
 }
 
+// the coarse correction fused with the next (black) half-sweep:
+// z = zin + omega c on red nodes, the Gauss-Seidel update from those values on
+// black regular nodes, 0 on non-regular nodes; omega = d[0] / d[1] (A-norm
+// optimal step along c).  zin and c vanish on non-regular nodes.
+__global__ void __launch_bounds__(BLOCK)
+    k_gs_corr(Stencil s, const uint8_t* __restrict__ nt, const double* __restrict__ r,
+              const double* __restrict__ zin, const double* __restrict__ c,
+              const double* __restrict__ d, double* __restrict__ z) {
+    const double om = d[1] != 0.0 ? d[0] / d[1] : 0.0;
+    long long end;
+    for (long long u = chunk_begin(s.nn, end) + threadIdx.x; u < end; u += BLOCK) {
+        if (nt[u] != NT_REG) {
+            z[u] = 0.0;
+            continue;
+        }
+        int i = (int)(u % s.ni);
+        long long row = u / s.ni;
+        int j = (int)(row % s.nj), k = (int)(row / s.nj);
+        if (((i + j + k) & 1) == 0) {
+            z[u] = zin[u] + om * c[u];
+        } else {
+            auto zc = [&](long long w) { return zin[w] + om * c[w]; };
+            z[u] = ((zc(u - 1) + zc(u + 1)) * s.cx + (zc(u - s.sj) + zc(u + s.sj)) * s.cy +
+                    (zc(u - s.sk) + zc(u + s.sk)) * s.cz - r[u]) /
+                   s.twos;
+        }
+    }
+}
+
 }  // namespace
 
+static void presmooth(pic_ctx* ctx, const Stencil& s, const double* r, double* z) {
+    long long half_items = (long long)((s.ni + 1) / 2) * s.nj * s.nk;
+    k_gs_first<<<stencil_blocks(s.nn), BLOCK, 0, ctx->stream>>>(s, ctx->ntype, r, z);
+    k_gs<<<stencil_blocks(half_items), BLOCK, 0, ctx->stream>>>(s, ctx->ntype, r, z, 1);
+}
+
+// z = one red-black Gauss-Seidel sweep on A z = r from z = 0 (device vectors):
+// the learned preconditioner's pre-smoother, also used for its training pairs
+int pic_learned_presmooth(pic_ctx* ctx, const double* r, double* z) {
+    presmooth(ctx, make_stencil(ctx->g, false), r, z);
+    PIC_HIP(ctx, hipGetLastError());
+    return PIC_OK;
+}
+
+// z = M(r); tmp holds two scratch vectors
 static int learned_precond(pic_ctx* ctx, const Stencil& s, const double* rin, double* zout,
                            double* tmp) {
     unsigned nb = stencil_blocks(s.nn);
     long long half_items = (long long)((s.ni + 1) / 2) * s.nj * s.nk;
     unsigned nbh = stencil_blocks(half_items);
-    k_gs_first<<<nb, BLOCK, 0, ctx->stream>>>(s, ctx->ntype, rin, zout);
-    k_gs<<<nbh, BLOCK, 0, ctx->stream>>>(s, ctx->ntype, rin, zout, 1);
-    int e = pic_unet_coarse_correct(ctx, rin, zout, tmp);
+    double* zpre = tmp;
+    double* c = tmp + s.nn;
+    double* dots = ctx->scratch + 66;
+    presmooth(ctx, s, rin, zpre);
+    int e = pic_unet_coarse_correct(ctx, rin, zpre, c, dots);
     if (e) return e;
-    k_gs<<<nbh, BLOCK, 0, ctx->stream>>>(s, ctx->ntype, rin, zout, 1);
+    k_gs_corr<<<nb, BLOCK, 0, ctx->stream>>>(s, ctx->ntype, rin, zpre, c, dots, zout);
     k_gs<<<nbh, BLOCK, 0, ctx->stream>>>(s, ctx->ntype, rin, zout, 0);
     PIC_HIP(ctx, hipGetLastError());
     return PIC_OK;
@@ -873,7 +919,7 @@ extern "C" int pic_learned_precondition(pic_ctx* ctx, const double* r, double* z
     rc = learned_check(ctx);
     if (rc) return rc;
     size_t nn = (size_t)ctx->g.nn;
-    rc = pic_ensure_work(ctx, 3 * nn * sizeof(double));
+    rc = pic_ensure_work(ctx, 4 * nn * sizeof(double));
     if (rc) return rc;
     double* rd = ctx->work;
     double* zd = rd + nn;
@@ -892,7 +938,7 @@ static int solve_learned(pic_ctx* ctx, const pic_solver_params* p, pic_solver_st
     const GridC& g = ctx->g;
     size_t nn = (size_t)g.nn;
     Stencil s = make_stencil(g, false);
-    rc = pic_ensure_work(ctx, 6 * nn * sizeof(double));
+    rc = pic_ensure_work(ctx, 7 * nn * sizeof(double));
     if (rc) return rc;
     double* r = ctx->work;
     double* z = r + nn;

@@ -98,7 +98,6 @@ __device__ inline void vox_xyz(const Dims& d, long long v, int& x, int& y, int& 
 
 // ---------------------------------------------------------------- MFMA conv
 constexpr int TX = 16, TY = 4, TZ = 4;
-constexpr int NBB = 4;  // N-blocks (16 output channels each) per workgroup
 
 struct ConvArgs {
     const bf16* in;
@@ -112,24 +111,28 @@ struct ConvArgs {
     Dims d;
 };
 
-template <int TAPS>
+// NB N-blocks of 16 output channels per workgroup (cout/16 divisible by NB).
+// Per 16-channel input chunk the workgroup stages the input tile (+ halo) and
+// the chunk's weights for its NB*16 outputs in LDS; the tap loop then issues
+// only LDS reads and MFMAs.
+template <int TAPS, int NB>
 __global__ void __launch_bounds__(256) k_conv_mfma(ConvArgs a) {
     constexpr int H = TAPS == 27 ? 1 : 0;
     constexpr int SX = TX + 2 * H, SY = TY + 2 * H, SZ = TZ + 2 * H;
     __shared__ uint4 tile[SX * SY * SZ * 2];  // 16 BF16 channels = 32 bytes per voxel
+    __shared__ uint4 wl[NB * TAPS * 16 * 2];  // rows (nb*TAPS + t)*16 + n, 16 BF16 inputs each
     const Dims d = a.d;
     const int ntx = (d.nx + TX - 1) / TX, nty = (d.ny + TY - 1) / TY;
     const int b = blockIdx.x;
     const int x0 = (b % ntx) * TX, y0 = ((b / ntx) % nty) * TY, z0 = (b / (ntx * nty)) * TZ;
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
     const int g = lane >> 4, m = lane & 15;
-    const int nb0 = blockIdx.y * NBB;
-    const int nbc = min(NBB, a.cout / 16 - nb0);
-    f4v acc[4][NBB];
+    const int nb0 = blockIdx.y * NB;
+    f4v acc[4][NB];
 #pragma unroll
     for (int mb = 0; mb < 4; mb++)
 #pragma unroll
-        for (int nb = 0; nb < NBB; nb++) acc[mb][nb] = f4v{0.f, 0.f, 0.f, 0.f};
+        for (int nb = 0; nb < NB; nb++) acc[mb][nb] = f4v{0.f, 0.f, 0.f, 0.f};
     for (int c0 = 0; c0 < a.cin; c0 += 16) {
         __syncthreads();
         for (int s = tid; s < SX * SY * SZ * 2; s += 256) {
@@ -142,19 +145,22 @@ __global__ void __launch_bounds__(256) k_conv_mfma(ConvArgs a) {
                     a.in + (((long long)gz * d.ny + gy) * d.nx + gx) * a.cin + c0 + hf * 8);
             tile[s] = val;
         }
+        for (int s = tid; s < NB * TAPS * 16 * 2; s += 256) {
+            int hf = s & 1, row = s >> 1;
+            int n16 = row % 16, t = (row / 16) % TAPS, nb = row / (16 * TAPS);
+            int n = (nb0 + nb) * 16 + n16;
+            wl[s] = *reinterpret_cast<const uint4*>(a.w + (long long)n * TAPS * a.cin + t * a.cin + c0 +
+                                                    hf * 8);
+        }
         __syncthreads();
         for (int t = 0; t < TAPS; t++) {
             const int ox = TAPS == 27 ? t % 3 : 0, oy = TAPS == 27 ? (t / 3) % 3 : 0,
                       oz = TAPS == 27 ? t / 9 : 0;
-            s4v bfr[NBB];
+            s4v bfr[NB];
 #pragma unroll
-            for (int nb = 0; nb < NBB; nb++) {
-                if (nb < nbc) {
-                    int n = (nb0 + nb) * 16 + m;
-                    bfr[nb] = *reinterpret_cast<const s4v*>(a.w + (long long)n * TAPS * a.cin +
-                                                            t * a.cin + c0 + 4 * g);
-                }
-            }
+            for (int nb = 0; nb < NB; nb++)
+                bfr[nb] = *reinterpret_cast<const s4v*>(reinterpret_cast<const char*>(wl) +
+                                                        ((nb * TAPS + t) * 16 + m) * 32 + 8 * g);
 #pragma unroll
             for (int mb = 0; mb < 4; mb++) {
                 int row = wave * 4 + mb;
@@ -163,10 +169,9 @@ __global__ void __launch_bounds__(256) k_conv_mfma(ConvArgs a) {
                 s4v afr = *reinterpret_cast<const s4v*>(reinterpret_cast<const char*>(tile) +
                                                         sv * 32 + 8 * g);
 #pragma unroll
-                for (int nb = 0; nb < NBB; nb++)
-                    if (nb < nbc)
-                        acc[mb][nb] =
-                            __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(afr, bfr[nb], acc[mb][nb], 0, 0, 0);
+                for (int nb = 0; nb < NB; nb++)
+                    acc[mb][nb] =
+                        __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(afr, bfr[nb], acc[mb][nb], 0, 0, 0);
             }
         }
     }
@@ -177,8 +182,7 @@ __global__ void __launch_bounds__(256) k_conv_mfma(ConvArgs a) {
         int gy = y0 + row % TY, gz = z0 + row / TY;
         if (gy >= d.ny || gz >= d.nz) continue;
 #pragma unroll
-        for (int nb = 0; nb < NBB; nb++) {
-            if (nb >= nbc) continue;
+        for (int nb = 0; nb < NB; nb++) {
             int n = (nb0 + nb) * 16 + m;
             float bv = a.bias ? a.bias[n] : 0.f;
 #pragma unroll
@@ -680,58 +684,54 @@ __device__ inline void block_sum2(double& a, double& b) {
 // ------------------------------------------- fine grid <-> network grid
 // The node (i, j, k) belongs to aggregate (i/CF, j/CF, k/CF).
 
-// training pair: x2[v] = {mean over regular nodes of a, fraction of other
-// nodes}, tgt[v] = mean over regular nodes of phi; sums += {x^2, 1, tgt^2}
-// over aggregates holding a regular node
+// training target: tgt[v] = mean of phi over the aggregate's regular nodes;
+// sums[2] += tgt^2 over aggregates holding a regular node
 template <int CF>
 __global__ void __launch_bounds__(256)
-    k_restrict_pair(Dims df, Dims dc, const uint8_t* __restrict__ nt, const float* __restrict__ a,
-                    const double* __restrict__ phi, float* __restrict__ x2, float* __restrict__ tgt,
-                    double* __restrict__ sums) {
-    long long v = (long long)blockIdx.x * 256 + threadIdx.x;
-    double acc[3] = {0, 0, 0};
-    if (v < dc.v) {
+    k_restrict_target(Dims df, Dims dc, const uint8_t* __restrict__ nt,
+                      const double* __restrict__ phi, float* __restrict__ tgt,
+                      double* __restrict__ sums) {
+    double acc = 0;
+    long long end;
+    for (long long v = xcd_chunk(dc.v, 256, end) + threadIdx.x; v < end; v += 256) {
         int cx, cy, cz;
         vox_xyz(dc, v, cx, cy, cz);
-        double sa = 0, sp = 0;
+        double sp = 0;
         int nreg = 0;
 #pragma unroll
         for (int q = 0; q < CF * CF * CF; q++) {
             int fx = CF * cx + q % CF, fy = CF * cy + (q / CF) % CF, fz = CF * cz + q / (CF * CF);
             long long u = ((long long)fz * df.ny + fy) * df.nx + fx;
             if (nt[u] == NT_REG) {
-                sa += a[u];
                 sp += phi[u];
                 nreg++;
             }
         }
-        float xv = nreg ? (float)(sa / nreg) : 0.f;
-        x2[2 * v] = xv;
-        x2[2 * v + 1] = 1.f - (float)nreg / (float)(CF * CF * CF);
         float tv = nreg ? (float)(sp / nreg) : 0.f;
         tgt[v] = tv;
-        if (nreg) {
-            acc[0] = (double)xv * xv;
-            acc[1] = 1;
-            acc[2] = (double)tv * tv;
-        }
+        acc += (double)tv * tv;
     }
-    block_sums<3>(acc);
-    if (threadIdx.x == 0)
-        for (int q = 0; q < 3; q++) atomicAdd(&sums[q], acc[q]);
+    double dummy = 0;
+    block_sum2(acc, dummy);
+    if (threadIdx.x == 0) atomicAdd(&sums[2], acc);
 }
 
-// input and target to unit RMS over the valid aggregates (the preconditioner
-// fixes the correction's scale by a line search, so only its shape is learnt)
+// input to unit RMS over the valid aggregates, the target by the same factor
+// (the map stays linear in the input's scale)
 __global__ void __launch_bounds__(256)
     k_scale_pair(long long n, const double* __restrict__ sums, float* __restrict__ x2,
                  float* __restrict__ tgt) {
     long long v = (long long)blockIdx.x * 256 + threadIdx.x;
     if (v >= n) return;
     double sx = sums[0] > 0 && sums[1] > 0 ? sqrt(sums[0] / sums[1]) : 1.0;
-    double st = sums[2] > 0 && sums[1] > 0 ? sqrt(sums[2] / sums[1]) : 1.0;
-    x2[2 * v] *= (float)(1.0 / sx);
-    tgt[v] *= (float)(1.0 / st);
+    float f = (float)(1.0 / sx);
+    x2[2 * v] *= f;
+    tgt[v] *= f;
+}
+
+__global__ void __launch_bounds__(256) k_scale_f(long long n, float f, float* __restrict__ a) {
+    long long v = (long long)blockIdx.x * 256 + threadIdx.x;
+    if (v < n) a[v] *= f;
 }
 
 // preconditioner input from the fine residual of A e = t after smoothing z:
@@ -743,11 +743,11 @@ __global__ void __launch_bounds__(256)
     k_res_restrict(Dims df, Dims dc, double h2, const uint8_t* __restrict__ nt,
                    const double* __restrict__ t, const double* __restrict__ z, float* __restrict__ x2,
                    double* __restrict__ sums) {
-    long long v = (long long)blockIdx.x * 256 + threadIdx.x;
     double s2 = 0, cnt = 0;
     const long long sj = df.nx, sk = (long long)df.nx * df.ny;
     const double ih2 = 1.0 / h2;
-    if (v < dc.v) {
+    long long end;
+    for (long long v = xcd_chunk(dc.v, 256, end) + threadIdx.x; v < end; v += 256) {
         int cx, cy, cz;
         vox_xyz(dc, v, cx, cy, cz);
         double acc = 0;
@@ -767,8 +767,8 @@ __global__ void __launch_bounds__(256)
         x2[2 * v] = xv;
         x2[2 * v + 1] = 1.f - (float)nreg / (float)(CF * CF * CF);
         if (nreg) {
-            s2 = (double)xv * xv;
-            cnt = 1;
+            s2 += (double)xv * xv;
+            cnt += 1;
         }
     }
     block_sum2(s2, cnt);
@@ -785,11 +785,11 @@ template <int CF>
 __global__ void __launch_bounds__(256)
     k_prolong(Dims df, Dims dc, const uint8_t* __restrict__ nt, const float* __restrict__ y,
               double* __restrict__ c) {
-    long long u = (long long)blockIdx.x * 256 + threadIdx.x;
-    if (u >= df.v) return;
+    long long end;
+    for (long long u = xcd_chunk(df.v, 256, end) + threadIdx.x; u < end; u += 256) {
     if (nt[u] != NT_REG) {
         c[u] = 0.0;
-        return;
+        continue;
     }
     int i, j, k;
     vox_xyz(df, u, i, j, k);
@@ -815,6 +815,7 @@ __global__ void __launch_bounds__(256)
         }
     }
     c[u] = (double)val;
+    }
 }
 
 // line search of the coarse correction c: with res = t - A z,
@@ -824,8 +825,8 @@ __global__ void __launch_bounds__(256)
             const double* __restrict__ z, const double* __restrict__ c, double* __restrict__ out) {
     const long long sj = df.nx, sk = (long long)df.nx * df.ny;
     double acc[2] = {0, 0};
-    for (long long u = (long long)blockIdx.x * 256 + threadIdx.x; u < df.v;
-         u += (long long)gridDim.x * 256) {
+    long long end;
+    for (long long u = xcd_chunk(df.v, 256, end) + threadIdx.x; u < end; u += 256) {
         if (nt[u] != NT_REG) continue;
         double az = (z[u - 1] + z[u + 1] + z[u - sj] + z[u + sj] + z[u - sk] + z[u + sk] - 6.0 * z[u]) * ih2;
         double ac = (c[u - 1] + c[u + 1] + c[u - sj] + c[u + sj] + c[u - sk] + c[u + sk] - 6.0 * c[u]) * ih2;
@@ -838,16 +839,6 @@ __global__ void __launch_bounds__(256)
         atomicAdd(&out[0], acc[0]);
         atomicAdd(&out[1], acc[1]);
     }
-}
-
-// z += omega c, omega = (c.res) / (c.A c) minimises the A-norm error of z
-__global__ void __launch_bounds__(256)
-    k_cupdate(long long n, const double* __restrict__ d, const double* __restrict__ c,
-              double* __restrict__ z) {
-    long long u = (long long)blockIdx.x * 256 + threadIdx.x;
-    if (u >= n) return;
-    double om = d[1] != 0.0 ? d[0] / d[1] : 0.0;
-    z[u] += om * c[u];
 }
 
 }  // namespace
@@ -902,7 +893,9 @@ static int ualloc(pic_ctx* ctx, UNet* u, void** p, size_t bytes) {
     return PIC_OK;
 }
 
-// deterministic He-style initialisation (host, then uploaded)
+// deterministic initialisation (host, then uploaded): weights and biases
+// uniform in +-1/sqrt(fan_in).  (The ReLU-gain bound sqrt(6/fan_in) with zero
+// biases trains markedly slower on this regression.)
 static void init_params(UNet* u, uint32_t seed, std::vector<float>& h) {
     h.assign(u->nparam, 0.f);
     uint32_t s = seed * 2654435761u + 12345u;
@@ -915,9 +908,9 @@ static void init_params(UNet* u, uint32_t seed, std::vector<float>& h) {
     for (size_t l = 0; l < u->L.size(); l++) {
         const UNetLayer& ly = u->L[l];
         size_t nw = (size_t)ly.cout * ly.taps * ly.cin;
-        float scale = std::sqrt(6.0f / (float)(ly.taps * ly.cin));
-        if ((int)l == u->l_out) scale *= 0.1f;
+        float scale = 1.0f / std::sqrt((float)(ly.taps * ly.cin));
         for (size_t q = 0; q < nw; q++) h[ly.woff + q] = scale * rnd();
+        for (int q = 0; q < ly.cout; q++) h[ly.boff + q] = scale * rnd();
     }
 }
 
@@ -1042,11 +1035,21 @@ static void conv(pic_ctx* ctx, const Dims& d, const bf16* in, int cin, const bf1
                  const float* bias, bf16* out, int cout, int taps, int relu, const bf16* mask) {
     ConvArgs a{in, cin, w, bias, out, cout, mask, relu, 0, d};
     int ntx = (d.nx + TX - 1) / TX, nty = (d.ny + TY - 1) / TY, ntz = (d.nz + TZ - 1) / TZ;
-    dim3 grid(ntx * nty * ntz, (cout / 16 + NBB - 1) / NBB);
-    if (taps == 27)
-        k_conv_mfma<27><<<grid, 256, 0, ctx->stream>>>(a);
-    else
-        k_conv_mfma<1><<<grid, 256, 0, ctx->stream>>>(a);
+    int tiles = ntx * nty * ntz;
+    // two N-blocks per workgroup when that still leaves >= 512 workgroups
+    int nb = (cout % 32 == 0 && tiles * (cout / 32) >= 512) ? 2 : 1;
+    dim3 grid(tiles, cout / 16 / nb);
+    if (taps == 27) {
+        if (nb == 2)
+            k_conv_mfma<27, 2><<<grid, 256, 0, ctx->stream>>>(a);
+        else
+            k_conv_mfma<27, 1><<<grid, 256, 0, ctx->stream>>>(a);
+    } else {
+        if (nb == 2)
+            k_conv_mfma<1, 2><<<grid, 256, 0, ctx->stream>>>(a);
+        else
+            k_conv_mfma<1, 1><<<grid, 256, 0, ctx->stream>>>(a);
+    }
 }
 
 static int wgrad_tiles(const Dims& d, int* tpb) {
@@ -1357,16 +1360,31 @@ __global__ void __launch_bounds__(256)
     if (threadIdx.x == 0) atomicAdd(out, red[0] + red[1] + red[2] + red[3]);
 }
 
-// in place: a *= 1/sqrt(sumsq/count); rho = eps0 * a / dx^2 (the plain solver's input)
+// t = a / rms(a) in FP64 (a training residual)
 __global__ void __launch_bounds__(256)
-    k_norm_rho(long long n, float* __restrict__ a, const double* __restrict__ sumsq, double count,
-               double rho_fac, double* __restrict__ rho) {
+    k_norm_t(long long n, const float* __restrict__ a, const double* __restrict__ sumsq, double count,
+             double* __restrict__ t) {
     long long e = (long long)blockIdx.x * 256 + threadIdx.x;
     if (e >= n) return;
     double s = sumsq[0] > 0 ? 1.0 / sqrt(sumsq[0] / count) : 1.0;
-    float v = (float)(a[e] * s);
-    a[e] = v;
-    rho[e] = rho_fac * v;
+    t[e] = (double)a[e] * s;
+}
+
+// rho = -eps0 (t - A z) on regular nodes, 0 elsewhere: the plain solver then
+// returns e with A e = t - A z (A = 7-point Laplacian / h^2, interior nodes)
+__global__ void __launch_bounds__(256)
+    k_res_rho(Dims df, double ih2, const uint8_t* __restrict__ nt, const double* __restrict__ t,
+              const double* __restrict__ z, double* __restrict__ rho) {
+    const long long sj = df.nx, sk = (long long)df.nx * df.ny;
+    long long end;
+    for (long long u = xcd_chunk(df.v, 256, end) + threadIdx.x; u < end; u += 256) {
+        double v = 0.0;
+        if (nt[u] == NT_REG) {
+            double az = (z[u - 1] + z[u + 1] + z[u - sj] + z[u + sj] + z[u - sk] + z[u + sk] - 6.0 * z[u]) * ih2;
+            v = -EPS0 * (t[u] - az);
+        }
+        rho[u] = v;
+    }
 }
 
 }  // namespace
@@ -1383,43 +1401,46 @@ bool pic_unet_ready(pic_ctx* ctx) {
 
 // Coarse correction of the two-level preconditioner: the residual t - A z
 // (t: the fine right-hand side, z: the smoothed approximation, both device
-// FP64) is restricted to the network grid, the network is applied, its
-// output interpolated to c (device FP64 scratch vector) and z += omega c with
-// the A-norm optimal omega.  Only for Dirichlet-walled uniform grids (every
-// regular node is interior).
-int pic_unet_coarse_correct(pic_ctx* ctx, const double* t, double* z, double* c) {
+// FP64) is restricted to the network grid, the network is applied and its
+// output interpolated to c (device FP64).  dots (device, 2 doubles) receives
+// {c.(t - A z), c.A c}, from which the caller takes the A-norm optimal step
+// omega = dots[0] / dots[1] for z += omega c.  Only for Dirichlet-walled
+// uniform grids (every regular node is interior).
+int pic_unet_coarse_correct(pic_ctx* ctx, const double* t, const double* z, double* c, double* dots) {
     UNet* u = get_unet(ctx);
     if (!u) return pic_fail(ctx, PIC_ERR_STATE, "no U-Net");
     const GridC& g = ctx->g;
     Dims df{g.ni, g.nj, g.nk, g.nn};
     const Dims& dc = u->lv[0].d;
     double h2 = g.dx[0] * g.dx[0];
-    double* sums = ctx->scratch + 64;  // [0..1] input normalisation, [2..3] line search
-    PIC_HIP(ctx, hipMemsetAsync(sums, 0, 4 * sizeof(double), ctx->stream));
+    double* sums = ctx->scratch + 64;  // input normalisation
+    PIC_HIP(ctx, hipMemsetAsync(sums, 0, 2 * sizeof(double), ctx->stream));
+    PIC_HIP(ctx, hipMemsetAsync(dots, 0, 2 * sizeof(double), ctx->stream));
+    unsigned nbc = xcd_blocks(dc.v, 256), nbf = xcd_blocks(df.v, 256);
     if (u->coarsen == 2)
-        k_res_restrict<2><<<ew(dc.v), 256, 0, ctx->stream>>>(df, dc, h2, ctx->ntype, t, z, u->x, sums);
+        k_res_restrict<2><<<nbc, 256, 0, ctx->stream>>>(df, dc, h2, ctx->ntype, t, z, u->x, sums);
     else
-        k_res_restrict<1><<<ew(dc.v), 256, 0, ctx->stream>>>(df, dc, h2, ctx->ntype, t, z, u->x, sums);
+        k_res_restrict<1><<<nbc, 256, 0, ctx->stream>>>(df, dc, h2, ctx->ntype, t, z, u->x, sums);
     int rc = unet_forward(ctx, u, u->x, sums);
     if (rc) return rc;
     if (u->coarsen == 2)
-        k_prolong<2><<<ew(df.v), 256, 0, ctx->stream>>>(df, dc, ctx->ntype, u->y, c);
+        k_prolong<2><<<nbf, 256, 0, ctx->stream>>>(df, dc, ctx->ntype, u->y, c);
     else
-        k_prolong<1><<<ew(df.v), 256, 0, ctx->stream>>>(df, dc, ctx->ntype, u->y, c);
-    unsigned nb = std::min<unsigned>(2048, ew(df.v));
-    k_cdots<<<nb, 256, 0, ctx->stream>>>(df, 1.0 / h2, ctx->ntype, t, z, c, sums + 2);
-    k_cupdate<<<ew(df.v), 256, 0, ctx->stream>>>(df.v, sums + 2, c, z);
+        k_prolong<1><<<nbf, 256, 0, ctx->stream>>>(df, dc, ctx->ntype, u->y, c);
+    k_cdots<<<nbf, 256, 0, ctx->stream>>>(df, 1.0 / h2, ctx->ntype, t, z, c, dots);
     PIC_HIP(ctx, hipGetLastError());
     return PIC_OK;
 }
 
 int pic_poisson_internal_sor(pic_ctx* ctx, double tol, int max_it, pic_solver_stats* st);
+int pic_learned_presmooth(pic_ctx* ctx, const double* r, double* z);
 
 extern "C" int pic_unet_train(pic_ctx* ctx, const pic_unet_train_desc* td,
                               pic_unet_train_stats* stats) {
     UNet* u = get_unet(ctx);
     if (!u || !td) return pic_fail(ctx, PIC_ERR_STATE, "create the U-Net first");
-    if (td->samples < 1 || td->epochs < 1 || td->batch < 1 || td->blobs_max < td->blobs_min)
+    if (td->samples < 1 || td->epochs < 1 || td->batch < 1 || td->blobs_max < td->blobs_min ||
+        !(td->rel_weight >= 0.0 && td->rel_weight < 1.0))
         return pic_fail(ctx, PIC_ERR_ARG, "bad training parameters");
     const GridC& g = ctx->g;
     if (!uniform_dx(g)) return pic_fail(ctx, PIC_ERR_ARG, "the learned solver needs uniform dx");
@@ -1431,7 +1452,7 @@ extern "C" int pic_unet_train(pic_ctx* ctx, const pic_unet_train_desc* td,
     long long Vc = dc.v;
     size_t S = (size_t)td->samples;
     float *dens = nullptr, *xs = nullptr, *ts = nullptr;
-    double *phi_keep = nullptr, *rho_keep = nullptr;
+    double *phi_keep = nullptr, *rho_keep = nullptr, *tf = nullptr, *zp = nullptr;
     Blob* dblobs = nullptr;
     auto cleanup = [&]() {
         (void)hipFree(dens);
@@ -1439,6 +1460,8 @@ extern "C" int pic_unet_train(pic_ctx* ctx, const pic_unet_train_desc* td,
         (void)hipFree(ts);
         (void)hipFree(phi_keep);
         (void)hipFree(rho_keep);
+        (void)hipFree(tf);
+        (void)hipFree(zp);
         (void)hipFree(dblobs);
     };
     hipError_t e = hipMalloc(&dens, V * sizeof(float));
@@ -1446,6 +1469,8 @@ extern "C" int pic_unet_train(pic_ctx* ctx, const pic_unet_train_desc* td,
     if (e == hipSuccess) e = hipMalloc(&ts, S * Vc * sizeof(float));
     if (e == hipSuccess) e = hipMalloc(&phi_keep, V * sizeof(double));
     if (e == hipSuccess) e = hipMalloc(&rho_keep, V * sizeof(double));
+    if (e == hipSuccess) e = hipMalloc(&tf, V * sizeof(double));
+    if (e == hipSuccess) e = hipMalloc(&zp, V * sizeof(double));
     if (e == hipSuccess) e = hipMalloc(&dblobs, 64 * sizeof(Blob));
     if (e == hipSuccess) e = hipMemcpyAsync(phi_keep, ctx->phi, V * 8, hipMemcpyDeviceToDevice, ctx->stream);
     if (e == hipSuccess) e = hipMemcpyAsync(rho_keep, ctx->rho, V * 8, hipMemcpyDeviceToDevice, ctx->stream);
@@ -1457,7 +1482,9 @@ extern "C" int pic_unet_train(pic_ctx* ctx, const pic_unet_train_desc* td,
         return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
     };
     double t0 = wall();
-    // ---- pairs from the plain solver ----
+    // ---- pairs from the plain solver: a residual t (blobs + white noise), its
+    // pre-smoothed remainder res = t - A S(t) restricted to the network grid
+    // (the preconditioner's input), and the restricted exact error A^-1 res ----
     uint32_t rs = td->seed * 747796405u + 2891336453u;
     auto urand = [&]() {
         rs ^= rs << 13;
@@ -1468,7 +1495,7 @@ extern "C" int pic_unet_train(pic_ctx* ctx, const pic_unet_train_desc* td,
     double dx2 = g.dx[0] * g.dx[0];
     double count = (double)(g.ni - 2) * (g.nj - 2) * (g.nk - 2);
     double* ss = ctx->scratch + 64;
-    std::vector<double> nvalid(S, 1.0);
+    std::vector<double> nvalid(S, 1.0), tms(S, 0.0);
     for (size_t s = 0; s < S && !rc; s++) {
         int nb = td->blobs_min + (int)(urand() * (td->blobs_max - td->blobs_min + 1));
         nb = std::min(nb, 64);
@@ -1486,11 +1513,23 @@ extern "C" int pic_unet_train(pic_ctx* ctx, const pic_unet_train_desc* td,
             rc = pic_hip_check(ctx, e, "blobs");
             break;
         }
-        k_blobs<<<ew(V), 256, 0, ctx->stream>>>(df, dblobs, nb, (float)td->noise, td->seed,
-                                               (uint32_t)s, ctx->ntype, dens);
-        (void)hipMemsetAsync(ss, 0, 2 * sizeof(double), ctx->stream);
+        float noise = (float)(td->noise * urand());
+        k_blobs<<<ew(V), 256, 0, ctx->stream>>>(df, dblobs, nb, noise, td->seed, (uint32_t)s, ctx->ntype,
+                                               dens);
+        (void)hipMemsetAsync(ss, 0, 3 * sizeof(double), ctx->stream);
         k_sumsq_f<<<std::min<unsigned>(1024, ew(V)), 256, 0, ctx->stream>>>(V, dens, ss);
-        k_norm_rho<<<ew(V), 256, 0, ctx->stream>>>(V, dens, ss, count, EPS0 / dx2, ctx->rho);
+        k_norm_t<<<ew(V), 256, 0, ctx->stream>>>(V, dens, ss, count, tf);
+        rc = pic_learned_presmooth(ctx, tf, zp);
+        if (rc) break;
+        float* xd = xs + s * Vc * 2;
+        float* td_ = ts + s * Vc;
+        (void)hipMemsetAsync(ss, 0, 3 * sizeof(double), ctx->stream);
+        unsigned nbc = xcd_blocks(Vc, 256), nbf = xcd_blocks(V, 256);
+        if (u->coarsen == 2)
+            k_res_restrict<2><<<nbc, 256, 0, ctx->stream>>>(df, dc, dx2, ctx->ntype, tf, zp, xd, ss);
+        else
+            k_res_restrict<1><<<nbc, 256, 0, ctx->stream>>>(df, dc, dx2, ctx->ntype, tf, zp, xd, ss);
+        k_res_rho<<<nbf, 256, 0, ctx->stream>>>(df, 1.0 / dx2, ctx->ntype, tf, zp, ctx->rho);
         (void)hipMemsetAsync(ctx->phi, 0, V * 8, ctx->stream);
         pic_solver_stats st;
         int r = pic_poisson_internal_sor(ctx, 1e-6, 20000, &st);
@@ -1498,15 +1537,12 @@ extern "C" int pic_unet_train(pic_ctx* ctx, const pic_unet_train_desc* td,
             rc = r;
             break;
         }
-        float* xd = xs + s * Vc * 2;
-        float* td_ = ts + s * Vc;
-        (void)hipMemsetAsync(ss, 0, 3 * sizeof(double), ctx->stream);
         if (u->coarsen == 2)
-            k_restrict_pair<2><<<ew(Vc), 256, 0, ctx->stream>>>(df, dc, ctx->ntype, dens, ctx->phi, xd, td_, ss);
+            k_restrict_target<2><<<nbc, 256, 0, ctx->stream>>>(df, dc, ctx->ntype, ctx->phi, td_, ss);
         else
-            k_restrict_pair<1><<<ew(Vc), 256, 0, ctx->stream>>>(df, dc, ctx->ntype, dens, ctx->phi, xd, td_, ss);
+            k_restrict_target<1><<<nbc, 256, 0, ctx->stream>>>(df, dc, ctx->ntype, ctx->phi, td_, ss);
         k_scale_pair<<<ew(Vc), 256, 0, ctx->stream>>>(Vc, ss, xd, td_);
-        double h2[2];
+        double h2[3];
         (void)hipMemcpyAsync(h2, ss, sizeof(h2), hipMemcpyDeviceToHost, ctx->stream);
         e = hipStreamSynchronize(ctx->stream);
         if (e != hipSuccess) {
@@ -1514,6 +1550,27 @@ extern "C" int pic_unet_train(pic_ctx* ctx, const pic_unet_train_desc* td,
             break;
         }
         nvalid[s] = std::max(1.0, h2[1]);
+        tms[s] = h2[0] > 0 ? h2[2] / h2[0] : 0.0;  // mean square of the scaled target
+    }
+    // one constant for the whole set brings the targets to unit RMS on average
+    // (the solver's line search makes the network's output scale irrelevant).
+    // The loss is (nearly) relative per sample: weight 1 / (valid count *
+    // (alpha ms + 1 - alpha)), ms = the sample's target mean square over the set
+    // mean, so residuals whose error is small -- the rough ones that dominate
+    // after the first iterations -- count about as much as smooth ones.
+    std::vector<double> inv_w(S, 1.0);
+    if (!rc) {
+        double m2 = 0;
+        for (size_t q = 0; q < S; q++) m2 += tms[q];
+        m2 /= (double)S;
+        if (m2 > 0)
+            k_scale_f<<<ew((long long)S * Vc), 256, 0, ctx->stream>>>((long long)S * Vc,
+                                                                     (float)(1.0 / std::sqrt(m2)), ts);
+        const double al = td->rel_weight;
+        for (size_t q = 0; q < S; q++) {
+            double ms = m2 > 0 ? tms[q] / m2 : 1.0;
+            inv_w[q] = 1.0 / (nvalid[q] * (al * ms + (1.0 - al)));
+        }
     }
     (void)hipMemcpyAsync(ctx->phi, phi_keep, V * 8, hipMemcpyDeviceToDevice, ctx->stream);
     (void)hipMemcpyAsync(ctx->rho, rho_keep, V * 8, hipMemcpyDeviceToDevice, ctx->stream);
@@ -1536,8 +1593,8 @@ extern "C" int pic_unet_train(pic_ctx* ctx, const pic_unet_train_desc* td,
                 const float* xd = xs + s * Vc * 2;
                 rc = unet_forward(ctx, u, xd, nullptr);
                 if (rc) break;
-                k_loss<<<ew(Vc), 256, 0, ctx->stream>>>(Vc, u->y, ts + s * Vc, xd,
-                                                        (float)(1.0 / nvalid[s]), u->dy, ctx->scratch);
+                k_loss<<<ew(Vc), 256, 0, ctx->stream>>>(Vc, u->y, ts + s * Vc, xd, (float)inv_w[s],
+                                                        u->dy, ctx->scratch);
                 rc = unet_backward(ctx, u, xd);
             }
             if (!rc) rc = adam_step(ctx, u, (float)td->lr, 1.0f / (float)(b1 - b0));

@@ -55,7 +55,8 @@ class UNetDesc(C.Structure):
 class UNetTrainDesc(C.Structure):
     _fields_ = [("samples", C.c_int), ("epochs", C.c_int), ("batch", C.c_int), ("lr", C.c_double),
                 ("seed", C.c_uint32), ("blobs_min", C.c_int), ("blobs_max", C.c_int),
-                ("sigma_min", C.c_double), ("sigma_max", C.c_double), ("noise", C.c_double)]
+                ("sigma_min", C.c_double), ("sigma_max", C.c_double), ("noise", C.c_double),
+                ("rel_weight", C.c_double)]
 
 
 class UNetTrainStats(C.Structure):
@@ -372,9 +373,9 @@ class Context:
         self._ck(self.L.pic_unet_adam_step(self.h, np.ascontiguousarray(grad, np.float32), lr))
 
     def unet_train(self, samples, epochs, batch=4, lr=1e-3, seed=1, blobs=(1, 8),
-                   sigma=(4.0, 16.0), noise=0.0):
+                   sigma=(4.0, 16.0), noise=0.0, rel_weight=0.9):
         d = UNetTrainDesc(samples, epochs, batch, lr, seed, blobs[0], blobs[1], sigma[0],
-                          sigma[1], noise)
+                          sigma[1], noise, rel_weight)
         st = UNetTrainStats()
         self._ck(self.L.pic_unet_train(self.h, C.byref(d), C.byref(st)))
         return {"loss_first": st.loss_first, "loss_last": st.loss_last,

@@ -87,6 +87,7 @@ def main():
     ap.add_argument("--noise", type=float, default=0.3)
     ap.add_argument("--sigma", type=float, nargs=2, default=(2.0, 20.0))
     ap.add_argument("--blobs", type=int, nargs=2, default=(0, 10))
+    ap.add_argument("--rel-weight", type=float, default=0.9)
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--init", default=None, help="start from these weights")
     ap.add_argument("--out", default=None)
@@ -106,7 +107,7 @@ def main():
         lr = args.lr * (0.5 ** (r // max(1, args.rounds // 3)))
         st = ctx.unet_train(samples=args.samples, epochs=args.epochs, batch=args.batch, lr=lr,
                             seed=args.seed * 1000 + r, blobs=tuple(args.blobs),
-                            sigma=tuple(args.sigma), noise=args.noise)
+                            sigma=tuple(args.sigma), noise=args.noise, rel_weight=args.rel_weight)
         hist.append(st)
         print("round %d: loss %.4e -> %.4e (data %.1fs, train %.1fs, lr %.1e)"
               % (r, st["loss_first"], st["loss_last"], st["seconds_data"], st["seconds_train"], lr),
@@ -118,6 +119,7 @@ def main():
             "params": int(p.size), "rounds": args.rounds, "samples_per_round": args.samples,
             "epochs": args.epochs, "batch": args.batch, "lr": args.lr, "noise": args.noise,
             "sigma": list(args.sigma), "blobs": list(args.blobs), "seed": args.seed,
+            "rel_weight": args.rel_weight,
             "loss_per_round": [h["loss_last"] for h in hist],
             "train_seconds": time.time() - t0}
     json.dump(meta, open(out + ".json", "w"), indent=1)

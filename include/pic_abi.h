@@ -203,18 +203,25 @@ typedef struct {
     int coarsen;                        /* 1 or 2 */
     uint32_t seed;                      /* weight initialisation */
 } pic_unet_desc;
+/* Training pairs: a residual t (Gaussian blobs plus white noise), the part
+ * res = t - A S(t) left by the pre-smoother S, restricted to the network grid
+ * (input), and the restricted exact error A^-1 res from the plain solver
+ * (target).  Loss per sample: 0.5 |y - target|^2 / (N (alpha ms + 1 - alpha))
+ * over the N voxels holding a regular node (alpha = 1: relative error). */
 typedef struct {
     int samples;          /* training pairs generated by the plain solver */
     int epochs;
     int batch;            /* samples per Adam step */
     double lr;
     uint32_t seed;
-    int blobs_min, blobs_max;     /* Gaussian blobs per density */
-    double sigma_min, sigma_max;  /* blob widths in cells */
-    double noise;                 /* white-noise RMS added to each density (particle noise) */
+    int blobs_min, blobs_max;     /* Gaussian blobs per residual */
+    double sigma_min, sigma_max;  /* blob widths in node spacings */
+    double noise;                 /* maximum white-noise RMS (uniform per sample) */
+    double rel_weight;            /* alpha in [0, 1): sample weight 1 / (alpha ms + 1 - alpha),
+                                     ms = the target's mean square over the set mean */
 } pic_unet_train_desc;
 typedef struct {
-    double loss_first, loss_last;  /* mean loss of the first and last epoch */
+    double loss_first, loss_last;  /* mean relative loss of the first and last epoch */
     double seconds_data, seconds_train;
     int steps;
 } pic_unet_train_stats;
