@@ -84,6 +84,22 @@ def cpu_baseline(threads, particles, steps):
             "kind": "port", "sample": sample.replace("push + deposit", "push")}
 
 
+def blob_density(n, seed=12345):
+    """six Gaussian blobs (widths 3 to 0.15 n nodes) plus white noise at a
+    tenth of their amplitude, zero on the walls; C/m^3, node order i-fastest"""
+    rng = np.random.default_rng(seed)
+    k, j, i = np.meshgrid(np.arange(n), np.arange(n), np.arange(n), indexing="ij")
+    rho = np.zeros((n, n, n))
+    for _ in range(6):
+        c = rng.uniform(4, n - 5, 3)
+        s = rng.uniform(3, 0.15 * n)
+        rho += rng.choice([-1, 1]) * np.exp(-((i - c[0]) ** 2 + (j - c[1]) ** 2 + (k - c[2]) ** 2)
+                                            / (2 * s * s))
+    rho += 0.1 * rng.normal(size=rho.shape)
+    rho[0, :, :] = rho[-1, :, :] = rho[:, 0, :] = rho[:, -1, :] = rho[:, :, 0] = rho[:, :, -1] = 0
+    return 1e-8 * rho.reshape(-1)
+
+
 def copy_calibration():
     """A plain copy of known size, for scaling FETCH_SIZE / WRITE_SIZE counters."""
     import torch
@@ -109,7 +125,8 @@ def main():
     ap.add_argument("--cpu-particles", type=int, default=8_000_000)
     ap.add_argument("--cpu-steps", type=int, default=8)
     ap.add_argument("--copy-calibration", action="store_true")
-    ap.add_argument("--step-solver", default="sor_rb", choices=["sor_rb", "pcg_mixed", "pcg_fp64"])
+    ap.add_argument("--step-solver", default="sor_rb",
+                    choices=["sor_rb", "pcg_mixed", "pcg_fp64", "learned"])
     args = ap.parse_args()
     if args.copy_calibration:
         copy_calibration()
@@ -151,10 +168,17 @@ def main():
         ctx.sort(s)
         ctx.deposit(s)
     ctx.charge_density()
+    weights = os.path.join(PKG, "data", "unet_%d.f32" % n)
+    have_net = os.path.exists(weights) and os.path.exists(weights + ".json")
+    if have_net:
+        ctx.unet_load(weights)
+    elif args.step_solver == "learned":
+        raise SystemExit("--step-solver learned needs %s(.json)" % weights)
     omega_opt = 2.0 / (1.0 + np.sin(np.pi / (n - 1)))
     solve_kw = {"sor_rb": dict(method=pg.SOLVER_SOR_RB, omega=omega_opt, check_every=10),
                 "pcg_mixed": dict(method=pg.SOLVER_PCG_MIXED),
-                "pcg_fp64": dict(method=pg.SOLVER_PCG)}[args.step_solver]
+                "pcg_fp64": dict(method=pg.SOLVER_PCG),
+                "learned": dict(method=pg.SOLVER_LEARNED, check_every=2)}[args.step_solver]
     solve_kw.update(tol=1e-6, tol_kind=pg.TOL_REL, max_it=20000)
     ctx.poisson(**solve_kw)
     ctx.efield()
@@ -163,19 +187,36 @@ def main():
     poisson = None
     if not args.no_poisson_compare:
         phi_keep = ctx.get_field(pg.FIELD_PHI)
+        rho_keep = ctx.get_field(pg.FIELD_RHO)
         zeros = np.zeros(ctx.nn)
-        poisson = {"grid": "%d^3" % n, "start": "phi = 0", "tol_rel": 1e-6}
-        for name, kw in (("pcg_fp64", dict(method=pg.SOLVER_PCG)),
-                         ("pcg_mixed", dict(method=pg.SOLVER_PCG_MIXED)),
-                         ("sor_rb", dict(method=pg.SOLVER_SOR_RB, check_every=50,
-                                         omega=2.0 / (1.0 + np.sin(np.pi / (n - 1)))))):
-            ctx.set_field(pg.FIELD_PHI, zeros)
-            st = ctx.poisson(tol=1e-6, tol_kind=pg.TOL_REL, max_it=20000, **kw)
-            poisson[name] = {"seconds": round(st["seconds"], 5), "iterations": st["iterations"],
+        # cold start on the step's own charge density and on a smooth
+        # blob-plus-noise density; each solve runs twice, the second is timed
+        cases = {"pic_density": rho_keep, "blobs": blob_density(n)}
+        poisson = {"grid": "%d^3" % n, "start": "phi = 0", "tol_rel": 1e-6,
+                   "weights": os.path.relpath(weights, ROOT) if have_net else None}
+        runs = [("pcg_fp64", dict(method=pg.SOLVER_PCG)),
+                ("pcg_mixed", dict(method=pg.SOLVER_PCG_MIXED)),
+                ("sor_rb", dict(method=pg.SOLVER_SOR_RB, check_every=25,
+                                omega=2.0 / (1.0 + np.sin(np.pi / (n - 1)))))]
+        if have_net:
+            runs.append(("learned", dict(method=pg.SOLVER_LEARNED)))
+        for cname, rho in cases.items():
+            ctx.set_field(pg.FIELD_RHO, rho)
+            res = {}
+            for name, kw in runs:
+                for _ in range(2):
+                    ctx.set_field(pg.FIELD_PHI, zeros)
+                    st = ctx.poisson(tol=1e-6, tol_kind=pg.TOL_REL, max_it=20000, **kw)
+                res[name] = {"seconds": round(st["seconds"], 5), "iterations": st["iterations"],
                              "rel_residual": st["rel_residual"], "converged": st["converged"]}
-            print("poisson %s: %.4f s, %d it, rel %.2e" % (name, st["seconds"], st["iterations"],
-                                                           st["rel_residual"]), file=sys.stderr)
-        poisson["learned"] = None
+                print("poisson %s %s: %.4f s, %d it, rel %.2e" % (
+                    cname, name, st["seconds"], st["iterations"], st["rel_residual"]), file=sys.stderr)
+            if have_net and res["learned"]["converged"]:
+                t = res["learned"]["seconds"]
+                res["learned_speedup_vs_sor_rb"] = round(res["sor_rb"]["seconds"] / t, 3)
+                res["learned_speedup_vs_pcg_fp64"] = round(res["pcg_fp64"]["seconds"] / t, 3)
+            poisson[cname] = res
+        ctx.set_field(pg.FIELD_RHO, rho_keep)
         ctx.set_field(pg.FIELD_PHI, phi_keep)
 
     solve_iters = []

@@ -101,6 +101,8 @@ extern "C" int pic_create(const pic_grid_desc* gd, pic_ctx** out) {
     *out = nullptr;
     if (gd->ni < 3 || gd->nj < 3 || gd->nk < 3) return PIC_ERR_ARG;
     if (gd->ni - 1 > 2047 || gd->nj - 1 > 2047 || gd->nk - 1 > 1023) return PIC_ERR_ARG;
+    // node indices are 32-bit in the grid kernels
+    if ((long long)gd->ni * gd->nj * gd->nk > 0x7FFFFFFFLL) return PIC_ERR_ARG;
     for (int d = 0; d < 3; d++)
         if (!(gd->xm[d] > gd->x0[d])) return PIC_ERR_ARG;
     int ndev = 0;
@@ -131,7 +133,7 @@ extern "C" int pic_create(const pic_grid_desc* gd, pic_ctx** out) {
     if (e == hipSuccess) e = hipMalloc(&ctx->rho, nn * sizeof(double));
     if (e == hipSuccess) e = hipMalloc(&ctx->ef, nn * sizeof(float4));
     if (e == hipSuccess) e = hipMalloc(&ctx->ntype, nn);
-    if (e == hipSuccess) e = hipMalloc(&ctx->scratch, 4096 * sizeof(double));
+    if (e == hipSuccess) e = hipMalloc(&ctx->scratch, SCRATCH_DOUBLES * sizeof(double));
     if (e == hipSuccess) e = hipMalloc(&ctx->scan_tmp, ((size_t)g.ncells / 4096 + 2) * 4);
     if (e == hipSuccess) e = hipMemsetAsync(ctx->phi, 0, nn * sizeof(double), ctx->stream);
     if (e == hipSuccess) e = hipMemsetAsync(ctx->rho, 0, nn * sizeof(double), ctx->stream);

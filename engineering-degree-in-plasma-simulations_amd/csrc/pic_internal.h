@@ -239,11 +239,36 @@ inline unsigned grid_for(size_t n, unsigned block) {
 // neighbour planes).  The grid size must be a multiple of 8 (xcd_blocks).
 constexpr int XCD_COUNT = 8;
 inline unsigned xcd_blocks(long long items, int block) {
-    long long b = (items + block * 8 - 1) / (block * 8);
-    if (b > 2048) b = 2048;
+    // about two items per thread: enough workgroups in flight to cover HBM latency
+    long long b = (items + block * 2 - 1) / (block * 2);
+    if (b > 16384) b = 16384;
     b = (b + XCD_COUNT - 1) / XCD_COUNT * XCD_COUNT;
     return (unsigned)b;
 }
+// Striped grid-wide sums: one scalar is NSTRIPE partial accumulators 128
+// bytes apart, workgroup b adds into stripe b % NSTRIPE, so the per-workgroup
+// atomics of a large grid spread over L2 channels instead of serialising on
+// one address.  A striped scalar occupies SSIZE doubles; zero all of them.
+constexpr int NSTRIPE = 32;
+constexpr int SSTRIDE = 16;
+constexpr int SSIZE = NSTRIPE * SSTRIDE;
+__device__ inline void sadd(double* s, double v) {
+    atomicAdd(s + (blockIdx.x % NSTRIPE) * SSTRIDE, v);
+}
+// the total, in every lane of the calling wave (all 64 lanes must call)
+__device__ inline double sget(const double* s) {
+    int l = threadIdx.x & 63;
+    double v = l < NSTRIPE ? s[l * SSTRIDE] : 0.0;
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+inline double shost(const double* s) {
+    double t = 0;
+    for (int q = 0; q < NSTRIPE; q++) t += s[q * SSTRIDE];
+    return t;
+}
+constexpr size_t SCRATCH_DOUBLES = 16384;  // ctx->scratch: [0, 4096) plain, [4096, ...) striped
+
 __device__ inline long long xcd_chunk(long long n, int block, long long& end) {
     int nb = gridDim.x;
     int b = blockIdx.x;

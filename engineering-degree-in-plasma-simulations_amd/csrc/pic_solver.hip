@@ -38,6 +38,16 @@ struct Stencil {
     int ni, nj, nk;
 };
 
+// (i, j, k) of node u in 32-bit arithmetic (grids hold fewer than 2^31 nodes)
+__device__ inline void node_ijk(const Stencil& s, long long u, int& i, int& j, int& k) {
+    unsigned uu = (unsigned)u, ni = (unsigned)s.ni, nj = (unsigned)s.nj;
+    unsigned row = uu / ni;
+    i = (int)(uu - row * ni);
+    unsigned kk = row / nj;
+    j = (int)(row - kk * nj);
+    k = (int)kk;
+}
+
 // gridDim.x is a multiple of NXCD (stencil_blocks), so lb is a bijection
 __device__ inline long long chunk_begin(long long nn, long long& end) {
     int nb = gridDim.x;
@@ -110,9 +120,8 @@ __global__ void __launch_bounds__(BLOCK)
     k_efield(GridC g, const double* __restrict__ phi, float4* __restrict__ ef) {
     long long u = (long long)blockIdx.x * BLOCK + threadIdx.x;
     if (u >= g.nn) return;
-    int i = (int)(u % g.ni);
-    long long r = u / g.ni;
-    int j = (int)(r % g.nj), k = (int)(r / g.nj);
+    unsigned uu = (unsigned)u, row = uu / (unsigned)g.ni, kk = row / (unsigned)g.nj;
+    int i = (int)(uu - row * (unsigned)g.ni), j = (int)(row - kk * (unsigned)g.nj), k = (int)kk;
     int ijk[3] = {i, j, k};
     int n[3] = {g.ni, g.nj, g.nk};
     long long st[3] = {1, g.ni, (long long)g.ni * g.nj};
@@ -148,9 +157,10 @@ __global__ void __launch_bounds__(BLOCK)
     long long items = half * s.nj * s.nk;
     long long end;
     for (long long m = chunk_begin(items, end) + threadIdx.x; m < end; m += BLOCK) {
-        long long row = m / half;
-        int h = (int)(m - row * half);
-        int j = (int)(row % s.nj), k = (int)(row / s.nj);
+        unsigned row = (unsigned)m / (unsigned)half;
+        int h = (int)((unsigned)m - row * (unsigned)half);
+        unsigned kk = row / (unsigned)s.nj;
+        int j = (int)(row - kk * (unsigned)s.nj), k = (int)kk;
         int i = 2 * h + ((j + k + a.color) & 1);
         if (i >= s.ni) continue;
         long long u = row * s.ni + i;
@@ -394,9 +404,19 @@ Stencil make_stencil(const GridC& g, bool axis_swap) {
     return s;
 }
 
-unsigned stencil_blocks(long long items) {
+// kernels ending in a single-address grid-wide sum: fewer, longer workgroups
+// (eight items per thread) keep the per-workgroup atomics few
+unsigned stencil_blocks_red(long long items) {
     long long b = (items + BLOCK * 8 - 1) / (BLOCK * 8);
     if (b > 2048) b = 2048;
+    b = (b + NXCD - 1) / NXCD * NXCD;
+    return (unsigned)b;
+}
+
+unsigned stencil_blocks(long long items) {
+    // about two items per thread: enough workgroups in flight to cover HBM latency
+    long long b = (items + BLOCK * 2 - 1) / (BLOCK * 2);
+    if (b > 16384) b = 16384;
     b = (b + NXCD - 1) / NXCD * NXCD;  // multiple of NXCD, at least NXCD
     return (unsigned)b;
 }
@@ -432,7 +452,7 @@ static bool has_neumann(pic_ctx* ctx) {
 
 static int residual_norms(pic_ctx* ctx, const Stencil& st, const SorArgs& a, double out[3]) {
     PIC_HIP(ctx, hipMemsetAsync(ctx->scratch, 0, 3 * sizeof(double), ctx->stream));
-    k_resid<<<stencil_blocks(st.nn), BLOCK, 0, ctx->stream>>>(st, a, ctx->ntype, ctx->phi, ctx->rho,
+    k_resid<<<stencil_blocks_red(st.nn), BLOCK, 0, ctx->stream>>>(st, a, ctx->ntype, ctx->phi, ctx->rho,
                                                               ctx->scratch);
     PIC_HIP(ctx, hipGetLastError());
     PIC_HIP(ctx, hipMemcpyAsync(out, ctx->scratch, 3 * sizeof(double), hipMemcpyDeviceToHost,
@@ -519,7 +539,7 @@ static int pcg_inner(pic_ctx* ctx, const Stencil& s, T* base, const double* P, d
     h.gs = hist + (max_it + 2);
     h.gg = hist + 2 * (max_it + 2);
     PIC_HIP(ctx, hipMemsetAsync(hist, 0, 3 * (size_t)(max_it + 2) * sizeof(double), ctx->stream));
-    unsigned nb = stencil_blocks(s.nn);
+    unsigned nb = stencil_blocks_red(s.nn);
     {
         // set-up launches are not counted as PCG iterations
         k_pcg0<T><<<nb, BLOCK, 0, ctx->stream>>>(s, ctx->ntype, v, h);
@@ -588,7 +608,7 @@ static int solve_pcg(pic_ctx* ctx, const pic_solver_params* p, pic_solver_stats*
     na.phi0 = p->phi0;
     na.n0 = p->n0;
     na.inv_te0 = 1.0 / (p->Te0 != 0 ? p->Te0 : 1.0);
-    unsigned nb = stencil_blocks(s.nn);
+    unsigned nb = stencil_blocks_red(s.nn);
     int total_it = 0, newton = 0;
     bool conv = false;
     double fr[3];
@@ -692,10 +712,12 @@ static int solve_pcg(pic_ctx* ctx, const pic_solver_params* p, pic_solver_stats*
 // residual, one sweep in the reverse colour order (black, red).
 // ------------------------------------------------------------------------
 bool pic_unet_ready(pic_ctx* ctx);
-int pic_unet_coarse_correct(pic_ctx* ctx, const double* t, const double* z, double* c, double* dots);
+int pic_unet_coarse_correct(pic_ctx* ctx, const double* t, const double* z, double* c, double* res,
+                            double* dots);
 
 namespace {
 
+// per-iteration striped scalars (slot k at base + k * SSIZE)
 struct FcgHist {
     double* rz;  // r_k . z_k
     double* pq;  // p_k . A p_k
@@ -739,8 +761,8 @@ __global__ void __launch_bounds__(BLOCK)
              FcgHist h, int k) {
     double beta = 0.0;
     if (k > 0) {
-        double den = h.pq[k - 1];
-        beta = den != 0.0 ? -h.zq[k - 1] / den : 0.0;
+        double den = sget(h.pq + (k - 1) * SSIZE);
+        beta = den != 0.0 ? -sget(h.zq + (k - 1) * SSIZE) / den : 0.0;
     }
     const bool first = k == 0;
     double acc = 0;
@@ -755,15 +777,15 @@ __global__ void __launch_bounds__(BLOCK)
         acc += pu * qv;
     }
     double a = block_sum(acc);
-    if (threadIdx.x == 0) atomicAdd(&h.pq[k], a);
+    if (threadIdx.x == 0) sadd(h.pq + k * SSIZE, a);
 }
 
 // alpha = rz[k] / pq[k]; x += alpha p; r -= alpha q; rr[k] += r.r
 __global__ void __launch_bounds__(BLOCK)
     k_fcg_update(long long n, FcgHist h, int k, double* __restrict__ x, const double* __restrict__ p,
                  double* __restrict__ r, const double* __restrict__ q) {
-    double den = h.pq[k];
-    double alpha = den != 0.0 ? h.rz[k] / den : 0.0;
+    double den = sget(h.pq + k * SSIZE);
+    double alpha = den != 0.0 ? sget(h.rz + k * SSIZE) / den : 0.0;
     double rr = 0;
     long long end;
     for (long long u = chunk_begin(n, end) + threadIdx.x; u < end; u += BLOCK) {
@@ -773,70 +795,62 @@ __global__ void __launch_bounds__(BLOCK)
         rr += rv * rv;
     }
     double a = block_sum(rr);
-    if (threadIdx.x == 0) atomicAdd(&h.rr[k], a);
+    if (threadIdx.x == 0) sadd(h.rr + k * SSIZE, a);
 }
 
-// zq[k] += z.q, rz[k+1] += z.r
+// One red-black Gauss-Seidel sweep on A z = r from z = 0, in one pass: the
+// red half-sweep gives z = -r / (2 sum 1/dx^2) on red regular nodes
+// ((i + j + k) even); the black half-sweep that follows reads only those red
+// values, so it is written directly in r.  Non-regular nodes get 0.
 __global__ void __launch_bounds__(BLOCK)
-    k_fcg_dots(long long n, FcgHist h, int k, const double* __restrict__ z,
-               const double* __restrict__ r, const double* __restrict__ q) {
+    k_presmooth(Stencil s, const uint8_t* __restrict__ nt, const double* __restrict__ r,
+                double* __restrict__ z) {
+    auto zr = [&](long long v) { return nt[v] == NT_REG ? -r[v] / s.twos : 0.0; };
+    long long end;
+    for (long long u = chunk_begin(s.nn, end) + threadIdx.x; u < end; u += BLOCK) {
+        if (nt[u] != NT_REG) {
+            z[u] = 0.0;
+            continue;
+        }
+        int i, j, k;
+        node_ijk(s, u, i, j, k);
+        if (((i + j + k) & 1) == 0)
+            z[u] = -r[u] / s.twos;
+        else
+            z[u] = ((zr(u - 1) + zr(u + 1)) * s.cx + (zr(u - s.sj) + zr(u + s.sj)) * s.cy +
+                    (zr(u - s.sk) + zr(u + s.sk)) * s.cz - r[u]) /
+                   s.twos;
+    }
+}
+
+// The closing red half-sweep of the preconditioner fused with FCG's dot
+// products: red regular nodes get the Gauss-Seidel update (their neighbours
+// are black and final); then zq += z.q (when q is given) and rz += z.r.
+__global__ void __launch_bounds__(BLOCK)
+    k_gs_red_dots(Stencil s, const uint8_t* __restrict__ nt, const double* __restrict__ r,
+                  double* __restrict__ z, const double* __restrict__ q, double* __restrict__ zq,
+                  double* __restrict__ rz) {
     double a = 0, b = 0;
     long long end;
-    for (long long u = chunk_begin(n, end) + threadIdx.x; u < end; u += BLOCK) {
-        double zv = z[u];
-        a += zv * q[u];
-        b += zv * r[u];
+    for (long long u = chunk_begin(s.nn, end) + threadIdx.x; u < end; u += BLOCK) {
+        double zu;
+        int i, j, k;
+        node_ijk(s, u, i, j, k);
+        if (nt[u] == NT_REG && ((i + j + k) & 1) == 0) {
+            zu = ((z[u - 1] + z[u + 1]) * s.cx + (z[u - s.sj] + z[u + s.sj]) * s.cy +
+                  (z[u - s.sk] + z[u + s.sk]) * s.cz - r[u]) /
+                 s.twos;
+            z[u] = zu;
+        } else {
+            zu = z[u];
+        }
+        if (q) a += zu * q[u];
+        b += zu * r[u];
     }
     double ta = block_sum(a), tb = block_sum(b);
     if (threadIdx.x == 0) {
-        atomicAdd(&h.zq[k], ta);
-        atomicAdd(&h.rz[k + 1], tb);
-    }
-}
-
-__global__ void __launch_bounds__(BLOCK)
-    k_dot(long long n, const double* __restrict__ a, const double* __restrict__ b,
-          double* __restrict__ out) {
-    double s = 0;
-    long long end;
-    for (long long u = chunk_begin(n, end) + threadIdx.x; u < end; u += BLOCK) s += a[u] * b[u];
-    double t = block_sum(s);
-    if (threadIdx.x == 0) atomicAdd(out, t);
-}
-
-// first red half-sweep on A z = r from z = 0: z = -r / (2 sum 1/dx^2) on red
-// regular nodes ((i + j + k) even), 0 everywhere else
-__global__ void __launch_bounds__(BLOCK)
-    k_gs_first(Stencil s, const uint8_t* __restrict__ nt, const double* __restrict__ r,
-               double* __restrict__ z) {
-    long long end;
-    for (long long u = chunk_begin(s.nn, end) + threadIdx.x; u < end; u += BLOCK) {
-        int i = (int)(u % s.ni);
-        long long row = u / s.ni;
-        int j = (int)(row % s.nj), k = (int)(row / s.nj);
-        bool red = ((i + j + k) & 1) == 0;
-        z[u] = (red && nt[u] == NT_REG) ? -r[u] / s.twos : 0.0;
-    }
-}
-
-// Gauss-Seidel half-sweep on A z = r over regular nodes of one colour
-__global__ void __launch_bounds__(BLOCK)
-    k_gs(Stencil s, const uint8_t* __restrict__ nt, const double* __restrict__ r,
-         double* __restrict__ z, int color) {
-    long long half = (s.ni + 1) / 2;
-    long long items = half * s.nj * s.nk;
-    long long end;
-    for (long long m = chunk_begin(items, end) + threadIdx.x; m < end; m += BLOCK) {
-        long long row = m / half;
-        int hh = (int)(m - row * half);
-        int j = (int)(row % s.nj), k = (int)(row / s.nj);
-        int i = 2 * hh + ((j + k + color) & 1);
-        if (i >= s.ni) continue;
-        long long u = row * s.ni + i;
-        if (nt[u] != NT_REG) continue;
-        z[u] = ((z[u - 1] + z[u + 1]) * s.cx + (z[u - s.sj] + z[u + s.sj]) * s.cy +
-                (z[u - s.sk] + z[u + s.sk]) * s.cz - r[u]) /
-               s.twos;
+        if (zq) sadd(zq, ta);
+        sadd(rz, tb);
     }
 }
 
@@ -848,16 +862,16 @@ __global__ void __launch_bounds__(BLOCK)
     k_gs_corr(Stencil s, const uint8_t* __restrict__ nt, const double* __restrict__ r,
               const double* __restrict__ zin, const double* __restrict__ c,
               const double* __restrict__ d, double* __restrict__ z) {
-    const double om = d[1] != 0.0 ? d[0] / d[1] : 0.0;
+    const double d1 = sget(d + SSIZE);
+    const double om = d1 != 0.0 ? sget(d) / d1 : 0.0;
     long long end;
     for (long long u = chunk_begin(s.nn, end) + threadIdx.x; u < end; u += BLOCK) {
         if (nt[u] != NT_REG) {
             z[u] = 0.0;
             continue;
         }
-        int i = (int)(u % s.ni);
-        long long row = u / s.ni;
-        int j = (int)(row % s.nj), k = (int)(row / s.nj);
+        int i, j, k;
+        node_ijk(s, u, i, j, k);
         if (((i + j + k) & 1) == 0) {
             z[u] = zin[u] + om * c[u];
         } else {
@@ -871,34 +885,30 @@ __global__ void __launch_bounds__(BLOCK)
 
 }  // namespace
 
-static void presmooth(pic_ctx* ctx, const Stencil& s, const double* r, double* z) {
-    long long half_items = (long long)((s.ni + 1) / 2) * s.nj * s.nk;
-    k_gs_first<<<stencil_blocks(s.nn), BLOCK, 0, ctx->stream>>>(s, ctx->ntype, r, z);
-    k_gs<<<stencil_blocks(half_items), BLOCK, 0, ctx->stream>>>(s, ctx->ntype, r, z, 1);
-}
-
 // z = one red-black Gauss-Seidel sweep on A z = r from z = 0 (device vectors):
 // the learned preconditioner's pre-smoother, also used for its training pairs
 int pic_learned_presmooth(pic_ctx* ctx, const double* r, double* z) {
-    presmooth(ctx, make_stencil(ctx->g, false), r, z);
+    Stencil s = make_stencil(ctx->g, false);
+    k_presmooth<<<stencil_blocks(s.nn), BLOCK, 0, ctx->stream>>>(s, ctx->ntype, r, z);
     PIC_HIP(ctx, hipGetLastError());
     return PIC_OK;
 }
 
-// z = M(r); tmp holds two scratch vectors
+// z = M(r); tmp holds three scratch vectors.  The closing half-sweep also
+// accumulates *zq += z.q (q may be null) and *rz += z.r; both must be zeroed
+// by the caller.
 static int learned_precond(pic_ctx* ctx, const Stencil& s, const double* rin, double* zout,
-                           double* tmp) {
+                           double* tmp, const double* q, double* zq, double* rz) {
     unsigned nb = stencil_blocks(s.nn);
-    long long half_items = (long long)((s.ni + 1) / 2) * s.nj * s.nk;
-    unsigned nbh = stencil_blocks(half_items);
     double* zpre = tmp;
     double* c = tmp + s.nn;
-    double* dots = ctx->scratch + 66;
-    presmooth(ctx, s, rin, zpre);
-    int e = pic_unet_coarse_correct(ctx, rin, zpre, c, dots);
+    double* res = c + s.nn;
+    double* dots = ctx->scratch + 4096;  // two striped scalars
+    k_presmooth<<<nb, BLOCK, 0, ctx->stream>>>(s, ctx->ntype, rin, zpre);
+    int e = pic_unet_coarse_correct(ctx, rin, zpre, c, res, dots);
     if (e) return e;
     k_gs_corr<<<nb, BLOCK, 0, ctx->stream>>>(s, ctx->ntype, rin, zpre, c, dots, zout);
-    k_gs<<<nbh, BLOCK, 0, ctx->stream>>>(s, ctx->ntype, rin, zout, 0);
+    k_gs_red_dots<<<nb, BLOCK, 0, ctx->stream>>>(s, ctx->ntype, rin, zout, q, zq, rz);
     PIC_HIP(ctx, hipGetLastError());
     return PIC_OK;
 }
@@ -919,12 +929,14 @@ extern "C" int pic_learned_precondition(pic_ctx* ctx, const double* r, double* z
     rc = learned_check(ctx);
     if (rc) return rc;
     size_t nn = (size_t)ctx->g.nn;
-    rc = pic_ensure_work(ctx, 4 * nn * sizeof(double));
+    rc = pic_ensure_work(ctx, 5 * nn * sizeof(double));
     if (rc) return rc;
     double* rd = ctx->work;
     double* zd = rd + nn;
+    double* rz = ctx->scratch + 4096 + 2 * SSIZE;  // striped
     PIC_HIP(ctx, hipMemcpyAsync(rd, r, nn * 8, hipMemcpyHostToDevice, ctx->stream));
-    rc = learned_precond(ctx, make_stencil(ctx->g, false), rd, zd, zd + nn);
+    PIC_HIP(ctx, hipMemsetAsync(rz, 0, SSIZE * sizeof(double), ctx->stream));
+    rc = learned_precond(ctx, make_stencil(ctx->g, false), rd, zd, zd + nn, nullptr, nullptr, rz);
     if (rc) return rc;
     PIC_HIP(ctx, hipMemcpyAsync(z, zd, nn * 8, hipMemcpyDeviceToHost, ctx->stream));
     PIC_HIP(ctx, hipStreamSynchronize(ctx->stream));
@@ -938,28 +950,31 @@ static int solve_learned(pic_ctx* ctx, const pic_solver_params* p, pic_solver_st
     const GridC& g = ctx->g;
     size_t nn = (size_t)g.nn;
     Stencil s = make_stencil(g, false);
-    rc = pic_ensure_work(ctx, 7 * nn * sizeof(double));
+    const int max_fcg = std::min(p->max_it, 500);
+    const int check = p->check_every > 0 ? p->check_every : 4;
+    const size_t hslots = (size_t)(max_fcg + 2) * SSIZE;  // striped history, per quantity
+    rc = pic_ensure_work(ctx, 8 * nn * sizeof(double) + 4 * hslots * sizeof(double));
     if (rc) return rc;
     double* r = ctx->work;
     double* z = r + nn;
     double* p0 = z + nn;
     double* p1 = p0 + nn;
     double* q = p1 + nn;
-    double* ctmp = q + nn;
-    const int max_fcg = std::min(p->max_it, 500);
-    const int check = p->check_every > 0 ? p->check_every : 4;
-    // history slots: scratch[1024 ..) (the network's normalisation uses scratch[64..65])
+    double* ctmp = q + nn;  // three vectors
     FcgHist h;
-    h.rz = ctx->scratch + 1024;
-    h.pq = h.rz + (max_fcg + 2);
-    h.zq = h.pq + (max_fcg + 2);
-    h.rr = h.zq + (max_fcg + 2);
+    h.rz = ctmp + 3 * nn;
+    h.pq = h.rz + hslots;
+    h.zq = h.pq + hslots;
+    h.rr = h.zq + hslots;
     double* sc = ctx->scratch + 8;
     unsigned nb = stencil_blocks(s.nn);
-    auto precond = [&](const double* rin, double* zout) { return learned_precond(ctx, s, rin, zout, ctmp); };
+    auto precond = [&](const double* rin, double* zout, const double* qv, double* zq, double* rz) {
+        return learned_precond(ctx, s, rin, zout, ctmp, qv, zq, rz);
+    };
     double h0[3];
     PIC_HIP(ctx, hipMemsetAsync(sc, 0, 3 * sizeof(double), ctx->stream));
-    k_fcg_resid<<<nb, BLOCK, 0, ctx->stream>>>(s, ctx->ntype, ctx->phi, ctx->rho, 1.0 / EPS0, r, sc);
+    k_fcg_resid<<<stencil_blocks_red(s.nn), BLOCK, 0, ctx->stream>>>(s, ctx->ntype, ctx->phi, ctx->rho,
+                                                                     1.0 / EPS0, r, sc);
     PIC_HIP(ctx, hipMemcpyAsync(h0, sc, sizeof(h0), hipMemcpyDeviceToHost, ctx->stream));
     PIC_HIP(ctx, hipStreamSynchronize(ctx->stream));
     double bnorm2 = h0[1] > 0 ? h0[1] : 1.0;
@@ -968,11 +983,10 @@ static int solve_learned(pic_ctx* ctx, const pic_solver_params* p, pic_solver_st
     int it = 0;
     bool conv = h0[0] <= target2;
     if (!conv) {
-        PIC_HIP(ctx, hipMemsetAsync(h.rz, 0, 4 * (size_t)(max_fcg + 2) * sizeof(double), ctx->stream));
-        rc = precond(r, z);
+        PIC_HIP(ctx, hipMemsetAsync(h.rz, 0, 4 * hslots * sizeof(double), ctx->stream));
+        rc = precond(r, z, nullptr, nullptr, h.rz);  // rz[0] = z.r
         if (rc) return rc;
-        k_dot<<<nb, BLOCK, 0, ctx->stream>>>(s.nn, r, z, h.rz);
-        std::vector<double> rr(max_fcg + 2);
+        std::vector<double> rr(max_fcg + 2), rrs(hslots);
         double* pold = p0;
         double* pnew = p1;
         int last = 0;
@@ -985,10 +999,12 @@ static int solve_learned(pic_ctx* ctx, const pic_solver_params* p, pic_solver_st
             PIC_HIP(ctx, hipGetLastError());
             it = k + 1;
             if (it % check == 0 || it == max_fcg) {
-                PIC_HIP(ctx, hipMemcpyAsync(rr.data() + last, h.rr + last, (size_t)(it - last) * sizeof(double),
+                PIC_HIP(ctx, hipMemcpyAsync(rrs.data() + (size_t)last * SSIZE, h.rr + (size_t)last * SSIZE,
+                                            (size_t)(it - last) * SSIZE * sizeof(double),
                                             hipMemcpyDeviceToHost, ctx->stream));
                 PIC_HIP(ctx, hipStreamSynchronize(ctx->stream));
                 for (int j = last; j < it; j++) {
+                    rr[j] = shost(rrs.data() + (size_t)j * SSIZE);
                     if (!(rr[j] == rr[j])) return pic_fail(ctx, PIC_ERR_NUMERIC, "learned PCG produced NaN");
                     if (rr[j] <= target2) conv = true;
                 }
@@ -996,12 +1012,8 @@ static int solve_learned(pic_ctx* ctx, const pic_solver_params* p, pic_solver_st
                 if (conv) break;
             }
             if (it == max_fcg) break;
-            rc = precond(r, z);
+            rc = precond(r, z, q, h.zq + (size_t)k * SSIZE, h.rz + (size_t)(k + 1) * SSIZE);
             if (rc) return rc;
-            {
-                PicProfScope ps(ctx, PIC_K_PCG);
-                k_fcg_dots<<<nb, BLOCK, 0, ctx->stream>>>(s.nn, h, k, z, r, q);
-            }
             std::swap(pold, pnew);
         }
     }

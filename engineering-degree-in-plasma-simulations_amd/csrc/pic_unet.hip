@@ -89,11 +89,14 @@ Dims scaled_dims(const Dims& d, int f) {
     return h;
 }
 
+// 32-bit arithmetic: grids hold fewer than 2^31 voxels
 __device__ inline void vox_xyz(const Dims& d, long long v, int& x, int& y, int& z) {
-    int r = (int)(v / d.nx);
-    x = (int)(v - (long long)r * d.nx);
-    z = r / d.ny;
-    y = r - z * d.ny;
+    unsigned vv = (unsigned)v, nx = (unsigned)d.nx, ny = (unsigned)d.ny;
+    unsigned r = vv / nx;
+    x = (int)(vv - r * nx);
+    unsigned zz = r / ny;
+    y = (int)(r - zz * ny);
+    z = (int)zz;
 }
 
 // ---------------------------------------------------------------- MFMA conv
@@ -200,21 +203,158 @@ __global__ void __launch_bounds__(256) k_conv_mfma(ConvArgs a) {
     }
 }
 
+// 3x3x3 convolution with v_mfma_f32_16x16x32_bf16: K = 32 covers two taps of
+// one 16-channel chunk (lane group g: tap 2p + (g >> 1), channels 8 (g & 1)..+7
+// for both operands).  The next chunk's input tile and weights are fetched
+// into registers while the current chunk computes.  SPLIT 1: 16x4x4 voxels,
+// each wave owns 4 rows of 16 voxels and all taps.  SPLIT 4 (small grids):
+// 16x2x2 voxels, every wave computes all 4 rows for every fourth tap pair and
+// the partial sums are reduced through LDS.
+typedef __bf16 bf8v __attribute__((ext_vector_type(8)));
+
+template <int NB, int SPLIT>
+__global__ void __launch_bounds__(256) k_conv3(ConvArgs a) {
+    constexpr int TY = SPLIT == 1 ? 4 : 2, TZ = TY;
+    constexpr int SX = TX + 2, SY = TY + 2, SZ = TZ + 2;
+    constexpr int NTILE = SX * SY * SZ * 2;  // uint4 per chunk (16 BF16 channels per voxel)
+    constexpr int NW = NB * 27 * 16 * 2;     // uint4 per chunk: rows (nb*27 + t)*16 + n
+    constexpr int RT = (NTILE + 255) / 256, RW = (NW + 255) / 256;
+    constexpr int LDS_STAGE = (NTILE + NW) * 16;
+    constexpr int LDS_RED = SPLIT == 1 ? 0 : 4 * 4 * NB * 64 * 16;
+    constexpr int LDS = LDS_STAGE > LDS_RED ? LDS_STAGE : LDS_RED;
+    __shared__ __attribute__((aligned(16))) unsigned char smem[LDS];
+    uint4* tile = reinterpret_cast<uint4*>(smem);
+    uint4* wl = tile + NTILE;
+    const Dims d = a.d;
+    const int ntx = (d.nx + TX - 1) / TX, nty = (d.ny + TY - 1) / TY;
+    const int b = blockIdx.x;
+    const int x0 = (b % ntx) * TX, y0 = ((b / ntx) % nty) * TY, z0 = (b / (ntx * nty)) * TZ;
+    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+    const int g = lane >> 4, m = lane & 15;
+    const int hi = g >> 1, half = g & 1;
+    const int nb0 = blockIdx.y * NB;
+    const long long wstride = 27LL * a.cin;
+    uint4 rt[RT], rw[RW];
+    auto fetch = [&](int c0) {
+#pragma unroll
+        for (int q = 0; q < RT; q++) {
+            int s = tid + 256 * q;
+            uint4 v = make_uint4(0, 0, 0, 0);
+            if (s < NTILE) {
+                int vox = s >> 1, hf = s & 1;
+                int sx = vox % SX, sy = (vox / SX) % SY, sz = vox / (SX * SY);
+                int gx = x0 + sx - 1, gy = y0 + sy - 1, gz = z0 + sz - 1;
+                if (gx >= 0 && gx < d.nx && gy >= 0 && gy < d.ny && gz >= 0 && gz < d.nz)
+                    v = *reinterpret_cast<const uint4*>(
+                        a.in + (((long long)gz * d.ny + gy) * d.nx + gx) * a.cin + c0 + hf * 8);
+            }
+            rt[q] = v;
+        }
+#pragma unroll
+        for (int q = 0; q < RW; q++) {
+            int s = tid + 256 * q;
+            uint4 v = make_uint4(0, 0, 0, 0);
+            if (s < NW) {
+                int hf = s & 1, row = s >> 1;
+                int n16 = row % 16, t = (row / 16) % 27, nb = row / (16 * 27);
+                int n = (nb0 + nb) * 16 + n16;
+                v = *reinterpret_cast<const uint4*>(a.w + n * wstride + t * a.cin + c0 + hf * 8);
+            }
+            rw[q] = v;
+        }
+    };
+    f4v acc[4][NB];
+#pragma unroll
+    for (int mb = 0; mb < 4; mb++)
+#pragma unroll
+        for (int nb = 0; nb < NB; nb++) acc[mb][nb] = f4v{0.f, 0.f, 0.f, 0.f};
+    fetch(0);
+    for (int c0 = 0; c0 < a.cin; c0 += 16) {
+        __syncthreads();
+#pragma unroll
+        for (int q = 0; q < RT; q++)
+            if (tid + 256 * q < NTILE) tile[tid + 256 * q] = rt[q];
+#pragma unroll
+        for (int q = 0; q < RW; q++)
+            if (tid + 256 * q < NW) wl[tid + 256 * q] = rw[q];
+        __syncthreads();
+        if (c0 + 16 < a.cin) fetch(c0 + 16);
+        for (int p = (SPLIT == 1 ? 0 : wave); p < 14; p += SPLIT) {
+            const int t = 2 * p + hi;
+            const bool live = t < 27;  // the last pair has one tap; its second half is zero
+            const int tt = live ? t : 26;
+            const int ox = tt % 3, oy = (tt / 3) % 3, oz = tt / 9;
+            bf8v bfr[NB];
+#pragma unroll
+            for (int nb = 0; nb < NB; nb++) {
+                uint4 q = wl[((nb * 27 + tt) * 16 + m) * 2 + half];
+                if (!live) q = make_uint4(0, 0, 0, 0);
+                bfr[nb] = __builtin_bit_cast(bf8v, q);
+            }
+#pragma unroll
+            for (int mb = 0; mb < 4; mb++) {
+                int row = SPLIT == 1 ? wave * 4 + mb : mb;
+                int ty = row % TY, tz = row / TY;
+                int sv = ((tz + oz) * SY + (ty + oy)) * SX + (m + ox);
+                uint4 q = tile[sv * 2 + half];
+                if (!live) q = make_uint4(0, 0, 0, 0);
+                bf8v afr = __builtin_bit_cast(bf8v, q);
+#pragma unroll
+                for (int nb = 0; nb < NB; nb++)
+                    acc[mb][nb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(afr, bfr[nb], acc[mb][nb], 0, 0, 0);
+            }
+        }
+    }
+    constexpr int ROWS = SPLIT == 1 ? 4 : 1;
+    if (SPLIT > 1) {
+        __syncthreads();  // every wave is done with the staged chunk
+        f4v* red = reinterpret_cast<f4v*>(smem);  // [wave][row][nb][lane]
+#pragma unroll
+        for (int mb = 0; mb < 4; mb++)
+#pragma unroll
+            for (int nb = 0; nb < NB; nb++) red[((wave * 4 + mb) * NB + nb) * 64 + lane] = acc[mb][nb];
+        __syncthreads();
+#pragma unroll
+        for (int nb = 0; nb < NB; nb++) {  // wave w finalises row w
+            f4v s4 = f4v{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+            for (int w2 = 0; w2 < 4; w2++) s4 += red[((w2 * 4 + wave) * NB + nb) * 64 + lane];
+            acc[0][nb] = s4;
+        }
+    }
+    // D fragment: lane holds rows 4g + j (x offset) of column m (channel)
+#pragma unroll
+    for (int mb = 0; mb < ROWS; mb++) {
+        int row = SPLIT == 1 ? wave * 4 + mb : wave;
+        int gy = y0 + row % TY, gz = z0 + row / TY;
+        if (gy >= d.ny || gz >= d.nz) continue;
+#pragma unroll
+        for (int nb = 0; nb < NB; nb++) {
+            int n = (nb0 + nb) * 16 + m;
+            float bv = a.bias ? a.bias[n] : 0.f;
+#pragma unroll
+            for (int j = 0; j < 4; j++) {
+                int gx = x0 + 4 * g + j;
+                if (gx >= d.nx) continue;
+                long long o = (((long long)gz * d.ny + gy) * d.nx + gx) * a.cout + n;
+                float v = acc[mb][nb][j] + bv;
+                if (a.relu) v = fmaxf(v, 0.f);
+                if (a.mask && !(bf2f(a.mask[o]) > 0.f)) v = 0.f;
+                a.out[o] = f2bf(v);
+            }
+        }
+    }
+}
+
 // ------------------------------------------------------- direct conv layers
 // input layer: FP32 [V][2] -> BF16 [V][C] with bias and ReLU.  nrm (optional,
-// device {sum x^2, count}) scales channel 0 to unit RMS.
+// device {sum x^2, count}) scales channel 0 to unit RMS.  wt is the tap-major
+// copy [27][2][C] of the weights; its indices are uniform across the wave, so
+// the weights arrive through scalar loads.
 template <int C>
 __global__ void __launch_bounds__(256)
     k_conv_in(Dims d, const float* __restrict__ x, const double* __restrict__ nrm,
-              const float* __restrict__ w, const float* __restrict__ bias, bf16* __restrict__ out) {
-    __shared__ float ws[27][2][C];  // transposed: [tap][input channel][output channel]
-    __shared__ float bs[C];
-    for (int e = threadIdx.x; e < C * 54; e += 256) {
-        int c = e / 54, k = e - c * 54;
-        ws[k >> 1][k & 1][c] = w[e];
-    }
-    if (threadIdx.x < C) bs[threadIdx.x] = bias[threadIdx.x];
-    __syncthreads();
+              const float* __restrict__ wt, const float* __restrict__ bias, bf16* __restrict__ out) {
     float sc = 1.f;
     if (nrm && nrm[0] > 0 && nrm[1] > 0) sc = (float)(1.0 / sqrt(nrm[0] / nrm[1]));
     long long v = (long long)blockIdx.x * 256 + threadIdx.x;
@@ -223,14 +363,16 @@ __global__ void __launch_bounds__(256)
     vox_xyz(d, v, xi, yi, zi);
     float acc[C];
 #pragma unroll
-    for (int c = 0; c < C; c++) acc[c] = bs[c];
+    for (int c = 0; c < C; c++) acc[c] = bias[c];
     for (int t = 0; t < 27; t++) {
         int gx = xi + t % 3 - 1, gy = yi + (t / 3) % 3 - 1, gz = zi + t / 9 - 1;
-        if (gx < 0 || gx >= d.nx || gy < 0 || gy >= d.ny || gz < 0 || gz >= d.nz) continue;
-        float2 in = *reinterpret_cast<const float2*>(x + 2 * (((long long)gz * d.ny + gy) * d.nx + gx));
+        float2 in = make_float2(0.f, 0.f);
+        if (gx >= 0 && gx < d.nx && gy >= 0 && gy < d.ny && gz >= 0 && gz < d.nz)
+            in = *reinterpret_cast<const float2*>(x + 2 * (((long long)gz * d.ny + gy) * d.nx + gx));
         in.x *= sc;
+        const float* w0 = wt + t * 2 * C;
 #pragma unroll
-        for (int c = 0; c < C; c++) acc[c] += in.x * ws[t][0][c] + in.y * ws[t][1][c];
+        for (int c = 0; c < C; c++) acc[c] += in.x * w0[c] + in.y * w0[C + c];
     }
     uint4* o = reinterpret_cast<uint4*>(out + v * C);
 #pragma unroll
@@ -243,14 +385,20 @@ __global__ void __launch_bounds__(256)
     }
 }
 
+// wt[t][i][c] = w[c][2 t + i] (input layer, FP32)
+__global__ void k_transpose_in(const float* __restrict__ w, int C, float* __restrict__ wt) {
+    int e = blockIdx.x * 256 + threadIdx.x;
+    if (e >= C * 54) return;
+    int c = e / 54, k = e - c * 54;
+    wt[(k >> 1) * 2 * C + (k & 1) * C + c] = w[e];
+}
+
 // output layer: BF16 [V][C] -> FP32 [V], bias, no activation; w is [27][C]
+// (uniform indices: scalar loads)
 template <int C>
 __global__ void __launch_bounds__(256)
     k_conv_out(Dims d, const bf16* __restrict__ x, const float* __restrict__ w,
                const float* __restrict__ bias, float* __restrict__ y) {
-    __shared__ float ws[27 * C];
-    for (int t = threadIdx.x; t < 27 * C; t += 256) ws[t] = w[t];
-    __syncthreads();
     long long v = (long long)blockIdx.x * 256 + threadIdx.x;
     if (v >= d.v) return;
     int xi, yi, zi;
@@ -260,12 +408,13 @@ __global__ void __launch_bounds__(256)
         int gx = xi + t % 3 - 1, gy = yi + (t / 3) % 3 - 1, gz = zi + t / 9 - 1;
         if (gx < 0 || gx >= d.nx || gy < 0 || gy >= d.ny || gz < 0 || gz >= d.nz) continue;
         const uint4* row = reinterpret_cast<const uint4*>(x + (((long long)gz * d.ny + gy) * d.nx + gx) * C);
+        const float* w0 = w + t * C;
 #pragma unroll
         for (int q = 0; q < C / 8; q++) {
             float f[8];
             unpack8(row[q], f);
 #pragma unroll
-            for (int k = 0; k < 8; k++) acc += f[k] * ws[t * C + 8 * q + k];
+            for (int k = 0; k < 8; k++) acc += f[k] * w0[8 * q + k];
         }
     }
     y[v] = acc;
@@ -736,13 +885,14 @@ __global__ void __launch_bounds__(256) k_scale_f(long long n, float f, float* __
 
 // preconditioner input from the fine residual of A e = t after smoothing z:
 // x2[v] = {-h^2 mean over regular nodes of (t - A z), fraction of other nodes};
-// sums += {x^2, 1}.  Regular nodes are interior (Dirichlet walls), so all six
-// neighbours exist; z is zero on non-regular nodes.
+// sums += {x^2, 1}; res (optional) receives t - A z on the fine grid (0 on
+// non-regular nodes).  Regular nodes are interior (Dirichlet walls), so all
+// six neighbours exist; z is zero on non-regular nodes.
 template <int CF>
 __global__ void __launch_bounds__(256)
     k_res_restrict(Dims df, Dims dc, double h2, const uint8_t* __restrict__ nt,
                    const double* __restrict__ t, const double* __restrict__ z, float* __restrict__ x2,
-                   double* __restrict__ sums) {
+                   double* __restrict__ sums, double* __restrict__ res) {
     double s2 = 0, cnt = 0;
     const long long sj = df.nx, sk = (long long)df.nx * df.ny;
     const double ih2 = 1.0 / h2;
@@ -756,12 +906,15 @@ __global__ void __launch_bounds__(256)
         for (int q = 0; q < CF * CF * CF; q++) {
             int fx = CF * cx + q % CF, fy = CF * cy + (q / CF) % CF, fz = CF * cz + q / (CF * CF);
             long long u = ((long long)fz * df.ny + fy) * df.nx + fx;
+            double rv = 0.0;
             if (nt[u] == NT_REG) {
                 double az = (z[u - 1] + z[u + 1] + z[u - sj] + z[u + sj] + z[u - sk] + z[u + sk] -
                              6.0 * z[u]) * ih2;
-                acc += t[u] - az;
+                rv = t[u] - az;
+                acc += rv;
                 nreg++;
             }
+            if (res) res[u] = rv;
         }
         float xv = nreg ? (float)(-h2 * acc / nreg) : 0.f;
         x2[2 * v] = xv;
@@ -778,66 +931,74 @@ __global__ void __launch_bounds__(256)
     }
 }
 
-// c = P y on regular nodes (0 elsewhere); P = identity (CF 1) or trilinear
-// from aggregate centres (CF 2: weights 3/4 on the own aggregate, 1/4 on the
-// neighbour on the node's side, per axis)
+// (P y) at node (i, j, k): identity (CF 1) or trilinear from aggregate
+// centres (CF 2: per axis weight 3/4 on the own aggregate, 1/4 on the
+// neighbour on the node's side, the own one again at the grid's ends)
 template <int CF>
-__global__ void __launch_bounds__(256)
-    k_prolong(Dims df, Dims dc, const uint8_t* __restrict__ nt, const float* __restrict__ y,
-              double* __restrict__ c) {
-    long long end;
-    for (long long u = xcd_chunk(df.v, 256, end) + threadIdx.x; u < end; u += 256) {
-    if (nt[u] != NT_REG) {
-        c[u] = 0.0;
-        continue;
-    }
-    int i, j, k;
-    vox_xyz(df, u, i, j, k);
-    float val;
-    if (CF == 1) {
-        val = y[u];
-    } else {
-        int c[3] = {i >> 1, j >> 1, k >> 1};
-        int o[3] = {(i & 1) ? 1 : -1, (j & 1) ? 1 : -1, (k & 1) ? 1 : -1};
-        int n[3] = {dc.nx, dc.ny, dc.nz};
-        int cn[3];
+__device__ inline double interp(const Dims& dc, const float* __restrict__ y, int i, int j, int k) {
+    if (CF == 1) return (double)y[((long long)k * dc.ny + j) * dc.nx + i];
+    int c[3] = {i >> 1, j >> 1, k >> 1};
+    int o[3] = {(i & 1) ? 1 : -1, (j & 1) ? 1 : -1, (k & 1) ? 1 : -1};
+    int n[3] = {dc.nx, dc.ny, dc.nz};
+    int cn[3];
 #pragma unroll
-        for (int a = 0; a < 3; a++) {
-            cn[a] = c[a] + o[a];
-            if (cn[a] < 0 || cn[a] >= n[a]) cn[a] = c[a];
-        }
-        val = 0.f;
+    for (int a = 0; a < 3; a++) {
+        cn[a] = c[a] + o[a];
+        if (cn[a] < 0 || cn[a] >= n[a]) cn[a] = c[a];
+    }
+    float val = 0.f;
 #pragma unroll
-        for (int q = 0; q < 8; q++) {
-            int xx = (q & 1) ? cn[0] : c[0], yy = (q & 2) ? cn[1] : c[1], zz = (q & 4) ? cn[2] : c[2];
-            float w = ((q & 1) ? 0.25f : 0.75f) * ((q & 2) ? 0.25f : 0.75f) * ((q & 4) ? 0.25f : 0.75f);
-            val += w * y[((long long)zz * dc.ny + yy) * dc.nx + xx];
-        }
+    for (int q = 0; q < 8; q++) {
+        int xx = (q & 1) ? cn[0] : c[0], yy = (q & 2) ? cn[1] : c[1], zz = (q & 4) ? cn[2] : c[2];
+        float w = ((q & 1) ? 0.25f : 0.75f) * ((q & 2) ? 0.25f : 0.75f) * ((q & 4) ? 0.25f : 0.75f);
+        val += w * y[((long long)zz * dc.ny + yy) * dc.nx + xx];
     }
-    c[u] = (double)val;
-    }
+    return (double)val;
 }
 
-// line search of the coarse correction c: with res = t - A z,
-// out += {c.res, c.A c} over regular nodes (interior: all neighbours exist)
+// c = P y on regular nodes (0 elsewhere), and the line-search sums
+// out += {c.res, c.A c} (two striped scalars: out, out + SSIZE).  c.A c is taken in edge form,
+//   -(1/h^2) [sum over regular-regular edges (c_u - c_v)^2
+//             + sum over regular nodes of c_u^2 per non-regular neighbour],
+// so each node needs c only at its three forward neighbours (interpolated
+// here again rather than read back).
+template <int CF>
 __global__ void __launch_bounds__(256)
-    k_cdots(Dims df, double ih2, const uint8_t* __restrict__ nt, const double* __restrict__ t,
-            const double* __restrict__ z, const double* __restrict__ c, double* __restrict__ out) {
+    k_prolong_dots(Dims df, Dims dc, double ih2, const uint8_t* __restrict__ nt,
+                   const float* __restrict__ y, const double* __restrict__ res, double* __restrict__ c,
+                   double* __restrict__ out) {
     const long long sj = df.nx, sk = (long long)df.nx * df.ny;
     double acc[2] = {0, 0};
     long long end;
     for (long long u = xcd_chunk(df.v, 256, end) + threadIdx.x; u < end; u += 256) {
-        if (nt[u] != NT_REG) continue;
-        double az = (z[u - 1] + z[u + 1] + z[u - sj] + z[u + sj] + z[u - sk] + z[u + sk] - 6.0 * z[u]) * ih2;
-        double ac = (c[u - 1] + c[u + 1] + c[u - sj] + c[u + sj] + c[u - sk] + c[u + sk] - 6.0 * c[u]) * ih2;
-        double cu = c[u];
-        acc[0] += cu * (t[u] - az);
-        acc[1] += cu * ac;
+        if (nt[u] != NT_REG) {
+            c[u] = 0.0;
+            continue;
+        }
+        int i, j, k;
+        vox_xyz(df, u, i, j, k);
+        double cu = interp<CF>(dc, y, i, j, k);
+        c[u] = cu;
+        acc[0] += cu * res[u];
+        double e = 0.0;
+        const long long st[3] = {1, sj, sk};
+        const int di[3] = {1, 0, 0}, dj[3] = {0, 1, 0}, dk[3] = {0, 0, 1};
+#pragma unroll
+        for (int a = 0; a < 3; a++) {
+            if (nt[u + st[a]] == NT_REG) {
+                double cv = interp<CF>(dc, y, i + di[a], j + dj[a], k + dk[a]);
+                e += (cu - cv) * (cu - cv);
+            } else {
+                e += cu * cu;
+            }
+            if (nt[u - st[a]] != NT_REG) e += cu * cu;
+        }
+        acc[1] -= e * ih2;
     }
     block_sums<2>(acc);
     if (threadIdx.x == 0) {
-        atomicAdd(&out[0], acc[0]);
-        atomicAdd(&out[1], acc[1]);
+        sadd(out, acc[0]);
+        sadd(out + SSIZE, acc[1]);
     }
 }
 
@@ -853,6 +1014,7 @@ struct UNetLayer {
     size_t woff, boff;    // offsets into the FP32 parameter vector
     bf16* wb = nullptr;   // BF16 weights [cout][taps*cin]
     bf16* wt = nullptr;   // BF16 transposed, tap-flipped
+    float* wf = nullptr;  // input layer: FP32 tap-major copy [27][2][C]
 };
 
 struct ULevel {
@@ -979,6 +1141,7 @@ static int unet_build(pic_ctx* ctx, const pic_unet_desc* desc) {
         rc |= ualloc(ctx, u, (void**)&u->L[l].wb, nw);
         rc |= ualloc(ctx, u, (void**)&u->L[l].wt, nw);
     }
+    rc |= ualloc(ctx, u, (void**)&u->L[u->l_in].wf, (size_t)54 * C[0] * sizeof(float));
     long long V0 = d.v;
     rc |= ualloc(ctx, u, (void**)&u->x, V0 * 2 * sizeof(float));
     rc |= ualloc(ctx, u, (void**)&u->y, V0 * sizeof(float));
@@ -1027,6 +1190,9 @@ static int unet_sync_weights(pic_ctx* ctx, UNet* u) {
         k_transpose_w<<<grid_for(n, 256), 256, 0, ctx->stream>>>(u->p + ly.woff, ly.cout, ly.cin,
                                                                 ly.taps, ly.wb, ly.wt);
     }
+    const UNetLayer& li = u->L[u->l_in];
+    k_transpose_in<<<grid_for((size_t)54 * li.cout, 256), 256, 0, ctx->stream>>>(u->p + li.woff, li.cout,
+                                                                               li.wf);
     PIC_HIP(ctx, hipGetLastError());
     return PIC_OK;
 }
@@ -1040,10 +1206,14 @@ static void conv(pic_ctx* ctx, const Dims& d, const bf16* in, int cin, const bf1
     int nb = (cout % 32 == 0 && tiles * (cout / 32) >= 512) ? 2 : 1;
     dim3 grid(tiles, cout / 16 / nb);
     if (taps == 27) {
-        if (nb == 2)
-            k_conv_mfma<27, 2><<<grid, 256, 0, ctx->stream>>>(a);
-        else
-            k_conv_mfma<27, 1><<<grid, 256, 0, ctx->stream>>>(a);
+        if (tiles * (cout / 16) < 512) {  // small grid: 16x2x2 tiles, taps split over the waves
+            int t4 = ntx * ((d.ny + 1) / 2) * ((d.nz + 1) / 2);
+            k_conv3<1, 4><<<dim3(t4, cout / 16), 256, 0, ctx->stream>>>(a);
+        } else if (nb == 2) {
+            k_conv3<2, 1><<<grid, 256, 0, ctx->stream>>>(a);
+        } else {
+            k_conv3<1, 1><<<grid, 256, 0, ctx->stream>>>(a);
+        }
     } else {
         if (nb == 2)
             k_conv_mfma<1, 2><<<grid, 256, 0, ctx->stream>>>(a);
@@ -1089,9 +1259,9 @@ static int unet_forward(pic_ctx* ctx, UNet* u, const float* x, const double* nrm
     const int C0 = lv[0].C;
     const UNetLayer& li = L[u->l_in];
     if (C0 == 16)
-        k_conv_in<16><<<ew(d0.v), 256, 0, ctx->stream>>>(d0, x, nrm, P + li.woff, P + li.boff, lv[0].a);
+        k_conv_in<16><<<ew(d0.v), 256, 0, ctx->stream>>>(d0, x, nrm, li.wf, P + li.boff, lv[0].a);
     else
-        k_conv_in<32><<<ew(d0.v), 256, 0, ctx->stream>>>(d0, x, nrm, P + li.woff, P + li.boff, lv[0].a);
+        k_conv_in<32><<<ew(d0.v), 256, 0, ctx->stream>>>(d0, x, nrm, li.wf, P + li.boff, lv[0].a);
     conv(ctx, d0, lv[0].a, C0, L[u->l_b[0]].wb, P + L[u->l_b[0]].boff, lv[0].s, C0, 27, 1, nullptr);
     for (int l = 1; l < D; l++) {
         const Dims& d = lv[l].d;
@@ -1399,14 +1569,15 @@ bool pic_unet_ready(pic_ctx* ctx) {
     return u && u->trained && uniform_dx(ctx->g);
 }
 
-// Coarse correction of the two-level preconditioner: the residual t - A z
-// (t: the fine right-hand side, z: the smoothed approximation, both device
-// FP64) is restricted to the network grid, the network is applied and its
-// output interpolated to c (device FP64).  dots (device, 2 doubles) receives
-// {c.(t - A z), c.A c}, from which the caller takes the A-norm optimal step
-// omega = dots[0] / dots[1] for z += omega c.  Only for Dirichlet-walled
-// uniform grids (every regular node is interior).
-int pic_unet_coarse_correct(pic_ctx* ctx, const double* t, const double* z, double* c, double* dots) {
+// Coarse correction of the two-level preconditioner: the residual
+// res = t - A z (t: the fine right-hand side, z: the smoothed approximation,
+// device FP64; res is stored) is restricted to the network grid, the network
+// is applied and its output interpolated to c (device FP64).  dots (device,
+// two striped scalars) receives {c.res, c.A c}, from which the caller takes
+// the A-norm optimal step omega = (c.res) / (c.A c) for z += omega c.  Only for
+// Dirichlet-walled uniform grids (every regular node is interior).
+int pic_unet_coarse_correct(pic_ctx* ctx, const double* t, const double* z, double* c, double* res,
+                            double* dots) {
     UNet* u = get_unet(ctx);
     if (!u) return pic_fail(ctx, PIC_ERR_STATE, "no U-Net");
     const GridC& g = ctx->g;
@@ -1415,19 +1586,18 @@ int pic_unet_coarse_correct(pic_ctx* ctx, const double* t, const double* z, doub
     double h2 = g.dx[0] * g.dx[0];
     double* sums = ctx->scratch + 64;  // input normalisation
     PIC_HIP(ctx, hipMemsetAsync(sums, 0, 2 * sizeof(double), ctx->stream));
-    PIC_HIP(ctx, hipMemsetAsync(dots, 0, 2 * sizeof(double), ctx->stream));
+    PIC_HIP(ctx, hipMemsetAsync(dots, 0, 2 * SSIZE * sizeof(double), ctx->stream));
     unsigned nbc = xcd_blocks(dc.v, 256), nbf = xcd_blocks(df.v, 256);
     if (u->coarsen == 2)
-        k_res_restrict<2><<<nbc, 256, 0, ctx->stream>>>(df, dc, h2, ctx->ntype, t, z, u->x, sums);
+        k_res_restrict<2><<<nbc, 256, 0, ctx->stream>>>(df, dc, h2, ctx->ntype, t, z, u->x, sums, res);
     else
-        k_res_restrict<1><<<nbc, 256, 0, ctx->stream>>>(df, dc, h2, ctx->ntype, t, z, u->x, sums);
+        k_res_restrict<1><<<nbc, 256, 0, ctx->stream>>>(df, dc, h2, ctx->ntype, t, z, u->x, sums, res);
     int rc = unet_forward(ctx, u, u->x, sums);
     if (rc) return rc;
     if (u->coarsen == 2)
-        k_prolong<2><<<nbf, 256, 0, ctx->stream>>>(df, dc, ctx->ntype, u->y, c);
+        k_prolong_dots<2><<<nbf, 256, 0, ctx->stream>>>(df, dc, 1.0 / h2, ctx->ntype, u->y, res, c, dots);
     else
-        k_prolong<1><<<nbf, 256, 0, ctx->stream>>>(df, dc, ctx->ntype, u->y, c);
-    k_cdots<<<nbf, 256, 0, ctx->stream>>>(df, 1.0 / h2, ctx->ntype, t, z, c, dots);
+        k_prolong_dots<1><<<nbf, 256, 0, ctx->stream>>>(df, dc, 1.0 / h2, ctx->ntype, u->y, res, c, dots);
     PIC_HIP(ctx, hipGetLastError());
     return PIC_OK;
 }
@@ -1526,9 +1696,9 @@ extern "C" int pic_unet_train(pic_ctx* ctx, const pic_unet_train_desc* td,
         (void)hipMemsetAsync(ss, 0, 3 * sizeof(double), ctx->stream);
         unsigned nbc = xcd_blocks(Vc, 256), nbf = xcd_blocks(V, 256);
         if (u->coarsen == 2)
-            k_res_restrict<2><<<nbc, 256, 0, ctx->stream>>>(df, dc, dx2, ctx->ntype, tf, zp, xd, ss);
+            k_res_restrict<2><<<nbc, 256, 0, ctx->stream>>>(df, dc, dx2, ctx->ntype, tf, zp, xd, ss, nullptr);
         else
-            k_res_restrict<1><<<nbc, 256, 0, ctx->stream>>>(df, dc, dx2, ctx->ntype, tf, zp, xd, ss);
+            k_res_restrict<1><<<nbc, 256, 0, ctx->stream>>>(df, dc, dx2, ctx->ntype, tf, zp, xd, ss, nullptr);
         k_res_rho<<<nbf, 256, 0, ctx->stream>>>(df, 1.0 / dx2, ctx->ntype, tf, zp, ctx->rho);
         (void)hipMemsetAsync(ctx->phi, 0, V * 8, ctx->stream);
         pic_solver_stats st;

@@ -346,6 +346,17 @@ class Context:
         self.unet_voxels = (self.ni // coarsen) * (self.nj // coarsen) * (self.nk // coarsen)
         return n.value
 
+    def unet_load(self, path):
+        """create the network described by path + '.json' and load its FP32
+        weights (the layout of unet_get_params) from path"""
+        import json
+
+        with open(path + ".json") as f:
+            meta = json.load(f)
+        self.unet_create(meta["channels"], seed=1, coarsen=meta["coarsen"])
+        self.unet_set_params(np.fromfile(path, dtype="<f4"))
+        return meta
+
     def unet_get_params(self):
         p = np.zeros(self.unet_nparam, np.float32)
         self._ck(self.L.pic_unet_get_params(self.h, p, p.size))

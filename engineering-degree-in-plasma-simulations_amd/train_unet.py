@@ -4,10 +4,12 @@
         [--nodes 128] [--channels 32 64 64 64] [--coarsen 2] [--rounds 10] \
         [--samples 500] [--epochs 2] [--out FILE]
 
-Each round generates fresh (charge density, Dirichlet mask) -> phi pairs with
-the plain red-black SOR solver on the device (random Gaussian blobs plus white
-noise, the particle-noise component of a PIC charge density), restricts them
-to the network grid and runs Adam over them.  The weights (FP32, the
+Each round generates fresh training pairs on the device: a residual t
+(random Gaussian blobs plus white noise, the particle-noise component of a
+PIC charge density), what one red-black Gauss-Seidel sweep leaves of it
+(restricted to the network grid: the input) and the exact error of that
+remainder from the plain red-black SOR solver (restricted: the target); then
+Adam runs over them.  The weights (FP32, the
 parameter order of pic_unet_get_params) go to data/unet_<nodes>.f32 with a
 JSON sidecar holding the architecture; bench.py and the learned solver tests
 load them.  Finally the trained network is compared on a held-out density
@@ -29,14 +31,6 @@ import pic_gpu as pg  # noqa: E402
 
 def weights_path(nodes):
     return os.path.join(HERE, "data", "unet_%d.f32" % nodes)
-
-
-def load_network(ctx, path):
-    """create the network described by path's sidecar and load its weights"""
-    meta = json.load(open(path + ".json"))
-    ctx.unet_create(meta["channels"], seed=1, coarsen=meta["coarsen"])
-    ctx.unet_set_params(np.fromfile(path, dtype="<f4"))
-    return meta
 
 
 def held_out_density(n, seed=12345):

@@ -144,6 +144,28 @@ def test_gpu_unet_forward_and_gradient_match_oracle(channels, n):
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("channels", [(32, 32), (16, 32)])
+def test_gpu_unet_forward_large_grid_matches_oracle(channels):
+    """64^3: the large-grid convolution variants (one and two N-blocks of 16
+    output channels per workgroup) against the FP64 oracle."""
+    import pic_gpu as pg
+
+    n = 64
+    print("U-Net forward parity", channels, n, flush=True)
+    ctx = _ctx(pg, n)
+    ctx.unet_create(channels, seed=5)
+    p = U.init_params(channels, seed=13).astype(np.float32)
+    ctx.unet_set_params(p)
+    x, _ = _inputs(n, 17)
+    y = ctx.unet_forward(x.astype(np.float32))
+    y_ref = U.forward(_bf16(p).astype(np.float64), x, channels)
+    err = np.abs(y - y_ref.ravel())
+    scale = np.max(np.abs(y_ref))
+    assert np.max(err) <= 3e-2 * scale
+    assert np.sqrt(np.mean(err ** 2)) <= 1e-2 * np.sqrt(np.mean(y_ref ** 2)) + 1e-3 * scale
+
+
+@pytest.mark.gpu
 @pytest.mark.parametrize("coarsen", [1, 2])
 def test_gpu_learned_preconditioner_matches_oracle(coarsen):
     """M(r) on the device (FP64 smoothing, BF16 network) against the FP64
