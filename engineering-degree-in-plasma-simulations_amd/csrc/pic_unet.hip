@@ -112,6 +112,7 @@ struct ConvArgs {
     const bf16* mask;  // optional: keep outputs where mask[v][n] > 0 (mask has cout channels)
     int relu, accumulate;
     Dims d;
+    float* outf = nullptr;  // k_conv3 only: write channel 0 as FP32 outf[v] (+ bias[0]), not out
 };
 
 // NB N-blocks of 16 output channels per workgroup (cout/16 divisible by NB).
@@ -212,7 +213,7 @@ __global__ void __launch_bounds__(256) k_conv_mfma(ConvArgs a) {
 // the partial sums are reduced through LDS.
 typedef __bf16 bf8v __attribute__((ext_vector_type(8)));
 
-template <int NB, int SPLIT>
+template <int NB, int SPLIT, bool PF>
 __global__ void __launch_bounds__(256) k_conv3(ConvArgs a) {
     constexpr int TY = SPLIT == 1 ? 4 : 2, TZ = TY;
     constexpr int SX = TX + 2, SY = TY + 2, SZ = TZ + 2;
@@ -234,8 +235,9 @@ __global__ void __launch_bounds__(256) k_conv3(ConvArgs a) {
     const int hi = g >> 1, half = g & 1;
     const int nb0 = blockIdx.y * NB;
     const long long wstride = 27LL * a.cin;
-    uint4 rt[RT], rw[RW];
+    uint4 rt[PF ? RT : 1], rw[PF ? RW : 1];
     auto fetch = [&](int c0) {
+        if (!PF) return;
 #pragma unroll
         for (int q = 0; q < RT; q++) {
             int s = tid + 256 * q;
@@ -248,7 +250,7 @@ __global__ void __launch_bounds__(256) k_conv3(ConvArgs a) {
                     v = *reinterpret_cast<const uint4*>(
                         a.in + (((long long)gz * d.ny + gy) * d.nx + gx) * a.cin + c0 + hf * 8);
             }
-            rt[q] = v;
+            rt[PF ? q : 0] = v;
         }
 #pragma unroll
         for (int q = 0; q < RW; q++) {
@@ -260,7 +262,7 @@ __global__ void __launch_bounds__(256) k_conv3(ConvArgs a) {
                 int n = (nb0 + nb) * 16 + n16;
                 v = *reinterpret_cast<const uint4*>(a.w + n * wstride + t * a.cin + c0 + hf * 8);
             }
-            rw[q] = v;
+            rw[PF ? q : 0] = v;
         }
     };
     f4v acc[4][NB];
@@ -268,17 +270,40 @@ __global__ void __launch_bounds__(256) k_conv3(ConvArgs a) {
     for (int mb = 0; mb < 4; mb++)
 #pragma unroll
         for (int nb = 0; nb < NB; nb++) acc[mb][nb] = f4v{0.f, 0.f, 0.f, 0.f};
-    fetch(0);
+    // without prefetch: stage one element at a time (fewer registers)
+    auto stage = [&](int c0) {
+        for (int s = tid; s < NTILE; s += 256) {
+            int vox = s >> 1, hf = s & 1;
+            int sx = vox % SX, sy = (vox / SX) % SY, sz = vox / (SX * SY);
+            int gx = x0 + sx - 1, gy = y0 + sy - 1, gz = z0 + sz - 1;
+            uint4 v = make_uint4(0, 0, 0, 0);
+            if (gx >= 0 && gx < d.nx && gy >= 0 && gy < d.ny && gz >= 0 && gz < d.nz)
+                v = *reinterpret_cast<const uint4*>(
+                    a.in + (((long long)gz * d.ny + gy) * d.nx + gx) * a.cin + c0 + hf * 8);
+            tile[s] = v;
+        }
+        for (int s = tid; s < NW; s += 256) {
+            int hf = s & 1, row = s >> 1;
+            int n16 = row % 16, t = (row / 16) % 27, nb = row / (16 * 27);
+            int n = (nb0 + nb) * 16 + n16;
+            wl[s] = *reinterpret_cast<const uint4*>(a.w + n * wstride + t * a.cin + c0 + hf * 8);
+        }
+    };
+    if (PF) fetch(0);
     for (int c0 = 0; c0 < a.cin; c0 += 16) {
         __syncthreads();
+        if (PF) {
 #pragma unroll
-        for (int q = 0; q < RT; q++)
-            if (tid + 256 * q < NTILE) tile[tid + 256 * q] = rt[q];
+            for (int q = 0; q < (PF ? RT : 1); q++)
+                if (tid + 256 * q < NTILE) tile[tid + 256 * q] = rt[q];
 #pragma unroll
-        for (int q = 0; q < RW; q++)
-            if (tid + 256 * q < NW) wl[tid + 256 * q] = rw[q];
+            for (int q = 0; q < (PF ? RW : 1); q++)
+                if (tid + 256 * q < NW) wl[tid + 256 * q] = rw[q];
+        } else {
+            stage(c0);
+        }
         __syncthreads();
-        if (c0 + 16 < a.cin) fetch(c0 + 16);
+        if (PF && c0 + 16 < a.cin) fetch(c0 + 16);
         for (int p = (SPLIT == 1 ? 0 : wave); p < 14; p += SPLIT) {
             const int t = 2 * p + hi;
             const bool live = t < 27;  // the last pair has one tap; its second half is zero
@@ -331,6 +356,16 @@ __global__ void __launch_bounds__(256) k_conv3(ConvArgs a) {
 #pragma unroll
         for (int nb = 0; nb < NB; nb++) {
             int n = (nb0 + nb) * 16 + m;
+            if (a.outf) {  // single FP32 output channel (the rest of the 16 are padding)
+                if (n != 0) continue;
+                float b0 = a.bias ? a.bias[0] : 0.f;
+#pragma unroll
+                for (int j = 0; j < 4; j++) {
+                    int gx = x0 + 4 * g + j;
+                    if (gx < d.nx) a.outf[((long long)gz * d.ny + gy) * d.nx + gx] = acc[mb][nb][j] + b0;
+                }
+                continue;
+            }
             float bv = a.bias ? a.bias[n] : 0.f;
 #pragma unroll
             for (int j = 0; j < 4; j++) {
@@ -346,80 +381,38 @@ __global__ void __launch_bounds__(256) k_conv3(ConvArgs a) {
     }
 }
 
-// ------------------------------------------------------- direct conv layers
-// input layer: FP32 [V][2] -> BF16 [V][C] with bias and ReLU.  nrm (optional,
-// device {sum x^2, count}) scales channel 0 to unit RMS.  wt is the tap-major
-// copy [27][2][C] of the weights; its indices are uniform across the wave, so
-// the weights arrive through scalar loads.
-template <int C>
+// network input for the MFMA input layer: BF16 [V][16] = {x0 * sc, x1, 0...}
+// with sc = 1 / RMS from nrm = {sum x^2, count} (sc = 1 without nrm)
 __global__ void __launch_bounds__(256)
-    k_conv_in(Dims d, const float* __restrict__ x, const double* __restrict__ nrm,
-              const float* __restrict__ wt, const float* __restrict__ bias, bf16* __restrict__ out) {
+    k_pack_input(long long V, const float* __restrict__ x, const double* __restrict__ nrm,
+                 bf16* __restrict__ out) {
+    long long v = (long long)blockIdx.x * 256 + threadIdx.x;
+    if (v >= V) return;
     float sc = 1.f;
     if (nrm && nrm[0] > 0 && nrm[1] > 0) sc = (float)(1.0 / sqrt(nrm[0] / nrm[1]));
-    long long v = (long long)blockIdx.x * 256 + threadIdx.x;
-    if (v >= d.v) return;
-    int xi, yi, zi;
-    vox_xyz(d, v, xi, yi, zi);
-    float acc[C];
-#pragma unroll
-    for (int c = 0; c < C; c++) acc[c] = bias[c];
-    for (int t = 0; t < 27; t++) {
-        int gx = xi + t % 3 - 1, gy = yi + (t / 3) % 3 - 1, gz = zi + t / 9 - 1;
-        float2 in = make_float2(0.f, 0.f);
-        if (gx >= 0 && gx < d.nx && gy >= 0 && gy < d.ny && gz >= 0 && gz < d.nz)
-            in = *reinterpret_cast<const float2*>(x + 2 * (((long long)gz * d.ny + gy) * d.nx + gx));
-        in.x *= sc;
-        const float* w0 = wt + t * 2 * C;
-#pragma unroll
-        for (int c = 0; c < C; c++) acc[c] += in.x * w0[c] + in.y * w0[C + c];
-    }
-    uint4* o = reinterpret_cast<uint4*>(out + v * C);
-#pragma unroll
-    for (int q = 0; q < C / 8; q++) {
-        const float* a8 = acc + 8 * q;
-        o[q] = make_uint4(pack2(fmaxf(a8[0], 0.f), fmaxf(a8[1], 0.f)),
-                          pack2(fmaxf(a8[2], 0.f), fmaxf(a8[3], 0.f)),
-                          pack2(fmaxf(a8[4], 0.f), fmaxf(a8[5], 0.f)),
-                          pack2(fmaxf(a8[6], 0.f), fmaxf(a8[7], 0.f)));
-    }
+    float2 in = *reinterpret_cast<const float2*>(x + 2 * v);
+    uint4* o = reinterpret_cast<uint4*>(out + v * 16);
+    o[0] = make_uint4(pack2(in.x * sc, in.y), 0u, 0u, 0u);
+    o[1] = make_uint4(0u, 0u, 0u, 0u);
 }
 
-// wt[t][i][c] = w[c][2 t + i] (input layer, FP32)
-__global__ void k_transpose_in(const float* __restrict__ w, int C, float* __restrict__ wt) {
+// BF16 MFMA copies of the direct layers' weights, zero-padded to 16 channels:
+// input layer wi[c][t*16 + i] = w[c][2t + i] (i < 2); output layer
+// wo[n][t*C + c] = (n == 0) w[t*C + c]
+__global__ void k_pad_io_w(const float* __restrict__ win, const float* __restrict__ wout, int C,
+                           bf16* __restrict__ wi, bf16* __restrict__ wo) {
     int e = blockIdx.x * 256 + threadIdx.x;
-    if (e >= C * 54) return;
-    int c = e / 54, k = e - c * 54;
-    wt[(k >> 1) * 2 * C + (k & 1) * C + c] = w[e];
-}
-
-// output layer: BF16 [V][C] -> FP32 [V], bias, no activation; w is [27][C]
-// (uniform indices: scalar loads)
-template <int C>
-__global__ void __launch_bounds__(256)
-    k_conv_out(Dims d, const bf16* __restrict__ x, const float* __restrict__ w,
-               const float* __restrict__ bias, float* __restrict__ y) {
-    long long v = (long long)blockIdx.x * 256 + threadIdx.x;
-    if (v >= d.v) return;
-    int xi, yi, zi;
-    vox_xyz(d, v, xi, yi, zi);
-    float acc = bias[0];
-    for (int t = 0; t < 27; t++) {
-        int gx = xi + t % 3 - 1, gy = yi + (t / 3) % 3 - 1, gz = zi + t / 9 - 1;
-        if (gx < 0 || gx >= d.nx || gy < 0 || gy >= d.ny || gz < 0 || gz >= d.nz) continue;
-        const uint4* row = reinterpret_cast<const uint4*>(x + (((long long)gz * d.ny + gy) * d.nx + gx) * C);
-        const float* w0 = w + t * C;
-#pragma unroll
-        for (int q = 0; q < C / 8; q++) {
-            float f[8];
-            unpack8(row[q], f);
-#pragma unroll
-            for (int k = 0; k < 8; k++) acc += f[k] * w0[8 * q + k];
-        }
+    if (e < C * 27 * 16) {
+        int c = e / (27 * 16), r = e - c * 27 * 16, t = r / 16, i = r % 16;
+        wi[e] = i < 2 ? f2bf(win[c * 54 + 2 * t + i]) : (bf16)0;
     }
-    y[v] = acc;
+    if (e < 16 * 27 * C) {
+        int n = e / (27 * C), k = e - n * 27 * C;
+        wo[e] = n == 0 ? f2bf(wout[k]) : (bf16)0;
+    }
 }
 
+// ------------------------------------------------------- direct conv layers
 // d(input of the output layer) = sum_t dy[v - off t] w[t][c], times the ReLU
 // mask of that input
 template <int C>
@@ -1014,7 +1007,6 @@ struct UNetLayer {
     size_t woff, boff;    // offsets into the FP32 parameter vector
     bf16* wb = nullptr;   // BF16 weights [cout][taps*cin]
     bf16* wt = nullptr;   // BF16 transposed, tap-flipped
-    float* wf = nullptr;  // input layer: FP32 tap-major copy [27][2][C]
 };
 
 struct ULevel {
@@ -1033,6 +1025,9 @@ struct UNet {
     size_t nparam = 0;
     float *p = nullptr, *g = nullptr, *m = nullptr, *v = nullptr;
     float* x = nullptr;   // network input [V0][2]
+    bf16* xin = nullptr;  // the same packed for the MFMA input layer [V0][16]
+    bf16* wi_pad = nullptr;  // input layer weights, BF16, channels padded to 16
+    bf16* wo_pad = nullptr;  // output layer weights, BF16, outputs padded to 16
     float* y = nullptr;   // [V0]
     float* dy = nullptr;  // [V0]
     long long adam_t = 0;
@@ -1141,7 +1136,9 @@ static int unet_build(pic_ctx* ctx, const pic_unet_desc* desc) {
         rc |= ualloc(ctx, u, (void**)&u->L[l].wb, nw);
         rc |= ualloc(ctx, u, (void**)&u->L[l].wt, nw);
     }
-    rc |= ualloc(ctx, u, (void**)&u->L[u->l_in].wf, (size_t)54 * C[0] * sizeof(float));
+    rc |= ualloc(ctx, u, (void**)&u->wi_pad, (size_t)C[0] * 27 * 16 * sizeof(bf16));
+    rc |= ualloc(ctx, u, (void**)&u->wo_pad, (size_t)16 * 27 * C[0] * sizeof(bf16));
+    rc |= ualloc(ctx, u, (void**)&u->xin, (size_t)d.v * 16 * sizeof(bf16));
     long long V0 = d.v;
     rc |= ualloc(ctx, u, (void**)&u->x, V0 * 2 * sizeof(float));
     rc |= ualloc(ctx, u, (void**)&u->y, V0 * sizeof(float));
@@ -1190,29 +1187,45 @@ static int unet_sync_weights(pic_ctx* ctx, UNet* u) {
         k_transpose_w<<<grid_for(n, 256), 256, 0, ctx->stream>>>(u->p + ly.woff, ly.cout, ly.cin,
                                                                 ly.taps, ly.wb, ly.wt);
     }
-    const UNetLayer& li = u->L[u->l_in];
-    k_transpose_in<<<grid_for((size_t)54 * li.cout, 256), 256, 0, ctx->stream>>>(u->p + li.woff, li.cout,
-                                                                               li.wf);
+    const UNetLayer &li = u->L[u->l_in], &lo = u->L[u->l_out];
+    k_pad_io_w<<<grid_for((size_t)432 * li.cout, 256), 256, 0, ctx->stream>>>(
+        u->p + li.woff, u->p + lo.woff, li.cout, u->wi_pad, u->wo_pad);
     PIC_HIP(ctx, hipGetLastError());
     return PIC_OK;
 }
 
 static void conv(pic_ctx* ctx, const Dims& d, const bf16* in, int cin, const bf16* w,
-                 const float* bias, bf16* out, int cout, int taps, int relu, const bf16* mask) {
+                 const float* bias, bf16* out, int cout, int taps, int relu, const bf16* mask,
+                 float* outf = nullptr) {
     ConvArgs a{in, cin, w, bias, out, cout, mask, relu, 0, d};
+    a.outf = outf;
     int ntx = (d.nx + TX - 1) / TX, nty = (d.ny + TY - 1) / TY, ntz = (d.nz + TZ - 1) / TZ;
     int tiles = ntx * nty * ntz;
     // two N-blocks per workgroup when that still leaves >= 512 workgroups
     int nb = (cout % 32 == 0 && tiles * (cout / 32) >= 512) ? 2 : 1;
     dim3 grid(tiles, cout / 16 / nb);
+    // register prefetch of the next chunk pays with several chunks despite its
+    // registers (measured: 11.5 ms learned solve at 128^3 vs 13.1 ms without)
+    const bool pf = cin > 16;
     if (taps == 27) {
         if (tiles * (cout / 16) < 512) {  // small grid: 16x2x2 tiles, taps split over the waves
             int t4 = ntx * ((d.ny + 1) / 2) * ((d.nz + 1) / 2);
-            k_conv3<1, 4><<<dim3(t4, cout / 16), 256, 0, ctx->stream>>>(a);
+            if (pf)
+                k_conv3<1, 4, true><<<dim3(t4, cout / 16), 256, 0, ctx->stream>>>(a);
+            else
+                k_conv3<1, 4, false><<<dim3(t4, cout / 16), 256, 0, ctx->stream>>>(a);
         } else if (nb == 2) {
-            k_conv3<2, 1><<<grid, 256, 0, ctx->stream>>>(a);
+            dim3 g2(tiles, cout / 32);
+            if (pf)
+                k_conv3<2, 1, true><<<g2, 256, 0, ctx->stream>>>(a);
+            else
+                k_conv3<2, 1, false><<<g2, 256, 0, ctx->stream>>>(a);
         } else {
-            k_conv3<1, 1><<<grid, 256, 0, ctx->stream>>>(a);
+            dim3 g1(tiles, cout / 16);
+            if (pf)
+                k_conv3<1, 1, true><<<g1, 256, 0, ctx->stream>>>(a);
+            else
+                k_conv3<1, 1, false><<<g1, 256, 0, ctx->stream>>>(a);
         }
     } else {
         if (nb == 2)
@@ -1258,10 +1271,8 @@ static int unet_forward(pic_ctx* ctx, UNet* u, const float* x, const double* nrm
     const Dims& d0 = lv[0].d;
     const int C0 = lv[0].C;
     const UNetLayer& li = L[u->l_in];
-    if (C0 == 16)
-        k_conv_in<16><<<ew(d0.v), 256, 0, ctx->stream>>>(d0, x, nrm, li.wf, P + li.boff, lv[0].a);
-    else
-        k_conv_in<32><<<ew(d0.v), 256, 0, ctx->stream>>>(d0, x, nrm, li.wf, P + li.boff, lv[0].a);
+    k_pack_input<<<ew(d0.v), 256, 0, ctx->stream>>>(d0.v, x, nrm, u->xin);
+    conv(ctx, d0, u->xin, 16, u->wi_pad, P + li.boff, lv[0].a, C0, 27, 1, nullptr);
     conv(ctx, d0, lv[0].a, C0, L[u->l_b[0]].wb, P + L[u->l_b[0]].boff, lv[0].s, C0, 27, 1, nullptr);
     for (int l = 1; l < D; l++) {
         const Dims& d = lv[l].d;
@@ -1281,10 +1292,7 @@ static int unet_forward(pic_ctx* ctx, UNet* u, const float* x, const double* nrm
              nullptr);
     }
     const UNetLayer& lo = L[u->l_out];
-    if (C0 == 16)
-        k_conv_out<16><<<ew(d0.v), 256, 0, ctx->stream>>>(d0, lv[0].h, P + lo.woff, P + lo.boff, u->y);
-    else
-        k_conv_out<32><<<ew(d0.v), 256, 0, ctx->stream>>>(d0, lv[0].h, P + lo.woff, P + lo.boff, u->y);
+    conv(ctx, d0, lv[0].h, C0, u->wo_pad, P + lo.boff, nullptr, 16, 27, 0, nullptr, u->y);
     PIC_HIP(ctx, hipGetLastError());
     return PIC_OK;
 }
