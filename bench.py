@@ -8,8 +8,10 @@ weight w = n0 L^3 / N with n0 = 1e12 m^-3, dt = 1e-10 s.  One step is
   push e-, push O+  (CIC gather + kick + drift + absorb + sort histogram)
   sort e-, sort O+  (counting sort by cell, every step)
   deposit e-, O+    (number density), charge density,
-  Poisson solve to a 1e-6 relative residual (FP64 residual, FP32 inner PCG,
-  warm-started from the previous phi), E-field.
+  Poisson solve to a 1e-6 relative residual (FP64), warm-started from the
+  previous phi: the learned-preconditioned flexible CG when its trained
+  weights are present (the same step with red-black SOR is timed beside
+  it), E-field.
 value = particle pushes per second over the K timed steps, inputs resident
 in HBM.  Also reported: per-kernel HIP-event times, the roofline of the
 dominant kernel, cold-start Poisson time to 1e-6 for each solver, and the
@@ -42,6 +44,9 @@ ALG_BYTES = {
     "pcg": 44.0,       # FP32 inner PCG: per node and iteration (88 B in FP64)
     "sor": 24.0,       # FP64 phi r/w + rho read per node and sweep
 }
+# the learned solver's flexible-CG vector updates per node and iteration (FP64):
+# p, q from z and the previous p (32 B), then x, r updated from p, q (48 B)
+ALG_BYTES_FCG = 80.0
 KERNEL_NAMES = {
     "push": "k_push", "scatter": "k_scatter", "deposit": "k_deposit<1>",
     "pcg": "k_pcg1<float> + k_pcg2<float>", "sor": "k_sor",
@@ -125,8 +130,8 @@ def main():
     ap.add_argument("--cpu-particles", type=int, default=8_000_000)
     ap.add_argument("--cpu-steps", type=int, default=8)
     ap.add_argument("--copy-calibration", action="store_true")
-    ap.add_argument("--step-solver", default="sor_rb",
-                    choices=["sor_rb", "pcg_mixed", "pcg_fp64", "learned"])
+    ap.add_argument("--step-solver", default="auto",  # learned when its weights exist, else sor_rb
+                    choices=["auto", "sor_rb", "pcg_mixed", "pcg_fp64", "learned"])
     args = ap.parse_args()
     if args.copy_calibration:
         copy_calibration()
@@ -174,12 +179,20 @@ def main():
         ctx.unet_load(weights)
     elif args.step_solver == "learned":
         raise SystemExit("--step-solver learned needs %s(.json)" % weights)
+    step_solver = args.step_solver
+    if step_solver == "auto":
+        step_solver = "learned" if have_net else "sor_rb"
     omega_opt = 2.0 / (1.0 + np.sin(np.pi / (n - 1)))
-    solve_kw = {"sor_rb": dict(method=pg.SOLVER_SOR_RB, omega=omega_opt, check_every=10),
-                "pcg_mixed": dict(method=pg.SOLVER_PCG_MIXED),
-                "pcg_fp64": dict(method=pg.SOLVER_PCG),
-                "learned": dict(method=pg.SOLVER_LEARNED, check_every=2)}[args.step_solver]
-    solve_kw.update(tol=1e-6, tol_kind=pg.TOL_REL, max_it=20000)
+
+    def solver_kw(name):
+        kw = {"sor_rb": dict(method=pg.SOLVER_SOR_RB, omega=omega_opt, check_every=10),
+              "pcg_mixed": dict(method=pg.SOLVER_PCG_MIXED),
+              "pcg_fp64": dict(method=pg.SOLVER_PCG),
+              "learned": dict(method=pg.SOLVER_LEARNED, check_every=2)}[name]
+        kw.update(tol=1e-6, tol_kind=pg.TOL_REL, max_it=20000)
+        return kw
+
+    solve_kw = solver_kw(step_solver)
     ctx.poisson(**solve_kw)
     ctx.efield()
     print("loaded %d + %d particles" % (ctx.count(se), ctx.count(si)), file=sys.stderr)
@@ -221,7 +234,7 @@ def main():
 
     solve_iters = []
 
-    def step():
+    def step(kw=solve_kw):
         for s in species:
             ctx.push(s, dt)
         for s in species:
@@ -229,7 +242,7 @@ def main():
         for s in species:
             ctx.deposit(s)
         ctx.charge_density()
-        st = ctx.poisson(**solve_kw)
+        st = ctx.poisson(**kw)
         if not st["converged"]:
             raise RuntimeError("Poisson solve did not converge: %r" % st)
         solve_iters.append(st["iterations"])
@@ -253,6 +266,23 @@ def main():
     K = args.steps
     n_alive = ctx.count(se) + ctx.count(si)
     value = 2 * N * K / T  # pushes issued (both species, every step)
+    iters_main = float(np.mean(solve_iters)) if solve_iters else None
+
+    # the same step with the plain red-black SOR solve, for comparison
+    compare = None
+    if step_solver != "sor_rb":
+        kw = solver_kw("sor_rb")
+        step(kw)
+        solve_iters.clear()
+        ctx.sync()
+        c0 = time.perf_counter()
+        for k in range(K):
+            step(kw)
+        ctx.sync()
+        c1 = time.perf_counter()
+        compare = {"sor_rb": {"value": 2 * N * K / (c1 - c0), "ms_per_step": (c1 - c0) / K * 1e3,
+                              "poisson_iterations_per_step": float(np.mean(solve_iters))}}
+        print("same step with sor_rb: %.3e pushes/s" % compare["sor_rb"]["value"], file=sys.stderr)
 
     stages = {k: {"ms_per_step": round(v[0] / K, 4), "launches": v[1]}
               for k, v in prof.items() if v[1]}
@@ -267,7 +297,10 @@ def main():
         units = ctx.nn  # one profiled launch = one full red-black sweep
     else:
         units = N
-    alg = ALG_BYTES[dom] * units
+    per_unit = ALG_BYTES[dom]
+    if dom == "pcg" and step_solver == "learned":
+        per_unit = ALG_BYTES_FCG
+    alg = per_unit * units
     achieved = alg / avg_s / 1e9
     traffic = None
     tpath = os.path.join(ROOT, "profiles", "pmc_traffic.json")
@@ -291,11 +324,15 @@ def main():
         "config": {"workload": "scaled plain PIC, %d^3 nodes, 2 x %d particles, counting sort "
                                "every step, Poisson to 1e-6 rel. residual every step" % (n, N),
                    "grid_nodes": n ** 3, "particles_per_species": N, "dt": dt,
-                   "poisson_step_solver": args.step_solver},
+                   "poisson_step_solver": step_solver},
         "particles_alive": n_alive,
-        "poisson_iterations_per_step": float(np.mean(solve_iters)) if solve_iters else None,
+        "poisson_iterations_per_step": iters_main,
+        "step_solver_compare": compare,
         "stages": stages,
-        "roofline": {"bound": "hbm", "kernel": KERNEL_NAMES[dom], "achieved": achieved,
+        "roofline": {"bound": "hbm",
+                     "kernel": ("k_fcg_pq + k_fcg_update" if dom == "pcg" and step_solver == "learned"
+                                else KERNEL_NAMES[dom]),
+                     "achieved": achieved,
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
                      "traffic": traffic, "alg_bytes_per_launch": alg,
                      "avg_launch_ms": avg_s * 1e3},

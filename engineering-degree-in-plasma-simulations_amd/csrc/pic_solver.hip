@@ -188,9 +188,10 @@ __global__ void __launch_bounds__(BLOCK)
     phi[u] = nbr_of<double>(t, phi, u, s);
 }
 
-// residual of the (Boltzmann) Poisson problem: out[0] += sum R^2 over all
-// non-Dirichlet rows (reference RMS numerator), out[1] += sum R^2 over
-// regular rows, out[2] += sum b^2 over regular rows.
+// residual of the (Boltzmann) Poisson problem, three striped scalars at
+// out, out + SSIZE, out + 2 SSIZE: sum R^2 over all non-Dirichlet rows
+// (reference RMS numerator), sum R^2 over regular rows, sum b^2 over
+// regular rows.
 __global__ void __launch_bounds__(BLOCK)
     k_resid(Stencil s, SorArgs a, const uint8_t* __restrict__ nt, const double* __restrict__ phi,
             const double* __restrict__ rho, double* __restrict__ out) {
@@ -215,10 +216,10 @@ __global__ void __launch_bounds__(BLOCK)
         b_reg += b * b;
     }
     double t0 = block_sum(r_all), t1 = block_sum(r_reg), t2 = block_sum(b_reg);
-    if (threadIdx.x == 0) {
-        atomicAdd(&out[0], t0);
-        atomicAdd(&out[1], t1);
-        atomicAdd(&out[2], t2);
+    if (threadIdx.x == 0) {  // three striped scalars
+        sadd(out, t0);
+        sadd(out + SSIZE, t1);
+        sadd(out + 2 * SSIZE, t2);
     }
 }
 
@@ -451,13 +452,15 @@ static bool has_neumann(pic_ctx* ctx) {
 }
 
 static int residual_norms(pic_ctx* ctx, const Stencil& st, const SorArgs& a, double out[3]) {
-    PIC_HIP(ctx, hipMemsetAsync(ctx->scratch, 0, 3 * sizeof(double), ctx->stream));
-    k_resid<<<stencil_blocks_red(st.nn), BLOCK, 0, ctx->stream>>>(st, a, ctx->ntype, ctx->phi, ctx->rho,
-                                                              ctx->scratch);
+    double* sums = ctx->scratch + 4096 + 3 * SSIZE;  // three striped scalars
+    PIC_HIP(ctx, hipMemsetAsync(sums, 0, 3 * SSIZE * sizeof(double), ctx->stream));
+    k_resid<<<stencil_blocks(st.nn), BLOCK, 0, ctx->stream>>>(st, a, ctx->ntype, ctx->phi, ctx->rho, sums);
     PIC_HIP(ctx, hipGetLastError());
-    PIC_HIP(ctx, hipMemcpyAsync(out, ctx->scratch, 3 * sizeof(double), hipMemcpyDeviceToHost,
+    std::vector<double> h(3 * SSIZE);
+    PIC_HIP(ctx, hipMemcpyAsync(h.data(), sums, h.size() * sizeof(double), hipMemcpyDeviceToHost,
                                 ctx->stream));
     PIC_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    for (int q = 0; q < 3; q++) out[q] = shost(h.data() + q * SSIZE);
     return PIC_OK;
 }
 

@@ -50,7 +50,7 @@ def test_learned_preconditioner_trains_and_beats_untrained_cycle(coarsen):
     ctx.unet_create((16, 16, 16), seed=3, coarsen=coarsen)
     st = ctx.unet_train(samples=128, epochs=20, batch=4, lr=2e-3, seed=5, blobs=(1, 6),
                         sigma=(2.0, 8.0), noise=0.05)
-    assert st["loss_last"] < 0.2 * st["loss_first"], st
+    assert st["loss_last"] < 0.4 * st["loss_first"], st
     s_learned, phi_l = _solve(ctx, pg, pg.SOLVER_LEARNED, rho, g)
     trained = ctx.unet_get_params()
     ctx.unet_set_params(np.zeros_like(trained))
