@@ -993,6 +993,10 @@ static int solve_learned(pic_ctx* ctx, const pic_solver_params* p, pic_solver_st
         double* pold = p0;
         double* pnew = p1;
         int last = 0;
+        // the host reads the residual history first at 3/4 of the previous
+        // solve's iteration count (a time loop's solves are alike), then every
+        // `check` iterations
+        int next_check = std::max(check, std::min(3 * ctx->learned_last_its / 4, max_fcg));
         for (int k = 0; k < max_fcg; k++) {
             {
                 PicProfScope ps(ctx, PIC_K_PCG);
@@ -1001,7 +1005,8 @@ static int solve_learned(pic_ctx* ctx, const pic_solver_params* p, pic_solver_st
             }
             PIC_HIP(ctx, hipGetLastError());
             it = k + 1;
-            if (it % check == 0 || it == max_fcg) {
+            if (it >= next_check || it == max_fcg) {
+                next_check = it + check;
                 PIC_HIP(ctx, hipMemcpyAsync(rrs.data() + (size_t)last * SSIZE, h.rr + (size_t)last * SSIZE,
                                             (size_t)(it - last) * SSIZE * sizeof(double),
                                             hipMemcpyDeviceToHost, ctx->stream));
@@ -1041,6 +1046,7 @@ static int solve_learned(pic_ctx* ctx, const pic_solver_params* p, pic_solver_st
     if (rc) return rc;
     st->converged = 1;
     st->iterations = it;
+    ctx->learned_last_its = it;
     st->newton_iterations = 0;
     st->ref_l2 = std::sqrt(rn[0] / (double)nn);
     st->rel_residual = rn[2] > 0 ? std::sqrt(rn[1] / rn[2]) : std::sqrt(rn[1]);
