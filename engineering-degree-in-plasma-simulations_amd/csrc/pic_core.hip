@@ -174,6 +174,7 @@ extern "C" void pic_destroy(pic_ctx* ctx) {
     (void)hipFree(ctx->scratch);
     (void)hipFree(ctx->scan_tmp);
     (void)hipFree(ctx->work);
+    (void)hipFree(ctx->dep_corner);
     for (auto& r : ctx->prof.recs) {
         (void)hipEventDestroy(r.a);
         (void)hipEventDestroy(r.b);

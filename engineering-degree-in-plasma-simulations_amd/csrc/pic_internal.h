@@ -203,6 +203,7 @@ struct pic_ctx {
     std::vector<pic::Mcc> mcc;
     pic::Prof prof;
     void* unet = nullptr;  // learned preconditioner state
+    float* dep_corner = nullptr;  // density deposit: 8 corner sums per cell (allocated on demand)
     int learned_last_its = 0;  // iterations of the last learned solve (first check near it)
 };
 

@@ -266,17 +266,22 @@ __global__ void __launch_bounds__(BLOCK)
 
 // ---- deposit: one workgroup per cell row (j, k); the row's particles are
 // contiguous after the sort and touch node lines (j..j+1, k..k+1). ----
-template <int NCH>
+template <int NCH, int J>
 __global__ void __launch_bounds__(BLOCK)
     k_deposit(GridC g, SpeciesBuf b, const uint32_t* __restrict__ off, double* __restrict__ den,
               double* __restrict__ mom, int want_den) {
-    extern __shared__ float tile[];  // NCH channels x 4 node lines x ni
-    const int r = blockIdx.x;
-    const int j = r % g.cj, k = r / g.cj;
-    const uint32_t p0 = off[(size_t)r * g.ci], p1 = off[(size_t)(r + 1) * g.ci];
+    // a workgroup owns J consecutive cell rows (j0 .. j0+J-1, k) of one k plane;
+    // node lines (j0 + jl, k + dk), jl = 0..J, dk = 0..1, sit in LDS as line
+    // jl + (J+1) dk, so the j lines shared inside the block add up in LDS
+    extern __shared__ float tile[];  // NCH channels x 2(J+1) node lines x ni
+    const int nbj = (g.cj + J - 1) / J;
+    const int k = blockIdx.x / nbj, j0 = (blockIdx.x - k * nbj) * J;
+    const int j1 = min(j0 + J, g.cj);
+    const size_t r0 = (size_t)k * g.cj + j0, r1 = (size_t)k * g.cj + j1;
+    const uint32_t p0 = off[r0 * g.ci], p1 = off[r1 * g.ci];
     if (p0 == p1) return;  // uniform for the whole workgroup
     const int ni = g.ni;
-    const int line = 4 * ni;
+    const int line = 2 * (J + 1) * ni;
     for (int t = threadIdx.x; t < NCH * line; t += BLOCK) tile[t] = 0.f;
     __syncthreads();
     // Each thread takes 4 consecutive particles with 16-byte loads (the
@@ -285,12 +290,13 @@ __global__ void __launch_bounds__(BLOCK)
     // particles of one cell are folded in registers; LDS sees one atomic per
     // corner per run of equal cells instead of one per particle.
     float acc[8][NCH];
-    int cur = -1;
+    int cur = -1;  // (jl << 11) | i of the cell being folded
     auto flush = [&]() {
         if (cur < 0) return;
+        const int ci_ = cur & 0x7FF, jl = cur >> 11;
 #pragma unroll
-        for (int q = 0; q < 8; q++) {
-            int slot = (q >> 1) * ni + cur + (q & 1);
+        for (int q = 0; q < 8; q++) {  // q = (dk*2 + dj)*2 + di
+            int slot = (jl + ((q >> 1) & 1) + (J + 1) * (q >> 2)) * ni + ci_ + (q & 1);
 #pragma unroll
             for (int ch = 0; ch < NCH; ch++) atomicAdd(&tile[ch * line + slot], acc[q][ch]);
         }
@@ -314,10 +320,10 @@ __global__ void __launch_bounds__(BLOCK)
         for (int m = 0; m < 4; m++) {
             uint32_t p = base + m;
             if (p < p0 || p >= p1) continue;
-            int i = (int)(cs[m] & 0x7FFu);
-            if (i != cur) {
+            int key = (int)(cs[m] & 0x7FFu) | ((int)(((cs[m] >> 11) & 0x7FFu) - (uint32_t)j0) << 11);
+            if (key != cur) {
                 flush();
-                cur = i;
+                cur = key;
 #pragma unroll
                 for (int q = 0; q < 8; q++)
 #pragma unroll
@@ -358,8 +364,10 @@ __global__ void __launch_bounds__(BLOCK)
     __syncthreads();
     const long long nn = g.nn;
     for (int t = threadIdx.x; t < line; t += BLOCK) {
-        int plane = t / ni, ii = t - plane * ni;
-        int jj = j + (plane & 1), kk = k + (plane >> 1);
+        int ln = t / ni, ii = t - ln * ni;
+        int dk = ln / (J + 1);
+        int jj = j0 + ln - dk * (J + 1), kk = k + dk;
+        if (jj > g.cj) continue;  // the last block of a plane may hold fewer than J rows
         long long u = ((long long)kk * g.nj + jj) * ni + ii;
         float v0 = tile[t];
         if (v0 != 0.f) {
@@ -374,6 +382,53 @@ __global__ void __launch_bounds__(BLOCK)
             }
         }
     }
+}
+
+// Number density without atomics, in two passes over the sorted particles.
+// Pass 1, one thread per cell: the cell's particles (contiguous after the
+// sort, so the cell word is not read) are folded into the eight corner sums
+// of the CIC weights w * W_q, stored per corner as corner[q][cell] (FP32).
+// Pass 2, one thread per node: den = (1 / V_node) * the sum, over the up to
+// eight cells around the node, of that cell's sum for this corner (plain
+// store; Species.cpp:401-416 scatters the same weights with atomics).
+// Corner q = (dk*2 + dj)*2 + di of cell (i, j, k) is node (i+di, j+dj, k+dk).
+__global__ void __launch_bounds__(BLOCK)
+    k_dep_cells(GridC g, SpeciesBuf b, const uint32_t* __restrict__ off, float* __restrict__ corner) {
+    int c = blockIdx.x * BLOCK + threadIdx.x;
+    if (c >= g.ncells) return;
+    uint32_t p0 = off[c], p1 = off[c + 1];
+    float acc[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+    for (uint32_t p = p0; p < p1; p++) {
+        float fx = b.f[0][p], fy = b.f[1][p], fz = b.f[2][p], w = b.w[p];
+        float gx = 1.f - fx, gy = 1.f - fy, gz = 1.f - fz;
+        float wgz = w * gz, wfz = w * fz;
+        acc[0] += gx * gy * wgz;
+        acc[1] += fx * gy * wgz;
+        acc[2] += gx * fy * wgz;
+        acc[3] += fx * fy * wgz;
+        acc[4] += gx * gy * wfz;
+        acc[5] += fx * gy * wfz;
+        acc[6] += gx * fy * wfz;
+        acc[7] += fx * fy * wfz;
+    }
+#pragma unroll
+    for (int q = 0; q < 8; q++) corner[(size_t)q * g.ncells + c] = acc[q];
+}
+
+__global__ void __launch_bounds__(BLOCK)
+    k_dep_nodes(GridC g, const float* __restrict__ corner, double* __restrict__ den) {
+    long long u = (long long)blockIdx.x * BLOCK + threadIdx.x;
+    if (u >= g.nn) return;
+    unsigned uu = (unsigned)u, row = uu / (unsigned)g.ni, kk = row / (unsigned)g.nj;
+    int i = (int)(uu - row * (unsigned)g.ni), j = (int)(row - kk * (unsigned)g.nj), k = (int)kk;
+    double s = 0.0;
+#pragma unroll
+    for (int q = 0; q < 8; q++) {
+        int ci = i - (q & 1), cj = j - ((q >> 1) & 1), ck = k - (q >> 2);
+        if (ci < 0 || ci >= g.ci || cj < 0 || cj >= g.cj || ck < 0 || ck >= g.ck) continue;
+        s += (double)corner[(size_t)q * g.ncells + cell_index(g, ci, cj, ck)];
+    }
+    den[u] = s * inv_node_vol(g, i, j, k);
 }
 
 struct RhoArgs {
@@ -546,23 +601,26 @@ extern "C" int pic_deposit(pic_ctx* ctx, int sid, int flags) {
     }
     size_t nn = (size_t)g.nn;
     int want_den = (flags & PIC_DEPOSIT_DENSITY) != 0;
-    if (want_den) PIC_HIP(ctx, hipMemsetAsync(s.den, 0, nn * 8, ctx->stream));
     bool mom = (flags & PIC_DEPOSIT_MOMENTS) != 0;
+    if (want_den && mom) PIC_HIP(ctx, hipMemsetAsync(s.den, 0, nn * 8, ctx->stream));
+    if (want_den && !mom && !ctx->dep_corner)
+        PIC_HIP(ctx, hipMalloc(&ctx->dep_corner, 8 * (size_t)g.ncells * sizeof(float)));
     if (mom && !s.mom) {
         PIC_HIP(ctx, hipMalloc(&s.mom, 7 * nn * 8));
         PIC_HIP(ctx, hipMemsetAsync(s.mom, 0, 7 * nn * 8, ctx->stream));
     }
-    int rows = g.cj * g.ck;
+    constexpr int JM = 4;  // cell rows per workgroup of the moments deposit
+    int rows_m = g.ck * ((g.cj + JM - 1) / JM);
     {
         PicProfScope ps(ctx, PIC_K_DEPOSIT);
         if (mom) {
-            size_t lds = 7 * 4 * (size_t)g.ni * sizeof(float);
-            k_deposit<7><<<rows, BLOCK, lds, ctx->stream>>>(g, s.buf[s.cur], s.off, s.den, s.mom,
-                                                            want_den);
+            size_t lds = 7 * 2 * (JM + 1) * (size_t)g.ni * sizeof(float);
+            k_deposit<7, JM><<<rows_m, BLOCK, lds, ctx->stream>>>(g, s.buf[s.cur], s.off, s.den, s.mom,
+                                                                  want_den);
         } else {
-            size_t lds = 4 * (size_t)g.ni * sizeof(float);
-            k_deposit<1><<<rows, BLOCK, lds, ctx->stream>>>(g, s.buf[s.cur], s.off, s.den,
-                                                            nullptr, want_den);
+            k_dep_cells<<<grid_for((size_t)g.ncells, BLOCK), BLOCK, 0, ctx->stream>>>(
+                g, s.buf[s.cur], s.off, ctx->dep_corner);
+            k_dep_nodes<<<grid_for(nn, BLOCK), BLOCK, 0, ctx->stream>>>(g, ctx->dep_corner, s.den);
         }
     }
     PIC_HIP(ctx, hipGetLastError());

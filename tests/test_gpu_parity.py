@@ -148,6 +148,11 @@ def test_sort_and_density(pg):
     for which, r in zip(names, acc):
         got = c.species_field(sid, which)
         assert np.max(np.abs(got - r)) <= REL32 * np.max(np.abs(r)), which
+    # the density-only deposit (cell corner sums, then a node gather) agrees too
+    c.deposit(sid, pg.DEPOSIT_DENSITY)
+    den1 = c.species_field(sid, pg.SFIELD_DEN)
+    assert np.max(np.abs(den1 - ref)) <= REL32 * np.max(ref)
+    assert abs(np.sum(den1 * vol) - np.sum(_rows(p)[:, 6])) <= 1e-6 * np.sum(_rows(p)[:, 6])
 
 
 def _cell_of(g, parts):
