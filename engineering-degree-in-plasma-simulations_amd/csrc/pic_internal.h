@@ -247,6 +247,12 @@ inline unsigned xcd_blocks(long long items, int block) {
     b = (b + XCD_COUNT - 1) / XCD_COUNT * XCD_COUNT;
     return (unsigned)b;
 }
+// one item per thread (latency-bound kernels with many independent loads)
+inline unsigned xcd_blocks1(long long items, int block) {
+    long long b = (items + block - 1) / block;
+    b = (b + XCD_COUNT - 1) / XCD_COUNT * XCD_COUNT;
+    return (unsigned)b;
+}
 // Striped grid-wide sums: one scalar is NSTRIPE partial accumulators 128
 // bytes apart, workgroup b adds into stripe b % NSTRIPE, so the per-workgroup
 // atomics of a large grid spread over L2 channels instead of serialising on
@@ -270,6 +276,9 @@ inline double shost(const double* s) {
     return t;
 }
 constexpr size_t SCRATCH_DOUBLES = 16384;  // ctx->scratch: [0, 4096) plain, [4096, ...) striped
+// learned preconditioner's striped scalars in ctx->scratch: restriction sums
+// {x^2, count}, line-search sums {c.res, c.A c}, then one spare (rz)
+constexpr int LEARNED_WS = 4096;
 
 __device__ inline long long xcd_chunk(long long n, int block, long long& end) {
     int nb = gridDim.x;

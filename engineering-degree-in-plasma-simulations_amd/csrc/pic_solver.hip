@@ -33,20 +33,11 @@ constexpr int NXCD = 8;
 
 struct Stencil {
     double cx, cy, cz, twos;  // +-1, +-ni, +-ni*nj coefficients; 2(sum 1/dx^2)
+    double itwos;             // 1 / twos
     double idx[3];            // 1/dx per axis (Neumann rows)
     long long sj, sk, nn;
     int ni, nj, nk;
 };
-
-// (i, j, k) of node u in 32-bit arithmetic (grids hold fewer than 2^31 nodes)
-__device__ inline void node_ijk(const Stencil& s, long long u, int& i, int& j, int& k) {
-    unsigned uu = (unsigned)u, ni = (unsigned)s.ni, nj = (unsigned)s.nj;
-    unsigned row = uu / ni;
-    i = (int)(uu - row * ni);
-    unsigned kk = row / nj;
-    j = (int)(row - kk * nj);
-    k = (int)kk;
-}
 
 // gridDim.x is a multiple of NXCD (stencil_blocks), so lb is a bijection
 __device__ inline long long chunk_begin(long long nn, long long& end) {
@@ -58,6 +49,28 @@ __device__ inline long long chunk_begin(long long nn, long long& end) {
     long long beg = (long long)lb * chunk;
     end = beg + chunk < nn ? beg + chunk : nn;
     return beg;
+}
+
+// Node pairs (2h, 2h+1) of every i-line, XCD-contiguous chunks per
+// workgroup: f(u, second, red) for the pair's first node u, whether the
+// second node exists (odd ni) and whether the first node is red
+// ((i + j + k) even; the second node has the other colour).  One (j, k)
+// division per pair; a wave's lanes never diverge on the colour.
+__device__ inline unsigned pair_count(const Stencil& s) {
+    return (unsigned)((s.ni + 1) / 2) * (unsigned)s.nj * (unsigned)s.nk;
+}
+template <typename F>
+__device__ inline void for_pairs(const Stencil& s, F f) {
+    const unsigned half = (unsigned)(s.ni + 1) >> 1;
+    long long end;
+    for (long long m = chunk_begin(pair_count(s), end) + threadIdx.x; m < end; m += BLOCK) {
+        unsigned row = (unsigned)m / half;
+        unsigned h = (unsigned)m - row * half;
+        unsigned k = row / (unsigned)s.nj;
+        unsigned j = row - k * (unsigned)s.nj;
+        unsigned u = row * (unsigned)s.ni + 2 * h;
+        f(u, 2 * h + 1 < (unsigned)s.ni, ((j + k) & 1u) == 0);
+    }
 }
 
 __device__ inline double block_sum(double v) {
@@ -395,6 +408,7 @@ Stencil make_stencil(const GridC& g, bool axis_swap) {
     s.cy = d2y;
     s.cz = axis_swap ? d2x : d2z;
     s.twos = 2.0 * (d2x + d2y + d2z);
+    s.itwos = 1.0 / s.twos;
     for (int d = 0; d < 3; d++) s.idx[d] = 1.0 / g.dx[d];
     s.sj = g.ni;
     s.sk = (long long)g.ni * g.nj;
@@ -452,7 +466,7 @@ static bool has_neumann(pic_ctx* ctx) {
 }
 
 static int residual_norms(pic_ctx* ctx, const Stencil& st, const SorArgs& a, double out[3]) {
-    double* sums = ctx->scratch + 4096 + 3 * SSIZE;  // three striped scalars
+    double* sums = ctx->scratch + LEARNED_WS + 5 * SSIZE;  // three striped scalars
     PIC_HIP(ctx, hipMemsetAsync(sums, 0, 3 * SSIZE * sizeof(double), ctx->stream));
     k_resid<<<stencil_blocks(st.nn), BLOCK, 0, ctx->stream>>>(st, a, ctx->ntype, ctx->phi, ctx->rho, sums);
     PIC_HIP(ctx, hipGetLastError());
@@ -716,7 +730,7 @@ static int solve_pcg(pic_ctx* ctx, const pic_solver_params* p, pic_solver_stats*
 // ------------------------------------------------------------------------
 bool pic_unet_ready(pic_ctx* ctx);
 int pic_unet_coarse_correct(pic_ctx* ctx, const double* t, const double* z, double* c, double* res,
-                            double* dots);
+                            double* ws);
 
 namespace {
 
@@ -757,7 +771,10 @@ __global__ void __launch_bounds__(BLOCK)
 }
 
 // p_k = z_k + beta p_{k-1} (beta = -zq[k-1] / pq[k-1]; p_0 = z_0), q = A p_k on
-// regular rows; pq[k] += p_k.q.  z and p vanish on non-regular nodes.
+// regular rows; pq[k] += p_k.q.  z and p vanish on non-regular nodes, and
+// regular nodes are interior (Dirichlet walls), so q is the 7-point stencil.
+// Neighbour offsets are zeroed on non-regular nodes so that every load is in
+// bounds and the loop has no branches.
 __global__ void __launch_bounds__(BLOCK)
     k_fcg_pq(Stencil s, const uint8_t* __restrict__ nt, const double* __restrict__ z,
              const double* __restrict__ pold, double* __restrict__ pnew, double* __restrict__ q,
@@ -768,13 +785,17 @@ __global__ void __launch_bounds__(BLOCK)
         beta = den != 0.0 ? -sget(h.zq + (k - 1) * SSIZE) / den : 0.0;
     }
     const bool first = k == 0;
+    auto pn = [&](unsigned w) { return first ? z[w] : z[w] + beta * pold[w]; };
     double acc = 0;
     long long end;
-    for (long long u = chunk_begin(s.nn, end) + threadIdx.x; u < end; u += BLOCK) {
-        auto pn = [&](long long w) { return first ? z[w] : z[w] + beta * pold[w]; };
-        uint8_t t = nt[u];
+    for (long long m = chunk_begin(s.nn, end) + threadIdx.x; m < end; m += BLOCK) {
+        const unsigned u = (unsigned)m;
+        const bool reg = nt[u] == NT_REG;
+        const unsigned oi = reg ? 1u : 0u, oj = reg ? (unsigned)s.sj : 0u, ok = reg ? (unsigned)s.sk : 0u;
         double pu = pn(u);
-        double qv = t == NT_REG ? apply_A<double>(t, u, s, pn) : 0.0;
+        double qv = (pn(u - ok) + pn(u + ok)) * s.cz + (pn(u - oj) + pn(u + oj)) * s.cy +
+                    (pn(u - oi) + pn(u + oi)) * s.cx - s.twos * pu;
+        qv = reg ? qv : 0.0;
         pnew[u] = pu;
         q[u] = qv;
         acc += pu * qv;
@@ -791,7 +812,8 @@ __global__ void __launch_bounds__(BLOCK)
     double alpha = den != 0.0 ? sget(h.rz + k * SSIZE) / den : 0.0;
     double rr = 0;
     long long end;
-    for (long long u = chunk_begin(n, end) + threadIdx.x; u < end; u += BLOCK) {
+    for (long long m = chunk_begin(n, end) + threadIdx.x; m < end; m += BLOCK) {
+        const unsigned u = (unsigned)m;
         x[u] += alpha * p[u];
         double rv = r[u] - alpha * q[u];
         r[u] = rv;
@@ -801,55 +823,71 @@ __global__ void __launch_bounds__(BLOCK)
     if (threadIdx.x == 0) sadd(h.rr + k * SSIZE, a);
 }
 
+// 7-point neighbour sum of f at node u, weighted per axis; offsets zeroed on
+// non-regular nodes (the result is then discarded) keep every load in bounds
+template <typename F>
+__device__ inline double nbr_sum(const Stencil& s, unsigned u, bool reg, F f) {
+    const unsigned oi = reg ? 1u : 0u, oj = reg ? (unsigned)s.sj : 0u, ok = reg ? (unsigned)s.sk : 0u;
+    return (f(u - oi) + f(u + oi)) * s.cx + (f(u - oj) + f(u + oj)) * s.cy +
+           (f(u - ok) + f(u + ok)) * s.cz;
+}
+
 // One red-black Gauss-Seidel sweep on A z = r from z = 0, in one pass: the
 // red half-sweep gives z = -r / (2 sum 1/dx^2) on red regular nodes
 // ((i + j + k) even); the black half-sweep that follows reads only those red
 // values, so it is written directly in r.  Non-regular nodes get 0.
 __global__ void __launch_bounds__(BLOCK)
     k_presmooth(Stencil s, const uint8_t* __restrict__ nt, const double* __restrict__ r,
-                double* __restrict__ z) {
-    auto zr = [&](long long v) { return nt[v] == NT_REG ? -r[v] / s.twos : 0.0; };
-    long long end;
-    for (long long u = chunk_begin(s.nn, end) + threadIdx.x; u < end; u += BLOCK) {
-        if (nt[u] != NT_REG) {
-            z[u] = 0.0;
-            continue;
-        }
-        int i, j, k;
-        node_ijk(s, u, i, j, k);
-        if (((i + j + k) & 1) == 0)
-            z[u] = -r[u] / s.twos;
+                double* __restrict__ z, double* __restrict__ clear, int nclear) {
+    // the coarse correction's striped sums start from zero (no memset node)
+    for (int q = blockIdx.x * BLOCK + threadIdx.x; q < nclear; q += gridDim.x * BLOCK) clear[q] = 0.0;
+    // r on regular nodes, 0 elsewhere (a red value is -that / twos)
+    auto rr = [&](unsigned v) {
+        double rv = r[v];
+        return nt[v] == NT_REG ? rv : 0.0;
+    };
+    auto node = [&](unsigned u, bool red) {
+        const bool reg = nt[u] == NT_REG;
+        double zu;
+        if (red)
+            zu = -r[u] * s.itwos;
         else
-            z[u] = ((zr(u - 1) + zr(u + 1)) * s.cx + (zr(u - s.sj) + zr(u + s.sj)) * s.cy +
-                    (zr(u - s.sk) + zr(u + s.sk)) * s.cz - r[u]) /
-                   s.twos;
-    }
+            zu = (-nbr_sum(s, u, reg, rr) * s.itwos - r[u]) * s.itwos;
+        z[u] = reg ? zu : 0.0;
+    };
+    for_pairs(s, [&](unsigned u, bool second, bool red) {
+        node(u, red);
+        if (second) node(u + 1, !red);
+    });
 }
 
 // The closing red half-sweep of the preconditioner fused with FCG's dot
 // products: red regular nodes get the Gauss-Seidel update (their neighbours
 // are black and final); then zq += z.q (when q is given) and rz += z.r.
+// z vanishes on non-regular nodes.
 __global__ void __launch_bounds__(BLOCK)
     k_gs_red_dots(Stencil s, const uint8_t* __restrict__ nt, const double* __restrict__ r,
                   double* __restrict__ z, const double* __restrict__ q, double* __restrict__ zq,
                   double* __restrict__ rz) {
     double a = 0, b = 0;
-    long long end;
-    for (long long u = chunk_begin(s.nn, end) + threadIdx.x; u < end; u += BLOCK) {
-        double zu;
-        int i, j, k;
-        node_ijk(s, u, i, j, k);
-        if (nt[u] == NT_REG && ((i + j + k) & 1) == 0) {
-            zu = ((z[u - 1] + z[u + 1]) * s.cx + (z[u - s.sj] + z[u + s.sj]) * s.cy +
-                  (z[u - s.sk] + z[u + s.sk]) * s.cz - r[u]) /
-                 s.twos;
-            z[u] = zu;
-        } else {
-            zu = z[u];
+    auto zv = [&](unsigned w) { return z[w]; };
+    auto node = [&](unsigned u, bool red) {
+        double zu = z[u];
+        if (red) {
+            const bool reg = nt[u] == NT_REG;
+            double upd = (nbr_sum(s, u, reg, zv) - r[u]) * s.itwos;
+            if (reg) {
+                zu = upd;
+                z[u] = zu;
+            }
         }
         if (q) a += zu * q[u];
         b += zu * r[u];
-    }
+    };
+    for_pairs(s, [&](unsigned u, bool second, bool red) {
+        node(u, red);
+        if (second) node(u + 1, !red);
+    });
     double ta = block_sum(a), tb = block_sum(b);
     if (threadIdx.x == 0) {
         if (zq) sadd(zq, ta);
@@ -867,23 +905,20 @@ __global__ void __launch_bounds__(BLOCK)
               const double* __restrict__ d, double* __restrict__ z) {
     const double d1 = sget(d + SSIZE);
     const double om = d1 != 0.0 ? sget(d) / d1 : 0.0;
-    long long end;
-    for (long long u = chunk_begin(s.nn, end) + threadIdx.x; u < end; u += BLOCK) {
-        if (nt[u] != NT_REG) {
-            z[u] = 0.0;
-            continue;
-        }
-        int i, j, k;
-        node_ijk(s, u, i, j, k);
-        if (((i + j + k) & 1) == 0) {
-            z[u] = zin[u] + om * c[u];
-        } else {
-            auto zc = [&](long long w) { return zin[w] + om * c[w]; };
-            z[u] = ((zc(u - 1) + zc(u + 1)) * s.cx + (zc(u - s.sj) + zc(u + s.sj)) * s.cy +
-                    (zc(u - s.sk) + zc(u + s.sk)) * s.cz - r[u]) /
-                   s.twos;
-        }
-    }
+    auto zc = [&](unsigned w) { return zin[w] + om * c[w]; };
+    auto node = [&](unsigned u, bool red) {
+        const bool reg = nt[u] == NT_REG;
+        double zu;
+        if (red)
+            zu = zc(u);
+        else
+            zu = (nbr_sum(s, u, reg, zc) - r[u]) * s.itwos;
+        z[u] = reg ? zu : 0.0;
+    };
+    for_pairs(s, [&](unsigned u, bool second, bool red) {
+        node(u, red);
+        if (second) node(u + 1, !red);
+    });
 }
 
 }  // namespace
@@ -892,7 +927,7 @@ __global__ void __launch_bounds__(BLOCK)
 // the learned preconditioner's pre-smoother, also used for its training pairs
 int pic_learned_presmooth(pic_ctx* ctx, const double* r, double* z) {
     Stencil s = make_stencil(ctx->g, false);
-    k_presmooth<<<stencil_blocks(s.nn), BLOCK, 0, ctx->stream>>>(s, ctx->ntype, r, z);
+    k_presmooth<<<stencil_blocks(s.nn), BLOCK, 0, ctx->stream>>>(s, ctx->ntype, r, z, nullptr, 0);
     PIC_HIP(ctx, hipGetLastError());
     return PIC_OK;
 }
@@ -906,9 +941,10 @@ static int learned_precond(pic_ctx* ctx, const Stencil& s, const double* rin, do
     double* zpre = tmp;
     double* c = tmp + s.nn;
     double* res = c + s.nn;
-    double* dots = ctx->scratch + 4096;  // two striped scalars
-    k_presmooth<<<nb, BLOCK, 0, ctx->stream>>>(s, ctx->ntype, rin, zpre);
-    int e = pic_unet_coarse_correct(ctx, rin, zpre, c, res, dots);
+    double* ws = ctx->scratch + LEARNED_WS;  // the coarse correction's striped sums
+    double* dots = ws + 2 * SSIZE;
+    k_presmooth<<<nb, BLOCK, 0, ctx->stream>>>(s, ctx->ntype, rin, zpre, ws, 4 * SSIZE);
+    int e = pic_unet_coarse_correct(ctx, rin, zpre, c, res, ws);
     if (e) return e;
     k_gs_corr<<<nb, BLOCK, 0, ctx->stream>>>(s, ctx->ntype, rin, zpre, c, dots, zout);
     k_gs_red_dots<<<nb, BLOCK, 0, ctx->stream>>>(s, ctx->ntype, rin, zout, q, zq, rz);
@@ -936,7 +972,7 @@ extern "C" int pic_learned_precondition(pic_ctx* ctx, const double* r, double* z
     if (rc) return rc;
     double* rd = ctx->work;
     double* zd = rd + nn;
-    double* rz = ctx->scratch + 4096 + 2 * SSIZE;  // striped
+    double* rz = ctx->scratch + LEARNED_WS + 4 * SSIZE;  // striped
     PIC_HIP(ctx, hipMemcpyAsync(rd, r, nn * 8, hipMemcpyHostToDevice, ctx->stream));
     PIC_HIP(ctx, hipMemsetAsync(rz, 0, SSIZE * sizeof(double), ctx->stream));
     rc = learned_precond(ctx, make_stencil(ctx->g, false), rd, zd, zd + nn, nullptr, nullptr, rz);

@@ -43,6 +43,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <type_traits>
 #include <vector>
 
 #include "pic_internal.h"
@@ -381,34 +382,386 @@ __global__ void __launch_bounds__(256) k_conv3(ConvArgs a) {
     }
 }
 
-// network input for the MFMA input layer: BF16 [V][16] = {x0 * sc, x1, 0...}
-// with sc = 1 / RMS from nrm = {sum x^2, count} (sc = 1 without nrm)
-__global__ void __launch_bounds__(256)
-    k_pack_input(long long V, const float* __restrict__ x, const double* __restrict__ nrm,
-                 bf16* __restrict__ out) {
-    long long v = (long long)blockIdx.x * 256 + threadIdx.x;
-    if (v >= V) return;
-    float sc = 1.f;
-    if (nrm && nrm[0] > 0 && nrm[1] > 0) sc = (float)(1.0 / sqrt(nrm[0] / nrm[1]));
-    float2 in = *reinterpret_cast<const float2*>(x + 2 * v);
-    uint4* o = reinterpret_cast<uint4*>(out + v * 16);
-    o[0] = make_uint4(pack2(in.x * sc, in.y), 0u, 0u, 0u);
-    o[1] = make_uint4(0u, 0u, 0u, 0u);
+// 3x3x3 convolution, register-blocked weights (large grids, 32 KS input
+// channels).  A workgroup's 4 waves are NWN along the output channels (one
+// 16-wide N-block each) times 4 / NWN along z; a wave computes 16 x RY x RZ
+// output voxels (RY * RZ M-rows) of its N-block.  The input tile with a
+// one-voxel halo and all channels is staged in LDS once (every global load
+// issued before the first LDS store; 16-channel-chunk-major, 32 bytes per
+// voxel).  Per 32-channel K step the wave keeps its N-block's 27 B fragments
+// in registers and walks the input rows: the A fragment of input row
+// (iy, iz) at x shift ox is read from LDS once and feeds the MFMAs of every
+// output row (iy - dy, iz - dz) of the block, tap (ox, dy, dz).
+template <int KS, int RY, int RZ, int NWN>
+__global__ void __launch_bounds__(256) k_conv3_rb(ConvArgs a) {
+    constexpr int NWZ = 4 / NWN;
+    constexpr int CIN = 32 * KS, NC = CIN / 16;
+    constexpr int SX = TX + 2, SY = RY + 2, SZ = RZ * NWZ + 2, NV = SX * SY * SZ;
+    constexpr int NT = NV * NC * 2, RT = (NT + 255) / 256;
+    __shared__ __attribute__((aligned(16))) uint4 tile[NT];  // [chunk][voxel][2]
+    const Dims d = a.d;
+    const int ntx = d.nx / TX, nty = d.ny / RY;
+    const int b = blockIdx.x;
+    const int x0 = (b % ntx) * TX, y0 = ((b / ntx) % nty) * RY, z0 = (b / (ntx * nty)) * (RZ * NWZ);
+    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+    const int g = lane >> 4, m = lane & 15;
+    const int wn = wave % NWN, wz = wave / NWN;
+    const int n0 = (blockIdx.y * NWN + wn) * 16;
+    {
+        uint4 rt[RT];
+#pragma unroll
+        for (int q = 0; q < RT; q++) {
+            const int e = tid + 256 * q;  // e = (chunk * NV + vox) * 2 + half
+            uint4 v = make_uint4(0, 0, 0, 0);
+            if (e < NT) {
+                const int hf = e & 1, vox = (e >> 1) % NV, ch = (e >> 1) / NV;
+                const int sx = vox % SX, sy = (vox / SX) % SY, sz = vox / (SX * SY);
+                const int gx = x0 + sx - 1, gy = y0 + sy - 1, gz = z0 + sz - 1;
+                if (gx >= 0 && gx < d.nx && gy >= 0 && gy < d.ny && gz >= 0 && gz < d.nz)
+                    v = *reinterpret_cast<const uint4*>(a.in + (((long long)gz * d.ny + gy) * d.nx + gx) * CIN +
+                                                        16 * ch + 8 * hf);
+            }
+            rt[q] = v;
+        }
+#pragma unroll
+        for (int q = 0; q < RT; q++)
+            if (tid + 256 * q < NT) tile[tid + 256 * q] = rt[q];
+    }
+    __syncthreads();
+    f4v acc[RY][RZ];
+#pragma unroll
+    for (int ry = 0; ry < RY; ry++)
+#pragma unroll
+        for (int rz = 0; rz < RZ; rz++) acc[ry][rz] = f4v{0.f, 0.f, 0.f, 0.f};
+    const int ch = g >> 1, hf = g & 1;  // lane group g: channels 8g..8g+7 of the K step
+    const bf16* wrow = a.w + (long long)(n0 + m) * 27 * CIN + 8 * g;
+#pragma unroll
+    for (int s = 0; s < KS; s++) {
+        bf8v bfr[27];
+#pragma unroll
+        for (int t = 0; t < 27; t++)
+            bfr[t] = __builtin_bit_cast(bf8v, *reinterpret_cast<const uint4*>(wrow + t * CIN + 32 * s));
+        const int c16 = 2 * s + ch;
+#pragma unroll
+        for (int iz = 0; iz < RZ + 2; iz++)
+#pragma unroll
+            for (int iy = 0; iy < RY + 2; iy++)
+#pragma unroll
+                for (int ox = 0; ox < 3; ox++) {
+                    const int sv = ((wz * RZ + iz) * SY + iy) * SX + m + ox;
+                    const bf8v afr = __builtin_bit_cast(bf8v, tile[(c16 * NV + sv) * 2 + hf]);
+#pragma unroll
+                    for (int dz = 0; dz < 3; dz++)
+#pragma unroll
+                        for (int dy = 0; dy < 3; dy++) {
+                            const int ry = iy - dy, rz = iz - dz;
+                            if (ry < 0 || ry >= RY || rz < 0 || rz >= RZ) continue;
+                            acc[ry][rz] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(
+                                afr, bfr[(dz * 3 + dy) * 3 + ox], acc[ry][rz], 0, 0, 0);
+                        }
+                }
+    }
+    // D fragment: lane holds rows 4g + j (x offset) of column m (channel)
+    const int n = n0 + m;
+    const float bv = a.bias ? a.bias[n] : 0.f;
+#pragma unroll
+    for (int ry = 0; ry < RY; ry++)
+#pragma unroll
+        for (int rz = 0; rz < RZ; rz++) {
+            const int gy = y0 + ry, gz = z0 + wz * RZ + rz;
+#pragma unroll
+            for (int j = 0; j < 4; j++) {
+                const int gx = x0 + 4 * g + j;
+                const long long o = (((long long)gz * d.ny + gy) * d.nx + gx) * a.cout + n;
+                float v = acc[ry][rz][j] + bv;
+                if (a.relu) v = fmaxf(v, 0.f);
+                if (a.mask && !(bf2f(a.mask[o]) > 0.f)) v = 0.f;
+                a.out[o] = f2bf(v);
+            }
+        }
 }
 
-// BF16 MFMA copies of the direct layers' weights, zero-padded to 16 channels:
-// input layer wi[c][t*16 + i] = w[c][2t + i] (i < 2); output layer
-// wo[n][t*C + c] = (n == 0) w[t*C + c]
-__global__ void k_pad_io_w(const float* __restrict__ win, const float* __restrict__ wout, int C,
-                           bf16* __restrict__ wi, bf16* __restrict__ wo) {
-    int e = blockIdx.x * 256 + threadIdx.x;
-    if (e < C * 27 * 16) {
-        int c = e / (27 * 16), r = e - c * 27 * 16, t = r / 16, i = r % 16;
-        wi[e] = i < 2 ? f2bf(win[c * 54 + 2 * t + i]) : (bf16)0;
+// 3x3x3 convolution for small grids (few workgroups): 16x2x2-voxel tiles,
+// one 16-wide N-block per workgroup, all CIN input channels and the
+// N-block's weights fetched in one round (every load issued before the first
+// LDS store) and staged in LDS 16-channel-chunk-major (32 bytes per voxel or
+// weight row, the conflict-free stride of the 16-byte fragment reads); K = 32
+// channels of one tap per v_mfma_f32_16x16x32_bf16 step, the 27 * CIN / 32
+// steps dealt over the 4 waves and reduced through LDS.
+template <int CIN>
+__global__ void __launch_bounds__(256) k_conv3_small(ConvArgs a) {
+    constexpr int TY = 2, TZ = 2, SX = TX + 2, SY = TY + 2, SZ = TZ + 2;
+    constexpr int NV = SX * SY * SZ;             // staged voxels
+    constexpr int NC = CIN / 16;                 // 16-channel chunks
+    constexpr int NT = NV * NC * 2, NW = 27 * 16 * NC * 2;  // uint4 to stage
+    constexpr int RT = (NT + 255) / 256, RW = (NW + 255) / 256;
+    constexpr int LDS_ST = (NT + NW) * 16, LDS_RED = 4 * 4 * 64 * 16;
+    constexpr int LDS = LDS_ST > LDS_RED ? LDS_ST : LDS_RED;
+    constexpr int KS = CIN / 32;                 // K steps per tap
+    __shared__ __attribute__((aligned(16))) unsigned char smem[LDS];
+    uint4* tile = reinterpret_cast<uint4*>(smem);  // [chunk][voxel][2]
+    uint4* wl = tile + NT;                         // [chunk][tap * 16 + n][2]
+    const Dims d = a.d;
+    const int ntx = (d.nx + TX - 1) / TX, nty = (d.ny + TY - 1) / TY;
+    const int b = blockIdx.x;
+    const int x0 = (b % ntx) * TX, y0 = ((b / ntx) % nty) * TY, z0 = (b / (ntx * nty)) * TZ;
+    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+    const int g = lane >> 4, m = lane & 15;
+    const int nb0 = blockIdx.y;
+    uint4 rt[RT], rw[RW];
+#pragma unroll
+    for (int q = 0; q < RT; q++) {
+        const int e = tid + 256 * q;  // e = (chunk * NV + vox) * 2 + half
+        uint4 v = make_uint4(0, 0, 0, 0);
+        if (e < NT) {
+            const int hf = e & 1, vox = (e >> 1) % NV, ch = (e >> 1) / NV;
+            const int sx = vox % SX, sy = (vox / SX) % SY, sz = vox / (SX * SY);
+            const int gx = x0 + sx - 1, gy = y0 + sy - 1, gz = z0 + sz - 1;
+            if (gx >= 0 && gx < d.nx && gy >= 0 && gy < d.ny && gz >= 0 && gz < d.nz)
+                v = *reinterpret_cast<const uint4*>(a.in + (((long long)gz * d.ny + gy) * d.nx + gx) * CIN +
+                                                    16 * ch + 8 * hf);
+        }
+        rt[q] = v;
     }
-    if (e < 16 * 27 * C) {
-        int n = e / (27 * C), k = e - n * 27 * C;
-        wo[e] = n == 0 ? f2bf(wout[k]) : (bf16)0;
+#pragma unroll
+    for (int q = 0; q < RW; q++) {
+        const int e = tid + 256 * q;  // e = (chunk * 432 + tap * 16 + n16) * 2 + half
+        uint4 v = make_uint4(0, 0, 0, 0);
+        if (e < NW) {
+            const int hf = e & 1, row = (e >> 1) % 432, ch = (e >> 1) / 432;
+            const int t = row >> 4, n = nb0 * 16 + (row & 15);
+            v = *reinterpret_cast<const uint4*>(a.w + (long long)n * 27 * CIN + t * CIN + 16 * ch + 8 * hf);
+        }
+        rw[q] = v;
+    }
+#pragma unroll
+    for (int q = 0; q < RT; q++)
+        if (tid + 256 * q < NT) tile[tid + 256 * q] = rt[q];
+#pragma unroll
+    for (int q = 0; q < RW; q++)
+        if (tid + 256 * q < NW) wl[tid + 256 * q] = rw[q];
+    __syncthreads();
+    f4v acc[4];
+#pragma unroll
+    for (int r = 0; r < 4; r++) acc[r] = f4v{0.f, 0.f, 0.f, 0.f};
+    const int ch = g >> 1, hf = g & 1;  // lane group g: channels 8g..8g+7 of the K step
+    for (int st = wave; st < 27 * KS; st += 4) {
+        const int t = st / KS, h = st - t * KS;
+        const int ox = t % 3, oy = (t / 3) % 3, oz = t / 9;
+        const int c16 = 2 * h + ch;
+        const bf8v bfr = __builtin_bit_cast(bf8v, wl[(c16 * 432 + t * 16 + m) * 2 + hf]);
+#pragma unroll
+        for (int r = 0; r < 4; r++) {
+            const int ty = r % TY, tz = r / TY;
+            const int sv = ((tz + oz) * SY + (ty + oy)) * SX + (m + ox);
+            const bf8v afr = __builtin_bit_cast(bf8v, tile[(c16 * NV + sv) * 2 + hf]);
+            acc[r] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(afr, bfr, acc[r], 0, 0, 0);
+        }
+    }
+    __syncthreads();  // every wave is done with the staged data
+    f4v* red = reinterpret_cast<f4v*>(smem);  // [wave][row][lane]
+#pragma unroll
+    for (int r = 0; r < 4; r++) red[(wave * 4 + r) * 64 + lane] = acc[r];
+    __syncthreads();
+    f4v s4 = f4v{0.f, 0.f, 0.f, 0.f};  // wave w finalises row w
+#pragma unroll
+    for (int w2 = 0; w2 < 4; w2++) s4 += red[(w2 * 4 + wave) * 64 + lane];
+    const int gy = y0 + wave % TY, gz = z0 + wave / TY;
+    if (gy >= d.ny || gz >= d.nz) return;
+    const int n = nb0 * 16 + m;
+    const float bv = a.bias ? a.bias[n] : 0.f;
+    // D fragment: lane holds rows 4g + j (x offset) of column m (channel)
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+        int gx = x0 + 4 * g + j;
+        if (gx >= d.nx) continue;
+        long long o = (((long long)gz * d.ny + gy) * d.nx + gx) * a.cout + n;
+        float v = s4[j] + bv;
+        if (a.relu) v = fmaxf(v, 0.f);
+        if (a.mask && !(bf2f(a.mask[o]) > 0.f)) v = 0.f;
+        a.out[o] = f2bf(v);
+    }
+}
+
+// Input layer (2 -> 16 NB channels, 3x3x3, ReLU) on v_mfma_f32_16x16x32_bf16
+// with K = tap * 2 + channel (54 used of two K = 32 steps): lane group g of
+// step s holds taps 16 s + 4 g .. + 3, both channels, so a 16x4x4-voxel tile
+// needs 2 MFMAs per row and N-block instead of one per tap.  The FP32 input
+// x [V][2] is read directly; channel 0 is scaled by 1 / RMS from the striped
+// sums nrm = {sum x^2, count} (1 without nrm).  w: BF16 [16 NB][64] (k >= 54
+// zero); out: BF16 [V][16 NB].
+template <int NB>
+__global__ void __launch_bounds__(256)
+    k_conv_in(Dims d, const float* __restrict__ x, const double* __restrict__ nrm,
+              const bf16* __restrict__ w, const float* __restrict__ bias, bf16* __restrict__ out) {
+    constexpr int TY = 4, TZ = 4, SX = TX + 2, SY = TY + 2, SZ = TZ + 2;
+    __shared__ uint32_t tile[SX * SY * SZ];  // one BF16 pair per voxel
+    float sc = 1.f;
+    if (nrm) {
+        const double s0 = sget(nrm), s1 = sget(nrm + SSIZE);  // whole wave
+        if (s0 > 0 && s1 > 0) sc = (float)(1.0 / sqrt(s0 / s1));
+    }
+    const int ntx = (d.nx + TX - 1) / TX, nty = (d.ny + TY - 1) / TY;
+    const int b = blockIdx.x;
+    const int x0 = (b % ntx) * TX, y0 = ((b / ntx) % nty) * TY, z0 = (b / (ntx * nty)) * TZ;
+    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+    const int g = lane >> 4, m = lane & 15;
+    constexpr int NS = (SX * SY * SZ + 255) / 256;
+    float2 xin[NS];
+#pragma unroll
+    for (int i = 0; i < NS; i++) {  // every load before the first LDS store
+        const int sidx = tid + 256 * i;
+        const int sx = sidx % SX, sy = (sidx / SX) % SY, sz = sidx / (SX * SY);
+        const int gx = x0 + sx - 1, gy = y0 + sy - 1, gz = z0 + sz - 1;
+        xin[i] = make_float2(0.f, 0.f);
+        if (sidx < SX * SY * SZ && gx >= 0 && gx < d.nx && gy >= 0 && gy < d.ny && gz >= 0 && gz < d.nz)
+            xin[i] = *reinterpret_cast<const float2*>(x + 2 * (((long long)gz * d.ny + gy) * d.nx + gx));
+    }
+#pragma unroll
+    for (int i = 0; i < NS; i++)
+        if (tid + 256 * i < SX * SY * SZ) tile[tid + 256 * i] = pack2(xin[i].x * sc, xin[i].y);
+    // this lane's tap offsets in the tile for both K steps (-1: past tap 26)
+    int off[2][4];
+#pragma unroll
+    for (int st = 0; st < 2; st++)
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            int t = 16 * st + 4 * g + q;
+            off[st][q] = t < 27 ? ((t / 9) * SY + (t / 3) % 3) * SX + t % 3 : -1;
+        }
+    bf8v bfr[2][NB];
+#pragma unroll
+    for (int st = 0; st < 2; st++)
+#pragma unroll
+        for (int nb = 0; nb < NB; nb++)
+            bfr[st][nb] = __builtin_bit_cast(
+                bf8v, *reinterpret_cast<const uint4*>(w + (nb * 16 + m) * 64 + 32 * st + 8 * g));
+    __syncthreads();
+    f4v acc[4][NB];
+#pragma unroll
+    for (int mb = 0; mb < 4; mb++) {
+        const int row = wave * 4 + mb;
+        const int base = ((row / TY) * SY + row % TY) * SX + m;
+#pragma unroll
+        for (int nb = 0; nb < NB; nb++) acc[mb][nb] = f4v{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+        for (int st = 0; st < 2; st++) {
+            uint4 q;
+            q.x = off[st][0] >= 0 ? tile[base + off[st][0]] : 0u;
+            q.y = off[st][1] >= 0 ? tile[base + off[st][1]] : 0u;
+            q.z = off[st][2] >= 0 ? tile[base + off[st][2]] : 0u;
+            q.w = off[st][3] >= 0 ? tile[base + off[st][3]] : 0u;
+            bf8v afr = __builtin_bit_cast(bf8v, q);
+#pragma unroll
+            for (int nb = 0; nb < NB; nb++)
+                acc[mb][nb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(afr, bfr[st][nb], acc[mb][nb], 0, 0, 0);
+        }
+    }
+    // D fragment: lane holds rows 4g + j (x offset) of column m (channel)
+#pragma unroll
+    for (int mb = 0; mb < 4; mb++) {
+        const int row = wave * 4 + mb;
+        const int gy = y0 + row % TY, gz = z0 + row / TY;
+        if (gy >= d.ny || gz >= d.nz) continue;
+#pragma unroll
+        for (int nb = 0; nb < NB; nb++) {
+            const int n = nb * 16 + m;
+            const float bv = bias[n];
+#pragma unroll
+            for (int j = 0; j < 4; j++) {
+                int gx = x0 + 4 * g + j;
+                if (gx >= d.nx) continue;
+                long long o = (((long long)gz * d.ny + gy) * d.nx + gx) * (16 * NB) + n;
+                out[o] = f2bf(fmaxf(acc[mb][nb][j] + bv, 0.f));
+            }
+        }
+    }
+}
+
+// Output layer (C -> 1, 3x3x3, no ReLU): y[v] = b + sum_t sum_c W[t][c] h[v + off_t][c].
+// Per 16x4x4-voxel tile the per-tap projections P[v][t] = sum_c W[t][c] h[v][c]
+// of the staged voxels (tile + one-voxel halo, zero outside the grid) come
+// from v_mfma_f32_16x16x32_bf16 with M = voxels, N = taps (two blocks of 16,
+// taps past 26 zero), K = the C channels (C = 16: upper half zero); P goes to
+// LDS and every thread sums its voxel's 27 entries.  w: FP32 [27 C]
+// (k = t C + c), rounded to BF16 like the other layers' weights.
+template <int C>
+__global__ void __launch_bounds__(256)
+    k_conv_out(Dims d, const bf16* __restrict__ h, const float* __restrict__ w,
+               const float* __restrict__ bias, float* __restrict__ y) {
+    constexpr int TY = 4, TZ = 4, SX = TX + 2, SY = TY + 2, SZ = TZ + 2;
+    constexpr int NV = SX * SY * SZ, MB = (NV + 15) / 16, MPW = (MB + 3) / 4;
+    constexpr int PS = 27;  // LDS row stride of P (floats; odd: conflict-free gather)
+    __shared__ float P[MB * 16 * PS];
+    const int ntx = (d.nx + TX - 1) / TX, nty = (d.ny + TY - 1) / TY;
+    const int b = blockIdx.x;
+    const int x0 = (b % ntx) * TX, y0 = ((b / ntx) % nty) * TY, z0 = (b / (ntx * nty)) * TZ;
+    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+    const int g = lane >> 4, m = lane & 15;
+    // B fragments: column n = tap nb * 16 + m, K elements = channels 8g..8g+7
+    bf8v bfr[2];
+#pragma unroll
+    for (int nb = 0; nb < 2; nb++) {
+        const int t = nb * 16 + m;
+        uint32_t q[4];
+#pragma unroll
+        for (int e = 0; e < 4; e++) {
+            const int c = 8 * g + 2 * e;
+            const bool ok = t < 27 && c < C;
+            q[e] = ok ? pack2(w[t * C + c], w[t * C + c + 1]) : 0u;
+        }
+        bfr[nb] = __builtin_bit_cast(bf8v, make_uint4(q[0], q[1], q[2], q[3]));
+    }
+    // every A fragment of this wave's M-blocks is loaded before the first MFMA
+    uint4 qa[MPW];
+#pragma unroll
+    for (int i = 0; i < MPW; i++) {
+        const int vox = (wave + 4 * i) * 16 + m;
+        uint4 q = make_uint4(0, 0, 0, 0);
+        if (vox < NV && 8 * g < C) {
+            const int sx = vox % SX, sy = (vox / SX) % SY, sz = vox / (SX * SY);
+            const int gx = x0 + sx - 1, gy = y0 + sy - 1, gz = z0 + sz - 1;
+            if (gx >= 0 && gx < d.nx && gy >= 0 && gy < d.ny && gz >= 0 && gz < d.nz)
+                q = *reinterpret_cast<const uint4*>(h + (((long long)gz * d.ny + gy) * d.nx + gx) * C + 8 * g);
+        }
+        qa[i] = q;
+    }
+#pragma unroll
+    for (int i = 0; i < MPW; i++) {
+        const int mb = wave + 4 * i;
+        if (mb >= MB) break;
+        const bf8v afr = __builtin_bit_cast(bf8v, qa[i]);
+#pragma unroll
+        for (int nb = 0; nb < 2; nb++) {
+            f4v acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(afr, bfr[nb], f4v{0.f, 0.f, 0.f, 0.f}, 0, 0, 0);
+            const int t = nb * 16 + m;
+            if (t < 27) {
+                // D fragment: lane holds rows 4g + j (voxels) of column m (tap)
+#pragma unroll
+                for (int j = 0; j < 4; j++) P[(mb * 16 + 4 * g + j) * PS + t] = acc[j];
+            }
+        }
+    }
+    __syncthreads();
+    const int tx = tid % TX, ty = (tid / TX) % TY, tz = tid / (TX * TY);
+    const int gx = x0 + tx, gy = y0 + ty, gz = z0 + tz;
+    if (gx >= d.nx || gy >= d.ny || gz >= d.nz) return;
+    float acc = bias[0];
+#pragma unroll
+    for (int t = 0; t < 27; t++) {
+        const int ox = t % 3, oy = (t / 3) % 3, oz = t / 9;
+        acc += P[(((tz + oz) * SY + ty + oy) * SX + tx + ox) * PS + t];
+    }
+    y[((long long)gz * d.ny + gy) * d.nx + gx] = acc;
+}
+
+// BF16 MFMA copy of the input layer's weights: wi[c][k] = w[c][k] for
+// k = 2 tap + channel < 54, 0 up to 64
+__global__ void k_pad_in_w(const float* __restrict__ win, int C, bf16* __restrict__ wi) {
+    int e = blockIdx.x * 256 + threadIdx.x;
+    if (e < C * 64) {
+        int c = e / 64, k = e - c * 64;
+        wi[e] = k < 54 ? f2bf(win[c * 54 + k]) : (bf16)0;
     }
 }
 
@@ -520,6 +873,80 @@ __global__ void __launch_bounds__(256)
     for (int k = 0; k < 8; k++) r[k] = a[k] + b[k];
     reinterpret_cast<uint4*>(fine + v * C)[c8] =
         make_uint4(pack2(r[0], r[1]), pack2(r[2], r[3]), pack2(r[4], r[5]), pack2(r[6], r[7]));
+}
+
+// Decoder input m = up(conv1(h)) + skip on the fine level: the 1x1
+// up-convolution of the coarse activation h (32 KS channels) on
+// v_mfma_f32_16x16x32_bf16 (M = 16 coarse voxels per wave, N = 16 NB output
+// channels, bias added), staged in LDS, then nearest upsampling plus the skip
+// connection for the 8 children of each coarse voxel, 16 bytes per access.
+// Fine dimensions are exactly twice the coarse ones.  w: BF16 [16 NB][32 KS].
+template <int KS, int NB>
+__global__ void __launch_bounds__(256)
+    k_up1_add(Dims dc, const bf16* __restrict__ h, const bf16* __restrict__ w,
+              const float* __restrict__ bias, const bf16* __restrict__ skip, bf16* __restrict__ out) {
+    constexpr int CN = 32 * KS, CL = 16 * NB, G = CL / 8, US = CL + 4;
+    __shared__ float ul[64 * US];
+    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+    const int g = lane >> 4, m = lane & 15;
+    const long long v0 = (long long)blockIdx.x * 64;
+    f4v acc[NB];
+#pragma unroll
+    for (int nb = 0; nb < NB; nb++) acc[nb] = f4v{0.f, 0.f, 0.f, 0.f};
+    const long long va = v0 + wave * 16 + m;  // A row m
+#pragma unroll
+    for (int st = 0; st < KS; st++) {
+        uint4 q = make_uint4(0, 0, 0, 0);
+        if (va < dc.v) q = *reinterpret_cast<const uint4*>(h + va * CN + 32 * st + 8 * g);
+        const bf8v afr = __builtin_bit_cast(bf8v, q);
+#pragma unroll
+        for (int nb = 0; nb < NB; nb++) {
+            const bf8v bfr = __builtin_bit_cast(
+                bf8v, *reinterpret_cast<const uint4*>(w + (nb * 16 + m) * CN + 32 * st + 8 * g));
+            acc[nb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(afr, bfr, acc[nb], 0, 0, 0);
+        }
+    }
+    // D fragment: lane holds rows 4g + j (coarse voxels) of column m (channel)
+#pragma unroll
+    for (int nb = 0; nb < NB; nb++) {
+        const float bv = bias[nb * 16 + m];
+#pragma unroll
+        for (int j = 0; j < 4; j++) ul[(wave * 16 + 4 * g + j) * US + nb * 16 + m] = acc[nb][j] + bv;
+    }
+    __syncthreads();
+    const int fx = 2 * dc.nx, fy = 2 * dc.ny;
+    // items ordered (y/z child, coarse voxel, x child, 8-channel group) so that
+    // consecutive threads touch consecutive 16-byte pieces of a fine x-row;
+    // every skip load is issued before the first store
+    constexpr int NIT = 4 * 64 * 2 * G / 256;
+    long long fo[NIT];
+    uint4 sq[NIT];
+#pragma unroll
+    for (int i = 0; i < NIT; i++) {
+        const int it = tid + 256 * i;
+        const int c8 = it % G, ax = (it / G) & 1, vl = (it / (2 * G)) % 64, bc = it / (128 * G);
+        const long long v = v0 + vl;
+        fo[i] = -1;
+        sq[i] = make_uint4(0, 0, 0, 0);
+        if (v < dc.v) {
+            int cx, cy, cz;
+            vox_xyz(dc, v, cx, cy, cz);
+            fo[i] = ((long long)(2 * cz + (bc >> 1)) * fy + 2 * cy + (bc & 1)) * fx + 2 * cx + ax;
+            sq[i] = reinterpret_cast<const uint4*>(skip + fo[i] * CL)[c8];
+        }
+    }
+#pragma unroll
+    for (int i = 0; i < NIT; i++) {
+        if (fo[i] < 0) continue;
+        const int it = tid + 256 * i;
+        const int c8 = it % G, vl = (it / (2 * G)) % 64;
+        float sk[8];
+        unpack8(sq[i], sk);
+        const float* uu = ul + vl * US + 8 * c8;
+        reinterpret_cast<uint4*>(out + fo[i] * CL)[c8] =
+            make_uint4(pack2(uu[0] + sk[0], uu[1] + sk[1]), pack2(uu[2] + sk[2], uu[3] + sk[3]),
+                       pack2(uu[4] + sk[4], uu[5] + sk[5]), pack2(uu[6] + sk[6], uu[7] + sk[7]));
+    }
 }
 
 // d_coarse = sum over the 8 children of d_fine
@@ -827,7 +1254,7 @@ __device__ inline void block_sum2(double& a, double& b) {
 // The node (i, j, k) belongs to aggregate (i/CF, j/CF, k/CF).
 
 // training target: tgt[v] = mean of phi over the aggregate's regular nodes;
-// sums[2] += tgt^2 over aggregates holding a regular node
+// striped sums + 2 SSIZE += tgt^2 over aggregates holding a regular node
 template <int CF>
 __global__ void __launch_bounds__(256)
     k_restrict_target(Dims df, Dims dc, const uint8_t* __restrict__ nt,
@@ -855,7 +1282,7 @@ __global__ void __launch_bounds__(256)
     }
     double dummy = 0;
     block_sum2(acc, dummy);
-    if (threadIdx.x == 0) atomicAdd(&sums[2], acc);
+    if (threadIdx.x == 0) sadd(sums + 2 * SSIZE, acc);
 }
 
 // input to unit RMS over the valid aggregates, the target by the same factor
@@ -864,8 +1291,9 @@ __global__ void __launch_bounds__(256)
     k_scale_pair(long long n, const double* __restrict__ sums, float* __restrict__ x2,
                  float* __restrict__ tgt) {
     long long v = (long long)blockIdx.x * 256 + threadIdx.x;
+    const double s0 = sget(sums), s1 = sget(sums + SSIZE);  // striped, whole wave
     if (v >= n) return;
-    double sx = sums[0] > 0 && sums[1] > 0 ? sqrt(sums[0] / sums[1]) : 1.0;
+    double sx = s0 > 0 && s1 > 0 ? sqrt(s0 / s1) : 1.0;
     float f = (float)(1.0 / sx);
     x2[2 * v] *= f;
     tgt[v] *= f;
@@ -878,16 +1306,19 @@ __global__ void __launch_bounds__(256) k_scale_f(long long n, float f, float* __
 
 // preconditioner input from the fine residual of A e = t after smoothing z:
 // x2[v] = {-h^2 mean over regular nodes of (t - A z), fraction of other nodes};
-// sums += {x^2, 1}; res (optional) receives t - A z on the fine grid (0 on
-// non-regular nodes).  Regular nodes are interior (Dirichlet walls), so all
-// six neighbours exist; z is zero on non-regular nodes.
+// striped sums (sums, sums + SSIZE) += {x^2, 1}; res (optional) receives
+// t - A z on the fine grid (0 on non-regular nodes).  Regular nodes are
+// interior (Dirichlet walls), so all six neighbours exist; z is zero on
+// non-regular nodes.  CF 2 loads z once per aggregate (the 2x2x2 cube and
+// its 24 face neighbours, coordinates clamped into the grid; values of
+// non-regular nodes are discarded), so the loop has no branches.
 template <int CF>
 __global__ void __launch_bounds__(256)
     k_res_restrict(Dims df, Dims dc, double h2, const uint8_t* __restrict__ nt,
                    const double* __restrict__ t, const double* __restrict__ z, float* __restrict__ x2,
                    double* __restrict__ sums, double* __restrict__ res) {
     double s2 = 0, cnt = 0;
-    const long long sj = df.nx, sk = (long long)df.nx * df.ny;
+    const unsigned sj = df.nx, sk = (unsigned)df.nx * df.ny;
     const double ih2 = 1.0 / h2;
     long long end;
     for (long long v = xcd_chunk(dc.v, 256, end) + threadIdx.x; v < end; v += 256) {
@@ -895,19 +1326,44 @@ __global__ void __launch_bounds__(256)
         vox_xyz(dc, v, cx, cy, cz);
         double acc = 0;
         int nreg = 0;
+        if constexpr (CF == 2) {
+            // clamped fine coordinates -1..2 around the aggregate, per axis
+            unsigned X[4], Y[4], Z[4];
 #pragma unroll
-        for (int q = 0; q < CF * CF * CF; q++) {
-            int fx = CF * cx + q % CF, fy = CF * cy + (q / CF) % CF, fz = CF * cz + q / (CF * CF);
-            long long u = ((long long)fz * df.ny + fy) * df.nx + fx;
-            double rv = 0.0;
-            if (nt[u] == NT_REG) {
-                double az = (z[u - 1] + z[u + 1] + z[u - sj] + z[u + sj] + z[u - sk] + z[u + sk] -
-                             6.0 * z[u]) * ih2;
-                rv = t[u] - az;
-                acc += rv;
-                nreg++;
+            for (int q = 0; q < 4; q++) {
+                X[q] = (unsigned)min(max(2 * cx - 1 + q, 0), df.nx - 1);
+                Y[q] = (unsigned)min(max(2 * cy - 1 + q, 0), df.ny - 1) * sj;
+                Z[q] = (unsigned)min(max(2 * cz - 1 + q, 0), df.nz - 1) * sk;
             }
-            if (res) res[u] = rv;
+            auto zat = [&](int a, int b, int c) { return z[Z[c] + Y[b] + X[a]]; };
+#pragma unroll
+            for (int q = 0; q < 8; q++) {
+                const int a = 1 + (q & 1), b = 1 + ((q >> 1) & 1), c = 1 + (q >> 2);
+                const unsigned u = Z[c] + Y[b] + X[a];
+                const bool reg = nt[u] == NT_REG;
+                double az = (zat(a - 1, b, c) + zat(a + 1, b, c) + zat(a, b - 1, c) + zat(a, b + 1, c) +
+                             zat(a, b, c - 1) + zat(a, b, c + 1) - 6.0 * zat(a, b, c)) *
+                            ih2;
+                double rv = reg ? t[u] - az : 0.0;
+                acc += rv;
+                nreg += reg;
+                if (res) res[u] = rv;
+            }
+        } else {
+#pragma unroll
+            for (int q = 0; q < CF * CF * CF; q++) {
+                unsigned fx = CF * cx + q % CF, fy = CF * cy + (q / CF) % CF, fz = CF * cz + q / (CF * CF);
+                unsigned u = (fz * df.ny + fy) * df.nx + fx;
+                const bool reg = nt[u] == NT_REG;
+                const unsigned oi = reg ? 1u : 0u, oj = reg ? sj : 0u, ok = reg ? sk : 0u;
+                double az = (z[u - oi] + z[u + oi] + z[u - oj] + z[u + oj] + z[u - ok] + z[u + ok] -
+                             6.0 * z[u]) *
+                            ih2;
+                double rv = reg ? t[u] - az : 0.0;
+                acc += rv;
+                nreg += reg;
+                if (res) res[u] = rv;
+            }
         }
         float xv = nreg ? (float)(-h2 * acc / nreg) : 0.f;
         x2[2 * v] = xv;
@@ -919,10 +1375,11 @@ __global__ void __launch_bounds__(256)
     }
     block_sum2(s2, cnt);
     if (threadIdx.x == 0) {
-        atomicAdd(&sums[0], s2);
-        atomicAdd(&sums[1], cnt);
+        sadd(sums, s2);
+        sadd(sums + SSIZE, cnt);
     }
 }
+
 
 // (P y) at node (i, j, k): identity (CF 1) or trilinear from aggregate
 // centres (CF 2: per axis weight 3/4 on the own aggregate, 1/4 on the
@@ -995,6 +1452,101 @@ __global__ void __launch_bounds__(256)
     }
 }
 
+
+// CF 2: c = P y and the line-search sums with one thread per aggregate
+// (ceil(n/2) per axis; aggregates past the network grid hold only
+// non-regular nodes).  Trilinear interpolation from aggregate centres is
+// separable: per axis, fine offset 0 (the aggregate's lower node) takes 3/4
+// of the own aggregate and 1/4 of the lower neighbour, offset 1 3/4 own and
+// 1/4 upper, offset 2 (the next aggregate's lower node) 3/4 upper and 1/4
+// own, neighbours clamped at the grid's ends as in interp.  From the 3x3x3
+// aggregate values the thread gets c at fine offsets {0,1,2}^3: its own 8
+// nodes and the forward neighbours of the edge form of c.A c, which with c
+// zero off the regular nodes is -(1/h^2) sum over grid edges (c_u - c_v)^2.
+__global__ void __launch_bounds__(256)
+    k_prolong_dots_agg(Dims df, Dims dc, double ih2, const uint8_t* __restrict__ nt,
+                       const float* __restrict__ y, const double* __restrict__ res,
+                       double* __restrict__ c, double* __restrict__ out) {
+    const int ex = (df.nx + 1) >> 1, ey = (df.ny + 1) >> 1, ez = (df.nz + 1) >> 1;
+    const long long na = (long long)ex * ey * ez;
+    const unsigned sj = df.nx, sk = (unsigned)df.nx * df.ny;
+    double acc0 = 0, acc1 = 0;
+    long long end;
+    for (long long v = xcd_chunk(na, 256, end) + threadIdx.x; v < end; v += 256) {
+        const unsigned vv = (unsigned)v;
+        const unsigned row = vv / (unsigned)ex;
+        const int ax = (int)(vv - row * (unsigned)ex);
+        const int az = (int)(row / (unsigned)ey), ay = (int)(row - (unsigned)az * ey);
+        const int ox = min(ax, dc.nx - 1), oy = min(ay, dc.ny - 1), oz = min(az, dc.nz - 1);
+        const int xs[3] = {max(ox - 1, 0), ox, min(ox + 1, dc.nx - 1)};
+        const int ys[3] = {max(oy - 1, 0), oy, min(oy + 1, dc.ny - 1)};
+        const int zs[3] = {max(oz - 1, 0), oz, min(oz + 1, dc.nz - 1)};
+        // separable passes: [z 0..2][y 0..2][x offset 0..2], then y, then z
+        float A[3][3][3];
+#pragma unroll
+        for (int k = 0; k < 3; k++)
+#pragma unroll
+            for (int j = 0; j < 3; j++) {
+                const float* yr = y + ((long long)zs[k] * dc.ny + ys[j]) * dc.nx;
+                float m = yr[xs[0]], o = yr[xs[1]], p = yr[xs[2]];
+                A[k][j][0] = 0.75f * o + 0.25f * m;
+                A[k][j][1] = 0.75f * o + 0.25f * p;
+                A[k][j][2] = 0.75f * p + 0.25f * o;
+            }
+        float B[3][3][3];  // [z][y offset][x offset]
+#pragma unroll
+        for (int k = 0; k < 3; k++)
+#pragma unroll
+            for (int a = 0; a < 3; a++) {
+                B[k][0][a] = 0.75f * A[k][1][a] + 0.25f * A[k][0][a];
+                B[k][1][a] = 0.75f * A[k][1][a] + 0.25f * A[k][2][a];
+                B[k][2][a] = 0.75f * A[k][2][a] + 0.25f * A[k][1][a];
+            }
+        double C[3][3][3];  // [z offset][y offset][x offset], 0 off regular nodes
+        const int fx0 = 2 * ax, fy0 = 2 * ay, fz0 = 2 * az;
+#pragma unroll
+        for (int cc = 0; cc < 3; cc++)
+#pragma unroll
+            for (int b = 0; b < 3; b++)
+#pragma unroll
+                for (int a = 0; a < 3; a++) {
+                    const int nfwd = (a == 2) + (b == 2) + (cc == 2);
+                    if (nfwd > 1) continue;  // not needed
+                    float w;
+                    if (cc == 0) w = 0.75f * B[1][b][a] + 0.25f * B[0][b][a];
+                    else if (cc == 1) w = 0.75f * B[1][b][a] + 0.25f * B[2][b][a];
+                    else w = 0.75f * B[2][b][a] + 0.25f * B[1][b][a];
+                    const int fx = fx0 + a, fy = fy0 + b, fz = fz0 + cc;
+                    bool reg = false;
+                    if (fx < df.nx && fy < df.ny && fz < df.nz)
+                        reg = nt[(unsigned)fz * sk + (unsigned)fy * sj + (unsigned)fx] == NT_REG;
+                    C[cc][b][a] = reg ? (double)w : 0.0;
+                }
+        double e = 0.0;
+#pragma unroll
+        for (int q = 0; q < 8; q++) {
+            const int a = q & 1, b = (q >> 1) & 1, cc = q >> 2;
+            const int fx = fx0 + a, fy = fy0 + b, fz = fz0 + cc;
+            if (fx >= df.nx || fy >= df.ny || fz >= df.nz) continue;
+            const unsigned u = (unsigned)fz * sk + (unsigned)fy * sj + (unsigned)fx;
+            const double cu = C[cc][b][a];
+            c[u] = cu;
+            const double rv = res[u];
+            acc0 += cu != 0.0 ? cu * rv : 0.0;  // res is defined on regular nodes only
+            if (fx + 1 < df.nx) e += (cu - C[cc][b][a + 1]) * (cu - C[cc][b][a + 1]);
+            if (fy + 1 < df.ny) e += (cu - C[cc][b + 1][a]) * (cu - C[cc][b + 1][a]);
+            if (fz + 1 < df.nz) e += (cu - C[cc + 1][b][a]) * (cu - C[cc + 1][b][a]);
+        }
+        acc1 -= e * ih2;
+    }
+    double acc[2] = {acc0, acc1};
+    block_sums<2>(acc);
+    if (threadIdx.x == 0) {
+        sadd(out, acc[0]);
+        sadd(out + SSIZE, acc[1]);
+    }
+}
+
 }  // namespace
 
 // =========================================================================
@@ -1025,9 +1577,7 @@ struct UNet {
     size_t nparam = 0;
     float *p = nullptr, *g = nullptr, *m = nullptr, *v = nullptr;
     float* x = nullptr;   // network input [V0][2]
-    bf16* xin = nullptr;  // the same packed for the MFMA input layer [V0][16]
-    bf16* wi_pad = nullptr;  // input layer weights, BF16, channels padded to 16
-    bf16* wo_pad = nullptr;  // output layer weights, BF16, outputs padded to 16
+    bf16* wi_pad = nullptr;  // input layer weights, BF16 [C0][64] (k = 2 tap + channel)
     float* y = nullptr;   // [V0]
     float* dy = nullptr;  // [V0]
     long long adam_t = 0;
@@ -1136,9 +1686,7 @@ static int unet_build(pic_ctx* ctx, const pic_unet_desc* desc) {
         rc |= ualloc(ctx, u, (void**)&u->L[l].wb, nw);
         rc |= ualloc(ctx, u, (void**)&u->L[l].wt, nw);
     }
-    rc |= ualloc(ctx, u, (void**)&u->wi_pad, (size_t)C[0] * 27 * 16 * sizeof(bf16));
-    rc |= ualloc(ctx, u, (void**)&u->wo_pad, (size_t)16 * 27 * C[0] * sizeof(bf16));
-    rc |= ualloc(ctx, u, (void**)&u->xin, (size_t)d.v * 16 * sizeof(bf16));
+    rc |= ualloc(ctx, u, (void**)&u->wi_pad, (size_t)C[0] * 64 * sizeof(bf16));
     long long V0 = d.v;
     rc |= ualloc(ctx, u, (void**)&u->x, V0 * 2 * sizeof(float));
     rc |= ualloc(ctx, u, (void**)&u->y, V0 * sizeof(float));
@@ -1187,9 +1735,9 @@ static int unet_sync_weights(pic_ctx* ctx, UNet* u) {
         k_transpose_w<<<grid_for(n, 256), 256, 0, ctx->stream>>>(u->p + ly.woff, ly.cout, ly.cin,
                                                                 ly.taps, ly.wb, ly.wt);
     }
-    const UNetLayer &li = u->L[u->l_in], &lo = u->L[u->l_out];
-    k_pad_io_w<<<grid_for((size_t)432 * li.cout, 256), 256, 0, ctx->stream>>>(
-        u->p + li.woff, u->p + lo.woff, li.cout, u->wi_pad, u->wo_pad);
+    const UNetLayer& li = u->L[u->l_in];
+    k_pad_in_w<<<grid_for((size_t)64 * li.cout, 256), 256, 0, ctx->stream>>>(u->p + li.woff, li.cout,
+                                                                             u->wi_pad);
     PIC_HIP(ctx, hipGetLastError());
     return PIC_OK;
 }
@@ -1208,9 +1756,33 @@ static void conv(pic_ctx* ctx, const Dims& d, const bf16* in, int cin, const bf1
     // registers (measured: 11.5 ms learned solve at 128^3 vs 13.1 ms without)
     const bool pf = cin > 16;
     if (taps == 27) {
+        // register-blocked kernel where the shape fits it (every large-grid
+        // layer of networks with 32 or 64 channels)
+        if ((cin == 32 || cin == 64) && cout == 32 && d.nx % 16 == 0 && d.ny % 4 == 0 &&
+            d.nz % 8 == 0 && (d.nx / 16) * (d.ny / 4) * (d.nz / 8) >= 256) {
+            dim3 gr((d.nx / 16) * (d.ny / 4) * (d.nz / 8), 1);
+            if (cin == 32)
+                k_conv3_rb<1, 4, 4, 2><<<gr, 256, 0, ctx->stream>>>(a);
+            else
+                k_conv3_rb<2, 4, 4, 2><<<gr, 256, 0, ctx->stream>>>(a);
+            return;
+        }
+        if ((cin == 32 || cin == 64) && cout == 64 && d.nx % 16 == 0 && d.ny % 2 == 0 &&
+            d.nz % 4 == 0 && (d.nx / 16) * (d.ny / 2) * (d.nz / 4) >= 256) {
+            dim3 gr((d.nx / 16) * (d.ny / 2) * (d.nz / 4), 1);
+            if (cin == 32)
+                k_conv3_rb<1, 2, 4, 4><<<gr, 256, 0, ctx->stream>>>(a);
+            else
+                k_conv3_rb<2, 2, 4, 4><<<gr, 256, 0, ctx->stream>>>(a);
+            return;
+        }
         if (tiles * (cout / 16) < 512) {  // small grid: 16x2x2 tiles, taps split over the waves
             int t4 = ntx * ((d.ny + 1) / 2) * ((d.nz + 1) / 2);
-            if (pf)
+            if (cin == 64 && !outf)
+                k_conv3_small<64><<<dim3(t4, cout / 16), 256, 0, ctx->stream>>>(a);
+            else if (cin == 32 && !outf)
+                k_conv3_small<32><<<dim3(t4, cout / 16), 256, 0, ctx->stream>>>(a);
+            else if (pf)
                 k_conv3<1, 4, true><<<dim3(t4, cout / 16), 256, 0, ctx->stream>>>(a);
             else
                 k_conv3<1, 4, false><<<dim3(t4, cout / 16), 256, 0, ctx->stream>>>(a);
@@ -1271,8 +1843,13 @@ static int unet_forward(pic_ctx* ctx, UNet* u, const float* x, const double* nrm
     const Dims& d0 = lv[0].d;
     const int C0 = lv[0].C;
     const UNetLayer& li = L[u->l_in];
-    k_pack_input<<<ew(d0.v), 256, 0, ctx->stream>>>(d0.v, x, nrm, u->xin);
-    conv(ctx, d0, u->xin, 16, u->wi_pad, P + li.boff, lv[0].a, C0, 27, 1, nullptr);
+    {
+        const int tiles = ((d0.nx + TX - 1) / TX) * ((d0.ny + 3) / 4) * ((d0.nz + 3) / 4);
+        if (C0 == 32)
+            k_conv_in<2><<<tiles, 256, 0, ctx->stream>>>(d0, x, nrm, u->wi_pad, P + li.boff, lv[0].a);
+        else
+            k_conv_in<1><<<tiles, 256, 0, ctx->stream>>>(d0, x, nrm, u->wi_pad, P + li.boff, lv[0].a);
+    }
     conv(ctx, d0, lv[0].a, C0, L[u->l_b[0]].wb, P + L[u->l_b[0]].boff, lv[0].s, C0, 27, 1, nullptr);
     for (int l = 1; l < D; l++) {
         const Dims& d = lv[l].d;
@@ -1285,14 +1862,38 @@ static int unet_forward(pic_ctx* ctx, UNet* u, const float* x, const double* nrm
         const Dims& d = lv[l].d;
         int Cl = lv[l].C, Cn = lv[l + 1].C;
         const bf16* src = l == D - 2 ? lv[D - 1].s : lv[l + 1].h;
-        conv(ctx, lv[l + 1].d, src, Cn, L[u->l_u[l]].wb, P + L[u->l_u[l]].boff, lv[l].u, Cl, 1, 0,
-             nullptr);
-        k_up_add<<<ew(d.v * Cl / 8), 256, 0, ctx->stream>>>(d, lv[l].u, lv[l].s, lv[l].m, Cl);
+        const UNetLayer& ul = L[u->l_u[l]];
+        const Dims& dn = lv[l + 1].d;
+        const unsigned nbu = (unsigned)((dn.v + 63) / 64);
+        auto up1 = [&](auto ks, auto nb) {
+            k_up1_add<decltype(ks)::value, decltype(nb)::value>
+                <<<nbu, 256, 0, ctx->stream>>>(dn, src, ul.wb, P + ul.boff, lv[l].s, lv[l].m);
+        };
+        using I1 = std::integral_constant<int, 1>;
+        using I2 = std::integral_constant<int, 2>;
+        using I4 = std::integral_constant<int, 4>;
+        const bool fused = (Cn == 32 || Cn == 64) && (Cl == 16 || Cl == 32 || Cl == 64);
+        if (fused) {
+            if (Cn == 32)
+                Cl == 16 ? up1(I1{}, I1{}) : Cl == 32 ? up1(I1{}, I2{}) : up1(I1{}, I4{});
+            else
+                Cl == 16 ? up1(I2{}, I1{}) : Cl == 32 ? up1(I2{}, I2{}) : up1(I2{}, I4{});
+        }
+        else {  // channel counts of 16 or 48: separate 1x1 convolution and upsampling
+            conv(ctx, dn, src, Cn, ul.wb, P + ul.boff, lv[l].u, Cl, 1, 0, nullptr);
+            k_up_add<<<ew(d.v * Cl / 8), 256, 0, ctx->stream>>>(d, lv[l].u, lv[l].s, lv[l].m, Cl);
+        }
         conv(ctx, d, lv[l].m, Cl, L[u->l_dec[l]].wb, P + L[u->l_dec[l]].boff, lv[l].h, Cl, 27, 1,
              nullptr);
     }
     const UNetLayer& lo = L[u->l_out];
-    conv(ctx, d0, lv[0].h, C0, u->wo_pad, P + lo.boff, nullptr, 16, 27, 0, nullptr, u->y);
+    {
+        const int tiles = ((d0.nx + TX - 1) / TX) * ((d0.ny + 3) / 4) * ((d0.nz + 3) / 4);
+        if (C0 == 32)
+            k_conv_out<32><<<tiles, 256, 0, ctx->stream>>>(d0, lv[0].h, P + lo.woff, P + lo.boff, u->y);
+        else
+            k_conv_out<16><<<tiles, 256, 0, ctx->stream>>>(d0, lv[0].h, P + lo.woff, P + lo.boff, u->y);
+    }
     PIC_HIP(ctx, hipGetLastError());
     return PIC_OK;
 }
@@ -1580,31 +2181,34 @@ bool pic_unet_ready(pic_ctx* ctx) {
 // Coarse correction of the two-level preconditioner: the residual
 // res = t - A z (t: the fine right-hand side, z: the smoothed approximation,
 // device FP64; res is stored) is restricted to the network grid, the network
-// is applied and its output interpolated to c (device FP64).  dots (device,
-// two striped scalars) receives {c.res, c.A c}, from which the caller takes
-// the A-norm optimal step omega = (c.res) / (c.A c) for z += omega c.  Only for
-// Dirichlet-walled uniform grids (every regular node is interior).
+// is applied and its output interpolated to c (device FP64).  ws holds four
+// striped scalars that the caller has zeroed: the restriction's {x^2, count}
+// and then dots = ws + 2 SSIZE, which receives {c.res, c.A c}, from which the
+// caller takes the A-norm optimal step omega = (c.res) / (c.A c) for
+// z += omega c.  Only for Dirichlet-walled uniform grids (every regular node
+// is interior).
 int pic_unet_coarse_correct(pic_ctx* ctx, const double* t, const double* z, double* c, double* res,
-                            double* dots) {
+                            double* ws) {
     UNet* u = get_unet(ctx);
     if (!u) return pic_fail(ctx, PIC_ERR_STATE, "no U-Net");
     const GridC& g = ctx->g;
     Dims df{g.ni, g.nj, g.nk, g.nn};
     const Dims& dc = u->lv[0].d;
     double h2 = g.dx[0] * g.dx[0];
-    double* sums = ctx->scratch + 64;  // input normalisation
-    PIC_HIP(ctx, hipMemsetAsync(sums, 0, 2 * sizeof(double), ctx->stream));
-    PIC_HIP(ctx, hipMemsetAsync(dots, 0, 2 * SSIZE * sizeof(double), ctx->stream));
-    unsigned nbc = xcd_blocks(dc.v, 256), nbf = xcd_blocks(df.v, 256);
+    double* sums = ws;  // input normalisation
+    double* dots = ws + 2 * SSIZE;
+    unsigned nbc = xcd_blocks1(dc.v, 256), nbf = xcd_blocks(df.v, 256);
     if (u->coarsen == 2)
         k_res_restrict<2><<<nbc, 256, 0, ctx->stream>>>(df, dc, h2, ctx->ntype, t, z, u->x, sums, res);
     else
         k_res_restrict<1><<<nbc, 256, 0, ctx->stream>>>(df, dc, h2, ctx->ntype, t, z, u->x, sums, res);
     int rc = unet_forward(ctx, u, u->x, sums);
     if (rc) return rc;
-    if (u->coarsen == 2)
-        k_prolong_dots<2><<<nbf, 256, 0, ctx->stream>>>(df, dc, 1.0 / h2, ctx->ntype, u->y, res, c, dots);
-    else
+    if (u->coarsen == 2) {
+        long long na = (long long)((g.ni + 1) / 2) * ((g.nj + 1) / 2) * ((g.nk + 1) / 2);
+        k_prolong_dots_agg<<<xcd_blocks1(na, 256), 256, 0, ctx->stream>>>(df, dc, 1.0 / h2, ctx->ntype,
+                                                                          u->y, res, c, dots);
+    } else
         k_prolong_dots<1><<<nbf, 256, 0, ctx->stream>>>(df, dc, 1.0 / h2, ctx->ntype, u->y, res, c, dots);
     PIC_HIP(ctx, hipGetLastError());
     return PIC_OK;
@@ -1701,8 +2305,8 @@ extern "C" int pic_unet_train(pic_ctx* ctx, const pic_unet_train_desc* td,
         if (rc) break;
         float* xd = xs + s * Vc * 2;
         float* td_ = ts + s * Vc;
-        (void)hipMemsetAsync(ss, 0, 3 * sizeof(double), ctx->stream);
-        unsigned nbc = xcd_blocks(Vc, 256), nbf = xcd_blocks(V, 256);
+        (void)hipMemsetAsync(ss, 0, 3 * SSIZE * sizeof(double), ctx->stream);  // striped
+        unsigned nbc = xcd_blocks1(Vc, 256), nbf = xcd_blocks(V, 256);
         if (u->coarsen == 2)
             k_res_restrict<2><<<nbc, 256, 0, ctx->stream>>>(df, dc, dx2, ctx->ntype, tf, zp, xd, ss, nullptr);
         else
@@ -1720,15 +2324,17 @@ extern "C" int pic_unet_train(pic_ctx* ctx, const pic_unet_train_desc* td,
         else
             k_restrict_target<1><<<nbc, 256, 0, ctx->stream>>>(df, dc, ctx->ntype, ctx->phi, td_, ss);
         k_scale_pair<<<ew(Vc), 256, 0, ctx->stream>>>(Vc, ss, xd, td_);
-        double h2[3];
-        (void)hipMemcpyAsync(h2, ss, sizeof(h2), hipMemcpyDeviceToHost, ctx->stream);
+        std::vector<double> hs(3 * SSIZE);
+        (void)hipMemcpyAsync(hs.data(), ss, hs.size() * sizeof(double), hipMemcpyDeviceToHost, ctx->stream);
         e = hipStreamSynchronize(ctx->stream);
         if (e != hipSuccess) {
             rc = pic_hip_check(ctx, e, "training data");
             break;
         }
-        nvalid[s] = std::max(1.0, h2[1]);
-        tms[s] = h2[0] > 0 ? h2[2] / h2[0] : 0.0;  // mean square of the scaled target
+        const double x2s = shost(hs.data()), cnt = shost(hs.data() + SSIZE),
+                     t2s = shost(hs.data() + 2 * SSIZE);
+        nvalid[s] = std::max(1.0, cnt);
+        tms[s] = x2s > 0 ? t2s / x2s : 0.0;  // mean square of the scaled target
     }
     // one constant for the whole set brings the targets to unit RMS on average
     // (the solver's line search makes the network's output scale irrelevant).

@@ -114,7 +114,8 @@ def _bf16(a):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("channels,n", [(C, 16), ((16, 32, 32, 32), 16), ((32, 32), 8)])
+@pytest.mark.parametrize("channels,n", [(C, 16), ((16, 32, 32, 32), 16), ((32, 32), 8),
+                                        ((16, 64, 64), 16)])
 def test_gpu_unet_forward_and_gradient_match_oracle(channels, n):
     import pic_gpu as pg
 
@@ -144,10 +145,11 @@ def test_gpu_unet_forward_and_gradient_match_oracle(channels, n):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("channels", [(32, 32), (16, 32)])
+@pytest.mark.parametrize("channels", [(32, 32), (16, 32), (32, 64)])
 def test_gpu_unet_forward_large_grid_matches_oracle(channels):
-    """64^3: the large-grid convolution variants (one and two N-blocks of 16
-    output channels per workgroup) against the FP64 oracle."""
+    """64^3: the large-grid convolution variants (register-blocked weights for
+    32 and 64 output channels, the LDS-staged kernel for 16, the input and
+    output layers, the fused up-convolution) against the FP64 oracle."""
     import pic_gpu as pg
 
     n = 64
