@@ -126,7 +126,8 @@ __global__ void __launch_bounds__(BLOCK)
                 if (ok && g.nrect) ok = !in_rect(g, i, j, k, fx, fy, fz);
             }
             if (ok) {
-                b.cell[p] = pack_cell(i, j, k);
+                const uint32_t nc = pack_cell(i, j, k);
+                if (nc != c) b.cell[p] = nc;  // most particles stay in their cell
                 key = (uint32_t)cell_index(g, i, j, k);
             } else {
                 b.cell[p] = DEAD;
@@ -244,24 +245,44 @@ __global__ void __launch_bounds__(BLOCK)
     }
 }
 
+// SC_PT particles per thread, BLOCK apart (coalesced); every load that
+// depends only on the particle index is issued before the offset lookup
+constexpr int SC_PT = 2;
 __global__ void __launch_bounds__(BLOCK)
     k_scatter(GridC g, SpeciesBuf src, SpeciesBuf dst, const uint32_t* __restrict__ rank,
               const uint32_t* __restrict__ off, const uint32_t* __restrict__ d_n, uint32_t cap) {
-    size_t p = (size_t)blockIdx.x * BLOCK + threadIdx.x;
-    if (p >= d_n[1]) return;
-    uint32_t c = src.cell[p];
-    if (c == DEAD) return;
-    int i, j, k;
-    unpack_cell(c, i, j, k);
-    uint32_t dst_i = off[cell_index(g, i, j, k)] + rank[p];
-    if (dst_i >= cap) return;  // cannot happen with a current histogram
-    dst.cell[dst_i] = c;
+    const uint32_t n = d_n[1];
+    const size_t base = (size_t)blockIdx.x * (BLOCK * SC_PT) + threadIdx.x;
+    uint32_t c[SC_PT], rk[SC_PT];
+    float f[SC_PT][3], v[SC_PT][3], w[SC_PT];
 #pragma unroll
-    for (int d = 0; d < 3; d++) {
-        dst.f[d][dst_i] = src.f[d][p];
-        dst.v[d][dst_i] = src.v[d][p];
+    for (int e = 0; e < SC_PT; e++) {
+        const size_t p = base + (size_t)BLOCK * e;
+        const bool valid = p < n;
+        c[e] = valid ? src.cell[p] : DEAD;
+        rk[e] = valid ? rank[p] : 0u;
+#pragma unroll
+        for (int d = 0; d < 3; d++) {
+            f[e][d] = valid ? src.f[d][p] : 0.f;
+            v[e][d] = valid ? src.v[d][p] : 0.f;
+        }
+        w[e] = valid ? src.w[p] : 0.f;
     }
-    dst.w[dst_i] = src.w[p];
+#pragma unroll
+    for (int e = 0; e < SC_PT; e++) {
+        if (c[e] == DEAD) continue;
+        int i, j, k;
+        unpack_cell(c[e], i, j, k);
+        const uint32_t dst_i = off[cell_index(g, i, j, k)] + rk[e];
+        if (dst_i >= cap) continue;  // cannot happen with a current histogram
+        dst.cell[dst_i] = c[e];
+#pragma unroll
+        for (int d = 0; d < 3; d++) {
+            dst.f[d][dst_i] = f[e][d];
+            dst.v[d][dst_i] = v[e][d];
+        }
+        dst.w[dst_i] = w[e];
+    }
 }
 
 // ---- deposit: one workgroup per cell row (j, k); the row's particles are
@@ -392,25 +413,67 @@ __global__ void __launch_bounds__(BLOCK)
 // eight cells around the node, of that cell's sum for this corner (plain
 // store; Species.cpp:401-416 scatters the same weights with atomics).
 // Corner q = (dk*2 + dj)*2 + di of cell (i, j, k) is node (i+di, j+dj, k+dk).
+// Pass 1 stages a workgroup's particles through LDS: the workgroup owns
+// BLOCK consecutive cells, whose particles form one contiguous range; it
+// loads the range in chunks of DEP_CHUNK particles with 16-byte loads (all
+// issued before the first LDS store), then every thread folds the part of
+// its cell's particles that lies in the chunk, in particle order.
+constexpr int DEP_CHUNK = 2048;
 __global__ void __launch_bounds__(BLOCK)
     k_dep_cells(GridC g, SpeciesBuf b, const uint32_t* __restrict__ off, float* __restrict__ corner) {
-    int c = blockIdx.x * BLOCK + threadIdx.x;
-    if (c >= g.ncells) return;
-    uint32_t p0 = off[c], p1 = off[c + 1];
-    float acc[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
-    for (uint32_t p = p0; p < p1; p++) {
-        float fx = b.f[0][p], fy = b.f[1][p], fz = b.f[2][p], w = b.w[p];
-        float gx = 1.f - fx, gy = 1.f - fy, gz = 1.f - fz;
-        float wgz = w * gz, wfz = w * fz;
-        acc[0] += gx * gy * wgz;
-        acc[1] += fx * gy * wgz;
-        acc[2] += gx * fy * wgz;
-        acc[3] += fx * fy * wgz;
-        acc[4] += gx * gy * wfz;
-        acc[5] += fx * gy * wfz;
-        acc[6] += gx * fy * wfz;
-        acc[7] += fx * fy * wfz;
+    __shared__ float4 sf[4][DEP_CHUNK / 4];  // fx, fy, fz, w of the chunk
+    const int tid = threadIdx.x;
+    const int c = blockIdx.x * BLOCK + tid;
+    const int clast = min((int)(blockIdx.x + 1) * BLOCK, g.ncells);  // one past the block's cells
+    const uint32_t P0 = off[blockIdx.x * BLOCK], P1 = off[clast];
+    uint32_t p0 = 0, p1 = 0;
+    if (c < g.ncells) {
+        p0 = off[c];
+        p1 = off[c + 1];
     }
+    float acc[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+    // chunk starts are multiples of 4 (16-byte loads); the particle arrays
+    // are padded to 64 entries, so a group of 4 never leaves its array
+    for (uint32_t cs = P0 & ~3u; cs < P1; cs += DEP_CHUNK) {
+        constexpr int PER = DEP_CHUNK / 4 / BLOCK;  // float4 per array per thread
+        float4 r[4][PER];
+#pragma unroll
+        for (int e = 0; e < PER; e++) {
+            const uint32_t q = cs + 4 * (tid + BLOCK * e);
+            const bool in = q < P1;
+            r[0][e] = in ? *reinterpret_cast<const float4*>(b.f[0] + q) : make_float4(0.f, 0.f, 0.f, 0.f);
+            r[1][e] = in ? *reinterpret_cast<const float4*>(b.f[1] + q) : make_float4(0.f, 0.f, 0.f, 0.f);
+            r[2][e] = in ? *reinterpret_cast<const float4*>(b.f[2] + q) : make_float4(0.f, 0.f, 0.f, 0.f);
+            r[3][e] = in ? *reinterpret_cast<const float4*>(b.w + q) : make_float4(0.f, 0.f, 0.f, 0.f);
+        }
+        __syncthreads();  // the previous chunk is consumed
+#pragma unroll
+        for (int e = 0; e < PER; e++)
+#pragma unroll
+            for (int a = 0; a < 4; a++) sf[a][tid + BLOCK * e] = r[a][e];
+        __syncthreads();
+        const float* fx_ = reinterpret_cast<const float*>(sf[0]);
+        const float* fy_ = reinterpret_cast<const float*>(sf[1]);
+        const float* fz_ = reinterpret_cast<const float*>(sf[2]);
+        const float* w_ = reinterpret_cast<const float*>(sf[3]);
+        const uint32_t ce = cs + DEP_CHUNK;
+        const uint32_t lo = p0 > cs ? p0 : cs, hi = p1 < ce ? p1 : ce;
+        for (uint32_t p = lo; p < hi; p++) {
+            const uint32_t l = p - cs;
+            float fx = fx_[l], fy = fy_[l], fz = fz_[l], w = w_[l];
+            float gx = 1.f - fx, gy = 1.f - fy, gz = 1.f - fz;
+            float wgz = w * gz, wfz = w * fz;
+            acc[0] += gx * gy * wgz;
+            acc[1] += fx * gy * wgz;
+            acc[2] += gx * fy * wgz;
+            acc[3] += fx * fy * wgz;
+            acc[4] += gx * gy * wfz;
+            acc[5] += fx * gy * wfz;
+            acc[6] += gx * fy * wfz;
+            acc[7] += fx * fy * wfz;
+        }
+    }
+    if (c >= g.ncells) return;
 #pragma unroll
     for (int q = 0; q < 8; q++) corner[(size_t)q * g.ncells + c] = acc[q];
 }
@@ -579,7 +642,7 @@ extern "C" int pic_sort(pic_ctx* ctx, int sid) {
     {
         PicProfScope ps(ctx, PIC_K_SCATTER);
         size_t n = s.n_upper ? s.n_upper : 1;
-        k_scatter<<<grid_for(n, BLOCK), BLOCK, 0, ctx->stream>>>(
+        k_scatter<<<grid_for((n + SC_PT - 1) / SC_PT, BLOCK), BLOCK, 0, ctx->stream>>>(
             g, s.buf[s.cur], s.buf[s.cur ^ 1], s.rank, s.off, s.d_n, (uint32_t)s.cap);
     }
     PIC_HIP(ctx, hipGetLastError());
