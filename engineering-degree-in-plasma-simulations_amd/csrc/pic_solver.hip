@@ -436,6 +436,10 @@ unsigned stencil_blocks(long long items) {
     return (unsigned)b;
 }
 
+// workgroups of the node-pair kernels (for_pairs): one pair per thread
+// (measured faster than two)
+unsigned pair_blocks(const Stencil& s) { return stencil_blocks(s.nn); }
+
 }  // namespace
 
 // ------------------------------------------------------------------------
@@ -729,7 +733,7 @@ static int solve_pcg(pic_ctx* ctx, const pic_solver_params* p, pic_solver_stats*
 // residual, one sweep in the reverse colour order (black, red).
 // ------------------------------------------------------------------------
 bool pic_unet_ready(pic_ctx* ctx);
-int pic_unet_coarse_correct(pic_ctx* ctx, const double* t, const double* z, double* c, double* res,
+int pic_unet_coarse_correct(pic_ctx* ctx, const double* t, const float* z, float* c, float* res,
                             double* ws);
 
 namespace {
@@ -835,10 +839,11 @@ __device__ inline double nbr_sum(const Stencil& s, unsigned u, bool reg, F f) {
 // One red-black Gauss-Seidel sweep on A z = r from z = 0, in one pass: the
 // red half-sweep gives z = -r / (2 sum 1/dx^2) on red regular nodes
 // ((i + j + k) even); the black half-sweep that follows reads only those red
-// values, so it is written directly in r.  Non-regular nodes get 0.
+// values, so it is written directly in r.  Non-regular nodes get 0.  z is
+// FP32 (the preconditioner's internal vectors; FP64 arithmetic).
 __global__ void __launch_bounds__(BLOCK)
     k_presmooth(Stencil s, const uint8_t* __restrict__ nt, const double* __restrict__ r,
-                double* __restrict__ z, double* __restrict__ clear, int nclear) {
+                float* __restrict__ z, double* __restrict__ clear, int nclear) {
     // the coarse correction's striped sums start from zero (no memset node)
     for (int q = blockIdx.x * BLOCK + threadIdx.x; q < nclear; q += gridDim.x * BLOCK) clear[q] = 0.0;
     // r on regular nodes, 0 elsewhere (a red value is -that / twos)
@@ -853,7 +858,7 @@ __global__ void __launch_bounds__(BLOCK)
             zu = -r[u] * s.itwos;
         else
             zu = (-nbr_sum(s, u, reg, rr) * s.itwos - r[u]) * s.itwos;
-        z[u] = reg ? zu : 0.0;
+        z[u] = reg ? (float)zu : 0.f;
     };
     for_pairs(s, [&](unsigned u, bool second, bool red) {
         node(u, red);
@@ -898,14 +903,14 @@ __global__ void __launch_bounds__(BLOCK)
 // the coarse correction fused with the next (black) half-sweep:
 // z = zin + omega c on red nodes, the Gauss-Seidel update from those values on
 // black regular nodes, 0 on non-regular nodes; omega = d[0] / d[1] (A-norm
-// optimal step along c).  zin and c vanish on non-regular nodes.
+// optimal step along c).  zin and c (FP32) vanish on non-regular nodes.
 __global__ void __launch_bounds__(BLOCK)
     k_gs_corr(Stencil s, const uint8_t* __restrict__ nt, const double* __restrict__ r,
-              const double* __restrict__ zin, const double* __restrict__ c,
+              const float* __restrict__ zin, const float* __restrict__ c,
               const double* __restrict__ d, double* __restrict__ z) {
     const double d1 = sget(d + SSIZE);
     const double om = d1 != 0.0 ? sget(d) / d1 : 0.0;
-    auto zc = [&](unsigned w) { return zin[w] + om * c[w]; };
+    auto zc = [&](unsigned w) { return (double)zin[w] + om * c[w]; };
     auto node = [&](unsigned u, bool red) {
         const bool reg = nt[u] == NT_REG;
         double zu;
@@ -925,22 +930,22 @@ __global__ void __launch_bounds__(BLOCK)
 
 // z = one red-black Gauss-Seidel sweep on A z = r from z = 0 (device vectors):
 // the learned preconditioner's pre-smoother, also used for its training pairs
-int pic_learned_presmooth(pic_ctx* ctx, const double* r, double* z) {
+int pic_learned_presmooth(pic_ctx* ctx, const double* r, float* z) {
     Stencil s = make_stencil(ctx->g, false);
-    k_presmooth<<<stencil_blocks(s.nn), BLOCK, 0, ctx->stream>>>(s, ctx->ntype, r, z, nullptr, 0);
+    k_presmooth<<<pair_blocks(s), BLOCK, 0, ctx->stream>>>(s, ctx->ntype, r, z, nullptr, 0);
     PIC_HIP(ctx, hipGetLastError());
     return PIC_OK;
 }
 
-// z = M(r); tmp holds three scratch vectors.  The closing half-sweep also
-// accumulates *zq += z.q (q may be null) and *rz += z.r; both must be zeroed
-// by the caller.
+// z = M(r); tmp holds the three FP32 vectors zpre, c, res.  The closing
+// half-sweep also accumulates *zq += z.q (q may be null) and *rz += z.r;
+// both must be zeroed by the caller.
 static int learned_precond(pic_ctx* ctx, const Stencil& s, const double* rin, double* zout,
-                           double* tmp, const double* q, double* zq, double* rz) {
-    unsigned nb = stencil_blocks(s.nn);
-    double* zpre = tmp;
-    double* c = tmp + s.nn;
-    double* res = c + s.nn;
+                           float* tmp, const double* q, double* zq, double* rz) {
+    unsigned nb = pair_blocks(s);
+    float* zpre = tmp;
+    float* c = tmp + s.nn;
+    float* res = c + s.nn;
     double* ws = ctx->scratch + LEARNED_WS;  // the coarse correction's striped sums
     double* dots = ws + 2 * SSIZE;
     k_presmooth<<<nb, BLOCK, 0, ctx->stream>>>(s, ctx->ntype, rin, zpre, ws, 4 * SSIZE);
@@ -975,7 +980,8 @@ extern "C" int pic_learned_precondition(pic_ctx* ctx, const double* r, double* z
     double* rz = ctx->scratch + LEARNED_WS + 4 * SSIZE;  // striped
     PIC_HIP(ctx, hipMemcpyAsync(rd, r, nn * 8, hipMemcpyHostToDevice, ctx->stream));
     PIC_HIP(ctx, hipMemsetAsync(rz, 0, SSIZE * sizeof(double), ctx->stream));
-    rc = learned_precond(ctx, make_stencil(ctx->g, false), rd, zd, zd + nn, nullptr, nullptr, rz);
+    rc = learned_precond(ctx, make_stencil(ctx->g, false), rd, zd, reinterpret_cast<float*>(zd + nn),
+                         nullptr, nullptr, rz);
     if (rc) return rc;
     PIC_HIP(ctx, hipMemcpyAsync(z, zd, nn * 8, hipMemcpyDeviceToHost, ctx->stream));
     PIC_HIP(ctx, hipStreamSynchronize(ctx->stream));
@@ -999,16 +1005,16 @@ static int solve_learned(pic_ctx* ctx, const pic_solver_params* p, pic_solver_st
     double* p0 = z + nn;
     double* p1 = p0 + nn;
     double* q = p1 + nn;
-    double* ctmp = q + nn;  // three vectors
+    float* ftmp = reinterpret_cast<float*>(q + nn);  // three FP32 vectors: 1.5 nn doubles
     FcgHist h;
-    h.rz = ctmp + 3 * nn;
+    h.rz = q + 3 * nn;  // past the FP32 vectors, inside the 8 nn doubles
     h.pq = h.rz + hslots;
     h.zq = h.pq + hslots;
     h.rr = h.zq + hslots;
     double* sc = ctx->scratch + 8;
     unsigned nb = stencil_blocks(s.nn);
     auto precond = [&](const double* rin, double* zout, const double* qv, double* zq, double* rz) {
-        return learned_precond(ctx, s, rin, zout, ctmp, qv, zq, rz);
+        return learned_precond(ctx, s, rin, zout, ftmp, qv, zq, rz);
     };
     double h0[3];
     PIC_HIP(ctx, hipMemsetAsync(sc, 0, 3 * sizeof(double), ctx->stream));

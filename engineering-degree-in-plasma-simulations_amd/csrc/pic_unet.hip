@@ -113,7 +113,7 @@ struct ConvArgs {
     const bf16* mask;  // optional: keep outputs where mask[v][n] > 0 (mask has cout channels)
     int relu, accumulate;
     Dims d;
-    float* outf = nullptr;  // k_conv3 only: write channel 0 as FP32 outf[v] (+ bias[0]), not out
+    bf16* pool = nullptr;  // k_conv3_rb only: also the 2x2x2 mean of the output on the half grid
 };
 
 // NB N-blocks of 16 output channels per workgroup (cout/16 divisible by NB).
@@ -357,16 +357,6 @@ __global__ void __launch_bounds__(256) k_conv3(ConvArgs a) {
 #pragma unroll
         for (int nb = 0; nb < NB; nb++) {
             int n = (nb0 + nb) * 16 + m;
-            if (a.outf) {  // single FP32 output channel (the rest of the 16 are padding)
-                if (n != 0) continue;
-                float b0 = a.bias ? a.bias[0] : 0.f;
-#pragma unroll
-                for (int j = 0; j < 4; j++) {
-                    int gx = x0 + 4 * g + j;
-                    if (gx < d.nx) a.outf[((long long)gz * d.ny + gy) * d.nx + gx] = acc[mb][nb][j] + b0;
-                }
-                continue;
-            }
             float bv = a.bias ? a.bias[n] : 0.f;
 #pragma unroll
             for (int j = 0; j < 4; j++) {
@@ -461,7 +451,15 @@ __global__ void __launch_bounds__(256) k_conv3_rb(ConvArgs a) {
                         }
                 }
     }
-    // D fragment: lane holds rows 4g + j (x offset) of column m (channel)
+    // D fragment: lane holds rows 4g + j (x offset) of column m (channel).
+    // The workgroup holds all CO = cout channels of its voxels (one N-block per
+    // wave along N): the output tile is staged in LDS as BF16, channels-last,
+    // then written with 16-byte stores (every (z, y) row of 16 voxels is
+    // contiguous in memory).
+    constexpr int CO = 16 * NWN, OZ = RZ * NWZ;
+    static_assert(OZ * RY * TX * CO * 2 <= NT * 16, "the output tile fits the staged input's LDS");
+    __syncthreads();  // every wave is done with the staged input
+    bf16* ot = reinterpret_cast<bf16*>(tile);
     const int n = n0 + m;
     const float bv = a.bias ? a.bias[n] : 0.f;
 #pragma unroll
@@ -471,14 +469,50 @@ __global__ void __launch_bounds__(256) k_conv3_rb(ConvArgs a) {
             const int gy = y0 + ry, gz = z0 + wz * RZ + rz;
 #pragma unroll
             for (int j = 0; j < 4; j++) {
-                const int gx = x0 + 4 * g + j;
-                const long long o = (((long long)gz * d.ny + gy) * d.nx + gx) * a.cout + n;
                 float v = acc[ry][rz][j] + bv;
                 if (a.relu) v = fmaxf(v, 0.f);
-                if (a.mask && !(bf2f(a.mask[o]) > 0.f)) v = 0.f;
-                a.out[o] = f2bf(v);
+                if (a.mask) {
+                    const long long o = (((long long)gz * d.ny + gy) * d.nx + x0 + 4 * g + j) * a.cout + n;
+                    if (!(bf2f(a.mask[o]) > 0.f)) v = 0.f;
+                }
+                ot[(((wz * RZ + rz) * RY + ry) * TX + 4 * g + j) * CO + wn * 16 + m] = f2bf(v);
             }
         }
+    __syncthreads();
+    {
+        constexpr int NQ = OZ * RY * TX * CO / 8;  // uint4 in the output tile
+        const uint4* oq = reinterpret_cast<const uint4*>(ot);
+#pragma unroll
+        for (int e = tid; e < NQ; e += 256) {
+            const int c8 = e % (CO / 8), vox = e / (CO / 8);
+            const int xx = vox % TX, yy = (vox / TX) % RY, zz = vox / (TX * RY);
+            *reinterpret_cast<uint4*>(a.out + (((long long)(z0 + zz) * d.ny + y0 + yy) * d.nx + x0 + xx) * CO +
+                                      8 * c8) = oq[e];
+        }
+    }
+    if (a.pool) {
+        // 2x2x2 means of the stored (BF16) outputs, children summed in
+        // k_pool's order; every pooling cell lies in this lane's registers
+        // (x pairs j, j+1; RY, RZ and the block origins are even)
+        const int px0 = (x0 + 4 * g) / 2, hy = d.ny / 2, hx = d.nx / 2;
+#pragma unroll
+        for (int py = 0; py < RY / 2; py++)
+#pragma unroll
+            for (int pz = 0; pz < RZ / 2; pz++)
+#pragma unroll
+                for (int pxx = 0; pxx < 2; pxx++) {
+                    float sum = 0.f;
+#pragma unroll
+                    for (int q = 0; q < 8; q++) {
+                        const int dx = q & 1, dy = (q >> 1) & 1, dz = q >> 2;
+                        float v = acc[2 * py + dy][2 * pz + dz][2 * pxx + dx] + bv;
+                        if (a.relu) v = fmaxf(v, 0.f);
+                        sum += bf2f(f2bf(v));
+                    }
+                    const int cz = (z0 + wz * RZ) / 2 + pz, cy = y0 / 2 + py, cx = px0 + pxx;
+                    a.pool[(((long long)cz * hy + cy) * hx + cx) * a.cout + n] = f2bf(0.125f * sum);
+                }
+    }
 }
 
 // 3x3x3 convolution for small grids (few workgroups): 16x2x2-voxel tiles,
@@ -1307,7 +1341,7 @@ __global__ void __launch_bounds__(256) k_scale_f(long long n, float f, float* __
 // preconditioner input from the fine residual of A e = t after smoothing z:
 // x2[v] = {-h^2 mean over regular nodes of (t - A z), fraction of other nodes};
 // striped sums (sums, sums + SSIZE) += {x^2, 1}; res (optional) receives
-// t - A z on the fine grid (0 on non-regular nodes).  Regular nodes are
+// t - A z on the fine grid (FP32, 0 on non-regular nodes).  z is FP32.  Regular nodes are
 // interior (Dirichlet walls), so all six neighbours exist; z is zero on
 // non-regular nodes.  CF 2 loads z once per aggregate (the 2x2x2 cube and
 // its 24 face neighbours, coordinates clamped into the grid; values of
@@ -1315,8 +1349,8 @@ __global__ void __launch_bounds__(256) k_scale_f(long long n, float f, float* __
 template <int CF>
 __global__ void __launch_bounds__(256)
     k_res_restrict(Dims df, Dims dc, double h2, const uint8_t* __restrict__ nt,
-                   const double* __restrict__ t, const double* __restrict__ z, float* __restrict__ x2,
-                   double* __restrict__ sums, double* __restrict__ res) {
+                   const double* __restrict__ t, const float* __restrict__ z, float* __restrict__ x2,
+                   double* __restrict__ sums, float* __restrict__ res) {
     double s2 = 0, cnt = 0;
     const unsigned sj = df.nx, sk = (unsigned)df.nx * df.ny;
     const double ih2 = 1.0 / h2;
@@ -1335,7 +1369,7 @@ __global__ void __launch_bounds__(256)
                 Y[q] = (unsigned)min(max(2 * cy - 1 + q, 0), df.ny - 1) * sj;
                 Z[q] = (unsigned)min(max(2 * cz - 1 + q, 0), df.nz - 1) * sk;
             }
-            auto zat = [&](int a, int b, int c) { return z[Z[c] + Y[b] + X[a]]; };
+            auto zat = [&](int a, int b, int c) { return (double)z[Z[c] + Y[b] + X[a]]; };
 #pragma unroll
             for (int q = 0; q < 8; q++) {
                 const int a = 1 + (q & 1), b = 1 + ((q >> 1) & 1), c = 1 + (q >> 2);
@@ -1347,7 +1381,7 @@ __global__ void __launch_bounds__(256)
                 double rv = reg ? t[u] - az : 0.0;
                 acc += rv;
                 nreg += reg;
-                if (res) res[u] = rv;
+                if (res) res[u] = (float)rv;
             }
         } else {
 #pragma unroll
@@ -1356,13 +1390,13 @@ __global__ void __launch_bounds__(256)
                 unsigned u = (fz * df.ny + fy) * df.nx + fx;
                 const bool reg = nt[u] == NT_REG;
                 const unsigned oi = reg ? 1u : 0u, oj = reg ? sj : 0u, ok = reg ? sk : 0u;
-                double az = (z[u - oi] + z[u + oi] + z[u - oj] + z[u + oj] + z[u - ok] + z[u + ok] -
+                double az = ((double)z[u - oi] + z[u + oi] + z[u - oj] + z[u + oj] + z[u - ok] + z[u + ok] -
                              6.0 * z[u]) *
                             ih2;
                 double rv = reg ? t[u] - az : 0.0;
                 acc += rv;
                 nreg += reg;
-                if (res) res[u] = rv;
+                if (res) res[u] = (float)rv;
             }
         }
         float xv = nreg ? (float)(-h2 * acc / nreg) : 0.f;
@@ -1415,20 +1449,20 @@ __device__ inline double interp(const Dims& dc, const float* __restrict__ y, int
 template <int CF>
 __global__ void __launch_bounds__(256)
     k_prolong_dots(Dims df, Dims dc, double ih2, const uint8_t* __restrict__ nt,
-                   const float* __restrict__ y, const double* __restrict__ res, double* __restrict__ c,
+                   const float* __restrict__ y, const float* __restrict__ res, float* __restrict__ c,
                    double* __restrict__ out) {
     const long long sj = df.nx, sk = (long long)df.nx * df.ny;
     double acc[2] = {0, 0};
     long long end;
     for (long long u = xcd_chunk(df.v, 256, end) + threadIdx.x; u < end; u += 256) {
         if (nt[u] != NT_REG) {
-            c[u] = 0.0;
+            c[u] = 0.f;
             continue;
         }
         int i, j, k;
         vox_xyz(df, u, i, j, k);
         double cu = interp<CF>(dc, y, i, j, k);
-        c[u] = cu;
+        c[u] = (float)cu;
         acc[0] += cu * res[u];
         double e = 0.0;
         const long long st[3] = {1, sj, sk};
@@ -1465,8 +1499,8 @@ __global__ void __launch_bounds__(256)
 // zero off the regular nodes is -(1/h^2) sum over grid edges (c_u - c_v)^2.
 __global__ void __launch_bounds__(256)
     k_prolong_dots_agg(Dims df, Dims dc, double ih2, const uint8_t* __restrict__ nt,
-                       const float* __restrict__ y, const double* __restrict__ res,
-                       double* __restrict__ c, double* __restrict__ out) {
+                       const float* __restrict__ y, const float* __restrict__ res,
+                       float* __restrict__ c, double* __restrict__ out) {
     const int ex = (df.nx + 1) >> 1, ey = (df.ny + 1) >> 1, ez = (df.nz + 1) >> 1;
     const long long na = (long long)ex * ey * ez;
     const unsigned sj = df.nx, sk = (unsigned)df.nx * df.ny;
@@ -1530,7 +1564,7 @@ __global__ void __launch_bounds__(256)
             if (fx >= df.nx || fy >= df.ny || fz >= df.nz) continue;
             const unsigned u = (unsigned)fz * sk + (unsigned)fy * sj + (unsigned)fx;
             const double cu = C[cc][b][a];
-            c[u] = cu;
+            c[u] = (float)cu;  // exact: interpolated in FP32
             const double rv = res[u];
             acc0 += cu != 0.0 ? cu * rv : 0.0;  // res is defined on regular nodes only
             if (fx + 1 < df.nx) e += (cu - C[cc][b][a + 1]) * (cu - C[cc][b][a + 1]);
@@ -1742,11 +1776,13 @@ static int unet_sync_weights(pic_ctx* ctx, UNet* u) {
     return PIC_OK;
 }
 
-static void conv(pic_ctx* ctx, const Dims& d, const bf16* in, int cin, const bf16* w,
+// returns whether the kernel also wrote the 2x2x2 mean of the output to pool
+// (only the register-blocked kernel does; the caller pools otherwise)
+static bool conv(pic_ctx* ctx, const Dims& d, const bf16* in, int cin, const bf16* w,
                  const float* bias, bf16* out, int cout, int taps, int relu, const bf16* mask,
-                 float* outf = nullptr) {
+                 bf16* pool = nullptr) {
     ConvArgs a{in, cin, w, bias, out, cout, mask, relu, 0, d};
-    a.outf = outf;
+    if (mask) pool = nullptr;  // pooling follows the unmasked forward outputs only
     int ntx = (d.nx + TX - 1) / TX, nty = (d.ny + TY - 1) / TY, ntz = (d.nz + TZ - 1) / TZ;
     int tiles = ntx * nty * ntz;
     // two N-blocks per workgroup when that still leaves >= 512 workgroups
@@ -1761,26 +1797,28 @@ static void conv(pic_ctx* ctx, const Dims& d, const bf16* in, int cin, const bf1
         if ((cin == 32 || cin == 64) && cout == 32 && d.nx % 16 == 0 && d.ny % 4 == 0 &&
             d.nz % 8 == 0 && (d.nx / 16) * (d.ny / 4) * (d.nz / 8) >= 256) {
             dim3 gr((d.nx / 16) * (d.ny / 4) * (d.nz / 8), 1);
+            a.pool = pool;
             if (cin == 32)
                 k_conv3_rb<1, 4, 4, 2><<<gr, 256, 0, ctx->stream>>>(a);
             else
                 k_conv3_rb<2, 4, 4, 2><<<gr, 256, 0, ctx->stream>>>(a);
-            return;
+            return pool != nullptr;
         }
         if ((cin == 32 || cin == 64) && cout == 64 && d.nx % 16 == 0 && d.ny % 2 == 0 &&
             d.nz % 4 == 0 && (d.nx / 16) * (d.ny / 2) * (d.nz / 4) >= 256) {
             dim3 gr((d.nx / 16) * (d.ny / 2) * (d.nz / 4), 1);
+            a.pool = pool;
             if (cin == 32)
                 k_conv3_rb<1, 2, 4, 4><<<gr, 256, 0, ctx->stream>>>(a);
             else
                 k_conv3_rb<2, 2, 4, 4><<<gr, 256, 0, ctx->stream>>>(a);
-            return;
+            return pool != nullptr;
         }
         if (tiles * (cout / 16) < 512) {  // small grid: 16x2x2 tiles, taps split over the waves
             int t4 = ntx * ((d.ny + 1) / 2) * ((d.nz + 1) / 2);
-            if (cin == 64 && !outf)
+            if (cin == 64)
                 k_conv3_small<64><<<dim3(t4, cout / 16), 256, 0, ctx->stream>>>(a);
-            else if (cin == 32 && !outf)
+            else if (cin == 32)
                 k_conv3_small<32><<<dim3(t4, cout / 16), 256, 0, ctx->stream>>>(a);
             else if (pf)
                 k_conv3<1, 4, true><<<dim3(t4, cout / 16), 256, 0, ctx->stream>>>(a);
@@ -1805,6 +1843,7 @@ static void conv(pic_ctx* ctx, const Dims& d, const bf16* in, int cin, const bf1
         else
             k_conv_mfma<1, 1><<<grid, 256, 0, ctx->stream>>>(a);
     }
+    return false;
 }
 
 static int wgrad_tiles(const Dims& d, int* tpb) {
@@ -1850,13 +1889,16 @@ static int unet_forward(pic_ctx* ctx, UNet* u, const float* x, const double* nrm
         else
             k_conv_in<1><<<tiles, 256, 0, ctx->stream>>>(d0, x, nrm, u->wi_pad, P + li.boff, lv[0].a);
     }
-    conv(ctx, d0, lv[0].a, C0, L[u->l_b[0]].wb, P + L[u->l_b[0]].boff, lv[0].s, C0, 27, 1, nullptr);
+    // B_l also writes the pooled input of level l + 1 where its kernel can
+    bool pooled = conv(ctx, d0, lv[0].a, C0, L[u->l_b[0]].wb, P + L[u->l_b[0]].boff, lv[0].s, C0, 27, 1,
+                       nullptr, lv[1].p);
     for (int l = 1; l < D; l++) {
         const Dims& d = lv[l].d;
         int Cp = lv[l - 1].C, Cl = lv[l].C;
-        k_pool<<<ew(d.v * Cp / 8), 256, 0, ctx->stream>>>(d, lv[l - 1].s, lv[l].p, Cp);
+        if (!pooled) k_pool<<<ew(d.v * Cp / 8), 256, 0, ctx->stream>>>(d, lv[l - 1].s, lv[l].p, Cp);
         conv(ctx, d, lv[l].p, Cp, L[u->l_a[l]].wb, P + L[u->l_a[l]].boff, lv[l].a, Cl, 27, 1, nullptr);
-        conv(ctx, d, lv[l].a, Cl, L[u->l_b[l]].wb, P + L[u->l_b[l]].boff, lv[l].s, Cl, 27, 1, nullptr);
+        pooled = conv(ctx, d, lv[l].a, Cl, L[u->l_b[l]].wb, P + L[u->l_b[l]].boff, lv[l].s, Cl, 27, 1,
+                      nullptr, l + 1 < D ? lv[l + 1].p : nullptr);
     }
     for (int l = D - 2; l >= 0; l--) {
         const Dims& d = lv[l].d;
@@ -2153,13 +2195,15 @@ __global__ void __launch_bounds__(256)
 // returns e with A e = t - A z (A = 7-point Laplacian / h^2, interior nodes)
 __global__ void __launch_bounds__(256)
     k_res_rho(Dims df, double ih2, const uint8_t* __restrict__ nt, const double* __restrict__ t,
-              const double* __restrict__ z, double* __restrict__ rho) {
+              const float* __restrict__ z, double* __restrict__ rho) {
     const long long sj = df.nx, sk = (long long)df.nx * df.ny;
     long long end;
     for (long long u = xcd_chunk(df.v, 256, end) + threadIdx.x; u < end; u += 256) {
         double v = 0.0;
         if (nt[u] == NT_REG) {
-            double az = (z[u - 1] + z[u + 1] + z[u - sj] + z[u + sj] + z[u - sk] + z[u + sk] - 6.0 * z[u]) * ih2;
+            double az = ((double)z[u - 1] + z[u + 1] + z[u - sj] + z[u + sj] + z[u - sk] + z[u + sk] -
+                         6.0 * z[u]) *
+                        ih2;
             v = -EPS0 * (t[u] - az);
         }
         rho[u] = v;
@@ -2179,15 +2223,15 @@ bool pic_unet_ready(pic_ctx* ctx) {
 }
 
 // Coarse correction of the two-level preconditioner: the residual
-// res = t - A z (t: the fine right-hand side, z: the smoothed approximation,
-// device FP64; res is stored) is restricted to the network grid, the network
-// is applied and its output interpolated to c (device FP64).  ws holds four
+// res = t - A z (t: the fine right-hand side, FP64; z: the smoothed
+// approximation, FP32; res is stored in FP32) is restricted to the network
+// grid, the network is applied and its output interpolated to c (FP32).  ws holds four
 // striped scalars that the caller has zeroed: the restriction's {x^2, count}
 // and then dots = ws + 2 SSIZE, which receives {c.res, c.A c}, from which the
 // caller takes the A-norm optimal step omega = (c.res) / (c.A c) for
 // z += omega c.  Only for Dirichlet-walled uniform grids (every regular node
 // is interior).
-int pic_unet_coarse_correct(pic_ctx* ctx, const double* t, const double* z, double* c, double* res,
+int pic_unet_coarse_correct(pic_ctx* ctx, const double* t, const float* z, float* c, float* res,
                             double* ws) {
     UNet* u = get_unet(ctx);
     if (!u) return pic_fail(ctx, PIC_ERR_STATE, "no U-Net");
@@ -2215,7 +2259,7 @@ int pic_unet_coarse_correct(pic_ctx* ctx, const double* t, const double* z, doub
 }
 
 int pic_poisson_internal_sor(pic_ctx* ctx, double tol, int max_it, pic_solver_stats* st);
-int pic_learned_presmooth(pic_ctx* ctx, const double* r, double* z);
+int pic_learned_presmooth(pic_ctx* ctx, const double* r, float* z);
 
 extern "C" int pic_unet_train(pic_ctx* ctx, const pic_unet_train_desc* td,
                               pic_unet_train_stats* stats) {
@@ -2234,7 +2278,8 @@ extern "C" int pic_unet_train(pic_ctx* ctx, const pic_unet_train_desc* td,
     long long Vc = dc.v;
     size_t S = (size_t)td->samples;
     float *dens = nullptr, *xs = nullptr, *ts = nullptr;
-    double *phi_keep = nullptr, *rho_keep = nullptr, *tf = nullptr, *zp = nullptr;
+    double *phi_keep = nullptr, *rho_keep = nullptr, *tf = nullptr;
+    float* zp = nullptr;
     Blob* dblobs = nullptr;
     auto cleanup = [&]() {
         (void)hipFree(dens);
@@ -2252,7 +2297,7 @@ extern "C" int pic_unet_train(pic_ctx* ctx, const pic_unet_train_desc* td,
     if (e == hipSuccess) e = hipMalloc(&phi_keep, V * sizeof(double));
     if (e == hipSuccess) e = hipMalloc(&rho_keep, V * sizeof(double));
     if (e == hipSuccess) e = hipMalloc(&tf, V * sizeof(double));
-    if (e == hipSuccess) e = hipMalloc(&zp, V * sizeof(double));
+    if (e == hipSuccess) e = hipMalloc(&zp, V * sizeof(float));
     if (e == hipSuccess) e = hipMalloc(&dblobs, 64 * sizeof(Blob));
     if (e == hipSuccess) e = hipMemcpyAsync(phi_keep, ctx->phi, V * 8, hipMemcpyDeviceToDevice, ctx->stream);
     if (e == hipSuccess) e = hipMemcpyAsync(rho_keep, ctx->rho, V * 8, hipMemcpyDeviceToDevice, ctx->stream);

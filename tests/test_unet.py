@@ -170,9 +170,10 @@ def test_gpu_unet_forward_large_grid_matches_oracle(channels):
 @pytest.mark.gpu
 @pytest.mark.parametrize("coarsen", [1, 2])
 def test_gpu_learned_preconditioner_matches_oracle(coarsen):
-    """M(r) on the device (FP64 smoothing, BF16 network) against the FP64
-    oracle with the same BF16-rounded weights; the smoothing is exact, the
-    network's correction carries the BF16 error."""
+    """M(r) on the device (FP64 smoothing arithmetic on FP32 intermediate
+    vectors, BF16 network) against the FP64 oracle with the same BF16-rounded
+    weights; the smoothing carries FP32 rounding (well below 1e-6 relative),
+    the network's correction the BF16 error."""
     import pic_gpu as pg
 
     n = 32
@@ -192,7 +193,7 @@ def test_gpu_learned_preconditioner_matches_oracle(coarsen):
     err = float(np.linalg.norm(z - z_ref))
     nc, nz = float(np.linalg.norm(corr)), float(np.linalg.norm(z_ref))
     assert nc > 1e-3 * nz  # the line-searched correction is present
-    assert err <= 3e-2 * nc + 1e-9 * nz, (err, nc, nz)
+    assert err <= 3e-2 * nc + 1e-6 * nz, (err, nc, nz)
     assert np.all(z[~regular] == 0)
 
 
