@@ -215,6 +215,15 @@ int pic_hip_check(pic_ctx* ctx, hipError_t e, const char* what);
         hipError_t e_ = (call);                                   \
         if (e_ != hipSuccess) return pic_hip_check(ctx, e_, #call); \
     } while (0)
+// Wait for a stream by polling it: the host resumes as soon as the queued
+// work ends.  For the solvers' short, frequent convergence checks, where a
+// blocking wait left the GPU idle for tens of microseconds per check.
+inline hipError_t pic_spin_wait(hipStream_t s) {
+    hipError_t e;
+    while ((e = hipStreamQuery(s)) == hipErrorNotReady) {
+    }
+    return e;
+}
 // profiling brackets around a launch
 void pic_prof_begin(pic_ctx* ctx, int cat, hipEvent_t* a, hipEvent_t* b);
 void pic_prof_end(pic_ctx* ctx, int cat, hipEvent_t a, hipEvent_t b);

@@ -477,7 +477,7 @@ static int residual_norms(pic_ctx* ctx, const Stencil& st, const SorArgs& a, dou
     std::vector<double> h(3 * SSIZE);
     PIC_HIP(ctx, hipMemcpyAsync(h.data(), sums, h.size() * sizeof(double), hipMemcpyDeviceToHost,
                                 ctx->stream));
-    PIC_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    PIC_HIP(ctx, pic_spin_wait(ctx->stream));
     for (int q = 0; q < 3; q++) out[q] = shost(h.data() + q * SSIZE);
     return PIC_OK;
 }
@@ -569,7 +569,7 @@ static int pcg_inner(pic_ctx* ctx, const Stencil& s, T* base, const double* P, d
     std::vector<double> hg(max_it + 2);
     PIC_HIP(ctx, hipMemcpyAsync(hg.data(), h.gg, sizeof(double), hipMemcpyDeviceToHost,
                                 ctx->stream));
-    PIC_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    PIC_HIP(ctx, pic_spin_wait(ctx->stream));
     *gg_last = hg[0];
     if (hg[0] <= target2) {  // already converged (zero RHS included)
         *iters = 0;
@@ -592,7 +592,7 @@ static int pcg_inner(pic_ctx* ctx, const Stencil& s, T* base, const double* P, d
         PIC_HIP(ctx, hipMemcpyAsync(hg.data() + last_checked + 1, h.gg + last_checked + 1,
                                     (size_t)(t - last_checked) * sizeof(double),
                                     hipMemcpyDeviceToHost, ctx->stream));
-        PIC_HIP(ctx, hipStreamSynchronize(ctx->stream));
+        PIC_HIP(ctx, pic_spin_wait(ctx->stream));
         for (int q = last_checked + 1; q <= t; q++) {
             if (!(hg[q] == hg[q])) return pic_fail(ctx, PIC_ERR_NUMERIC, "PCG produced NaN");
             if (hg[q] <= target2) {
@@ -648,7 +648,7 @@ static int solve_pcg(pic_ctx* ctx, const pic_solver_params* p, pic_solver_stats*
         PIC_HIP(ctx, hipGetLastError());
         PIC_HIP(ctx, hipMemcpyAsync(fr, ctx->scratch, sizeof(fr), hipMemcpyDeviceToHost,
                                     ctx->stream));
-        PIC_HIP(ctx, hipStreamSynchronize(ctx->stream));
+        PIC_HIP(ctx, pic_spin_wait(ctx->stream));
         // target on the inner residual g (g = -F at y = 0)
         double target2;
         if (p->tol_kind == PIC_TOL_REF_RMS)
@@ -688,7 +688,7 @@ static int solve_pcg(pic_ctx* ctx, const pic_solver_params* p, pic_solver_stats*
         double yy;
         PIC_HIP(ctx, hipMemcpyAsync(&yy, ctx->scratch, sizeof(double), hipMemcpyDeviceToHost,
                                     ctx->stream));
-        PIC_HIP(ctx, hipStreamSynchronize(ctx->stream));
+        PIC_HIP(ctx, pic_spin_wait(ctx->stream));
         if (nonlinear) {
             if (std::sqrt(yy / (double)nn) < 1e-3) {  // NR_TOL, PotentialSolver.cpp:180, :229-233
                 conv = true;
@@ -984,7 +984,7 @@ extern "C" int pic_learned_precondition(pic_ctx* ctx, const double* r, double* z
                          nullptr, nullptr, rz);
     if (rc) return rc;
     PIC_HIP(ctx, hipMemcpyAsync(z, zd, nn * 8, hipMemcpyDeviceToHost, ctx->stream));
-    PIC_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    PIC_HIP(ctx, pic_spin_wait(ctx->stream));
     return PIC_OK;
 }
 
@@ -1012,7 +1012,7 @@ static int solve_learned(pic_ctx* ctx, const pic_solver_params* p, pic_solver_st
     h.zq = h.pq + hslots;
     h.rr = h.zq + hslots;
     double* sc = ctx->scratch + 8;
-    unsigned nb = stencil_blocks(s.nn);
+    unsigned nb = stencil_blocks(s.nn), nbr = stencil_blocks_red(s.nn);
     auto precond = [&](const double* rin, double* zout, const double* qv, double* zq, double* rz) {
         return learned_precond(ctx, s, rin, zout, ftmp, qv, zq, rz);
     };
@@ -1021,7 +1021,7 @@ static int solve_learned(pic_ctx* ctx, const pic_solver_params* p, pic_solver_st
     k_fcg_resid<<<stencil_blocks_red(s.nn), BLOCK, 0, ctx->stream>>>(s, ctx->ntype, ctx->phi, ctx->rho,
                                                                      1.0 / EPS0, r, sc);
     PIC_HIP(ctx, hipMemcpyAsync(h0, sc, sizeof(h0), hipMemcpyDeviceToHost, ctx->stream));
-    PIC_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    PIC_HIP(ctx, pic_spin_wait(ctx->stream));
     double bnorm2 = h0[1] > 0 ? h0[1] : 1.0;
     double target2 = (p->tol_kind == PIC_TOL_REF_RMS) ? p->tol * p->tol * (double)nn
                                                       : p->tol * p->tol * bnorm2;
@@ -1042,7 +1042,7 @@ static int solve_learned(pic_ctx* ctx, const pic_solver_params* p, pic_solver_st
         for (int k = 0; k < max_fcg; k++) {
             {
                 PicProfScope ps(ctx, PIC_K_PCG);
-                k_fcg_pq<<<nb, BLOCK, 0, ctx->stream>>>(s, ctx->ntype, z, pold, pnew, q, h, k);
+                k_fcg_pq<<<nbr, BLOCK, 0, ctx->stream>>>(s, ctx->ntype, z, pold, pnew, q, h, k);
                 k_fcg_update<<<nb, BLOCK, 0, ctx->stream>>>(s.nn, h, k, ctx->phi, pnew, r, q);
             }
             PIC_HIP(ctx, hipGetLastError());
@@ -1052,7 +1052,7 @@ static int solve_learned(pic_ctx* ctx, const pic_solver_params* p, pic_solver_st
                 PIC_HIP(ctx, hipMemcpyAsync(rrs.data() + (size_t)last * SSIZE, h.rr + (size_t)last * SSIZE,
                                             (size_t)(it - last) * SSIZE * sizeof(double),
                                             hipMemcpyDeviceToHost, ctx->stream));
-                PIC_HIP(ctx, hipStreamSynchronize(ctx->stream));
+                PIC_HIP(ctx, pic_spin_wait(ctx->stream));
                 for (int j = last; j < it; j++) {
                     rr[j] = shost(rrs.data() + (size_t)j * SSIZE);
                     if (!(rr[j] == rr[j])) return pic_fail(ctx, PIC_ERR_NUMERIC, "learned PCG produced NaN");

@@ -628,8 +628,9 @@ template <int NB>
 __global__ void __launch_bounds__(256)
     k_conv_in(Dims d, const float* __restrict__ x, const double* __restrict__ nrm,
               const bf16* __restrict__ w, const float* __restrict__ bias, bf16* __restrict__ out) {
-    constexpr int TY = 4, TZ = 4, SX = TX + 2, SY = TY + 2, SZ = TZ + 2;
-    __shared__ uint32_t tile[SX * SY * SZ];  // one BF16 pair per voxel
+    constexpr int TY = 4, TZ = 4, SX = TX + 2, SY = TY + 2, SZ = TZ + 2, CO = 16 * NB;
+    __shared__ uint32_t tile[SX * SY * SZ];         // one BF16 pair per voxel
+    __shared__ uint4 ot[TX * TY * TZ * CO / 8];     // output tile, channels-last BF16
     float sc = 1.f;
     if (nrm) {
         const double s0 = sget(nrm), s1 = sget(nrm + SSIZE);  // whole wave
@@ -691,24 +692,27 @@ __global__ void __launch_bounds__(256)
                 acc[mb][nb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(afr, bfr[st][nb], acc[mb][nb], 0, 0, 0);
         }
     }
-    // D fragment: lane holds rows 4g + j (x offset) of column m (channel)
+    // D fragment: lane holds rows 4g + j (x offset) of column m (channel);
+    // staged in LDS, then written with 16-byte stores
+    bf16* otb = reinterpret_cast<bf16*>(ot);
 #pragma unroll
     for (int mb = 0; mb < 4; mb++) {
         const int row = wave * 4 + mb;
-        const int gy = y0 + row % TY, gz = z0 + row / TY;
-        if (gy >= d.ny || gz >= d.nz) continue;
 #pragma unroll
         for (int nb = 0; nb < NB; nb++) {
             const int n = nb * 16 + m;
             const float bv = bias[n];
 #pragma unroll
-            for (int j = 0; j < 4; j++) {
-                int gx = x0 + 4 * g + j;
-                if (gx >= d.nx) continue;
-                long long o = (((long long)gz * d.ny + gy) * d.nx + gx) * (16 * NB) + n;
-                out[o] = f2bf(fmaxf(acc[mb][nb][j] + bv, 0.f));
-            }
+            for (int j = 0; j < 4; j++) otb[(row * TX + 4 * g + j) * CO + n] = f2bf(fmaxf(acc[mb][nb][j] + bv, 0.f));
         }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int e = tid; e < TX * TY * TZ * CO / 8; e += 256) {
+        const int c8 = e % (CO / 8), vox = e / (CO / 8);
+        const int row = vox / TX, gx = x0 + vox % TX, gy = y0 + row % TY, gz = z0 + row / TY;
+        if (gx < d.nx && gy < d.ny && gz < d.nz)
+            *reinterpret_cast<uint4*>(out + (((long long)gz * d.ny + gy) * d.nx + gx) * CO + 8 * c8) = ot[e];
     }
 }
 
@@ -911,58 +915,53 @@ __global__ void __launch_bounds__(256)
 
 // Decoder input m = up(conv1(h)) + skip on the fine level: the 1x1
 // up-convolution of the coarse activation h (32 KS channels) on
-// v_mfma_f32_16x16x32_bf16 (M = 16 coarse voxels per wave, N = 16 NB output
-// channels, bias added), staged in LDS, then nearest upsampling plus the skip
-// connection for the 8 children of each coarse voxel, 16 bytes per access.
-// Fine dimensions are exactly twice the coarse ones.  w: BF16 [16 NB][32 KS].
+// v_mfma_f32_16x16x32_bf16 (16 coarse voxels per workgroup, wave w computes
+// N-block w of the 16 NB output channels, bias added), staged in LDS, then
+// nearest upsampling plus the skip connection for the 8 children of each
+// coarse voxel, 16 bytes per access.  Fine dimensions are exactly twice the
+// coarse ones.  w: BF16 [16 NB][32 KS].
 template <int KS, int NB>
 __global__ void __launch_bounds__(256)
     k_up1_add(Dims dc, const bf16* __restrict__ h, const bf16* __restrict__ w,
               const float* __restrict__ bias, const bf16* __restrict__ skip, bf16* __restrict__ out) {
-    constexpr int CN = 32 * KS, CL = 16 * NB, G = CL / 8, US = CL + 4;
-    __shared__ float ul[64 * US];
+    static_assert(NB <= 4, "one N-block per wave");
+    constexpr int CN = 32 * KS, CL = 16 * NB, G = CL / 8, US = CL + 4, VB = 16;
+    __shared__ float ul[VB * US];
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
     const int g = lane >> 4, m = lane & 15;
-    const long long v0 = (long long)blockIdx.x * 64;
-    f4v acc[NB];
+    const long long v0 = (long long)blockIdx.x * VB;
+    if (wave < NB) {
+        f4v acc = f4v{0.f, 0.f, 0.f, 0.f};
+        const long long va = v0 + m;  // A row m
 #pragma unroll
-    for (int nb = 0; nb < NB; nb++) acc[nb] = f4v{0.f, 0.f, 0.f, 0.f};
-    const long long va = v0 + wave * 16 + m;  // A row m
-#pragma unroll
-    for (int st = 0; st < KS; st++) {
-        uint4 q = make_uint4(0, 0, 0, 0);
-        if (va < dc.v) q = *reinterpret_cast<const uint4*>(h + va * CN + 32 * st + 8 * g);
-        const bf8v afr = __builtin_bit_cast(bf8v, q);
-#pragma unroll
-        for (int nb = 0; nb < NB; nb++) {
+        for (int st = 0; st < KS; st++) {
+            uint4 q = make_uint4(0, 0, 0, 0);
+            if (va < dc.v) q = *reinterpret_cast<const uint4*>(h + va * CN + 32 * st + 8 * g);
             const bf8v bfr = __builtin_bit_cast(
-                bf8v, *reinterpret_cast<const uint4*>(w + (nb * 16 + m) * CN + 32 * st + 8 * g));
-            acc[nb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(afr, bfr, acc[nb], 0, 0, 0);
+                bf8v, *reinterpret_cast<const uint4*>(w + (wave * 16 + m) * CN + 32 * st + 8 * g));
+            acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf8v, q), bfr, acc, 0, 0, 0);
         }
-    }
-    // D fragment: lane holds rows 4g + j (coarse voxels) of column m (channel)
+        // D fragment: lane holds rows 4g + j (coarse voxels) of column m (channel)
+        const float bv = bias[wave * 16 + m];
 #pragma unroll
-    for (int nb = 0; nb < NB; nb++) {
-        const float bv = bias[nb * 16 + m];
-#pragma unroll
-        for (int j = 0; j < 4; j++) ul[(wave * 16 + 4 * g + j) * US + nb * 16 + m] = acc[nb][j] + bv;
+        for (int j = 0; j < 4; j++) ul[(4 * g + j) * US + wave * 16 + m] = acc[j] + bv;
     }
     __syncthreads();
     const int fx = 2 * dc.nx, fy = 2 * dc.ny;
     // items ordered (y/z child, coarse voxel, x child, 8-channel group) so that
     // consecutive threads touch consecutive 16-byte pieces of a fine x-row;
     // every skip load is issued before the first store
-    constexpr int NIT = 4 * 64 * 2 * G / 256;
+    constexpr int NITEM = 4 * VB * 2 * G, NIT = (NITEM + 255) / 256;
     long long fo[NIT];
     uint4 sq[NIT];
 #pragma unroll
     for (int i = 0; i < NIT; i++) {
         const int it = tid + 256 * i;
-        const int c8 = it % G, ax = (it / G) & 1, vl = (it / (2 * G)) % 64, bc = it / (128 * G);
+        const int c8 = it % G, ax = (it / G) & 1, vl = (it / (2 * G)) % VB, bc = it / (2 * G * VB);
         const long long v = v0 + vl;
         fo[i] = -1;
         sq[i] = make_uint4(0, 0, 0, 0);
-        if (v < dc.v) {
+        if (it < NITEM && v < dc.v) {
             int cx, cy, cz;
             vox_xyz(dc, v, cx, cy, cz);
             fo[i] = ((long long)(2 * cz + (bc >> 1)) * fy + 2 * cy + (bc & 1)) * fx + 2 * cx + ax;
@@ -973,7 +972,7 @@ __global__ void __launch_bounds__(256)
     for (int i = 0; i < NIT; i++) {
         if (fo[i] < 0) continue;
         const int it = tid + 256 * i;
-        const int c8 = it % G, vl = (it / (2 * G)) % 64;
+        const int c8 = it % G, vl = (it / (2 * G)) % VB;
         float sk[8];
         unpack8(sq[i], sk);
         const float* uu = ul + vl * US + 8 * c8;
@@ -1906,7 +1905,7 @@ static int unet_forward(pic_ctx* ctx, UNet* u, const float* x, const double* nrm
         const bf16* src = l == D - 2 ? lv[D - 1].s : lv[l + 1].h;
         const UNetLayer& ul = L[u->l_u[l]];
         const Dims& dn = lv[l + 1].d;
-        const unsigned nbu = (unsigned)((dn.v + 63) / 64);
+        const unsigned nbu = (unsigned)((dn.v + 15) / 16);
         auto up1 = [&](auto ks, auto nb) {
             k_up1_add<decltype(ks)::value, decltype(nb)::value>
                 <<<nbu, 256, 0, ctx->stream>>>(dn, src, ul.wb, P + ul.boff, lv[l].s, lv[l].m);
