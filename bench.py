@@ -35,6 +35,7 @@ ME = 9.10938215e-31
 AMU = 1.660538921e-27
 EV_K = 11604.52
 HBM_PEAK_GBS = 8000.0
+MFMA_BF16_PEAK_TFLOPS = 2500.0  # dense BF16 (MI355X)
 
 # algorithmic bytes per unit of work (per particle, or per node and iteration)
 ALG_BYTES = {
@@ -51,6 +52,24 @@ KERNEL_NAMES = {
     "push": "k_push", "scatter": "k_scatter", "deposit": "k_deposit<1>",
     "pcg": "k_pcg1<float> + k_pcg2<float>", "sor": "k_sor",
 }
+
+
+def unet_flops(channels, coarsen, n):
+    """algorithmic FLOPs of one U-Net forward (2 per multiply-add; every
+    convolution at its true channel counts, no padding) on the network grid
+    of an n^3 node grid: input 2 -> C0 and B_0 on level 0, A_l and B_l on
+    level l > 0, per decoder level a 1x1 up-convolution and Dec_l, output
+    C0 -> 1"""
+    vox = [(n // coarsen // 2 ** l) ** 3 for l in range(len(channels))]
+    f = 2 * 27 * 2 * channels[0] * vox[0] + 2 * 27 * channels[0] * 1 * vox[0]  # in, out
+    for l, c in enumerate(channels):
+        if l:
+            f += 2 * 27 * channels[l - 1] * c * vox[l]  # A_l
+        f += 2 * 27 * c * c * vox[l]  # B_l
+    for l in range(len(channels) - 1):
+        f += 2 * channels[l + 1] * channels[l] * vox[l + 1]  # 1x1 up-convolution
+        f += 2 * 27 * channels[l] * channels[l] * vox[l]  # Dec_l
+    return f
 
 
 def cpu_baseline(threads, particles, steps):
@@ -175,8 +194,9 @@ def main():
     ctx.charge_density()
     weights = os.path.join(PKG, "data", "unet_%d.f32" % n)
     have_net = os.path.exists(weights) and os.path.exists(weights + ".json")
+    net_meta = None
     if have_net:
-        ctx.unet_load(weights)
+        net_meta = ctx.unet_load(weights)
     elif args.step_solver == "learned":
         raise SystemExit("--step-solver learned needs %s(.json)" % weights)
     step_solver = args.step_solver
@@ -307,6 +327,17 @@ def main():
         per_unit = ALG_BYTES_FCG
     alg = per_unit * units
     achieved = alg / avg_s / 1e9
+    # the network's forward as a whole against the dense BF16 MFMA peak
+    roof_net = None
+    if net_meta is not None and prof.get("unet", (0, 0))[1]:
+        fwd_s = prof["unet"][0] * 1e-3 / prof["unet"][1]
+        fl = unet_flops(net_meta["channels"], net_meta["coarsen"], n)
+        ach = fl / fwd_s / 1e12
+        roof_net = {"bound": "mfma", "kernel": "U-Net forward (k_conv_in, k_conv3_rb, k_conv3_small, "
+                                               "k_up1_add, k_conv_out, k_pool)",
+                    "achieved": ach, "peak": MFMA_BF16_PEAK_TFLOPS, "unit": "TFLOP/s",
+                    "frac": ach / MFMA_BF16_PEAK_TFLOPS, "traffic": None,
+                    "alg_flops_per_forward": fl, "avg_forward_ms": fwd_s * 1e3}
     traffic = None
     tpath = os.path.join(ROOT, "profiles", "pmc_traffic.json")
     if os.path.exists(tpath):
@@ -341,6 +372,7 @@ def main():
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
                      "traffic": traffic, "alg_bytes_per_launch": alg,
                      "avg_launch_ms": avg_s * 1e3},
+        "roofline_network": roof_net,
         "poisson": poisson,
         "cpu_baseline": cpu,
     }
