@@ -208,7 +208,7 @@ def main():
         kw = {"sor_rb": dict(method=pg.SOLVER_SOR_RB, omega=omega_opt, check_every=10),
               "pcg_mixed": dict(method=pg.SOLVER_PCG_MIXED),
               "pcg_fp64": dict(method=pg.SOLVER_PCG),
-              "learned": dict(method=pg.SOLVER_LEARNED, check_every=2)}[name]
+              "learned": dict(method=pg.SOLVER_LEARNED, check_every=1)}[name]
         kw.update(tol=1e-6, tol_kind=pg.TOL_REL, max_it=20000)
         return kw
 

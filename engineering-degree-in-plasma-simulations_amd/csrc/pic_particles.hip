@@ -245,43 +245,69 @@ __global__ void __launch_bounds__(BLOCK)
     }
 }
 
-// SC_PT particles per thread, BLOCK apart (coalesced); every load that
-// depends only on the particle index is issued before the offset lookup
-constexpr int SC_PT = 2;
+// A workgroup moves SC_V * BLOCK consecutive particles: it reads them with
+// 16-byte loads (SC_V = 4 consecutive particles per lane, every array), stages
+// them in LDS, and writes them out one particle per lane so consecutive lanes
+// store to consecutive destinations (the order inside a cell is the rank from
+// the push's histogram).  Vector loads only where all four particles are
+// below the live count; the tail is read element by element.
+constexpr int SC_V = 4;
 __global__ void __launch_bounds__(BLOCK)
     k_scatter(GridC g, SpeciesBuf src, SpeciesBuf dst, const uint32_t* __restrict__ rank,
               const uint32_t* __restrict__ off, const uint32_t* __restrict__ d_n, uint32_t cap) {
+    constexpr int NP = BLOCK * SC_V;
+    __shared__ uint4 st[9][BLOCK];  // cell, fx, fy, fz, vx, vy, vz, w, rank; 4 particles each
     const uint32_t n = d_n[1];
-    const size_t base = (size_t)blockIdx.x * (BLOCK * SC_PT) + threadIdx.x;
-    uint32_t c[SC_PT], rk[SC_PT];
-    float f[SC_PT][3], v[SC_PT][3], w[SC_PT];
+    const size_t base = (size_t)blockIdx.x * NP;
+    const int tid = threadIdx.x;
+    const size_t p4 = base + (size_t)SC_V * tid;
+    const uint32_t* arr[9] = {src.cell,
+                              reinterpret_cast<const uint32_t*>(src.f[0]),
+                              reinterpret_cast<const uint32_t*>(src.f[1]),
+                              reinterpret_cast<const uint32_t*>(src.f[2]),
+                              reinterpret_cast<const uint32_t*>(src.v[0]),
+                              reinterpret_cast<const uint32_t*>(src.v[1]),
+                              reinterpret_cast<const uint32_t*>(src.v[2]),
+                              reinterpret_cast<const uint32_t*>(src.w),
+                              rank};
+    uint4 r[9];
+    if (p4 + SC_V <= n) {
 #pragma unroll
-    for (int e = 0; e < SC_PT; e++) {
-        const size_t p = base + (size_t)BLOCK * e;
-        const bool valid = p < n;
-        c[e] = valid ? src.cell[p] : DEAD;
-        rk[e] = valid ? rank[p] : 0u;
+        for (int a = 0; a < 9; a++) r[a] = *reinterpret_cast<const uint4*>(arr[a] + p4);
+    } else {
 #pragma unroll
-        for (int d = 0; d < 3; d++) {
-            f[e][d] = valid ? src.f[d][p] : 0.f;
-            v[e][d] = valid ? src.v[d][p] : 0.f;
+        for (int a = 0; a < 9; a++) {
+            uint32_t e[4] = {DEAD, DEAD, DEAD, DEAD};  // past the live count: skipped below
+#pragma unroll
+            for (int k = 0; k < 4; k++)
+                if (p4 + k < n) e[k] = arr[a][p4 + k];
+            r[a] = make_uint4(e[0], e[1], e[2], e[3]);
         }
-        w[e] = valid ? src.w[p] : 0.f;
     }
 #pragma unroll
-    for (int e = 0; e < SC_PT; e++) {
-        if (c[e] == DEAD) continue;
-        int i, j, k;
-        unpack_cell(c[e], i, j, k);
-        const uint32_t dst_i = off[cell_index(g, i, j, k)] + rk[e];
-        if (dst_i >= cap) continue;  // cannot happen with a current histogram
-        dst.cell[dst_i] = c[e];
+    for (int a = 0; a < 9; a++) st[a][tid] = r[a];
+    __syncthreads();
+    const uint32_t* s32[9];
 #pragma unroll
-        for (int d = 0; d < 3; d++) {
-            dst.f[d][dst_i] = f[e][d];
-            dst.v[d][dst_i] = v[e][d];
-        }
-        dst.w[dst_i] = w[e];
+    for (int a = 0; a < 9; a++) s32[a] = reinterpret_cast<const uint32_t*>(st[a]);
+#pragma unroll
+    for (int e = 0; e < SC_V; e++) {
+        const int q = tid + BLOCK * e;
+        if (base + q >= n) continue;
+        const uint32_t c = s32[0][q];
+        if (c == DEAD) continue;
+        int i, j, k;
+        unpack_cell(c, i, j, k);
+        const uint32_t dst_i = off[cell_index(g, i, j, k)] + s32[8][q];
+        if (dst_i >= cap) continue;  // cannot happen with a current histogram
+        dst.cell[dst_i] = c;
+        dst.f[0][dst_i] = __uint_as_float(s32[1][q]);
+        dst.f[1][dst_i] = __uint_as_float(s32[2][q]);
+        dst.f[2][dst_i] = __uint_as_float(s32[3][q]);
+        dst.v[0][dst_i] = __uint_as_float(s32[4][q]);
+        dst.v[1][dst_i] = __uint_as_float(s32[5][q]);
+        dst.v[2][dst_i] = __uint_as_float(s32[6][q]);
+        dst.w[dst_i] = __uint_as_float(s32[7][q]);
     }
 }
 
@@ -642,7 +668,7 @@ extern "C" int pic_sort(pic_ctx* ctx, int sid) {
     {
         PicProfScope ps(ctx, PIC_K_SCATTER);
         size_t n = s.n_upper ? s.n_upper : 1;
-        k_scatter<<<grid_for((n + SC_PT - 1) / SC_PT, BLOCK), BLOCK, 0, ctx->stream>>>(
+        k_scatter<<<grid_for((n + SC_V - 1) / SC_V, BLOCK), BLOCK, 0, ctx->stream>>>(
             g, s.buf[s.cur], s.buf[s.cur ^ 1], s.rank, s.off, s.d_n, (uint32_t)s.cap);
     }
     PIC_HIP(ctx, hipGetLastError());

@@ -996,7 +996,9 @@ static int solve_learned(pic_ctx* ctx, const pic_solver_params* p, pic_solver_st
     size_t nn = (size_t)g.nn;
     Stencil s = make_stencil(g, false);
     const int max_fcg = std::min(p->max_it, 500);
-    const int check = p->check_every > 0 ? p->check_every : 2;
+    // after the first check, every iteration: an iteration costs far more
+    // than a check (a polled wait on a few bytes)
+    const int check = p->check_every > 0 ? p->check_every : 1;
     const size_t hslots = (size_t)(max_fcg + 2) * SSIZE;  // striped history, per quantity
     rc = pic_ensure_work(ctx, 8 * nn * sizeof(double) + 4 * hslots * sizeof(double));
     if (rc) return rc;
