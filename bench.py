@@ -5,8 +5,9 @@ Workload (the scaled plain-PIC configuration): 128^3 nodes, dx = 1e-2 m,
 Dirichlet phi = 0 walls, kinetic e- (Te = 2 eV) and O+ (16 amu, Ti = 0.1 eV),
 20M macro-particles each, uniform positions, Maxwellian velocities, uniform
 weight w = n0 L^3 / N with n0 = 1e12 m^-3, dt = 1e-10 s.  One step is
-  push e-, push O+  (CIC gather + kick + drift + absorb + sort histogram)
-  sort e-, sort O+  (counting sort by cell, every step)
+  push + sort e-, O+  (CIC gather + kick + drift + absorb, counting sort by
+                      cell every step: pic_push_sort, a histogram pass and a
+                      pass that pushes again into the sorted position)
   deposit e-, O+    (number density), charge density,
   Poisson solve to a 1e-6 relative residual (FP64), warm-started from the
   previous phi: the learned-preconditioned flexible CG when its trained
@@ -39,8 +40,8 @@ MFMA_BF16_PEAK_TFLOPS = 2500.0  # dense BF16 (MI355X)
 
 # algorithmic bytes per unit of work (per particle, or per node and iteration)
 ALG_BYTES = {
-    "push": 48.0,      # read + write x, y, z, vx, vy, vz (FP32)
-    "scatter": 64.0,   # counting-sort scatter: key r/w + 28 B SoA read + 28 B write
+    "push": 24.0,      # histogram pass of the push: read x, y, z, vx, vy, vz (FP32)
+    "scatter": 64.0,   # push into the sorted position: the 32-byte particle read and written
     "deposit": 16.0,   # x, y, z, w
     "pcg": 44.0,       # FP32 inner PCG: per node and iteration (88 B in FP64)
     "sor": 24.0,       # FP64 phi r/w + rho read per node and sweep
@@ -49,7 +50,7 @@ ALG_BYTES = {
 # p, q from z and the previous p (32 B), then x, r updated from p, q (48 B)
 ALG_BYTES_FCG = 80.0
 KERNEL_NAMES = {
-    "push": "k_push", "scatter": "k_scatter", "deposit": "k_deposit<1>",
+    "push": "k_push<false>", "scatter": "k_push_scatter", "deposit": "k_dep_cells + k_dep_nodes",
     "pcg": "k_pcg1<float> + k_pcg2<float>", "sor": "k_sor",
 }
 
@@ -256,9 +257,7 @@ def main():
 
     def step(kw=solve_kw):
         for s in species:
-            ctx.push(s, dt)
-        for s in species:
-            ctx.sort(s)
+            ctx.push_sort(s, dt)
         for s in species:
             ctx.deposit(s)
         ctx.charge_density()

@@ -4,6 +4,8 @@
 //   k_hist     histogram of unchanged particles (Species.cpp:905-915)
 //   k_scan_*   exclusive scan of the per-cell counts -> cell offsets
 //   k_scatter  stable-per-wave scatter into the second buffer (compacts)
+//   k_push_scatter  second pass of pic_push_sort: push again and write each
+//              particle to its sorted position
 //   k_deposit  one workgroup per cell row, LDS tile of the row's 4 node
 //              lines, flushed with FP64 global atomics (Species.cpp:401-416,
 //              :767-776, Field.h:157-199)
@@ -13,6 +15,11 @@
 #include <cmath>
 
 #include "pic_internal.h"
+
+// Every floating-point operation in this file is rounded on its own (no
+// fused multiply-add contraction): the two passes of pic_push_sort compute
+// each particle's new cell from the same inputs and must agree bit for bit.
+#pragma clang fp contract(off)
 
 using namespace pic;
 
@@ -89,6 +96,46 @@ __device__ inline bool in_rect(const GridC& g, int i, int j, int k, float fx, fl
     return false;
 }
 
+// One particle's push (Species.cpp:356-399): CIC gather of E at the old
+// position, kick, drift, relocation into the new cell, removal at the walls
+// (World::inBounds, World.cpp:201-205) and inside objects.  Updates f and v;
+// returns the new cell word, DEAD when the particle is removed.
+__device__ inline uint32_t advance(const GridC& g, const float4* __restrict__ E, const PushArgs& a,
+                                   uint32_t c, float (&f)[3], float (&v)[3]) {
+    int i, j, k;
+    unpack_cell(c, i, j, k);
+    if (a.kick) {
+        float3 e = gather_e(g, E, i, j, k, f[0], f[1], f[2]);
+        v[0] += e.x * a.qmdt;
+        v[1] += e.y * a.qmdt;
+        v[2] += e.z * a.qmdt;
+    }
+    f[0] += v[0] * a.dtx[0];
+    f[1] += v[1] * a.dtx[1];
+    f[2] += v[2] * a.dtx[2];
+    bool ok = (f[0] == f[0]) && (f[1] == f[1]) && (f[2] == f[2]);
+    if (ok) {
+        i = relocate(f[0], i);
+        j = relocate(f[1], j);
+        k = relocate(f[2], k);
+        ok = i >= 0 && i < g.ci && j >= 0 && j < g.cj && k >= 0 && k < g.ck;
+        if (ok && g.nrect) ok = !in_rect(g, i, j, k, f[0], f[1], f[2]);
+    }
+    return ok ? pack_cell(i, j, k) : DEAD;
+}
+
+// the histogram key of a cell word: the cell index, ncells for removed
+__device__ inline uint32_t cell_key(const GridC& g, uint32_t c) {
+    if (c == DEAD) return (uint32_t)g.ncells;
+    int i, j, k;
+    unpack_cell(c, i, j, k);
+    return (uint32_t)cell_index(g, i, j, k);
+}
+
+// WRITE: the push proper (positions, velocities and changed cell words are
+// written); otherwise only the counting sort's histogram and ranks of the
+// pushed particles (the first pass of pic_push_sort).
+template <bool WRITE>
 __global__ void __launch_bounds__(BLOCK)
     k_push(GridC g, SpeciesBuf b, const float4* __restrict__ E, PushArgs a,
            uint32_t* __restrict__ d_n, uint32_t* __restrict__ count, uint32_t* __restrict__ rank) {
@@ -103,41 +150,18 @@ __global__ void __launch_bounds__(BLOCK)
         uint32_t c = b.cell[p];
         key = (uint32_t)g.ncells;
         if (c != DEAD) {
-            int i, j, k;
-            unpack_cell(c, i, j, k);
-            float fx = b.f[0][p], fy = b.f[1][p], fz = b.f[2][p];
-            float vx = b.v[0][p], vy = b.v[1][p], vz = b.v[2][p];
-            if (a.kick) {
-                float3 e = gather_e(g, E, i, j, k, fx, fy, fz);
-                vx += e.x * a.qmdt;
-                vy += e.y * a.qmdt;
-                vz += e.z * a.qmdt;
-            }
-            fx += vx * a.dtx[0];
-            fy += vy * a.dtx[1];
-            fz += vz * a.dtx[2];
-            bool ok = (fx == fx) && (fy == fy) && (fz == fz);
-            if (ok) {
-                i = relocate(fx, i);
-                j = relocate(fy, j);
-                k = relocate(fz, k);
-                // World::inBounds (World.cpp:201-205): 0 <= lc < n-1 per axis
-                ok = i >= 0 && i < g.ci && j >= 0 && j < g.cj && k >= 0 && k < g.ck;
-                if (ok && g.nrect) ok = !in_rect(g, i, j, k, fx, fy, fz);
-            }
-            if (ok) {
-                const uint32_t nc = pack_cell(i, j, k);
+            float f[3] = {b.f[0][p], b.f[1][p], b.f[2][p]};
+            float v[3] = {b.v[0][p], b.v[1][p], b.v[2][p]};
+            const uint32_t nc = advance(g, E, a, c, f, v);
+            key = cell_key(g, nc);
+            if (WRITE) {
                 if (nc != c) b.cell[p] = nc;  // most particles stay in their cell
-                key = (uint32_t)cell_index(g, i, j, k);
-            } else {
-                b.cell[p] = DEAD;
+#pragma unroll
+                for (int d = 0; d < 3; d++) {
+                    b.f[d][p] = f[d];
+                    b.v[d][p] = v[d];
+                }
             }
-            b.f[0][p] = fx;
-            b.f[1][p] = fy;
-            b.f[2][p] = fz;
-            b.v[0][p] = vx;
-            b.v[1][p] = vy;
-            b.v[2][p] = vz;
         }
     }
     uint32_t r = run_histogram(key, valid, count);
@@ -309,6 +333,35 @@ __global__ void __launch_bounds__(BLOCK)
         dst.v[2][dst_i] = __uint_as_float(s32[6][q]);
         dst.w[dst_i] = __uint_as_float(s32[7][q]);
     }
+}
+
+// Second pass of pic_push_sort: every particle is pushed again from the
+// unchanged source arrays (the same arithmetic as the first pass, so the
+// same new cell) and written straight to its sorted position
+// off[new cell] + rank; removed particles are dropped.
+__global__ void __launch_bounds__(BLOCK)
+    k_push_scatter(GridC g, SpeciesBuf src, SpeciesBuf dst, const float4* __restrict__ E, PushArgs a,
+                   const uint32_t* __restrict__ rank, const uint32_t* __restrict__ off,
+                   const uint32_t* __restrict__ d_n, uint32_t cap) {
+    const size_t p = (size_t)blockIdx.x * BLOCK + threadIdx.x;
+    if (p >= d_n[1]) return;
+    const uint32_t c = src.cell[p];
+    if (c == DEAD) return;
+    float f[3] = {src.f[0][p], src.f[1][p], src.f[2][p]};
+    float v[3] = {src.v[0][p], src.v[1][p], src.v[2][p]};
+    const float w = src.w[p];
+    const uint32_t r = rank[p];
+    const uint32_t nc = advance(g, E, a, c, f, v);
+    if (nc == DEAD) return;
+    const uint32_t dst_i = off[cell_key(g, nc)] + r;
+    if (dst_i >= cap) return;  // cannot happen with a current histogram
+    dst.cell[dst_i] = nc;
+#pragma unroll
+    for (int d = 0; d < 3; d++) {
+        dst.f[d][dst_i] = f[d];
+        dst.v[d][dst_i] = v[d];
+    }
+    dst.w[dst_i] = w;
 }
 
 // ---- deposit: one workgroup per cell row (j, k); the row's particles are
@@ -610,21 +663,26 @@ __global__ void __launch_bounds__(BLOCK)
 // ------------------------------------------------------------------------
 // host entry points
 // ------------------------------------------------------------------------
+static PushArgs push_args(const pic_ctx* ctx, const Species& s, double dt) {
+    PushArgs a;
+    a.qmdt = (float)(dt * s.charge / s.mass);
+    for (int d = 0; d < 3; d++) a.dtx[d] = (float)(dt * ctx->g.inv_dx[d]);
+    a.kick = s.charge != 0.0;
+    return a;
+}
+
 extern "C" int pic_push(pic_ctx* ctx, int sid, double dt) {
     int rc = pic_check_sid(ctx, sid);
     if (rc) return rc;
     Species& s = ctx->sp[sid];
     const GridC& g = ctx->g;
-    PushArgs a;
-    a.qmdt = (float)(dt * s.charge / s.mass);
-    for (int d = 0; d < 3; d++) a.dtx[d] = (float)(dt * g.inv_dx[d]);
-    a.kick = s.charge != 0.0;
+    const PushArgs a = push_args(ctx, s, dt);
     PIC_HIP(ctx, hipMemsetAsync(s.count, 0, ((size_t)g.ncells + 1) * 4, ctx->stream));
     {
         PicProfScope ps(ctx, PIC_K_PUSH);
         size_t n = s.n_upper ? s.n_upper : 1;
-        k_push<<<grid_for(n, BLOCK), BLOCK, 0, ctx->stream>>>(g, s.buf[s.cur], ctx->ef, a, s.d_n,
-                                                              s.count, s.rank);
+        k_push<true><<<grid_for(n, BLOCK), BLOCK, 0, ctx->stream>>>(g, s.buf[s.cur], ctx->ef, a, s.d_n,
+                                                                    s.count, s.rank);
     }
     PIC_HIP(ctx, hipGetLastError());
     s.hist_valid = true;
@@ -647,6 +705,18 @@ int pic_histogram(pic_ctx* ctx, int sid) {
     return PIC_OK;
 }
 
+// exclusive scan of the species' cell counts into its offsets (and the live
+// count into d_n[0])
+static void scan_counts(pic_ctx* ctx, Species& s) {
+    const GridC& g = ctx->g;
+    int M = g.ncells + 1;
+    int nb = (M + SCAN_TILE - 1) / SCAN_TILE;
+    PicProfScope ps(ctx, PIC_K_SORT);
+    k_scan_reduce<<<nb, BLOCK, 0, ctx->stream>>>(s.count, M, ctx->scan_tmp);
+    k_scan_top<<<1, BLOCK, 0, ctx->stream>>>(ctx->scan_tmp, nb);
+    k_scan_down<<<nb, BLOCK, 0, ctx->stream>>>(s.count, M, ctx->scan_tmp, s.off, g.ncells, s.d_n);
+}
+
 extern "C" int pic_sort(pic_ctx* ctx, int sid) {
     int rc = pic_check_sid(ctx, sid);
     if (rc) return rc;
@@ -656,20 +726,43 @@ extern "C" int pic_sort(pic_ctx* ctx, int sid) {
         rc = pic_histogram(ctx, sid);
         if (rc) return rc;
     }
-    int M = g.ncells + 1;
-    int nb = (M + SCAN_TILE - 1) / SCAN_TILE;
-    {
-        PicProfScope ps(ctx, PIC_K_SORT);
-        k_scan_reduce<<<nb, BLOCK, 0, ctx->stream>>>(s.count, M, ctx->scan_tmp);
-        k_scan_top<<<1, BLOCK, 0, ctx->stream>>>(ctx->scan_tmp, nb);
-        k_scan_down<<<nb, BLOCK, 0, ctx->stream>>>(s.count, M, ctx->scan_tmp, s.off, g.ncells,
-                                                   s.d_n);
-    }
+    scan_counts(ctx, s);
     {
         PicProfScope ps(ctx, PIC_K_SCATTER);
         size_t n = s.n_upper ? s.n_upper : 1;
         k_scatter<<<grid_for((n + SC_V - 1) / SC_V, BLOCK), BLOCK, 0, ctx->stream>>>(
             g, s.buf[s.cur], s.buf[s.cur ^ 1], s.rank, s.off, s.d_n, (uint32_t)s.cap);
+    }
+    PIC_HIP(ctx, hipGetLastError());
+    s.cur ^= 1;
+    s.sorted = true;
+    s.hist_valid = false;
+    return PIC_OK;
+}
+
+// pic_push then pic_sort in two passes over the particles: the first pushes
+// without writing them back and builds the sort's histogram and ranks, the
+// second pushes again from the same inputs and writes every particle to its
+// sorted position.  Same result as the two calls (the order inside a cell
+// follows the histogram's ranks in both), about 20% less particle traffic.
+extern "C" int pic_push_sort(pic_ctx* ctx, int sid, double dt) {
+    int rc = pic_check_sid(ctx, sid);
+    if (rc) return rc;
+    Species& s = ctx->sp[sid];
+    const GridC& g = ctx->g;
+    const PushArgs a = push_args(ctx, s, dt);
+    const size_t n = s.n_upper ? s.n_upper : 1;
+    PIC_HIP(ctx, hipMemsetAsync(s.count, 0, ((size_t)g.ncells + 1) * 4, ctx->stream));
+    {
+        PicProfScope ps(ctx, PIC_K_PUSH);
+        k_push<false><<<grid_for(n, BLOCK), BLOCK, 0, ctx->stream>>>(g, s.buf[s.cur], ctx->ef, a, s.d_n,
+                                                                     s.count, s.rank);
+    }
+    scan_counts(ctx, s);
+    {
+        PicProfScope ps(ctx, PIC_K_SCATTER);
+        k_push_scatter<<<grid_for(n, BLOCK), BLOCK, 0, ctx->stream>>>(
+            g, s.buf[s.cur], s.buf[s.cur ^ 1], ctx->ef, a, s.rank, s.off, s.d_n, (uint32_t)s.cap);
     }
     PIC_HIP(ctx, hipGetLastError());
     s.cur ^= 1;

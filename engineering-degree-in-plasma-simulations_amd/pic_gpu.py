@@ -33,7 +33,7 @@ ABI_FUNCTIONS = [
     "pic_create", "pic_destroy", "pic_last_error", "pic_set_dt", "pic_sync",
     "pic_add_rectangle", "pic_dirichlet_walls", "pic_get_node_flags",
     "pic_species_add", "pic_particles_upload", "pic_particles_download",
-    "pic_particles_count", "pic_load_box_thermal", "pic_push", "pic_sort",
+    "pic_particles_count", "pic_load_box_thermal", "pic_push", "pic_sort", "pic_push_sort",
     "pic_deposit", "pic_clear_samples", "pic_charge_density", "pic_poisson",
     "pic_efield", "pic_field_upload", "pic_field_download",
     "pic_species_field_download", "pic_diag_sums", "pic_field_energy",
@@ -127,6 +127,7 @@ def load_library():
                                        C.c_uint32, C.c_int, C.POINTER(C.c_size_t)]
     L.pic_push.argtypes = [P, C.c_int, C.c_double]
     L.pic_sort.argtypes = [P, C.c_int]
+    L.pic_push_sort.argtypes = [P, C.c_int, C.c_double]
     L.pic_deposit.argtypes = [P, C.c_int, C.c_int]
     L.pic_clear_samples.argtypes = [P, C.c_int]
     L.pic_charge_density.argtypes = [P]
@@ -255,6 +256,10 @@ class Context:
 
     def sort(self, sid):
         self._ck(self.L.pic_sort(self.h, sid))
+
+    def push_sort(self, sid, dt):
+        """push then sort in two passes (pic_push_sort)"""
+        self._ck(self.L.pic_push_sort(self.h, sid, dt))
 
     def deposit(self, sid, flags=DEPOSIT_DENSITY):
         self._ck(self.L.pic_deposit(self.h, sid, flags))

@@ -100,6 +100,13 @@ int pic_push(pic_ctx* ctx, int sid, double dt);
 /* sortIndexes() (Species.cpp:905-929): counting sort of the particle arrays
  * by cell; compacts removed particles; cell offsets stay on the device. */
 int pic_sort(pic_ctx* ctx, int sid);
+/* pic_push followed by pic_sort (advanceElectrons + sortIndexes,
+ * Species.cpp:258-399, :905-929) in two passes over the particles: the
+ * first pushes without writing back and builds the sort's histogram, the
+ * second pushes again from the same inputs (identical arithmetic) and
+ * writes each particle to its sorted position.  Same result as the two
+ * calls, with about 20% less particle traffic. */
+int pic_push_sort(pic_ctx* ctx, int sid, double dt);
 /* computeNumberDensity() (Species.cpp:401-416); with PIC_DEPOSIT_MOMENTS
  * also sampleMoments() (Species.cpp:767-776), accumulated until
  * pic_clear_samples (Species.cpp:805-812).  Particles must be sorted. */

@@ -91,6 +91,35 @@ def test_push_matches_oracle(pg, golden):
     assert np.max(np.abs(a[:, 3:6] - b[:, 3:6])) < REL32 * vmax
 
 
+@pytest.mark.parametrize("case,dt", [("ele_loaded", 2e-9), ("ele_pre_long", 4e-8)])
+def test_push_sort_equals_push_then_sort(pg, golden, case, dt):
+    """pic_push_sort (push without write-back + histogram, then push again
+    into the sorted position) gives exactly what pic_push then pic_sort give:
+    the same particles, sorted by cell, removed ones dropped."""
+    gr = golden["grid"]
+    g = O.make_grid(*_grid_args(gr))
+    c = _ctx(pg, gr)
+    c.set_field(pg.FIELD_EF, golden["ef_analytic"])
+    p = _f32_particles(golden[case])
+    cap = p.size // 7 + 16
+    a = c.species_add(O.ME, -O.QE, 1.5e5, cap)
+    b = c.species_add(O.ME, -O.QE, 1.5e5, cap)
+    c.upload(a, p)
+    c.upload(b, p)
+    for _ in range(2):
+        c.push(a, dt)
+        c.sort(a)
+        c.push_sort(b, dt)
+    qa, qb = _rows(c.download(a)), _rows(c.download(b))
+    assert qa.shape == qb.shape
+    if case == "ele_pre_long":
+        assert qa.shape[0] < _rows(p).shape[0]  # some particles left the box
+    assert np.all(np.diff(_cell_of(g, qb)) >= 0)
+    ka, kb = np.lexsort(qa.T[::-1]), np.lexsort(qb.T[::-1])
+    assert np.array_equal(qa[ka], qb[kb])
+    assert np.array_equal(c.species_field(a, pg.SFIELD_MACRO_COUNT), c.species_field(b, pg.SFIELD_MACRO_COUNT))
+
+
 def test_push_absorbs_leaving_particles(pg, golden):
     gr = golden["grid"]
     g = O.make_grid(*_grid_args(gr))
