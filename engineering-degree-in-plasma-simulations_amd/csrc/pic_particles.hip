@@ -132,9 +132,9 @@ __device__ inline uint32_t cell_key(const GridC& g, uint32_t c) {
     return (uint32_t)cell_index(g, i, j, k);
 }
 
-// WRITE: the push proper (positions, velocities and changed cell words are
-// written); otherwise only the counting sort's histogram and ranks of the
-// pushed particles (the first pass of pic_push_sort).
+// WRITE: the push proper (positions, velocities, changed cell words and the
+// ranks inside the cells are written); otherwise only the counting sort's
+// histogram of the pushed particles (the first pass of pic_push_sort).
 template <bool WRITE>
 __global__ void __launch_bounds__(BLOCK)
     k_push(GridC g, SpeciesBuf b, const float4* __restrict__ E, PushArgs a,
@@ -165,7 +165,7 @@ __global__ void __launch_bounds__(BLOCK)
         }
     }
     uint32_t r = run_histogram(key, valid, count);
-    if (valid) rank[p] = r;
+    if (WRITE && valid) rank[p] = r;
 }
 
 __global__ void __launch_bounds__(BLOCK)
@@ -338,22 +338,35 @@ __global__ void __launch_bounds__(BLOCK)
 // Second pass of pic_push_sort: every particle is pushed again from the
 // unchanged source arrays (the same arithmetic as the first pass, so the
 // same new cell) and written straight to its sorted position
-// off[new cell] + rank; removed particles are dropped.
+// off[new cell] + its slot in the cell, taken from the per-cell cursor
+// (zeroed; wave-aggregated atomics as in the histogram); removed particles
+// are dropped.
 __global__ void __launch_bounds__(BLOCK)
     k_push_scatter(GridC g, SpeciesBuf src, SpeciesBuf dst, const float4* __restrict__ E, PushArgs a,
-                   const uint32_t* __restrict__ rank, const uint32_t* __restrict__ off,
+                   uint32_t* __restrict__ cursor, const uint32_t* __restrict__ off,
                    const uint32_t* __restrict__ d_n, uint32_t cap) {
     const size_t p = (size_t)blockIdx.x * BLOCK + threadIdx.x;
-    if (p >= d_n[1]) return;
-    const uint32_t c = src.cell[p];
-    if (c == DEAD) return;
-    float f[3] = {src.f[0][p], src.f[1][p], src.f[2][p]};
-    float v[3] = {src.v[0][p], src.v[1][p], src.v[2][p]};
-    const float w = src.w[p];
-    const uint32_t r = rank[p];
-    const uint32_t nc = advance(g, E, a, c, f, v);
-    if (nc == DEAD) return;
-    const uint32_t dst_i = off[cell_key(g, nc)] + r;
+    bool valid = p < d_n[1];
+    const uint32_t c = valid ? src.cell[p] : DEAD;
+    valid = valid && c != DEAD;
+    float f[3] = {0.f, 0.f, 0.f}, v[3] = {0.f, 0.f, 0.f};
+    float w = 0.f;
+    uint32_t nc = DEAD;
+    if (valid) {
+        f[0] = src.f[0][p];
+        f[1] = src.f[1][p];
+        f[2] = src.f[2][p];
+        v[0] = src.v[0][p];
+        v[1] = src.v[1][p];
+        v[2] = src.v[2][p];
+        w = src.w[p];
+        nc = advance(g, E, a, c, f, v);
+        valid = nc != DEAD;
+    }
+    const uint32_t key = valid ? cell_key(g, nc) : 0xFFFFFFFFu;
+    const uint32_t r = run_histogram(key, valid, cursor);  // every lane of the wave takes part
+    if (!valid) return;
+    const uint32_t dst_i = off[key] + r;
     if (dst_i >= cap) return;  // cannot happen with a current histogram
     dst.cell[dst_i] = nc;
 #pragma unroll
@@ -741,10 +754,11 @@ extern "C" int pic_sort(pic_ctx* ctx, int sid) {
 }
 
 // pic_push then pic_sort in two passes over the particles: the first pushes
-// without writing them back and builds the sort's histogram and ranks, the
-// second pushes again from the same inputs and writes every particle to its
-// sorted position.  Same result as the two calls (the order inside a cell
-// follows the histogram's ranks in both), about 20% less particle traffic.
+// without writing them back and builds the sort's histogram, the second
+// pushes again from the same inputs and writes every particle to its sorted
+// position (a per-cell cursor gives the slot).  Same result as the two calls
+// (the order inside a cell follows atomic order in both), about 25% less
+// particle traffic.
 extern "C" int pic_push_sort(pic_ctx* ctx, int sid, double dt) {
     int rc = pic_check_sid(ctx, sid);
     if (rc) return rc;
@@ -759,10 +773,12 @@ extern "C" int pic_push_sort(pic_ctx* ctx, int sid, double dt) {
                                                                      s.count, s.rank);
     }
     scan_counts(ctx, s);
+    // the counts are consumed by the scan: reuse them as the per-cell cursor
+    PIC_HIP(ctx, hipMemsetAsync(s.count, 0, ((size_t)g.ncells + 1) * 4, ctx->stream));
     {
         PicProfScope ps(ctx, PIC_K_SCATTER);
         k_push_scatter<<<grid_for(n, BLOCK), BLOCK, 0, ctx->stream>>>(
-            g, s.buf[s.cur], s.buf[s.cur ^ 1], ctx->ef, a, s.rank, s.off, s.d_n, (uint32_t)s.cap);
+            g, s.buf[s.cur], s.buf[s.cur ^ 1], ctx->ef, a, s.count, s.off, s.d_n, (uint32_t)s.cap);
     }
     PIC_HIP(ctx, hipGetLastError());
     s.cur ^= 1;

@@ -746,31 +746,41 @@ struct FcgHist {
     double* rr;  // r_{k+1} . r_{k+1}
 };
 
+// 7-point neighbour sum of f at node u, weighted per axis; offsets zeroed on
+// non-regular nodes (the result is then discarded) keep every load in bounds
+template <typename F>
+__device__ inline double nbr_sum(const Stencil& s, unsigned u, bool reg, F f) {
+    const unsigned oi = reg ? 1u : 0u, oj = reg ? (unsigned)s.sj : 0u, ok = reg ? (unsigned)s.sk : 0u;
+    return (f(u - oi) + f(u + oi)) * s.cx + (f(u - oj) + f(u + oj)) * s.cy +
+           (f(u - ok) + f(u + ok)) * s.cz;
+}
+
 // r = b - A phi on regular rows (0 elsewhere); out += {r.r, b.b, #(phi != 0)}
 __global__ void __launch_bounds__(BLOCK)
     k_fcg_resid(Stencil s, const uint8_t* __restrict__ nt, const double* __restrict__ phi,
                 const double* __restrict__ rho, double ieps, double* __restrict__ r,
                 double* __restrict__ out) {
     double rr = 0, bb = 0, cold = 0;
+    auto ph = [&](unsigned w) { return phi[w]; };
     long long end;
-    for (long long u = chunk_begin(s.nn, end) + threadIdx.x; u < end; u += BLOCK) {
-        uint8_t t = nt[u];
-        double rv = 0;
-        if (t == NT_REG) {
-            double b = -rho[u] * ieps;
-            double Ap = apply_A<double>(t, u, s, [&](long long q) { return phi[q]; });
-            rv = b - Ap;
+    for (long long m = chunk_begin(s.nn, end) + threadIdx.x; m < end; m += BLOCK) {
+        const unsigned u = (unsigned)m;
+        const bool reg = nt[u] == NT_REG;  // regular nodes are interior (Dirichlet walls)
+        const double pu = phi[u];
+        const double b = -rho[u] * ieps;
+        const double rv = reg ? b - (nbr_sum(s, u, reg, ph) - s.twos * pu) : 0.0;
+        if (reg) {
             bb += b * b;
-            cold += phi[u] != 0.0;
+            cold += pu != 0.0;
         }
         r[u] = rv;
         rr += rv * rv;
     }
     double a = block_sum(rr), b2 = block_sum(bb), c = block_sum(cold);
-    if (threadIdx.x == 0) {
-        atomicAdd(&out[0], a);
-        atomicAdd(&out[1], b2);
-        atomicAdd(&out[2], c);
+    if (threadIdx.x == 0) {  // striped: out, out + SSIZE, out + 2 SSIZE
+        sadd(out, a);
+        sadd(out + SSIZE, b2);
+        sadd(out + 2 * SSIZE, c);
     }
 }
 
@@ -827,14 +837,6 @@ __global__ void __launch_bounds__(BLOCK)
     if (threadIdx.x == 0) sadd(h.rr + k * SSIZE, a);
 }
 
-// 7-point neighbour sum of f at node u, weighted per axis; offsets zeroed on
-// non-regular nodes (the result is then discarded) keep every load in bounds
-template <typename F>
-__device__ inline double nbr_sum(const Stencil& s, unsigned u, bool reg, F f) {
-    const unsigned oi = reg ? 1u : 0u, oj = reg ? (unsigned)s.sj : 0u, ok = reg ? (unsigned)s.sk : 0u;
-    return (f(u - oi) + f(u + oi)) * s.cx + (f(u - oj) + f(u + oj)) * s.cy +
-           (f(u - ok) + f(u + ok)) * s.cz;
-}
 
 // One red-black Gauss-Seidel sweep on A z = r from z = 0, in one pass: the
 // red half-sweep gives z = -r / (2 sum 1/dx^2) on red regular nodes
@@ -1013,17 +1015,21 @@ static int solve_learned(pic_ctx* ctx, const pic_solver_params* p, pic_solver_st
     h.pq = h.rz + hslots;
     h.zq = h.pq + hslots;
     h.rr = h.zq + hslots;
-    double* sc = ctx->scratch + 8;
+    double* sc = ctx->scratch + LEARNED_WS + 5 * SSIZE;  // three striped scalars
     unsigned nb = stencil_blocks(s.nn), nbr = stencil_blocks_red(s.nn);
     auto precond = [&](const double* rin, double* zout, const double* qv, double* zq, double* rz) {
         return learned_precond(ctx, s, rin, zout, ftmp, qv, zq, rz);
     };
     double h0[3];
-    PIC_HIP(ctx, hipMemsetAsync(sc, 0, 3 * sizeof(double), ctx->stream));
-    k_fcg_resid<<<stencil_blocks_red(s.nn), BLOCK, 0, ctx->stream>>>(s, ctx->ntype, ctx->phi, ctx->rho,
-                                                                     1.0 / EPS0, r, sc);
-    PIC_HIP(ctx, hipMemcpyAsync(h0, sc, sizeof(h0), hipMemcpyDeviceToHost, ctx->stream));
-    PIC_HIP(ctx, pic_spin_wait(ctx->stream));
+    {
+        std::vector<double> hs(3 * SSIZE);
+        PIC_HIP(ctx, hipMemsetAsync(sc, 0, hs.size() * sizeof(double), ctx->stream));
+        k_fcg_resid<<<nb, BLOCK, 0, ctx->stream>>>(s, ctx->ntype, ctx->phi, ctx->rho, 1.0 / EPS0, r, sc);
+        PIC_HIP(ctx, hipMemcpyAsync(hs.data(), sc, hs.size() * sizeof(double), hipMemcpyDeviceToHost,
+                                    ctx->stream));
+        PIC_HIP(ctx, pic_spin_wait(ctx->stream));
+        for (int q = 0; q < 3; q++) h0[q] = shost(hs.data() + q * SSIZE);
+    }
     double bnorm2 = h0[1] > 0 ? h0[1] : 1.0;
     double target2 = (p->tol_kind == PIC_TOL_REF_RMS) ? p->tol * p->tol * (double)nn
                                                       : p->tol * p->tol * bnorm2;
