@@ -52,12 +52,18 @@ struct PushArgs {
     int kick;
 };
 
+// 32-bit node offsets (grids hold fewer than 2^31 nodes)
 __device__ inline float3 gather_e(const GridC& g, const float4* __restrict__ E, int i, int j, int k,
                                   float fx, float fy, float fz) {
-    long long u = ((long long)k * g.nj + j) * g.ni + i;
-    long long sj = g.ni, sk = (long long)g.ni * g.nj;
-    float4 e000 = E[u], e001 = E[u + sk], e010 = E[u + sj], e011 = E[u + sj + sk];
-    float4 e100 = E[u + 1], e101 = E[u + 1 + sk], e110 = E[u + 1 + sj], e111 = E[u + 1 + sj + sk];
+    const unsigned sj = (unsigned)g.ni, sk = (unsigned)g.ni * (unsigned)g.nj;
+    const unsigned u = (unsigned)k * sk + (unsigned)j * sj + (unsigned)i;
+    // four node pairs (i, i+1): one address each, the +1 is an immediate offset
+    const float4* p00 = E + u;
+    const float4* p01 = p00 + sk;
+    const float4* p10 = p00 + sj;
+    const float4* p11 = p10 + sk;
+    float4 e000 = p00[0], e001 = p01[0], e010 = p10[0], e011 = p11[0];
+    float4 e100 = p00[1], e101 = p01[1], e110 = p10[1], e111 = p11[1];
     float gx = 1.f - fx, gy = 1.f - fy, gz = 1.f - fz;
     float a = gx * gy, b = gx * fy, c = fx * gy, d = fx * fy;
     float w000 = a * gz, w001 = a * fz, w010 = b * gz, w011 = b * fz;

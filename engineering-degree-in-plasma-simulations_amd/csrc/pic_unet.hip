@@ -383,7 +383,7 @@ __global__ void __launch_bounds__(256) k_conv3(ConvArgs a) {
 // (iy, iz) at x shift ox is read from LDS once and feeds the MFMAs of every
 // output row (iy - dy, iz - dz) of the block, tap (ox, dy, dz).
 template <int KS, int RY, int RZ, int NWN>
-__global__ void __launch_bounds__(256) k_conv3_rb(ConvArgs a) {
+__global__ void __launch_bounds__(256, 2) k_conv3_rb(ConvArgs a) {  // two workgroups per CU
     constexpr int NWZ = 4 / NWN;
     constexpr int CIN = 32 * KS, NC = CIN / 16;
     constexpr int SX = TX + 2, SY = RY + 2, SZ = RZ * NWZ + 2, NV = SX * SY * SZ;
@@ -397,6 +397,13 @@ __global__ void __launch_bounds__(256) k_conv3_rb(ConvArgs a) {
     const int g = lane >> 4, m = lane & 15;
     const int wn = wave % NWN, wz = wave / NWN;
     const int n0 = (blockIdx.y * NWN + wn) * 16;
+    const bf16* wrow = a.w + (long long)(n0 + m) * 27 * CIN + 8 * g;
+    // the first K step's B fragments are requested before the input tile, so
+    // the two loads' latencies overlap
+    bf8v bfr[27];
+#pragma unroll
+    for (int t = 0; t < 27; t++)
+        bfr[t] = __builtin_bit_cast(bf8v, *reinterpret_cast<const uint4*>(wrow + t * CIN));
     {
         uint4 rt[RT];
 #pragma unroll
@@ -424,13 +431,13 @@ __global__ void __launch_bounds__(256) k_conv3_rb(ConvArgs a) {
 #pragma unroll
         for (int rz = 0; rz < RZ; rz++) acc[ry][rz] = f4v{0.f, 0.f, 0.f, 0.f};
     const int ch = g >> 1, hf = g & 1;  // lane group g: channels 8g..8g+7 of the K step
-    const bf16* wrow = a.w + (long long)(n0 + m) * 27 * CIN + 8 * g;
 #pragma unroll
     for (int s = 0; s < KS; s++) {
-        bf8v bfr[27];
+        if (s > 0) {
 #pragma unroll
-        for (int t = 0; t < 27; t++)
-            bfr[t] = __builtin_bit_cast(bf8v, *reinterpret_cast<const uint4*>(wrow + t * CIN + 32 * s));
+            for (int t = 0; t < 27; t++)
+                bfr[t] = __builtin_bit_cast(bf8v, *reinterpret_cast<const uint4*>(wrow + t * CIN + 32 * s));
+        }
         const int c16 = 2 * s + ch;
 #pragma unroll
         for (int iz = 0; iz < RZ + 2; iz++)
