@@ -1810,6 +1810,8 @@ static bool conv(pic_ctx* ctx, const Dims& d, const bf16* in, int cin, const bf1
                 k_conv3_rb<2, 4, 4, 2><<<gr, 256, 0, ctx->stream>>>(a);
             return pool != nullptr;
         }
+        // 64 output channels: 16x2x4-voxel blocks (measured faster than 16x2x2
+        // with twice the workgroups at 32^3)
         if ((cin == 32 || cin == 64) && cout == 64 && d.nx % 16 == 0 && d.ny % 2 == 0 &&
             d.nz % 4 == 0 && (d.nx / 16) * (d.ny / 2) * (d.nz / 4) >= 256) {
             dim3 gr((d.nx / 16) * (d.ny / 2) * (d.nz / 4), 1);
