@@ -439,24 +439,55 @@ __global__ void __launch_bounds__(256, 2) k_conv3_rb(ConvArgs a) {  // two workg
                 bfr[t] = __builtin_bit_cast(bf8v, *reinterpret_cast<const uint4*>(wrow + t * CIN + 32 * s));
         }
         const int c16 = 2 * s + ch;
+        if constexpr (RY * RZ >= 16) {
+            // 16 output rows per wave: registers leave no room for a queue
 #pragma unroll
-        for (int iz = 0; iz < RZ + 2; iz++)
+            for (int iz = 0; iz < RZ + 2; iz++)
 #pragma unroll
-            for (int iy = 0; iy < RY + 2; iy++)
+                for (int iy = 0; iy < RY + 2; iy++)
 #pragma unroll
-                for (int ox = 0; ox < 3; ox++) {
-                    const int sv = ((wz * RZ + iz) * SY + iy) * SX + m + ox;
-                    const bf8v afr = __builtin_bit_cast(bf8v, tile[(c16 * NV + sv) * 2 + hf]);
+                    for (int ox = 0; ox < 3; ox++) {
+                        const int sv = ((wz * RZ + iz) * SY + iy) * SX + m + ox;
+                        const bf8v afr = __builtin_bit_cast(bf8v, tile[(c16 * NV + sv) * 2 + hf]);
 #pragma unroll
-                    for (int dz = 0; dz < 3; dz++)
+                        for (int dz = 0; dz < 3; dz++)
 #pragma unroll
-                        for (int dy = 0; dy < 3; dy++) {
-                            const int ry = iy - dy, rz = iz - dz;
-                            if (ry < 0 || ry >= RY || rz < 0 || rz >= RZ) continue;
-                            acc[ry][rz] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(
-                                afr, bfr[(dz * 3 + dy) * 3 + ox], acc[ry][rz], 0, 0, 0);
-                        }
-                }
+                            for (int dy = 0; dy < 3; dy++) {
+                                const int ry = iy - dy, rz = iz - dz;
+                                if (ry < 0 || ry >= RY || rz < 0 || rz >= RZ) continue;
+                                acc[ry][rz] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(
+                                    afr, bfr[(dz * 3 + dy) * 3 + ox], acc[ry][rz], 0, 0, 0);
+                            }
+                    }
+        } else {
+            // the (iz, iy, ox) walk flattened, A fragments read PD steps ahead
+            // into a register queue so the LDS latency hides behind the MFMAs
+            constexpr int NQ = (RZ + 2) * (RY + 2) * 3, PD = 4;
+            auto afrag = [&](int q) {
+                const int ox = q % 3, iy = (q / 3) % (RY + 2), iz = q / (3 * (RY + 2));
+                const int sv = ((wz * RZ + iz) * SY + iy) * SX + m + ox;
+                return __builtin_bit_cast(bf8v, tile[(c16 * NV + sv) * 2 + hf]);
+            };
+            bf8v aq[PD];
+#pragma unroll
+            for (int q = 0; q < PD; q++) aq[q] = afrag(q);
+#pragma unroll
+            for (int q = 0; q < NQ; q++) {
+                const bf8v afr = aq[q % PD];
+                if (q + PD < NQ) aq[q % PD] = afrag(q + PD);
+                __builtin_amdgcn_sched_barrier(0);  // keep the read ahead of these MFMAs
+                const int ox = q % 3, iy = (q / 3) % (RY + 2), iz = q / (3 * (RY + 2));
+#pragma unroll
+                for (int dz = 0; dz < 3; dz++)
+#pragma unroll
+                    for (int dy = 0; dy < 3; dy++) {
+                        const int ry = iy - dy, rz = iz - dz;
+                        if (ry < 0 || ry >= RY || rz < 0 || rz >= RZ) continue;
+                        acc[ry][rz] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(
+                            afr, bfr[(dz * 3 + dy) * 3 + ox], acc[ry][rz], 0, 0, 0);
+                    }
+            }
+        }
     }
     // D fragment: lane holds rows 4g + j (x offset) of column m (channel).
     // The workgroup holds all CO = cout channels of its voxels (one N-block per
@@ -1800,6 +1831,8 @@ static bool conv(pic_ctx* ctx, const Dims& d, const bf16* in, int cin, const bf1
     if (taps == 27) {
         // register-blocked kernel where the shape fits it (every large-grid
         // layer of networks with 32 or 64 channels)
+        // 32 output channels: 16x4x4-voxel blocks per wave, two waves along z
+        // (measured faster than 16x2x4 blocks with the A-fragment queue)
         if ((cin == 32 || cin == 64) && cout == 32 && d.nx % 16 == 0 && d.ny % 4 == 0 &&
             d.nz % 8 == 0 && (d.nx / 16) * (d.ny / 4) * (d.nz / 8) >= 256) {
             dim3 gr((d.nx / 16) * (d.ny / 4) * (d.nz / 8), 1);
