@@ -152,6 +152,8 @@ def main():
     ap.add_argument("--copy-calibration", action="store_true")
     ap.add_argument("--step-solver", default="auto",  # learned when its weights exist, else sor_rb
                     choices=["auto", "sor_rb", "pcg_mixed", "pcg_fp64", "learned"])
+    # each step's solve starts from 2 phi_n - phi_(n-1) ("extrapolate") or from phi_n ("previous")
+    ap.add_argument("--warm-start", default="extrapolate", choices=["extrapolate", "previous"])
     args = ap.parse_args()
     if args.copy_calibration:
         copy_calibration()
@@ -210,7 +212,8 @@ def main():
               "pcg_mixed": dict(method=pg.SOLVER_PCG_MIXED),
               "pcg_fp64": dict(method=pg.SOLVER_PCG),
               "learned": dict(method=pg.SOLVER_LEARNED, check_every=1)}[name]
-        kw.update(tol=1e-6, tol_kind=pg.TOL_REL, max_it=20000)
+        kw.update(tol=1e-6, tol_kind=pg.TOL_REL, max_it=20000,
+                  flags=pg.SOLVE_EXTRAPOLATE if args.warm_start == "extrapolate" else 0)
         return kw
 
     solve_kw = solver_kw(step_solver)
@@ -359,7 +362,7 @@ def main():
         "config": {"workload": "scaled plain PIC, %d^3 nodes, 2 x %d particles, counting sort "
                                "every step, Poisson to 1e-6 rel. residual every step" % (n, N),
                    "grid_nodes": n ** 3, "particles_per_species": N, "dt": dt,
-                   "poisson_step_solver": step_solver},
+                   "poisson_step_solver": step_solver, "poisson_warm_start": args.warm_start},
         "particles_alive": n_alive,
         "poisson_iterations_per_step": iters_main,
         "step_solver_compare": compare,

@@ -168,6 +168,7 @@ extern "C" void pic_destroy(pic_ctx* ctx) {
     }
     pic_unet_free(ctx);
     (void)hipFree(ctx->phi);
+    (void)hipFree(ctx->phi_prev);
     (void)hipFree(ctx->rho);
     (void)hipFree(ctx->ef);
     (void)hipFree(ctx->ntype);
@@ -591,6 +592,7 @@ extern "C" int pic_field_upload(pic_ctx* ctx, int which, const double* src) {
     size_t nn = (size_t)ctx->g.nn;
     if (which == PIC_FIELD_PHI) {
         PIC_HIP(ctx, hipMemcpyAsync(ctx->phi, src, nn * 8, hipMemcpyHostToDevice, ctx->stream));
+        ctx->phi_prev_valid = false;  // the solution history no longer leads to this phi
     } else if (which == PIC_FIELD_RHO) {
         PIC_HIP(ctx, hipMemcpyAsync(ctx->rho, src, nn * 8, hipMemcpyHostToDevice, ctx->stream));
     } else if (which == PIC_FIELD_EF) {

@@ -192,6 +192,8 @@ struct pic_ctx {
     bool ntype_dirty = true;
     // device fields
     double* phi = nullptr;
+    double* phi_prev = nullptr;  // previous solution (PIC_SOLVE_EXTRAPOLATE), allocated on demand
+    bool phi_prev_valid = false;
     double* rho = nullptr;
     float4* ef = nullptr;
     uint8_t* ntype = nullptr;

@@ -155,6 +155,17 @@ __global__ void __launch_bounds__(BLOCK)
     ef[u] = make_float4(e[0], e[1], e[2], 0.f);
 }
 
+// warm start of a time loop's solve: phi = 2 phi - prev, prev = the old phi
+__global__ void __launch_bounds__(BLOCK)
+    k_extrapolate(long long n, double* __restrict__ phi, double* __restrict__ prev) {
+    long long end;
+    for (long long m = chunk_begin(n, end) + threadIdx.x; m < end; m += BLOCK) {
+        const double p = phi[m];
+        phi[m] = 2.0 * p - prev[m];
+        prev[m] = p;
+    }
+}
+
 // ------------------------------------------------------------------------
 // red-black SOR
 // ------------------------------------------------------------------------
@@ -1130,6 +1141,19 @@ extern "C" int pic_poisson(pic_ctx* ctx, const pic_solver_params* p, pic_solver_
     PIC_HIP(ctx, hipEventCreate(&e0));
     PIC_HIP(ctx, hipEventCreate(&e1));
     PIC_HIP(ctx, hipEventRecord(e0, ctx->stream));
+    if (p->flags & PIC_SOLVE_EXTRAPOLATE) {
+        const size_t nn = (size_t)ctx->g.nn;
+        if (!ctx->phi_prev) PIC_HIP(ctx, hipMalloc(&ctx->phi_prev, nn * sizeof(double)));
+        if (ctx->phi_prev_valid) {
+            k_extrapolate<<<stencil_blocks((long long)nn), BLOCK, 0, ctx->stream>>>((long long)nn, ctx->phi,
+                                                                                    ctx->phi_prev);
+            PIC_HIP(ctx, hipGetLastError());
+        } else {
+            PIC_HIP(ctx, hipMemcpyAsync(ctx->phi_prev, ctx->phi, nn * sizeof(double), hipMemcpyDeviceToDevice,
+                                        ctx->stream));
+            ctx->phi_prev_valid = true;
+        }
+    }
     switch (p->method) {
         case PIC_SOLVER_SOR_RB: rc = solve_sor(ctx, p, st); break;
         case PIC_SOLVER_PCG: rc = solve_pcg(ctx, p, st, false); break;

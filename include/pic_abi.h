@@ -128,6 +128,12 @@ int pic_charge_density(pic_ctx* ctx);
 #define PIC_TOL_REF_RMS 0    /* reference criterion: sqrt(sum R^2 / N) < tol */
 #define PIC_TOL_REL 1        /* ||b - A phi||_2 / ||b||_2 < tol (regular nodes) */
 #define PIC_SOLVE_AXIS_SWAP 1 /* PCG coefficients placed as in PotentialSolver.cpp:457-463 */
+/* Start from the linear extrapolation 2 phi_n - phi_(n-1) of the context's
+ * last two solutions instead of the current phi (a time loop's warm start;
+ * no counterpart in the reference, which starts from the current phi).  The
+ * first flagged solve, and the first after phi is uploaded, starts from phi.
+ * The solve runs to the same tolerance either way. */
+#define PIC_SOLVE_EXTRAPOLATE 2
 typedef struct {
     int method;
     int tol_kind;

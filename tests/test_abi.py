@@ -23,6 +23,20 @@ def test_header_declares_expected_surface():
     assert names == sorted(pic_gpu.ABI_FUNCTIONS)
 
 
+def test_header_constants_match_binding():
+    """Every solver, tolerance, flag and field constant the header defines
+    has the same value in the Python binding."""
+    import pic_gpu
+
+    text = open(HEADER).read()
+    consts = re.findall(r"^#define PIC_((?:SOLVER|SOLVE|TOL|FIELD|SFIELD|DEPOSIT)_\w+)\s+(\d+)", text, re.M)
+    assert len(consts) >= 20
+    for name, value in consts:
+        assert getattr(pic_gpu, name) == int(value), name
+    kcount = int(re.search(r"^#define PIC_K_COUNT (\d+)", text, re.M).group(1))
+    assert len(pic_gpu.K_NAMES) == kcount
+
+
 def test_library_exports_every_declared_symbol():
     import pic_gpu
 

@@ -300,6 +300,43 @@ def test_sor_dirichlet_matches_oracle(pg, golden):
     assert np.max(np.abs(phi - ref)) <= 1e-5 * np.max(np.abs(exact))
 
 
+@pytest.mark.parametrize("method", ["SOLVER_SOR_RB", "SOLVER_PCG"])
+def test_extrapolated_warm_start(pg, golden, method):
+    """PIC_SOLVE_EXTRAPOLATE on a charge density that varies linearly in time:
+    the problem is linear, so 2 phi_n - phi_(n-1) misses the next solution
+    only by the previous solves' tolerance and the solve needs far fewer
+    iterations than the one started from phi_n; every solution matches the
+    dense one.  Uploading phi drops the history, so the next flagged solve
+    is the unflagged solve bit for bit."""
+    gr, g, nt = _dirichlet_problem(golden)
+    c = _ctx(pg, gr)
+    c.dirichlet_walls()
+    m = getattr(pg, method)
+    kw = dict(tol=1e-8, tol_kind=pg.TOL_REL, max_it=20000, omega=1.4, check_every=5)
+    rho0 = golden["rho"]
+    drho = 0.2 * np.roll(rho0, 5)
+    its = []
+    for t in range(4):
+        rho = rho0 + t * drho
+        c.set_field(pg.FIELD_RHO, rho)
+        st = c.poisson(m, flags=pg.SOLVE_EXTRAPOLATE, **kw)
+        assert st["converged"] and st["rel_residual"] <= 1e-8, (t, st)
+        exact = O.dense_poisson(g, np.zeros(O.nn(g)), rho)
+        phi = c.get_field(pg.FIELD_PHI)
+        assert np.max(np.abs(phi - exact)) <= 1e-5 * np.max(np.abs(exact)), t
+        its.append(st["iterations"])
+    # t = 1 starts from phi_0 (one solution in the history), t >= 2 extrapolate
+    assert its[2] < 0.6 * its[1] and its[3] < 0.6 * its[1], its
+    zeros = np.zeros(O.nn(g))
+    c.set_field(pg.FIELD_PHI, zeros)
+    a = c.poisson(m, flags=pg.SOLVE_EXTRAPOLATE, **kw)
+    phi_a = c.get_field(pg.FIELD_PHI)
+    c.set_field(pg.FIELD_PHI, zeros)
+    b = c.poisson(m, **kw)
+    assert a["iterations"] == b["iterations"]
+    assert np.array_equal(phi_a, c.get_field(pg.FIELD_PHI))
+
+
 def test_sor_v3_neumann_object_matches_oracle(pg, golden):
     gr = golden["grid"]
     g = O.make_grid(*_grid_args(gr))
