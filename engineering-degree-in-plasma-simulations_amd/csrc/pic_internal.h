@@ -291,6 +291,17 @@ constexpr size_t SCRATCH_DOUBLES = 16384;  // ctx->scratch: [0, 4096) plain, [40
 // {x^2, count}, line-search sums {c.res, c.A c}, then one spare (rz)
 constexpr int LEARNED_WS = 4096;
 
+// Workgroup b's logical index: with a grid that is a multiple of 8, XCD x
+// (which receives workgroups x, x + 8, ...) takes the contiguous range
+// [x nb / 8, (x + 1) nb / 8), so a particle kernel's field gathers and
+// per-cell atomics over a cell-sorted array stay in one XCD's L2 slab
+// instead of every XCD fetching the whole field.  Other grids: b.
+__device__ inline size_t xcd_wg() {
+    const unsigned nb = gridDim.x, b = blockIdx.x;
+    if (nb % XCD_COUNT) return b;
+    return (size_t)(b % XCD_COUNT) * (nb / XCD_COUNT) + b / XCD_COUNT;
+}
+
 __device__ inline long long xcd_chunk(long long n, int block, long long& end) {
     int nb = gridDim.x;
     int b = blockIdx.x;

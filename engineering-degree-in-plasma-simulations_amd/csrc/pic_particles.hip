@@ -145,7 +145,7 @@ template <bool WRITE>
 __global__ void __launch_bounds__(BLOCK)
     k_push(GridC g, SpeciesBuf b, const float4* __restrict__ E, PushArgs a,
            uint32_t* __restrict__ d_n, uint32_t* __restrict__ count, uint32_t* __restrict__ rank) {
-    size_t p = (size_t)blockIdx.x * BLOCK + threadIdx.x;
+    size_t p = xcd_wg() * BLOCK + threadIdx.x;
     uint32_t n = d_n[0];
     uint32_t covered = gridDim.x * BLOCK;
     if (n > covered) n = covered;
@@ -177,7 +177,7 @@ __global__ void __launch_bounds__(BLOCK)
 __global__ void __launch_bounds__(BLOCK)
     k_hist(GridC g, SpeciesBuf b, uint32_t* __restrict__ d_n, uint32_t* __restrict__ count,
            uint32_t* __restrict__ rank) {
-    size_t p = (size_t)blockIdx.x * BLOCK + threadIdx.x;
+    size_t p = xcd_wg() * BLOCK + threadIdx.x;
     uint32_t n = d_n[0];
     uint32_t covered = gridDim.x * BLOCK;
     if (n > covered) n = covered;
@@ -288,7 +288,7 @@ __global__ void __launch_bounds__(BLOCK)
     constexpr int NP = BLOCK * SC_V;
     __shared__ uint4 st[9][BLOCK];  // cell, fx, fy, fz, vx, vy, vz, w, rank; 4 particles each
     const uint32_t n = d_n[1];
-    const size_t base = (size_t)blockIdx.x * NP;
+    const size_t base = xcd_wg() * NP;
     const int tid = threadIdx.x;
     const size_t p4 = base + (size_t)SC_V * tid;
     const uint32_t* arr[9] = {src.cell,
@@ -351,7 +351,7 @@ __global__ void __launch_bounds__(BLOCK)
     k_push_scatter(GridC g, SpeciesBuf src, SpeciesBuf dst, const float4* __restrict__ E, PushArgs a,
                    uint32_t* __restrict__ cursor, const uint32_t* __restrict__ off,
                    const uint32_t* __restrict__ d_n, uint32_t cap) {
-    const size_t p = (size_t)blockIdx.x * BLOCK + threadIdx.x;
+    const size_t p = xcd_wg() * BLOCK + threadIdx.x;
     bool valid = p < d_n[1];
     const uint32_t c = valid ? src.cell[p] : DEAD;
     valid = valid && c != DEAD;
@@ -700,7 +700,7 @@ extern "C" int pic_push(pic_ctx* ctx, int sid, double dt) {
     {
         PicProfScope ps(ctx, PIC_K_PUSH);
         size_t n = s.n_upper ? s.n_upper : 1;
-        k_push<true><<<grid_for(n, BLOCK), BLOCK, 0, ctx->stream>>>(g, s.buf[s.cur], ctx->ef, a, s.d_n,
+        k_push<true><<<xcd_blocks1(n, BLOCK), BLOCK, 0, ctx->stream>>>(g, s.buf[s.cur], ctx->ef, a, s.d_n,
                                                                     s.count, s.rank);
     }
     PIC_HIP(ctx, hipGetLastError());
@@ -716,7 +716,7 @@ int pic_histogram(pic_ctx* ctx, int sid) {
     {
         PicProfScope ps(ctx, PIC_K_SORT);
         size_t n = s.n_upper ? s.n_upper : 1;
-        k_hist<<<grid_for(n, BLOCK), BLOCK, 0, ctx->stream>>>(g, s.buf[s.cur], s.d_n, s.count,
+        k_hist<<<xcd_blocks1(n, BLOCK), BLOCK, 0, ctx->stream>>>(g, s.buf[s.cur], s.d_n, s.count,
                                                               s.rank);
     }
     PIC_HIP(ctx, hipGetLastError());
@@ -749,7 +749,7 @@ extern "C" int pic_sort(pic_ctx* ctx, int sid) {
     {
         PicProfScope ps(ctx, PIC_K_SCATTER);
         size_t n = s.n_upper ? s.n_upper : 1;
-        k_scatter<<<grid_for((n + SC_V - 1) / SC_V, BLOCK), BLOCK, 0, ctx->stream>>>(
+        k_scatter<<<xcd_blocks1((n + SC_V - 1) / SC_V, BLOCK), BLOCK, 0, ctx->stream>>>(
             g, s.buf[s.cur], s.buf[s.cur ^ 1], s.rank, s.off, s.d_n, (uint32_t)s.cap);
     }
     PIC_HIP(ctx, hipGetLastError());
@@ -775,7 +775,7 @@ extern "C" int pic_push_sort(pic_ctx* ctx, int sid, double dt) {
     PIC_HIP(ctx, hipMemsetAsync(s.count, 0, ((size_t)g.ncells + 1) * 4, ctx->stream));
     {
         PicProfScope ps(ctx, PIC_K_PUSH);
-        k_push<false><<<grid_for(n, BLOCK), BLOCK, 0, ctx->stream>>>(g, s.buf[s.cur], ctx->ef, a, s.d_n,
+        k_push<false><<<xcd_blocks1(n, BLOCK), BLOCK, 0, ctx->stream>>>(g, s.buf[s.cur], ctx->ef, a, s.d_n,
                                                                      s.count, s.rank);
     }
     scan_counts(ctx, s);
@@ -783,7 +783,7 @@ extern "C" int pic_push_sort(pic_ctx* ctx, int sid, double dt) {
     PIC_HIP(ctx, hipMemsetAsync(s.count, 0, ((size_t)g.ncells + 1) * 4, ctx->stream));
     {
         PicProfScope ps(ctx, PIC_K_SCATTER);
-        k_push_scatter<<<grid_for(n, BLOCK), BLOCK, 0, ctx->stream>>>(
+        k_push_scatter<<<xcd_blocks1(n, BLOCK), BLOCK, 0, ctx->stream>>>(
             g, s.buf[s.cur], s.buf[s.cur ^ 1], ctx->ef, a, s.count, s.off, s.d_n, (uint32_t)s.cap);
     }
     PIC_HIP(ctx, hipGetLastError());
