@@ -176,6 +176,9 @@ extern "C" void pic_destroy(pic_ctx* ctx) {
     (void)hipFree(ctx->scan_tmp);
     (void)hipFree(ctx->work);
     (void)hipFree(ctx->dep_corner);
+    if (ctx->pin) (void)hipHostFree(ctx->pin);
+    for (auto e : ctx->solve_ev)
+        if (e) (void)hipEventDestroy(e);
     for (auto& r : ctx->prof.recs) {
         (void)hipEventDestroy(r.a);
         (void)hipEventDestroy(r.b);
@@ -201,6 +204,17 @@ int pic_check_sid(pic_ctx* ctx, int sid) {
     if (!ctx) return PIC_ERR_ARG;
     if (sid < 0 || sid >= (int)ctx->sp.size())
         return pic_fail(ctx, PIC_ERR_ARG, "invalid species id");
+    return PIC_OK;
+}
+
+int pic_ensure_pin(pic_ctx* ctx, size_t bytes) {
+    if (ctx->pin_bytes >= bytes) return PIC_OK;
+    PIC_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    if (ctx->pin) (void)hipHostFree(ctx->pin);
+    ctx->pin = nullptr;
+    ctx->pin_bytes = 0;
+    PIC_HIP(ctx, hipHostMalloc(reinterpret_cast<void**>(&ctx->pin), bytes, hipHostMallocDefault));
+    ctx->pin_bytes = bytes;
     return PIC_OK;
 }
 

@@ -207,6 +207,9 @@ struct pic_ctx {
     void* unet = nullptr;  // learned preconditioner state
     float* dep_corner = nullptr;  // density deposit: 8 corner sums per cell (allocated on demand)
     int learned_last_its = 0;  // iterations of the last learned solve (first check near it)
+    double* pin = nullptr;     // pinned host memory for the solvers' convergence reads
+    size_t pin_bytes = 0;
+    hipEvent_t solve_ev[2] = {nullptr, nullptr};  // pic_poisson's timing events
 };
 
 // ---- host helpers shared between translation units ----
@@ -226,6 +229,12 @@ inline hipError_t pic_spin_wait(hipStream_t s) {
     }
     return e;
 }
+inline hipError_t pic_spin_event(hipEvent_t ev) {
+    hipError_t e;
+    while ((e = hipEventQuery(ev)) == hipErrorNotReady) {
+    }
+    return e;
+}
 // profiling brackets around a launch
 void pic_prof_begin(pic_ctx* ctx, int cat, hipEvent_t* a, hipEvent_t* b);
 void pic_prof_end(pic_ctx* ctx, int cat, hipEvent_t a, hipEvent_t b);
@@ -239,6 +248,9 @@ struct PicProfScope {
 int pic_check_sid(pic_ctx* ctx, int sid);
 int pic_ensure_ntype(pic_ctx* ctx);
 int pic_ensure_work(pic_ctx* ctx, size_t bytes);
+// ctx->pin holds at least `bytes` of pinned host memory (device-to-host
+// copies into it are asynchronous, unlike copies into pageable memory)
+int pic_ensure_pin(pic_ctx* ctx, size_t bytes);
 // histogram + ranks of the current arrays (run-aggregated atomics)
 int pic_histogram(pic_ctx* ctx, int sid);
 inline unsigned grid_for(size_t n, unsigned block) {

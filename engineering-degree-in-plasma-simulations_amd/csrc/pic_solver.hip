@@ -1001,6 +1001,18 @@ extern "C" int pic_learned_precondition(pic_ctx* ctx, const double* r, double* z
     return PIC_OK;
 }
 
+// the residual of the linear problem (no Boltzmann term)
+static SorArgs plain_args(const Stencil& s) {
+    SorArgs a;
+    a.omega = 1;
+    a.ieps = 1.0 / EPS0;
+    a.itwos = 1.0 / s.twos;
+    a.phi0 = 0;
+    a.n0 = 0;
+    a.inv_te0 = 1;
+    return a;
+}
+
 static int solve_learned(pic_ctx* ctx, const pic_solver_params* p, pic_solver_stats* st) {
     int rc = learned_check(ctx);
     if (rc) return rc;
@@ -1031,33 +1043,49 @@ static int solve_learned(pic_ctx* ctx, const pic_solver_params* p, pic_solver_st
     auto precond = [&](const double* rin, double* zout, const double* qv, double* zq, double* rz) {
         return learned_precond(ctx, s, rin, zout, ftmp, qv, zq, rz);
     };
-    double h0[3];
-    {
-        std::vector<double> hs(3 * SSIZE);
-        PIC_HIP(ctx, hipMemsetAsync(sc, 0, hs.size() * sizeof(double), ctx->stream));
-        k_fcg_resid<<<nb, BLOCK, 0, ctx->stream>>>(s, ctx->ntype, ctx->phi, ctx->rho, 1.0 / EPS0, r, sc);
-        PIC_HIP(ctx, hipMemcpyAsync(hs.data(), sc, hs.size() * sizeof(double), hipMemcpyDeviceToHost,
-                                    ctx->stream));
-        PIC_HIP(ctx, pic_spin_wait(ctx->stream));
-        for (int q = 0; q < 3; q++) h0[q] = shost(hs.data() + q * SSIZE);
-    }
-    double bnorm2 = h0[1] > 0 ? h0[1] : 1.0;
-    double target2 = (p->tol_kind == PIC_TOL_REF_RMS) ? p->tol * p->tol * (double)nn
-                                                      : p->tol * p->tol * bnorm2;
+    // pinned host copies: [0, 3 S) initial sums, [3 S, 6 S) final FP64
+    // residual sums, then the striped r.r history (S = SSIZE doubles)
+    rc = pic_ensure_pin(ctx, (6 * SSIZE + hslots) * sizeof(double));
+    if (rc) return rc;
+    double* hp = ctx->pin;
+    double* hrr = hp + 6 * SSIZE;
+    PIC_HIP(ctx, hipMemsetAsync(sc, 0, 3 * SSIZE * sizeof(double), ctx->stream));
+    k_fcg_resid<<<nb, BLOCK, 0, ctx->stream>>>(s, ctx->ntype, ctx->phi, ctx->rho, 1.0 / EPS0, r, sc);
+    PIC_HIP(ctx, hipMemcpyAsync(hp, sc, 3 * SSIZE * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    double bnorm2 = 1.0, target2 = 0.0;
+    bool have_b = false;
+    auto read_initial = [&]() {  // after a wait that covers the copy above
+        const double rr0 = shost(hp), bb = shost(hp + SSIZE);
+        bnorm2 = bb > 0 ? bb : 1.0;
+        target2 = (p->tol_kind == PIC_TOL_REF_RMS) ? p->tol * p->tol * (double)nn : p->tol * p->tol * bnorm2;
+        have_b = true;
+        return rr0;
+    };
+    // A solve that follows another learned solve on this context (a time
+    // loop) does not wait for the initial residual: it starts iterating and
+    // reads it with the first check, since such a start is almost never
+    // converged already (if it is, one iteration runs and the checks below
+    // still decide convergence).  Otherwise the host waits for it here.
+    const int prev = ctx->learned_last_its;
     int it = 0;
-    bool conv = h0[0] <= target2;
+    bool conv = false;
+    if (prev == 0) {
+        PIC_HIP(ctx, pic_spin_wait(ctx->stream));
+        conv = read_initial() <= target2;
+    }
+    bool have_final = false;  // FP64 residual of the final phi already copied to hp + 3 S
     if (!conv) {
         PIC_HIP(ctx, hipMemsetAsync(h.rz, 0, 4 * hslots * sizeof(double), ctx->stream));
         rc = precond(r, z, nullptr, nullptr, h.rz);  // rz[0] = z.r
         if (rc) return rc;
-        std::vector<double> rr(max_fcg + 2), rrs(hslots);
         double* pold = p0;
         double* pnew = p1;
         int last = 0;
         // the host reads the residual history first at 3/4 of the previous
-        // solve's iteration count (a time loop's solves are alike), then every
-        // `check` iterations
-        int next_check = std::max(check, std::min(3 * ctx->learned_last_its / 4, max_fcg));
+        // solve's iteration count (a time loop's solves are alike; a later
+        // first check would overshoot whenever a solve needs fewer
+        // iterations than the last), then every `check` iterations
+        int next_check = std::max(check, std::min(3 * prev / 4, max_fcg));
         for (int k = 0; k < max_fcg; k++) {
             {
                 PicProfScope ps(ctx, PIC_K_PCG);
@@ -1068,17 +1096,32 @@ static int solve_learned(pic_ctx* ctx, const pic_solver_params* p, pic_solver_st
             it = k + 1;
             if (it >= next_check || it == max_fcg) {
                 next_check = it + check;
-                PIC_HIP(ctx, hipMemcpyAsync(rrs.data() + (size_t)last * SSIZE, h.rr + (size_t)last * SSIZE,
+                PIC_HIP(ctx, hipMemcpyAsync(hrr + (size_t)last * SSIZE, h.rr + (size_t)last * SSIZE,
                                             (size_t)(it - last) * SSIZE * sizeof(double),
                                             hipMemcpyDeviceToHost, ctx->stream));
+                // where convergence is expected (the previous solve's count
+                // or one more), the final FP64 residual is queued with the
+                // check, saving a second wait
+                const bool expect = prev > 0 && it >= prev && it <= prev + 1;
+                if (expect) {
+                    PIC_HIP(ctx, hipMemsetAsync(sc, 0, 3 * SSIZE * sizeof(double), ctx->stream));
+                    k_resid<<<nb, BLOCK, 0, ctx->stream>>>(s, plain_args(s), ctx->ntype, ctx->phi, ctx->rho, sc);
+                    PIC_HIP(ctx, hipGetLastError());
+                    PIC_HIP(ctx, hipMemcpyAsync(hp + 3 * SSIZE, sc, 3 * SSIZE * sizeof(double),
+                                                hipMemcpyDeviceToHost, ctx->stream));
+                }
                 PIC_HIP(ctx, pic_spin_wait(ctx->stream));
+                if (!have_b) read_initial();
                 for (int j = last; j < it; j++) {
-                    rr[j] = shost(rrs.data() + (size_t)j * SSIZE);
-                    if (!(rr[j] == rr[j])) return pic_fail(ctx, PIC_ERR_NUMERIC, "learned PCG produced NaN");
-                    if (rr[j] <= target2) conv = true;
+                    const double rj = shost(hrr + (size_t)j * SSIZE);
+                    if (!(rj == rj)) return pic_fail(ctx, PIC_ERR_NUMERIC, "learned PCG produced NaN");
+                    if (rj <= target2) conv = true;
                 }
                 last = it;
-                if (conv) break;
+                if (conv) {
+                    have_final = expect;
+                    break;
+                }
             }
             if (it == max_fcg) break;
             rc = precond(r, z, q, h.zq + (size_t)k * SSIZE, h.rz + (size_t)(k + 1) * SSIZE);
@@ -1095,16 +1138,13 @@ static int solve_learned(pic_ctx* ctx, const pic_solver_params* p, pic_solver_st
         st->iterations += it;
         return rc;
     }
-    SorArgs a;
-    a.omega = 1;
-    a.ieps = 1.0 / EPS0;
-    a.itwos = 1.0 / s.twos;
-    a.phi0 = 0;
-    a.n0 = 0;
-    a.inv_te0 = 1;
     double rn[3];
-    rc = residual_norms(ctx, s, a, rn);
-    if (rc) return rc;
+    if (have_final) {
+        for (int q2 = 0; q2 < 3; q2++) rn[q2] = shost(hp + (3 + q2) * SSIZE);
+    } else {
+        rc = residual_norms(ctx, s, plain_args(s), rn);
+        if (rc) return rc;
+    }
     st->converged = 1;
     st->iterations = it;
     ctx->learned_last_its = it;
@@ -1137,9 +1177,9 @@ extern "C" int pic_poisson(pic_ctx* ctx, const pic_solver_params* p, pic_solver_
     pic_solver_stats local;
     if (!st) st = &local;
     *st = pic_solver_stats{};
-    hipEvent_t e0, e1;
-    PIC_HIP(ctx, hipEventCreate(&e0));
-    PIC_HIP(ctx, hipEventCreate(&e1));
+    for (auto& e : ctx->solve_ev)
+        if (!e) PIC_HIP(ctx, hipEventCreate(&e));
+    hipEvent_t e0 = ctx->solve_ev[0], e1 = ctx->solve_ev[1];
     PIC_HIP(ctx, hipEventRecord(e0, ctx->stream));
     if (p->flags & PIC_SOLVE_EXTRAPOLATE) {
         const size_t nn = (size_t)ctx->g.nn;
@@ -1168,11 +1208,9 @@ extern "C" int pic_poisson(pic_ctx* ctx, const pic_solver_params* p, pic_solver_
         default: rc = pic_fail(ctx, PIC_ERR_ARG, "unknown solver method");
     }
     (void)hipEventRecord(e1, ctx->stream);
-    (void)hipEventSynchronize(e1);
+    (void)pic_spin_event(e1);  // polled: the solvers' last wait has usually drained the stream
     float ms = 0;
     (void)hipEventElapsedTime(&ms, e0, e1);
     st->seconds = ms * 1e-3;
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
     return rc;
 }

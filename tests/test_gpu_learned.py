@@ -71,3 +71,43 @@ def test_learned_solver_refuses_without_network():
     ctx.dirichlet_walls()
     with pytest.raises(pg.PicError):
         ctx.poisson(method=pg.SOLVER_LEARNED, tol=1e-6)
+
+
+def test_learned_solver_time_loop_checks():
+    """Learned solves in sequence on one context, as in a time loop: after the
+    first, a solve reads its initial residual with its first convergence check
+    and queues the final FP64 residual with the check where it expects to
+    converge.  Every solve reports the oracle's residual of the phi it leaves;
+    a solve that starts converged (the same density again) stays converged."""
+    import pic_gpu as pg
+
+    n = 32
+    L = 0.01 * (n - 1)
+    ctx = pg.Context(n, n, n, [0, 0, 0], [L, L, L])
+    ctx.dirichlet_walls()
+    g = O.make_grid(n, n, n, [0, 0, 0], [L, L, L])
+    ctx.unet_create((16, 16, 16), seed=3, coarsen=2)
+    ctx.unet_set_params(ctx.unet_get_params())  # the random initial network: more iterations, same checks
+    rho0 = _blob_density(n, 7)
+    drho = 0.05 * _blob_density(n, 8)
+    ctx.set_field(pg.FIELD_PHI, np.zeros(ctx.nn))
+    kw = dict(method=pg.SOLVER_LEARNED, tol=1e-6, tol_kind=pg.TOL_REL, max_it=5000)
+    its = []
+    for t in range(6):
+        rho = rho0 + t * drho
+        ctx.set_field(pg.FIELD_RHO, rho)
+        s = ctx.poisson(flags=pg.SOLVE_EXTRAPOLATE, **kw)
+        phi = ctx.get_field(pg.FIELD_PHI)
+        r2, b2 = O.residual_dirichlet(g, phi, rho)
+        print("step", t, s["iterations"], s["rel_residual"], r2 / b2, flush=True)
+        assert s["converged"] and s["rel_residual"] <= 1e-6, (t, s)
+        assert s["rel_residual"] == pytest.approx(r2 / b2, rel=1e-6), t
+        its.append(s["iterations"])
+    # a density changing linearly in time: the extrapolated start is almost
+    # the solution, and the checks must not carry the solve far past it
+    assert its[-1] < 0.6 * its[0], its
+    s = ctx.poisson(**kw)  # already converged: stays so
+    phi2 = ctx.get_field(pg.FIELD_PHI)
+    r2, b2 = O.residual_dirichlet(g, phi2, rho)
+    assert s["converged"] and r2 / b2 <= 1e-6 and s["rel_residual"] == pytest.approx(r2 / b2, rel=1e-6)
+    assert np.max(np.abs(phi2 - phi)) <= 1e-5 * np.max(np.abs(phi))
