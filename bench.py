@@ -147,8 +147,8 @@ def main():
     ap.add_argument("--nodes", type=int, default=128)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-poisson-compare", action="store_true")
-    ap.add_argument("--cpu-particles", type=int, default=8_000_000)
-    ap.add_argument("--cpu-steps", type=int, default=8)
+    ap.add_argument("--cpu-particles", type=int, default=20_000_000)  # one species of the workload
+    ap.add_argument("--cpu-steps", type=int, default=20)  # about 10 s of host work
     ap.add_argument("--copy-calibration", action="store_true")
     ap.add_argument("--step-solver", default="auto",  # learned when its weights exist, else sor_rb
                     choices=["auto", "sor_rb", "pcg_mixed", "pcg_fp64", "learned"])
