@@ -141,37 +141,57 @@ __device__ inline uint32_t cell_key(const GridC& g, uint32_t c) {
 // WRITE: the push proper (positions, velocities, changed cell words and the
 // ranks inside the cells are written); otherwise only the counting sort's
 // histogram of the pushed particles (the first pass of pic_push_sort).
+// PUSH_ILP particles per thread, BLOCK apart: every thread issues the loads
+// of all its particles (cell word, offsets, velocities: no load waits on the
+// cell word) before it waits for any, so twice the bytes are in flight.
+constexpr int PUSH_ILP = 2;
 template <bool WRITE>
 __global__ void __launch_bounds__(BLOCK)
     k_push(GridC g, SpeciesBuf b, const float4* __restrict__ E, PushArgs a,
            uint32_t* __restrict__ d_n, uint32_t* __restrict__ count, uint32_t* __restrict__ rank) {
-    size_t p = xcd_wg() * BLOCK + threadIdx.x;
+    const size_t p0 = xcd_wg() * (BLOCK * PUSH_ILP) + threadIdx.x;
     uint32_t n = d_n[0];
-    uint32_t covered = gridDim.x * BLOCK;
+    uint32_t covered = gridDim.x * BLOCK * PUSH_ILP;
     if (n > covered) n = covered;
-    if (p == 0) d_n[1] = n;
-    bool valid = p < n;
-    uint32_t key = 0xFFFFFFFFu;
-    if (valid) {
-        uint32_t c = b.cell[p];
-        key = (uint32_t)g.ncells;
-        if (c != DEAD) {
-            float f[3] = {b.f[0][p], b.f[1][p], b.f[2][p]};
-            float v[3] = {b.v[0][p], b.v[1][p], b.v[2][p]};
-            const uint32_t nc = advance(g, E, a, c, f, v);
-            key = cell_key(g, nc);
-            if (WRITE) {
-                if (nc != c) b.cell[p] = nc;  // most particles stay in their cell
+    if (p0 == 0) d_n[1] = n;
+    uint32_t c[PUSH_ILP];
+    float f[PUSH_ILP][3], v[PUSH_ILP][3];
 #pragma unroll
-                for (int d = 0; d < 3; d++) {
-                    b.f[d][p] = f[d];
-                    b.v[d][p] = v[d];
-                }
+    for (int q = 0; q < PUSH_ILP; q++) {
+        const size_t p = p0 + (size_t)q * BLOCK;
+        c[q] = DEAD;
+        if (p < n) {
+            c[q] = b.cell[p];
+#pragma unroll
+            for (int d = 0; d < 3; d++) {
+                f[q][d] = b.f[d][p];
+                v[q][d] = b.v[d][p];
             }
         }
     }
-    uint32_t r = run_histogram(key, valid, count);
-    if (WRITE && valid) rank[p] = r;
+#pragma unroll
+    for (int q = 0; q < PUSH_ILP; q++) {
+        const size_t p = p0 + (size_t)q * BLOCK;
+        const bool valid = p < n;
+        uint32_t key = 0xFFFFFFFFu;
+        if (valid) {
+            key = (uint32_t)g.ncells;
+            if (c[q] != DEAD) {
+                const uint32_t nc = advance(g, E, a, c[q], f[q], v[q]);
+                key = cell_key(g, nc);
+                if (WRITE) {
+                    if (nc != c[q]) b.cell[p] = nc;  // most particles stay in their cell
+#pragma unroll
+                    for (int d = 0; d < 3; d++) {
+                        b.f[d][p] = f[q][d];
+                        b.v[d][p] = v[q][d];
+                    }
+                }
+            }
+        }
+        const uint32_t r = run_histogram(key, valid, count);
+        if (WRITE && valid) rank[p] = r;
+    }
 }
 
 __global__ void __launch_bounds__(BLOCK)
@@ -347,40 +367,50 @@ __global__ void __launch_bounds__(BLOCK)
 // off[new cell] + its slot in the cell, taken from the per-cell cursor
 // (zeroed; wave-aggregated atomics as in the histogram); removed particles
 // are dropped.
+// ILP particles per thread, BLOCK apart (one measured fastest: the stores
+// to the sorted positions of more particles per thread slowed it down)
+template <int ILP>
 __global__ void __launch_bounds__(BLOCK)
     k_push_scatter(GridC g, SpeciesBuf src, SpeciesBuf dst, const float4* __restrict__ E, PushArgs a,
                    uint32_t* __restrict__ cursor, const uint32_t* __restrict__ off,
                    const uint32_t* __restrict__ d_n, uint32_t cap) {
-    const size_t p = xcd_wg() * BLOCK + threadIdx.x;
-    bool valid = p < d_n[1];
-    const uint32_t c = valid ? src.cell[p] : DEAD;
-    valid = valid && c != DEAD;
-    float f[3] = {0.f, 0.f, 0.f}, v[3] = {0.f, 0.f, 0.f};
-    float w = 0.f;
-    uint32_t nc = DEAD;
-    if (valid) {
-        f[0] = src.f[0][p];
-        f[1] = src.f[1][p];
-        f[2] = src.f[2][p];
-        v[0] = src.v[0][p];
-        v[1] = src.v[1][p];
-        v[2] = src.v[2][p];
-        w = src.w[p];
-        nc = advance(g, E, a, c, f, v);
-        valid = nc != DEAD;
-    }
-    const uint32_t key = valid ? cell_key(g, nc) : 0xFFFFFFFFu;
-    const uint32_t r = run_histogram(key, valid, cursor);  // every lane of the wave takes part
-    if (!valid) return;
-    const uint32_t dst_i = off[key] + r;
-    if (dst_i >= cap) return;  // cannot happen with a current histogram
-    dst.cell[dst_i] = nc;
+    const size_t p0 = xcd_wg() * (BLOCK * ILP) + threadIdx.x;
+    const uint32_t n = d_n[1];
+    uint32_t c[ILP];
+    float f[ILP][3], v[ILP][3], w[ILP];
 #pragma unroll
-    for (int d = 0; d < 3; d++) {
-        dst.f[d][dst_i] = f[d];
-        dst.v[d][dst_i] = v[d];
+    for (int q = 0; q < ILP; q++) {  // every load before any wait
+        const size_t p = p0 + (size_t)q * BLOCK;
+        c[q] = DEAD;
+        w[q] = 0.f;
+        if (p < n) {
+            c[q] = src.cell[p];
+#pragma unroll
+            for (int d = 0; d < 3; d++) {
+                f[q][d] = src.f[d][p];
+                v[q][d] = src.v[d][p];
+            }
+            w[q] = src.w[p];
+        }
     }
-    dst.w[dst_i] = w;
+#pragma unroll
+    for (int q = 0; q < ILP; q++) {
+        uint32_t nc = DEAD;
+        if (c[q] != DEAD) nc = advance(g, E, a, c[q], f[q], v[q]);
+        const bool valid = nc != DEAD;
+        const uint32_t key = valid ? cell_key(g, nc) : 0xFFFFFFFFu;
+        const uint32_t r = run_histogram(key, valid, cursor);  // every lane of the wave takes part
+        if (!valid) continue;
+        const uint32_t dst_i = off[key] + r;
+        if (dst_i >= cap) continue;  // cannot happen with a current histogram
+        dst.cell[dst_i] = nc;
+#pragma unroll
+        for (int d = 0; d < 3; d++) {
+            dst.f[d][dst_i] = f[q][d];
+            dst.v[d][dst_i] = v[q][d];
+        }
+        dst.w[dst_i] = w[q];
+    }
 }
 
 // ---- deposit: one workgroup per cell row (j, k); the row's particles are
@@ -700,7 +730,7 @@ extern "C" int pic_push(pic_ctx* ctx, int sid, double dt) {
     {
         PicProfScope ps(ctx, PIC_K_PUSH);
         size_t n = s.n_upper ? s.n_upper : 1;
-        k_push<true><<<xcd_blocks1(n, BLOCK), BLOCK, 0, ctx->stream>>>(g, s.buf[s.cur], ctx->ef, a, s.d_n,
+        k_push<true><<<xcd_blocks1(n, BLOCK * PUSH_ILP), BLOCK, 0, ctx->stream>>>(g, s.buf[s.cur], ctx->ef, a, s.d_n,
                                                                     s.count, s.rank);
     }
     PIC_HIP(ctx, hipGetLastError());
@@ -775,7 +805,7 @@ extern "C" int pic_push_sort(pic_ctx* ctx, int sid, double dt) {
     PIC_HIP(ctx, hipMemsetAsync(s.count, 0, ((size_t)g.ncells + 1) * 4, ctx->stream));
     {
         PicProfScope ps(ctx, PIC_K_PUSH);
-        k_push<false><<<xcd_blocks1(n, BLOCK), BLOCK, 0, ctx->stream>>>(g, s.buf[s.cur], ctx->ef, a, s.d_n,
+        k_push<false><<<xcd_blocks1(n, BLOCK * PUSH_ILP), BLOCK, 0, ctx->stream>>>(g, s.buf[s.cur], ctx->ef, a, s.d_n,
                                                                      s.count, s.rank);
     }
     scan_counts(ctx, s);
@@ -783,7 +813,7 @@ extern "C" int pic_push_sort(pic_ctx* ctx, int sid, double dt) {
     PIC_HIP(ctx, hipMemsetAsync(s.count, 0, ((size_t)g.ncells + 1) * 4, ctx->stream));
     {
         PicProfScope ps(ctx, PIC_K_SCATTER);
-        k_push_scatter<<<xcd_blocks1(n, BLOCK), BLOCK, 0, ctx->stream>>>(
+        k_push_scatter<1><<<xcd_blocks1(n, BLOCK), BLOCK, 0, ctx->stream>>>(
             g, s.buf[s.cur], s.buf[s.cur ^ 1], ctx->ef, a, s.count, s.off, s.d_n, (uint32_t)s.cap);
     }
     PIC_HIP(ctx, hipGetLastError());
