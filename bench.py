@@ -35,6 +35,8 @@ QE = 1.602176565e-19
 ME = 9.10938215e-31
 AMU = 1.660538921e-27
 EV_K = 11604.52
+KB = 1.380648e-23
+SIGMA_TABLE = os.path.join(PKG, "data", "sigma_el_O.dat")
 HBM_PEAK_GBS = 8000.0
 MFMA_BF16_PEAK_TFLOPS = 2500.0  # dense BF16 (MI355X)
 
@@ -109,6 +111,105 @@ def cpu_baseline(threads, particles, steps):
             "kind": "port", "sample": sample.replace("push + deposit", "push")}
 
 
+def cpu_baseline_dsmc(nxy, nz, n_neu, n_ele, steps, dt):
+    """The reference's MC_MEX_Ionization::apply (serial: its cell sort and the
+    NTC pair loop) on a box of the discharge workload's cell densities."""
+    harness = os.path.join(ROOT, "oracle", "_ref", "ref_harness")
+    if not os.path.exists(harness):
+        return None
+    out = subprocess.run([harness, "bench_mcc", str(nxy), str(nz), str(n_neu), str(n_ele), str(steps),
+                          repr(dt), SIGMA_TABLE], capture_output=True, text=True, timeout=900)
+    r = json.loads([ln for ln in out.stdout.splitlines() if ln.startswith("{")][-1])
+    return {"value": r["particles_per_s"], "unit": "DSMC particles/s", "cores": 1, "kind": "reference",
+            "sample": "%d cells, %d O2 + %d e- at the workload's cell densities, %d apply calls (median)"
+                      % (r["cells"], n_neu, n_ele, steps), "apply_s": r["apply_s"]}
+
+
+def dsmc_bench(pg, args, cpu):
+    """The DSMC step of the discharge workload on one GPU: 256 x 64 x 64 nodes,
+    O2 at 10 Pa / 300 K as variable-weight neutrals, electrons at 10 eV and
+    ions, 50M particles in all; a step is pic_mcc_apply (the counting sort of
+    neutrals and electrons, then one thread per cell running the NTC groups,
+    collisions, ionization and births), as the reference's apply sorts and
+    pairs.  Returns the section of the bench line."""
+    nxy, nz, dx = 64, 256, 1e-4
+    L = np.array([dx * (nxy - 1), dx * (nxy - 1), dx * (nz - 1)])
+    box = 0.98 * L
+    vol = float(np.prod(box))
+    n_gas = 10.0 / (KB * 300.0)
+    Nn, Ne, Ni = args.dsmc_neutrals, args.dsmc_electrons, args.dsmc_ions
+    w_n = n_gas * vol / Nn
+    w_e = w_n / 1000.0
+    dt = args.dsmc_dt
+    ctx = pg.Context(nxy, nxy, nz, [0, 0, 0], L)
+    sn = ctx.species_add(32 * AMU, 0.0, w_n, 2 * Nn)  # elastic splits append neutrals
+    si = ctx.species_add(32 * AMU, QE, w_e, Ni + Ne // 2 + 1024)
+    se = ctx.species_add(ME, -QE, w_e, Ne + Ne // 2 + 1024)
+    c = 0.5 * L
+    ctx.load_box_thermal(sn, c, box, n_gas, 300.0, seed=21, mode=0)
+    ctx.load_box_thermal(se, c, box, Ne * w_e / vol, 10.0 * EV_K, seed=22, mode=0)
+    ctx.load_box_thermal(si, c, box, Ni * w_e / vol, 300.0, seed=23, mode=0)
+    t = np.loadtxt(SIGMA_TABLE)
+    mid = ctx.mcc_create(sn, si, se, 12.07 * QE, t[:, 0], t[:, 1])
+    step = 0
+    for _ in range(args.dsmc_warmup):  # also brings W sigma v_max to its working value
+        ctx.mcc_apply(mid, dt, 7, step)
+        step += 1
+    K = args.dsmc_steps
+    n0 = ctx.count(sn) + ctx.count(se)
+    ctx.sync()
+    t0 = time.perf_counter()
+    tot = {"collisions": 0, "ionizations": 0, "groups": 0}
+    for _ in range(K):
+        st = ctx.mcc_apply(mid, dt, 7, step)
+        step += 1
+        for k in tot:
+            tot[k] += st[k]
+    ctx.sync()
+    T = time.perf_counter() - t0
+    n1 = ctx.count(sn) + ctx.count(se)
+    # one more step with HIP events around every launch
+    ctx.profile(True)
+    ctx.profile_reset()
+    ncount = ctx.count(sn)
+    st = ctx.mcc_apply(mid, dt, 7, step)
+    ctx.sync()
+    prof = ctx.profile_read()
+    ctx.profile(False)
+    ncells = (nxy - 1) * (nxy - 1) * (nz - 1)
+    mcc_ms = prof["mcc"][0]
+    # k_mcc bytes: both species' cell offsets (8 B per cell), the neutral and
+    # electron of every NTC group (w and v: 32 B), per accepted collision the
+    # electron velocity and neutral weight written back (16 B) and one 32-B
+    # birth (split neutral or ion), per ionization one more (the electron)
+    alg = 8.0 * ncells + 32.0 * st["groups"] + 48.0 * st["collisions"] + 32.0 * st["ionizations"]
+    ach = alg / (mcc_ms * 1e-3) / 1e9
+    sc_ms, sc_n = prof["scatter"]
+    sort_alg = 64.0 * ncount  # the neutrals' scatter dominates: 32 B read + 32 B written per particle
+    out = {
+        "metric": "DSMC particles/s (neutrals + electrons through pic_mcc_apply: sort + pair selection "
+                  "+ collisions)",
+        "value": (n0 + n1) / 2.0 * K / T, "unit": "DSMC particles/s", "steps": K, "warmup": args.dsmc_warmup,
+        "ms_per_step": T / K * 1e3,
+        "config": {"workload": "DSMC ionization, discharge cell densities: %dx%dx%d nodes, O2 10 Pa / "
+                               "300 K, e- 10 eV, %d + %d + %d particles, dt %.1e s, 64-point sigma_el "
+                               "table" % (nz, nxy, nxy, Nn, Ne, Ni, dt),
+                   "cells": ncells},
+        "particles_start": n0, "particles_end": n1,
+        "per_step": {k: v / K for k, v in tot.items()},
+        "stages": {k: {"ms": round(v[0], 4), "launches": v[1]} for k, v in prof.items() if v[1]},
+        "roofline": {"bound": "hbm", "kernel": "k_mcc", "achieved": ach, "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": ach / HBM_PEAK_GBS, "traffic": None,
+                     "alg_bytes_per_launch": alg, "avg_launch_ms": mcc_ms},
+        "roofline_sort": ({"bound": "hbm", "kernel": "k_scatter (neutrals)",
+                           "achieved": sort_alg / (sc_ms / sc_n * 1e-3) / 1e9, "peak": HBM_PEAK_GBS,
+                           "unit": "GB/s"} if sc_n else None),
+        "cpu_baseline": cpu,
+    }
+    ctx.close()
+    return out
+
+
 def blob_density(n, seed=12345):
     """six Gaussian blobs (widths 3 to 0.15 n nodes) plus white noise at a
     tenth of their amplitude, zero on the walls; C/m^3, node order i-fastest"""
@@ -154,6 +255,14 @@ def main():
                     choices=["auto", "sor_rb", "pcg_mixed", "pcg_fp64", "learned"])
     # each step's solve starts from 2 phi_n - phi_(n-1) ("extrapolate") or from phi_n ("previous")
     ap.add_argument("--warm-start", default="extrapolate", choices=["extrapolate", "previous"])
+    # the DSMC step of the discharge workload, reported beside the PIC line
+    ap.add_argument("--no-dsmc", action="store_true")
+    ap.add_argument("--dsmc-neutrals", type=int, default=40_000_000)
+    ap.add_argument("--dsmc-electrons", type=int, default=5_000_000)
+    ap.add_argument("--dsmc-ions", type=int, default=5_000_000)
+    ap.add_argument("--dsmc-steps", type=int, default=10)
+    ap.add_argument("--dsmc-warmup", type=int, default=3)
+    ap.add_argument("--dsmc-dt", type=float, default=2e-9)  # a few NTC groups per cell
     args = ap.parse_args()
     if args.copy_calibration:
         copy_calibration()
@@ -164,11 +273,20 @@ def main():
         return 2
 
     cpu = None
+    cpu_dsmc = None
     if not args.no_cpu_baseline:  # before the GPU is initialised
         threads = max(1, min(16, os.cpu_count() or 1))
         cpu = cpu_baseline(threads, args.cpu_particles, args.cpu_steps)
         print("cpu baseline: %.3e pushes/s (%s, %d threads)" % (cpu["value"], cpu["kind"],
                                                                  threads), file=sys.stderr)
+        if not args.no_dsmc:
+            # 32 x 32 x 128 cells (an eighth of the workload's) at its densities
+            f = 32 * 32 * 128 / (63 * 63 * 255)
+            cpu_dsmc = cpu_baseline_dsmc(33, 129, int(args.dsmc_neutrals * f), int(args.dsmc_electrons * f),
+                                         args.dsmc_warmup + 7, args.dsmc_dt)
+            if cpu_dsmc:
+                print("cpu baseline dsmc: %.3e particles/s (reference apply, 1 thread)" % cpu_dsmc["value"],
+                      file=sys.stderr)
 
     import pic_gpu as pg
 
@@ -379,6 +497,10 @@ def main():
         "cpu_baseline": cpu,
     }
     ctx.close()
+    if not args.no_dsmc:
+        line["dsmc"] = dsmc_bench(pg, args, cpu_dsmc)
+        print("dsmc: %.3e particles/s, %.3f ms per step" % (line["dsmc"]["value"], line["dsmc"]["ms_per_step"]),
+              file=sys.stderr)
     print(json.dumps(line))
     return 0
 

@@ -165,6 +165,7 @@ extern "C" void pic_destroy(pic_ctx* ctx) {
         (void)hipFree(m.sig);
         (void)hipFree(m.d_wsv);
         (void)hipFree(m.d_stats);
+        (void)hipFree(m.d_part);
     }
     pic_unet_free(ctx);
     (void)hipFree(ctx->phi);

@@ -160,6 +160,9 @@ struct Mcc {
     int nsig = 0;
     double* d_wsv = nullptr;  // [0] current max, [1] step max (as bits for atomicMax)
     unsigned long long* d_stats = nullptr;  // collisions, ionizations, groups, overflow
+    // per-workgroup partials of d_stats and of the step maximum (as bits),
+    // striped so that the workgroups' atomics spread over NSTRIPE lines
+    unsigned long long* d_part = nullptr;
     double wsv_host = 0;
 };
 

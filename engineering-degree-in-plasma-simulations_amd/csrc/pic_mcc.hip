@@ -122,27 +122,106 @@ __device__ inline bool collide(const MccArgs& a, Draws& dr, const double* vn, do
     return false;
 }
 
-__device__ inline void append(Sp& s, uint32_t cell, float fx, float fy, float fz, const double* v,
-                              double w, unsigned long long* overflow) {
-    uint32_t idx = atomicAdd(&s.d_n[0], 1u);
+// Births are staged in LDS per workgroup (slots from LDS atomics, which
+// return within the CU) and appended to the species' tails once per
+// workgroup at the end: a returning global atomic per birth left the wave
+// waiting on a device-wide round trip up to three times per NTC group.
+// Staging slots per species (neutral splits dominate); a full buffer falls
+// back to a global slot for that birth.
+struct Birth {
+    uint32_t cell;
+    float f[3], v[3], w;
+};
+constexpr int BIRTH_CAP_N = 512, BIRTH_CAP_I = 64, BIRTH_CAP_E = 96;
+struct BirthStage {
+    Birth b[BIRTH_CAP_N + BIRTH_CAP_I + BIRTH_CAP_E];
+    uint32_t count[3], base[3];
+};
+__device__ inline int birth_cap(int k) { return k == 0 ? BIRTH_CAP_N : (k == 1 ? BIRTH_CAP_I : BIRTH_CAP_E); }
+__device__ inline int birth_off(int k) { return k == 0 ? 0 : (k == 1 ? BIRTH_CAP_N : BIRTH_CAP_N + BIRTH_CAP_I); }
+
+__device__ inline void store_birth(Sp& s, uint32_t idx, const Birth& b) {
+    s.b.cell[idx] = b.cell;
+    s.b.f[0][idx] = b.f[0];
+    s.b.f[1][idx] = b.f[1];
+    s.b.f[2][idx] = b.f[2];
+    s.b.v[0][idx] = b.v[0];
+    s.b.v[1][idx] = b.v[1];
+    s.b.v[2][idx] = b.v[2];
+    s.b.w[idx] = b.w;
+}
+
+// species k (0 neutrals, 1 ions, 2 electrons) gains a particle
+__device__ inline void append(BirthStage& st, int k, Sp& s, uint32_t cell, float fx, float fy, float fz,
+                              const double* v, double w, unsigned long long* overflow) {
+    Birth b;
+    b.cell = cell;
+    b.f[0] = fx;
+    b.f[1] = fy;
+    b.f[2] = fz;
+    b.v[0] = (float)v[0];
+    b.v[1] = (float)v[1];
+    b.v[2] = (float)v[2];
+    b.w = (float)w;
+    const uint32_t i = atomicAdd(&st.count[k], 1u);
+    if (i < (uint32_t)birth_cap(k)) {
+        st.b[birth_off(k) + i] = b;
+        return;
+    }
+    const uint32_t idx = atomicAdd(&s.d_n[0], 1u);  // staging full
     if (idx >= s.cap) {
         atomicAdd(overflow, 1ull);
         return;
     }
-    s.b.cell[idx] = cell;
-    s.b.f[0][idx] = fx;
-    s.b.f[1][idx] = fy;
-    s.b.f[2][idx] = fz;
-    s.b.v[0][idx] = (float)v[0];
-    s.b.v[1][idx] = (float)v[1];
-    s.b.v[2][idx] = (float)v[2];
-    s.b.w[idx] = (float)w;
+    store_birth(s, idx, b);
 }
 
+// the staged births of the workgroup to the species' tails (every thread
+// of the workgroup calls this)
+__device__ inline void flush_births(BirthStage& st, Sp& N, Sp& I, Sp& E, unsigned long long* overflow) {
+    __syncthreads();
+    if (threadIdx.x < 3) {
+        const int k = threadIdx.x;
+        const uint32_t n = min(st.count[k], (uint32_t)birth_cap(k));
+        Sp& s = k == 0 ? N : (k == 1 ? I : E);
+        st.base[k] = n ? atomicAdd(&s.d_n[0], n) : 0u;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+        Sp& s = k == 0 ? N : (k == 1 ? I : E);
+        const uint32_t n = min(st.count[k], (uint32_t)birth_cap(k));
+        for (uint32_t i = threadIdx.x; i < n; i += BLOCK) {
+            const uint32_t idx = st.base[k] + i;
+            if (idx >= s.cap) {
+                atomicAdd(overflow, 1ull);
+                continue;
+            }
+            store_birth(s, idx, st.b[birth_off(k) + i]);
+        }
+    }
+}
+
+// cross-section tables up to this many points are searched in LDS (a
+// table's binary search otherwise costs a chain of dependent global loads
+// per NTC group)
+constexpr int SIG_LDS_MAX = 256;
+
+template <bool LDS_TABLE>
 __global__ void __launch_bounds__(BLOCK)
     k_mcc(GridC g, MccArgs a, Sp N, const uint32_t* __restrict__ offN, Sp I, Sp E,
           const uint32_t* __restrict__ offE, const double* __restrict__ table,
-          double* __restrict__ wsv, unsigned long long* __restrict__ stats) {
+          const double* __restrict__ wsv, unsigned long long* __restrict__ part) {
+    // this workgroup's stripe: collisions, ionizations, groups, overflow, step maximum
+    unsigned long long* stats = part + (blockIdx.x % NSTRIPE) * SSTRIDE;
+    __shared__ double stab[LDS_TABLE ? 2 * SIG_LDS_MAX : 1];
+    __shared__ BirthStage births;
+    if (threadIdx.x < 3) births.count[threadIdx.x] = 0;
+    if (LDS_TABLE) {
+        for (int q = threadIdx.x; q < 2 * a.nsig; q += BLOCK) stab[q] = table[q];
+        table = stab;
+    }
+    __syncthreads();
     int c = blockIdx.x * BLOCK + threadIdx.x;
     double wsv_tmp = 0;
     unsigned long long ncoll = 0, niz = 0, ngr = 0;
@@ -164,12 +243,20 @@ __global__ void __launch_bounds__(BLOCK)
                 int ns = (int)(dr(0) * (double)Nn);
                 int es = (int)(dr(1) * (double)Ne);
                 uint32_t pn = n0 + (uint32_t)ns, pe = e0 + (uint32_t)es;
+                // both particles' weights and velocities requested together
                 double wn = N.b.w[pn], we = E.b.w[pe];
-                if (wn == 0.0) continue;  // neutral removed earlier in this cell
                 double vn[3] = {N.b.v[0][pn], N.b.v[1][pn], N.b.v[2][pn]};
                 double ve[3] = {E.b.v[0][pe], E.b.v[1][pe], E.b.v[2][pe]};
+                // the neutral's cell and offsets (its births' position) too
+                const uint32_t ncell = N.b.cell[pn];
+                const float nfx = N.b.f[0][pn], nfy = N.b.f[1][pn], nfz = N.b.f[2][pn];
+                // (an empty asm that takes them all: otherwise the compiler
+                // tests wn first and waits for memory more than once)
+                asm volatile("" ::"v"(wn), "v"(we), "v"(vn[0]), "v"(vn[1]), "v"(vn[2]), "v"(ve[0]), "v"(ve[1]),
+                             "v"(ve[2]), "v"(ncell), "v"(nfx), "v"(nfy), "v"(nfz));
                 double vr[3] = {vn[0] - ve[0], vn[1] - ve[1], vn[2] - ve[2]};
                 double v_rel = sqrt(vr[0] * vr[0] + vr[1] * vr[1] + vr[2] * vr[2]);
+                if (wn == 0.0) continue;  // neutral removed earlier in this cell
                 double E_rel = a.E_pre_rel * v_rel * v_rel;
                 double sigma = sigma_el(tE, tV, a.nsig, E_rel);
                 double Wg = wn > we ? wn : we;
@@ -178,28 +265,26 @@ __global__ void __launch_bounds__(BLOCK)
                 double P = wsv_g / wmax;
                 if (!(dr(2) < P)) continue;
                 ncoll++;
-                uint32_t ncell = N.b.cell[pn];
-                float nfx = N.b.f[0][pn], nfy = N.b.f[1][pn], nfz = N.b.f[2][pn];
                 double vnew[3] = {0, 0, 0};
                 if (wn > we) {  // Interactions.cpp:684-703
                     N.b.w[pn] = (float)(wn - we);
                     bool iz = collide(a, dr, vn, ve, vnew, sigma);
                     if (iz) {
                         niz++;
-                        append(I, ncell, nfx, nfy, nfz, vn, we, &stats[3]);
-                        append(E, ncell, nfx, nfy, nfz, vnew, we, &stats[3]);
+                        append(births, 1, I, ncell, nfx, nfy, nfz, vn, we, &stats[3]);
+                        append(births, 2, E, ncell, nfx, nfy, nfz, vnew, we, &stats[3]);
                     } else {
-                        append(N, ncell, nfx, nfy, nfz, vn, we, &stats[3]);
+                        append(births, 0, N, ncell, nfx, nfy, nfz, vn, we, &stats[3]);
                     }
                 } else if (wn < we) {  // Interactions.cpp:705-726, defined semantics
                     E.b.w[pe] = (float)wn;
-                    append(E, E.b.cell[pe], E.b.f[0][pe], E.b.f[1][pe], E.b.f[2][pe], ve, we - wn,
+                    append(births, 2, E, E.b.cell[pe], E.b.f[0][pe], E.b.f[1][pe], E.b.f[2][pe], ve, we - wn,
                            &stats[3]);
                     bool iz = collide(a, dr, vn, ve, vnew, sigma);
                     if (iz) {
                         niz++;
-                        append(I, ncell, nfx, nfy, nfz, vn, wn, &stats[3]);
-                        append(E, ncell, nfx, nfy, nfz, vnew, wn, &stats[3]);
+                        append(births, 1, I, ncell, nfx, nfy, nfz, vn, wn, &stats[3]);
+                        append(births, 2, E, ncell, nfx, nfy, nfz, vnew, wn, &stats[3]);
                         N.b.w[pn] = 0.0f;
                         N.b.cell[pn] = DEAD;
                     }
@@ -210,6 +295,7 @@ __global__ void __launch_bounds__(BLOCK)
             }
         }
     }
+    flush_births(births, N, I, E, &stats[3]);
     // block reduction of the counters and the step maximum
     __shared__ double smax[BLOCK / 64];
     __shared__ unsigned long long scnt[BLOCK / 64][3];
@@ -236,8 +322,8 @@ __global__ void __launch_bounds__(BLOCK)
             t1 += scnt[w][1];
             t2 += scnt[w][2];
         }
-        if (t2) {
-            atomicMax((unsigned long long*)&wsv[1], (unsigned long long)__double_as_longlong(m));
+        if (t2) {  // non-negative doubles order as their bit patterns
+            atomicMax(&stats[4], (unsigned long long)__double_as_longlong(m));
             atomicAdd(&stats[0], t0);
             atomicAdd(&stats[1], t1);
             atomicAdd(&stats[2], t2);
@@ -245,11 +331,19 @@ __global__ void __launch_bounds__(BLOCK)
     }
 }
 
+// the stripes summed into stats (and the step maximum into wsv[1]);
 // W_sigma_v_rel_max <- step maximum when any collision happened
 // (Interactions.cpp:751-756)
-__global__ void k_mcc_finish(double* wsv, const unsigned long long* stats, uint32_t* caps_n,
-                             uint32_t cap_n, uint32_t* caps_i, uint32_t cap_i, uint32_t* caps_e,
-                             uint32_t cap_e) {
+__global__ void k_mcc_finish(double* wsv, const unsigned long long* part, unsigned long long* stats,
+                             uint32_t* caps_n, uint32_t cap_n, uint32_t* caps_i, uint32_t cap_i,
+                             uint32_t* caps_e, uint32_t cap_e) {
+    unsigned long long t[4] = {0, 0, 0, 0}, mb = 0;
+    for (int s = 0; s < NSTRIPE; s++) {
+        for (int q = 0; q < 4; q++) t[q] += part[s * SSTRIDE + q];
+        mb = part[s * SSTRIDE + 4] > mb ? part[s * SSTRIDE + 4] : mb;
+    }
+    for (int q = 0; q < 4; q++) stats[q] = t[q];
+    wsv[1] = __longlong_as_double((long long)mb);
     if (stats[0] > 0) wsv[0] = wsv[1];
     if (caps_n[0] > cap_n) caps_n[0] = cap_n;
     if (caps_i[0] > cap_i) caps_i[0] = cap_i;
@@ -283,6 +377,7 @@ extern "C" int pic_mcc_create(pic_ctx* ctx, int sn, int si, int se, const pic_mc
     PIC_HIP(ctx, hipMalloc(&m.sig, 2 * (size_t)nsig * sizeof(double)));
     PIC_HIP(ctx, hipMalloc(&m.d_wsv, 2 * sizeof(double)));
     PIC_HIP(ctx, hipMalloc(&m.d_stats, 4 * sizeof(unsigned long long)));
+    PIC_HIP(ctx, hipMalloc(&m.d_part, (size_t)NSTRIPE * SSTRIDE * sizeof(unsigned long long)));
     PIC_HIP(ctx, hipMemcpyAsync(m.sig, sig_E, nsig * sizeof(double), hipMemcpyHostToDevice,
                                 ctx->stream));
     PIC_HIP(ctx, hipMemcpyAsync(m.sig + nsig, sig_v, nsig * sizeof(double), hipMemcpyHostToDevice,
@@ -328,14 +423,18 @@ extern "C" int pic_mcc_apply(pic_ctx* ctx, int id, double dt, uint32_t seed, uin
     Sp sn{N.buf[N.cur], N.d_n, (uint32_t)N.cap};
     Sp si{I.buf[I.cur], I.d_n, (uint32_t)I.cap};
     Sp se{E.buf[E.cur], E.d_n, (uint32_t)E.cap};
-    PIC_HIP(ctx, hipMemsetAsync(m.d_stats, 0, 4 * sizeof(unsigned long long), ctx->stream));
-    PIC_HIP(ctx, hipMemsetAsync(m.d_wsv + 1, 0, sizeof(double), ctx->stream));
+    PIC_HIP(ctx, hipMemsetAsync(m.d_part, 0, (size_t)NSTRIPE * SSTRIDE * sizeof(unsigned long long),
+                                ctx->stream));
     {
         PicProfScope ps(ctx, PIC_K_MCC);
-        k_mcc<<<grid_for((size_t)g.ncells, BLOCK), BLOCK, 0, ctx->stream>>>(
-            g, a, sn, N.off, si, se, E.off, m.sig, m.d_wsv, m.d_stats);
-        k_mcc_finish<<<1, 1, 0, ctx->stream>>>(m.d_wsv, m.d_stats, N.d_n, (uint32_t)N.cap, I.d_n,
-                                               (uint32_t)I.cap, E.d_n, (uint32_t)E.cap);
+        if (m.nsig <= SIG_LDS_MAX)
+            k_mcc<true><<<grid_for((size_t)g.ncells, BLOCK), BLOCK, 0, ctx->stream>>>(
+                g, a, sn, N.off, si, se, E.off, m.sig, m.d_wsv, m.d_part);
+        else
+            k_mcc<false><<<grid_for((size_t)g.ncells, BLOCK), BLOCK, 0, ctx->stream>>>(
+                g, a, sn, N.off, si, se, E.off, m.sig, m.d_wsv, m.d_part);
+        k_mcc_finish<<<1, 1, 0, ctx->stream>>>(m.d_wsv, m.d_part, m.d_stats, N.d_n, (uint32_t)N.cap,
+                                               I.d_n, (uint32_t)I.cap, E.d_n, (uint32_t)E.cap);
     }
     PIC_HIP(ctx, hipGetLastError());
     unsigned long long hs[4];

@@ -7,7 +7,8 @@
 //                  manifest.txt) that pin the CPU oracle, and
 //   bench ...      time the reference's multithreaded CPU path (threaded
 //                  electron push, Species.cpp:258-355) as the reported CPU
-//                  baseline.
+//                  baseline, and
+//   bench_mcc ...  time its MC_MEX_Ionization::apply (the DSMC step).
 // Nothing here is reference source; it only calls the reference's API.
 #include <algorithm>
 #include <chrono>
@@ -328,6 +329,50 @@ static void bench(int n, long np, int steps, int threads, int do_solve) {
     delete ps;
 }
 
+// reported CPU baseline of the DSMC step: the reference's
+// MC_MEX_Ionization::apply (its per-call cell sort and the serial NTC pair
+// loop, Interactions.cpp:567-762) on an O2 / electron box of the discharge
+// configuration's cell densities; medians over calls.
+static void bench_mcc(int nxy, int nz, long n_neu, long n_ele, int steps, double dt,
+                      const std::string& table_path) {
+    Config& cfg = Config::getInstance();
+    cfg.setMULTITHREADING(false);
+    const double dx = 1e-4;
+    type_calc3 x0(0, 0, 0), xm(dx * (nxy - 1), dx * (nxy - 1), dx * (nz - 1));
+    World world(nxy, nxy, nz, x0, xm);
+    world.setTime(dt, steps);
+    type_calc3 L = world.getL(), box = L * 0.98;
+    const double vol = box[0] * box[1] * box[2];
+    const double n_gas = 10.0 / (Const::k * 300.0);  // O2 at 10 Pa, 300 K
+    const double w_n = n_gas * vol / (double)n_neu, w_e = w_n / 1000.0;
+    Species neu("O2", 32 * Const::amu, 0, world, w_n, 12.07 * Const::q_e);
+    Species ion("O2+", 32 * Const::amu, Const::q_e, world, w_e);
+    Species ele("e-", Const::m_e, -Const::q_e, world, w_e);
+    rnd = Rnd(31u);
+    neu.loadParticleBoxThermal(world.getXc(), box, n_gas, 300.0);
+    ele.loadParticleBoxThermal(world.getXc(), box, (double)n_ele * w_e / vol, 10.0 * 11604.52);
+    MccProbe mcc(neu, ion, ele, world, table_path);
+    // per call: the neutrals and electrons it starts with over its time
+    // (elastic splits grow the neutral count from call to call)
+    std::vector<double> t, rate;
+    for (int s = 0; s < steps; s++) {
+        const double np = (double)(neu.getNumParticles() + ele.getNumParticles());
+        auto a = std::chrono::steady_clock::now();
+        mcc.apply(dt);
+        auto b = std::chrono::steady_clock::now();
+        t.push_back(std::chrono::duration<double>(b - a).count());
+        rate.push_back(np / t.back());
+    }
+    auto med = [](std::vector<double> v) {
+        std::sort(v.begin(), v.end());
+        return v[v.size() / 2];
+    };
+    printf("{\"cells\": %d, \"neutrals\": %zu, \"electrons\": %zu, \"ions\": %zu, \"steps\": %d, "
+           "\"apply_s\": %.6e, \"particles_per_s\": %.6e, \"wsv_max\": %.6e}\n",
+           (nxy - 1) * (nxy - 1) * (nz - 1), neu.getNumParticles(), ele.getNumParticles(),
+           ion.getNumParticles(), steps, med(t), med(rate), mcc.wsv());
+}
+
 int main(int argc, char** argv) {
     if (argc >= 4 && std::string(argv[1]) == "golden") {
         golden(argv[2], argv[3]);
@@ -337,8 +382,15 @@ int main(int argc, char** argv) {
         bench(atoi(argv[2]), atol(argv[3]), atoi(argv[4]), atoi(argv[5]), atoi(argv[6]));
         return 0;
     }
+    if (argc >= 9 && std::string(argv[1]) == "bench_mcc") {
+        bench_mcc(atoi(argv[2]), atoi(argv[3]), atol(argv[4]), atol(argv[5]), atoi(argv[6]), atof(argv[7]),
+                  argv[8]);
+        return 0;
+    }
     fprintf(stderr,
             "usage: ref_harness golden <outdir> <sigma_el_table>\n"
-            "       ref_harness bench <nodes_per_axis> <particles> <steps> <threads> <solve0|1>\n");
+            "       ref_harness bench <nodes_per_axis> <particles> <steps> <threads> <solve0|1>\n"
+            "       ref_harness bench_mcc <nodes_xy> <nodes_z> <neutrals> <electrons> <steps> <dt> "
+            "<sigma_el_table>\n");
     return 1;
 }
