@@ -257,6 +257,7 @@ def main():
     ap.add_argument("--warm-start", default="extrapolate", choices=["extrapolate", "previous"])
     # the DSMC step of the discharge workload, reported beside the PIC line
     ap.add_argument("--no-dsmc", action="store_true")
+    ap.add_argument("--only-dsmc", action="store_true", help="print the DSMC section alone")
     ap.add_argument("--dsmc-neutrals", type=int, default=40_000_000)
     ap.add_argument("--dsmc-electrons", type=int, default=5_000_000)
     ap.add_argument("--dsmc-ions", type=int, default=5_000_000)
@@ -276,9 +277,10 @@ def main():
     cpu_dsmc = None
     if not args.no_cpu_baseline:  # before the GPU is initialised
         threads = max(1, min(16, os.cpu_count() or 1))
-        cpu = cpu_baseline(threads, args.cpu_particles, args.cpu_steps)
-        print("cpu baseline: %.3e pushes/s (%s, %d threads)" % (cpu["value"], cpu["kind"],
-                                                                 threads), file=sys.stderr)
+        if not args.only_dsmc:
+            cpu = cpu_baseline(threads, args.cpu_particles, args.cpu_steps)
+            print("cpu baseline: %.3e pushes/s (%s, %d threads)" % (cpu["value"], cpu["kind"],
+                                                                     threads), file=sys.stderr)
         if not args.no_dsmc:
             # 32 x 32 x 128 cells (an eighth of the workload's) at its densities
             f = 32 * 32 * 128 / (63 * 63 * 255)
@@ -289,6 +291,13 @@ def main():
                       file=sys.stderr)
 
     import pic_gpu as pg
+
+    if args.only_dsmc:
+        d = dsmc_bench(pg, args, cpu_dsmc)
+        d["n_gpus"] = 1
+        d["higher_is_better"] = True
+        print(json.dumps(d))
+        return 0
 
     n = args.nodes
     dx = 1e-2
