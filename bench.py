@@ -89,7 +89,11 @@ def cpu_baseline(threads, particles, steps):
         per_particle = r["push_s_per_particle"] + r["deposit_s_per_particle"]
         return {"value": 1.0 / per_particle, "unit": "particle pushes/s", "cores": threads,
                 "kind": "reference", "sample": sample,
-                "push_only_pushes_per_s": r["pushes_per_s"], "wall_s": round(time.time() - t0, 1)}
+                "push_only_pushes_per_s": r["pushes_per_s"], "wall_s": round(time.time() - t0, 1),
+                # serial reference loops on the same electrons (the secondary rows)
+                "load_particles_per_s": 1.0 / r["load_s_per_particle"],
+                "moments_particles_per_s": 1.0 / r["moments_s_per_particle"],
+                "diag_particles_per_s": 1.0 / r["diag_s_per_particle"]}
     # the oracle's v3-structured threaded push (reference binary not built)
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as O
@@ -171,7 +175,7 @@ def dsmc_bench(pg, args, cpu):
     # one more step with HIP events around every launch
     ctx.profile(True)
     ctx.profile_reset()
-    ncount = ctx.count(sn)
+    nsorted = ctx.count(sn) + ctx.count(se)  # both species are counting-sorted in this step
     st = ctx.mcc_apply(mid, dt, 7, step)
     ctx.sync()
     prof = ctx.profile_read()
@@ -184,8 +188,21 @@ def dsmc_bench(pg, args, cpu):
     # birth (split neutral or ion), per ionization one more (the electron)
     alg = 8.0 * ncells + 32.0 * st["groups"] + 48.0 * st["collisions"] + 32.0 * st["ionizations"]
     ach = alg / (mcc_ms * 1e-3) / 1e9
+    roof_mcc = {"bound": "hbm", "kernel": "k_mcc", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": ach / HBM_PEAK_GBS, "traffic": None, "alg_bytes_per_launch": alg,
+                "avg_launch_ms": mcc_ms}
+    # the counting sort's scatter (neutrals and electrons: 32 B read and 32 B
+    # written per particle) against k_mcc: the larger is the step's dominant kernel
     sc_ms, sc_n = prof["scatter"]
-    sort_alg = 64.0 * ncount  # the neutrals' scatter dominates: 32 B read + 32 B written per particle
+    roof_sort = None
+    if sc_n:
+        sa = 64.0 * nsorted / sc_n
+        sach = sa / (sc_ms / sc_n * 1e-3) / 1e9
+        roof_sort = {"bound": "hbm", "kernel": "k_scatter (counting sort of neutrals and electrons)",
+                     "achieved": sach, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": sach / HBM_PEAK_GBS,
+                     "traffic": None, "alg_bytes_per_launch": sa, "avg_launch_ms": sc_ms / sc_n,
+                     "launches_per_step": sc_n}
+    dominant_sort = roof_sort is not None and sc_ms > mcc_ms
     out = {
         "metric": "DSMC particles/s (neutrals + electrons through pic_mcc_apply: sort + pair selection "
                   "+ collisions)",
@@ -198,15 +215,132 @@ def dsmc_bench(pg, args, cpu):
         "particles_start": n0, "particles_end": n1,
         "per_step": {k: v / K for k, v in tot.items()},
         "stages": {k: {"ms": round(v[0], 4), "launches": v[1]} for k, v in prof.items() if v[1]},
-        "roofline": {"bound": "hbm", "kernel": "k_mcc", "achieved": ach, "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": ach / HBM_PEAK_GBS, "traffic": None,
-                     "alg_bytes_per_launch": alg, "avg_launch_ms": mcc_ms},
-        "roofline_sort": ({"bound": "hbm", "kernel": "k_scatter (neutrals)",
-                           "achieved": sort_alg / (sc_ms / sc_n * 1e-3) / 1e9, "peak": HBM_PEAK_GBS,
-                           "unit": "GB/s"} if sc_n else None),
+        "roofline": roof_sort if dominant_sort else roof_mcc,
+        "roofline_other": roof_mcc if dominant_sort else roof_sort,
         "cpu_baseline": cpu,
     }
     ctx.close()
+    return out
+
+
+def cpu_baseline_newton(n, threads):
+    """The reference's Newton-Boltzmann PCG solve (solveNRPCG) on an n^3 box."""
+    harness = os.path.join(ROOT, "oracle", "_ref", "ref_harness")
+    if not os.path.exists(harness):
+        return None
+    out = subprocess.run([harness, "bench_newton", str(n), str(threads), "1e-6"], capture_output=True,
+                         text=True, timeout=600)
+    r = json.loads([ln for ln in out.stdout.splitlines() if ln.startswith("{")][-1])
+    return {"value": r["solve_s"], "unit": "s per solve", "cores": threads, "kind": "reference",
+            "sample": "%d^3 nodes, Dirichlet walls, uniform ion charge 1.5 n0 q, Boltzmann electrons "
+                      "n0 = 1e12, Te0 = 2 eV, reference criterion 1e-6" % n,
+            "converged": bool(r["converged"])}
+
+
+def timed(ctx, fn, reps=3):
+    """median wall time of fn() between stream synchronisations (s)"""
+    ts = []
+    for _ in range(reps):
+        ctx.sync()
+        t0 = time.perf_counter()
+        fn()
+        ctx.sync()
+        ts.append(time.perf_counter() - t0)
+    return float(np.median(ts))
+
+
+def newton_solve(pg, ctx, n0=1e12, Te0=2.0):
+    """Newton-Boltzmann solve of the uniform-ion problem from phi = 0 (as the
+    reference harness's): returns (seconds, stats)"""
+    ctx.set_field(pg.FIELD_RHO, np.full(ctx.nn, QE * 1.5 * n0))
+    ctx.set_field(pg.FIELD_PHI, np.zeros(ctx.nn))
+    ctx.sync()
+    t0 = time.perf_counter()
+    st = ctx.poisson(method=pg.SOLVER_PCG, tol=1e-6, tol_kind=pg.TOL_REF_RMS, max_it=20000, n0=n0, Te0=Te0)
+    ctx.sync()
+    return time.perf_counter() - t0, st
+
+
+def extras_bench(pg, ctx, args, se, N, n, cpu, cpu_newton, x0, xm):
+    """The secondary rows on the main workload's context: density moments of
+    one species (N particles), the diagnostic sums, the loader, and the
+    Newton-Boltzmann solve (here at n^3 and at the CPU sample's 64^3)."""
+    out = {}
+    nn = ctx.nn
+
+    def mom():
+        ctx.clear_samples(se)
+        ctx.deposit(se, pg.DEPOSIT_MOMENTS)
+
+    t = timed(ctx, mom)
+    # particle: cell word, offsets, velocity, weight (32 B); nodes: 7 FP64 channels written
+    alg = 32.0 * N + 56.0 * nn
+    out["moments"] = {"row": "a21 sampleMoments", "particles": N, "seconds": t, "particles_per_s": N / t,
+                      "roofline": {"bound": "hbm", "achieved": alg / t / 1e9, "peak": HBM_PEAK_GBS,
+                                   "unit": "GB/s", "frac": alg / t / 1e9 / HBM_PEAK_GBS,
+                                   "alg_bytes_per_call": alg},
+                      "cpu_baseline": ({"value": cpu["moments_particles_per_s"], "unit": "particles/s",
+                                        "cores": 1, "kind": "reference", "sample": cpu["sample"]}
+                                       if cpu and "moments_particles_per_s" in cpu else None)}
+
+    def diag():
+        ctx.diag_sums(se)
+        ctx.field_energy()
+
+    t = timed(ctx, diag)
+    alg = 32.0 * N + 16.0 * nn  # every particle read; E (float4) per node
+    out["diagnostics"] = {"row": "a22 micro count, momentum, KE, PE, field energy", "particles": N,
+                          "seconds": t, "particles_per_s": N / t,
+                          "roofline": {"bound": "hbm", "achieved": alg / t / 1e9, "peak": HBM_PEAK_GBS,
+                                       "unit": "GB/s", "frac": alg / t / 1e9 / HBM_PEAK_GBS,
+                                       "alg_bytes_per_call": alg},
+                          "note": "includes the host reads of the sums",
+                          "cpu_baseline": ({"value": cpu["diag_particles_per_s"], "unit": "particles/s",
+                                            "cores": 1, "kind": "reference", "sample": cpu["sample"]}
+                                           if cpu and "diag_particles_per_s" in cpu else None)}
+    # loader: a fresh species of N electrons, the reference's sampler (mode 0)
+    sl = ctx.species_add(ME, -QE, 1.0, N + 1024)
+    L = np.asarray(xm, float) - np.asarray(x0, float)
+    vol = float(np.prod(0.98 * L))
+    ctx.sync()
+    t0 = time.perf_counter()
+    got = ctx.load_box_thermal(sl, np.asarray(x0, float) + 0.5 * L, 0.98 * L, N * 1.0 / vol, 2.0 * EV_K,
+                               seed=5, mode=0)
+    ctx.sync()
+    t = time.perf_counter() - t0
+    alg = 32.0 * got
+    out["loader"] = {"row": "a23 loadParticleBoxThermal", "particles": got, "seconds": t,
+                     "particles_per_s": got / t,
+                     "roofline": {"bound": "hbm", "achieved": alg / t / 1e9, "peak": HBM_PEAK_GBS,
+                                  "unit": "GB/s", "frac": alg / t / 1e9 / HBM_PEAK_GBS, "alg_bytes_per_call": alg},
+                     "note": "wall time of the call, including its host-side checks",
+                     "cpu_baseline": ({"value": cpu["load_particles_per_s"], "unit": "particles/s", "cores": 1,
+                                       "kind": "reference", "sample": cpu["sample"]}
+                                      if cpu and "load_particles_per_s" in cpu else None)}
+    # Newton-Boltzmann (FP64 outer Newton, FP64 Jacobi-PCG inner): at n^3 here
+    phi_keep, rho_keep = ctx.get_field(pg.FIELD_PHI), ctx.get_field(pg.FIELD_RHO)
+    t, st = newton_solve(pg, ctx)
+    its = st["iterations"]
+    alg = 88.0 * nn * its  # the two fused PCG kernels per inner iteration (FP64)
+    res = {"row": "a11 solveNRPCG", "grid": "%d^3" % n, "seconds": t, "newton_iterations": st["newton_iterations"],
+           "pcg_iterations": its, "converged": st["converged"],
+           "roofline": {"bound": "hbm", "kernel": "k_pcg1 + k_pcg2", "achieved": alg / t / 1e9,
+                        "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": alg / t / 1e9 / HBM_PEAK_GBS,
+                        "note": "whole solve time, cache-resident vectors (16.8 MB each)"}}
+    ctx.set_field(pg.FIELD_RHO, rho_keep)
+    ctx.set_field(pg.FIELD_PHI, phi_keep)
+    # the same problem at the CPU sample's size
+    c64 = pg.Context(64, 64, 64, [0, 0, 0], [0.63, 0.63, 0.63])
+    c64.dirichlet_walls()
+    newton_solve(pg, c64)  # first call allocates the solver's vectors
+    t64, st64 = newton_solve(pg, c64)
+    c64.close()
+    res["at_64cubed"] = {"seconds": t64, "newton_iterations": st64["newton_iterations"],
+                         "pcg_iterations": st64["iterations"], "converged": st64["converged"]}
+    res["cpu_baseline"] = cpu_newton
+    if cpu_newton:
+        res["speedup_vs_cpu_at_64cubed"] = cpu_newton["value"] / t64
+    out["newton_boltzmann"] = res
     return out
 
 
@@ -257,6 +391,8 @@ def main():
     ap.add_argument("--warm-start", default="extrapolate", choices=["extrapolate", "previous"])
     # the DSMC step of the discharge workload, reported beside the PIC line
     ap.add_argument("--no-dsmc", action="store_true")
+    ap.add_argument("--no-extras", action="store_true", help="skip the secondary rows (moments, "
+                                                             "diagnostics, loader, Newton-Boltzmann)")
     ap.add_argument("--only-dsmc", action="store_true", help="print the DSMC section alone")
     ap.add_argument("--dsmc-neutrals", type=int, default=40_000_000)
     ap.add_argument("--dsmc-electrons", type=int, default=5_000_000)
@@ -275,6 +411,7 @@ def main():
 
     cpu = None
     cpu_dsmc = None
+    cpu_newton = None
     if not args.no_cpu_baseline:  # before the GPU is initialised
         threads = max(1, min(16, os.cpu_count() or 1))
         if not args.only_dsmc:
@@ -289,6 +426,8 @@ def main():
             if cpu_dsmc:
                 print("cpu baseline dsmc: %.3e particles/s (reference apply, 1 thread)" % cpu_dsmc["value"],
                       file=sys.stderr)
+        if not args.only_dsmc and not args.no_extras:
+            cpu_newton = cpu_baseline_newton(64, threads)
 
     import pic_gpu as pg
 
@@ -505,6 +644,10 @@ def main():
         "poisson": poisson,
         "cpu_baseline": cpu,
     }
+    if not args.no_extras:
+        line["extras"] = extras_bench(pg, ctx, args, se, N, n, cpu, cpu_newton, x0, xm)
+        print("extras: %s" % json.dumps({k: round(v.get("seconds", 0), 5) for k, v in line["extras"].items()}),
+              file=sys.stderr)
     ctx.close()
     if not args.no_dsmc:
         line["dsmc"] = dsmc_bench(pg, args, cpu_dsmc)

@@ -283,8 +283,11 @@ static void bench(int n, long np, int steps, int threads, int do_solve) {
     double vol = L[0] * L[1] * L[2];
     // 0.98 L: a particle within rounding of xm gets lc == n-1 and the
     // reference gather (Field.h:204-221) would read past the last node
+    auto l0 = std::chrono::steady_clock::now();
     ele.loadParticleBoxThermal(world.getXc(), L * 0.98, (double)np / (vol * 0.98 * 0.98 * 0.98),
                                2.0 * 11604.52);
+    auto l1 = std::chrono::steady_clock::now();
+    const double t_load = std::chrono::duration<double>(l1 - l0).count() / (double)ele.getNumParticles();
     // a smooth field so the gather does real work
     for (int i = 0; i < n; i++)
         for (int j = 0; j < n; j++)
@@ -321,11 +324,21 @@ static void bench(int n, long np, int steps, int threads, int do_solve) {
         std::sort(v.begin(), v.end());
         return v[v.size() / 2];
     };
+    // moments (n, nv, nuu, nvv, nww) and the diagnostic sums on the final state
+    const double npart = (double)ele.getNumParticles();
+    auto m0 = std::chrono::steady_clock::now();
+    ele.sampleMoments();
+    auto m1 = std::chrono::steady_clock::now();
+    volatile double sink = ele.getMicroCount() + ele.getKE() + ele.getMomentum()[0];
+    (void)sink;
+    auto m2 = std::chrono::steady_clock::now();
     printf("{\"n\": %d, \"particles\": %zu, \"steps\": %d, \"threads\": %d, "
            "\"push_s_per_particle\": %.6e, \"deposit_s_per_particle\": %.6e, "
-           "\"pushes_per_s\": %.6e, \"solve_s\": %.6e}\n",
+           "\"pushes_per_s\": %.6e, \"solve_s\": %.6e, \"load_s_per_particle\": %.6e, "
+           "\"moments_s_per_particle\": %.6e, \"diag_s_per_particle\": %.6e}\n",
            n, ele.getNumParticles(), steps, threads, med(t_push), med(t_den), 1.0 / med(t_push),
-           med(t_solve));
+           med(t_solve), t_load, std::chrono::duration<double>(m1 - m0).count() / npart,
+           std::chrono::duration<double>(m2 - m1).count() / npart);
     delete ps;
 }
 
@@ -373,6 +386,30 @@ static void bench_mcc(int nxy, int nz, long n_neu, long n_ele, int steps, double
            ion.getNumParticles(), steps, med(t), med(rate), mcc.wsv());
 }
 
+// reported CPU baseline of the Newton-Boltzmann solve (PotentialSolver::
+// solveNRPCG, PotentialSolver.cpp:178-241): n^3 box with Dirichlet walls, a
+// uniform ion charge density n0 q (1.5 n0), Boltzmann electrons n0, Te0 = 2 eV
+static void bench_newton(int n, int threads, double tol) {
+    Config& cfg = Config::getInstance();
+    cfg.setMULTITHREADING(threads > 1);
+    cfg.setNUM_THREADS(threads);
+    const double dx = 1e-2;
+    World world(n, n, n, type_calc3(0, 0, 0), type_calc3(dx * (n - 1), dx * (n - 1), dx * (n - 1)));
+    world.setTime(1e-10, 1);
+    world.dirichletBoundries();
+    const double n0 = 1e12;
+    for (int i = 0; i < n; i++)
+        for (int j = 0; j < n; j++)
+            for (int k = 0; k < n; k++) world.rho[i][j][k] = Const::q_e * 1.5 * n0;
+    PotentialSolver ps(world, 20000, tol, PCG);
+    ps.setReferenceValues(0.0, n0, 2.0);
+    auto a = std::chrono::steady_clock::now();
+    bool ok = ps.solve();
+    auto b = std::chrono::steady_clock::now();
+    printf("{\"n\": %d, \"threads\": %d, \"tol\": %.1e, \"converged\": %d, \"solve_s\": %.6e}\n", n,
+           threads, tol, (int)ok, std::chrono::duration<double>(b - a).count());
+}
+
 int main(int argc, char** argv) {
     if (argc >= 4 && std::string(argv[1]) == "golden") {
         golden(argv[2], argv[3]);
@@ -380,6 +417,10 @@ int main(int argc, char** argv) {
     }
     if (argc >= 7 && std::string(argv[1]) == "bench") {
         bench(atoi(argv[2]), atol(argv[3]), atoi(argv[4]), atoi(argv[5]), atoi(argv[6]));
+        return 0;
+    }
+    if (argc >= 5 && std::string(argv[1]) == "bench_newton") {
+        bench_newton(atoi(argv[2]), atoi(argv[3]), atof(argv[4]));
         return 0;
     }
     if (argc >= 9 && std::string(argv[1]) == "bench_mcc") {
@@ -391,6 +432,7 @@ int main(int argc, char** argv) {
             "usage: ref_harness golden <outdir> <sigma_el_table>\n"
             "       ref_harness bench <nodes_per_axis> <particles> <steps> <threads> <solve0|1>\n"
             "       ref_harness bench_mcc <nodes_xy> <nodes_z> <neutrals> <electrons> <steps> <dt> "
-            "<sigma_el_table>\n");
+            "<sigma_el_table>\n"
+            "       ref_harness bench_newton <nodes_per_axis> <threads> <tol>\n");
     return 1;
 }
