@@ -208,7 +208,8 @@ struct pic_ctx {
     std::vector<pic::Mcc> mcc;
     pic::Prof prof;
     void* unet = nullptr;  // learned preconditioner state
-    float* dep_corner = nullptr;  // density deposit: 8 corner sums per cell (allocated on demand)
+    float* dep_corner = nullptr;  // deposits' corner sums per cell (8, or 7 x 8 for the moments; on demand)
+    size_t dep_corner_floats = 0;
     int learned_last_its = 0;  // iterations of the last learned solve (first check near it)
     double* pin = nullptr;     // pinned host memory for the solvers' convergence reads
     size_t pin_bytes = 0;

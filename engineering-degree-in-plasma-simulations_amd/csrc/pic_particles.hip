@@ -6,9 +6,11 @@
 //   k_scatter  stable-per-wave scatter into the second buffer (compacts)
 //   k_push_scatter  second pass of pic_push_sort: push again and write each
 //              particle to its sorted position
-//   k_deposit  one workgroup per cell row, LDS tile of the row's 4 node
-//              lines, flushed with FP64 global atomics (Species.cpp:401-416,
-//              :767-776, Field.h:157-199)
+//   k_dep_cells, k_dep_nodes  number density in two passes without atomics:
+//              per-cell corner sums, then a node gather (Species.cpp:401-416,
+//              Field.h:157-199)
+//   k_mom_cells, k_mom_nodes  the velocity moments the same way
+//              (Species.cpp:767-776)
 //   k_rho      charge density (World.cpp:193-200)
 //   k_load     Philox loader (Species.cpp:564-599, :420-434, :855-869)
 #include <algorithm>
@@ -413,126 +415,6 @@ __global__ void __launch_bounds__(BLOCK)
     }
 }
 
-// ---- deposit: one workgroup per cell row (j, k); the row's particles are
-// contiguous after the sort and touch node lines (j..j+1, k..k+1). ----
-template <int NCH, int J>
-__global__ void __launch_bounds__(BLOCK)
-    k_deposit(GridC g, SpeciesBuf b, const uint32_t* __restrict__ off, double* __restrict__ den,
-              double* __restrict__ mom, int want_den) {
-    // a workgroup owns J consecutive cell rows (j0 .. j0+J-1, k) of one k plane;
-    // node lines (j0 + jl, k + dk), jl = 0..J, dk = 0..1, sit in LDS as line
-    // jl + (J+1) dk, so the j lines shared inside the block add up in LDS
-    extern __shared__ float tile[];  // NCH channels x 2(J+1) node lines x ni
-    const int nbj = (g.cj + J - 1) / J;
-    const int k = blockIdx.x / nbj, j0 = (blockIdx.x - k * nbj) * J;
-    const int j1 = min(j0 + J, g.cj);
-    const size_t r0 = (size_t)k * g.cj + j0, r1 = (size_t)k * g.cj + j1;
-    const uint32_t p0 = off[r0 * g.ci], p1 = off[r1 * g.ci];
-    if (p0 == p1) return;  // uniform for the whole workgroup
-    const int ni = g.ni;
-    const int line = 2 * (J + 1) * ni;
-    for (int t = threadIdx.x; t < NCH * line; t += BLOCK) tile[t] = 0.f;
-    __syncthreads();
-    // Each thread takes 4 consecutive particles with 16-byte loads (the
-    // arrays are 256-byte aligned and sized to a multiple of 8 entries, so the
-    // aligned quad around [p0, p1) stays inside the allocation).  Sorted
-    // particles of one cell are folded in registers; LDS sees one atomic per
-    // corner per run of equal cells instead of one per particle.
-    float acc[8][NCH];
-    int cur = -1;  // (jl << 11) | i of the cell being folded
-    auto flush = [&]() {
-        if (cur < 0) return;
-        const int ci_ = cur & 0x7FF, jl = cur >> 11;
-#pragma unroll
-        for (int q = 0; q < 8; q++) {  // q = (dk*2 + dj)*2 + di
-            int slot = (jl + ((q >> 1) & 1) + (J + 1) * (q >> 2)) * ni + ci_ + (q & 1);
-#pragma unroll
-            for (int ch = 0; ch < NCH; ch++) atomicAdd(&tile[ch * line + slot], acc[q][ch]);
-        }
-    };
-    for (uint32_t base = (p0 & ~3u) + 4u * threadIdx.x; base < p1; base += 4u * BLOCK) {
-        uint4 c4 = *reinterpret_cast<const uint4*>(b.cell + base);
-        float4 fx4 = *reinterpret_cast<const float4*>(b.f[0] + base);
-        float4 fy4 = *reinterpret_cast<const float4*>(b.f[1] + base);
-        float4 fz4 = *reinterpret_cast<const float4*>(b.f[2] + base);
-        float4 w4 = *reinterpret_cast<const float4*>(b.w + base);
-        float4 vx4, vy4, vz4;
-        if (NCH > 1) {
-            vx4 = *reinterpret_cast<const float4*>(b.v[0] + base);
-            vy4 = *reinterpret_cast<const float4*>(b.v[1] + base);
-            vz4 = *reinterpret_cast<const float4*>(b.v[2] + base);
-        }
-        const uint32_t cs[4] = {c4.x, c4.y, c4.z, c4.w};
-        const float fxs[4] = {fx4.x, fx4.y, fx4.z, fx4.w}, fys[4] = {fy4.x, fy4.y, fy4.z, fy4.w};
-        const float fzs[4] = {fz4.x, fz4.y, fz4.z, fz4.w}, ws[4] = {w4.x, w4.y, w4.z, w4.w};
-#pragma unroll
-        for (int m = 0; m < 4; m++) {
-            uint32_t p = base + m;
-            if (p < p0 || p >= p1) continue;
-            int key = (int)(cs[m] & 0x7FFu) | ((int)(((cs[m] >> 11) & 0x7FFu) - (uint32_t)j0) << 11);
-            if (key != cur) {
-                flush();
-                cur = key;
-#pragma unroll
-                for (int q = 0; q < 8; q++)
-#pragma unroll
-                    for (int ch = 0; ch < NCH; ch++) acc[q][ch] = 0.f;
-            }
-            float fx = fxs[m], fy = fys[m], fz = fzs[m], w = ws[m];
-            float gx = 1.f - fx, gy = 1.f - fy, gz = 1.f - fz;
-            float wt[8];  // index (dk*2 + dj)*2 + di
-            wt[0] = gx * gy * gz;
-            wt[1] = fx * gy * gz;
-            wt[2] = gx * fy * gz;
-            wt[3] = fx * fy * gz;
-            wt[4] = gx * gy * fz;
-            wt[5] = fx * gy * fz;
-            wt[6] = gx * fy * fz;
-            wt[7] = fx * fy * fz;
-            float val[NCH];
-            val[0] = w;
-            if (NCH > 1) {
-                const float vxs[4] = {vx4.x, vx4.y, vx4.z, vx4.w};
-                const float vys[4] = {vy4.x, vy4.y, vy4.z, vy4.w};
-                const float vzs[4] = {vz4.x, vz4.y, vz4.z, vz4.w};
-                float vx = vxs[m], vy = vys[m], vz = vzs[m];
-                val[1 % NCH] = w * vx;
-                val[2 % NCH] = w * vy;
-                val[3 % NCH] = w * vz;
-                val[4 % NCH] = w * vx * vx;
-                val[5 % NCH] = w * vy * vy;
-                val[6 % NCH] = w * vz * vz;
-            }
-#pragma unroll
-            for (int q = 0; q < 8; q++)
-#pragma unroll
-                for (int ch = 0; ch < NCH; ch++) acc[q][ch] += val[ch] * wt[q];
-        }
-    }
-    flush();
-    __syncthreads();
-    const long long nn = g.nn;
-    for (int t = threadIdx.x; t < line; t += BLOCK) {
-        int ln = t / ni, ii = t - ln * ni;
-        int dk = ln / (J + 1);
-        int jj = j0 + ln - dk * (J + 1), kk = k + dk;
-        if (jj > g.cj) continue;  // the last block of a plane may hold fewer than J rows
-        long long u = ((long long)kk * g.nj + jj) * ni + ii;
-        float v0 = tile[t];
-        if (v0 != 0.f) {
-            if (want_den) atomicAdd(&den[u], (double)v0 * inv_node_vol(g, ii, jj, kk));
-            if (NCH > 1) atomicAdd(&mom[u], (double)v0);
-        }
-        if (NCH > 1) {
-#pragma unroll
-            for (int ch = 1; ch < NCH; ch++) {
-                float vc = tile[ch * line + t];
-                if (vc != 0.f) atomicAdd(&mom[(long long)ch * nn + u], (double)vc);
-            }
-        }
-    }
-}
-
 // Number density without atomics, in two passes over the sorted particles.
 // Pass 1, one thread per cell: the cell's particles (contiguous after the
 // sort, so the cell word is not read) are folded into the eight corner sums
@@ -620,6 +502,106 @@ __global__ void __launch_bounds__(BLOCK)
         s += (double)corner[(size_t)q * g.ncells + cell_index(g, ci, cj, ck)];
     }
     den[u] = s * inv_node_vol(g, i, j, k);
+}
+
+// Velocity moments (Species.cpp:767-776: n, nv, nuu, nvv, nww accumulated
+// with CIC weights) in the same two passes as the density: per cell, the
+// seven channels' eight corner sums (FP32, particle order) from LDS-staged
+// chunks; per node, the sum over its cells added to the FP64 moments (the
+// node's thread owns it: no atomics), and with want_den the density.
+constexpr int MOM_CHUNK = 1024;
+constexpr int MOM_CH = 7;
+__global__ void __launch_bounds__(BLOCK)
+    k_mom_cells(GridC g, SpeciesBuf b, const uint32_t* __restrict__ off, float* __restrict__ corner) {
+    __shared__ float4 sf[7][MOM_CHUNK / 4];  // fx, fy, fz, w, vx, vy, vz of the chunk
+    const int tid = threadIdx.x;
+    const int c = blockIdx.x * BLOCK + tid;
+    const int clast = min((int)(blockIdx.x + 1) * BLOCK, g.ncells);
+    const uint32_t P0 = off[blockIdx.x * BLOCK], P1 = off[clast];
+    uint32_t p0 = 0, p1 = 0;
+    if (c < g.ncells) {
+        p0 = off[c];
+        p1 = off[c + 1];
+    }
+    float acc[MOM_CH][8];
+#pragma unroll
+    for (int ch = 0; ch < MOM_CH; ch++)
+#pragma unroll
+        for (int q = 0; q < 8; q++) acc[ch][q] = 0.f;
+    const float* src[7] = {b.f[0], b.f[1], b.f[2], b.w, b.v[0], b.v[1], b.v[2]};
+    for (uint32_t cs = P0 & ~3u; cs < P1; cs += MOM_CHUNK) {
+        constexpr int PER = MOM_CHUNK / 4 / BLOCK;
+        float4 r[7][PER];
+#pragma unroll
+        for (int e = 0; e < PER; e++) {
+            const uint32_t q = cs + 4 * (tid + BLOCK * e);
+            const bool in = q < P1;  // arrays padded to 64 entries: a quad never leaves its array
+#pragma unroll
+            for (int a = 0; a < 7; a++)
+                r[a][e] = in ? *reinterpret_cast<const float4*>(src[a] + q) : make_float4(0.f, 0.f, 0.f, 0.f);
+        }
+        __syncthreads();  // the previous chunk is consumed
+#pragma unroll
+        for (int e = 0; e < PER; e++)
+#pragma unroll
+            for (int a = 0; a < 7; a++) sf[a][tid + BLOCK * e] = r[a][e];
+        __syncthreads();
+        const float* sv[7];
+#pragma unroll
+        for (int a = 0; a < 7; a++) sv[a] = reinterpret_cast<const float*>(sf[a]);
+        const uint32_t ce = cs + MOM_CHUNK;
+        const uint32_t lo = p0 > cs ? p0 : cs, hi = p1 < ce ? p1 : ce;
+        for (uint32_t p = lo; p < hi; p++) {
+            const uint32_t l = p - cs;
+            const float fx = sv[0][l], fy = sv[1][l], fz = sv[2][l], w = sv[3][l];
+            const float vx = sv[4][l], vy = sv[5][l], vz = sv[6][l];
+            const float gx = 1.f - fx, gy = 1.f - fy, gz = 1.f - fz;
+            float wt[8];  // corner (dk*2 + dj)*2 + di
+            wt[0] = gx * gy * gz;
+            wt[1] = fx * gy * gz;
+            wt[2] = gx * fy * gz;
+            wt[3] = fx * fy * gz;
+            wt[4] = gx * gy * fz;
+            wt[5] = fx * gy * fz;
+            wt[6] = gx * fy * fz;
+            wt[7] = fx * fy * fz;
+            const float val[MOM_CH] = {w, w * vx, w * vy, w * vz, w * vx * vx, w * vy * vy, w * vz * vz};
+#pragma unroll
+            for (int ch = 0; ch < MOM_CH; ch++)
+#pragma unroll
+                for (int q = 0; q < 8; q++) acc[ch][q] += val[ch] * wt[q];
+        }
+    }
+    if (c >= g.ncells) return;
+#pragma unroll
+    for (int ch = 0; ch < MOM_CH; ch++)
+#pragma unroll
+        for (int q = 0; q < 8; q++) corner[((size_t)ch * 8 + q) * g.ncells + c] = acc[ch][q];
+}
+
+__global__ void __launch_bounds__(BLOCK)
+    k_mom_nodes(GridC g, const float* __restrict__ corner, double* __restrict__ mom, double* __restrict__ den) {
+    long long u = (long long)blockIdx.x * BLOCK + threadIdx.x;
+    if (u >= g.nn) return;
+    unsigned uu = (unsigned)u, row = uu / (unsigned)g.ni, kk = row / (unsigned)g.nj;
+    int i = (int)(uu - row * (unsigned)g.ni), j = (int)(row - kk * (unsigned)g.nj), k = (int)kk;
+    long long cell[8];
+#pragma unroll
+    for (int q = 0; q < 8; q++) {
+        int ci = i - (q & 1), cj = j - ((q >> 1) & 1), ck = k - (q >> 2);
+        cell[q] = (ci < 0 || ci >= g.ci || cj < 0 || cj >= g.cj || ck < 0 || ck >= g.ck)
+                      ? -1
+                      : (long long)cell_index(g, ci, cj, ck);
+    }
+#pragma unroll
+    for (int ch = 0; ch < MOM_CH; ch++) {
+        double s = 0.0;
+#pragma unroll
+        for (int q = 0; q < 8; q++)
+            if (cell[q] >= 0) s += (double)corner[((size_t)ch * 8 + q) * g.ncells + cell[q]];
+        mom[(long long)ch * g.nn + u] += s;
+        if (ch == 0 && den) den[u] = s * inv_node_vol(g, i, j, k);
+    }
 }
 
 struct RhoArgs {
@@ -836,21 +818,27 @@ extern "C" int pic_deposit(pic_ctx* ctx, int sid, int flags) {
     size_t nn = (size_t)g.nn;
     int want_den = (flags & PIC_DEPOSIT_DENSITY) != 0;
     bool mom = (flags & PIC_DEPOSIT_MOMENTS) != 0;
-    if (want_den && mom) PIC_HIP(ctx, hipMemsetAsync(s.den, 0, nn * 8, ctx->stream));
-    if (want_den && !mom && !ctx->dep_corner)
-        PIC_HIP(ctx, hipMalloc(&ctx->dep_corner, 8 * (size_t)g.ncells * sizeof(float)));
+    // corner sums: 8 per cell for the density, 7 x 8 for the moments
+    const size_t corner_floats = (mom ? 7 * 8 : 8) * (size_t)g.ncells;
+    if (ctx->dep_corner_floats < corner_floats) {
+        PIC_HIP(ctx, hipStreamSynchronize(ctx->stream));
+        (void)hipFree(ctx->dep_corner);
+        ctx->dep_corner = nullptr;
+        ctx->dep_corner_floats = 0;
+        PIC_HIP(ctx, hipMalloc(&ctx->dep_corner, corner_floats * sizeof(float)));
+        ctx->dep_corner_floats = corner_floats;
+    }
     if (mom && !s.mom) {
         PIC_HIP(ctx, hipMalloc(&s.mom, 7 * nn * 8));
         PIC_HIP(ctx, hipMemsetAsync(s.mom, 0, 7 * nn * 8, ctx->stream));
     }
-    constexpr int JM = 4;  // cell rows per workgroup of the moments deposit
-    int rows_m = g.ck * ((g.cj + JM - 1) / JM);
     {
         PicProfScope ps(ctx, PIC_K_DEPOSIT);
         if (mom) {
-            size_t lds = 7 * 2 * (JM + 1) * (size_t)g.ni * sizeof(float);
-            k_deposit<7, JM><<<rows_m, BLOCK, lds, ctx->stream>>>(g, s.buf[s.cur], s.off, s.den, s.mom,
-                                                                  want_den);
+            k_mom_cells<<<grid_for((size_t)g.ncells, BLOCK), BLOCK, 0, ctx->stream>>>(
+                g, s.buf[s.cur], s.off, ctx->dep_corner);
+            k_mom_nodes<<<grid_for(nn, BLOCK), BLOCK, 0, ctx->stream>>>(g, ctx->dep_corner, s.mom,
+                                                                        want_den ? s.den : nullptr);
         } else {
             k_dep_cells<<<grid_for((size_t)g.ncells, BLOCK), BLOCK, 0, ctx->stream>>>(
                 g, s.buf[s.cur], s.off, ctx->dep_corner);

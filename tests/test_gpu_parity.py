@@ -177,6 +177,13 @@ def test_sort_and_density(pg):
     for which, r in zip(names, acc):
         got = c.species_field(sid, which)
         assert np.max(np.abs(got - r)) <= REL32 * np.max(np.abs(r)), which
+    # the moments accumulate over calls until the samples are cleared
+    c.deposit(sid, pg.DEPOSIT_MOMENTS)
+    for which, r in zip(names, acc):
+        got = c.species_field(sid, which)
+        assert np.max(np.abs(got - 2 * r)) <= 2 * REL32 * np.max(np.abs(r)), which
+    c.clear_samples(sid)
+    assert np.all(c.species_field(sid, pg.SFIELD_NUU) == 0)
     # the density-only deposit (cell corner sums, then a node gather) agrees too
     c.deposit(sid, pg.DEPOSIT_DENSITY)
     den1 = c.species_field(sid, pg.SFIELD_DEN)
