@@ -160,6 +160,7 @@ extern "C" void pic_destroy(pic_ctx* ctx) {
         (void)hipFree(s.d_n);
         (void)hipFree(s.den);
         (void)hipFree(s.mom);
+        (void)hipFree(s.toff);
     }
     for (auto& m : ctx->mcc) {
         (void)hipFree(m.sig);
@@ -495,6 +496,8 @@ extern "C" int pic_particles_upload(pic_ctx* ctx, int sid, const double* aos7, s
     s.n_upper = n;
     s.hist_valid = false;
     s.sorted = false;
+    s.two_part = false;
+    s.tail_ok = false;
     return PIC_OK;
 }
 

@@ -209,8 +209,8 @@ constexpr int SIG_LDS_MAX = 256;
 
 template <bool LDS_TABLE>
 __global__ void __launch_bounds__(BLOCK)
-    k_mcc(GridC g, MccArgs a, Sp N, const uint32_t* __restrict__ offN, Sp I, Sp E,
-          const uint32_t* __restrict__ offE, const double* __restrict__ table,
+    k_mcc(GridC g, MccArgs a, Sp N, const uint32_t* __restrict__ offN, const uint32_t* __restrict__ toffN,
+          uint32_t tbase, Sp I, Sp E, const uint32_t* __restrict__ offE, const double* __restrict__ table,
           const double* __restrict__ wsv, unsigned long long* __restrict__ part) {
     // this workgroup's stripe: collisions, ionizations, groups, overflow, step maximum
     unsigned long long* stats = part + (blockIdx.x % NSTRIPE) * SSTRIDE;
@@ -224,11 +224,19 @@ __global__ void __launch_bounds__(BLOCK)
     __syncthreads();
     int c = blockIdx.x * BLOCK + threadIdx.x;
     double wsv_tmp = 0;
-    unsigned long long ncoll = 0, niz = 0, ngr = 0;
+    unsigned long long ncoll = 0, niz = 0, ngr = 0, ndead = 0;
     if (c < g.ncells) {
         uint32_t e0 = offE[c], e1 = offE[c + 1];
         uint32_t n0 = offN[c], n1 = offN[c + 1];
-        int Ne = (int)(e1 - e0), Nn = (int)(n1 - n0);
+        // a cell's neutrals: its range of the sorted part, then (toffN) its
+        // range of the sorted appended part at tbase
+        uint32_t t0 = 0, t1 = 0;
+        if (toffN) {
+            t0 = tbase + toffN[c];
+            t1 = tbase + toffN[c + 1];
+        }
+        const int Nb = (int)(n1 - n0);
+        int Ne = (int)(e1 - e0), Nn = Nb + (int)(t1 - t0);
         if (Ne > 0 && Nn > 0) {
             const double wmax = wsv[0];
             double frac = (double)(Nn * Ne) * wmax * a.dt_inv_dv;  // Interactions.cpp:646
@@ -242,7 +250,7 @@ __global__ void __launch_bounds__(BLOCK)
                 ngr++;
                 int ns = (int)(dr(0) * (double)Nn);
                 int es = (int)(dr(1) * (double)Ne);
-                uint32_t pn = n0 + (uint32_t)ns, pe = e0 + (uint32_t)es;
+                uint32_t pn = ns < Nb ? n0 + (uint32_t)ns : t0 + (uint32_t)(ns - Nb), pe = e0 + (uint32_t)es;
                 // both particles' weights and velocities requested together
                 double wn = N.b.w[pn], we = E.b.w[pe];
                 double vn[3] = {N.b.v[0][pn], N.b.v[1][pn], N.b.v[2][pn]};
@@ -287,6 +295,7 @@ __global__ void __launch_bounds__(BLOCK)
                         append(births, 2, E, ncell, nfx, nfy, nfz, vnew, wn, &stats[3]);
                         N.b.w[pn] = 0.0f;
                         N.b.cell[pn] = DEAD;
+                        ndead++;
                     }
                 }
                 E.b.v[0][pe] = (float)ve[0];
@@ -295,6 +304,7 @@ __global__ void __launch_bounds__(BLOCK)
             }
         }
     }
+    if (ndead) atomicAdd(&stats[5], ndead);  // neutrals removed (rare)
     flush_births(births, N, I, E, &stats[3]);
     // block reduction of the counters and the step maximum
     __shared__ double smax[BLOCK / 64];
@@ -337,12 +347,13 @@ __global__ void __launch_bounds__(BLOCK)
 __global__ void k_mcc_finish(double* wsv, const unsigned long long* part, unsigned long long* stats,
                              uint32_t* caps_n, uint32_t cap_n, uint32_t* caps_i, uint32_t cap_i,
                              uint32_t* caps_e, uint32_t cap_e) {
-    unsigned long long t[4] = {0, 0, 0, 0}, mb = 0;
+    unsigned long long t[5] = {0, 0, 0, 0, 0}, mb = 0;
     for (int s = 0; s < NSTRIPE; s++) {
         for (int q = 0; q < 4; q++) t[q] += part[s * SSTRIDE + q];
+        t[4] += part[s * SSTRIDE + 5];
         mb = part[s * SSTRIDE + 4] > mb ? part[s * SSTRIDE + 4] : mb;
     }
-    for (int q = 0; q < 4; q++) stats[q] = t[q];
+    for (int q = 0; q < 5; q++) stats[q] = t[q];
     wsv[1] = __longlong_as_double((long long)mb);
     if (stats[0] > 0) wsv[0] = wsv[1];
     if (caps_n[0] > cap_n) caps_n[0] = cap_n;
@@ -376,7 +387,7 @@ extern "C" int pic_mcc_create(pic_ctx* ctx, int sn, int si, int se, const pic_mc
     m.nsig = nsig;
     PIC_HIP(ctx, hipMalloc(&m.sig, 2 * (size_t)nsig * sizeof(double)));
     PIC_HIP(ctx, hipMalloc(&m.d_wsv, 2 * sizeof(double)));
-    PIC_HIP(ctx, hipMalloc(&m.d_stats, 4 * sizeof(unsigned long long)));
+    PIC_HIP(ctx, hipMalloc(&m.d_stats, 5 * sizeof(unsigned long long)));
     PIC_HIP(ctx, hipMalloc(&m.d_part, (size_t)NSTRIPE * SSTRIDE * sizeof(unsigned long long)));
     PIC_HIP(ctx, hipMemcpyAsync(m.sig, sig_E, nsig * sizeof(double), hipMemcpyHostToDevice,
                                 ctx->stream));
@@ -399,9 +410,19 @@ extern "C" int pic_mcc_apply(pic_ctx* ctx, int id, double dt, uint32_t seed, uin
     Mcc& m = ctx->mcc[id];
     Species &N = ctx->sp[m.sn], &I = ctx->sp[m.si], &E = ctx->sp[m.se];
     int rc = 0;
-    if (!N.sorted) rc = pic_sort(ctx, m.sn);
+    // the neutrals: only the births appended since their last sort are
+    // sorted while they stay a few (the neutrals move once per many steps),
+    // else the whole species
+    if (!N.sorted) {
+        const size_t tail = N.n_upper > N.bulk_n ? N.n_upper - N.bulk_n : 0;
+        if (N.tail_ok && N.n_upper >= N.bulk_n && tail <= N.bulk_n / 4)
+            rc = pic_sort_tail(ctx, m.sn);
+        else
+            rc = pic_sort(ctx, m.sn);
+    }
     if (!rc && !E.sorted) rc = pic_sort(ctx, m.se);
     if (rc) return rc;
+    const bool two = !N.sorted && N.two_part;
     const GridC& g = ctx->g;
     MccArgs a;
     a.m_n = N.mass;
@@ -429,23 +450,27 @@ extern "C" int pic_mcc_apply(pic_ctx* ctx, int id, double dt, uint32_t seed, uin
         PicProfScope ps(ctx, PIC_K_MCC);
         if (m.nsig <= SIG_LDS_MAX)
             k_mcc<true><<<grid_for((size_t)g.ncells, BLOCK), BLOCK, 0, ctx->stream>>>(
-                g, a, sn, N.off, si, se, E.off, m.sig, m.d_wsv, m.d_part);
+                g, a, sn, N.off, two ? N.toff : nullptr, (uint32_t)N.bulk_n, si, se, E.off, m.sig, m.d_wsv,
+                m.d_part);
         else
             k_mcc<false><<<grid_for((size_t)g.ncells, BLOCK), BLOCK, 0, ctx->stream>>>(
-                g, a, sn, N.off, si, se, E.off, m.sig, m.d_wsv, m.d_part);
+                g, a, sn, N.off, two ? N.toff : nullptr, (uint32_t)N.bulk_n, si, se, E.off, m.sig, m.d_wsv,
+                m.d_part);
         k_mcc_finish<<<1, 1, 0, ctx->stream>>>(m.d_wsv, m.d_part, m.d_stats, N.d_n, (uint32_t)N.cap,
                                                I.d_n, (uint32_t)I.cap, E.d_n, (uint32_t)E.cap);
     }
     PIC_HIP(ctx, hipGetLastError());
-    unsigned long long hs[4];
+    unsigned long long hs[5];
     double hw[2];
-    uint32_t cn[3];
+    uint32_t cn[4];
     PIC_HIP(ctx, hipMemcpyAsync(hs, m.d_stats, sizeof(hs), hipMemcpyDeviceToHost, ctx->stream));
     PIC_HIP(ctx, hipMemcpyAsync(hw, m.d_wsv, sizeof(hw), hipMemcpyDeviceToHost, ctx->stream));
     PIC_HIP(ctx, hipMemcpyAsync(&cn[0], N.d_n, 4, hipMemcpyDeviceToHost, ctx->stream));
     PIC_HIP(ctx, hipMemcpyAsync(&cn[1], I.d_n, 4, hipMemcpyDeviceToHost, ctx->stream));
     PIC_HIP(ctx, hipMemcpyAsync(&cn[2], E.d_n, 4, hipMemcpyDeviceToHost, ctx->stream));
+    PIC_HIP(ctx, hipMemcpyAsync(&cn[3], N.d_n + 3, 4, hipMemcpyDeviceToHost, ctx->stream));
     PIC_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    N.bulk_n = cn[3];  // the sorted part (the live count of the last whole sort)
     N.n_upper = cn[0];
     I.n_upper = cn[1];
     E.n_upper = cn[2];
@@ -454,6 +479,12 @@ extern "C" int pic_mcc_apply(pic_ctx* ctx, int id, double dt, uint32_t seed, uin
         N.sorted = I.sorted = E.sorted = false;
         N.hist_valid = I.hist_valid = E.hist_valid = false;
     }
+    // births only appended: the neutrals' next sort may take just those,
+    // unless a neutral was removed (a removed particle leaves the species
+    // only in a whole sort); ions and electrons are sorted whole
+    if (hs[4]) N.tail_ok = false;
+    I.tail_ok = E.tail_ok = false;
+    I.two_part = E.two_part = false;
     if (stats) {
         stats->collisions = (long long)hs[0];
         stats->ionizations = (long long)hs[1];

@@ -198,12 +198,12 @@ __global__ void __launch_bounds__(BLOCK)
 
 __global__ void __launch_bounds__(BLOCK)
     k_hist(GridC g, SpeciesBuf b, uint32_t* __restrict__ d_n, uint32_t* __restrict__ count,
-           uint32_t* __restrict__ rank) {
-    size_t p = xcd_wg() * BLOCK + threadIdx.x;
+           uint32_t* __restrict__ rank, uint32_t start) {  // particles [start, live count)
+    size_t p = start + xcd_wg() * BLOCK + threadIdx.x;
     uint32_t n = d_n[0];
-    uint32_t covered = gridDim.x * BLOCK;
+    uint32_t covered = start + gridDim.x * BLOCK;
     if (n > covered) n = covered;
-    if (p == 0) d_n[1] = n;
+    if (p == start) d_n[1] = n;
     bool valid = p < n;
     uint32_t key = 0xFFFFFFFFu;
     if (valid) {
@@ -290,7 +290,10 @@ __global__ void __launch_bounds__(BLOCK)
         size_t idx = base + q;
         if (idx < (size_t)M) {
             out[idx] = ex;
-            if (idx == (size_t)ncells) d_n[0] = ex;  // particles alive after compaction
+            if (idx == (size_t)ncells && d_n) {  // particles alive after compaction
+                d_n[0] = ex;
+                d_n[3] = ex;  // all of them sorted: the sorted part of a two-part species
+            }
             if (idx == (size_t)M - 1) out[M] = ex + v[q];
         }
         ex += v[q];
@@ -306,11 +309,12 @@ __global__ void __launch_bounds__(BLOCK)
 constexpr int SC_V = 4;
 __global__ void __launch_bounds__(BLOCK)
     k_scatter(GridC g, SpeciesBuf src, SpeciesBuf dst, const uint32_t* __restrict__ rank,
-              const uint32_t* __restrict__ off, const uint32_t* __restrict__ d_n, uint32_t cap) {
+              const uint32_t* __restrict__ off, const uint32_t* __restrict__ d_n, uint32_t cap,
+              uint32_t start) {  // particles [start, d_n[1]) to start + off[cell] + rank
     constexpr int NP = BLOCK * SC_V;
     __shared__ uint4 st[9][BLOCK];  // cell, fx, fy, fz, vx, vy, vz, w, rank; 4 particles each
     const uint32_t n = d_n[1];
-    const size_t base = xcd_wg() * NP;
+    const size_t base = (start & ~3u) + xcd_wg() * NP;  // 16-byte aligned groups of 4
     const int tid = threadIdx.x;
     const size_t p4 = base + (size_t)SC_V * tid;
     const uint32_t* arr[9] = {src.cell,
@@ -323,7 +327,7 @@ __global__ void __launch_bounds__(BLOCK)
                               reinterpret_cast<const uint32_t*>(src.w),
                               rank};
     uint4 r[9];
-    if (p4 + SC_V <= n) {
+    if (p4 >= start && p4 + SC_V <= n) {
 #pragma unroll
         for (int a = 0; a < 9; a++) r[a] = *reinterpret_cast<const uint4*>(arr[a] + p4);
     } else {
@@ -332,7 +336,7 @@ __global__ void __launch_bounds__(BLOCK)
             uint32_t e[4] = {DEAD, DEAD, DEAD, DEAD};  // past the live count: skipped below
 #pragma unroll
             for (int k = 0; k < 4; k++)
-                if (p4 + k < n) e[k] = arr[a][p4 + k];
+                if (p4 + k >= start && p4 + k < n) e[k] = arr[a][p4 + k];
             r[a] = make_uint4(e[0], e[1], e[2], e[3]);
         }
     }
@@ -345,12 +349,12 @@ __global__ void __launch_bounds__(BLOCK)
 #pragma unroll
     for (int e = 0; e < SC_V; e++) {
         const int q = tid + BLOCK * e;
-        if (base + q >= n) continue;
+        if (base + q >= n || base + q < start) continue;
         const uint32_t c = s32[0][q];
         if (c == DEAD) continue;
         int i, j, k;
         unpack_cell(c, i, j, k);
-        const uint32_t dst_i = off[cell_index(g, i, j, k)] + s32[8][q];
+        const uint32_t dst_i = start + off[cell_index(g, i, j, k)] + s32[8][q];
         if (dst_i >= cap) continue;  // cannot happen with a current histogram
         dst.cell[dst_i] = c;
         dst.f[0][dst_i] = __uint_as_float(s32[1][q]);
@@ -718,6 +722,8 @@ extern "C" int pic_push(pic_ctx* ctx, int sid, double dt) {
     PIC_HIP(ctx, hipGetLastError());
     s.hist_valid = true;
     s.sorted = false;
+    s.two_part = false;
+    s.tail_ok = false;  // cells changed
     return PIC_OK;
 }
 
@@ -729,23 +735,24 @@ int pic_histogram(pic_ctx* ctx, int sid) {
         PicProfScope ps(ctx, PIC_K_SORT);
         size_t n = s.n_upper ? s.n_upper : 1;
         k_hist<<<xcd_blocks1(n, BLOCK), BLOCK, 0, ctx->stream>>>(g, s.buf[s.cur], s.d_n, s.count,
-                                                              s.rank);
+                                                              s.rank, 0u);
     }
     PIC_HIP(ctx, hipGetLastError());
     s.hist_valid = true;
     return PIC_OK;
 }
 
-// exclusive scan of the species' cell counts into its offsets (and the live
-// count into d_n[0])
-static void scan_counts(pic_ctx* ctx, Species& s) {
+// exclusive scan of the species' cell counts into out (the offsets; for a
+// whole-species sort also the live count into d_n[0] and d_n[3])
+static void scan_counts(pic_ctx* ctx, Species& s, uint32_t* out, bool whole) {
     const GridC& g = ctx->g;
     int M = g.ncells + 1;
     int nb = (M + SCAN_TILE - 1) / SCAN_TILE;
     PicProfScope ps(ctx, PIC_K_SORT);
     k_scan_reduce<<<nb, BLOCK, 0, ctx->stream>>>(s.count, M, ctx->scan_tmp);
     k_scan_top<<<1, BLOCK, 0, ctx->stream>>>(ctx->scan_tmp, nb);
-    k_scan_down<<<nb, BLOCK, 0, ctx->stream>>>(s.count, M, ctx->scan_tmp, s.off, g.ncells, s.d_n);
+    k_scan_down<<<nb, BLOCK, 0, ctx->stream>>>(s.count, M, ctx->scan_tmp, out, g.ncells,
+                                               whole ? s.d_n : nullptr);
 }
 
 extern "C" int pic_sort(pic_ctx* ctx, int sid) {
@@ -757,16 +764,59 @@ extern "C" int pic_sort(pic_ctx* ctx, int sid) {
         rc = pic_histogram(ctx, sid);
         if (rc) return rc;
     }
-    scan_counts(ctx, s);
+    scan_counts(ctx, s, s.off, true);
     {
         PicProfScope ps(ctx, PIC_K_SCATTER);
         size_t n = s.n_upper ? s.n_upper : 1;
         k_scatter<<<xcd_blocks1((n + SC_V - 1) / SC_V, BLOCK), BLOCK, 0, ctx->stream>>>(
-            g, s.buf[s.cur], s.buf[s.cur ^ 1], s.rank, s.off, s.d_n, (uint32_t)s.cap);
+            g, s.buf[s.cur], s.buf[s.cur ^ 1], s.rank, s.off, s.d_n, (uint32_t)s.cap, 0u);
     }
     PIC_HIP(ctx, hipGetLastError());
     s.cur ^= 1;
     s.sorted = true;
+    s.hist_valid = false;
+    s.two_part = false;
+    s.tail_ok = true;  // all sorted: an empty appended part
+    return PIC_OK;
+}
+
+// Sort only the particles appended since the last whole-species sort
+// (indices [bulk_n, n_upper), both exact: DSMC births, nothing removed), by
+// cell into toff, in place: the species is then a sorted part with offsets
+// off followed by a sorted appended part with offsets toff.
+int pic_sort_tail(pic_ctx* ctx, int sid) {
+    Species& s = ctx->sp[sid];
+    const GridC& g = ctx->g;
+    const size_t start = s.bulk_n, n = s.n_upper;
+    if (n < start) return pic_fail(ctx, PIC_ERR_STATE, "appended part before the sorted part");
+    if (!s.toff) PIC_HIP(ctx, hipMalloc(&s.toff, ((size_t)g.ncells + 2) * 4));
+    PIC_HIP(ctx, hipMemsetAsync(s.count, 0, ((size_t)g.ncells + 1) * 4, ctx->stream));
+    const size_t cnt = n - start;
+    {
+        PicProfScope ps(ctx, PIC_K_SORT);
+        k_hist<<<xcd_blocks1(cnt ? cnt : 1, BLOCK), BLOCK, 0, ctx->stream>>>(g, s.buf[s.cur], s.d_n, s.count,
+                                                                          s.rank, (uint32_t)start);
+    }
+    scan_counts(ctx, s, s.toff, false);
+    {
+        PicProfScope ps(ctx, PIC_K_SCATTER);
+        const size_t span = n - (start & ~(size_t)3);  // from the aligned group holding start
+        k_scatter<<<xcd_blocks1((span + SC_V - 1) / SC_V, BLOCK), BLOCK, 0, ctx->stream>>>(
+            g, s.buf[s.cur], s.buf[s.cur ^ 1], s.rank, s.toff, s.d_n, (uint32_t)s.cap, (uint32_t)start);
+    }
+    PIC_HIP(ctx, hipGetLastError());
+    if (cnt) {  // the sorted appended part back behind the sorted part
+        const SpeciesBuf &a = s.buf[s.cur], &b = s.buf[s.cur ^ 1];
+        PIC_HIP(ctx, hipMemcpyAsync(a.cell + start, b.cell + start, cnt * 4, hipMemcpyDeviceToDevice, ctx->stream));
+        for (int d = 0; d < 3; d++) {
+            PIC_HIP(ctx, hipMemcpyAsync(a.f[d] + start, b.f[d] + start, cnt * 4, hipMemcpyDeviceToDevice,
+                                        ctx->stream));
+            PIC_HIP(ctx, hipMemcpyAsync(a.v[d] + start, b.v[d] + start, cnt * 4, hipMemcpyDeviceToDevice,
+                                        ctx->stream));
+        }
+        PIC_HIP(ctx, hipMemcpyAsync(a.w + start, b.w + start, cnt * 4, hipMemcpyDeviceToDevice, ctx->stream));
+    }
+    s.two_part = true;
     s.hist_valid = false;
     return PIC_OK;
 }
@@ -790,7 +840,7 @@ extern "C" int pic_push_sort(pic_ctx* ctx, int sid, double dt) {
         k_push<false><<<xcd_blocks1(n, BLOCK * PUSH_ILP), BLOCK, 0, ctx->stream>>>(g, s.buf[s.cur], ctx->ef, a, s.d_n,
                                                                      s.count, s.rank);
     }
-    scan_counts(ctx, s);
+    scan_counts(ctx, s, s.off, true);
     // the counts are consumed by the scan: reuse them as the per-cell cursor
     PIC_HIP(ctx, hipMemsetAsync(s.count, 0, ((size_t)g.ncells + 1) * 4, ctx->stream));
     {
@@ -802,6 +852,8 @@ extern "C" int pic_push_sort(pic_ctx* ctx, int sid, double dt) {
     s.cur ^= 1;
     s.sorted = true;
     s.hist_valid = false;
+    s.two_part = false;
+    s.tail_ok = true;
     return PIC_OK;
 }
 
@@ -917,6 +969,8 @@ extern "C" int pic_load_box_thermal(pic_ctx* ctx, int sid, const double centre[3
     s.n_upper = base + num_macro;
     s.hist_valid = false;
     s.sorted = false;
+    s.two_part = false;
+    s.tail_ok = false;
     if (n_loaded) *n_loaded = num_macro;
     return PIC_OK;
 }

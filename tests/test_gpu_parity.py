@@ -398,6 +398,67 @@ def test_loader_matches_oracle_philox(pg, golden):
     assert np.max(np.abs(a[:, 3:6] - b[:, 3:6])) <= 1e-6 * np.max(np.abs(a[:, 3:6]))
 
 
+def _dsmc_box(pg, sigma_table_path, seed):
+    """a small box at the discharge workload's cell densities: 8 x 8 x 16
+    cells, ~40 O2 and ~5 electrons (10 eV) per cell"""
+    n, nz, dx = 9, 17, 1e-4
+    L = np.array([dx * (n - 1), dx * (n - 1), dx * (nz - 1)])
+    c = pg.Context(n, n, nz, [0, 0, 0], L)
+    box = 0.98 * L
+    vol = float(np.prod(box))
+    n_gas = 10.0 / (1.380648e-23 * 300.0)
+    Nn, Ne = 40 * 1024, 5 * 1024
+    w_n = n_gas * vol / Nn
+    w_e = w_n / 1000.0
+    sn = c.species_add(32 * O.AMU, 0.0, w_n, 3 * Nn)
+    si = c.species_add(32 * O.AMU, O.QE, w_e, Ne)
+    se = c.species_add(O.ME, -O.QE, w_e, 2 * Ne)
+    c.load_box_thermal(sn, 0.5 * L, box, n_gas, 300.0, seed=seed, mode=0)
+    c.load_box_thermal(se, 0.5 * L, box, Ne * w_e / vol, 10.0 * 11604.52, seed=seed + 1, mode=0)
+    sE, sV = O.load_sigma_table(sigma_table_path)
+    mid = c.mcc_create(sn, si, se, 12.07 * O.QE, sE, sV)
+    return c, g_of(n, nz, L), (sn, si, se), mid
+
+
+def g_of(n, nz, L):
+    return O.make_grid(n, n, nz, [0, 0, 0], L)
+
+
+def test_mcc_sorts_births_as_appended_part(pg, sigma_table_path):
+    """DSMC steps in a row: from the second step on, only the neutrals born
+    since their last whole sort are sorted (behind the sorted part).  The
+    stored neutrals are then two cell-sorted runs followed by the step's
+    births; neutral + ion and electron - ion weights are conserved; a run
+    with a whole sort before every step picks from the same per-cell sets
+    and gives the same collision statistics."""
+    dt = 2e-9
+    a, g, (sn, si, se), ma = _dsmc_box(pg, sigma_table_path, 41)
+    b, _, _, mb = _dsmc_box(pg, sigma_table_path, 41)
+
+    def wsum(c, sid):
+        return float(np.sum(_rows(c.download(sid))[:, 6]))
+
+    m0 = wsum(a, sn) + wsum(a, si)
+    q0 = wsum(a, se) - wsum(a, si)
+    coll_a = coll_b = 0
+    for step in range(6):
+        na = a.count(sn)
+        sa = a.mcc_apply(ma, dt, 3, step)
+        b.sort(sn)  # a whole sort before every step
+        sb = b.mcc_apply(mb, dt, 3, step)
+        if step == 0:  # the same per-cell sets (their order follows atomics): the same group counts
+            assert sa["groups"] == sb["groups"]
+        else:
+            coll_a += sa["collisions"]
+            coll_b += sb["collisions"]
+        cells = _cell_of(g, a.download(sn))[:na]
+        assert np.count_nonzero(np.diff(cells) < 0) <= (0 if step == 0 else 1), step
+    assert wsum(a, sn) + wsum(a, si) == pytest.approx(m0, rel=1e-6)
+    assert wsum(a, se) - wsum(a, si) == pytest.approx(q0, rel=1e-6)
+    assert coll_a > 1000
+    assert abs(coll_a - coll_b) <= 0.1 * coll_b, (coll_a, coll_b)
+
+
 def test_mcc_matches_oracle_philox(pg, golden, sigma_table_path):
     gr = golden["mcc_grid"]
     g = O.make_grid(*_grid_args(gr))
