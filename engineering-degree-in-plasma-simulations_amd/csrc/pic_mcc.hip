@@ -460,16 +460,20 @@ extern "C" int pic_mcc_apply(pic_ctx* ctx, int id, double dt, uint32_t seed, uin
                                                I.d_n, (uint32_t)I.cap, E.d_n, (uint32_t)E.cap);
     }
     PIC_HIP(ctx, hipGetLastError());
-    unsigned long long hs[5];
-    double hw[2];
-    uint32_t cn[4];
-    PIC_HIP(ctx, hipMemcpyAsync(hs, m.d_stats, sizeof(hs), hipMemcpyDeviceToHost, ctx->stream));
-    PIC_HIP(ctx, hipMemcpyAsync(hw, m.d_wsv, sizeof(hw), hipMemcpyDeviceToHost, ctx->stream));
+    // the step's results through pinned memory (asynchronous copies), one wait
+    rc = pic_ensure_pin(ctx, 128);
+    if (rc) return rc;
+    unsigned long long* hs = reinterpret_cast<unsigned long long*>(ctx->pin);  // 5 counters
+    double* hw = ctx->pin + 5;                                                 // 2 maxima
+    uint32_t* cn = reinterpret_cast<uint32_t*>(ctx->pin + 7);                 // 4 counts
+    PIC_HIP(ctx, hipMemcpyAsync(hs, m.d_stats, 5 * sizeof(unsigned long long), hipMemcpyDeviceToHost,
+                                ctx->stream));
+    PIC_HIP(ctx, hipMemcpyAsync(hw, m.d_wsv, 2 * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
     PIC_HIP(ctx, hipMemcpyAsync(&cn[0], N.d_n, 4, hipMemcpyDeviceToHost, ctx->stream));
     PIC_HIP(ctx, hipMemcpyAsync(&cn[1], I.d_n, 4, hipMemcpyDeviceToHost, ctx->stream));
     PIC_HIP(ctx, hipMemcpyAsync(&cn[2], E.d_n, 4, hipMemcpyDeviceToHost, ctx->stream));
     PIC_HIP(ctx, hipMemcpyAsync(&cn[3], N.d_n + 3, 4, hipMemcpyDeviceToHost, ctx->stream));
-    PIC_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    PIC_HIP(ctx, pic_spin_wait(ctx->stream));
     N.bulk_n = cn[3];  // the sorted part (the live count of the last whole sort)
     N.n_upper = cn[0];
     I.n_upper = cn[1];

@@ -608,6 +608,20 @@ __global__ void __launch_bounds__(BLOCK)
     }
 }
 
+// particles [start, start + cnt) of one buffer set to the other (all eight arrays)
+__global__ void __launch_bounds__(BLOCK) k_copy_part(SpeciesBuf src, SpeciesBuf dst, size_t start, size_t cnt) {
+    const size_t i = (size_t)blockIdx.x * BLOCK + threadIdx.x;
+    if (i >= cnt) return;
+    const size_t p = start + i;
+    dst.cell[p] = src.cell[p];
+#pragma unroll
+    for (int d = 0; d < 3; d++) {
+        dst.f[d][p] = src.f[d][p];
+        dst.v[d][p] = src.v[d][p];
+    }
+    dst.w[p] = src.w[p];
+}
+
 struct RhoArgs {
     const double* den[MAX_SPECIES];
     double q[MAX_SPECIES];
@@ -806,15 +820,9 @@ int pic_sort_tail(pic_ctx* ctx, int sid) {
     }
     PIC_HIP(ctx, hipGetLastError());
     if (cnt) {  // the sorted appended part back behind the sorted part
-        const SpeciesBuf &a = s.buf[s.cur], &b = s.buf[s.cur ^ 1];
-        PIC_HIP(ctx, hipMemcpyAsync(a.cell + start, b.cell + start, cnt * 4, hipMemcpyDeviceToDevice, ctx->stream));
-        for (int d = 0; d < 3; d++) {
-            PIC_HIP(ctx, hipMemcpyAsync(a.f[d] + start, b.f[d] + start, cnt * 4, hipMemcpyDeviceToDevice,
-                                        ctx->stream));
-            PIC_HIP(ctx, hipMemcpyAsync(a.v[d] + start, b.v[d] + start, cnt * 4, hipMemcpyDeviceToDevice,
-                                        ctx->stream));
-        }
-        PIC_HIP(ctx, hipMemcpyAsync(a.w + start, b.w + start, cnt * 4, hipMemcpyDeviceToDevice, ctx->stream));
+        PicProfScope ps(ctx, PIC_K_SCATTER);
+        k_copy_part<<<grid_for(cnt, BLOCK), BLOCK, 0, ctx->stream>>>(s.buf[s.cur ^ 1], s.buf[s.cur], start, cnt);
+        PIC_HIP(ctx, hipGetLastError());
     }
     s.two_part = true;
     s.hist_valid = false;
