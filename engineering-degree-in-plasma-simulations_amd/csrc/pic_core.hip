@@ -387,7 +387,8 @@ extern "C" int pic_species_add(pic_ctx* ctx, double mass, double charge, double 
         }
         s.buf[b].w = (float*)(base + 7 * stride);
     }
-    if (e == hipSuccess) e = hipMalloc(&s.rank, per);
+    // ranks; pic_push_sort keeps 64 run bases per group of 64 particles here
+    if (e == hipSuccess) e = hipMalloc(&s.rank, (c + 63) / 64 * 64 * 4);
     if (e == hipSuccess) e = hipMalloc(&s.count, ((size_t)g.ncells + 2) * 4);
     if (e == hipSuccess) e = hipMalloc(&s.off, ((size_t)g.ncells + 2) * 4);
     if (e == hipSuccess) e = hipMalloc(&s.d_n, 4 * sizeof(uint32_t));

@@ -459,6 +459,42 @@ def test_mcc_sorts_births_as_appended_part(pg, sigma_table_path):
     assert abs(coll_a - coll_b) <= 0.1 * coll_b, (coll_a, coll_b)
 
 
+def test_mcc_step_without_neutral_births(pg, sigma_table_path):
+    """Electrons individually heavier than the neutrals, too slow to ionize:
+    every collision takes the electron-heavier branch (Interactions.cpp:
+    705-726), which splits the electron and leaves the neutrals' count alone.
+    The next step then finds the neutrals unsorted with nothing appended,
+    and sorts nothing for them."""
+    n, dx = 9, 1e-4
+    L = dx * (n - 1)
+    c = pg.Context(n, n, n, [0, 0, 0], [L, L, L])
+    rng = np.random.default_rng(8)
+
+    def parts(count, vth, w):
+        p = np.zeros((count, 7))
+        p[:, :3] = rng.uniform(0.01 * L, 0.99 * L, (count, 3))
+        p[:, 3:6] = rng.normal(0, vth, (count, 3))
+        p[:, 6] = w
+        return p.reshape(-1)
+
+    sn = c.species_add(16 * O.AMU, 0.0, 1e4, 20000)
+    si = c.species_add(16 * O.AMU, O.QE, 1.0, 1000)
+    se = c.species_add(O.ME, -O.QE, 1.0, 20000)
+    c.upload(sn, parts(30 * 512, 300.0, 1.0))
+    c.upload(se, parts(10 * 512, 4e5, 10.0))  # below 1 eV: no ionization
+    sE, sV = O.load_sigma_table(sigma_table_path)
+    mid = c.mcc_create(sn, si, se, 12.07 * O.QE, sE, sV)
+    n_neu = c.count(sn)
+    s0 = c.mcc_apply(mid, 1e-4, 5, 0)
+    assert s0["collisions"] > 0 and s0["ionizations"] == 0
+    assert c.count(sn) == n_neu and c.count(se) > 10 * 512  # electron splits only
+    # W sigma v_max is now the first step's (far below its starting value):
+    # a longer step gives the second one NTC groups again
+    s1 = c.mcc_apply(mid, 50.0, 5, 1)
+    assert s1["groups"] > 0
+    assert c.count(sn) == n_neu and c.count(si) == 0
+
+
 def test_mcc_matches_oracle_philox(pg, golden, sigma_table_path):
     gr = golden["mcc_grid"]
     g = O.make_grid(*_grid_args(gr))
