@@ -402,12 +402,6 @@ __global__ void __launch_bounds__(BLOCK)
 
 // Second pass of pic_push_sort: every particle is pushed again from the
 // unchanged source arrays (the same arithmetic as the first pass, so the
-// same new cell) and written straight to its sorted position
-// off[new cell] + its slot in the cell, taken from the per-cell cursor
-// (zeroed; wave-aggregated atomics as in the histogram); removed particles
-// are dropped.
-// Second pass of pic_push_sort: every particle is pushed again from the
-// unchanged source arrays (the same arithmetic as the first pass, so the
 // same new cell) and written straight to its sorted position off[new cell] +
 // its slot in the cell.  The slot comes from the first pass's run bases of
 // the particle's group of 64 (same groups, same keys, same runs): no atomics.
