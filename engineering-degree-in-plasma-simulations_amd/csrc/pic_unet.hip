@@ -639,19 +639,47 @@ __global__ void __launch_bounds__(256) k_conv3_small(ConvArgs a) {
 #pragma unroll
     for (int w2 = 0; w2 < 4; w2++) s4 += red[(w2 * 4 + wave) * 64 + lane];
     const int gy = y0 + wave % TY, gz = z0 + wave / TY;
-    if (gy >= d.ny || gz >= d.nz) return;
+    const bool row_in = gy < d.ny && gz < d.nz;
     const int n = nb0 * 16 + m;
     const float bv = a.bias ? a.bias[n] : 0.f;
     // D fragment: lane holds rows 4g + j (x offset) of column m (channel)
+    float4 stored = make_float4(0.f, 0.f, 0.f, 0.f);  // the BF16 outputs, as floats
 #pragma unroll
     for (int j = 0; j < 4; j++) {
         int gx = x0 + 4 * g + j;
-        if (gx >= d.nx) continue;
+        if (!row_in || gx >= d.nx) continue;
         long long o = (((long long)gz * d.ny + gy) * d.nx + gx) * a.cout + n;
         float v = s4[j] + bv;
         if (a.relu) v = fmaxf(v, 0.f);
         if (a.mask && !(bf2f(a.mask[o]) > 0.f)) v = 0.f;
-        a.out[o] = f2bf(v);
+        const bf16 vb = f2bf(v);
+        a.out[o] = vb;
+        (&stored.x)[j] = bf2f(vb);
+    }
+    if (a.pool) {
+        // the 2x2x2 means of the stored outputs (the caller asks only for
+        // even extents and nx a multiple of 16): the tile's four rows are
+        // one 2x2 of (y, z), each lane holds two x pairs; children summed in
+        // k_pool's order (x, then y, then z)
+        __syncthreads();  // the reduction buffer is free
+        float4* pr = reinterpret_cast<float4*>(smem);  // [wave][lane]
+        pr[wave * 64 + lane] = stored;
+        __syncthreads();
+        if (wave == 0) {
+            const int hy = d.ny / 2, hx = d.nx / 2;
+#pragma unroll
+            for (int px = 0; px < 2; px++) {
+                float sum = 0.f;
+#pragma unroll
+                for (int q = 0; q < 8; q++) {
+                    const int dx = q & 1, dy = (q >> 1) & 1, dz = q >> 2;
+                    const float4 t = pr[(dy + 2 * dz) * 64 + lane];
+                    sum += (&t.x)[2 * px + dx];
+                }
+                const int cx = (x0 + 4 * g) / 2 + px, cy = y0 / 2, cz = z0 / 2;
+                a.pool[(((long long)cz * hy + cy) * hx + cx) * a.cout + n] = f2bf(0.125f * sum);
+            }
+        }
     }
 }
 
@@ -1857,10 +1885,17 @@ static bool conv(pic_ctx* ctx, const Dims& d, const bf16* in, int cin, const bf1
         }
         if (tiles * (cout / 16) < 512) {  // small grid: 16x2x2 tiles, taps split over the waves
             int t4 = ntx * ((d.ny + 1) / 2) * ((d.nz + 1) / 2);
-            if (cin == 64)
+            // the pooled output too where a tile is whole pooling cells
+            const bool pooling = pool && (cin == 64 || cin == 32) && d.nx % 16 == 0 && d.ny % 2 == 0 &&
+                                 d.nz % 2 == 0;
+            a.pool = pooling ? pool : nullptr;
+            if (cin == 64) {
                 k_conv3_small<64><<<dim3(t4, cout / 16), 256, 0, ctx->stream>>>(a);
-            else if (cin == 32)
+                return pooling;
+            } else if (cin == 32) {
                 k_conv3_small<32><<<dim3(t4, cout / 16), 256, 0, ctx->stream>>>(a);
+                return pooling;
+            }
             else if (pf)
                 k_conv3<1, 4, true><<<dim3(t4, cout / 16), 256, 0, ctx->stream>>>(a);
             else

@@ -115,7 +115,8 @@ def _bf16(a):
 
 @pytest.mark.gpu
 @pytest.mark.parametrize("channels,n", [(C, 16), ((16, 32, 32, 32), 16), ((32, 32), 8),
-                                        ((16, 64, 64), 16)])
+                                        ((16, 64, 64), 16),
+                                        ((16, 32, 32), 32)])  # level 1 at 16^3 pools in its conv
 def test_gpu_unet_forward_and_gradient_match_oracle(channels, n):
     import pic_gpu as pg
 
