@@ -52,7 +52,7 @@ ALG_BYTES = {
 # p, q from z and the previous p (32 B), then x, r updated from p, q (48 B)
 ALG_BYTES_FCG = 80.0
 KERNEL_NAMES = {
-    "push": "k_push<false>", "scatter": "k_push_scatter", "deposit": "k_dep_cells + k_dep_nodes",
+    "push": "k_push<false,2>", "scatter": "k_push_scatter", "deposit": "k_dep_cells + k_dep_nodes",
     "pcg": "k_pcg1<float> + k_pcg2<float>", "sor": "k_sor",
 }
 
