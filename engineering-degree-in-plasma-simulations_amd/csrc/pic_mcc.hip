@@ -255,13 +255,10 @@ __global__ void __launch_bounds__(BLOCK)
                 double wn = N.b.w[pn], we = E.b.w[pe];
                 double vn[3] = {N.b.v[0][pn], N.b.v[1][pn], N.b.v[2][pn]};
                 double ve[3] = {E.b.v[0][pe], E.b.v[1][pe], E.b.v[2][pe]};
-                // the neutral's cell and offsets (its births' position) too
-                const uint32_t ncell = N.b.cell[pn];
-                const float nfx = N.b.f[0][pn], nfy = N.b.f[1][pn], nfz = N.b.f[2][pn];
                 // (an empty asm that takes them all: otherwise the compiler
                 // tests wn first and waits for memory more than once)
                 asm volatile("" ::"v"(wn), "v"(we), "v"(vn[0]), "v"(vn[1]), "v"(vn[2]), "v"(ve[0]), "v"(ve[1]),
-                             "v"(ve[2]), "v"(ncell), "v"(nfx), "v"(nfy), "v"(nfz));
+                             "v"(ve[2]));
                 double vr[3] = {vn[0] - ve[0], vn[1] - ve[1], vn[2] - ve[2]};
                 double v_rel = sqrt(vr[0] * vr[0] + vr[1] * vr[1] + vr[2] * vr[2]);
                 if (wn == 0.0) continue;  // neutral removed earlier in this cell
@@ -273,6 +270,11 @@ __global__ void __launch_bounds__(BLOCK)
                 double P = wsv_g / wmax;
                 if (!(dr(2) < P)) continue;
                 ncoll++;
+                // the neutral's cell and offsets (its births' position), read
+                // only for accepted groups: random picks from the SoA arrays
+                // fetch a whole line per field, and most groups are rejected
+                const uint32_t ncell = N.b.cell[pn];
+                const float nfx = N.b.f[0][pn], nfy = N.b.f[1][pn], nfz = N.b.f[2][pn];
                 double vnew[3] = {0, 0, 0};
                 if (wn > we) {  // Interactions.cpp:684-703
                     N.b.w[pn] = (float)(wn - we);
