@@ -57,6 +57,16 @@ KERNEL_NAMES = {
 }
 
 
+
+def measured_traffic(kernel):
+    """HBM bytes per launch of a kernel from the committed PMC summary
+    (profiles/pmc_traffic.json: 2 x FETCH_SIZE + 1 x WRITE_SIZE, calibrated
+    on a device copy), or None where it was not measured."""
+    tpath = os.path.join(ROOT, "profiles", "pmc_traffic.json")
+    if not os.path.exists(tpath):
+        return None
+    return json.load(open(tpath)).get(kernel, {}).get("bytes_per_launch")
+
 def unet_flops(channels, coarsen, n):
     """algorithmic FLOPs of one U-Net forward (2 per multiply-add; every
     convolution at its true channel counts, no padding) on the network grid
@@ -188,7 +198,7 @@ def dsmc_bench(pg, args, cpu):
     alg = 8.0 * ncells + 32.0 * st["groups"] + 48.0 * st["collisions"] + 32.0 * st["ionizations"]
     ach = alg / (mcc_ms * 1e-3) / 1e9
     roof_mcc = {"bound": "hbm", "kernel": "k_mcc", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": ach / HBM_PEAK_GBS, "traffic": None, "alg_bytes_per_launch": alg,
+                "frac": ach / HBM_PEAK_GBS, "traffic": measured_traffic("k_mcc"), "alg_bytes_per_launch": alg,
                 "avg_launch_ms": mcc_ms}
     # the sorts (the neutrals' appended births, or the whole species, and the
     # electrons): their time only -- how many particles a sort moved is not
@@ -599,11 +609,7 @@ def main():
                     "achieved": ach, "peak": MFMA_BF16_PEAK_TFLOPS, "unit": "TFLOP/s",
                     "frac": ach / MFMA_BF16_PEAK_TFLOPS, "traffic": None,
                     "alg_flops_per_forward": fl, "avg_forward_ms": fwd_s * 1e3}
-    traffic = None
-    tpath = os.path.join(ROOT, "profiles", "pmc_traffic.json")
-    if os.path.exists(tpath):
-        t = json.load(open(tpath))
-        traffic = t.get(dom, {}).get("bytes_per_launch")
+    traffic = measured_traffic(dom)
 
     line = {
         "metric": "particle pushes/s + time-to-1e-6 Poisson residual, learned vs plain, 1 GPU",
