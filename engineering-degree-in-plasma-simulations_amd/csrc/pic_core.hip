@@ -240,8 +240,8 @@ static void rebuild_rect_logical(pic_ctx* ctx) {
     for (int r = 0; r < g.nrect; r++)
         for (int d = 0; d < 3; d++) {
             const auto& R = ctx->rects[r];
-            g.rlo[r][d] = (float)((R.c[d] - R.half[d] - g.x0[d]) * g.inv_dx[d]);
-            g.rhi[r][d] = (float)((R.c[d] + R.half[d] - g.x0[d]) * g.inv_dx[d]);
+            g.rlo[r][d] = (R.c[d] - R.half[d] - g.x0[d]) * g.inv_dx[d];
+            g.rhi[r][d] = (R.c[d] + R.half[d] - g.x0[d]) * g.inv_dx[d];
         }
 }
 
@@ -430,8 +430,7 @@ __device__ inline uint32_t locate(const GridC& g, const double* x, float* f) {
     for (int r = 0; r < g.nrect; r++) {
         bool in = true;
         for (int d = 0; d < 3; d++) {
-            float l = (float)lc[d];
-            if (l < g.rlo[r][d] || l > g.rhi[r][d]) in = false;
+            if (lc[d] < g.rlo[r][d] || lc[d] > g.rhi[r][d]) in = false;
         }
         if (in) return DEAD;
     }

@@ -52,7 +52,9 @@ struct GridC {
     long long nn;
     double x0[3], dx[3], inv_dx[3];
     int nrect;
-    float rlo[MAX_RECT][3], rhi[MAX_RECT][3];  // rectangles in logical coordinates
+    // rectangles in logical coordinates (FP64: containment is decided as the
+    // reference does, in FP64, Object.cpp:231-238)
+    double rlo[MAX_RECT][3], rhi[MAX_RECT][3];
 };
 
 __host__ __device__ inline uint32_t pack_cell(int i, int j, int k) {

@@ -261,7 +261,7 @@ __global__ void __launch_bounds__(BLOCK)
                              "v"(ve[2]));
                 double vr[3] = {vn[0] - ve[0], vn[1] - ve[1], vn[2] - ve[2]};
                 double v_rel = sqrt(vr[0] * vr[0] + vr[1] * vr[1] + vr[2] * vr[2]);
-                if (wn == 0.0) continue;  // neutral removed earlier in this cell
+                if (wn == 0.0) continue;  // a weightless neutral (uploaded so) never collides
                 double E_rel = a.E_pre_rel * v_rel * v_rel;
                 double sigma = sigma_el(tE, tV, a.nsig, E_rel);
                 double Wg = wn > we ? wn : we;
@@ -295,8 +295,26 @@ __global__ void __launch_bounds__(BLOCK)
                         niz++;
                         append(births, 1, I, ncell, nfx, nfy, nfz, vn, wn, &stats[3]);
                         append(births, 2, E, ncell, nfx, nfy, nfz, vnew, wn, &stats[3]);
-                        N.b.w[pn] = 0.0f;
-                        N.b.cell[pn] = DEAD;
+                        // the ionised neutral leaves the cell's pool as in
+                        // the reference (Interactions.cpp:720-721): the
+                        // pool's last neutral takes its slot and the pool
+                        // shrinks by one.  This thread owns the cell's
+                        // ranges, so the move is a plain copy; the vacated
+                        // last slot is dropped at the next whole sort.
+                        const int last = Nn - 1;
+                        const uint32_t pl = last < Nb ? n0 + (uint32_t)last : t0 + (uint32_t)(last - Nb);
+                        if (pl != pn) {
+                            N.b.cell[pn] = N.b.cell[pl];
+#pragma unroll
+                            for (int d = 0; d < 3; d++) {
+                                N.b.f[d][pn] = N.b.f[d][pl];
+                                N.b.v[d][pn] = N.b.v[d][pl];
+                            }
+                            N.b.w[pn] = N.b.w[pl];
+                        }
+                        N.b.w[pl] = 0.0f;
+                        N.b.cell[pl] = DEAD;
+                        Nn--;
                         ndead++;
                     }
                 }

@@ -119,8 +119,10 @@ __device__ inline int relocate(float& f, int i) {
     return i + (int)o;
 }
 
+// Rectangle::inObject (Object.cpp:231-238): inclusive box, decided in FP64
+// on the exact logical coordinate i + f
 __device__ inline bool in_rect(const GridC& g, int i, int j, int k, float fx, float fy, float fz) {
-    float l[3] = {(float)i + fx, (float)j + fy, (float)k + fz};
+    const double l[3] = {(double)i + (double)fx, (double)j + (double)fy, (double)k + (double)fz};
     for (int r = 0; r < g.nrect; r++) {
         bool in = true;
 #pragma unroll

@@ -1129,7 +1129,20 @@ static int solve_learned(pic_ctx* ctx, const pic_solver_params* p, pic_solver_st
             std::swap(pold, pnew);
         }
     }
-    if (!conv) {  // plain PCG is the fallback
+    double rn[3];
+    if (conv) {
+        if (have_final) {
+            for (int q2 = 0; q2 < 3; q2++) rn[q2] = shost(hp + (3 + q2) * SSIZE);
+        } else {
+            rc = residual_norms(ctx, s, plain_args(s), rn);
+            if (rc) return rc;
+        }
+        // the recursive residual history decided convergence; the FP64
+        // residual of the final phi is the judge (CG residual norms are not
+        // monotone, and the recursion drifts from the true residual)
+        if (!(crit_value(p, rn, (double)nn) <= p->tol)) conv = false;
+    }
+    if (!conv) {  // plain PCG is the fallback, from the current phi
         pic_solver_params q2 = *p;
         q2.method = PIC_SOLVER_PCG;
         pic_solver_stats s2;
@@ -1137,13 +1150,6 @@ static int solve_learned(pic_ctx* ctx, const pic_solver_params* p, pic_solver_st
         *st = s2;
         st->iterations += it;
         return rc;
-    }
-    double rn[3];
-    if (have_final) {
-        for (int q2 = 0; q2 < 3; q2++) rn[q2] = shost(hp + (3 + q2) * SSIZE);
-    } else {
-        rc = residual_norms(ctx, s, plain_args(s), rn);
-        if (rc) return rc;
     }
     st->converged = 1;
     st->iterations = it;

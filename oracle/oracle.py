@@ -75,7 +75,7 @@ def lib():
                                R, C.c_int, C.c_int]
         L.orc_push.restype = C.c_size_t
         L.orc_push_threaded.argtypes = [G, _dp, _dp, C.c_size_t, C.c_double, C.c_double,
-                                        C.c_double, C.c_int]
+                                        C.c_double, C.c_int, C.c_int]
         L.orc_push_threaded.restype = C.c_size_t
         L.orc_number_density.argtypes = [G, _dp, C.c_size_t, _dp, _dp]
         L.orc_charge_accumulate.argtypes = [C.c_size_t, _dp, C.c_double, _dp]
@@ -124,6 +124,15 @@ def lib():
                                     C.POINTER(Rng), _dp, C.POINTER(C.c_size_t), C.c_size_t,
                                     _dp, C.POINTER(C.c_size_t), C.c_size_t, _dp,
                                     C.POINTER(C.c_size_t), C.c_size_t, C.POINTER(MccStats)]
+        _i64 = np.ctypeslib.ndpointer(dtype=np.int64, flags="C_CONTIGUOUS")
+        L.orc_rect_line_intersect.argtypes = [R, _dp, _dp, C.POINTER(C.c_double), _dp, _dp]
+        L.orc_push_nonelectron.argtypes = [G, _dp, _dp, C.c_size_t, C.c_double, C.c_double,
+                                           C.c_double, R, C.c_int, _dp, C.POINTER(C.c_size_t),
+                                           C.c_size_t, C.c_double, C.POINTER(Rng), C.c_int,
+                                           C.c_int, _i64]
+        L.orc_push_nonelectron.restype = C.c_size_t
+        L.orc_merge.argtypes = [G, C.c_void_p, _dp, C.POINTER(C.c_size_t), C.c_size_t,
+                                C.c_double, C.c_double, C.c_double, C.POINTER(Rng), C.c_int, _i64]
         _lib = L
     return _lib
 
@@ -179,9 +188,11 @@ def push(g, ef, parts, charge, mass, dt, rects=None, stable=False):
     return p[: 7 * n]
 
 
-def push_threaded(g, ef, parts, charge, mass, dt, nthreads):
+def push_threaded(g, ef, parts, charge, mass, dt, nthreads, remove=False):
+    """the reference's threaded electron push in place; returns the count
+    (remove=False: as the reference, which removes nothing)"""
     n = lib().orc_push_threaded(C.byref(g), ef, parts, parts.size // 7, charge, mass, dt,
-                                nthreads)
+                                nthreads, int(remove))
     return n
 
 
@@ -368,3 +379,56 @@ def mcc_apply(g, params, wsv_max, rng, neu, ion, ele, extra=None):
                              C.byref(ce), ne_ + extra, C.byref(st))
     assert rc == 0, "oracle capacity exceeded"
     return bn[: 7 * cn.value], bi[: 7 * ci.value], be[: 7 * ce.value], w.value, st
+
+
+def rect_line_intersect(rect, x1, x2):
+    """Rectangle::lineIntersect: (t_entry, entry point, normal)"""
+    ra, _ = rects_arg([rect])
+    t = C.c_double()
+    pos = np.zeros(3)
+    n = np.zeros(3)
+    lib().orc_rect_line_intersect(ra, np.asarray(x1, np.float64), np.asarray(x2, np.float64),
+                                  C.byref(t), pos, n)
+    return t.value, pos, n
+
+
+def push_nonelectron(g, ef, parts, charge, mass, dt, rects, mpw0_n, rng, neu=None,
+                     accept_on_surface=False, stable=False, extra=None):
+    """advanceNonElectron with object surfaces; returns (survivors, neutrals
+    with the emitted ones appended, stats[hits, emitted, rejected, removed])"""
+    p = np.ascontiguousarray(parts, np.float64).copy()
+    neu = np.zeros(0) if neu is None else np.ascontiguousarray(neu, np.float64)
+    n0 = neu.size // 7
+    if extra is None:
+        extra = 8 * (p.size // 7) + 64
+    buf = np.zeros(7 * (n0 + extra))
+    buf[: neu.size] = neu
+    cn = C.c_size_t(n0)
+    ra, nr = rects_arg(rects)
+    st = np.zeros(4, np.int64)
+    n = lib().orc_push_nonelectron(C.byref(g), np.ascontiguousarray(ef, np.float64), p,
+                                   p.size // 7, charge, mass, dt, ra, nr, buf, C.byref(cn),
+                                   n0 + extra, mpw0_n, C.byref(rng), int(accept_on_surface),
+                                   int(stable), st)
+    assert n != C.c_size_t(-1).value, "oracle neutral capacity exceeded"
+    return p[: 7 * n], buf[: 7 * cn.value], st
+
+
+def merge(g, parts, charge, mass, dt, rng, ef=None, gpu_mode=False, extra=None):
+    """Species::merge; returns (particles, stats[bins merged, particles in them])"""
+    p = np.ascontiguousarray(parts, np.float64)
+    n0 = p.size // 7
+    if extra is None:
+        extra = n0 + 64
+    buf = np.zeros(7 * (n0 + extra))
+    buf[: p.size] = p
+    cn = C.c_size_t(n0)
+    efp = None
+    if ef is not None:
+        ef = np.ascontiguousarray(ef, np.float64)
+        efp = ef.ctypes.data_as(C.c_void_p)
+    st = np.zeros(2, np.int64)
+    rc = lib().orc_merge(C.byref(g), efp, buf, C.byref(cn), n0 + extra, charge, mass, dt,
+                         C.byref(rng), int(gpu_mode), st)
+    assert rc == 0, "oracle merge capacity exceeded"
+    return buf[: 7 * cn.value], st

@@ -1284,8 +1284,14 @@ static int cmp_desc(const void* a, const void* b) {
     size_t x = *(const size_t*)a, y = *(const size_t*)b;
     return x < y ? 1 : (x > y ? -1 : 0);
 }
+/* v3/Species.cpp:263-310.  The per-thread deletion buffers reach the
+ * workers through ThreadPool::AddTask's std::bind (ThreadPool.h:63, :77-78),
+ * which stores copies: the workers fill the copies, the caller's buffers
+ * stay empty, and the reference removes nothing (remove == 0 restates
+ * that).  remove != 0 applies the removal the code intends: the buffers
+ * merged, sorted descending, each slot swapped with the last. */
 size_t orc_push_threaded(const orc_grid* g, const double* ef, double* parts, size_t n,
-                         double charge, double mass, double dt, int nthreads) {
+                         double charge, double mass, double dt, int nthreads, int remove) {
     if (nthreads < 1) nthreads = 1;
     push_task* tasks = (push_task*)calloc(nthreads, sizeof(push_task));
     pthread_t* th = (pthread_t*)calloc(nthreads, sizeof(pthread_t));
@@ -1300,7 +1306,7 @@ size_t orc_push_threaded(const orc_grid* g, const double* ef, double* parts, siz
     size_t ndel = 0;
     for (int t = 0; t < nthreads; t++) {
         pthread_join(th[t], NULL);
-        ndel += tasks[t].ndel;
+        if (remove) ndel += tasks[t].ndel;
     }
     if (ndel) {
         size_t* all = (size_t*)malloc(ndel * sizeof(size_t));
@@ -1319,4 +1325,386 @@ size_t orc_push_threaded(const orc_grid* g, const double* ef, double* parts, siz
     free(tasks);
     free(th);
     return n - ndel;
+}
+
+/* ------------------------------------------------------------------------ */
+/* object surfaces: v3/Object.cpp:271-317, v3/Species.cpp:170-256, :835-861  */
+/* ------------------------------------------------------------------------ */
+
+/* v3/Object.cpp:310-316 (Rectangle::find_n): n += outward normal of side s */
+static void rect_add_normal(int side, double* n) {
+    static const double nrm[6][3] = {{-1, 0, 0}, {0, -1, 0}, {0, 0, -1},
+                                     {1, 0, 0},  {0, 1, 0},  {0, 0, 1}};
+    for (int d = 0; d < 3; d++) n[d] += nrm[side][d];
+}
+
+/* v3/Object.cpp:271-308 (Rectangle::lineIntersect; x2 inside the box).
+ * Entry time from the slab method, the entry side's outward normal; within
+ * 1e-6 of another side's time the normals are summed and normalised. */
+void orc_rect_line_intersect(const orc_rect* r, const double* x1, const double* x2, double* t_entry,
+                             double* pos, double* n) {
+    double A[3], tmin[3], tmax[3];
+    int side[3] = {0, 1, 2};
+    for (int d = 0; d < 3; d++) {
+        double xmin = r->c[d] - r->half[d], xmax = r->c[d] + r->half[d]; /* Object.cpp:205-206 */
+        A[d] = x2[d] - x1[d];
+        tmin[d] = (xmin - x1[d]) / A[d];
+        tmax[d] = (xmax - x1[d]) / A[d];
+    }
+    for (int d = 0; d < 3; d++)
+        if (tmin[d] > tmax[d]) {
+            double t = tmin[d];
+            tmin[d] = tmax[d];
+            tmax[d] = t;
+            side[d] += 3;
+        }
+    double te = tmin[0];
+    int j = 0, k = 0, t = 0;
+    for (int i = 0; i < 2; i++) {
+        if (te < tmin[i + 1]) {
+            te = tmin[i + 1];
+            j = i + 1;
+            k = 0;
+        } else if (fabs(te - tmin[i + 1]) < 1e-6) {
+            k++;
+            t = i + 1;
+        }
+    }
+    n[0] = n[1] = n[2] = 0;
+    rect_add_normal(side[j], n);
+    if (k != 0) {
+        if (k < 2) {
+            rect_add_normal(side[t], n);
+        } else {
+            rect_add_normal(side[2], n);
+            rect_add_normal(side[1], n);
+        }
+        double len = sqrt(n[0] * n[0] + n[1] * n[1] + n[2] * n[2]); /* Vec3.h:102-108 */
+        double inv = 1.0 / len;
+        for (int d = 0; d < 3; d++) n[d] *= inv;
+    }
+    *t_entry = te;
+    for (int d = 0; d < 3; d++) pos[d] = x1[d] + te * (x2[d] - x1[d]);
+}
+
+/* v3/Species.cpp:855-861 (sampleVth) then :835-853 (sampleReflectedVelocity):
+ * 300 K wall, full accommodation, the diffuse direction built from
+ * t1 = n x e_x (n x e_y when n . e_x == 0) and t2 = n x t1 as in the source
+ * (not normalised; sin(psi) t2 carries no sin(theta)).  Draw slots s0 ..
+ * s0 + 10: nine for the thermal speed, then sin(theta), then psi. */
+static void reflect_velocity(double mass, double vmag1, const double* n, orc_rng* rng, int s0,
+                             double* out) {
+    double vth0 = sqrt(2 * ORC_KB * 300.0 / mass);
+    double v[3];
+    for (int c = 0; c < 3; c++) {
+        double u1 = orc_rng_draw(rng, s0 + 3 * c);
+        double u2 = orc_rng_draw(rng, s0 + 3 * c + 1);
+        double u3 = orc_rng_draw(rng, s0 + 3 * c + 2);
+        v[c] = vth0 * (u1 + u2 + u3 - 1.5);
+    }
+    double v_th = sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
+    double vmag2 = vmag1 + 1.0 * (v_th - vmag1);
+    double sin_theta = orc_rng_draw(rng, s0 + 9);
+    double cos_theta = sqrt(1 - sin_theta * sin_theta);
+    double psi = 2 * ORC_PI * orc_rng_draw(rng, s0 + 10);
+    static const double ex[3] = {1, 0, 0}, ey[3] = {0, 1, 0};
+    double t1[3], t2[3];
+    if (n[0] * 1.0 + n[1] * 0.0 + n[2] * 0.0 != 0)
+        cross3(n, ex, t1);
+    else
+        cross3(n, ey, t1);
+    cross3(n, t1, t2);
+    double a = sin_theta * cos(psi), b = sin(psi);
+    for (int d = 0; d < 3; d++) out[d] = ((t1[d] * a + t2[d] * b) + n[d] * cos_theta) * vmag2;
+}
+
+/* v3/Species.cpp:170-256 (advanceNoSputteringSerial) for a species of the
+ * given charge: kick, then the drift loop -- leaving the box removes; ending
+ * in an object (the first containing the point) moves the particle to the
+ * entry point; a neutral (charge 0) reflects diffusely and drifts on for the
+ * remaining fraction of the step, an ion is removed and emits
+ * int(w / mpw0_n + U) neutrals there (addParticle rejects a point that still
+ * tests inside the object unless accept_on_surface); more than 20 bounces
+ * remove.  Draws per (particle, bounce): slot 0 the emission count, then 11
+ * per sampled velocity (reflect_velocity); in Philox mode the counter words
+ * are (input index, bounce).  stable != 0 keeps the survivors' order (and
+ * the input index equals the position); otherwise the reference's
+ * swap-with-last removal.  stats (may be NULL): {object hits, neutrals
+ * emitted, emissions rejected, particles removed}.  Returns survivors, or
+ * (size_t)-1 when the neutral capacity is exceeded. */
+size_t orc_push_nonelectron(const orc_grid* g, const double* ef, double* parts, size_t n,
+                            double charge, double mass, double dt, const orc_rect* rects, int nrect,
+                            double* neu, size_t* n_neu, size_t cap_neu, double mpw0_n,
+                            orc_rng* rng, int accept_on_surface, int stable, long long* stats) {
+    double qmdt = dt * charge / mass;
+    long long st[4] = {0, 0, 0, 0};
+    size_t np = n, out = 0;
+    for (size_t p = 0; p < np; p++) {
+        double* q = parts + 7 * p;
+        double lc[3], e[3];
+        xtol(g, q, lc);
+        orc_gather3(g, ef, lc, e);
+        for (int d = 0; d < 3; d++) q[3 + d] += e[d] * qmdt;
+        double t_rem = 1;
+        int nb = 0, alive = 1;
+        while (t_rem > 0) {
+            if (++nb > 20) {
+                alive = 0;
+                break;
+            }
+            rng->a = (uint32_t)p;
+            rng->b = (uint32_t)nb;
+            double old[3] = {q[0], q[1], q[2]};
+            for (int d = 0; d < 3; d++) q[d] += q[3 + d] * t_rem * dt;
+            int obj = orc_in_object(rects, nrect, q);
+            if (!orc_in_bounds(g, q)) {
+                alive = 0;
+                break;
+            }
+            if (obj) {
+                double tp, nrm[3];
+                st[0]++;
+                orc_rect_line_intersect(&rects[obj - 1], old, q, &tp, q, nrm);
+                double vm = sqrt(q[3] * q[3] + q[4] * q[4] + q[5] * q[5]);
+                if (charge == 0) {
+                    reflect_velocity(mass, vm, nrm, rng, 0, q + 3);
+                } else {
+                    int mp = (int)(q[6] / mpw0_n + orc_rng_draw(rng, 0));
+                    for (int m = 0; m < mp; m++) {
+                        double v[3];
+                        reflect_velocity(mass, vm, nrm, rng, 1 + 11 * m, v);
+                        /* neutrals.addParticle (Species.cpp:420-434; no rewind at charge 0) */
+                        if (isnan(q[0]) || isnan(q[1]) || isnan(q[2]) || isnan(v[0]) ||
+                            isnan(v[1]) || isnan(v[2]) || !orc_in_bounds(g, q) ||
+                            (!accept_on_surface && orc_in_object(rects, nrect, q))) {
+                            st[2]++;
+                            continue;
+                        }
+                        if (*n_neu >= cap_neu) return (size_t)-1;
+                        double* r = neu + 7 * (*n_neu);
+                        for (int d = 0; d < 3; d++) {
+                            r[d] = q[d];
+                            r[3 + d] = v[d];
+                        }
+                        r[6] = mpw0_n;
+                        (*n_neu)++;
+                        st[1]++;
+                    }
+                    alive = 0;
+                    break;
+                }
+                t_rem *= (1 - tp);
+                continue;
+            }
+            t_rem = 0;
+        }
+        if (!alive) st[3]++;
+        if (stable) {
+            if (alive) {
+                if (out != p) memmove(parts + 7 * out, q, 7 * sizeof(double));
+                out++;
+            }
+        } else if (!alive) {
+            memcpy(q, parts + 7 * (np - 1), 7 * sizeof(double));
+            np--;
+            p--;
+        }
+    }
+    if (stats)
+        for (int s = 0; s < 4; s++) stats[s] = st[s];
+    return stable ? out : np;
+}
+
+/* ------------------------------------------------------------------------ */
+/* variable-weight merge: v3/Species.cpp:987-1145                            */
+/* ------------------------------------------------------------------------ */
+#define MERGE_BINS 15 /* m_vel_grid_n, Species.h:129 */
+
+typedef struct {
+    int bin; /* (i * 15 + j) * 15 + k */
+    int pos; /* place in the cell's list */
+} bin_item;
+static int cmp_bin_item(const void* a, const void* b) {
+    const bin_item *x = (const bin_item*)a, *y = (const bin_item*)b;
+    if (x->bin != y->bin) return x->bin < y->bin ? -1 : 1;
+    return x->pos < y->pos ? -1 : (x->pos > y->pos);
+}
+
+/* v3/Species.cpp:1017-1145.  gpu_mode 0 restates the reference: velocity
+ * range over the cell and particles[0] (:1020-1033), bins of width
+ * range / 15 after widening the maximum by 0.1% (:1043-1067), per bin of
+ * more than two particles the pair v = mean +- sqrt(variance) per axis with
+ * half the weight each (:1078-1103), positions of two distinct random
+ * members (:1127-1133), appended through addParticle (half-step rewind,
+ * :1141-1142), a negative variance below -1e-4 stops the clamping loop
+ * (:1091-1098), zero-weight particles swap-removed at the end (:1149-1156);
+ * mt19937 draws.  gpu_mode 1 is the device's defined variant: the range
+ * over the cell only, every negative variance clamped to 0, the pair
+ * written in place of the bin's first two members (in cell order) and the
+ * rest removed, Philox draws with counter words (cell, bin), survivors kept
+ * in order.  A bin index past 14 (a component equal to a zero maximum) or a
+ * NaN one (an all-zero component) is clamped into [0, 14] in both modes
+ * (undefined behaviour in the source).  parts must have room for cap
+ * particles.  stats (may be NULL): {bins merged, particles in them}.
+ * Returns 0, or -1 when the capacity is exceeded. */
+int orc_merge(const orc_grid* g, const double* ef, double* parts, size_t* n, size_t cap,
+              double charge, double mass, double dt, orc_rng* rng, int gpu_mode, long long* stats) {
+    size_t n0 = *n;
+    int nc = (g->ni - 1) * (g->nj - 1) * (g->nk - 1);
+    int32_t* off = (int32_t*)malloc((size_t)(nc + 1) * sizeof(int32_t));
+    int32_t* idx = (int32_t*)malloc((n0 + 1) * sizeof(int32_t));
+    orc_sort_indexes(g, parts, n0, off, idx);
+    long long st[2] = {0, 0};
+    int err = 0;
+    bin_item* items = (bin_item*)malloc((n0 + 1) * sizeof(bin_item));
+    for (int c = 0; c < nc && !err; c++) {
+        int cn = off[c + 1] - off[c];
+        if (cn < 10) continue;
+        const int32_t* cell = idx + off[c];
+        double vmin[3], vmax[3];
+        for (int d = 0; d < 3; d++)
+            vmin[d] = vmax[d] = gpu_mode ? parts[7 * (size_t)cell[0] + 3 + d] : parts[3 + d];
+        for (int q = 0; q < cn; q++) {
+            const double* v = parts + 7 * (size_t)cell[q] + 3;
+            for (int d = 0; d < 3; d++) {
+                if (v[d] < vmin[d])
+                    vmin[d] = v[d];
+                else if (v[d] > vmax[d])
+                    vmax[d] = v[d];
+            }
+        }
+        double dv[3];
+        for (int d = 0; d < 3; d++) {
+            if (vmax[d] > 0)
+                vmax[d] *= 1.001;
+            else
+                vmax[d] *= 0.999;
+            dv[d] = (vmax[d] - vmin[d]) / (double)MERGE_BINS;
+        }
+        for (int q = 0; q < cn; q++) {
+            const double* v = parts + 7 * (size_t)cell[q] + 3;
+            int b[3];
+            for (int d = 0; d < 3; d++) {
+                double x = (v[d] - vmin[d]) / dv[d];
+                int bi = x == x ? (x < (double)MERGE_BINS ? (int)x : MERGE_BINS - 1) : 0;
+                b[d] = bi < 0 ? 0 : bi;
+            }
+            items[q].bin = (b[0] * MERGE_BINS + b[1]) * MERGE_BINS + b[2];
+            items[q].pos = q;
+        }
+        qsort(items, cn, sizeof(bin_item), cmp_bin_item);
+        for (int s = 0; s < cn && !err;) {
+            int e = s;
+            while (e < cn && items[e].bin == items[s].bin) e++;
+            int len = e - s;
+            if (len > 2) {
+                st[0]++;
+                st[1] += len;
+                double tw = 0, tm[3] = {0, 0, 0}, te[3] = {0, 0, 0};
+                for (int q = s; q < e; q++) {
+                    double* pp = parts + 7 * (size_t)cell[items[q].pos];
+                    double w = pp[6];
+                    tw += w;
+                    for (int d = 0; d < 3; d++) tm[d] += pp[3 + d] * w;
+                    for (int d = 0; d < 3; d++) te[d] += (pp[3 + d] * pp[3 + d]) * w;
+                    pp[6] = 0;
+                }
+                double wa = tw * 0.5, wb = tw * 0.5;
+                double inv = 1.0 / tw;
+                for (int d = 0; d < 3; d++) tm[d] *= inv;
+                for (int d = 0; d < 3; d++) te[d] *= inv;
+                for (int d = 0; d < 3; d++) te[d] -= tm[d] * tm[d];
+                for (int d = 0; d < 3; d++) {
+                    if (te[d] < 0) {
+                        if (te[d] < -10e-5 && !gpu_mode) break;
+                        te[d] = 0;
+                    }
+                }
+                double va[3], vb[3];
+                for (int d = 0; d < 3; d++) {
+                    double sq = sqrt(te[d]);
+                    va[d] = tm[d] + sq;
+                    vb[d] = tm[d] - sq;
+                }
+                rng->a = (uint32_t)c;
+                rng->b = (uint32_t)items[s].bin;
+                int slot = 0;
+                int ia = (int)(0.0 + orc_rng_draw(rng, slot++) * ((double)len - 0.0));
+                int ib = (int)(0.0 + orc_rng_draw(rng, slot++) * ((double)len - 0.0));
+                while (ib == ia) ib = (int)(0.0 + orc_rng_draw(rng, slot++) * ((double)len - 0.0));
+                double pa[3], pb[3];
+                const double* qa = parts + 7 * (size_t)cell[items[s + ia].pos];
+                const double* qb = parts + 7 * (size_t)cell[items[s + ib].pos];
+                for (int d = 0; d < 3; d++) {
+                    pa[d] = qa[d];
+                    pb[d] = qb[d];
+                }
+                double* dst[2] = {NULL, NULL};
+                if (gpu_mode) {
+                    dst[0] = parts + 7 * (size_t)cell[items[s].pos];
+                    dst[1] = parts + 7 * (size_t)cell[items[s + 1].pos];
+                }
+                const double* pos2[2] = {pa, pb};
+                double* vel2[2] = {va, vb};
+                double w2[2] = {wa, wb};
+                for (int m = 0; m < 2; m++) {
+                    double* v = vel2[m];
+                    const double* x = pos2[m];
+                    /* addParticle (v3/Species.cpp:420-434) */
+                    if (isnan(x[0]) || isnan(x[1]) || isnan(x[2]) || isnan(v[0]) || isnan(v[1]) ||
+                        isnan(v[2]) || !orc_in_bounds(g, x))
+                        continue;
+                    if (ef) {
+                        double lc[3], E[3];
+                        xtol(g, x, lc);
+                        orc_gather3(g, ef, lc, E);
+                        for (int d = 0; d < 3; d++) v[d] -= E[d] * (charge / mass) * (0.5 * dt);
+                    }
+                    double* r = dst[m];
+                    if (!r) {
+                        if (*n >= cap) {
+                            err = 1;
+                            break;
+                        }
+                        r = parts + 7 * (*n);
+                        (*n)++;
+                    }
+                    for (int d = 0; d < 3; d++) {
+                        r[d] = x[d];
+                        r[3 + d] = v[d];
+                    }
+                    r[6] = w2[m];
+                }
+            }
+            s = e;
+        }
+    }
+    free(items);
+    free(off);
+    free(idx);
+    if (err) return -1;
+    size_t np = *n;
+    if (gpu_mode) {
+        size_t out = 0;
+        for (size_t p = 0; p < np; p++) {
+            if (parts[7 * p + 6] == 0) continue;
+            if (out != p) memmove(parts + 7 * out, parts + 7 * p, 7 * sizeof(double));
+            out++;
+        }
+        np = out;
+    } else {
+        for (size_t p = 0; p < np; p++) {
+            if (parts[7 * p + 6] == 0) {
+                memcpy(parts + 7 * p, parts + 7 * (np - 1), 7 * sizeof(double));
+                np--;
+                p--;
+            }
+        }
+    }
+    *n = np;
+    if (stats) {
+        stats[0] = st[0];
+        stats[1] = st[1];
+    }
+    return 0;
 }

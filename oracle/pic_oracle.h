@@ -184,7 +184,25 @@ int orc_mcc_collide(const orc_mcc_params* p, orc_rng* rng, double* vel_neu, doub
 /* ---- threaded CPU baseline: v3 structure (Species.cpp:258-355 chunked
  * electron push over a thread pool; Matrix.cpp:117-179 chunked SpMV). ---- */
 size_t orc_push_threaded(const orc_grid* g, const double* ef, double* parts, size_t n,
-                         double charge, double mass, double dt, int nthreads);
+                         double charge, double mass, double dt, int nthreads, int remove);
+
+/* ---- object surfaces (v3/Object.cpp:271-317, v3/Species.cpp:170-256,
+ * :835-861) ---- */
+void orc_rect_line_intersect(const orc_rect* r, const double* x1, const double* x2, double* t_entry,
+                             double* pos, double* n);
+/* advanceNonElectron -> advanceNoSputteringSerial: neutrals (charge 0)
+ * reflect diffusely, ions are removed at an object and emit
+ * int(w/mpw0_n + U) neutrals into neu; see pic_oracle.c for draws and
+ * flags.  Returns survivors or (size_t)-1 on neutral overflow. */
+size_t orc_push_nonelectron(const orc_grid* g, const double* ef, double* parts, size_t n,
+                            double charge, double mass, double dt, const orc_rect* rects, int nrect,
+                            double* neu, size_t* n_neu, size_t cap_neu, double mpw0_n,
+                            orc_rng* rng, int accept_on_surface, int stable, long long* stats);
+
+/* ---- Species::merge (v3/Species.cpp:987-1145); gpu_mode selects the
+ * device's defined variant (see pic_oracle.c). ---- */
+int orc_merge(const orc_grid* g, const double* ef, double* parts, size_t* n, size_t cap,
+              double charge, double mass, double dt, orc_rng* rng, int gpu_mode, long long* stats);
 
 #ifdef __cplusplus
 }

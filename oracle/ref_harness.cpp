@@ -17,6 +17,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <fstream>
+#include <random>
 #include <string>
 #include <vector>
 
@@ -260,6 +261,135 @@ static void golden(const std::string& dir, const std::string& table_path) {
             dump("mcc_ele_" + s, parts(el));
             dump("mcc_wsv_" + s, {mcc.wsv()});
         }
+    }
+
+    // ---- threaded electron push (ThreadPool path, Species.cpp:263-310) ----
+    {
+        cfg.setMULTITHREADING(true);
+        cfg.setNUM_THREADS(3);
+        World wt(NI, NJ, NK, x0, xm);
+        wt.setTime(2e-9, 10);
+        wt.ef = world.ef;
+        Species et("e-", Const::m_e, -Const::q_e, wt, 1.5e5);
+        set_parts(et, before);
+        et.advance_time_multi = -1;  // force the threaded path
+        et.advance_time_serial = 1e30;
+        et.advanceElectrons(4e-8);
+        dump("thr_post_long", parts(et));
+        cfg.setMULTITHREADING(false);
+    }
+
+    // ---- object surfaces: electrons absorbed (Species.cpp:356-399), ions
+    // recycled into neutrals and neutrals reflected (Species.cpp:170-256) ----
+    {
+        World ws(NI, NJ, NK, x0, xm);
+        type_calc3 c = ws.getXc(), L = ws.getL();
+        // plates at both z ends (as main.cpp:94-98) and a block in the middle
+        // (x and y faces, edges and corners)
+        std::vector<std::pair<type_calc3, type_calc3>> rects = {
+            {type_calc3(c[0], c[1], ws.getX0()[2]), type_calc3(L[0], L[1], L[2] * 0.2)},
+            {type_calc3(c[0], c[1], ws.getXm()[2]), type_calc3(L[0], L[1], L[2] * 0.2)},
+            {type_calc3(c[0] + 0.1 * L[0], c[1] - 0.05 * L[1], c[2]),
+             type_calc3(L[0] * 0.3, L[1] * 0.25, L[2] * 0.2)}};
+        std::vector<double> rd;
+        for (auto& r : rects) {
+            ws.addObject<Rectangle>(r.first, 0.0, r.second);
+            for (int d = 0; d < 3; d++) rd.push_back(r.first[d]);
+            for (int d = 0; d < 3; d++) rd.push_back(r.second[d] * 0.5);
+        }
+        ws.computeObjectID();
+        dump("sf_rects", rd);
+        for (int i = 0; i < NI; i++)
+            for (int j = 0; j < NJ; j++)
+                for (int k = 0; k < NK; k++) ws.ef[i][j][k] = world.ef[i][j][k] * 1e-3;
+        dump("sf_ef", flat3(ws.ef));
+        const double dt_e = 2e-9, dt_i = 5e-6, dt_n = 4e-5;
+        dump("sf_dt", {dt_e, dt_i, dt_n});
+        ws.setTime(dt_e, 10);
+        rnd = Rnd(555u);
+        Species sn("O", 16 * Const::amu, 0, ws, 1e10);
+        Species si("O+", 16 * Const::amu, Const::q_e, ws, 2.5e10);
+        Species se("e-", Const::m_e, -Const::q_e, ws, 2e4);
+        sn.loadParticleBoxThermal(c, L * 0.99, 1e10 * 3000 / (L[0] * L[1] * L[2]), 300.0);
+        si.loadParticleBoxThermal(c, L * 0.99, 2.5e10 * 2000 / (L[0] * L[1] * L[2]), 1e4);
+        se.loadParticleBoxThermal(c, L * 0.99, 2e4 * 2000 / (L[0] * L[1] * L[2]), 2.0 * 11604.52);
+        dump("sf_neu_in", parts(sn));
+        dump("sf_ion_in", parts(si));
+        dump("sf_ele_in", parts(se));
+        se.advance_time_multi = 1e30;  // serial path
+        se.advance_time_serial = 0;
+        se.advanceElectrons(dt_e);
+        dump("sf_ele_out", parts(se));
+        rnd = Rnd(556u);
+        si.advanceNonElectron(sn, sn, dt_i);
+        dump("sf_ion_out", parts(si));
+        dump("sf_neu_mid", parts(sn));
+        rnd = Rnd(557u);
+        sn.advanceNonElectron(sn, sn, dt_n);
+        dump("sf_neu_out", parts(sn));
+    }
+
+    // ---- Newton-PCG whose PCG gives up: the GS fallback (:222-224, :315-347) ----
+    {
+        World w5(NI, NJ, NK, x0, xm);
+        w5.dirichletBoundries();
+        w5.rho = world.rho;
+        PotentialSolver fb(w5, 2, 1e-6, PCG);
+        fb.setReferenceValues(0, 0, 1e20);
+        bool ok = fb.solve();
+        dump("fb_phi_out", flat(w5.phi));
+        dump("fb_ok", {(double)ok});
+    }
+
+    // ---- Species::merge (Species.cpp:987-1145) on a neutral and an
+    // electron species: a few crowded cells, velocities from a small set
+    // (elastic-split copies) plus scattered ones ----
+    {
+        World wm(NI, NJ, NK, x0, xm);
+        wm.setTime(2e-9, 10);
+        wm.ef = world.ef;
+        std::mt19937 gen(2718u);
+        std::uniform_real_distribution<double> U(0.0, 1.0);
+        type_calc3 dxm = wm.getDx();
+        auto make = [&](double vs, double w0) {
+            std::vector<double> v;
+            // particle 0 first: a far-away cell with an extreme velocity
+            // (the source starts every cell's velocity range from it)
+            v.insert(v.end(), {x0[0] + 0.5 * dxm[0], x0[1] + 0.5 * dxm[1], x0[2] + 0.5 * dxm[2],
+                               2.5 * vs, -1.7 * vs, 0.3 * vs, w0});
+            const int cells[5][3] = {{3, 2, 2}, {4, 4, 3}, {1, 5, 1}, {6, 1, 4}, {2, 3, 5}};
+            for (int ci = 0; ci < 5; ci++) {
+                double base[4][3];
+                for (auto& b : base)
+                    for (double& x : b) x = vs * (2 * U(gen) - 1);
+                int np = 14 + 12 * ci;
+                for (int p = 0; p < np; p++) {
+                    double pos[3];
+                    for (int d = 0; d < 3; d++) pos[d] = x0[d] + (cells[ci][d] + 0.02 + 0.96 * U(gen)) * dxm[d];
+                    double vel[3];
+                    int pick = p % 6;
+                    for (int d = 0; d < 3; d++)
+                        vel[d] = pick < 4 ? base[pick][d] * (p % 5 == 4 ? 1.0 + 1e-4 * U(gen) : 1.0)
+                                          : vs * (2 * U(gen) - 1);
+                    double w = w0 * (0.5 + U(gen));
+                    v.insert(v.end(), {pos[0], pos[1], pos[2], vel[0], vel[1], vel[2], w});
+                }
+            }
+            return v;
+        };
+        std::vector<double> nin = make(600.0, 1e10), ein = make(8e5, 1e4);
+        Species mn("O", 16 * Const::amu, 0, wm, 1e10);
+        Species me("e-", Const::m_e, -Const::q_e, wm, 1e4);
+        set_parts(mn, nin);
+        set_parts(me, ein);
+        dump("mg_neu_in", nin);
+        dump("mg_ele_in", ein);
+        rnd = Rnd(31415u);
+        mn.merge();
+        dump("mg_neu_out", parts(mn));
+        rnd = Rnd(27182u);
+        me.merge();
+        dump("mg_ele_out", parts(me));
     }
     g_manifest.close();
 }

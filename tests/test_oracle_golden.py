@@ -151,3 +151,116 @@ def test_mcc_apply_mt19937_exact(golden, sigma_table_path):
         assert np.array_equal(ion, golden["mcc_ion_" + s])
         assert np.array_equal(ele, golden["mcc_ele_" + s])
     assert total_iz > 100
+
+
+def test_push_threaded_exact(G, golden):
+    """The reference's threaded electron push (3 threads): every particle is
+    pushed, and none is removed -- the per-thread deletion buffers reach the
+    workers as std::bind copies (ThreadPool.h:63, :77-78), so the buffers
+    Species.cpp:287-308 reads stay empty."""
+    ef = golden["ef_analytic"]
+    p = golden["ele_pre_long"].copy()
+    n = O.push_threaded(G, ef, p, -O.QE, O.ME, 4e-8, 3)
+    assert n == p.size // 7
+    assert np.array_equal(p, golden["thr_post_long"])
+    # the pushed positions equal the serial push's (which removes leavers)
+    lo, hi = np.array(G.x0[:]), np.array(G.xm[:])
+    rows = p.reshape(-1, 7)
+    inside = rows[np.all((rows[:, :3] >= lo) & (rows[:, :3] < hi), axis=1)]
+    a = np.sort(inside, axis=0)
+    b = np.sort(golden["ele_post_long"].reshape(-1, 7), axis=0)
+    assert np.array_equal(a, b)
+
+
+def test_push_threaded_with_removal_keeps_serial_set(G, golden):
+    """remove=True applies the removal the source intends (descending
+    swap-with-last over the merged buffers): the survivors are exactly the
+    serial push's survivors."""
+    ef = golden["ef_analytic"]
+    for threads in (1, 3, 8):
+        p = golden["ele_pre_long"].copy()
+        n = O.push_threaded(G, ef, p, -O.QE, O.ME, 4e-8, threads, remove=True)
+        a = np.sort(p[: 7 * n].reshape(-1, 7), axis=0)
+        b = np.sort(golden["ele_post_long"].reshape(-1, 7), axis=0)
+        assert np.array_equal(a, b)
+
+
+def _surface_setup(golden):
+    G = _grid(golden["grid"])
+    r = golden["sf_rects"].reshape(-1, 6)
+    rects = [(row[:3], row[3:], 0.0) for row in r]
+    return G, rects, golden["sf_ef"], golden["sf_dt"]
+
+
+def test_push_electrons_absorbed_in_objects_exact(golden):
+    G, rects, ef, dts = _surface_setup(golden)
+    e = O.push(G, ef, golden["sf_ele_in"], -O.QE, O.ME, dts[0], rects=rects)
+    assert e.size < golden["sf_ele_in"].size
+    assert np.array_equal(e, golden["sf_ele_out"])
+
+
+def test_push_ions_recycled_into_neutrals_exact(golden):
+    """Ions hitting an object are removed and emit int(w/mpw0_n + U)
+    neutrals with diffusely reflected velocities; addParticle rejects an
+    emission point that still tests inside the object (Species.cpp:220-236)."""
+    G, rects, ef, dts = _surface_setup(golden)
+    rng = O.rng_mt(556)
+    ion, neu, st = O.push_nonelectron(G, ef, golden["sf_ion_in"], O.QE, 16 * O.AMU, dts[1],
+                                      rects, 1e10, rng, neu=golden["sf_neu_in"])
+    assert st[0] > 100 and st[1] > 50 and st[2] > 0  # hits, emissions, rejected emissions
+    assert np.array_equal(ion, golden["sf_ion_out"])
+    assert np.array_equal(neu, golden["sf_neu_mid"])
+
+
+def test_push_neutrals_reflected_exact(golden):
+    """Neutrals reflect diffusely at 300 K and drift on for the rest of the
+    step (Species.cpp:216-219, :835-861)."""
+    G, rects, ef, dts = _surface_setup(golden)
+    rng = O.rng_mt(557)
+    neu, _, st = O.push_nonelectron(G, ef, golden["sf_neu_mid"], 0.0, 16 * O.AMU, dts[2], rects,
+                                    1e10, rng)
+    assert st[0] > 100 and st[1] == 0
+    assert np.array_equal(neu, golden["sf_neu_out"])
+
+
+def test_line_intersect_closed_forms():
+    """Entry time, point and outward normal of a segment into a box; a
+    segment through an edge gets the normalised sum of both normals."""
+    box = ([0.0, 0.0, 0.0], [1.0, 2.0, 3.0], 0.0)
+    t, pos, n = O.rect_line_intersect(box, [-3.0, 0.5, 0.1], [0.0, 0.5, 0.1])
+    assert t == pytest.approx(2.0 / 3.0) and np.allclose(pos, [-1.0, 0.5, 0.1])
+    assert np.array_equal(n, [-1.0, 0.0, 0.0])
+    t, pos, n = O.rect_line_intersect(box, [0.2, 5.0, -1.0], [0.2, 1.0, -1.0])
+    assert t == pytest.approx(0.75) and np.allclose(pos, [0.2, 2.0, -1.0])
+    assert np.array_equal(n, [0.0, 1.0, 0.0])
+    t, pos, n = O.rect_line_intersect(box, [-2.0, -3.0, 0.0], [0.0, -1.0, 0.0])
+    assert t == pytest.approx(0.5) and np.allclose(pos, [-1.0, -2.0, 0.0])
+    assert np.allclose(n, [-np.sqrt(0.5), -np.sqrt(0.5), 0.0])
+
+
+def test_pcg_gs_fallback_exact(G, golden):
+    """Newton-PCG whose PCG stops after 2 iterations: every Newton step then
+    runs the Gauss-Seidel fallback (PotentialSolver.cpp:222-224, :315-347)."""
+    nt = np.zeros(O.nn(G), np.int32)
+    O.lib().orc_dirichlet_walls(G, nt)
+    oid = np.zeros(O.nn(G), np.int32)
+    phi, ok, nit, pit, l2 = O.nrpcg(G, oid, nt, np.zeros(O.nn(G)), golden["rho"], 0.0, 0.0,
+                                    1e20, 2, 1e-6)
+    assert ok == bool(golden["fb_ok"][0])
+    assert nit > 1
+    assert np.array_equal(phi, golden["fb_phi_out"])
+
+
+@pytest.mark.parametrize("case,charge,mass,seed", [("neu", 0.0, 16 * 1.660538921e-27, 31415),
+                                                   ("ele", -1.602176565e-19, 9.10938215e-31,
+                                                    27182)])
+def test_merge_exact(G, golden, case, charge, mass, seed):
+    """Species::merge draw for draw: velocity bins over the cell and
+    particles[0], pairs conserving weight, momentum and per-axis energy,
+    appended through addParticle (half-step rewind), zero-weight
+    particles swap-removed (Species.cpp:987-1145)."""
+    pin = golden["mg_%s_in" % case]
+    out, st = O.merge(G, pin, charge, mass, 2e-9, O.rng_mt(seed), ef=golden["ef_analytic"])
+    assert st[0] >= 10
+    assert out.size < pin.size
+    assert np.array_equal(out, golden["mg_%s_out" % case])
