@@ -3,8 +3,9 @@
     python engineering-degree-in-plasma-simulations_amd/build.py
 
 compiles every csrc/*.hip with hipcc for gfx950 into
-engineering-degree-in-plasma-simulations_amd/libpic_mi355.so.  The library
-exports exactly the C ABI declared in include/pic_abi.h.
+engineering-degree-in-plasma-simulations_amd/libpic_mi355.so (the C ABI of
+include/pic_abi.h), the host-only output writers into libpic_io.so
+(include/pic_io.h), and the driver over the host classes (pic_driver).
 """
 import concurrent.futures as cf
 import glob
@@ -48,11 +49,19 @@ def build(verbose=True):
     if r.returncode != 0:
         raise RuntimeError("link failed:\n" + r.stderr[-4000:])
     os.replace(tmp, LIB)
-    # host classes (the reference's interfaces) + driver, linked to the library
+    # output writers (host only, no GPU): libpic_io.so
+    io_lib = os.path.join(HERE, "libpic_io.so")
+    cmd = ["g++", "-O2", "-std=c++17", "-Wall", "-fPIC", "-shared", "-I", os.path.join(ROOT, "include"),
+           os.path.join(HERE, "host", "pic_io.cpp"), "-o", io_lib + ".tmp"]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError("pic_io build failed:\n" + r.stderr[-4000:])
+    os.replace(io_lib + ".tmp", io_lib)
+    # host classes (the reference's interfaces) + driver, linked to the libraries
     driver = os.path.join(HERE, "pic_driver")
     cmd = ["g++", "-O2", "-std=c++17", "-Wall", "-I", os.path.join(ROOT, "include"),
            os.path.join(HERE, "host", "pic_host.cpp"), os.path.join(HERE, "host", "pic_main.cpp"),
-           "-o", driver + ".tmp", "-L", HERE, "-lpic_mi355", "-Wl,-rpath,$ORIGIN",
+           "-o", driver + ".tmp", "-L", HERE, "-lpic_mi355", "-lpic_io", "-Wl,-rpath,$ORIGIN",
            "-Wl,-rpath,/opt/rocm/lib"]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:

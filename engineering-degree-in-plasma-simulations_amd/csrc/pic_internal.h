@@ -128,6 +128,8 @@ struct Draws {
 
 constexpr uint32_t TAG_LOAD = 0x4C4F4144u;  // stream tags
 constexpr uint32_t TAG_MCC = 0x4D434301u;
+constexpr uint32_t TAG_SURF = 0x53524601u;
+constexpr uint32_t TAG_MERGE = 0x4D524701u;
 
 // one species: SoA particle arrays (two buffer sets for the sort) + grids
 struct SpeciesBuf {
@@ -162,6 +164,12 @@ struct Species {
     double* den = nullptr;       // number density
     double* mom = nullptr;       // n, nvx, nvy, nvz, nuu, nvv, nww (7 * nn)
     uint32_t seq = 0;            // loader stream id
+    // behaviour at objects (PIC_SURF_*), the species that receives emitted
+    // particles, and the key of the surface draws (seed, push count)
+    int surf_mode = PIC_SURF_ABSORB;
+    int surf_target = -1;
+    uint32_t surf_seed = 0, surf_step = 0;
+    uint32_t merge_calls = 0;    // key word of the merge draws
 };
 
 struct Mcc {

@@ -151,9 +151,16 @@ __device__ inline void store_birth(Sp& s, uint32_t idx, const Birth& b) {
     s.b.w[idx] = b.w;
 }
 
-// species k (0 neutrals, 1 ions, 2 electrons) gains a particle
-__device__ inline void append(BirthStage& st, int k, Sp& s, uint32_t cell, float fx, float fy, float fz,
-                              const double* v, double w, unsigned long long* overflow) {
+// Where a neutral of a cell's pick pool lives: a global index of the
+// species' arrays, or (REF_LDS set) a slot of the workgroup's birth staging
+// (a neutral split off in this call, not yet flushed).  REF_NONE: dropped.
+constexpr uint32_t REF_LDS = 0x80000000u;
+constexpr uint32_t REF_NONE = 0xFFFFFFFFu;
+
+// species k (0 neutrals, 1 ions, 2 electrons) gains a particle; returns
+// where it is staged (see REF_LDS)
+__device__ inline uint32_t append(BirthStage& st, int k, Sp& s, uint32_t cell, float fx, float fy, float fz,
+                                  const double* v, double w, unsigned long long* overflow) {
     Birth b;
     b.cell = cell;
     b.f[0] = fx;
@@ -166,14 +173,15 @@ __device__ inline void append(BirthStage& st, int k, Sp& s, uint32_t cell, float
     const uint32_t i = atomicAdd(&st.count[k], 1u);
     if (i < (uint32_t)birth_cap(k)) {
         st.b[birth_off(k) + i] = b;
-        return;
+        return (uint32_t)(birth_off(k) + i) | REF_LDS;
     }
     const uint32_t idx = atomicAdd(&s.d_n[0], 1u);  // staging full
     if (idx >= s.cap) {
         atomicAdd(overflow, 1ull);
-        return;
+        return REF_NONE;
     }
     store_birth(s, idx, b);
+    return idx;
 }
 
 // the staged births of the workgroup to the species' tails (every thread
@@ -206,6 +214,10 @@ __device__ inline void flush_births(BirthStage& st, Sp& N, Sp& I, Sp& E, unsigne
 // table's binary search otherwise costs a chain of dependent global loads
 // per NTC group)
 constexpr int SIG_LDS_MAX = 256;
+// split neutrals of one cell and call that join its pick pool; more, or one
+// whose staging slot was full, stay out of it (counted, reported by
+// pic_mcc_apply)
+constexpr int POOL_OWN = 32;
 
 template <bool LDS_TABLE>
 __global__ void __launch_bounds__(BLOCK)
@@ -222,9 +234,12 @@ __global__ void __launch_bounds__(BLOCK)
         table = stab;
     }
     __syncthreads();
+    // this thread's split neutrals in its cell's pool: their staging slots
+    // (16 bits each, so the workgroup's LDS still admits four per CU)
+    __shared__ uint16_t own[POOL_OWN][BLOCK];
     int c = blockIdx.x * BLOCK + threadIdx.x;
     double wsv_tmp = 0;
-    unsigned long long ncoll = 0, niz = 0, ngr = 0, ndead = 0;
+    unsigned long long ncoll = 0, niz = 0, ngr = 0, ndead = 0, npool_full = 0;
     if (c < g.ncells) {
         uint32_t e0 = offE[c], e1 = offE[c + 1];
         uint32_t n0 = offN[c], n1 = offN[c + 1];
@@ -235,25 +250,49 @@ __global__ void __launch_bounds__(BLOCK)
             t0 = tbase + toffN[c];
             t1 = tbase + toffN[c + 1];
         }
-        const int Nb = (int)(n1 - n0);
-        int Ne = (int)(e1 - e0), Nn = Nb + (int)(t1 - t0);
-        if (Ne > 0 && Nn > 0) {
+        // the cell's pick pool (Interactions.cpp:633-635, :698-700): its
+        // range of the sorted part (Nb), its range of the sorted appended
+        // part (Nt), then the neutrals split off in this call (nown, as the
+        // reference appends their indices to the cell's list); removals take
+        // the last entry into the removed slot (:719-721)
+        int Nb = (int)(n1 - n0), Nt = (int)(t1 - t0), nown = 0;
+        const int Ne = (int)(e1 - e0), Nn0 = Nb + Nt;
+        auto loc_of = [&](int slot) -> uint32_t {
+            if (slot < Nb) return n0 + (uint32_t)slot;
+            if (slot < Nb + Nt) return t0 + (uint32_t)(slot - Nb);
+            return REF_LDS | (uint32_t)own[slot - Nb - Nt][threadIdx.x];
+        };
+        if (Ne > 0 && Nn0 > 0) {
             const double wmax = wsv[0];
-            double frac = (double)(Nn * Ne) * wmax * a.dt_inv_dv;  // Interactions.cpp:646
+            double frac = (double)(Nn0 * Ne) * wmax * a.dt_inv_dv;  // Interactions.cpp:646
             int n_groups = (int)(frac + 0.5);
-            if (n_groups > Nn) n_groups = Nn - 1;
+            if (n_groups > Nn0) n_groups = Nn0 - 1;
             const double* tE = table;
             const double* tV = table + a.nsig;
             for (int gi = 0; gi < n_groups; gi++) {
                 Draws dr;
                 dr.init((uint32_t)c, (uint32_t)gi, a.seed, a.step, TAG_MCC);
                 ngr++;
+                const int Nn = Nb + Nt + nown;
                 int ns = (int)(dr(0) * (double)Nn);
                 int es = (int)(dr(1) * (double)Ne);
-                uint32_t pn = ns < Nb ? n0 + (uint32_t)ns : t0 + (uint32_t)(ns - Nb), pe = e0 + (uint32_t)es;
+                const uint32_t pn = loc_of(ns), pe = e0 + (uint32_t)es;
+                const bool staged = (pn & REF_LDS) != 0;
+                Birth* nb = staged ? &births.b[pn & ~REF_LDS] : nullptr;
                 // both particles' weights and velocities requested together
-                double wn = N.b.w[pn], we = E.b.w[pe];
-                double vn[3] = {N.b.v[0][pn], N.b.v[1][pn], N.b.v[2][pn]};
+                double wn, vn[3];
+                if (staged) {
+                    wn = nb->w;
+                    vn[0] = nb->v[0];
+                    vn[1] = nb->v[1];
+                    vn[2] = nb->v[2];
+                } else {
+                    wn = N.b.w[pn];
+                    vn[0] = N.b.v[0][pn];
+                    vn[1] = N.b.v[1][pn];
+                    vn[2] = N.b.v[2][pn];
+                }
+                double we = E.b.w[pe];
                 double ve[3] = {E.b.v[0][pe], E.b.v[1][pe], E.b.v[2][pe]};
                 // (an empty asm that takes them all: otherwise the compiler
                 // tests wn first and waits for memory more than once)
@@ -273,18 +312,39 @@ __global__ void __launch_bounds__(BLOCK)
                 // the neutral's cell and offsets (its births' position), read
                 // only for accepted groups: random picks from the SoA arrays
                 // fetch a whole line per field, and most groups are rejected
-                const uint32_t ncell = N.b.cell[pn];
-                const float nfx = N.b.f[0][pn], nfy = N.b.f[1][pn], nfz = N.b.f[2][pn];
+                uint32_t ncell;
+                float nfx, nfy, nfz;
+                if (staged) {
+                    ncell = nb->cell;
+                    nfx = nb->f[0];
+                    nfy = nb->f[1];
+                    nfz = nb->f[2];
+                } else {
+                    ncell = N.b.cell[pn];
+                    nfx = N.b.f[0][pn];
+                    nfy = N.b.f[1][pn];
+                    nfz = N.b.f[2][pn];
+                }
                 double vnew[3] = {0, 0, 0};
                 if (wn > we) {  // Interactions.cpp:684-703
-                    N.b.w[pn] = (float)(wn - we);
+                    if (staged)
+                        nb->w = (float)(wn - we);
+                    else
+                        N.b.w[pn] = (float)(wn - we);
                     bool iz = collide(a, dr, vn, ve, vnew, sigma);
                     if (iz) {
                         niz++;
                         append(births, 1, I, ncell, nfx, nfy, nfz, vn, we, &stats[3]);
                         append(births, 2, E, ncell, nfx, nfy, nfz, vnew, we, &stats[3]);
                     } else {
-                        append(births, 0, N, ncell, nfx, nfy, nfz, vn, we, &stats[3]);
+                        // the split neutral joins the cell's pool (:698-700)
+                        const uint32_t r = append(births, 0, N, ncell, nfx, nfy, nfz, vn, we, &stats[3]);
+                        if ((r & REF_LDS) && r != REF_NONE && nown < POOL_OWN) {
+                            own[nown][threadIdx.x] = (uint16_t)(r & ~REF_LDS);
+                            nown++;
+                        } else if (r != REF_NONE) {
+                            npool_full++;  // past the per-cell list or the staging
+                        }
                     }
                 } else if (wn < we) {  // Interactions.cpp:705-726, defined semantics
                     E.b.w[pe] = (float)wn;
@@ -296,25 +356,44 @@ __global__ void __launch_bounds__(BLOCK)
                         append(births, 1, I, ncell, nfx, nfy, nfz, vn, wn, &stats[3]);
                         append(births, 2, E, ncell, nfx, nfy, nfz, vnew, wn, &stats[3]);
                         // the ionised neutral leaves the cell's pool as in
-                        // the reference (Interactions.cpp:720-721): the
+                        // the reference (Interactions.cpp:719-721): the
                         // pool's last neutral takes its slot and the pool
                         // shrinks by one.  This thread owns the cell's
-                        // ranges, so the move is a plain copy; the vacated
-                        // last slot is dropped at the next whole sort.
-                        const int last = Nn - 1;
-                        const uint32_t pl = last < Nb ? n0 + (uint32_t)last : t0 + (uint32_t)(last - Nb);
+                        // ranges and its staged splits, so the move is a
+                        // plain copy; the vacated entry is removed (DEAD, at
+                        // the next whole sort).
+                        const uint32_t pl = loc_of(Nn - 1);
                         if (pl != pn) {
-                            N.b.cell[pn] = N.b.cell[pl];
+                            Birth m;
+                            if (pl & REF_LDS) {
+                                m = births.b[pl & ~REF_LDS];
+                            } else {
+                                m.cell = N.b.cell[pl];
 #pragma unroll
-                            for (int d = 0; d < 3; d++) {
-                                N.b.f[d][pn] = N.b.f[d][pl];
-                                N.b.v[d][pn] = N.b.v[d][pl];
+                                for (int d = 0; d < 3; d++) {
+                                    m.f[d] = N.b.f[d][pl];
+                                    m.v[d] = N.b.v[d][pl];
+                                }
+                                m.w = N.b.w[pl];
                             }
-                            N.b.w[pn] = N.b.w[pl];
+                            if (staged)
+                                *nb = m;
+                            else
+                                store_birth(N, pn, m);
                         }
-                        N.b.w[pl] = 0.0f;
-                        N.b.cell[pl] = DEAD;
-                        Nn--;
+                        if (pl & REF_LDS) {
+                            births.b[pl & ~REF_LDS].w = 0.0f;
+                            births.b[pl & ~REF_LDS].cell = DEAD;
+                        } else {
+                            N.b.w[pl] = 0.0f;
+                            N.b.cell[pl] = DEAD;
+                        }
+                        if (nown > 0)
+                            nown--;
+                        else if (Nt > 0)
+                            Nt--;
+                        else
+                            Nb--;
                         ndead++;
                     }
                 }
@@ -324,6 +403,7 @@ __global__ void __launch_bounds__(BLOCK)
             }
         }
     }
+    if (npool_full) atomicAdd(&stats[6], npool_full);
     if (ndead) atomicAdd(&stats[5], ndead);  // neutrals removed (rare)
     flush_births(births, N, I, E, &stats[3]);
     // block reduction of the counters and the step maximum
@@ -367,13 +447,14 @@ __global__ void __launch_bounds__(BLOCK)
 __global__ void k_mcc_finish(double* wsv, const unsigned long long* part, unsigned long long* stats,
                              uint32_t* caps_n, uint32_t cap_n, uint32_t* caps_i, uint32_t cap_i,
                              uint32_t* caps_e, uint32_t cap_e) {
-    unsigned long long t[5] = {0, 0, 0, 0, 0}, mb = 0;
+    unsigned long long t[6] = {0, 0, 0, 0, 0, 0}, mb = 0;
     for (int s = 0; s < NSTRIPE; s++) {
         for (int q = 0; q < 4; q++) t[q] += part[s * SSTRIDE + q];
         t[4] += part[s * SSTRIDE + 5];
+        t[5] += part[s * SSTRIDE + 6];
         mb = part[s * SSTRIDE + 4] > mb ? part[s * SSTRIDE + 4] : mb;
     }
-    for (int q = 0; q < 5; q++) stats[q] = t[q];
+    for (int q = 0; q < 6; q++) stats[q] = t[q];
     wsv[1] = __longlong_as_double((long long)mb);
     if (stats[0] > 0) wsv[0] = wsv[1];
     if (caps_n[0] > cap_n) caps_n[0] = cap_n;
@@ -407,7 +488,7 @@ extern "C" int pic_mcc_create(pic_ctx* ctx, int sn, int si, int se, const pic_mc
     m.nsig = nsig;
     PIC_HIP(ctx, hipMalloc(&m.sig, 2 * (size_t)nsig * sizeof(double)));
     PIC_HIP(ctx, hipMalloc(&m.d_wsv, 2 * sizeof(double)));
-    PIC_HIP(ctx, hipMalloc(&m.d_stats, 5 * sizeof(unsigned long long)));
+    PIC_HIP(ctx, hipMalloc(&m.d_stats, 6 * sizeof(unsigned long long)));
     PIC_HIP(ctx, hipMalloc(&m.d_part, (size_t)NSTRIPE * SSTRIDE * sizeof(unsigned long long)));
     PIC_HIP(ctx, hipMemcpyAsync(m.sig, sig_E, nsig * sizeof(double), hipMemcpyHostToDevice,
                                 ctx->stream));
@@ -483,10 +564,10 @@ extern "C" int pic_mcc_apply(pic_ctx* ctx, int id, double dt, uint32_t seed, uin
     // the step's results through pinned memory (asynchronous copies), one wait
     rc = pic_ensure_pin(ctx, 128);
     if (rc) return rc;
-    unsigned long long* hs = reinterpret_cast<unsigned long long*>(ctx->pin);  // 5 counters
-    double* hw = ctx->pin + 5;                                                 // 2 maxima
-    uint32_t* cn = reinterpret_cast<uint32_t*>(ctx->pin + 7);                 // 4 counts
-    PIC_HIP(ctx, hipMemcpyAsync(hs, m.d_stats, 5 * sizeof(unsigned long long), hipMemcpyDeviceToHost,
+    unsigned long long* hs = reinterpret_cast<unsigned long long*>(ctx->pin);  // 6 counters
+    double* hw = ctx->pin + 6;                                                 // 2 maxima
+    uint32_t* cn = reinterpret_cast<uint32_t*>(ctx->pin + 8);                 // 4 counts
+    PIC_HIP(ctx, hipMemcpyAsync(hs, m.d_stats, 6 * sizeof(unsigned long long), hipMemcpyDeviceToHost,
                                 ctx->stream));
     PIC_HIP(ctx, hipMemcpyAsync(hw, m.d_wsv, 2 * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
     PIC_HIP(ctx, hipMemcpyAsync(&cn[0], N.d_n, 4, hipMemcpyDeviceToHost, ctx->stream));
@@ -514,6 +595,7 @@ extern "C" int pic_mcc_apply(pic_ctx* ctx, int id, double dt, uint32_t seed, uin
         stats->ionizations = (long long)hs[1];
         stats->groups = (long long)hs[2];
         stats->wsv_max = hw[0];
+        stats->pool_overflow = (long long)hs[5];
     }
     if (hs[3]) return pic_fail(ctx, PIC_ERR_CAPACITY, "species capacity exceeded by DSMC births");
     return PIC_OK;

@@ -75,10 +75,25 @@ __device__ inline void run_bases(uint32_t key, bool valid, uint32_t* __restrict_
     if (head && valid) bases[run] = atomicAdd(&count[key], (uint32_t)len);
 }
 
+// What a particle that ends its drift inside an object does (PIC_SURF_*);
+// see surface_path.
+struct SurfArgs {
+    int mode;
+    int emit;          // this pass writes emitted particles (the pass that writes the push)
+    double dt;
+    double vth_wall;   // sqrt(2 k 300 K / m) of the pushed species (Species.cpp:836, :856)
+    double mpw0_t;     // weight of an emitted particle (the target's mpw0, Species.cpp:225)
+    uint32_t seed, step;
+    SpeciesBuf tgt;    // emission target (PIC_SURF_RECYCLE)
+    uint32_t* tgt_n;   // its counts: [0] live count, [2] emissions that did not fit
+    uint32_t tgt_cap;
+};
+
 struct PushArgs {
     float qmdt;    // dt * q / m   (Species.cpp:372)
     float dtx[3];  // dt / dx      (drift in cells)
     int kick;
+    SurfArgs s;
 };
 
 // 32-bit node offsets (grids hold fewer than 2^31 nodes)
@@ -133,12 +148,211 @@ __device__ inline bool in_rect(const GridC& g, int i, int j, int k, float fx, fl
     return false;
 }
 
+// ---- object surfaces (Species.cpp:170-256), FP64, entered only by
+// particles of a reflecting or recycling species that end inside an object ----
+
+// first rectangle holding logical point l (Rectangle::inObject, inclusive;
+// World::inObject takes the first, World.cpp:293-301), -1 if none
+__device__ inline int rect_index(const GridC& g, const double* l) {
+    for (int r = 0; r < g.nrect; r++) {
+        bool in = true;
+#pragma unroll
+        for (int d = 0; d < 3; d++)
+            if (l[d] < g.rlo[r][d] || l[d] > g.rhi[r][d]) in = false;
+        if (in) return r;
+    }
+    return -1;
+}
+
+// Rectangle::lineIntersect (Object.cpp:271-308) in logical coordinates (an
+// affine map per axis leaves the entry time unchanged): slab entry time,
+// the entry side's outward normal, summed and normalised with every side
+// whose time is within 1e-6; pos = the entry point
+__device__ inline double rect_entry(const GridC& g, int r, const double* x1, const double* x2,
+                                    double* pos, double* n) {
+    double tmin[3], tmax[3];
+    int side[3] = {0, 1, 2};
+#pragma unroll
+    for (int d = 0; d < 3; d++) {
+        const double A = x2[d] - x1[d];
+        tmin[d] = (g.rlo[r][d] - x1[d]) / A;
+        tmax[d] = (g.rhi[r][d] - x1[d]) / A;
+        if (tmin[d] > tmax[d]) {
+            const double t = tmin[d];
+            tmin[d] = tmax[d];
+            tmax[d] = t;
+            side[d] += 3;
+        }
+    }
+    double te = tmin[0];
+    int j = 0, k = 0, t = 0;
+    for (int i = 0; i < 2; i++) {
+        if (te < tmin[i + 1]) {
+            te = tmin[i + 1];
+            j = i + 1;
+            k = 0;
+        } else if (fabs(te - tmin[i + 1]) < 1e-6) {
+            k++;
+            t = i + 1;
+        }
+    }
+    n[0] = n[1] = n[2] = 0.0;
+    auto add = [&](int sd) { n[sd % 3] += sd < 3 ? -1.0 : 1.0; };  // Object.cpp:310-316
+    add(side[j]);
+    if (k != 0) {
+        if (k < 2) {
+            add(side[t]);
+        } else {
+            add(side[2]);
+            add(side[1]);
+        }
+        const double inv = 1.0 / sqrt(n[0] * n[0] + n[1] * n[1] + n[2] * n[2]);
+#pragma unroll
+        for (int d = 0; d < 3; d++) n[d] *= inv;
+    }
+#pragma unroll
+    for (int d = 0; d < 3; d++) pos[d] = x1[d] + te * (x2[d] - x1[d]);
+    return te;
+}
+
+__device__ inline void cross3d(const double* a, const double* b, double* o) {
+    o[0] = a[1] * b[2] - a[2] * b[1];
+    o[1] = a[2] * b[0] - a[0] * b[2];
+    o[2] = a[0] * b[1] - a[1] * b[0];
+}
+
+// sampleVth(300 K) then sampleReflectedVelocity (Species.cpp:855-861,
+// :835-853): full accommodation, the direction built as in the source from
+// t1 = n x e_x (n x e_y when n . e_x == 0) and t2 = n x t1, not normalised,
+// sin(psi) t2 without sin(theta).  Draws s0 .. s0 + 10.
+__device__ inline void reflect_velocity(double vth0, double vmag1, const double* n, Draws& dr, int s0,
+                                        double* out) {
+    double v[3];
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+        const double u1 = dr(s0 + 3 * c), u2 = dr(s0 + 3 * c + 1), u3 = dr(s0 + 3 * c + 2);
+        v[c] = vth0 * (u1 + u2 + u3 - 1.5);
+    }
+    const double v_th = sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
+    const double vmag2 = vmag1 + 1.0 * (v_th - vmag1);
+    const double sin_theta = dr(s0 + 9);
+    const double cos_theta = sqrt(1 - sin_theta * sin_theta);
+    const double psi = 2 * PI_REF * dr(s0 + 10);
+    const double ex[3] = {1, 0, 0}, ey[3] = {0, 1, 0};
+    double t1[3], t2[3];
+    if (n[0] * 1.0 + n[1] * 0.0 + n[2] * 0.0 != 0)
+        cross3d(n, ex, t1);
+    else
+        cross3d(n, ey, t1);
+    cross3d(n, t1, t2);
+    const double a = sin_theta * cos(psi), b = sin(psi);
+#pragma unroll
+    for (int d = 0; d < 3; d++) out[d] = ((t1[d] * a + t2[d] * b) + n[d] * cos_theta) * vmag2;
+}
+
+// The drift of a particle whose FP32 drift ended inside an object, redone
+// in FP64 from its stored position with the kicked velocity v, as the
+// reference's bounce loop (Species.cpp:194-249): leaving the box removes;
+// ending in an object moves the particle to the entry point, where a
+// reflecting species takes a diffusely reflected velocity and drifts on for
+// the remaining fraction of the step (at most 20 bounces), and a recycling
+// species is removed after emitting int(w / mpw0_t + U) particles of the
+// target there (when s.emit).  Emitted particles at the entry point are
+// kept (the source's addParticle rejects those its rounding leaves inside
+// the object, Species.cpp:424).  Both passes of pic_push_sort run this with
+// the same inputs and draws and so agree.  Updates f and v; returns the
+// new cell word or DEAD.
+__device__ inline uint32_t surface_path(const GridC& g, const SurfArgs& s, uint32_t c,
+                                              const SpeciesBuf& b, size_t p, float (&f)[3],
+                                              float (&v)[3]) {
+    int ijk[3];
+    unpack_cell(c, ijk[0], ijk[1], ijk[2]);
+    double l[3], vel[3], step[3];
+    const int lim[3] = {g.ci, g.cj, g.ck};
+#pragma unroll
+    for (int d = 0; d < 3; d++) {
+        l[d] = (double)ijk[d] + (double)b.f[d][p];  // the position before this push
+        vel[d] = (double)v[d];
+        step[d] = s.dt * g.inv_dx[d];
+    }
+    double t_rem = 1.0;
+    int nb = 0;
+    while (t_rem > 0) {
+        if (++nb > 20) return DEAD;
+        double old[3];
+#pragma unroll
+        for (int d = 0; d < 3; d++) {
+            old[d] = l[d];
+            l[d] += vel[d] * t_rem * step[d];
+        }
+        bool inb = true;
+#pragma unroll
+        for (int d = 0; d < 3; d++)
+            if (!(l[d] >= 0.0 && l[d] < (double)lim[d])) inb = false;
+        if (!inb) return DEAD;
+        const int r = rect_index(g, l);
+        if (r < 0) break;
+        double nrm[3];
+        const double tp = rect_entry(g, r, old, l, l, nrm);
+        const double vm = sqrt(vel[0] * vel[0] + vel[1] * vel[1] + vel[2] * vel[2]);
+        Draws dr;
+        dr.init((uint32_t)p, (uint32_t)nb, s.seed, s.step, TAG_SURF);
+        if (s.mode == PIC_SURF_REFLECT) {
+            reflect_velocity(s.vth_wall, vm, nrm, dr, 0, vel);
+            t_rem *= (1 - tp);
+            continue;
+        }
+        if (s.emit) {  // PIC_SURF_RECYCLE
+            const int mp = (int)((double)b.w[p] / s.mpw0_t + dr(0));
+            int cc[3];
+            float fe[3];
+            bool inside = true;
+#pragma unroll
+            for (int d = 0; d < 3; d++) {
+                if (!(l[d] >= 0.0 && l[d] < (double)lim[d])) inside = false;
+                cc[d] = (int)l[d];
+                const float fr = (float)(l[d] - (double)cc[d]);
+                fe[d] = fr < 1.0f ? fr : 0.99999994f;
+            }
+            for (int m = 0; inside && m < mp; m++) {
+                double vn[3];
+                reflect_velocity(s.vth_wall, vm, nrm, dr, 1 + 11 * m, vn);
+                const uint32_t idx = atomicAdd(s.tgt_n, 1u);
+                if (idx >= s.tgt_cap) {
+                    atomicAdd(s.tgt_n + 2, 1u);
+                    continue;
+                }
+                s.tgt.cell[idx] = pack_cell(cc[0], cc[1], cc[2]);
+#pragma unroll
+                for (int d = 0; d < 3; d++) {
+                    s.tgt.f[d][idx] = fe[d];
+                    s.tgt.v[d][idx] = (float)vn[d];
+                }
+                s.tgt.w[idx] = (float)s.mpw0_t;
+            }
+        }
+        return DEAD;
+    }
+#pragma unroll
+    for (int d = 0; d < 3; d++) {
+        ijk[d] = (int)l[d];
+        const float fr = (float)(l[d] - (double)ijk[d]);
+        f[d] = fr < 1.0f ? fr : 0.99999994f;
+        v[d] = (float)vel[d];
+    }
+    return pack_cell(ijk[0], ijk[1], ijk[2]);
+}
+
 // One particle's push (Species.cpp:356-399): CIC gather of E at the old
 // position, kick, drift, relocation into the new cell, removal at the walls
-// (World::inBounds, World.cpp:201-205) and inside objects.  Updates f and v;
+// (World::inBounds, World.cpp:201-205); a particle ending inside an object
+// is removed or, for a reflecting or recycling species, handed to
+// surface_path (b and p locate its stored position).  Updates f and v;
 // returns the new cell word, DEAD when the particle is removed.
+template <bool SURF>
 __device__ inline uint32_t advance(const GridC& g, const float4* __restrict__ E, const PushArgs& a,
-                                   uint32_t c, float (&f)[3], float (&v)[3]) {
+                                   uint32_t c, float (&f)[3], float (&v)[3], const SpeciesBuf& b,
+                                   size_t p) {
     int i, j, k;
     unpack_cell(c, i, j, k);
     if (a.kick) {
@@ -156,7 +370,10 @@ __device__ inline uint32_t advance(const GridC& g, const float4* __restrict__ E,
         j = relocate(f[1], j);
         k = relocate(f[2], k);
         ok = i >= 0 && i < g.ci && j >= 0 && j < g.cj && k >= 0 && k < g.ck;
-        if (ok && g.nrect) ok = !in_rect(g, i, j, k, f[0], f[1], f[2]);
+        if (ok && g.nrect && in_rect(g, i, j, k, f[0], f[1], f[2])) {
+            if (!SURF) return DEAD;  // the kernels without the surface path serve absorbing species
+            return surface_path(g, a.s, c, b, p, f, v);
+        }
     }
     return ok ? pack_cell(i, j, k) : DEAD;
 }
@@ -178,7 +395,7 @@ __device__ inline uint32_t cell_key(const GridC& g, uint32_t c) {
 constexpr int PUSH_ILP = 2;
 // the histogram pass of pic_push_sort (4 per thread measured slower)
 constexpr int PUSH_SORT_ILP = 2;
-template <bool WRITE, int ILP>
+template <bool WRITE, int ILP, bool SURF>
 __global__ void __launch_bounds__(BLOCK)
     k_push(GridC g, SpeciesBuf b, const float4* __restrict__ E, PushArgs a,
            uint32_t* __restrict__ d_n, uint32_t* __restrict__ count, uint32_t* __restrict__ rank) {
@@ -210,7 +427,7 @@ __global__ void __launch_bounds__(BLOCK)
         if (valid) {
             key = (uint32_t)g.ncells;
             if (c[q] != DEAD) {
-                const uint32_t nc = advance(g, E, a, c[q], f[q], v[q]);
+                const uint32_t nc = advance<SURF>(g, E, a, c[q], f[q], v[q], b, p);
                 key = cell_key(g, nc);
                 if (WRITE) {
                     if (nc != c[q]) b.cell[p] = nc;  // most particles stay in their cell
@@ -408,6 +625,7 @@ __global__ void __launch_bounds__(BLOCK)
 // its slot in the cell.  The slot comes from the first pass's run bases of
 // the particle's group of 64 (same groups, same keys, same runs): no atomics.
 // Removed particles (key ncells) are dropped.
+template <bool SURF>
 __global__ void __launch_bounds__(BLOCK)
     k_push_scatter(GridC g, SpeciesBuf src, SpeciesBuf dst, const float4* __restrict__ E, PushArgs a,
                    const uint32_t* __restrict__ bases, const uint32_t* __restrict__ off,
@@ -427,7 +645,7 @@ __global__ void __launch_bounds__(BLOCK)
         w = src.w[p];
     }
     uint32_t nc = DEAD;
-    if (c != DEAD) nc = advance(g, E, a, c, f, v);
+    if (c != DEAD) nc = advance<SURF>(g, E, a, c, f, v, src, p);
     // the first pass's key: removed particles in bucket ncells, lanes past n none
     const uint32_t key = valid ? cell_key(g, nc) : 0xFFFFFFFFu;
     int run, head_lane, len;
@@ -734,17 +952,269 @@ __global__ void __launch_bounds__(BLOCK)
     b.w[o] = a.w;
 }
 
+// Variable-weight merge (Species::merge, Species.cpp:987-1145) on the
+// cell-sorted arrays, one workgroup per cell of at least 10 particles
+// (:1029): the cell's velocity range widened by 0.1% at the maximum
+// (:1043-1057) is split into 15 bins per axis (Species.h:129, :1059-1067);
+// the workgroup sorts (bin, place in the cell) keys in LDS (bitonic), and
+// one thread per bin of more than two particles sums weight, momentum and
+// per-axis energy in cell order in FP64, and writes the pair
+// v = mean +- sqrt(variance) (a negative variance clamped to 0), half the
+// weight each, at the positions of two distinct random members, in place
+// of the bin's first two members; the other members are removed (DEAD).
+// Charged species get addParticle's half-step rewind (:1141-1142,
+// Species.cpp:427-433).  Draws: Philox (seed, call), counter (cell, bin).
+constexpr int MERGE_BLOCK = 256;
+constexpr int MERGE_MAX = 8192;  // particles of one cell (LDS keys)
+constexpr int MERGE_BINS = 15;
+constexpr int MERGE_IDX_BITS = 13;
+struct MergeArgs {
+    uint32_t seed, call;
+    double rewind;  // (q/m) * 0.5 dt, 0 for neutrals
+};
+__global__ void __launch_bounds__(MERGE_BLOCK)
+    k_merge(GridC g, SpeciesBuf b, const uint32_t* __restrict__ off, const float4* __restrict__ E,
+            MergeArgs a, unsigned long long* __restrict__ stats) {
+    __shared__ uint32_t key[MERGE_MAX];
+    __shared__ double red[6][MERGE_BLOCK / 64];
+    const int c = blockIdx.x;
+    const int tid = threadIdx.x;
+    const uint32_t p0 = off[c], p1 = off[c + 1];
+    const int n = (int)(p1 - p0);
+    if (n < 10) return;
+    if (n > MERGE_MAX) {
+        if (tid == 0) atomicAdd(&stats[2], 1ull);
+        return;
+    }
+    // velocity range of the cell (exact min and max)
+    double mn[3] = {INFINITY, INFINITY, INFINITY}, mx[3] = {-INFINITY, -INFINITY, -INFINITY};
+    for (int q = tid; q < n; q += MERGE_BLOCK)
+#pragma unroll
+        for (int d = 0; d < 3; d++) {
+            const double v = (double)b.v[d][p0 + q];
+            mn[d] = fmin(mn[d], v);
+            mx[d] = fmax(mx[d], v);
+        }
+#pragma unroll
+    for (int d = 0; d < 3; d++)
+        for (int o = 32; o > 0; o >>= 1) {
+            mn[d] = fmin(mn[d], __shfl_xor(mn[d], o, 64));
+            mx[d] = fmax(mx[d], __shfl_xor(mx[d], o, 64));
+        }
+    if ((tid & 63) == 0)
+#pragma unroll
+        for (int d = 0; d < 3; d++) {
+            red[d][tid >> 6] = mn[d];
+            red[3 + d][tid >> 6] = mx[d];
+        }
+    __syncthreads();
+    double vmin[3], dv[3];
+#pragma unroll
+    for (int d = 0; d < 3; d++) {
+        double lo = red[d][0], hi = red[3 + d][0];
+        for (int w = 1; w < MERGE_BLOCK / 64; w++) {
+            lo = fmin(lo, red[d][w]);
+            hi = fmax(hi, red[3 + d][w]);
+        }
+        if (hi > 0)
+            hi *= 1.001;
+        else
+            hi *= 0.999;
+        vmin[d] = lo;
+        dv[d] = (hi - lo) / (double)MERGE_BINS;
+    }
+    // keys: bin << 13 | place; padded to a power of two
+    int npad = 16;
+    while (npad < n) npad <<= 1;
+    for (int q = tid; q < npad; q += MERGE_BLOCK) {
+        uint32_t k = 0xFFFFFFFFu;
+        if (q < n) {
+            int bi[3];
+#pragma unroll
+            for (int d = 0; d < 3; d++) {
+                const double x = ((double)b.v[d][p0 + q] - vmin[d]) / dv[d];
+                int t = x == x ? (x < (double)MERGE_BINS ? (int)x : MERGE_BINS - 1) : 0;
+                bi[d] = t < 0 ? 0 : t;
+            }
+            const uint32_t bin = (uint32_t)((bi[0] * MERGE_BINS + bi[1]) * MERGE_BINS + bi[2]);
+            k = (bin << MERGE_IDX_BITS) | (uint32_t)q;
+        }
+        key[q] = k;
+    }
+    __syncthreads();
+    for (int k = 2; k <= npad; k <<= 1)
+        for (int j = k >> 1; j > 0; j >>= 1) {
+            for (int i = tid; i < npad; i += MERGE_BLOCK) {
+                const int ixj = i ^ j;
+                if (ixj > i) {
+                    const uint32_t x = key[i], y = key[ixj];
+                    if ((x > y) == ((i & k) == 0)) {
+                        key[i] = y;
+                        key[ixj] = x;
+                    }
+                }
+            }
+            __syncthreads();
+        }
+    constexpr uint32_t MASK = (1u << MERGE_IDX_BITS) - 1u;
+    unsigned long long nbins = 0, nin = 0;
+    for (int s0 = tid; s0 < n; s0 += MERGE_BLOCK) {
+        const uint32_t bin = key[s0] >> MERGE_IDX_BITS;
+        if (s0 > 0 && (key[s0 - 1] >> MERGE_IDX_BITS) == bin) continue;  // not a bin's first
+        int e = s0 + 1;
+        while (e < n && (key[e] >> MERGE_IDX_BITS) == bin) e++;
+        const int len = e - s0;
+        if (len <= 2) continue;
+        nbins++;
+        nin += (unsigned long long)len;
+        double tw = 0, tm[3] = {0, 0, 0}, te[3] = {0, 0, 0};
+        for (int q = s0; q < e; q++) {
+            const uint32_t idx = p0 + (key[q] & MASK);
+            const double w = (double)b.w[idx];
+            tw += w;
+#pragma unroll
+            for (int d = 0; d < 3; d++) {
+                const double v = (double)b.v[d][idx];
+                tm[d] += v * w;
+                te[d] += (v * v) * w;
+            }
+        }
+        const double wh = tw * 0.5, inv = 1.0 / tw;
+        double va[3], vb[3];
+#pragma unroll
+        for (int d = 0; d < 3; d++) {
+            tm[d] *= inv;
+            te[d] *= inv;
+            te[d] -= tm[d] * tm[d];
+            if (te[d] < 0) te[d] = 0;
+            const double sq = sqrt(te[d]);
+            va[d] = tm[d] + sq;
+            vb[d] = tm[d] - sq;
+        }
+        Draws dr;
+        dr.init((uint32_t)c, bin, a.seed, a.call, TAG_MERGE);
+        int slot = 0;
+        const int ia = (int)(dr(slot++) * (double)len);
+        int ib = (int)(dr(slot++) * (double)len);
+        while (ib == ia) ib = (int)(dr(slot++) * (double)len);
+        const uint32_t xa = p0 + (key[s0 + ia] & MASK), xb = p0 + (key[s0 + ib] & MASK);
+        const uint32_t ca = b.cell[xa], cb = b.cell[xb];
+        float fa[3], fb[3];
+#pragma unroll
+        for (int d = 0; d < 3; d++) {
+            fa[d] = b.f[d][xa];
+            fb[d] = b.f[d][xb];
+        }
+        if (a.rewind != 0.0) {
+            int i, j, k;
+            unpack_cell(ca, i, j, k);
+            const float3 ea = gather_e(g, E, i, j, k, fa[0], fa[1], fa[2]);
+            unpack_cell(cb, i, j, k);
+            const float3 eb = gather_e(g, E, i, j, k, fb[0], fb[1], fb[2]);
+            va[0] -= (double)ea.x * a.rewind;
+            va[1] -= (double)ea.y * a.rewind;
+            va[2] -= (double)ea.z * a.rewind;
+            vb[0] -= (double)eb.x * a.rewind;
+            vb[1] -= (double)eb.y * a.rewind;
+            vb[2] -= (double)eb.z * a.rewind;
+        }
+        const uint32_t da = p0 + (key[s0] & MASK), db = p0 + (key[s0 + 1] & MASK);
+        b.cell[da] = ca;
+        b.cell[db] = cb;
+#pragma unroll
+        for (int d = 0; d < 3; d++) {
+            b.f[d][da] = fa[d];
+            b.f[d][db] = fb[d];
+            b.v[d][da] = (float)va[d];
+            b.v[d][db] = (float)vb[d];
+        }
+        b.w[da] = (float)wh;
+        b.w[db] = (float)wh;
+        for (int q = s0 + 2; q < e; q++) {
+            const uint32_t idx = p0 + (key[q] & MASK);
+            b.cell[idx] = DEAD;
+            b.w[idx] = 0.0f;
+        }
+    }
+    for (int o = 32; o > 0; o >>= 1) {
+        nbins += __shfl_xor(nbins, o, 64);
+        nin += __shfl_xor(nin, o, 64);
+    }
+    if ((tid & 63) == 0 && nbins) {
+        atomicAdd(&stats[0], nbins);
+        atomicAdd(&stats[1], nin);
+    }
+}
+
 }  // namespace
 
 // ------------------------------------------------------------------------
 // host entry points
 // ------------------------------------------------------------------------
-static PushArgs push_args(const pic_ctx* ctx, const Species& s, double dt) {
+static PushArgs push_args(pic_ctx* ctx, Species& s, double dt) {
     PushArgs a;
     a.qmdt = (float)(dt * s.charge / s.mass);
     for (int d = 0; d < 3; d++) a.dtx[d] = (float)(dt * ctx->g.inv_dx[d]);
     a.kick = s.charge != 0.0;
+    SurfArgs& f = a.s;
+    f = SurfArgs{};
+    f.mode = ctx->g.nrect ? s.surf_mode : PIC_SURF_ABSORB;
+    f.dt = dt;
+    f.vth_wall = sqrt(2 * KB * 300.0 / s.mass);  // Species.cpp:836, :856
+    f.seed = s.surf_seed;
+    f.step = s.surf_step++;
+    if (f.mode == PIC_SURF_RECYCLE) {
+        Species& t = ctx->sp[s.surf_target];
+        f.mpw0_t = t.mpw0;
+        f.tgt = t.buf[t.cur];
+        f.tgt_n = t.d_n;
+        f.tgt_cap = (uint32_t)t.cap;
+    }
     return a;
+}
+
+// after a push of species s that may have emitted particles into its
+// target: the target's live count (its launch sizes), its flags (only
+// appended: a sorted target keeps its sorted part and may sort the tail
+// alone), and the emissions that did not fit
+static int after_emission(pic_ctx* ctx, const Species& s, const PushArgs& a) {
+    if (a.s.mode != PIC_SURF_RECYCLE) return PIC_OK;
+    Species& t = ctx->sp[s.surf_target];
+    uint32_t hn[4];
+    PIC_HIP(ctx, hipMemcpyAsync(hn, t.d_n, sizeof(hn), hipMemcpyDeviceToHost, ctx->stream));
+    PIC_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    if (hn[0] > t.cap) hn[0] = (uint32_t)t.cap;
+    if ((size_t)hn[0] != t.n_upper) {
+        t.n_upper = hn[0];
+        t.sorted = false;
+        t.hist_valid = false;
+    }
+    if (hn[2]) {
+        uint32_t z = 0;
+        PIC_HIP(ctx, hipMemcpyAsync(t.d_n + 2, &z, sizeof(z), hipMemcpyHostToDevice, ctx->stream));
+        PIC_HIP(ctx, hipStreamSynchronize(ctx->stream));
+        PIC_HIP(ctx, hipMemcpyAsync(t.d_n, &hn[0], sizeof(uint32_t), hipMemcpyHostToDevice, ctx->stream));
+        PIC_HIP(ctx, hipStreamSynchronize(ctx->stream));
+        return pic_fail(ctx, PIC_ERR_CAPACITY, "particles emitted at objects exceed the target's capacity");
+    }
+    return PIC_OK;
+}
+
+extern "C" int pic_species_set_surface(pic_ctx* ctx, int sid, int mode, int sid_target, uint32_t seed) {
+    int rc = pic_check_sid(ctx, sid);
+    if (rc) return rc;
+    if (mode != PIC_SURF_ABSORB && mode != PIC_SURF_REFLECT && mode != PIC_SURF_RECYCLE)
+        return pic_fail(ctx, PIC_ERR_ARG, "unknown surface mode");
+    if (mode == PIC_SURF_RECYCLE) {
+        rc = pic_check_sid(ctx, sid_target);
+        if (rc) return rc;
+        if (sid_target == sid) return pic_fail(ctx, PIC_ERR_ARG, "a species cannot emit into itself");
+    }
+    Species& s = ctx->sp[sid];
+    s.surf_mode = mode;
+    s.surf_target = mode == PIC_SURF_RECYCLE ? sid_target : -1;
+    s.surf_seed = seed;
+    return PIC_OK;
 }
 
 extern "C" int pic_push(pic_ctx* ctx, int sid, double dt) {
@@ -752,20 +1222,26 @@ extern "C" int pic_push(pic_ctx* ctx, int sid, double dt) {
     if (rc) return rc;
     Species& s = ctx->sp[sid];
     const GridC& g = ctx->g;
-    const PushArgs a = push_args(ctx, s, dt);
+    PushArgs a = push_args(ctx, s, dt);
+    a.s.emit = 1;
     PIC_HIP(ctx, hipMemsetAsync(s.count, 0, ((size_t)g.ncells + 1) * 4, ctx->stream));
     {
         PicProfScope ps(ctx, PIC_K_PUSH);
         size_t n = s.n_upper ? s.n_upper : 1;
-        k_push<true, PUSH_ILP><<<xcd_blocks1(n, BLOCK * PUSH_ILP), BLOCK, 0, ctx->stream>>>(g, s.buf[s.cur], ctx->ef, a, s.d_n,
-                                                                    s.count, s.rank);
+        const unsigned nb = xcd_blocks1(n, BLOCK * PUSH_ILP);
+        if (a.s.mode == PIC_SURF_ABSORB)
+            k_push<true, PUSH_ILP, false><<<nb, BLOCK, 0, ctx->stream>>>(g, s.buf[s.cur], ctx->ef, a, s.d_n,
+                                                                         s.count, s.rank);
+        else
+            k_push<true, PUSH_ILP, true><<<nb, BLOCK, 0, ctx->stream>>>(g, s.buf[s.cur], ctx->ef, a, s.d_n,
+                                                                        s.count, s.rank);
     }
     PIC_HIP(ctx, hipGetLastError());
     s.hist_valid = true;
     s.sorted = false;
     s.two_part = false;
     s.tail_ok = false;  // cells changed
-    return PIC_OK;
+    return after_emission(ctx, s, a);
 }
 
 int pic_histogram(pic_ctx* ctx, int sid) {
@@ -871,19 +1347,29 @@ extern "C" int pic_push_sort(pic_ctx* ctx, int sid, double dt) {
     if (rc) return rc;
     Species& s = ctx->sp[sid];
     const GridC& g = ctx->g;
-    const PushArgs a = push_args(ctx, s, dt);
+    PushArgs a = push_args(ctx, s, dt);  // emission in the second pass only
     const size_t n = s.n_upper ? s.n_upper : 1;
     PIC_HIP(ctx, hipMemsetAsync(s.count, 0, ((size_t)g.ncells + 1) * 4, ctx->stream));
     {
         PicProfScope ps(ctx, PIC_K_PUSH);
-        k_push<false, PUSH_SORT_ILP><<<xcd_blocks1(n, BLOCK * PUSH_SORT_ILP), BLOCK, 0, ctx->stream>>>(g, s.buf[s.cur], ctx->ef, a, s.d_n,
-                                                                     s.count, s.rank);
+        const unsigned nb = xcd_blocks1(n, BLOCK * PUSH_SORT_ILP);
+        if (a.s.mode == PIC_SURF_ABSORB)
+            k_push<false, PUSH_SORT_ILP, false><<<nb, BLOCK, 0, ctx->stream>>>(g, s.buf[s.cur], ctx->ef, a,
+                                                                               s.d_n, s.count, s.rank);
+        else
+            k_push<false, PUSH_SORT_ILP, true><<<nb, BLOCK, 0, ctx->stream>>>(g, s.buf[s.cur], ctx->ef, a,
+                                                                              s.d_n, s.count, s.rank);
     }
     scan_counts(ctx, s, s.off, true);
+    a.s.emit = 1;
     {
         PicProfScope ps(ctx, PIC_K_SCATTER);
-        k_push_scatter<<<xcd_blocks1(n, BLOCK), BLOCK, 0, ctx->stream>>>(
-            g, s.buf[s.cur], s.buf[s.cur ^ 1], ctx->ef, a, s.rank, s.off, s.d_n, (uint32_t)s.cap);
+        if (a.s.mode == PIC_SURF_ABSORB)
+            k_push_scatter<false><<<xcd_blocks1(n, BLOCK), BLOCK, 0, ctx->stream>>>(
+                g, s.buf[s.cur], s.buf[s.cur ^ 1], ctx->ef, a, s.rank, s.off, s.d_n, (uint32_t)s.cap);
+        else
+            k_push_scatter<true><<<xcd_blocks1(n, BLOCK), BLOCK, 0, ctx->stream>>>(
+                g, s.buf[s.cur], s.buf[s.cur ^ 1], ctx->ef, a, s.rank, s.off, s.d_n, (uint32_t)s.cap);
     }
     PIC_HIP(ctx, hipGetLastError());
     s.cur ^= 1;
@@ -891,7 +1377,7 @@ extern "C" int pic_push_sort(pic_ctx* ctx, int sid, double dt) {
     s.hist_valid = false;
     s.two_part = false;
     s.tail_ok = true;
-    return PIC_OK;
+    return after_emission(ctx, s, a);
 }
 
 extern "C" int pic_deposit(pic_ctx* ctx, int sid, int flags) {
@@ -1009,5 +1495,47 @@ extern "C" int pic_load_box_thermal(pic_ctx* ctx, int sid, const double centre[3
     s.two_part = false;
     s.tail_ok = false;
     if (n_loaded) *n_loaded = num_macro;
+    return PIC_OK;
+}
+
+extern "C" int pic_merge(pic_ctx* ctx, int sid, uint32_t seed, pic_merge_stats* stats) {
+    int rc = pic_check_sid(ctx, sid);
+    if (rc) return rc;
+    Species& s = ctx->sp[sid];
+    const GridC& g = ctx->g;
+    if (!s.sorted || s.two_part) {  // one contiguous range per cell
+        rc = pic_sort(ctx, sid);
+        if (rc) return rc;
+    }
+    const size_t n_before = s.n_upper;
+    rc = pic_ensure_pin(ctx, 64);
+    if (rc) return rc;
+    unsigned long long* d = reinterpret_cast<unsigned long long*>(ctx->scratch);  // 3 counters
+    PIC_HIP(ctx, hipMemsetAsync(d, 0, 3 * sizeof(unsigned long long), ctx->stream));
+    MergeArgs a;
+    a.seed = seed;
+    a.call = s.merge_calls++;
+    a.rewind = s.charge != 0.0 ? s.charge / s.mass * (0.5 * ctx->dt) : 0.0;
+    k_merge<<<(unsigned)g.ncells, MERGE_BLOCK, 0, ctx->stream>>>(g, s.buf[s.cur], s.off, ctx->ef, a, d);
+    PIC_HIP(ctx, hipGetLastError());
+    s.sorted = false;  // removed members leave at the compaction below
+    s.hist_valid = false;
+    s.tail_ok = false;
+    s.two_part = false;
+    rc = pic_sort(ctx, sid);
+    if (rc) return rc;
+    unsigned long long* h = reinterpret_cast<unsigned long long*>(ctx->pin);
+    uint32_t* cnt = reinterpret_cast<uint32_t*>(ctx->pin + 4);
+    PIC_HIP(ctx, hipMemcpyAsync(h, d, 3 * sizeof(unsigned long long), hipMemcpyDeviceToHost, ctx->stream));
+    PIC_HIP(ctx, hipMemcpyAsync(cnt, s.d_n, sizeof(uint32_t), hipMemcpyDeviceToHost, ctx->stream));
+    PIC_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    s.n_upper = cnt[0];
+    if (stats) {
+        stats->bins = (long long)h[0];
+        stats->merged = (long long)h[1];
+        stats->removed = (long long)n_before - (long long)cnt[0];
+    }
+    if (h[2])
+        return pic_fail(ctx, PIC_ERR_CAPACITY, "merge: a cell holds more than 8192 particles");
     return PIC_OK;
 }

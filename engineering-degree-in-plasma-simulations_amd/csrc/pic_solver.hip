@@ -670,10 +670,21 @@ static int solve_pcg(pic_ctx* ctx, const pic_solver_params* p, pic_solver_stats*
             conv = true;
             break;
         }
+        // relative criterion on the nonlinear problem: the FP64 residual F
+        // of the current phi over the regular rows (the Newton iterate is
+        // judged by the residual, not by the size of its last step)
+        if (nonlinear && p->tol_kind == PIC_TOL_REL && fr[1] <= target2) {
+            conv = true;
+            break;
+        }
         double inner_target2 = target2;
         if (mixed) {  // FP32 inner solve: reduce the residual by ~1e-3 per pass
             double floor2 = fr[0] * 1e-6;
             inner_target2 = std::max(target2 * 0.25, floor2);
+        } else if (nonlinear && p->tol_kind == PIC_TOL_REL) {
+            // inexact Newton: each linear solve to 1e-4 of the current
+            // residual, or below the final tolerance
+            inner_target2 = std::max(target2 * 0.25, fr[0] * 1e-8);
         }
         int its = 0;
         double gg = 0;
@@ -700,7 +711,7 @@ static int solve_pcg(pic_ctx* ctx, const pic_solver_params* p, pic_solver_stats*
         PIC_HIP(ctx, hipMemcpyAsync(&yy, ctx->scratch, sizeof(double), hipMemcpyDeviceToHost,
                                     ctx->stream));
         PIC_HIP(ctx, pic_spin_wait(ctx->stream));
-        if (nonlinear) {
+        if (nonlinear && p->tol_kind == PIC_TOL_REF_RMS) {
             if (std::sqrt(yy / (double)nn) < 1e-3) {  // NR_TOL, PotentialSolver.cpp:180, :229-233
                 conv = true;
                 newton++;
@@ -745,7 +756,7 @@ static int solve_pcg(pic_ctx* ctx, const pic_solver_params* p, pic_solver_stats*
 // ------------------------------------------------------------------------
 bool pic_unet_ready(pic_ctx* ctx);
 int pic_unet_coarse_correct(pic_ctx* ctx, const double* t, const float* z, float* c, float* res,
-                            double* ws);
+                            double* ws, const double* P);
 
 namespace {
 
@@ -803,7 +814,7 @@ __global__ void __launch_bounds__(BLOCK)
 __global__ void __launch_bounds__(BLOCK)
     k_fcg_pq(Stencil s, const uint8_t* __restrict__ nt, const double* __restrict__ z,
              const double* __restrict__ pold, double* __restrict__ pnew, double* __restrict__ q,
-             FcgHist h, int k) {
+             FcgHist h, int k, const double* __restrict__ P) {
     double beta = 0.0;
     if (k > 0) {
         double den = sget(h.pq + (k - 1) * SSIZE);
@@ -820,6 +831,7 @@ __global__ void __launch_bounds__(BLOCK)
         double pu = pn(u);
         double qv = (pn(u - ok) + pn(u + ok)) * s.cz + (pn(u - oj) + pn(u + oj)) * s.cy +
                     (pn(u - oi) + pn(u + oi)) * s.cx - s.twos * pu;
+        if (P) qv -= P[u] * pu;  // J = A - diag(P) (the Newton Jacobian)
         qv = reg ? qv : 0.0;
         pnew[u] = pu;
         q[u] = qv;
@@ -854,23 +866,31 @@ __global__ void __launch_bounds__(BLOCK)
 // ((i + j + k) even); the black half-sweep that follows reads only those red
 // values, so it is written directly in r.  Non-regular nodes get 0.  z is
 // FP32 (the preconditioner's internal vectors; FP64 arithmetic).
+// 1 / the diagonal of the operator's regular row: 2 sum 1/dx^2, plus P_u
+// for the Newton Jacobian J = A - diag(P)
+__device__ inline double inv_diag(const Stencil& s, const double* __restrict__ P, unsigned u) {
+    return P ? 1.0 / (s.twos + P[u]) : s.itwos;
+}
+
 __global__ void __launch_bounds__(BLOCK)
     k_presmooth(Stencil s, const uint8_t* __restrict__ nt, const double* __restrict__ r,
-                float* __restrict__ z, double* __restrict__ clear, int nclear) {
+                float* __restrict__ z, double* __restrict__ clear, int nclear,
+                const double* __restrict__ P) {
     // the coarse correction's striped sums start from zero (no memset node)
     for (int q = blockIdx.x * BLOCK + threadIdx.x; q < nclear; q += gridDim.x * BLOCK) clear[q] = 0.0;
-    // r on regular nodes, 0 elsewhere (a red value is -that / twos)
+    // r / diag on regular nodes, 0 elsewhere (a red value is -that)
     auto rr = [&](unsigned v) {
-        double rv = r[v];
+        double rv = r[v] * inv_diag(s, P, v);
         return nt[v] == NT_REG ? rv : 0.0;
     };
     auto node = [&](unsigned u, bool red) {
         const bool reg = nt[u] == NT_REG;
+        const double id = inv_diag(s, P, u);
         double zu;
         if (red)
-            zu = -r[u] * s.itwos;
+            zu = -r[u] * id;
         else
-            zu = (-nbr_sum(s, u, reg, rr) * s.itwos - r[u]) * s.itwos;
+            zu = (-nbr_sum(s, u, reg, rr) - r[u]) * id;
         z[u] = reg ? (float)zu : 0.f;
     };
     for_pairs(s, [&](unsigned u, bool second, bool red) {
@@ -886,14 +906,14 @@ __global__ void __launch_bounds__(BLOCK)
 __global__ void __launch_bounds__(BLOCK)
     k_gs_red_dots(Stencil s, const uint8_t* __restrict__ nt, const double* __restrict__ r,
                   double* __restrict__ z, const double* __restrict__ q, double* __restrict__ zq,
-                  double* __restrict__ rz) {
+                  double* __restrict__ rz, const double* __restrict__ P) {
     double a = 0, b = 0;
     auto zv = [&](unsigned w) { return z[w]; };
     auto node = [&](unsigned u, bool red) {
         double zu = z[u];
         if (red) {
             const bool reg = nt[u] == NT_REG;
-            double upd = (nbr_sum(s, u, reg, zv) - r[u]) * s.itwos;
+            double upd = (nbr_sum(s, u, reg, zv) - r[u]) * inv_diag(s, P, u);
             if (reg) {
                 zu = upd;
                 z[u] = zu;
@@ -920,7 +940,7 @@ __global__ void __launch_bounds__(BLOCK)
 __global__ void __launch_bounds__(BLOCK)
     k_gs_corr(Stencil s, const uint8_t* __restrict__ nt, const double* __restrict__ r,
               const float* __restrict__ zin, const float* __restrict__ c,
-              const double* __restrict__ d, double* __restrict__ z) {
+              const double* __restrict__ d, double* __restrict__ z, const double* __restrict__ P) {
     const double d1 = sget(d + SSIZE);
     const double om = d1 != 0.0 ? sget(d) / d1 : 0.0;
     auto zc = [&](unsigned w) { return (double)zin[w] + om * c[w]; };
@@ -930,7 +950,7 @@ __global__ void __launch_bounds__(BLOCK)
         if (red)
             zu = zc(u);
         else
-            zu = (nbr_sum(s, u, reg, zc) - r[u]) * s.itwos;
+            zu = (nbr_sum(s, u, reg, zc) - r[u]) * inv_diag(s, P, u);
         z[u] = reg ? zu : 0.0;
     };
     for_pairs(s, [&](unsigned u, bool second, bool red) {
@@ -945,7 +965,7 @@ __global__ void __launch_bounds__(BLOCK)
 // the learned preconditioner's pre-smoother, also used for its training pairs
 int pic_learned_presmooth(pic_ctx* ctx, const double* r, float* z) {
     Stencil s = make_stencil(ctx->g, false);
-    k_presmooth<<<pair_blocks(s), BLOCK, 0, ctx->stream>>>(s, ctx->ntype, r, z, nullptr, 0);
+    k_presmooth<<<pair_blocks(s), BLOCK, 0, ctx->stream>>>(s, ctx->ntype, r, z, nullptr, 0, nullptr);
     PIC_HIP(ctx, hipGetLastError());
     return PIC_OK;
 }
@@ -954,18 +974,19 @@ int pic_learned_presmooth(pic_ctx* ctx, const double* r, float* z) {
 // half-sweep also accumulates *zq += z.q (q may be null) and *rz += z.r;
 // both must be zeroed by the caller.
 static int learned_precond(pic_ctx* ctx, const Stencil& s, const double* rin, double* zout,
-                           float* tmp, const double* q, double* zq, double* rz) {
+                           float* tmp, const double* q, double* zq, double* rz,
+                           const double* P = nullptr) {
     unsigned nb = pair_blocks(s);
     float* zpre = tmp;
     float* c = tmp + s.nn;
     float* res = c + s.nn;
     double* ws = ctx->scratch + LEARNED_WS;  // the coarse correction's striped sums
     double* dots = ws + 2 * SSIZE;
-    k_presmooth<<<nb, BLOCK, 0, ctx->stream>>>(s, ctx->ntype, rin, zpre, ws, 4 * SSIZE);
-    int e = pic_unet_coarse_correct(ctx, rin, zpre, c, res, ws);
+    k_presmooth<<<nb, BLOCK, 0, ctx->stream>>>(s, ctx->ntype, rin, zpre, ws, 4 * SSIZE, P);
+    int e = pic_unet_coarse_correct(ctx, rin, zpre, c, res, ws, P);
     if (e) return e;
-    k_gs_corr<<<nb, BLOCK, 0, ctx->stream>>>(s, ctx->ntype, rin, zpre, c, dots, zout);
-    k_gs_red_dots<<<nb, BLOCK, 0, ctx->stream>>>(s, ctx->ntype, rin, zout, q, zq, rz);
+    k_gs_corr<<<nb, BLOCK, 0, ctx->stream>>>(s, ctx->ntype, rin, zpre, c, dots, zout, P);
+    k_gs_red_dots<<<nb, BLOCK, 0, ctx->stream>>>(s, ctx->ntype, rin, zout, q, zq, rz, P);
     PIC_HIP(ctx, hipGetLastError());
     return PIC_OK;
 }
@@ -1016,7 +1037,6 @@ static SorArgs plain_args(const Stencil& s) {
 static int solve_learned(pic_ctx* ctx, const pic_solver_params* p, pic_solver_stats* st) {
     int rc = learned_check(ctx);
     if (rc) return rc;
-    if (p->n0 != 0.0) return pic_fail(ctx, PIC_ERR_ARG, "learned solver: linear problems only");
     const GridC& g = ctx->g;
     size_t nn = (size_t)g.nn;
     Stencil s = make_stencil(g, false);
@@ -1089,7 +1109,7 @@ static int solve_learned(pic_ctx* ctx, const pic_solver_params* p, pic_solver_st
         for (int k = 0; k < max_fcg; k++) {
             {
                 PicProfScope ps(ctx, PIC_K_PCG);
-                k_fcg_pq<<<nbr, BLOCK, 0, ctx->stream>>>(s, ctx->ntype, z, pold, pnew, q, h, k);
+                k_fcg_pq<<<nbr, BLOCK, 0, ctx->stream>>>(s, ctx->ntype, z, pold, pnew, q, h, k, nullptr);
                 k_fcg_update<<<nb, BLOCK, 0, ctx->stream>>>(s.nn, h, k, ctx->phi, pnew, r, q);
             }
             PIC_HIP(ctx, hipGetLastError());
@@ -1160,6 +1180,147 @@ static int solve_learned(pic_ctx* ctx, const pic_solver_params* p, pic_solver_st
     return PIC_OK;
 }
 
+// Newton-Boltzmann (PotentialSolver::solveNRPCG, PotentialSolver.cpp:178-241)
+// with the learned preconditioner inside: per Newton step, F and the
+// Jacobian shift P from the FP64 phi (k_F), then flexible CG on
+// J y = F, J = A - diag(P), preconditioned by the two-level cycle whose
+// smoothing sweeps, residual restriction and line search carry diag(P)
+// (the network's coarse solve approximates A^-1 and the A-norm optimal step
+// becomes the J-norm optimal one); phi -= y.  Converged when the FP64
+// residual of the nonlinear problem over the regular rows is within the
+// tolerance (PIC_TOL_REL), or, with PIC_TOL_REF_RMS, when RMS(y) < 1e-3 as
+// in the reference (:229-233).  Each inner solve reduces its residual by
+// 1e-4 (inexact Newton) or to below the tolerance.  Plain Newton-PCG is the
+// fallback if the learned iteration stalls; the final residual is FP64.
+static int solve_newton_learned(pic_ctx* ctx, const pic_solver_params* p, pic_solver_stats* st) {
+    int rc = learned_check(ctx);
+    if (rc) return rc;
+    const GridC& g = ctx->g;
+    const size_t nn = (size_t)g.nn;
+    Stencil s = make_stencil(g, false);
+    const int max_fcg = std::min(p->max_it, 1000);
+    const size_t hslots = (size_t)(max_fcg + 2) * SSIZE;
+    // r (F), z, p0, p1, q, y, P (FP64), three FP32 vectors, four histories
+    rc = pic_ensure_work(ctx, 9 * nn * sizeof(double) + 4 * hslots * sizeof(double));
+    if (rc) return rc;
+    double* r = ctx->work;
+    double* z = r + nn;
+    double* p0 = z + nn;
+    double* p1 = p0 + nn;
+    double* q = p1 + nn;
+    double* y = q + nn;
+    double* P = y + nn;
+    float* ftmp = reinterpret_cast<float*>(P + nn);  // 1.5 nn doubles
+    FcgHist h;
+    h.rz = P + nn + 2 * nn;
+    h.pq = h.rz + hslots;
+    h.zq = h.pq + hslots;
+    h.rr = h.zq + hslots;
+    rc = pic_ensure_pin(ctx, (4 + hslots) * sizeof(double));
+    if (rc) return rc;
+    double* hf = ctx->pin;       // the Newton residual sums
+    double* hrr = ctx->pin + 4;  // the inner residual history (striped)
+    NewtonArgs na;
+    na.ieps = 1.0 / EPS0;
+    na.phi0 = p->phi0;
+    na.n0 = p->n0;
+    na.inv_te0 = 1.0 / (p->Te0 != 0 ? p->Te0 : 1.0);
+    unsigned nb = stencil_blocks(s.nn), nbr = stencil_blocks_red(s.nn);
+    int total_it = 0, newton = 0;
+    bool conv = false, stalled = false;
+    for (newton = 0; newton < 20; newton++) {  // NR_MAX_IT, PotentialSolver.cpp:179
+        PIC_HIP(ctx, hipMemsetAsync(ctx->scratch, 0, 4 * sizeof(double), ctx->stream));
+        k_F<double><<<nbr, BLOCK, 0, ctx->stream>>>(s, na, ctx->ntype, ctx->phi, ctx->rho, r, P, ctx->scratch);
+        PIC_HIP(ctx, hipGetLastError());
+        PIC_HIP(ctx, hipMemcpyAsync(hf, ctx->scratch, 3 * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+        PIC_HIP(ctx, pic_spin_wait(ctx->stream));
+        const double bb = hf[2] > 0 ? hf[2] : 1.0;
+        const double target2 = p->tol_kind == PIC_TOL_REF_RMS ? p->tol * p->tol * (double)nn
+                                                              : p->tol * p->tol * bb;
+        if (p->tol_kind == PIC_TOL_REL && hf[1] <= target2) {
+            conv = true;
+            break;
+        }
+        if (!(hf[0] == hf[0])) return pic_fail(ctx, PIC_ERR_NUMERIC, "Newton residual is NaN");
+        const double inner2 = std::max(0.25 * target2, hf[0] * 1e-8);
+        // flexible CG on J y = F from y = 0 (r = F; F vanishes off the regular rows)
+        PIC_HIP(ctx, hipMemsetAsync(y, 0, nn * sizeof(double), ctx->stream));
+        PIC_HIP(ctx, hipMemsetAsync(h.rz, 0, 4 * hslots * sizeof(double), ctx->stream));
+        rc = learned_precond(ctx, s, r, z, ftmp, nullptr, nullptr, h.rz, P);
+        if (rc) return rc;
+        double* pold = p0;
+        double* pnew = p1;
+        int it = 0;
+        bool inner_conv = false;
+        for (int k = 0; k < max_fcg; k++) {
+            {
+                PicProfScope ps(ctx, PIC_K_PCG);
+                k_fcg_pq<<<nbr, BLOCK, 0, ctx->stream>>>(s, ctx->ntype, z, pold, pnew, q, h, k, P);
+                k_fcg_update<<<nb, BLOCK, 0, ctx->stream>>>(s.nn, h, k, y, pnew, r, q);
+            }
+            PIC_HIP(ctx, hipGetLastError());
+            it = k + 1;
+            PIC_HIP(ctx, hipMemcpyAsync(hrr + (size_t)k * SSIZE, h.rr + (size_t)k * SSIZE, SSIZE * sizeof(double),
+                                        hipMemcpyDeviceToHost, ctx->stream));
+            PIC_HIP(ctx, pic_spin_wait(ctx->stream));
+            const double rk = shost(hrr + (size_t)k * SSIZE);
+            if (!(rk == rk)) return pic_fail(ctx, PIC_ERR_NUMERIC, "learned Newton inner solve produced NaN");
+            if (rk <= inner2) {
+                inner_conv = true;
+                break;
+            }
+            if (it == max_fcg) break;
+            rc = learned_precond(ctx, s, r, z, ftmp, q, h.zq + (size_t)k * SSIZE,
+                                 h.rz + (size_t)(k + 1) * SSIZE, P);
+            if (rc) return rc;
+            std::swap(pold, pnew);
+        }
+        total_it += it;
+        if (!inner_conv) {
+            stalled = true;
+            break;
+        }
+        PIC_HIP(ctx, hipMemsetAsync(ctx->scratch, 0, sizeof(double), ctx->stream));
+        k_update_phi<double><<<nbr, BLOCK, 0, ctx->stream>>>(s, ctx->ntype, ctx->phi, y, ctx->scratch);
+        PIC_HIP(ctx, hipGetLastError());
+        double yy;
+        PIC_HIP(ctx, hipMemcpyAsync(hf + 3, ctx->scratch, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+        PIC_HIP(ctx, pic_spin_wait(ctx->stream));
+        yy = hf[3];
+        if (p->tol_kind == PIC_TOL_REF_RMS && std::sqrt(yy / (double)nn) < 1e-3) {  // :229-233
+            conv = true;
+            newton++;
+            break;
+        }
+    }
+    if (stalled || !conv) {  // plain Newton-PCG from the current phi is the fallback
+        pic_solver_params q2 = *p;
+        q2.method = PIC_SOLVER_PCG;
+        pic_solver_stats s2;
+        rc = solve_pcg(ctx, &q2, &s2, false);
+        *st = s2;
+        st->iterations += total_it;
+        st->newton_iterations += newton;
+        return rc;
+    }
+    SorArgs a;
+    a.omega = 1;
+    a.ieps = 1.0 / EPS0;
+    a.itwos = 1.0 / s.twos;
+    a.phi0 = p->phi0;
+    a.n0 = p->n0;
+    a.inv_te0 = na.inv_te0;
+    double rn[3];
+    rc = residual_norms(ctx, s, a, rn);
+    if (rc) return rc;
+    st->converged = 1;
+    st->iterations = total_it;
+    st->newton_iterations = newton;
+    st->ref_l2 = std::sqrt(rn[0] / (double)nn);
+    st->rel_residual = rn[2] > 0 ? std::sqrt(rn[1] / rn[2]) : std::sqrt(rn[1]);
+    return PIC_OK;
+}
+
 // plain red-black SOR to a relative tolerance (used to make training pairs)
 int pic_poisson_internal_sor(pic_ctx* ctx, double tol, int max_it, pic_solver_stats* st) {
     pic_solver_params p{};
@@ -1203,14 +1364,10 @@ extern "C" int pic_poisson(pic_ctx* ctx, const pic_solver_params* p, pic_solver_
     switch (p->method) {
         case PIC_SOLVER_SOR_RB: rc = solve_sor(ctx, p, st); break;
         case PIC_SOLVER_PCG: rc = solve_pcg(ctx, p, st, false); break;
-        case PIC_SOLVER_PCG_MIXED:
-            if (p->n0 != 0.0) {
-                rc = pic_fail(ctx, PIC_ERR_ARG, "mixed precision PCG is linear only");
-                break;
-            }
-            rc = solve_pcg(ctx, p, st, true);
+        case PIC_SOLVER_PCG_MIXED: rc = solve_pcg(ctx, p, st, true); break;
+        case PIC_SOLVER_LEARNED:
+            rc = p->n0 != 0.0 ? solve_newton_learned(ctx, p, st) : solve_learned(ctx, p, st);
             break;
-        case PIC_SOLVER_LEARNED: rc = solve_learned(ctx, p, st); break;
         default: rc = pic_fail(ctx, PIC_ERR_ARG, "unknown solver method");
     }
     (void)hipEventRecord(e1, ctx->stream);

@@ -1415,7 +1415,7 @@ template <int CF>
 __global__ void __launch_bounds__(256)
     k_res_restrict(Dims df, Dims dc, double h2, const uint8_t* __restrict__ nt,
                    const double* __restrict__ t, const float* __restrict__ z, float* __restrict__ x2,
-                   double* __restrict__ sums, float* __restrict__ res) {
+                   double* __restrict__ sums, float* __restrict__ res, const double* __restrict__ P) {
     double s2 = 0, cnt = 0;
     const unsigned sj = df.nx, sk = (unsigned)df.nx * df.ny;
     const double ih2 = 1.0 / h2;
@@ -1443,6 +1443,7 @@ __global__ void __launch_bounds__(256)
                 double az = (zat(a - 1, b, c) + zat(a + 1, b, c) + zat(a, b - 1, c) + zat(a, b + 1, c) +
                              zat(a, b, c - 1) + zat(a, b, c + 1) - 6.0 * zat(a, b, c)) *
                             ih2;
+                if (P) az -= P[u] * zat(a, b, c);  // J z, J = A - diag(P)
                 double rv = reg ? t[u] - az : 0.0;
                 acc += rv;
                 nreg += reg;
@@ -1458,6 +1459,7 @@ __global__ void __launch_bounds__(256)
                 double az = ((double)z[u - oi] + z[u + oi] + z[u - oj] + z[u + oj] + z[u - ok] + z[u + ok] -
                              6.0 * z[u]) *
                             ih2;
+                if (P) az -= P[u] * (double)z[u];
                 double rv = reg ? t[u] - az : 0.0;
                 acc += rv;
                 nreg += reg;
@@ -1515,7 +1517,7 @@ template <int CF>
 __global__ void __launch_bounds__(256)
     k_prolong_dots(Dims df, Dims dc, double ih2, const uint8_t* __restrict__ nt,
                    const float* __restrict__ y, const float* __restrict__ res, float* __restrict__ c,
-                   double* __restrict__ out) {
+                   double* __restrict__ out, const double* __restrict__ P) {
     const long long sj = df.nx, sk = (long long)df.nx * df.ny;
     double acc[2] = {0, 0};
     long long end;
@@ -1543,6 +1545,7 @@ __global__ void __launch_bounds__(256)
             if (nt[u - st[a]] != NT_REG) e += cu * cu;
         }
         acc[1] -= e * ih2;
+        if (P) acc[1] -= P[u] * cu * cu;  // c.J c = c.A c - sum P c^2
     }
     block_sums<2>(acc);
     if (threadIdx.x == 0) {
@@ -1565,11 +1568,11 @@ __global__ void __launch_bounds__(256)
 __global__ void __launch_bounds__(256)
     k_prolong_dots_agg(Dims df, Dims dc, double ih2, const uint8_t* __restrict__ nt,
                        const float* __restrict__ y, const float* __restrict__ res,
-                       float* __restrict__ c, double* __restrict__ out) {
+                       float* __restrict__ c, double* __restrict__ out, const double* __restrict__ P) {
     const int ex = (df.nx + 1) >> 1, ey = (df.ny + 1) >> 1, ez = (df.nz + 1) >> 1;
     const long long na = (long long)ex * ey * ez;
     const unsigned sj = df.nx, sk = (unsigned)df.nx * df.ny;
-    double acc0 = 0, acc1 = 0;
+    double acc0 = 0, acc1 = 0, accp = 0;
     long long end;
     for (long long v = xcd_chunk(na, 256, end) + threadIdx.x; v < end; v += 256) {
         const unsigned vv = (unsigned)v;
@@ -1632,12 +1635,14 @@ __global__ void __launch_bounds__(256)
             c[u] = (float)cu;  // exact: interpolated in FP32
             const double rv = res[u];
             acc0 += cu != 0.0 ? cu * rv : 0.0;  // res is defined on regular nodes only
+            if (P && cu != 0.0) accp += P[u] * cu * cu;
             if (fx + 1 < df.nx) e += (cu - C[cc][b][a + 1]) * (cu - C[cc][b][a + 1]);
             if (fy + 1 < df.ny) e += (cu - C[cc][b + 1][a]) * (cu - C[cc][b + 1][a]);
             if (fz + 1 < df.nz) e += (cu - C[cc + 1][b][a]) * (cu - C[cc + 1][b][a]);
         }
         acc1 -= e * ih2;
     }
+    acc1 -= accp;  // c.J c = c.A c - sum P c^2 (the Newton Jacobian)
     double acc[2] = {acc0, acc1};
     block_sums<2>(acc);
     if (threadIdx.x == 0) {
@@ -2308,7 +2313,7 @@ bool pic_unet_ready(pic_ctx* ctx) {
 // z += omega c.  Only for Dirichlet-walled uniform grids (every regular node
 // is interior).
 int pic_unet_coarse_correct(pic_ctx* ctx, const double* t, const float* z, float* c, float* res,
-                            double* ws) {
+                            double* ws, const double* P) {
     UNet* u = get_unet(ctx);
     if (!u) return pic_fail(ctx, PIC_ERR_STATE, "no U-Net");
     const GridC& g = ctx->g;
@@ -2319,17 +2324,18 @@ int pic_unet_coarse_correct(pic_ctx* ctx, const double* t, const float* z, float
     double* dots = ws + 2 * SSIZE;
     unsigned nbc = xcd_blocks1(dc.v, 256), nbf = xcd_blocks(df.v, 256);
     if (u->coarsen == 2)
-        k_res_restrict<2><<<nbc, 256, 0, ctx->stream>>>(df, dc, h2, ctx->ntype, t, z, u->x, sums, res);
+        k_res_restrict<2><<<nbc, 256, 0, ctx->stream>>>(df, dc, h2, ctx->ntype, t, z, u->x, sums, res, P);
     else
-        k_res_restrict<1><<<nbc, 256, 0, ctx->stream>>>(df, dc, h2, ctx->ntype, t, z, u->x, sums, res);
+        k_res_restrict<1><<<nbc, 256, 0, ctx->stream>>>(df, dc, h2, ctx->ntype, t, z, u->x, sums, res, P);
     int rc = unet_forward(ctx, u, u->x, sums);
     if (rc) return rc;
     if (u->coarsen == 2) {
         long long na = (long long)((g.ni + 1) / 2) * ((g.nj + 1) / 2) * ((g.nk + 1) / 2);
         k_prolong_dots_agg<<<xcd_blocks1(na, 256), 256, 0, ctx->stream>>>(df, dc, 1.0 / h2, ctx->ntype,
-                                                                          u->y, res, c, dots);
+                                                                          u->y, res, c, dots, P);
     } else
-        k_prolong_dots<1><<<nbf, 256, 0, ctx->stream>>>(df, dc, 1.0 / h2, ctx->ntype, u->y, res, c, dots);
+        k_prolong_dots<1><<<nbf, 256, 0, ctx->stream>>>(df, dc, 1.0 / h2, ctx->ntype, u->y, res, c, dots,
+                                                        P);
     PIC_HIP(ctx, hipGetLastError());
     return PIC_OK;
 }
@@ -2429,9 +2435,11 @@ extern "C" int pic_unet_train(pic_ctx* ctx, const pic_unet_train_desc* td,
         (void)hipMemsetAsync(ss, 0, 3 * SSIZE * sizeof(double), ctx->stream);  // striped
         unsigned nbc = xcd_blocks1(Vc, 256), nbf = xcd_blocks(V, 256);
         if (u->coarsen == 2)
-            k_res_restrict<2><<<nbc, 256, 0, ctx->stream>>>(df, dc, dx2, ctx->ntype, tf, zp, xd, ss, nullptr);
+            k_res_restrict<2><<<nbc, 256, 0, ctx->stream>>>(df, dc, dx2, ctx->ntype, tf, zp, xd, ss, nullptr,
+                                                            nullptr);
         else
-            k_res_restrict<1><<<nbc, 256, 0, ctx->stream>>>(df, dc, dx2, ctx->ntype, tf, zp, xd, ss, nullptr);
+            k_res_restrict<1><<<nbc, 256, 0, ctx->stream>>>(df, dc, dx2, ctx->ntype, tf, zp, xd, ss, nullptr,
+                                                            nullptr);
         k_res_rho<<<nbf, 256, 0, ctx->stream>>>(df, 1.0 / dx2, ctx->ntype, tf, zp, ctx->rho);
         (void)hipMemsetAsync(ctx->phi, 0, V * 8, ctx->stream);
         pic_solver_stats st;

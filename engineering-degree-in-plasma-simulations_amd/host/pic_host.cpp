@@ -4,9 +4,13 @@
 #include "pic_host.hpp"
 
 #include <fstream>
+#include <iomanip>
 #include <iostream>
 #include <limits>
 #include <map>
+#include <sstream>
+
+#include "../../include/pic_io.h"
 
 // ---------------------------------------------------------------- World
 World::World(int ni_, int nj_, int nk_, type_calc3 x0_, type_calc3 xm_)
@@ -88,6 +92,7 @@ Species::Species(std::string name_, type_calc mass_, type_calc charge_, World& w
       E_ion(E_ion_),
       den(w.ni, w.nj, w.nk),
       T(w.ni, w.nj, w.nk),
+      den_avg(w.ni, w.nj, w.nk),
       vel(w.ni, w.nj, w.nk),
       world(w) {
     w.check(pic_species_add(w.ctx, mass, charge, mpw0, capacity, &sid), "Species");
@@ -131,17 +136,39 @@ size_t Species::getNumParticles() const {
     world.check(pic_particles_count(world.ctx, sid, &n), "getNumParticles");
     return n;
 }
-void Species::advanceElectrons(type_calc dt) {
+void Species::advanceElectrons(type_calc dt) {  // objects absorb (Species.cpp:333-338)
     sync_to_device();
     world.flush_host_fields();
-    world.check(pic_push(world.ctx, sid, dt), "advanceElectrons");
-    world.check(pic_sort(world.ctx, sid), "advanceElectrons");
+    world.check(pic_species_set_surface(world.ctx, sid, PIC_SURF_ABSORB, -1, surface_seed),
+                "advanceElectrons");
+    world.check(pic_push_sort(world.ctx, sid, dt), "advanceElectrons");
     host_valid = false;
 }
+// advanceNoSputteringSerial (Species.cpp:170-256): neutrals reflect at
+// objects, ions are recycled into `neutrals`.  Sputtering into `spherium`
+// (advanceSputteringSerial, :79-168) is off in the reference's runs
+// (Config SPUTTERING) and not built.
 void Species::advanceNonElectron(Species& neutrals, Species& spherium, type_calc dt) {
-    (void)neutrals;
-    (void)spherium;  // surface emission is not on this path: object hits absorb
-    advanceElectrons(dt);
+    (void)spherium;
+    sync_to_device();
+    neutrals.sync_to_device();
+    world.flush_host_fields();
+    const bool neutral = charge == 0;
+    // an ion species passed as its own neutrals (a run without neutrals)
+    // has nowhere to recycle into: its object hits absorb
+    const int mode = neutral ? PIC_SURF_REFLECT : (neutrals.sid == sid ? PIC_SURF_ABSORB : PIC_SURF_RECYCLE);
+    world.check(pic_species_set_surface(world.ctx, sid, mode, mode == PIC_SURF_RECYCLE ? neutrals.sid : -1,
+                                        surface_seed),
+                "advanceNonElectron");
+    world.check(pic_push_sort(world.ctx, sid, dt), "advanceNonElectron");
+    host_valid = false;
+    if (!neutral) neutrals.host_valid = false;
+}
+void Species::merge() {
+    sync_to_device();
+    world.flush_host_fields();
+    world.check(pic_merge(world.ctx, sid, merge_seed, &last_merge), "merge");
+    host_valid = false;
 }
 void Species::computeNumberDensity() {
     sync_to_device();
@@ -175,6 +202,17 @@ void Species::computeGasProperties() {
                d = ww[u] / c - V[u][2] * V[u][2];
         TT[u] = mass / (2 * Const::k) * (a + b + d);
     }
+}
+// Species::updateAverages -> Field::updateMovingAverage (Field.h:246-261):
+// the mean of up to 20 samples, restarted after the 20th
+void Species::updateAverages() {
+    std::vector<double>& a = den_avg.raw();
+    const std::vector<double>& d = den.raw();
+    if (++avg_samples > 20) {
+        std::fill(a.begin(), a.end(), 0.0);
+        avg_samples = 1;
+    }
+    for (size_t u = 0; u < a.size(); u++) a[u] = (d[u] + a[u] * (avg_samples - 1.0)) / avg_samples;
 }
 void Species::computeMacroParticlesCount() {
     sync_to_device();
@@ -316,3 +354,89 @@ void MC_MEX_Ionization::apply(type_calc dt) noexcept {
     last_status = pic_mcc_apply(world.ctx, id, dt, seed, step++, &last);
     if (last_status != PIC_OK) std::cerr << "MC_MEX_Ionization: " << pic_last_error(world.ctx) << "\n";
 }
+
+// ----------------------------------------------------------------- Output
+namespace Output {
+
+static std::string step_name(const std::string& stem, int ts, const char* ext) {
+    std::stringstream name;  // Outputs.cpp:17, :220
+    name << stem << std::setfill('0') << std::setw(5) << ts << ext;
+    return name.str();
+}
+
+void fieldsOutput(World& world, std::vector<Species>& species, std::string name1) {
+    for (Species& sp : species) {
+        sp.computeGasProperties();
+        sp.computeMacroParticlesCount();
+    }
+    const std::string path = step_name("results/" + name1 + "fields_", world.getTs(), ".vti");
+    const size_t nn = (size_t)world.nv;
+    std::vector<int32_t> oid(nn), nt(nn);
+    world.check(pic_get_node_flags(world.ctx, oid.data(), nt.data()), "fieldsOutput");
+    std::vector<double> ef(3 * nn);
+    const std::vector<type_calc3>& e = world.ef.raw();
+    for (size_t u = 0; u < nn; u++)
+        for (int c = 0; c < 3; c++) ef[3 * u + c] = e[u][c];
+    std::vector<std::vector<double>> vel(species.size());
+    std::vector<pic_io_species> sp(species.size());
+    for (size_t s = 0; s < species.size(); s++) {
+        Species& x = species[s];
+        vel[s].resize(3 * nn);
+        const std::vector<type_calc3>& v = x.vel.raw();
+        for (size_t u = 0; u < nn; u++)
+            for (int c = 0; c < 3; c++) vel[s][3 * u + c] = v[u][c];
+        sp[s] = pic_io_species{x.name.c_str(), x.den.raw().data(), x.den_avg.raw().data(), vel[s].data(),
+                               x.T.raw().data(), x.macro_part_count.data()};
+    }
+    const type_calc3 x0 = world.getX0(), dx = world.getDx();
+    pic_io_fields f{world.ni, world.nj, world.nk, {x0[0], x0[1], x0[2]}, {dx[0], dx[1], dx[2]},
+                    world.node_vol.raw().data(), oid.data(), nt.data(), world.phi.raw().data(),
+                    world.rho.raw().data(), ef.data(), (int)sp.size(), sp.data()};
+    if (pic_io_write_vti(path.c_str(), &f) != 0) {
+        std::cerr << "Could not open " << path << std::endl;
+        return;
+    }
+    for (Species& x : species) x.clearSamples();  // Outputs.cpp:119-121 (no steady-state test here)
+}
+
+void particlesOutput(World& world, std::vector<Species>& species, int num_parts_to_output_base,
+                     std::string name1) {
+    for (Species& sp : species) {
+        const std::string path =
+            step_name("results/" + name1 + "parts_" + sp.name + "_", world.getTs(), ".vtp");
+        const std::vector<Particle>& p = sp.getConstPartRef();
+        if (pic_io_write_vtp(path.c_str(), sp.name.c_str(), reinterpret_cast<const double*>(p.data()), p.size(),
+                             num_parts_to_output_base) != 0) {
+            std::cerr << "Could not open " << path << std::endl;
+            return;
+        }
+    }
+}
+
+static bool diag_started = false;
+
+void diagOutput(World& world, std::vector<Species>& species) {
+    std::vector<const char*> names;
+    std::vector<double> d;
+    for (Species& sp : species) {
+        names.push_back(sp.name.c_str());
+        type_calc3 m = sp.getMomentum();
+        d.insert(d.end(), {(double)sp.getNumParticles(), sp.getMicroCount(), m[0], m[1], m[2], sp.getKE()});
+    }
+    const double pe = world.getPE();
+    if (pic_io_diag_row("results/runtime_diags.csv", diag_started ? 0 : 1, (int)names.size(), names.data(),
+                        world.getTs(), world.getTime(), world.getWallTime(), d.data(), pe) != 0) {
+        std::cerr << "Could not open results/runtime_diags.csv" << std::endl;
+        return;
+    }
+    diag_started = true;
+}
+
+void screenOutput(World& world, std::vector<Species>& species) {  // Outputs.cpp:127-136
+    std::cout << "\r                                                                  \rts: " << world.getTs();
+    for (Species& sp : species)
+        std::cout << std::setprecision(3) << "\t " << sp.name << ":" << sp.getNumParticles() << " ";
+    std::cout << std::flush;
+}
+
+}  // namespace Output

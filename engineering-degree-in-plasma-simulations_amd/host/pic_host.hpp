@@ -190,6 +190,7 @@ class Species {
     const type_calc mass, charge, mpw0;
     const type_calc E_ion = -666;
     Field<type_calc> den, T;
+    Field<type_calc> den_avg;  // running mean of den over up to 20 samples (Field.h:246-261)
     Field<type_calc3> vel;
     std::vector<type_calc> macro_part_count;  // per cell, c = (k*(nj-1)+j)*(ni-1)+i
 
@@ -207,7 +208,8 @@ class Species {
     void loadParticleBoxThermal(type_calc3 x0, type_calc3 sides, type_calc num_den,
                                 type_calc T);                                  // :564-599
     void sortIndexes();                                                        // :905-929
-    void updateAverages() {}  // moving averages are host-side output work (not on the path)
+    void merge();                                                              // :987-1145
+    void updateAverages();                                                     // :764-766
     std::vector<Particle>& getPartRef();              // host mirror, pushed back on next use
     const std::vector<Particle>& getConstPartRef();
     type_calc getMicroCount();       // :732-738
@@ -217,13 +219,27 @@ class Species {
     int sid = -1;
     World& world;
     uint32_t load_seed = 1;
+    uint32_t surface_seed = 1;  // draws of object reflections / emissions
+    uint32_t merge_seed = 1;    // draws of merge's position picks
+    pic_merge_stats last_merge{};
 
   private:
     std::vector<Particle> host;
     bool host_valid = false, host_dirty = false;
+    int avg_samples = 0;
     void sync_to_device();
     void diag(double out[6]);
 };
+
+// Outputs.h / Outputs.cpp:9-205: the reference's files (results/ under the
+// working directory), written through libpic_io (include/pic_io.h)
+namespace Output {
+void fieldsOutput(World& world, std::vector<Species>& species, std::string name1 = "");
+void diagOutput(World& world, std::vector<Species>& species);
+void particlesOutput(World& world, std::vector<Species>& species, int num_parts_to_output_base,
+                     std::string name1 = "");
+void screenOutput(World& world, std::vector<Species>& species);
+}  // namespace Output
 
 enum SolverType { GS, PCG, QN };  // PotentialSolver.h:24
 

@@ -5,7 +5,7 @@
 //   pic_driver --config plain     [--nodes N] [--particles P] [--num_ts K]
 //              --config discharge [--nodes N] [--particles P] [--num_ts K]
 //   common:    [--tol T] [--tol-rel] [--solver GS|PCG] [--dump PREFIX]
-//              [--diag FILE.csv] [--seed S]
+//              [--diag FILE.csv] [--output N] [--seed S]
 //
 // plain:     Dirichlet phi = 0 box, dx = 1e-2 m, kinetic e- (2 eV) and O+
 //            (16 amu, 0.1 eV), n0 = 1e12 m^-3, dt = 1e-10 s, no subcycling,
@@ -22,6 +22,7 @@
 #include <string>
 #include <vector>
 
+#include "../../include/pic_io.h"
 #include "pic_host.hpp"
 
 static const char* arg(int argc, char** argv, const char* key, const char* def) {
@@ -43,19 +44,19 @@ static void dump(const std::string& path, Species& sp) {
     std::fclose(f);
 }
 
-// runtime diagnostics in the columns of Output::diagOutput (Outputs.cpp:143-179)
-static void diag_line(std::ostream& out, World& w, std::vector<Species>& sps) {
-    out << w.getTs() << "," << w.getTime() << "," << w.getWallTime();
-    double ke = 0;
+// runtime diagnostics in the columns of Output::diagOutput
+// (Outputs.cpp:143-179), to a file of the caller's choosing
+static void diag_line(const std::string& path, bool header, World& w, std::vector<Species>& sps) {
+    std::vector<const char*> names;
+    std::vector<double> d;
     for (Species& s : sps) {
+        names.push_back(s.name.c_str());
         type_calc3 m = s.getMomentum();
-        double k = s.getKE();
-        ke += k;
-        out << "," << s.getNumParticles() << "," << s.getMicroCount() << "," << m[0] << "," << m[1]
-            << "," << m[2] << "," << k;
+        d.insert(d.end(), {(double)s.getNumParticles(), s.getMicroCount(), m[0], m[1], m[2], s.getKE()});
     }
-    double pe = w.getPE();
-    out << "," << pe << "," << ke + pe << "\n";
+    if (pic_io_diag_row(path.c_str(), header ? 1 : 0, (int)names.size(), names.data(), w.getTs(), w.getTime(),
+                        w.getWallTime(), d.data(), w.getPE()) != 0)
+        throw std::runtime_error("cannot write " + path);
 }
 
 int main(int argc, char** argv) {
@@ -68,8 +69,12 @@ int main(int argc, char** argv) {
     std::string dump_prefix = arg(argc, argv, "--dump", "");
     std::string diag_path = arg(argc, argv, "--diag", "");
     uint32_t seed = (uint32_t)std::atoi(arg(argc, argv, "--seed", "1"));
-    std::ofstream diag;
-    if (!diag_path.empty()) diag.open(diag_path);
+    // --output N: the reference's files under results/ (Outputs.cpp): the
+    // fields and 1000 sampled particles per species every N steps, the
+    // runtime diagnostics every 10
+    const int output_every = std::atoi(arg(argc, argv, "--output", "0"));
+    if (output_every > 0) std::system("mkdir -p results");
+    bool diag_header = true;
 
     if (config == "plain") {
         double dx = 1e-2, L = dx * (n - 1), n0 = 1e12, dt = 1e-10;
@@ -104,8 +109,21 @@ int main(int argc, char** argv) {
             world.computeChargeDensity(species);
             if (!solver_obj.solve()) return 3;
             solver_obj.computeEF();
-            if (diag.is_open() && (world.getTs() % 10 == 0 || world.isLastTimeStep()))
-                diag_line(diag, world, species);
+            if (!diag_path.empty() && (world.getTs() % 10 == 0 || world.isLastTimeStep())) {
+                diag_line(diag_path, diag_header, world, species);
+                diag_header = false;
+            }
+            if (output_every > 0) {
+                if (world.getTs() % 10 == 0) Output::diagOutput(world, species);
+                if (world.getTs() % output_every == 0) {
+                    for (Species& sp : species) {
+                        sp.sampleMoments();
+                        sp.updateAverages();
+                    }
+                    Output::fieldsOutput(world, species);
+                    Output::particlesOutput(world, species, 1000);
+                }
+            }
         }
         if (!dump_prefix.empty()) {
             dump(dump_prefix + "_e1.bin", species[0]);
@@ -182,8 +200,21 @@ int main(int argc, char** argv) {
                 neu.computeNumberDensity();
             }
             world.computeChargeDensity(species);
-            if (diag.is_open() && (world.getTs() % 10 == 0 || world.isLastTimeStep()))
-                diag_line(diag, world, species);
+            if (!diag_path.empty() && (world.getTs() % 10 == 0 || world.isLastTimeStep())) {
+                diag_line(diag_path, diag_header, world, species);
+                diag_header = false;
+            }
+            if (output_every > 0) {
+                if (world.getTs() % 10 == 0) Output::diagOutput(world, species);
+                if (world.getTs() % output_every == 0) {
+                    for (Species& sp : species) {
+                        sp.sampleMoments();
+                        sp.updateAverages();
+                    }
+                    Output::fieldsOutput(world, species);
+                    Output::particlesOutput(world, species, 1000);
+                }
+            }
         }
         std::printf("{\"config\": \"discharge\", \"steps\": %d, \"neutrals\": %zu, \"ions\": %zu, "
                     "\"electrons\": %zu, \"collisions\": %lld, \"ionizations\": %lld, "

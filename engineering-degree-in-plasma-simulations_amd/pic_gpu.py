@@ -27,6 +27,7 @@ SOLVE_EXTRAPOLATE = 2
 FIELD_PHI, FIELD_RHO, FIELD_EF, FIELD_NODE_VOL = 0, 1, 2, 3
 SFIELD_DEN, SFIELD_NSUM, SFIELD_NVSUM, SFIELD_NUU, SFIELD_NVV, SFIELD_NWW, SFIELD_MACRO_COUNT = range(7)
 DEPOSIT_DENSITY, DEPOSIT_MOMENTS = 1, 2
+SURF_ABSORB, SURF_REFLECT, SURF_RECYCLE = 0, 1, 2
 K_NAMES = ["push", "sort", "deposit", "rho", "efield", "sor", "pcg", "mcc", "load", "unet",
            "scatter"]
 
@@ -41,7 +42,7 @@ ABI_FUNCTIONS = [
     "pic_mcc_create", "pic_mcc_apply", "pic_profile_enable", "pic_profile_read",
     "pic_profile_reset", "pic_unet_create", "pic_unet_param_count", "pic_unet_get_params",
     "pic_unet_set_params", "pic_unet_forward", "pic_unet_loss_grad", "pic_unet_adam_step",
-    "pic_unet_train", "pic_learned_precondition",
+    "pic_unet_train", "pic_learned_precondition", "pic_species_set_surface", "pic_merge",
 ]
 
 
@@ -63,6 +64,10 @@ class UNetTrainDesc(C.Structure):
 class UNetTrainStats(C.Structure):
     _fields_ = [("loss_first", C.c_double), ("loss_last", C.c_double),
                 ("seconds_data", C.c_double), ("seconds_train", C.c_double), ("steps", C.c_int)]
+
+
+class MergeStats(C.Structure):
+    _fields_ = [("bins", C.c_longlong), ("merged", C.c_longlong), ("removed", C.c_longlong)]
 
 
 class GridDesc(C.Structure):
@@ -90,7 +95,7 @@ class MccDesc(C.Structure):
 
 class MccStats(C.Structure):
     _fields_ = [("collisions", C.c_longlong), ("ionizations", C.c_longlong),
-                ("groups", C.c_longlong), ("wsv_max", C.c_double)]
+                ("groups", C.c_longlong), ("wsv_max", C.c_double), ("pool_overflow", C.c_longlong)]
 
 
 _dp = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
@@ -156,6 +161,8 @@ def load_library():
     L.pic_unet_adam_step.argtypes = [P, _fp, C.c_float]
     L.pic_unet_train.argtypes = [P, C.POINTER(UNetTrainDesc), C.POINTER(UNetTrainStats)]
     L.pic_learned_precondition.argtypes = [P, _dp, _dp]
+    L.pic_species_set_surface.argtypes = [P, C.c_int, C.c_int, C.c_int, C.c_uint32]
+    L.pic_merge.argtypes = [P, C.c_int, C.c_uint32, C.POINTER(MergeStats)]
     _lib = L
     return L
 
@@ -262,6 +269,15 @@ class Context:
         """push then sort in two passes (pic_push_sort)"""
         self._ck(self.L.pic_push_sort(self.h, sid, dt))
 
+    def set_surface(self, sid, mode, target=-1, seed=1):
+        """behaviour at objects in later pushes (SURF_ABSORB / REFLECT / RECYCLE)"""
+        self._ck(self.L.pic_species_set_surface(self.h, sid, mode, target, seed))
+
+    def merge(self, sid, seed=1):
+        st = MergeStats()
+        self._ck(self.L.pic_merge(self.h, sid, seed, C.byref(st)))
+        return {"bins": st.bins, "merged": st.merged, "removed": st.removed}
+
     def deposit(self, sid, flags=DEPOSIT_DENSITY):
         self._ck(self.L.pic_deposit(self.h, sid, flags))
 
@@ -336,7 +352,7 @@ class Context:
         st = MccStats()
         self._ck(self.L.pic_mcc_apply(self.h, mid, dt, seed, step, C.byref(st)))
         return {"collisions": st.collisions, "ionizations": st.ionizations,
-                "groups": st.groups, "wsv_max": st.wsv_max}
+                "groups": st.groups, "wsv_max": st.wsv_max, "pool_overflow": st.pool_overflow}
 
     # learned preconditioner
     def unet_create(self, channels=(16, 32, 64), seed=1, coarsen=1):

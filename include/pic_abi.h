@@ -94,8 +94,9 @@ int pic_load_box_thermal(pic_ctx* ctx, int sid, const double centre[3], const do
 
 /* advanceElectrons(dt) / advanceNonElectron(..., dt) (Species.cpp:258-399,
  * :47-77, :170-256): gather E (CIC), kick, drift, remove particles leaving
- * the box or entering an object.  Removed particles disappear at the next
- * pic_sort.  Object hits are absorbing for every species. */
+ * the box; a particle ending in an object follows the species' surface mode
+ * (pic_species_set_surface; removed by default).  Removed particles
+ * disappear at the next pic_sort. */
 int pic_push(pic_ctx* ctx, int sid, double dt);
 /* sortIndexes() (Species.cpp:905-929): counting sort of the particle arrays
  * by cell; compacts removed particles; cell offsets stay on the device. */
@@ -108,6 +109,44 @@ int pic_sort(pic_ctx* ctx, int sid);
  * run of equal cells' first slot, so the second needs no atomics).  Same
  * result as the two calls, with about 25% less particle traffic. */
 int pic_push_sort(pic_ctx* ctx, int sid, double dt);
+/* Behaviour of a species at objects during pic_push / pic_push_sort
+ * (advanceNonElectron -> advanceNoSputteringSerial, Species.cpp:170-256;
+ * advanceElectrons removes, Species.cpp:333-338):
+ *   PIC_SURF_ABSORB   removed (the default for every species);
+ *   PIC_SURF_REFLECT  moved to the entry point, diffusely reflected at 300 K
+ *                     (sampleReflectedVelocity, Species.cpp:835-861) and
+ *                     drifted on for the rest of the step; removed after 20
+ *                     bounces (neutrals, Species.cpp:216-219);
+ *   PIC_SURF_RECYCLE  removed, emitting int(w / mpw0_t + U) particles of
+ *                     species sid_target at the entry point with reflected
+ *                     velocities (ions -> neutrals, Species.cpp:220-236).
+ * An object hit is decided and traced in FP64 (Rectangle::lineIntersect,
+ * Object.cpp:271-317).  Draws: Philox4x32-10 keyed by (seed, the species'
+ * push count), counter (particle index, bounce).  Emitted particles are
+ * appended to the target; the push fails with PIC_ERR_CAPACITY when they
+ * do not fit. */
+#define PIC_SURF_ABSORB 0
+#define PIC_SURF_REFLECT 1
+#define PIC_SURF_RECYCLE 2
+int pic_species_set_surface(pic_ctx* ctx, int sid, int mode, int sid_target, uint32_t seed);
+/* Species::merge() (Species.cpp:987-1145, called at main.cpp:179-195): in
+ * every cell of at least 10 particles the velocities are binned 15 x 15 x 15
+ * over the cell's range; each bin of more than two particles becomes two
+ * particles of half its weight each, v = mean +- sqrt(variance) per axis
+ * (weight, momentum and per-axis energy conserved), placed at two distinct
+ * random members' positions; charged species get addParticle's half-step
+ * rewind.  The species is sorted and compacted afterwards.  Defined where
+ * the reference is not: the range is the cell's own (the source also
+ * counts particles[0], :1020), a negative variance is clamped to 0 (the
+ * source stops clamping at one below -1e-4, :1091-1098).  Draws: Philox
+ * keyed by (seed, the species' merge count), counter (cell, bin).  A cell
+ * of more than 8192 particles fails with PIC_ERR_CAPACITY. */
+typedef struct {
+    long long bins;     /* bins merged */
+    long long merged;   /* particles in those bins */
+    long long removed;  /* net decrease of the particle count */
+} pic_merge_stats;
+int pic_merge(pic_ctx* ctx, int sid, uint32_t seed, pic_merge_stats* stats);
 /* computeNumberDensity() (Species.cpp:401-416); with PIC_DEPOSIT_MOMENTS
  * also sampleMoments() (Species.cpp:767-776), accumulated until
  * pic_clear_samples (Species.cpp:805-812).  Unsorted particles are sorted
@@ -121,11 +160,15 @@ int pic_charge_density(pic_ctx* ctx);
 
 /* ---- field solve ---- */
 /* PotentialSolver::solve() (PotentialSolver.cpp:55-67): GS-SOR (:69-166),
- * Newton + PCG (:178-313), plus the learned-preconditioned variant. */
+ * Newton + PCG (:178-313), plus the learned-preconditioned variant.  With
+ * n0 != 0 (Boltzmann electrons) the PCG, mixed and learned methods run the
+ * FP64 Newton loop (:178-241) around their inner solve of J y = F,
+ * J = A - diag(P); PIC_TOL_REL then stops on the FP64 nonlinear residual,
+ * PIC_TOL_REF_RMS on the reference's RMS(y) < 1e-3. */
 #define PIC_SOLVER_SOR_RB 0  /* red-black SOR, Boltzmann term, Neumann faces */
 #define PIC_SOLVER_PCG 1     /* Newton outer loop + Jacobi PCG, FP64 */
-#define PIC_SOLVER_PCG_MIXED 2 /* FP64 residual + FP32 inner PCG (refinement) */
-#define PIC_SOLVER_LEARNED 3 /* U-Net warm start + preconditioner, flexible PCG */
+#define PIC_SOLVER_PCG_MIXED 2 /* FP64 residual / Newton + FP32 inner PCG (refinement) */
+#define PIC_SOLVER_LEARNED 3 /* U-Net warm start + preconditioner, flexible CG (Dirichlet walls) */
 #define PIC_TOL_REF_RMS 0    /* reference criterion: sqrt(sum R^2 / N) < tol */
 #define PIC_TOL_REL 1        /* ||b - A phi||_2 / ||b||_2 < tol (regular nodes) */
 #define PIC_SOLVE_AXIS_SWAP 1 /* PCG coefficients placed as in PotentialSolver.cpp:457-463 */
@@ -190,14 +233,21 @@ typedef struct {
 typedef struct {
     long long collisions, ionizations, groups;
     double wsv_max; /* W_sigma_v_rel_max after the call */
+    /* split neutrals that did not join their cell's pick pool (more than 32
+     * in one cell in one call, or the workgroup's 512 staging slots full;
+     * the reference adds every one, :698-700) */
+    long long pool_overflow;
 } pic_mcc_stats;
 /* MC_MEX_Ionization(neutrals, ions, electrons, world, table_path) ctor
  * (Interactions.cpp:476-539); the elastic table is passed as arrays. */
 int pic_mcc_create(pic_ctx* ctx, int sid_neutral, int sid_ion, int sid_electron,
                    const pic_mcc_desc* desc, const double* sig_E, const double* sig_v, int nsig,
                    int* mcc_id);
-/* Interaction::apply(dt) (Interactions.cpp:567-598 -> :600-762).  Needs all
- * three species sorted.  Draws: Philox4x32-10 keyed by (seed, step). */
+/* Interaction::apply(dt) (Interactions.cpp:567-598 -> :600-762).  Sorts the
+ * neutrals and electrons first when needed.  A cell's pick pool holds its
+ * neutrals and, as in the reference, the neutrals split off earlier in the
+ * same call (:698-700).  Draws: Philox4x32-10 keyed by (seed, step),
+ * counter (cell, group). */
 int pic_mcc_apply(pic_ctx* ctx, int mcc_id, double dt, uint32_t seed, uint32_t step,
                   pic_mcc_stats* stats);
 

@@ -18,11 +18,14 @@
 #include <cstring>
 #include <fstream>
 #include <random>
+#include <sys/stat.h>
+#include <unistd.h>
 #include <string>
 #include <vector>
 
 #include "Config.h"
 #include "Interactions.h"
+#include "Outputs.h"
 #include "PotentialSolver.h"
 #include "Rnd.h"
 #include "Species.h"
@@ -390,6 +393,106 @@ static void golden(const std::string& dir, const std::string& table_path) {
         rnd = Rnd(27182u);
         me.merge();
         dump("mg_ele_out", parts(me));
+    }
+    // ---- Output writers (Outputs.cpp:9-205): the reference writes its own
+    // files for a small world into <dir>/results; the inputs they print are
+    // dumped beside them ----
+    {
+        char cwd[4096];
+        if (!getcwd(cwd, sizeof(cwd))) exit(2);
+        if (chdir(dir.c_str()) != 0) exit(2);
+        mkdir("results", 0755);
+        const int OI = 5, OJ = 4, OK = 3;
+        type_calc3 y0(-0.01, 0.0, 0.02), ym(0.03, 0.045, 0.05);
+        World wo(OI, OJ, OK, y0, ym);
+        wo.setTime(1e-9, 10);
+        for (int t = 0; t < 3; t++) wo.advanceTime();
+        wo.addObject<Rectangle>(type_calc3(0.01, 0.0225, 0.035), 2.0, type_calc3(0.012, 0.016, 0.014));
+        wo.computeObjectID();
+        wo.dirichletBoundries();
+        for (int i = 0; i < OI; i++)
+            for (int j = 0; j < OJ; j++)
+                for (int k = 0; k < OK; k++) {
+                    type_calc3 x = wo.LtoX(i, j, k);
+                    wo.phi[i][j][k] = 3.0 * std::sin(80.0 * x[0]) + 1e-7 * j - 0.25 * k;
+                    wo.rho[i][j][k] = 1e-9 * (i - 2.0 * j + 0.5 * k) / 3.0;
+                }
+        {
+            PotentialSolver ps(wo, 10, 1e-3, GS);
+            ps.computeEF();
+        }
+        std::mt19937 gen(99u);
+        std::uniform_real_distribution<double> U(0.0, 1.0);
+        type_calc3 L = wo.getL();
+        auto cloud = [&](int n, double vs, double w) {
+            std::vector<double> v;
+            for (int q = 0; q < n; q++) {
+                for (int d = 0; d < 3; d++) v.push_back(y0[d] + (0.01 + 0.98 * U(gen)) * L[d]);
+                for (int d = 0; d < 3; d++) v.push_back(vs * (2 * U(gen) - 1));
+                v.push_back(w * (0.5 + U(gen)));
+            }
+            return v;
+        };
+        std::vector<Species> sps;
+        sps.reserve(2);
+        sps.emplace_back("e-", Const::m_e, -Const::q_e, wo, 10.0);
+        sps.emplace_back("O+", 16 * Const::amu, Const::q_e, wo, 20.0);
+        set_parts(sps[0], cloud(30, 1e6, 10.0));
+        set_parts(sps[1], cloud(41, 2e3, 20.0));
+        for (Species& sp : sps) {
+            sp.computeNumberDensity();
+            sp.sampleMoments();
+            sp.updateAverages();
+            sp.computeNumberDensity();
+            sp.updateAverages();
+            sp.computeMacroParticlesCount();
+        }
+        Output::fieldsOutput(wo, sps);
+        Output::particlesOutput(wo, sps, 7);
+        std::vector<double> drows;
+        for (int row = 0; row < 2; row++) {
+            Output::diagOutput(wo, sps);
+            drows.push_back(wo.getTs());
+            drows.push_back(wo.getTime());
+            for (Species& sp : sps) {
+                type_calc3 m = sp.getMomentum();
+                drows.insert(drows.end(), {(double)sp.getNumParticles(), sp.getMicroCount(), m[0], m[1], m[2],
+                                           sp.getKE()});
+            }
+            drows.push_back(wo.getPE());
+            wo.advanceTime();
+        }
+        if (chdir(cwd) != 0) exit(2);
+        dump("out_grid", {(double)OI, (double)OJ, (double)OK, y0[0], y0[1], y0[2], wo.getDx()[0], wo.getDx()[1],
+                          wo.getDx()[2], 2.0});
+        dump("out_node_vol", flat(wo.node_vol));
+        std::vector<double> oid, nty;
+        for (int k = 0; k < OK; k++)
+            for (int j = 0; j < OJ; j++)
+                for (int i = 0; i < OI; i++) {
+                    oid.push_back(wo.object_id[i][j][k]);
+                    nty.push_back(wo.node_type[i][j][k]);
+                }
+        dump("out_object_id", oid);
+        dump("out_node_type", nty);
+        dump("out_phi", flat(wo.phi));
+        dump("out_rho", flat(wo.rho));
+        dump("out_ef", flat3(wo.ef));
+        for (int s = 0; s < 2; s++) {
+            Species& sp = sps[s];
+            std::string t = std::to_string(s);
+            dump("out_den_" + t, flat(sp.den));
+            dump("out_den_avg_" + t, flat(sp.den_avg));
+            dump("out_vel_" + t, flat3(sp.vel));
+            dump("out_T_" + t, flat(sp.T));
+            std::vector<double> mc;
+            for (int k = 0; k < OK - 1; k++)
+                for (int j = 0; j < OJ - 1; j++)
+                    for (int i = 0; i < OI - 1; i++) mc.push_back(sp.macro_part_count[i][j][k]);
+            dump("out_mpc_" + t, mc);
+            dump("out_parts_" + t, parts(sp));
+        }
+        dump("out_diag_rows", drows);
     }
     g_manifest.close();
 }

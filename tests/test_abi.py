@@ -29,7 +29,7 @@ def test_header_constants_match_binding():
     import pic_gpu
 
     text = open(HEADER).read()
-    consts = re.findall(r"^#define PIC_((?:SOLVER|SOLVE|TOL|FIELD|SFIELD|DEPOSIT)_\w+)\s+(\d+)", text, re.M)
+    consts = re.findall(r"^#define PIC_((?:SOLVER|SOLVE|TOL|FIELD|SFIELD|DEPOSIT|SURF)_\w+)\s+(\d+)", text, re.M)
     assert len(consts) >= 20
     for name, value in consts:
         assert getattr(pic_gpu, name) == int(value), name

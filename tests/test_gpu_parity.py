@@ -514,7 +514,7 @@ def test_mcc_matches_oracle_philox(pg, golden, sigma_table_path):
     c.sort(se)
     neu_s, ele_s = c.download(sn), c.download(se)  # device (cell-sorted) order
     prm = O.mcc_params(16 * O.AMU, O.ME, E_ion, sE, sV, dt, g.dx[0] * g.dx[1] * g.dx[2],
-                       pool_append=False)
+                       pool_append=True)
     rng = O.rng_philox(31, 0, 0x4D434301)
     on, oi, oe, wsv, st = O.mcc_apply(g, prm, 1e-14 * 5e11, rng, neu_s, np.zeros(0), ele_s)
     gs = c.mcc_apply(mid, dt, 31, 0)
