@@ -129,6 +129,8 @@ struct Draws {
 constexpr uint32_t TAG_LOAD = 0x4C4F4144u;  // stream tags
 constexpr uint32_t TAG_MCC = 0x4D434301u;
 constexpr uint32_t TAG_SURF = 0x53524601u;
+constexpr uint32_t TAG_DSMC = 0x44534D01u;
+constexpr uint32_t TAG_SRC = 0x53524301u;
 constexpr uint32_t TAG_MERGE = 0x4D524701u;
 
 // one species: SoA particle arrays (two buffer sets for the sort) + grids
@@ -185,6 +187,22 @@ struct Mcc {
     double wsv_host = 0;
 };
 
+// DSMC_MEX between one species or two of equal weight
+struct Dsmc {
+    int s1, s2;
+    double* d_sv = nullptr;                 // [0] sigma v_rel max, [1] the step's maximum
+    unsigned long long* d_part = nullptr;   // striped {collisions, groups, step max bits}
+    unsigned long long* d_stats = nullptr;  // summed
+};
+
+// a cold or warm beam source through one face
+struct Src {
+    int sid, face;
+    double v_drift, den, T, dt;
+    double num_micro;  // particles per call (den v_drift A dt)
+    uint32_t calls = 0;
+};
+
 struct Prof {
     bool on = false;
     struct Rec {
@@ -225,6 +243,8 @@ struct pic_ctx {
     size_t work_bytes = 0;
     std::vector<pic::Species> sp;
     std::vector<pic::Mcc> mcc;
+    std::vector<pic::Dsmc> dsmc;
+    std::vector<pic::Src> src;
     pic::Prof prof;
     void* unet = nullptr;  // learned preconditioner state
     float* dep_corner = nullptr;  // deposits' corner sums per cell (8, or 7 x 8 for the moments; on demand)

@@ -43,6 +43,7 @@ ABI_FUNCTIONS = [
     "pic_profile_reset", "pic_unet_create", "pic_unet_param_count", "pic_unet_get_params",
     "pic_unet_set_params", "pic_unet_forward", "pic_unet_loss_grad", "pic_unet_adam_step",
     "pic_unet_train", "pic_learned_precondition", "pic_species_set_surface", "pic_merge",
+    "pic_dsmc_create", "pic_dsmc_apply", "pic_source_create", "pic_source_sample",
 ]
 
 
@@ -68,6 +69,11 @@ class UNetTrainStats(C.Structure):
 
 class MergeStats(C.Structure):
     _fields_ = [("bins", C.c_longlong), ("merged", C.c_longlong), ("removed", C.c_longlong)]
+
+
+class DsmcStats(C.Structure):
+    _fields_ = [("collisions", C.c_longlong), ("groups", C.c_longlong),
+                ("sigma_v_rel_max", C.c_double)]
 
 
 class GridDesc(C.Structure):
@@ -163,6 +169,12 @@ def load_library():
     L.pic_learned_precondition.argtypes = [P, _dp, _dp]
     L.pic_species_set_surface.argtypes = [P, C.c_int, C.c_int, C.c_int, C.c_uint32]
     L.pic_merge.argtypes = [P, C.c_int, C.c_uint32, C.POINTER(MergeStats)]
+    L.pic_dsmc_create.argtypes = [P, C.c_int, C.c_int, C.POINTER(C.c_int)]
+    L.pic_dsmc_apply.argtypes = [P, C.c_int, C.c_double, C.c_uint32, C.c_uint32,
+                                 C.POINTER(DsmcStats)]
+    L.pic_source_create.argtypes = [P, C.c_int, C.c_int, C.c_double, C.c_double, C.c_double,
+                                    C.POINTER(C.c_int)]
+    L.pic_source_sample.argtypes = [P, C.c_int, C.c_uint32, C.POINTER(C.c_size_t)]
     _lib = L
     return L
 
@@ -353,6 +365,29 @@ class Context:
         self._ck(self.L.pic_mcc_apply(self.h, mid, dt, seed, step, C.byref(st)))
         return {"collisions": st.collisions, "ionizations": st.ionizations,
                 "groups": st.groups, "wsv_max": st.wsv_max, "pool_overflow": st.pool_overflow}
+
+    # DSMC_MEX and sources
+    def dsmc_create(self, s1, s2=None):
+        i = C.c_int()
+        self._ck(self.L.pic_dsmc_create(self.h, s1, s1 if s2 is None else s2, C.byref(i)))
+        return i.value
+
+    def dsmc_apply(self, did, dt, seed, step):
+        st = DsmcStats()
+        self._ck(self.L.pic_dsmc_apply(self.h, did, dt, seed, step, C.byref(st)))
+        return {"collisions": st.collisions, "groups": st.groups,
+                "sigma_v_rel_max": st.sigma_v_rel_max}
+
+    def source_create(self, sid, face, v_drift, den, T=0.0):
+        """face 0 x-, 1 x+, 2 y-, 3 y+, 4 z-, 5 z+; T <= 0: cold beam"""
+        i = C.c_int()
+        self._ck(self.L.pic_source_create(self.h, sid, face, v_drift, den, T, C.byref(i)))
+        return i.value
+
+    def source_sample(self, src, seed):
+        n = C.c_size_t()
+        self._ck(self.L.pic_source_sample(self.h, src, seed, C.byref(n)))
+        return n.value
 
     # learned preconditioner
     def unet_create(self, channels=(16, 32, 64), seed=1, coarsen=1):

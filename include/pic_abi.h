@@ -251,6 +251,43 @@ int pic_mcc_create(pic_ctx* ctx, int sid_neutral, int sid_ion, int sid_electron,
 int pic_mcc_apply(pic_ctx* ctx, int mcc_id, double dt, uint32_t seed, uint32_t step,
                   pic_mcc_stats* stats);
 
+/* ---- DSMC_MEX (Interactions.h:43-66): equal-weight DSMC with Bird's VHS
+ * cross section within one species or between two of equal mpw0 ---- */
+/* DSMC_MEX(species, world) / DSMC_MEX(species1, species2, world)
+ * (Interactions.cpp:143-172): sid2 == sid1 for one species; two species
+ * must have the same mpw0 (the constructor's guard).  sigma v_rel max
+ * starts at 1e-14 (Interactions.h:60). */
+int pic_dsmc_create(pic_ctx* ctx, int sid1, int sid2, int* dsmc_id);
+typedef struct {
+    long long collisions, groups;
+    double sigma_v_rel_max; /* after the call */
+} pic_dsmc_stats;
+/* DSMC_MEX::apply(dt) (Interactions.cpp:177-259): per cell (int)(0.5 N1 N2
+ * mpw0 svmax dt / dV + 0.5) random pairs (distinct within one species),
+ * accepted with P = sigma v_rel / svmax, isotropic scattering in the centre
+ * of mass; svmax becomes the step's maximum when anything collided.  One
+ * thread per cell over the cell-sorted species (sorted first when needed).
+ * Draws: Philox keyed (seed, step), counter (cell, group). */
+int pic_dsmc_apply(pic_ctx* ctx, int dsmc_id, double dt, uint32_t seed, uint32_t step,
+                   pic_dsmc_stats* stats);
+
+/* ---- sources (Source.h / Source.cpp:3-185) ---- */
+/* ColdBeamSource (T <= 0) / WarmBeamSource(species, world, v_drift, den,
+ * [T,] face) (Source.cpp:3-28, :32-37, :105-109): face 0 x-, 1 x+, 2 y-, 3 y+,
+ * 4 z-, 5 z+ (inletName2Index, World.cpp:369-390).  The face's nodes become
+ * Dirichlet at phi = 0 (World::addInlet, World.cpp:206-262); each sample
+ * injects (int)(den v_drift A dt / mpw0 + U) particles, dt the context's at
+ * creation. */
+int pic_source_create(pic_ctx* ctx, int sid, int face, double v_drift, double den, double T,
+                      int* source_id);
+/* Source::sample() (Source.cpp:100-103, :181-185): positions uniform on the
+ * face (a + face one cell inside, :47-48), velocity v_drift along the inward
+ * normal plus a warm beam's sampleV3th(T), addParticle's rejection and
+ * half-step rewind (Species.cpp:420-434); the particles are appended.
+ * Draws: Philox keyed (seed, the source's call count), counter (particle,
+ * 0); the count from counter (~0, ~0). */
+int pic_source_sample(pic_ctx* ctx, int source_id, uint32_t seed, size_t* n_added);
+
 /* ---- learned warm start / preconditioner (no counterpart in the reference) ----
  * A 2..5-level 3-D conv U-Net, BF16 weights on MFMA, FP32 accumulation, on
  * the node grid coarsened by `coarsen` (1: one voxel per node; 2: one voxel

@@ -133,6 +133,14 @@ def lib():
         L.orc_push_nonelectron.restype = C.c_size_t
         L.orc_merge.argtypes = [G, C.c_void_p, _dp, C.POINTER(C.c_size_t), C.c_size_t,
                                 C.c_double, C.c_double, C.c_double, C.POINTER(Rng), C.c_int, _i64]
+        L.orc_dsmc_apply.argtypes = [G, _dp, C.c_size_t, C.c_double, C.c_void_p, C.c_size_t,
+                                     C.c_double, C.c_double, C.c_double, C.POINTER(C.c_double),
+                                     C.POINTER(Rng), C.POINTER(C.c_longlong)]
+        L.orc_dsmc_apply.restype = C.c_longlong
+        L.orc_source_sample.argtypes = [G, R, C.c_int, C.c_void_p, C.c_double, C.c_double,
+                                        C.c_double, C.c_int, C.c_double, C.c_double, C.c_double,
+                                        C.c_double, C.POINTER(Rng), _dp, C.c_size_t]
+        L.orc_source_sample.restype = C.c_size_t
         _lib = L
     return _lib
 
@@ -432,3 +440,31 @@ def merge(g, parts, charge, mass, dt, rng, ef=None, gpu_mode=False, extra=None):
                          C.byref(rng), int(gpu_mode), st)
     assert rc == 0, "oracle merge capacity exceeded"
     return buf[: 7 * cn.value], st
+
+
+def dsmc_apply(g, p1, mass1, mpw0, dt, svmax, rng, p2=None, mass2=0.0):
+    """DSMC_MEX::apply on copies; returns (p1, p2, svmax, collisions, groups)"""
+    a = np.ascontiguousarray(p1, np.float64).copy()
+    b = None if p2 is None else np.ascontiguousarray(p2, np.float64).copy()
+    sv = C.c_double(svmax)
+    ng = C.c_longlong()
+    nc = lib().orc_dsmc_apply(C.byref(g), a, a.size // 7, mass1,
+                              None if b is None else b.ctypes.data_as(C.c_void_p),
+                              0 if b is None else b.size // 7, mass2, mpw0, dt, C.byref(sv),
+                              C.byref(rng), C.byref(ng))
+    return a, b, sv.value, nc, ng.value
+
+
+def source_sample(g, rng, charge, mass, mpw0, face, v_drift, den, T, dt, ef=None, rects=None,
+                  max_out=100000):
+    """one ColdBeamSource (T <= 0) / WarmBeamSource sample; returns particles"""
+    out = np.zeros(7 * max_out)
+    ra, nr = rects_arg(rects)
+    efp = None
+    if ef is not None:
+        ef = np.ascontiguousarray(ef, np.float64)
+        efp = ef.ctypes.data_as(C.c_void_p)
+    n = lib().orc_source_sample(C.byref(g), ra, nr, efp, charge, mass, mpw0, face, v_drift, den, T,
+                                dt, C.byref(rng), out, max_out)
+    assert n != C.c_size_t(-1).value
+    return out[: 7 * n]

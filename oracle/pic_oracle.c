@@ -1708,3 +1708,181 @@ int orc_merge(const orc_grid* g, const double* ef, double* parts, size_t* n, siz
     }
     return 0;
 }
+
+/* ------------------------------------------------------------------------ */
+/* DSMC_MEX: v3/Interactions.cpp:143-284 (equal-weight DSMC, Bird's VHS)     */
+/* ------------------------------------------------------------------------ */
+/* v3/Interactions.cpp:153-156, :175-177 */
+static double vhs_sigma(double m_reduced, double v_rel) {
+    double c0 = 4.07e-10, c1 = 0.77, c2 = 2 * ORC_KB * 273.15 / m_reduced, c3 = tgamma(2.5 - c1);
+    return ORC_PI * c0 * c0 * pow(c2 / (v_rel * v_rel), c1 - 0.5) / c3;
+}
+
+/* v3/Interactions.cpp:264-284 */
+static void dsmc_collide(double m1, double m2, double* v1, double* v2, double U1, double U2) {
+    double sum = m1 + m2, inv = 1.0 / sum;
+    double cm[3], vr[3];
+    for (int d = 0; d < 3; d++) {
+        cm[d] = (v1[d] * m1 + v2[d] * m2) * inv;
+        vr[d] = v1[d] - v2[d];
+    }
+    double vm = sqrt(vr[0] * vr[0] + vr[1] * vr[1] + vr[2] * vr[2]);
+    double cos_ksi = 2 * U1 - 1;
+    double sin_ksi = sqrt(1 - cos_ksi * cos_ksi);
+    double eps = 2 * ORC_PI * U2;
+    vr[0] = vm * cos_ksi;
+    vr[1] = vm * sin_ksi * cos(eps);
+    vr[2] = vm * sin_ksi * sin(eps);
+    double f2 = m2 / sum, f1 = m1 / sum;
+    for (int d = 0; d < 3; d++) {
+        v1[d] = cm[d] + vr[d] * f2;
+        v2[d] = cm[d] - vr[d] * f1;
+    }
+}
+
+/* v3/Interactions.cpp:181-259: applyOneSpecies (p2 == NULL) or
+ * applyTwoSpecies.  Per cell: n_groups = (int)(0.5 N1 N2 mpw0 svmax dt/dV +
+ * 0.5) (N1 = N2 = N for one species, which needs N >= 2); random pairs
+ * (distinct within one species), accept if P = sigma v / svmax > U, then
+ * an isotropic scattering in the centre of mass; svmax <- the step maximum
+ * when anything collided.  mt19937 order: picks, accept, cos, eps.  Philox
+ * slots per (cell, group): 0 accept, 1 cos, 2 eps, 3 first pick, 4.. the
+ * second pick's draws.  Returns collisions; *groups the NTC groups. */
+long long orc_dsmc_apply(const orc_grid* g, double* p1, size_t n1, double mass1, double* p2,
+                         size_t n2, double mass2, double mpw0, double dt, double* svmax,
+                         orc_rng* rng, long long* groups) {
+    int nc = (g->ni - 1) * (g->nj - 1) * (g->nk - 1);
+    int one = p2 == NULL;
+    if (one) {
+        p2 = p1;
+        n2 = n1;
+        mass2 = mass1;
+    }
+    double m_red = mass1 * mass2 / (mass1 + mass2);
+    double dv = g->dx[0] * g->dx[1] * g->dx[2];
+    int32_t* off1 = (int32_t*)malloc((size_t)(nc + 1) * sizeof(int32_t));
+    int32_t* idx1 = (int32_t*)malloc((n1 + 1) * sizeof(int32_t));
+    int32_t* off2 = off1;
+    int32_t* idx2 = idx1;
+    orc_sort_indexes(g, p1, n1, off1, idx1);
+    if (!one) {
+        off2 = (int32_t*)malloc((size_t)(nc + 1) * sizeof(int32_t));
+        idx2 = (int32_t*)malloc((n2 + 1) * sizeof(int32_t));
+        orc_sort_indexes(g, p2, n2, off2, idx2);
+    }
+    double tmp_max = 0;
+    long long ncoll = 0, ngr = 0;
+    for (int c = 0; c < nc; c++) {
+        int N1 = off1[c + 1] - off1[c], N2 = off2[c + 1] - off2[c];
+        if (one ? N1 < 2 : (N1 < 1 || N2 < 1)) continue;
+        double frac = 0.5 * N1 * N2 * mpw0 * (*svmax) * dt / dv;
+        int ng = (int)(frac + 0.5);
+        for (int gi = 0; gi < ng; gi++) {
+            ngr++;
+            rng->a = (uint32_t)c;
+            rng->b = (uint32_t)gi;
+            int slot = 3;
+            int a = (int)(orc_rng_draw(rng, slot++) * N1);
+            int b = (int)(orc_rng_draw(rng, slot++) * N2);
+            while (one && a == b) b = (int)(orc_rng_draw(rng, slot++) * N2);
+            double* va = p1 + 7 * (size_t)idx1[off1[c] + a] + 3;
+            double* vb = p2 + 7 * (size_t)idx2[off2[c] + b] + 3;
+            double vr[3] = {va[0] - vb[0], va[1] - vb[1], va[2] - vb[2]};
+            double v_rel = sqrt(vr[0] * vr[0] + vr[1] * vr[1] + vr[2] * vr[2]);
+            double sv = vhs_sigma(m_red, v_rel) * v_rel;
+            if (sv > tmp_max) tmp_max = sv;
+            double P = sv / (*svmax);
+            if (P > orc_rng_draw(rng, 0)) {
+                ncoll++;
+                double U1 = orc_rng_draw(rng, 1);
+                double U2 = orc_rng_draw(rng, 2);
+                dsmc_collide(mass1, mass2, va, vb, U1, U2);
+            }
+        }
+    }
+    if (ncoll) *svmax = tmp_max;
+    free(off1);
+    free(idx1);
+    if (!one) {
+        free(off2);
+        free(idx2);
+    }
+    if (groups) *groups = ngr;
+    return ncoll;
+}
+
+/* ------------------------------------------------------------------------ */
+/* sources: v3/Source.cpp:3-185 (cold and warm beams through a face)          */
+/* ------------------------------------------------------------------------ */
+/* Source::Source (:3-28): the injected count per call is
+ * (int)(den v_drift A dt / mpw0 + U), A the face area; ColdBeamSource and
+ * WarmBeamSource (:32-185): positions uniform on the face (a + face at
+ * x0 + L - dx, :47-48), velocity v_drift along the inward normal, plus
+ * sampleV3th(T) for a warm beam (T > 0, drawn before the position);
+ * addParticle (Species.cpp:420-434) rejects points outside the box or in an
+ * object and rewinds v by half a step in E.  face: 0 x-, 1 x+, 2 y-, 3 y+,
+ * 4 z-, 5 z+.  Draws, mt19937: the count, then per particle (warm) the
+ * eleven of sampleV3th and the two position draws.  Philox: the count at
+ * counter (0xFFFFFFFF, 0xFFFFFFFF) slot 0, particle i at (i, 0) slots 0-10
+ * (warm) and 11-12 (position).  Returns particles appended, or (size_t)-1
+ * past max_out. */
+size_t orc_source_sample(const orc_grid* g, const orc_rect* rects, int nrect, const double* ef,
+                         double charge, double mass, double mpw0, int face, double v_drift,
+                         double den, double T, double dt, orc_rng* rng, double* parts,
+                         size_t max_out) {
+    double L[3];
+    for (int d = 0; d < 3; d++) L[d] = g->dx[d] * ((d == 0 ? g->ni : d == 1 ? g->nj : g->nk) - 1);
+    int ax = face / 2, plus = face & 1;
+    int t0 = ax == 0 ? 1 : 0, t1 = ax == 2 ? 1 : 2;
+    double A = L[t0] * L[t1];
+    double num_micro = den * v_drift * A * dt;
+    if (plus) L[ax] -= g->dx[ax];
+    rng->a = 0xFFFFFFFFu;
+    rng->b = 0xFFFFFFFFu;
+    int num_macro = (int)(num_micro / mpw0 + orc_rng_draw(rng, 0));
+    size_t out = 0;
+    for (int i = 0; i < num_macro; i++) {
+        rng->a = (uint32_t)i;
+        rng->b = 0;
+        double vel[3] = {0, 0, 0};
+        if (T > 0) { /* sampleV3th, Species.cpp:862-869 */
+            double vth = sqrt(2 * ORC_KB * T / mass), v[3];
+            for (int c = 0; c < 3; c++) {
+                double u1 = orc_rng_draw(rng, 3 * c), u2 = orc_rng_draw(rng, 3 * c + 1);
+                double u3 = orc_rng_draw(rng, 3 * c + 2);
+                v[c] = vth * (u1 + u2 + u3 - 1.5);
+            }
+            double sp = sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
+            double theta = 2 * ORC_PI * orc_rng_draw(rng, 9);
+            double r = -1.0 + 2 * orc_rng_draw(rng, 10);
+            double a = sqrt(1 - r * r);
+            vel[0] = sp * r;
+            vel[1] = sp * (cos(theta) * a);
+            vel[2] = sp * (sin(theta) * a);
+        }
+        if (plus)
+            vel[ax] -= v_drift;
+        else
+            vel[ax] += v_drift;
+        double pos[3];
+        pos[ax] = g->x0[ax] + (plus ? L[ax] : 0.0);
+        pos[t0] = g->x0[t0] + orc_rng_draw(rng, 11) * L[t0];
+        pos[t1] = g->x0[t1] + orc_rng_draw(rng, 12) * L[t1];
+        if (!orc_in_bounds(g, pos) || orc_in_object(rects, nrect, pos)) continue;
+        if (ef) {
+            double lc[3], e[3];
+            xtol(g, pos, lc);
+            orc_gather3(g, ef, lc, e);
+            for (int d = 0; d < 3; d++) vel[d] -= e[d] * (charge / mass) * (0.5 * dt);
+        }
+        if (out >= max_out) return (size_t)-1;
+        double* q = parts + 7 * out;
+        for (int d = 0; d < 3; d++) {
+            q[d] = pos[d];
+            q[3 + d] = vel[d];
+        }
+        q[6] = mpw0;
+        out++;
+    }
+    return out;
+}

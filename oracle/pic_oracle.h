@@ -204,6 +204,17 @@ size_t orc_push_nonelectron(const orc_grid* g, const double* ef, double* parts, 
 int orc_merge(const orc_grid* g, const double* ef, double* parts, size_t* n, size_t cap,
               double charge, double mass, double dt, orc_rng* rng, int gpu_mode, long long* stats);
 
+/* ---- DSMC_MEX (v3/Interactions.cpp:143-284); p2 == NULL: one species ---- */
+long long orc_dsmc_apply(const orc_grid* g, double* p1, size_t n1, double mass1, double* p2,
+                         size_t n2, double mass2, double mpw0, double dt, double* svmax,
+                         orc_rng* rng, long long* groups);
+
+/* ---- ColdBeamSource / WarmBeamSource (v3/Source.cpp:3-185); T <= 0 cold ---- */
+size_t orc_source_sample(const orc_grid* g, const orc_rect* rects, int nrect, const double* ef,
+                         double charge, double mass, double mpw0, int face, double v_drift,
+                         double den, double T, double dt, orc_rng* rng, double* parts,
+                         size_t max_out);
+
 #ifdef __cplusplus
 }
 #endif

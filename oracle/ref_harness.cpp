@@ -28,6 +28,7 @@
 #include "Outputs.h"
 #include "PotentialSolver.h"
 #include "Rnd.h"
+#include "Source.h"
 #include "Species.h"
 #include "World.h"
 
@@ -394,6 +395,63 @@ static void golden(const std::string& dir, const std::string& table_path) {
         me.merge();
         dump("mg_ele_out", parts(me));
     }
+    // ---- DSMC_MEX (Interactions.cpp:143-284): one species and two species
+    // of equal weight on a dense box ----
+    {
+        type_calc3 y0(0.0, 0.0, 0.0), ym(8e-4, 7e-4, 6e-4);
+        World wd(9, 8, 7, y0, ym);
+        const double ddt = 1e-7;
+        wd.setTime(ddt, 10);
+        dump("dm_grid", {9, 8, 7, y0[0], y0[1], y0[2], ym[0], ym[1], ym[2], ddt});
+        Species a("O", 16 * Const::amu, 0, wd, 1e9);
+        Species b("Ar", 40 * Const::amu, 0, wd, 1e9);
+        rnd = Rnd(8080u);
+        type_calc3 c = wd.getXc(), box = wd.getL() * 0.999;
+        const double vol = box[0] * box[1] * box[2];
+        a.loadParticleBoxThermal(c, box, 336.0 * 12 * 1e9 / vol, 3000.0);
+        b.loadParticleBoxThermal(c, box, 336.0 * 6 * 1e9 / vol, 2000.0);
+        dump("dm_a_in", parts(a));
+        dump("dm_b_in", parts(b));
+        DSMC_MEX one(a, wd);
+        DSMC_MEX two(a, b, wd);
+        rnd = Rnd(8081u);
+        for (int call = 0; call < 2; call++) {
+            one.apply(ddt);
+            dump("dm_one_" + std::to_string(call), parts(a));
+        }
+        rnd = Rnd(8082u);
+        for (int call = 0; call < 2; call++) {
+            two.apply(ddt);
+            dump("dm_two_a_" + std::to_string(call), parts(a));
+            dump("dm_two_b_" + std::to_string(call), parts(b));
+        }
+    }
+
+    // ---- sources (Source.cpp:3-185): a cold beam through x- and a warm
+    // beam through z+, with a block in front of part of the z+ face ----
+    {
+        World wsrc(NI, NJ, NK, x0, xm);
+        wsrc.setTime(2e-9, 10);
+        type_calc3 c = wsrc.getXc(), L = wsrc.getL();
+        wsrc.addObject<Rectangle>(type_calc3(c[0], c[1], wsrc.getXm()[2]), 0.0,
+                                  type_calc3(L[0] * 0.4, L[1] * 0.3, L[2] * 0.2));
+        wsrc.computeObjectID();
+        for (int i = 0; i < NI; i++)
+            for (int j = 0; j < NJ; j++)
+                for (int k = 0; k < NK; k++) wsrc.ef[i][j][k] = world.ef[i][j][k];
+        Species cold("e-", Const::m_e, -Const::q_e, wsrc, 1e4);
+        Species warm("O+", 16 * Const::amu, Const::q_e, wsrc, 5e3);
+        ColdBeamSource s1(cold, wsrc, 2e5, 1e12, "x-");
+        WarmBeamSource s2(warm, wsrc, 3e3, 2e12, 5000.0, "z+");
+        rnd = Rnd(1234u);
+        s1.sample();
+        s1.sample();
+        rnd = Rnd(4321u);
+        s2.sample();
+        dump("src_cold", parts(cold));
+        dump("src_warm", parts(warm));
+    }
+
     // ---- Output writers (Outputs.cpp:9-205): the reference writes its own
     // files for a small world into <dir>/results; the inputs they print are
     // dumped beside them ----

@@ -264,3 +264,63 @@ def test_merge_exact(G, golden, case, charge, mass, seed):
     assert st[0] >= 10
     assert out.size < pin.size
     assert np.array_equal(out, golden["mg_%s_out" % case])
+
+
+def test_dsmc_mex_one_and_two_species_exact(golden):
+    """DSMC_MEX::apply draw for draw: equal-weight pairs per cell, Bird's VHS
+    cross section, isotropic scattering (Interactions.cpp:143-284)"""
+    gr = golden["dm_grid"]
+    g = _grid(gr)
+    dt = gr[9]
+    a, b = golden["dm_a_in"], golden["dm_b_in"]
+    rng = O.rng_mt(8081)
+    sv = 1e-14
+    total = 0
+    for call in range(2):
+        a, _, sv, nc, ng = O.dsmc_apply(g, a, 16 * O.AMU, 1e9, dt, sv, rng)
+        total += nc
+        assert np.array_equal(a, golden["dm_one_%d" % call])
+    assert total > 500
+    rng = O.rng_mt(8082)
+    sv = 1e-14
+    for call in range(2):
+        a, b, sv, nc, ng = O.dsmc_apply(g, a, 16 * O.AMU, 1e9, dt, sv, rng, p2=b, mass2=40 * O.AMU)
+        assert nc > 100
+        assert np.array_equal(a, golden["dm_two_a_%d" % call])
+        assert np.array_equal(b, golden["dm_two_b_%d" % call])
+
+
+def test_dsmc_mex_conserves_momentum_and_energy(golden):
+    gr = golden["dm_grid"]
+    g = _grid(gr)
+    a, b = golden["dm_a_in"].reshape(-1, 7), golden["dm_b_in"].reshape(-1, 7)
+    a2, b2, sv, nc, ng = O.dsmc_apply(g, a.reshape(-1), 16 * O.AMU, 1e9, gr[9], 1e-14,
+                                      O.rng_philox(5, 0, 1), p2=b.reshape(-1), mass2=40 * O.AMU)
+    a2, b2 = a2.reshape(-1, 7), b2.reshape(-1, 7)
+    ma, mb = 16 * O.AMU, 40 * O.AMU
+    p0 = ma * a[:, 3:6].sum(0) + mb * b[:, 3:6].sum(0)
+    p1 = ma * a2[:, 3:6].sum(0) + mb * b2[:, 3:6].sum(0)
+    e0 = ma * (a[:, 3:6] ** 2).sum() + mb * (b[:, 3:6] ** 2).sum()
+    e1 = ma * (a2[:, 3:6] ** 2).sum() + mb * (b2[:, 3:6] ** 2).sum()
+    assert nc > 100
+    assert np.allclose(p0, p1, rtol=0, atol=1e-12 * np.sqrt(e0 * ma))
+    assert e1 == pytest.approx(e0, rel=1e-12)
+
+
+def test_sources_exact(G, golden):
+    """ColdBeamSource (two samples, x- face) and WarmBeamSource (z+ face, a
+    block in front of part of it) draw for draw (Source.cpp:3-185)"""
+    L = _L(G)
+    c = _xc(G)
+    rect = ([c[0], c[1], G.xm[2]], [L[0] * 0.2, L[1] * 0.15, L[2] * 0.1], 0.0)
+    ef = golden["ef_analytic"]
+    rng = O.rng_mt(1234)
+    cold = np.concatenate([O.source_sample(G, rng, -O.QE, O.ME, 1e4, 0, 2e5, 1e12, 0.0, 2e-9,
+                                           ef=ef, rects=[rect]) for _ in range(2)])
+    assert cold.size > 0
+    assert np.array_equal(cold, golden["src_cold"])
+    rng = O.rng_mt(4321)
+    warm = O.source_sample(G, rng, O.QE, 16 * O.AMU, 5e3, 5, 3e3, 2e12, 5000.0, 2e-9, ef=ef,
+                           rects=[rect])
+    assert warm.size > 0
+    assert np.array_equal(warm, golden["src_warm"])
