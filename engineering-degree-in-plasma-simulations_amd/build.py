@@ -67,6 +67,16 @@ def build(verbose=True):
     if r.returncode != 0:
         raise RuntimeError("driver build failed:\n" + r.stderr[-4000:])
     os.replace(driver + ".tmp", driver)
+    # the host classes' test executable (run by the GPU tests)
+    ht = os.path.join(HERE, "pic_host_test")
+    cmd = ["g++", "-O2", "-std=c++17", "-Wall", "-I", os.path.join(ROOT, "include"),
+           os.path.join(HERE, "host", "pic_host.cpp"), os.path.join(HERE, "host", "pic_host_test.cpp"),
+           "-o", ht + ".tmp", "-L", HERE, "-lpic_mi355", "-lpic_io", "-Wl,-rpath,$ORIGIN",
+           "-Wl,-rpath,/opt/rocm/lib"]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError("host test build failed:\n" + r.stderr[-4000:])
+    os.replace(ht + ".tmp", ht)
     if verbose:
         print("built", LIB, "and", driver)
     return LIB

@@ -293,9 +293,13 @@ bool PotentialSolver::solve() {
     int rc = pic_poisson(world.ctx, &p, &last);
     if (rc < 0) world.check(rc, "solve");
     world.device_fields_changed();
-    if (rc == PIC_NOT_CONVERGED && solver_type == PCG) {  // fall back to GS (:222-224)
+    if (rc == PIC_NOT_CONVERGED && solver_type == PCG) {
+        // fall back to Gauss-Seidel (:222-224): the reference runs up to
+        // GS_max_solver_it sweeps in each of its up to 20 Newton steps
+        // (:179, :315-347); here one red-black SOR solve of up to that many
+        // sweeps in all, from the phi PCG left, to the same criterion
         p.method = PIC_SOLVER_SOR_RB;
-        p.max_it = (int)GS_max_solver_it;
+        p.max_it = (int)(20 * GS_max_solver_it);
         p.check_every = 25;
         rc = pic_poisson(world.ctx, &p, &last);
         if (rc < 0) world.check(rc, "solve");
