@@ -85,6 +85,20 @@ def unet_flops(channels, coarsen, n):
     return f
 
 
+def host_info():
+    """nproc and the CPU model of this host (recorded with every CPU baseline)"""
+    model = None
+    try:
+        with open("/proc/cpuinfo") as f:
+            for ln in f:
+                if ln.startswith("model name"):
+                    model = ln.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return {"nproc": os.cpu_count(), "cpu_model": model}
+
+
 def cpu_baseline(threads, particles, steps):
     """The reference's threaded electron push + serial deposit on this host."""
     harness = os.path.join(ROOT, "oracle", "_ref", "ref_harness")
@@ -98,7 +112,9 @@ def cpu_baseline(threads, particles, steps):
         r = json.loads(line)
         per_particle = r["push_s_per_particle"] + r["deposit_s_per_particle"]
         return {"value": 1.0 / per_particle, "unit": "particle pushes/s", "cores": threads,
-                "kind": "reference", "sample": sample,
+                "kind": "reference", "sample": sample, "host": host_info(),
+                "note": "the reference's ThreadPool push removes no particle (its deletion "
+                        "buffers are std::bind copies, ThreadPool.h:63); the box keeps them inside",
                 "push_only_pushes_per_s": r["pushes_per_s"], "wall_s": round(time.time() - t0, 1),
                 # serial reference loops on the same electrons (the secondary rows)
                 "load_particles_per_s": 1.0 / r["load_s_per_particle"],
@@ -122,7 +138,7 @@ def cpu_baseline(threads, particles, steps):
         O.push_threaded(g, ef, p, -QE, ME, 1e-10, threads)
     dt = time.time() - t0
     return {"value": particles * steps / dt, "unit": "particle pushes/s", "cores": threads,
-            "kind": "port", "sample": sample.replace("push + deposit", "push")}
+            "kind": "port", "sample": sample.replace("push + deposit", "push"), "host": host_info()}
 
 
 def cpu_baseline_dsmc(nxy, nz, n_neu, n_ele, steps, dt):
@@ -156,7 +172,9 @@ def dsmc_bench(pg, args, cpu):
     w_e = w_n / 1000.0
     dt = args.dsmc_dt
     ctx = pg.Context(nxy, nxy, nz, [0, 0, 0], L)
-    sn = ctx.species_add(32 * AMU, 0.0, w_n, 2 * Nn)  # elastic splits append neutrals
+    # elastic splits append ~5% of the neutrals per step: room for the
+    # warm-up, timed and profiled steps
+    sn = ctx.species_add(32 * AMU, 0.0, w_n, int(Nn * (1.0 + 0.06 * (args.dsmc_warmup + 2 * args.dsmc_steps + 2))))
     si = ctx.species_add(32 * AMU, QE, w_e, Ni + Ne // 2 + 1024)
     se = ctx.species_add(ME, -QE, w_e, Ne + Ne // 2 + 1024)
     c = 0.5 * L
@@ -182,12 +200,18 @@ def dsmc_bench(pg, args, cpu):
     ctx.sync()
     T = time.perf_counter() - t0
     n1 = ctx.count(sn) + ctx.count(se)
-    # one more step with HIP events around every launch
+    # as many more steps with HIP events around every launch (the same
+    # mix of tail sorts and whole sorts as the timed steps), averaged
     ctx.profile(True)
     ctx.profile_reset()
-    st = ctx.mcc_apply(mid, dt, 7, step)
+    st = {"collisions": 0, "ionizations": 0, "groups": 0}
+    for _ in range(K):
+        r = ctx.mcc_apply(mid, dt, 7, step)
+        step += 1
+        for k in st:
+            st[k] += r[k] / K
     ctx.sync()
-    prof = ctx.profile_read()
+    prof = {k: (v[0] / K, v[1]) for k, v in ctx.profile_read().items()}
     ctx.profile(False)
     ncells = (nxy - 1) * (nxy - 1) * (nz - 1)
     mcc_ms = prof["mcc"][0]
@@ -216,7 +240,8 @@ def dsmc_bench(pg, args, cpu):
                    "cells": ncells},
         "particles_start": n0, "particles_end": n1,
         "per_step": {k: v / K for k, v in tot.items()},
-        "stages": {k: {"ms": round(v[0], 4), "launches": v[1]} for k, v in prof.items() if v[1]},
+        "stages_ms_per_step": {k: round(v[0], 4) for k, v in prof.items() if v[1]},
+        "stages_profiled_steps": K,
         "roofline": roof_mcc,
         "sorts_ms_per_step": sc_ms + so_ms,
         "dominant": "k_mcc" if mcc_ms >= sc_ms + so_ms else "sorts",
@@ -224,6 +249,203 @@ def dsmc_bench(pg, args, cpu):
     }
     ctx.close()
     return out
+
+
+def discharge_bench(pg, args):
+    """The DSMC ionization discharge as a loop (the reference's main.cpp:
+    177-288 over the build's classes' calls): 64 x 64 x 256 nodes, dx =
+    1e-4 m, Rectangle plates of 0.1 L at both z ends at +300 V and 0 V, O2 at
+    10 Pa / 300 K between them as variable-weight neutrals, electrons at
+    10 eV and O2+ ions (50M particles in all).  The field is solved once
+    (v3 never re-solves in its loop, main.cpp:172-173, :260-261).  A step:
+    MC_MEX_Ionization::apply, the electrons pushed every step, the ions every
+    10th with 10 dt (recycled into neutrals at the plates), the neutrals
+    every 100th with 50 dt (diffuse reflection at the plates), the pushed
+    species' densities and the charge density; every 50th step a merge of
+    every species keeps the particle count bounded (the reference merges
+    every 50th step past the 250th, main.cpp:179)."""
+    nxy, nz, dx = 64, 256, 1e-4
+    L = np.array([dx * (nxy - 1), dx * (nxy - 1), dx * (nz - 1)])
+    dt = args.discharge_dt
+    ctx = pg.Context(nxy, nxy, nz, [0, 0, 0], L)
+    ctx.set_dt(dt)
+    ctx.add_rectangle([L[0] / 2, L[1] / 2, 0.0], [L[0], L[1], 0.1 * L[2]], 300.0)
+    ctx.add_rectangle([L[0] / 2, L[1] / 2, L[2]], [L[0], L[1], 0.1 * L[2]], 0.0)
+    gas = np.array([L[0], L[1], 0.9 * L[2]]) * 0.999
+    c = 0.5 * L
+    vol = float(np.prod(gas))
+    n_gas = 10.0 / (KB * 300.0)
+    Nn, Ne, Ni = args.dsmc_neutrals, args.dsmc_electrons, args.dsmc_ions
+    w_n = n_gas * vol / Nn
+    w_e = w_n / 1000.0
+    sn = ctx.species_add(32 * AMU, 0.0, w_n, int(1.6 * Nn))
+    si = ctx.species_add(32 * AMU, QE, w_e, 2 * Ni)
+    se = ctx.species_add(ME, -QE, w_e, 2 * Ne)
+    ctx.load_box_thermal(sn, c, gas, n_gas, 300.0, seed=31, mode=0)
+    ctx.load_box_thermal(se, c, gas, Ne * w_e / vol, 10.0 * EV_K, seed=32, mode=0)
+    ctx.load_box_thermal(si, c, gas, Ni * w_e / vol, 300.0, seed=33, mode=0)
+    ctx.set_surface(sn, pg.SURF_REFLECT, seed=34)
+    ctx.set_surface(si, pg.SURF_RECYCLE, sn, seed=35)
+    t = np.loadtxt(SIGMA_TABLE)
+    # W sigma v_rel max from the start at its bound over the table (the
+    # reference's 1e-14 max(mpw0), Interactions.cpp:534, is 1/100 of it
+    # here: no cell would reach half an NTC group and the maximum would never
+    # be raised)
+    m_red = 32 * AMU * ME / (32 * AMU + ME)
+    wsv0 = w_n * float(np.max(t[:, 1] * np.sqrt(2.0 * t[:, 0] * QE / m_red)))
+    mid = ctx.mcc_create(sn, si, se, 12.07 * QE, t[:, 0], t[:, 1], wsv_max_init=wsv0)
+    for s_ in (sn, si, se):
+        ctx.deposit(s_)
+    ctx.charge_density()
+    st = ctx.poisson(method=pg.SOLVER_SOR_RB, tol=1e-6, tol_kind=pg.TOL_REL, max_it=50000,
+                     omega=2.0 / (1.0 + np.sin(np.pi / (nz - 1))), check_every=25)
+    ctx.efield()
+    counts0 = {"neutrals": ctx.count(sn), "ions": ctx.count(si), "electrons": ctx.count(se)}
+    tot = {"collisions": 0, "ionizations": 0, "groups": 0, "merged_bins": 0, "merge_removed": 0}
+
+    def step(ts):
+        m = ctx.mcc_apply(mid, dt, 9, ts)
+        for k in ("collisions", "ionizations", "groups"):
+            tot[k] += m[k]
+        ctx.push_sort(se, dt)
+        ctx.deposit(se)
+        if ts % 10 == 0:
+            ctx.push_sort(si, 10 * dt)
+            ctx.deposit(si)
+        if ts % 100 == 0:
+            ctx.push_sort(sn, 50 * dt)  # main.cpp:229-234
+            ctx.deposit(sn)
+        ctx.charge_density()
+        if ts % 50 == 0 and ts > 0:
+            for s_ in (sn, si, se):
+                r = ctx.merge(s_, seed=40 + ts)
+                tot["merged_bins"] += r["bins"]
+                tot["merge_removed"] += r["removed"]
+
+    K = args.discharge_steps
+    ctx.sync()
+    t0 = time.perf_counter()
+    for ts in range(1, K + 1):
+        step(ts)
+    ctx.sync()
+    T = time.perf_counter() - t0
+    counts1 = {"neutrals": ctx.count(sn), "ions": ctx.count(si), "electrons": ctx.count(se)}
+    # per-stage device times over the next 100 steps (HIP events around
+    # every launch; a separate pass so they stay out of the timed loop)
+    PK = 100
+    ctx.profile(True)
+    ctx.profile_reset()
+    for ts in range(K + 1, K + PK + 1):
+        step(ts)
+    ctx.sync()
+    prof = ctx.profile_read()
+    ctx.profile(False)
+    ctx.close()
+    return {"workload": "DSMC ionization discharge: %dx%dx%d nodes, +300 V / 0 V plates, O2 10 Pa / "
+                        "300 K, %d + %d + %d particles, dt %.1e s, subcycled ions (10 dt) and neutrals "
+                        "(50 dt), solve once, merge every 50 steps" % (nz, nxy, nxy, Nn, Ne, Ni, dt),
+            "steps": K, "ms_per_step": T / K * 1e3, "initial_solve": {"iterations": st["iterations"],
+                                                                      "rel_residual": st["rel_residual"]},
+            "particles_start": counts0, "particles_end": counts1,
+            "per_step": {k: v / K for k, v in tot.items()},
+            "stages_ms_per_step": {k: round(v[0] / PK, 4) for k, v in prof.items() if v[1]},
+            "stages_profiled_steps": PK}
+
+
+def c5_bench(pg, args):
+    """The full loop at scale with Boltzmann electrons: 256^3 nodes, dx =
+    1e-2 m, Dirichlet walls, 200M kinetic O+ (0.1 eV), Boltzmann electrons
+    n0 = 1e12 m^-3, Te0 = 2 eV in a Newton-Boltzmann field solve to a 1e-6
+    relative FP64 residual every step (learned inner solver with the shipped
+    network, which is fully convolutional and runs on the 128^3 network grid
+    of this 256^3 node grid; FP32-inner and FP64 Newton-PCG beside it).  A
+    step: push + sort of the ions, their density, the charge density, the
+    Newton solve warm-started from the extrapolated potential, the E-field.
+    The final FP64 residual of the nonlinear problem is recomputed here."""
+    n = args.c5_nodes
+    L = 1e-2 * (n - 1)
+    N = args.c5_particles
+    n0, Te0, dt = 1e12, 2.0, 1e-10
+    ctx = pg.Context(n, n, n, [0, 0, 0], [L, L, L])
+    ctx.dirichlet_walls()
+    ctx.set_dt(dt)
+    w = 1.5 * n0 * L ** 3 / N  # ions at 1.5 n0: a positive space charge against n0 electrons
+    si = ctx.species_add(16 * AMU, QE, w, int(N * 1.02) + 1024)
+    t0 = time.perf_counter()
+    ctx.load_box_thermal(si, [L / 2] * 3, [L] * 3, 1.5 * n0, 0.1 * EV_K, seed=51, mode=1)
+    ctx.sort(si)
+    t_load = time.perf_counter() - t0
+    weights = os.path.join(PKG, "data", "unet_128.f32")
+    have_net = os.path.exists(weights) and os.path.exists(weights + ".json")
+    if have_net:
+        ctx.unet_load(weights)
+    methods = [("learned", pg.SOLVER_LEARNED)] if have_net else []
+    methods += [("pcg_mixed", pg.SOLVER_PCG_MIXED), ("pcg_fp64", pg.SOLVER_PCG)]
+    out = {"workload": "%d^3 nodes, %d O+ kinetic, Boltzmann electrons n0 = 1e12, Te0 = 2 eV, Newton "
+                       "to 1e-6 relative FP64 residual every step" % (n, N),
+           "load_sort_s": t_load, "solvers": {}}
+    # every solver starts from the same converged potential of the loaded
+    # state (a cold FP64 solve, untimed), with the extrapolation history reset
+    ctx.deposit(si)
+    ctx.charge_density()
+    ctx.poisson(method=pg.SOLVER_PCG, tol=1e-6, tol_kind=pg.TOL_REL, max_it=20000, n0=n0, Te0=Te0)
+    phi_start = ctx.get_field(pg.FIELD_PHI)
+    for name, method in methods:
+        kw = dict(method=method, tol=1e-6, tol_kind=pg.TOL_REL, max_it=20000, n0=n0, Te0=Te0,
+                  flags=pg.SOLVE_EXTRAPOLATE)
+        its = []
+
+        def step():
+            ctx.push_sort(si, dt)
+            ctx.deposit(si)
+            ctx.charge_density()
+            st = ctx.poisson(**kw)
+            if not st["converged"]:
+                raise RuntimeError("C5 Newton solve did not converge: %r" % st)
+            its.append((st["newton_iterations"], st["iterations"], st["seconds"]))
+            ctx.efield()
+
+        ctx.set_field(pg.FIELD_PHI, phi_start)
+        step()  # warm-up
+        its.clear()
+        ctx.profile(True)
+        ctx.profile_reset()
+        ctx.sync()
+        t0 = time.perf_counter()
+        for _ in range(args.c5_steps):
+            step()
+        ctx.sync()
+        T = time.perf_counter() - t0
+        prof = ctx.profile_read()
+        ctx.profile(False)
+        phi = ctx.get_field(pg.FIELD_PHI)
+        rho = ctx.get_field(pg.FIELD_RHO)
+        out["solvers"][name] = {
+            "ms_per_step": T / args.c5_steps * 1e3, "pushes_per_s": N * args.c5_steps / T,
+            "newton_iterations_per_step": float(np.mean([a for a, _, _ in its])),
+            "inner_iterations_per_step": float(np.mean([b for _, b, _ in its])),
+            "solve_ms_per_step": float(np.mean([c for _, _, c in its])) * 1e3,
+            "final_fp64_residual": nonlinear_residual(n, L, phi, rho, 0.0, n0, Te0),
+            "stages_ms_per_step": {k: round(v[0] / args.c5_steps, 4) for k, v in prof.items() if v[1]},
+            "note": "profiled with HIP events around every launch"}
+        print("c5 %s: %.1f ms per step" % (name, out["solvers"][name]["ms_per_step"]), file=sys.stderr)
+    ctx.close()
+    return out
+
+
+def nonlinear_residual(n, L, phi, rho, phi0, n0, Te0):
+    """||F|| / ||b|| over the interior nodes, F = A phi + rho/eps0 - q n0
+    exp((phi - phi0)/Te0)/eps0 (FP64, here on the host)"""
+    eps0 = 8.85418782e-12
+    f = phi.reshape(n, n, n)
+    r = rho.reshape(n, n, n)
+    ih2 = 1.0 / (L / (n - 1)) ** 2
+    c = f[1:-1, 1:-1, 1:-1]
+    A = (f[2:, 1:-1, 1:-1] + f[:-2, 1:-1, 1:-1] + f[1:-1, 2:, 1:-1] + f[1:-1, :-2, 1:-1] +
+         f[1:-1, 1:-1, 2:] + f[1:-1, 1:-1, :-2] - 6 * c) * ih2
+    b = -r[1:-1, 1:-1, 1:-1] / eps0
+    F = A - b - QE * n0 * np.exp((c - phi0) / Te0) / eps0
+    return float(np.sqrt(np.sum(F ** 2) / np.sum(b ** 2)))
 
 
 def cpu_baseline_newton(n, threads):
@@ -403,6 +625,15 @@ def main():
     ap.add_argument("--dsmc-steps", type=int, default=10)
     ap.add_argument("--dsmc-warmup", type=int, default=3)
     ap.add_argument("--dsmc-dt", type=float, default=2e-9)  # a few NTC groups per cell
+    # the discharge as a loop (plates, subcycling, surfaces, merge)
+    ap.add_argument("--no-discharge", action="store_true")
+    ap.add_argument("--discharge-steps", type=int, default=200)
+    ap.add_argument("--discharge-dt", type=float, default=5e-10)  # about one NTC group per cell
+    # the Boltzmann-electron loop at scale
+    ap.add_argument("--no-c5", action="store_true")
+    ap.add_argument("--c5-nodes", type=int, default=256)
+    ap.add_argument("--c5-particles", type=int, default=200_000_000)
+    ap.add_argument("--c5-steps", type=int, default=3)
     args = ap.parse_args()
     if args.copy_calibration:
         copy_calibration()
@@ -416,7 +647,9 @@ def main():
     cpu_dsmc = None
     cpu_newton = None
     if not args.no_cpu_baseline:  # before the GPU is initialised
-        threads = max(1, min(16, os.cpu_count() or 1))
+        # the reference's default is all cores but one (Config.cpp:68-75), of
+        # the 16 host cores a one-GPU job may use here
+        threads = max(1, min(16, os.cpu_count() or 1) - 1)
         if not args.only_dsmc:
             cpu = cpu_baseline(threads, args.cpu_particles, args.cpu_steps)
             print("cpu baseline: %.3e pushes/s (%s, %d threads)" % (cpu["value"], cpu["kind"],
@@ -610,6 +843,24 @@ def main():
                     "frac": ach / MFMA_BF16_PEAK_TFLOPS, "traffic": None,
                     "alg_flops_per_forward": fl, "avg_forward_ms": fwd_s * 1e3}
     traffic = measured_traffic(dom)
+    # every bandwidth stage of the step against the HBM peak
+    rooflines = {}
+    for k in ("push", "scatter", "deposit"):
+        if prof.get(k, (0, 0))[1]:
+            launches_k = prof[k][1]
+            avg_k = prof[k][0] * 1e-3 / launches_k
+            if k == "push":  # the histogram pass (one launch per species)
+                b = ALG_BYTES["push"] * N
+            elif k == "scatter":
+                b = ALG_BYTES["scatter"] * N
+            else:  # one deposit call (k_dep_cells + k_dep_nodes): 16 B per particle + 8 B per node
+                b = ALG_BYTES["deposit"] * N + 8.0 * ctx.nn
+            rooflines[k] = {"kernel": KERNEL_NAMES[k], "achieved": b / avg_k / 1e9, "peak": HBM_PEAK_GBS,
+                            "unit": "GB/s", "frac": b / avg_k / 1e9 / HBM_PEAK_GBS, "alg_bytes_per_launch": b,
+                            "avg_launch_ms": avg_k * 1e3, "traffic": measured_traffic(k)}
+    deposits_per_s = None
+    if "deposit" in rooflines:
+        deposits_per_s = N / (rooflines["deposit"]["avg_launch_ms"] * 1e-3)
 
     line = {
         "metric": "particle pushes/s + time-to-1e-6 Poisson residual, learned vs plain, 1 GPU",
@@ -640,6 +891,8 @@ def main():
                      "traffic": traffic, "alg_bytes_per_launch": alg,
                      "avg_launch_ms": avg_s * 1e3},
         "roofline_network": roof_net,
+        "rooflines": rooflines,
+        "deposits_per_s": deposits_per_s,
         "poisson": poisson,
         "cpu_baseline": cpu,
     }
@@ -652,6 +905,11 @@ def main():
         line["dsmc"] = dsmc_bench(pg, args, cpu_dsmc)
         print("dsmc: %.3e particles/s, %.3f ms per step" % (line["dsmc"]["value"], line["dsmc"]["ms_per_step"]),
               file=sys.stderr)
+    if not args.no_discharge:
+        line["discharge_loop"] = discharge_bench(pg, args)
+        print("discharge loop: %.3f ms per step" % line["discharge_loop"]["ms_per_step"], file=sys.stderr)
+    if not args.no_c5:
+        line["c5"] = c5_bench(pg, args)
     print(json.dumps(line))
     return 0
 

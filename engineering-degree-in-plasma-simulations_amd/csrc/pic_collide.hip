@@ -26,7 +26,7 @@ constexpr int BLOCK = 128;
 struct DsmcArgs {
     double m1, m2, sum_m, m_red;
     double c0, c1, c2, c3;  // Bird's VHS parameters (Interactions.cpp:153-156)
-    double groups_pre;      // 0.5 mpw0 dt / dV
+    double mpw0, dt, dv;    // dv = dx dy dz (World::getCellVolume, World.cpp:101-103)
     uint32_t seed, step;
     int one;                // one species (distinct picks)
 };
@@ -49,7 +49,9 @@ __global__ void __launch_bounds__(BLOCK)
         const int N1 = (int)(off1[c + 1] - p1), N2 = (int)(off2[c + 1] - p2);
         const bool act = a.one ? N1 >= 2 : (N1 >= 1 && N2 >= 1);
         const double svmax = sv[0];
-        const int ng = act ? (int)(a.groups_pre * (double)N1 * (double)N2 * svmax + 0.5) : 0;
+        // Interactions.cpp:195, :241 in the same operation order (an exact
+        // tie at .5 then rounds the same way)
+        const int ng = act ? (int)(0.5 * (double)N1 * (double)N2 * a.mpw0 * svmax * a.dt / a.dv + 0.5) : 0;
         for (int gi = 0; gi < ng; gi++) {
             Draws dr;
             dr.init((uint32_t)c, (uint32_t)gi, a.seed, a.step, TAG_DSMC);
@@ -265,7 +267,9 @@ extern "C" int pic_dsmc_apply(pic_ctx* ctx, int id, double dt, uint32_t seed, ui
     a.c1 = 0.77;
     a.c2 = 2 * KB * 273.15 / a.m_red;
     a.c3 = std::tgamma(2.5 - a.c1);
-    a.groups_pre = 0.5 * A.mpw0 * dt / (g.dx[0] * g.dx[1] * g.dx[2]);
+    a.mpw0 = A.mpw0;
+    a.dt = dt;
+    a.dv = g.dx[0] * g.dx[1] * g.dx[2];
     a.seed = seed;
     a.step = step;
     a.one = D.s1 == D.s2;

@@ -167,6 +167,7 @@ extern "C" void pic_destroy(pic_ctx* ctx) {
         (void)hipFree(m.d_wsv);
         (void)hipFree(m.d_stats);
         (void)hipFree(m.d_part);
+        (void)hipFree(m.d_spill);
     }
     pic_unet_free(ctx);
     (void)hipFree(ctx->phi);

@@ -184,6 +184,7 @@ struct Mcc {
     // per-workgroup partials of d_stats and of the step maximum (as bits),
     // striped so that the workgroups' atomics spread over NSTRIPE lines
     unsigned long long* d_part = nullptr;
+    uint32_t* d_spill = nullptr;  // pick-pool entries past the LDS list
     double wsv_host = 0;
 };
 

@@ -30,7 +30,7 @@ struct MccArgs {
     double E_pre_ele;  // m_e 0.5 / q            (Interactions.cpp:123)
     double two_q_me;   // 2 q / m_e              (Interactions.cpp:124)
     double E_ion_eV, c0, c1, c2, B_inc;
-    double dt_inv_dv;
+    double dt, inv_dv;  // inv_dv = 1 / (dx dy dz) (Interactions.cpp:493-494)
     uint32_t seed, step;
     int nsig;
 };
@@ -214,16 +214,21 @@ __device__ inline void flush_births(BirthStage& st, Sp& N, Sp& I, Sp& E, unsigne
 // table's binary search otherwise costs a chain of dependent global loads
 // per NTC group)
 constexpr int SIG_LDS_MAX = 256;
-// split neutrals of one cell and call that join its pick pool; more, or one
-// whose staging slot was full, stay out of it (counted, reported by
-// pic_mcc_apply)
-constexpr int POOL_OWN = 32;
+// A cell's split neutrals of one call that join its pick pool: the first
+// POOL_OWN are listed in LDS; later ones in chunks of a global arena
+// (SPILL_CHUNK words: the previous chunk's base, then SPILL_CHUNK - 1
+// entries), taken with one atomic per chunk.  A split neutral that finds the
+// arena full stays out of the pool (counted, reported by pic_mcc_apply).
+constexpr int POOL_OWN = 16;
+constexpr int SPILL_CHUNK = 32;
+constexpr uint32_t SPILL_CAP = 1u << 20;  // words (4 MiB) per MC_MEX object
+constexpr uint32_t CHUNK_NONE = 0xFFFFFFFFu;
 
 template <bool LDS_TABLE>
 __global__ void __launch_bounds__(BLOCK)
     k_mcc(GridC g, MccArgs a, Sp N, const uint32_t* __restrict__ offN, const uint32_t* __restrict__ toffN,
           uint32_t tbase, Sp I, Sp E, const uint32_t* __restrict__ offE, const double* __restrict__ table,
-          const double* __restrict__ wsv, unsigned long long* __restrict__ part) {
+          const double* __restrict__ wsv, unsigned long long* __restrict__ part, uint32_t* __restrict__ spill) {
     // this workgroup's stripe: collisions, ionizations, groups, overflow, step maximum
     unsigned long long* stats = part + (blockIdx.x % NSTRIPE) * SSTRIDE;
     __shared__ double stab[LDS_TABLE ? 2 * SIG_LDS_MAX : 1];
@@ -234,12 +239,15 @@ __global__ void __launch_bounds__(BLOCK)
         table = stab;
     }
     __syncthreads();
-    // this thread's split neutrals in its cell's pool: their staging slots
-    // (16 bits each, so the workgroup's LDS still admits four per CU)
-    __shared__ uint16_t own[POOL_OWN][BLOCK];
+    // this thread's split neutrals in its cell's pool: where they are (a
+    // staging slot, or a global index once the staging is full)
+    __shared__ uint32_t own[POOL_OWN][BLOCK];
     int c = blockIdx.x * BLOCK + threadIdx.x;
     double wsv_tmp = 0;
     unsigned long long ncoll = 0, niz = 0, ngr = 0, ndead = 0, npool_full = 0;
+    // the arena's fill (stripe 0's spare word, cleared with the partials)
+    unsigned long long* spill_fill = part + 7;
+    uint32_t chunk = CHUNK_NONE;  // this thread's last arena chunk
     if (c < g.ncells) {
         uint32_t e0 = offE[c], e1 = offE[c + 1];
         uint32_t n0 = offN[c], n1 = offN[c + 1];
@@ -260,11 +268,20 @@ __global__ void __launch_bounds__(BLOCK)
         auto loc_of = [&](int slot) -> uint32_t {
             if (slot < Nb) return n0 + (uint32_t)slot;
             if (slot < Nb + Nt) return t0 + (uint32_t)(slot - Nb);
-            return REF_LDS | (uint32_t)own[slot - Nb - Nt][threadIdx.x];
+            const int k = slot - Nb - Nt;
+            if (k < POOL_OWN) return own[k][threadIdx.x];
+            // walk back from the last chunk to the one holding entry k
+            const int j = k - POOL_OWN;
+            const int last = (nown - POOL_OWN - 1) / (SPILL_CHUNK - 1);
+            uint32_t ch = chunk;
+            for (int h = last; h > j / (SPILL_CHUNK - 1); h--) ch = spill[ch];
+            return spill[ch + 1 + j % (SPILL_CHUNK - 1)];
         };
         if (Ne > 0 && Nn0 > 0) {
             const double wmax = wsv[0];
-            double frac = (double)(Nn0 * Ne) * wmax * a.dt_inv_dv;  // Interactions.cpp:646
+            // Interactions.cpp:646, the same operation order (an exact tie
+            // at .5 then rounds the same way)
+            double frac = (double)(Nn0 * Ne) * wmax * a.dt * a.inv_dv;
             int n_groups = (int)(frac + 0.5);
             if (n_groups > Nn0) n_groups = Nn0 - 1;
             const double* tE = table;
@@ -326,35 +343,60 @@ __global__ void __launch_bounds__(BLOCK)
                     nfz = N.b.f[2][pn];
                 }
                 double vnew[3] = {0, 0, 0};
-                if (wn > we) {  // Interactions.cpp:684-703
-                    if (staged)
-                        nb->w = (float)(wn - we);
-                    else
-                        N.b.w[pn] = (float)(wn - we);
-                    bool iz = collide(a, dr, vn, ve, vnew, sigma);
+                // equal weights: no state change (only the electron's
+                // velocity is written back)
+                if (wn != we) {
+                    const bool n_heavier = wn > we;
+                    if (n_heavier) {  // Interactions.cpp:684-703
+                        if (staged)
+                            nb->w = (float)(wn - we);
+                        else
+                            N.b.w[pn] = (float)(wn - we);
+                    } else {  // Interactions.cpp:705-726, defined semantics
+                        // the electron keeps the neutral's weight; the rest
+                        // leaves as an electron of the pre-collision velocity
+                        E.b.w[pe] = (float)wn;
+                        append(births, 2, E, E.b.cell[pe], E.b.f[0][pe], E.b.f[1][pe], E.b.f[2][pe], ve, we - wn,
+                               &stats[3]);
+                    }
+                    // one call site for both branches (register pressure)
+                    const bool iz = collide(a, dr, vn, ve, vnew, sigma);
+                    const double wb = n_heavier ? we : wn;  // the weight that collided
                     if (iz) {
                         niz++;
-                        append(births, 1, I, ncell, nfx, nfy, nfz, vn, we, &stats[3]);
-                        append(births, 2, E, ncell, nfx, nfy, nfz, vnew, we, &stats[3]);
-                    } else {
+                        append(births, 1, I, ncell, nfx, nfy, nfz, vn, wb, &stats[3]);
+                        append(births, 2, E, ncell, nfx, nfy, nfz, vnew, wb, &stats[3]);
+                    }
+                    if (n_heavier && !iz) {
                         // the split neutral joins the cell's pool (:698-700)
                         const uint32_t r = append(births, 0, N, ncell, nfx, nfy, nfz, vn, we, &stats[3]);
-                        if ((r & REF_LDS) && r != REF_NONE && nown < POOL_OWN) {
-                            own[nown][threadIdx.x] = (uint16_t)(r & ~REF_LDS);
-                            nown++;
-                        } else if (r != REF_NONE) {
-                            npool_full++;  // past the per-cell list or the staging
+                        if (r != REF_NONE) {
+                            if (nown < POOL_OWN) {
+                                own[nown][threadIdx.x] = r;
+                                nown++;
+                            } else {
+                                const int j = nown - POOL_OWN;
+                                bool room = true;
+                                if (j % (SPILL_CHUNK - 1) == 0) {  // a new chunk
+                                    const unsigned long long bb =
+                                        atomicAdd(spill_fill, (unsigned long long)SPILL_CHUNK);
+                                    if (bb + SPILL_CHUNK <= SPILL_CAP) {
+                                        spill[bb] = chunk;
+                                        chunk = (uint32_t)bb;
+                                    } else {
+                                        room = false;
+                                    }
+                                }
+                                if (room) {
+                                    spill[chunk + 1 + j % (SPILL_CHUNK - 1)] = r;
+                                    nown++;
+                                } else {
+                                    npool_full++;
+                                }
+                            }
                         }
                     }
-                } else if (wn < we) {  // Interactions.cpp:705-726, defined semantics
-                    E.b.w[pe] = (float)wn;
-                    append(births, 2, E, E.b.cell[pe], E.b.f[0][pe], E.b.f[1][pe], E.b.f[2][pe], ve, we - wn,
-                           &stats[3]);
-                    bool iz = collide(a, dr, vn, ve, vnew, sigma);
-                    if (iz) {
-                        niz++;
-                        append(births, 1, I, ncell, nfx, nfy, nfz, vn, wn, &stats[3]);
-                        append(births, 2, E, ncell, nfx, nfy, nfz, vnew, wn, &stats[3]);
+                    if (!n_heavier && iz) {
                         // the ionised neutral leaves the cell's pool as in
                         // the reference (Interactions.cpp:719-721): the
                         // pool's last neutral takes its slot and the pool
@@ -388,9 +430,12 @@ __global__ void __launch_bounds__(BLOCK)
                             N.b.w[pl] = 0.0f;
                             N.b.cell[pl] = DEAD;
                         }
-                        if (nown > 0)
+                        if (nown > 0) {
                             nown--;
-                        else if (Nt > 0)
+                            // the last chunk emptied: back to the previous one
+                            if (nown >= POOL_OWN && (nown - POOL_OWN) % (SPILL_CHUNK - 1) == 0)
+                                chunk = spill[chunk];
+                        } else if (Nt > 0)
                             Nt--;
                         else
                             Nb--;
@@ -490,6 +535,7 @@ extern "C" int pic_mcc_create(pic_ctx* ctx, int sn, int si, int se, const pic_mc
     PIC_HIP(ctx, hipMalloc(&m.d_wsv, 2 * sizeof(double)));
     PIC_HIP(ctx, hipMalloc(&m.d_stats, 6 * sizeof(unsigned long long)));
     PIC_HIP(ctx, hipMalloc(&m.d_part, (size_t)NSTRIPE * SSTRIDE * sizeof(unsigned long long)));
+    PIC_HIP(ctx, hipMalloc(&m.d_spill, (size_t)SPILL_CAP * sizeof(uint32_t)));
     PIC_HIP(ctx, hipMemcpyAsync(m.sig, sig_E, nsig * sizeof(double), hipMemcpyHostToDevice,
                                 ctx->stream));
     PIC_HIP(ctx, hipMemcpyAsync(m.sig + nsig, sig_v, nsig * sizeof(double), hipMemcpyHostToDevice,
@@ -538,7 +584,8 @@ extern "C" int pic_mcc_apply(pic_ctx* ctx, int id, double dt, uint32_t seed, uin
     a.c1 = m.desc.c1;
     a.c2 = m.desc.c2;
     a.B_inc = m.desc.B_inc;
-    a.dt_inv_dv = dt * (1 / (g.dx[0] * g.dx[1] * g.dx[2]));
+    a.dt = dt;
+    a.inv_dv = 1 / (g.dx[0] * g.dx[1] * g.dx[2]);  // World::getCellVolume (World.cpp:101-103)
     a.seed = seed;
     a.step = step;
     a.nsig = m.nsig;
@@ -552,11 +599,11 @@ extern "C" int pic_mcc_apply(pic_ctx* ctx, int id, double dt, uint32_t seed, uin
         if (m.nsig <= SIG_LDS_MAX)
             k_mcc<true><<<grid_for((size_t)g.ncells, BLOCK), BLOCK, 0, ctx->stream>>>(
                 g, a, sn, N.off, two ? N.toff : nullptr, (uint32_t)N.bulk_n, si, se, E.off, m.sig, m.d_wsv,
-                m.d_part);
+                m.d_part, m.d_spill);
         else
             k_mcc<false><<<grid_for((size_t)g.ncells, BLOCK), BLOCK, 0, ctx->stream>>>(
                 g, a, sn, N.off, two ? N.toff : nullptr, (uint32_t)N.bulk_n, si, se, E.off, m.sig, m.d_wsv,
-                m.d_part);
+                m.d_part, m.d_spill);
         k_mcc_finish<<<1, 1, 0, ctx->stream>>>(m.d_wsv, m.d_part, m.d_stats, N.d_n, (uint32_t)N.cap,
                                                I.d_n, (uint32_t)I.cap, E.d_n, (uint32_t)E.cap);
     }

@@ -3,6 +3,7 @@
 // library; the host only keeps mirrors for the reference's public fields.
 #include "pic_host.hpp"
 
+#include <cctype>
 #include <fstream>
 #include <iomanip>
 #include <iostream>
@@ -358,6 +359,67 @@ void MC_MEX_Ionization::apply(type_calc dt) noexcept {
     last_status = pic_mcc_apply(world.ctx, id, dt, seed, step++, &last);
     if (last_status != PIC_OK) std::cerr << "MC_MEX_Ionization: " << pic_last_error(world.ctx) << "\n";
 }
+
+// --------------------------------------------------------------- DSMC_MEX
+DSMC_MEX::DSMC_MEX(Species& species, World& w) noexcept : world(w) {
+    if (pic_dsmc_create(world.ctx, species.sid, species.sid, &id) != PIC_OK) {
+        std::cerr << "DSMC_MEX: " << pic_last_error(world.ctx) << "\n";
+        id = -1;
+    }
+}
+DSMC_MEX::DSMC_MEX(Species& species1, Species& species2, World& w) : world(w) {
+    if (species1.mpw0 != species2.mpw0)  // Interactions.cpp:160-162
+        throw std::invalid_argument(
+            "species must have the same macroparticle weight for this algorithm to work properly.");
+    world.check(pic_dsmc_create(world.ctx, species1.sid, species2.sid, &id), "DSMC_MEX");
+}
+void DSMC_MEX::apply(type_calc dt) noexcept {
+    if (id < 0) {
+        last_status = PIC_ERR_STATE;
+        return;
+    }
+    last_status = pic_dsmc_apply(world.ctx, id, dt, seed, step++, &last);
+    if (last_status != PIC_OK) std::cerr << "DSMC_MEX: " << pic_last_error(world.ctx) << "\n";
+}
+
+// ----------------------------------------------------------------- Source
+// inletName2Index (World.cpp:369-390); an unknown name throws, which a
+// noexcept constructor turns into std::terminate as in the reference
+static int inlet_index(std::string face) {
+    for (char& ch : face) ch = (char)std::tolower((unsigned char)ch);
+    static const std::map<std::string, int> faces{{"x-", 0}, {"x+", 1}, {"y-", 2}, {"y+", 3},
+                                                  {"z-", 4}, {"z+", 5}, {"-x", 0}, {"+x", 1},
+                                                  {"-y", 2}, {"+y", 3}, {"-z", 4}, {"+z", 5}};
+    auto it = faces.find(face);
+    if (it == faces.end()) throw std::invalid_argument("Passed wrong face name.");
+    return it->second;
+}
+Source::Source(Species& species, World& w, type_calc v_drift, type_calc den, type_calc T,
+               std::string inlet_face, type_calc area_frac) noexcept
+    : world(w) {
+    (void)area_frac;
+    const int face = inlet_index(inlet_face);
+    world.flush_host_fields();
+    if (pic_source_create(world.ctx, species.sid, face, v_drift, den, T, &id) != PIC_OK) {
+        std::cerr << "Source: " << pic_last_error(world.ctx) << "\n";
+        id = -1;
+    }
+    world.device_fields_changed();  // the inlet face is Dirichlet at phi = 0
+}
+void Source::sample() const noexcept {
+    if (id < 0) {
+        last_status = PIC_ERR_STATE;
+        return;
+    }
+    last_status = pic_source_sample(world.ctx, id, seed, &last_added);
+    if (last_status != PIC_OK) std::cerr << "Source: " << pic_last_error(world.ctx) << "\n";
+}
+ColdBeamSource::ColdBeamSource(Species& species, World& w, type_calc v_drift, type_calc den,
+                               std::string inlet_face, type_calc area_frac) noexcept
+    : Source(species, w, v_drift, den, 0.0, std::move(inlet_face), area_frac) {}
+WarmBeamSource::WarmBeamSource(Species& species, World& w, type_calc v_drift, type_calc den,
+                               type_calc T, std::string inlet_face, type_calc area_frac) noexcept
+    : Source(species, w, v_drift, den, T, std::move(inlet_face), area_frac) {}
 
 // ----------------------------------------------------------------- Output
 namespace Output {

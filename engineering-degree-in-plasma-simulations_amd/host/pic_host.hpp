@@ -294,3 +294,47 @@ class MC_MEX_Ionization : public Interaction {  // Interactions.h:99-143
     pic_mcc_desc desc{};
     uint32_t step = 0;
 };
+
+// DSMC_MEX (Interactions.h:42-67): equal-weight DSMC with Bird's VHS cross
+// section inside one species or between two of the same mpw0
+class DSMC_MEX : public Interaction {
+  public:
+    DSMC_MEX(Species& species, World& world) noexcept;                 // Interactions.cpp:143-155
+    DSMC_MEX(Species& species1, Species& species2, World& world);      // :156-172
+    void apply(type_calc dt) noexcept override;                       // :179-181
+    pic_dsmc_stats last{};
+    uint32_t seed = 1;
+    int last_status = PIC_OK;
+
+  private:
+    World& world;
+    int id = -1;
+    uint32_t step = 0;
+};
+
+// Source.h: particles injected through one face of the box every sample()
+class Source {
+  public:
+    virtual ~Source() = default;
+    virtual void sample() const noexcept;  // Source.cpp:101-104, :178-181
+    uint32_t seed = 1;
+    mutable size_t last_added = 0;
+    mutable int last_status = PIC_OK;
+
+  protected:
+    // Source.cpp:3-28; area_frac is accepted and unused, as in the reference
+    Source(Species& species, World& world, type_calc v_drift, type_calc den, type_calc T,
+           std::string inlet_face, type_calc area_frac) noexcept;
+    World& world;
+    int id = -1;
+};
+class ColdBeamSource : public Source {  // Source.cpp:32-104
+  public:
+    ColdBeamSource(Species& species, World& world, type_calc v_drift, type_calc den,
+                   std::string inlet_face = "-z", type_calc area_frac = 1.0) noexcept;
+};
+class WarmBeamSource : public Source {  // Source.cpp:105-181
+  public:
+    WarmBeamSource(Species& species, World& world, type_calc v_drift, type_calc den, type_calc T,
+                   std::string inlet_face = "-z", type_calc area_frac = 1.0) noexcept;
+};

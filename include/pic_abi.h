@@ -233,9 +233,10 @@ typedef struct {
 typedef struct {
     long long collisions, ionizations, groups;
     double wsv_max; /* W_sigma_v_rel_max after the call */
-    /* split neutrals that did not join their cell's pick pool (more than 32
-     * in one cell in one call, or the workgroup's 512 staging slots full;
-     * the reference adds every one, :698-700) */
+    /* split neutrals that did not join their cell's pick pool: beyond 16 in
+     * one cell and call they are listed in a 2^20-word arena per MC_MEX
+     * object, and only those finding it full stay out (the reference adds
+     * every one, :698-700); 0 in every case tested */
     long long pool_overflow;
 } pic_mcc_stats;
 /* MC_MEX_Ionization(neutrals, ions, electrons, world, table_path) ctor

@@ -63,14 +63,17 @@ def test_electron_heavier_branch_matches_oracle(pg, sigma_table_path):
     c.upload(se, parts(10 * 512, 3e6, 10.0))  # ~75 eV mean: well above E_ion
     sE, sV = O.load_sigma_table(sigma_table_path)
     E_ion = 1313.9 * 1000 / 6.02214076e23
-    mid = c.mcc_create(sn, si, se, E_ion, sE, sV)
-    prm = O.mcc_params(16 * O.AMU, O.ME, E_ion, sE, sV, 1e-4, dx ** 3, pool_append=True)
-    wsv = 1e-14 * 1e4
+    # W sigma v_max starts at its working value (1.5e-12 for these weights
+    # and energies), so most groups collide
+    dt, wsv = 5e-3, 1.5e-12
+    mid = c.mcc_create(sn, si, se, E_ion, sE, sV, wsv_max_init=wsv)
+    prm = O.mcc_params(16 * O.AMU, O.ME, E_ion, sE, sV, dt, g.dx[0] * g.dx[1] * g.dx[2],
+                       pool_append=True)
     for step in range(2):
         c.sort(sn)
         c.sort(se)
         neu, ele, ion = c.download(sn), c.download(se), c.download(si)
-        st = c.mcc_apply(mid, 1e-4, 13, step)
+        st = c.mcc_apply(mid, dt, 13, step)
         on, oi, oe, wsv_o, ost = O.mcc_apply(g, prm, wsv, O.rng_philox(13, step, TAG_MCC), neu,
                                              ion, ele)
         print("step", step, st, ost.collisions, ost.ionizations, flush=True)

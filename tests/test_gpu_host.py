@@ -28,8 +28,9 @@ def _run(args):
 def test_pcg_fallback_to_gauss_seidel(golden, tmp_path):
     """PCG limited to 2 iterations cannot converge; solve() falls back to
     the sweeps and converges to the reference's criterion.  The result
-    agrees with the oracle's tight solve and with the reference's own
-    fallback run (its loose Newton-step criterion, 1e-3 V)."""
+    agrees with the oracle's dense solve; the reference's own run of the
+    same set-up (its Newton criterion, 1e-3 V, on its axis-swapped matrix)
+    agrees with the oracle's restatement of it."""
     gr = golden["grid"]
     g = O.make_grid(int(gr[0]), int(gr[1]), int(gr[2]), gr[3:6], gr[6:9])
     gr[:9].astype(np.float64).tofile(tmp_path / "grid.bin")
@@ -37,14 +38,25 @@ def test_pcg_fallback_to_gauss_seidel(golden, tmp_path):
     rho.tofile(tmp_path / "rho.bin")
     res = _run(["fallback", str(tmp_path / "grid.bin"), str(tmp_path / "rho.bin"),
                 str(tmp_path / "phi.bin")])
-    assert res["converged"] == 1 and res["ref_l2"] < 1e-6, res
     phi = np.fromfile(tmp_path / "phi.bin")
     nt = np.zeros(O.nn(g), np.int32)
     O.lib().orc_dirichlet_walls(g, nt)
     tight, ok, *_ = O.nrpcg(g, np.zeros(O.nn(g), np.int32), nt, np.zeros(O.nn(g)), rho, 0.0, 0.0,
                             1e20, 5000, 1e-12)
-    scale = np.max(np.abs(tight))
-    assert np.max(np.abs(phi - tight)) <= 1e-8 * scale
+    # the device solves the physical operator; the reference's PCG matrix
+    # places the axis coefficients swapped (PotentialSolver.cpp:457-463),
+    # which on this grid (dx, dy, dz all different) is another problem
+    exact = O.dense_poisson(g, np.zeros(O.nn(g)), rho)
+    scale = np.max(np.abs(exact))
+    r2, b2 = O.residual_dirichlet(g, phi, rho)
+    print("fallback", res, "oracle residual rms %.3e rel %.3e, max |phi - exact| %.3e of %.3e" % (
+        np.sqrt(r2 / O.nn(g)), np.sqrt(r2 / b2), np.max(np.abs(phi - exact)), scale), flush=True)
+    assert res["converged"] == 1 and res["ref_l2"] < 1e-6, res
+    # the reference's criterion sqrt(sum R^2 / N) < 1e-6 (R in V/m^2, the
+    # right-hand side here up to 3e4 V/m^2) leaves phi within 1e-8 of the
+    # exact solution's scale
+    assert np.sqrt(r2 / O.nn(g)) < 1e-6
+    assert np.max(np.abs(phi - exact)) <= 1e-8 * scale
     assert golden["fb_ok"][0] == 1
     assert np.max(np.abs(golden["fb_phi_out"] - tight)) <= 1e-3  # the reference's own accuracy
 
@@ -63,5 +75,43 @@ def test_gas_properties_through_host_classes(golden, tmp_path):
     T = np.fromfile(tmp_path / "T.bin")
     ref_v, ref_T = golden["ele_vel"], golden["ele_T"]
     assert np.max(np.abs(vel - ref_v)) <= 1e-5 * np.max(np.abs(ref_v))
-    # T is a difference of moments: relative to the largest, FP32 particle data
-    assert np.max(np.abs(T - ref_T)) <= 1e-4 * np.max(np.abs(ref_T))
+    # T is a difference of moments (m/(3k) (<v^2> - <v>^2)): nodes reached by
+    # one particle only have T = 0 up to the cancellation of FP32 particle
+    # data, so the bound is relative to the particles' kinetic temperature
+    # scale m max|v|^2 / (3k)
+    q = p.reshape(-1, 7)
+    t_scale = O.ME * np.max(np.sum(q[:, 3:6] ** 2, axis=1)) / (3 * O.KB)
+    print("T: max |T - ref| %.3e, max |ref| %.3e, scale %.3e" % (
+        np.max(np.abs(T - ref_T)), np.max(np.abs(ref_T)), t_scale), flush=True)
+    assert np.max(np.abs(T - ref_T)) <= 1e-6 * t_scale
+
+
+def test_dsmc_and_source_through_host_classes(golden, tmp_path):
+    """DSMC_MEX::apply and ColdBeamSource::sample through the C++ classes on
+    the reference fixture's atomic-O box: the first apply's NTC group count
+    equals the oracle's (it depends only on the cell counts and the initial
+    sigma v_rel max, Interactions.cpp:195), the collisions conserve momentum
+    and kinetic energy, and the beam adds the oracle's number of particles
+    (the draw keyed like the device's)."""
+    gr = golden["dm_grid"]
+    g = O.make_grid(int(gr[0]), int(gr[1]), int(gr[2]), gr[3:6], gr[6:9])
+    dt = float(gr[9])
+    a0 = golden["dm_a_in"].reshape(-1, 7).copy()
+    a0[:, 3:7] = a0[:, 3:7].astype(np.float32).astype(np.float64)
+    gr[:9].astype(np.float64).tofile(tmp_path / "grid.bin")
+    a0.tofile(tmp_path / "parts.bin")
+    res = _run(["collide", str(tmp_path / "grid.bin"), str(tmp_path / "parts.bin"),
+                repr(16 * O.AMU), "1e9", repr(dt), "3"])
+    print("host collide", res, flush=True)
+    _, _, _, nc, ng = O.dsmc_apply(g, a0.reshape(-1), 16 * O.AMU, 1e9, dt, 1e-14,
+                                   O.rng_philox(55, 0, 0x44534D01))
+    assert res["groups"][0] == ng and min(res["collisions"]) > 100
+    m0, m1 = np.array(res["momentum0"]), np.array(res["momentum1"])
+    p_scale = 16 * O.AMU * np.sum(a0[:, 6] * np.linalg.norm(a0[:, 3:6], axis=1))
+    # FP32 velocities: each collision conserves to FP32 rounding
+    assert np.max(np.abs(m1 - m0)) <= 1e-6 * p_scale
+    assert abs(res["ke1"] - res["ke0"]) <= 1e-6 * res["ke0"]
+    ref = O.source_sample(g, O.rng_philox(77, 0, 0x53524301), 0.0, 16 * O.AMU, 1e9, 5, 1e5, 2e22,
+                          0.0, dt)
+    assert res["source_added"] == ref.size // 7 > 50
+    assert res["particles_after_source"] == res["particles_before_source"] + res["source_added"]

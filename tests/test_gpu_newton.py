@@ -106,8 +106,10 @@ def test_newton_learned_inner_solver_32cubed(pg):
     O.lib().orc_dirichlet_walls(g, nt)
     tight, ok, nit, pit, _ = O.nrpcg(g, np.zeros(O.nn(g), np.int32), nt, np.zeros(O.nn(g)), rho,
                                      0.0, n0, Te0, 5000, 1e-13)
+    # a relative residual of 1e-8 leaves an error of up to cond(J) 1e-8 in
+    # phi; cond(J) <= cond(A) ~ 4 (n-1)^2 / pi^2 ~ 400 at 32^3
     for name, (s, phi) in res.items():
-        assert np.max(np.abs(phi - tight)) <= 1e-6 * np.max(np.abs(tight)), name
+        assert np.max(np.abs(phi - tight)) <= 400 * 1e-8 * np.max(np.abs(tight)), name
     assert res["learned"][0]["iterations"] < res["pcg"][0]["iterations"]
 
 
