@@ -101,7 +101,7 @@ int main(int argc, char** argv) {
         type_calc3 m1 = sp.getMomentum();
         const double ke1 = sp.getKE();
         const size_t n_before = sp.getNumParticles();
-        ColdBeamSource src(sp, world, 1e5, 2e22, "+z");
+        ColdBeamSource src(sp, world, 1e5, 2e19, "+z");
         src.seed = 77;
         src.sample();
         if (src.last_status != PIC_OK) return 4;

@@ -111,7 +111,7 @@ def test_dsmc_and_source_through_host_classes(golden, tmp_path):
     # FP32 velocities: each collision conserves to FP32 rounding
     assert np.max(np.abs(m1 - m0)) <= 1e-6 * p_scale
     assert abs(res["ke1"] - res["ke0"]) <= 1e-6 * res["ke0"]
-    ref = O.source_sample(g, O.rng_philox(77, 0, 0x53524301), 0.0, 16 * O.AMU, 1e9, 5, 1e5, 2e22,
+    ref = O.source_sample(g, O.rng_philox(77, 0, 0x53524301), 0.0, 16 * O.AMU, 1e9, 5, 1e5, 2e19,
                           0.0, dt)
     assert res["source_added"] == ref.size // 7 > 50
     assert res["particles_after_source"] == res["particles_before_source"] + res["source_added"]
