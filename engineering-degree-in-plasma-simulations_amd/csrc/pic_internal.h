@@ -298,6 +298,9 @@ int pic_sort_tail(pic_ctx* ctx, int sid);
 int pic_ensure_pin(pic_ctx* ctx, size_t bytes);
 // histogram + ranks of the current arrays (run-aggregated atomics)
 int pic_histogram(pic_ctx* ctx, int sid);
+// number density of a sorted species with its per-cell corner sums in
+// `corner` (8 * ncells floats): launches only
+void pic_density_launch(pic_ctx* ctx, int sid, float* corner);
 inline unsigned grid_for(size_t n, unsigned block) {
     size_t g = (n + block - 1) / block;
     return (unsigned)(g ? g : 1);

@@ -1415,13 +1415,21 @@ extern "C" int pic_deposit(pic_ctx* ctx, int sid, int flags) {
             k_mom_nodes<<<grid_for(nn, BLOCK), BLOCK, 0, ctx->stream>>>(g, ctx->dep_corner, s.mom,
                                                                         want_den ? s.den : nullptr);
         } else {
-            k_dep_cells<<<grid_for((size_t)g.ncells, BLOCK), BLOCK, 0, ctx->stream>>>(
-                g, s.buf[s.cur], s.off, ctx->dep_corner);
-            k_dep_nodes<<<grid_for(nn, BLOCK), BLOCK, 0, ctx->stream>>>(g, ctx->dep_corner, s.den);
+            pic_density_launch(ctx, sid, ctx->dep_corner);
         }
     }
     PIC_HIP(ctx, hipGetLastError());
     return PIC_OK;
+}
+
+// the density of a sorted species through corner sums in `corner` (8
+// floats per cell, every one written before it is read): the two launches,
+// queued on the context's stream, no allocation, no wait
+void pic_density_launch(pic_ctx* ctx, int sid, float* corner) {
+    Species& s = ctx->sp[sid];
+    const GridC& g = ctx->g;
+    k_dep_cells<<<grid_for((size_t)g.ncells, BLOCK), BLOCK, 0, ctx->stream>>>(g, s.buf[s.cur], s.off, corner);
+    k_dep_nodes<<<grid_for((size_t)g.nn, BLOCK), BLOCK, 0, ctx->stream>>>(g, corner, s.den);
 }
 
 extern "C" int pic_clear_samples(pic_ctx* ctx, int sid) {

@@ -44,6 +44,7 @@ ABI_FUNCTIONS = [
     "pic_unet_set_params", "pic_unet_forward", "pic_unet_loss_grad", "pic_unet_adam_step",
     "pic_unet_train", "pic_learned_precondition", "pic_species_set_surface", "pic_merge",
     "pic_dsmc_create", "pic_dsmc_apply", "pic_source_create", "pic_source_sample",
+    "pic_deposit_density_noalloc",
 ]
 
 
@@ -141,6 +142,7 @@ def load_library():
     L.pic_sort.argtypes = [P, C.c_int]
     L.pic_push_sort.argtypes = [P, C.c_int, C.c_double]
     L.pic_deposit.argtypes = [P, C.c_int, C.c_int]
+    L.pic_deposit_density_noalloc.argtypes = [P, C.c_int]
     L.pic_clear_samples.argtypes = [P, C.c_int]
     L.pic_charge_density.argtypes = [P]
     L.pic_poisson.argtypes = [P, C.POINTER(SolverParams), C.POINTER(SolverStats)]
@@ -292,6 +294,11 @@ class Context:
 
     def deposit(self, sid, flags=DEPOSIT_DENSITY):
         self._ck(self.L.pic_deposit(self.h, sid, flags))
+
+    def deposit_density_noalloc(self, sid):
+        """pic_deposit_density_noalloc: the density of a sorted species with
+        no device allocation; returns the status code (0 = PIC_OK)"""
+        return self.L.pic_deposit_density_noalloc(self.h, sid)
 
     def clear_samples(self, sid):
         self._ck(self.L.pic_clear_samples(self.h, sid))

@@ -154,6 +154,18 @@ int pic_merge(pic_ctx* ctx, int sid, uint32_t seed, pic_merge_stats* stats);
 #define PIC_DEPOSIT_DENSITY 1
 #define PIC_DEPOSIT_MOMENTS 2
 int pic_deposit(pic_ctx* ctx, int sid, int flags);
+/* No counterpart in the reference: for callers that cannot allocate device
+ * memory (code under HIP graph capture, serving loops, harnesses that forbid
+ * hipMalloc).  The density of pic_deposit(ctx, sid, PIC_DEPOSIT_DENSITY),
+ * the same kernels and the same result, with the per-cell corner sums in a
+ * fixed 64 MiB device array that each GPU holds until the process ends
+ * instead of a buffer the library allocates.  Allocates nothing, sorts
+ * nothing and does not wait: the species must be sorted (PIC_ERR_STATE
+ * otherwise), the grid at most 2^21 cells (PIC_ERR_CAPACITY otherwise),
+ * and the context's device current.  Every call on one GPU shares the array
+ * (each call writes all it reads), so calls that overlap on one GPU, from
+ * contexts on different streams or threads, race. */
+int pic_deposit_density_noalloc(pic_ctx* ctx, int sid);
 int pic_clear_samples(pic_ctx* ctx, int sid);
 /* World::computeChargeDensity (World.cpp:193-200) over all species */
 int pic_charge_density(pic_ctx* ctx);
