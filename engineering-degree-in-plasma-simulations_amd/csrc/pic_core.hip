@@ -172,6 +172,7 @@ extern "C" void pic_destroy(pic_ctx* ctx) {
     pic_unet_free(ctx);
     (void)hipFree(ctx->phi);
     (void)hipFree(ctx->phi_prev);
+    (void)hipFree(ctx->sor_split);
     (void)hipFree(ctx->rho);
     (void)hipFree(ctx->ef);
     (void)hipFree(ctx->ntype);
@@ -305,6 +306,7 @@ int pic_ensure_ntype(pic_ctx* ctx) {
     if (!ctx->ntype_dirty) return PIC_OK;
     const GridC& g = ctx->g;
     std::vector<uint8_t> nt((size_t)g.nn);
+    bool neumann = false;
     for (int k = 0; k < g.nk; k++)
         for (int j = 0; j < g.nj; j++)
             for (int i = 0; i < g.ni; i++) {
@@ -327,11 +329,13 @@ int pic_ensure_ntype(pic_ctx* ctx) {
                 else
                     t = NT_REG;
                 nt[u] = t;
+                neumann |= t != NT_REG && t != NT_DIR;
             }
     PIC_HIP(ctx, hipMemcpyAsync(ctx->ntype, nt.data(), nt.size(), hipMemcpyHostToDevice,
                                 ctx->stream));
     PIC_HIP(ctx, hipStreamSynchronize(ctx->stream));
     ctx->ntype_dirty = false;
+    ctx->has_neumann = neumann;
     return PIC_OK;
 }
 

@@ -231,9 +231,12 @@ struct pic_ctx {
     };
     std::vector<RectH> rects;
     bool ntype_dirty = true;
+    bool has_neumann = true;  // any Neumann face node (set with the node types)
     // device fields
     double* phi = nullptr;
     double* phi_prev = nullptr;  // previous solution (PIC_SOLVE_EXTRAPOLATE), allocated on demand
+    void* sor_split = nullptr;  // the SOR sweeps' red-black split phi, rho, node types (on demand)
+    size_t sor_split_bytes = 0;
     bool phi_prev_valid = false;
     double* rho = nullptr;
     float4* ef = nullptr;

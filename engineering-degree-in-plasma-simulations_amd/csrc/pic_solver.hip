@@ -174,6 +174,17 @@ struct SorArgs {
     int color;
 };
 
+// The SOR update of a regular node from its neighbour sums along each axis
+// (shared by both sweep kernels, rounded the same way: no contraction).
+__device__ inline double sor_node(const Stencil& s, const SorArgs& a, double p, double rho_u, double sx,
+                                  double sy, double sz) {
+#pragma clang fp contract(off)
+    double q = rho_u;
+    if (a.n0 != 0.0) q -= QE * (a.n0 * exp((p - a.phi0) * a.inv_te0));
+    const double nw = (q * a.ieps + sx * s.cx + sy * s.cy + sz * s.cz) * a.itwos;
+    return p + a.omega * (nw - p);
+}
+
 __global__ void __launch_bounds__(BLOCK)
     k_sor(Stencil s, SorArgs a, const uint8_t* __restrict__ nt, double* __restrict__ phi,
           const double* __restrict__ rho) {
@@ -189,13 +200,66 @@ __global__ void __launch_bounds__(BLOCK)
         if (i >= s.ni) continue;
         long long u = row * s.ni + i;
         if (nt[u] != NT_REG) continue;
-        double p = phi[u];
-        double q = rho[u];
-        if (a.n0 != 0.0) q -= QE * (a.n0 * exp((p - a.phi0) * a.inv_te0));
-        double nw = (q * a.ieps + (phi[u - 1] + phi[u + 1]) * s.cx +
-                     (phi[u - s.sj] + phi[u + s.sj]) * s.cy + (phi[u - s.sk] + phi[u + s.sk]) * s.cz) *
-                    a.itwos;
-        phi[u] = p + a.omega * (nw - p);
+        phi[u] = sor_node(s, a, phi[u], rho[u], phi[u - 1] + phi[u + 1], phi[u - s.sj] + phi[u + s.sj],
+                          phi[u - s.sk] + phi[u + s.sk]);
+    }
+}
+
+// Red-black split layout for the SOR sweeps: each colour of a row (j, k)
+// stored contiguously, hr = (ni + 1) / 2 entries per row and colour; node
+// (i, j, k) of colour c = (i + j + k) & 1 sits at (k nj + j) hr + (i >> 1)
+// of colour c's half.  A colour's sweep then reads and writes whole lines
+// (the natural layout's stride-2 stores write half of every line twice).
+// The neighbours of a node of one colour are the other colour's entries at
+// (i -+ 1) >> 1 of its row and at i >> 1 of the rows j -+ 1 and k -+ 1.
+__global__ void __launch_bounds__(BLOCK)
+    k_split(Stencil s, int hr, long long half, const double* __restrict__ phi, const double* __restrict__ rho,
+            const uint8_t* __restrict__ nt, double* __restrict__ sphi, double* __restrict__ srho,
+            uint8_t* __restrict__ snt) {
+    long long end;
+    for (long long u = chunk_begin(s.nn, end) + threadIdx.x; u < end; u += BLOCK) {
+        const unsigned row = (unsigned)u / (unsigned)s.ni;
+        const int i = (int)((unsigned)u - row * (unsigned)s.ni);
+        const unsigned k = row / (unsigned)s.nj, j = row - k * (unsigned)s.nj;
+        const long long v = ((i + j + k) & 1) * half + (long long)row * hr + (i >> 1);
+        sphi[v] = phi[u];
+        srho[v] = rho[u];
+        snt[v] = nt[u];
+    }
+}
+
+__global__ void __launch_bounds__(BLOCK)
+    k_unsplit(Stencil s, int hr, long long half, const double* __restrict__ sphi, double* __restrict__ phi) {
+    long long end;
+    for (long long u = chunk_begin(s.nn, end) + threadIdx.x; u < end; u += BLOCK) {
+        const unsigned row = (unsigned)u / (unsigned)s.ni;
+        const int i = (int)((unsigned)u - row * (unsigned)s.ni);
+        const unsigned k = row / (unsigned)s.nj, j = row - k * (unsigned)s.nj;
+        phi[u] = sphi[((i + j + k) & 1) * half + (long long)row * hr + (i >> 1)];
+    }
+}
+
+// one colour's half sweep on the split layout: the same node update as
+// k_sor (sor_node), so the sweeps agree with k_sor bit for bit
+__global__ void __launch_bounds__(BLOCK)
+    k_sor_split(Stencil s, SorArgs a, int hr, long long half, const uint8_t* __restrict__ snt,
+                double* __restrict__ sphi, const double* __restrict__ srho) {
+    const int c = a.color;
+    double* own = sphi + c * half;
+    const double* oth = sphi + (c ^ 1) * half;
+    const uint8_t* tn = snt + c * half;
+    const double* rh = srho + c * half;
+    const long long items = (long long)hr * s.nj * s.nk;
+    long long end;
+    for (long long m = chunk_begin(items, end) + threadIdx.x; m < end; m += BLOCK) {
+        const unsigned row = (unsigned)m / (unsigned)hr;
+        const int h = (int)((unsigned)m - row * (unsigned)hr);
+        const unsigned k = row / (unsigned)s.nj, j = row - k * (unsigned)s.nj;
+        const int i = 2 * h + (int)((j + k + c) & 1);
+        if (i >= s.ni || tn[m] != NT_REG) continue;
+        const long long rb = (long long)row * hr;
+        own[m] = sor_node(s, a, own[m], rh[m], oth[rb + ((i - 1) >> 1)] + oth[rb + ((i + 1) >> 1)],
+                          oth[m - hr] + oth[m + hr], oth[m - (long long)s.nj * hr] + oth[m + (long long)s.nj * hr]);
     }
 }
 
@@ -465,25 +529,19 @@ extern "C" int pic_efield(pic_ctx* ctx) {
     return PIC_OK;
 }
 
+// any Neumann face node (a face node that is neither an object node nor
+// flagged Dirichlet); the node types are current
 static bool has_neumann(pic_ctx* ctx) {
-    const GridC& g = ctx->g;
-    // a face node that is neither an object node nor flagged Dirichlet
-    for (int k = 0; k < g.nk; k++)
-        for (int j = 0; j < g.nj; j++)
-            for (int i = 0; i < g.ni; i++) {
-                bool face = i == 0 || i == g.ni - 1 || j == 0 || j == g.nj - 1 || k == 0 ||
-                            k == g.nk - 1;
-                if (!face) continue;
-                size_t u = ((size_t)k * g.nj + j) * g.ni + i;
-                if (ctx->object_id[u] == 0 && ctx->node_type_in[u] != 2) return true;
-            }
-    return false;
+    if (pic_ensure_ntype(ctx) != PIC_OK) return true;
+    return ctx->has_neumann;
 }
 
-static int residual_norms(pic_ctx* ctx, const Stencil& st, const SorArgs& a, double out[3]) {
+static int residual_norms(pic_ctx* ctx, const Stencil& st, const SorArgs& a, double out[3],
+                          const double* phi = nullptr) {
     double* sums = ctx->scratch + LEARNED_WS + 5 * SSIZE;  // three striped scalars
     PIC_HIP(ctx, hipMemsetAsync(sums, 0, 3 * SSIZE * sizeof(double), ctx->stream));
-    k_resid<<<stencil_blocks(st.nn), BLOCK, 0, ctx->stream>>>(st, a, ctx->ntype, ctx->phi, ctx->rho, sums);
+    k_resid<<<stencil_blocks(st.nn), BLOCK, 0, ctx->stream>>>(st, a, ctx->ntype, phi ? phi : ctx->phi,
+                                                             ctx->rho, sums);
     PIC_HIP(ctx, hipGetLastError());
     std::vector<double> h(3 * SSIZE);
     PIC_HIP(ctx, hipMemcpyAsync(h.data(), sums, h.size() * sizeof(double), hipMemcpyDeviceToHost,
@@ -511,27 +569,56 @@ static int solve_sor(pic_ctx* ctx, const pic_solver_params* p, pic_solver_stats*
     bool neu = has_neumann(ctx);
     int check = p->check_every > 0 ? p->check_every : 25;
     long long half_items = (long long)((g.ni + 1) / 2) * g.nj * g.nk;
+    // without Neumann faces the sweeps run on the red-black split layout
+    // (the same values as the natural layout's launches)
+    const bool split = !neu && !(p->flags & PIC_SOLVE_SOR_PLAIN);
+    const int hr = (g.ni + 1) / 2;
+    const long long half = (long long)hr * g.nj * g.nk;
+    double *sphi = nullptr, *srho = nullptr;
+    uint8_t* snt = nullptr;
+    if (split) {
+        const size_t bytes = (size_t)half * 2 * (2 * sizeof(double) + 1);
+        if (ctx->sor_split_bytes < bytes) {
+            PIC_HIP(ctx, hipStreamSynchronize(ctx->stream));
+            (void)hipFree(ctx->sor_split);
+            ctx->sor_split = nullptr;
+            ctx->sor_split_bytes = 0;
+            PIC_HIP(ctx, hipMalloc(&ctx->sor_split, bytes));
+            ctx->sor_split_bytes = bytes;
+        }
+        sphi = reinterpret_cast<double*>(ctx->sor_split);
+        srho = sphi + 2 * half;
+        snt = reinterpret_cast<uint8_t*>(srho + 2 * half);
+        k_split<<<stencil_blocks(s.nn), BLOCK, 0, ctx->stream>>>(s, hr, half, ctx->phi, ctx->rho, ctx->ntype, sphi,
+                                                                 srho, snt);
+    }
     double r[3] = {0, 0, 0};
     int it;
-    bool conv = false;
+    bool conv = false, fresh = false;  // fresh: r holds the residual of the current potential
     double crit = 0;
     for (it = 0; it < p->max_it; it++) {
         {
             PicProfScope ps(ctx, PIC_K_SOR);
             for (int c = 0; c < 2; c++) {
                 a.color = c;
-                k_sor<<<stencil_blocks(half_items), BLOCK, 0, ctx->stream>>>(s, a, ctx->ntype,
-                                                                            ctx->phi, ctx->rho);
+                if (split)
+                    k_sor_split<<<stencil_blocks(half), BLOCK, 0, ctx->stream>>>(s, a, hr, half, snt, sphi, srho);
+                else
+                    k_sor<<<stencil_blocks(half_items), BLOCK, 0, ctx->stream>>>(s, a, ctx->ntype, ctx->phi,
+                                                                                ctx->rho);
             }
             if (neu)
                 for (int ph = 0; ph < 3; ph++)
-                    k_neumann<<<grid_for((size_t)g.nn, BLOCK), BLOCK, 0, ctx->stream>>>(
-                        s, ctx->ntype, ctx->phi, ph);
+                    k_neumann<<<grid_for((size_t)g.nn, BLOCK), BLOCK, 0, ctx->stream>>>(s, ctx->ntype, ctx->phi,
+                                                                                        ph);
         }
         PIC_HIP(ctx, hipGetLastError());
+        fresh = false;
         if (it % check == 0) {  // PotentialSolver.cpp:124
+            if (split) k_unsplit<<<stencil_blocks(s.nn), BLOCK, 0, ctx->stream>>>(s, hr, half, sphi, ctx->phi);
             int rc = residual_norms(ctx, s, a, r);
             if (rc) return rc;
+            fresh = true;
             crit = crit_value(p, r, (double)g.nn);
             if (crit < p->tol) {
                 conv = true;
@@ -539,8 +626,11 @@ static int solve_sor(pic_ctx* ctx, const pic_solver_params* p, pic_solver_stats*
             }
         }
     }
-    int rc = residual_norms(ctx, s, a, r);
-    if (rc) return rc;
+    if (split && !fresh) k_unsplit<<<stencil_blocks(s.nn), BLOCK, 0, ctx->stream>>>(s, hr, half, sphi, ctx->phi);
+    if (!fresh) {
+        int rc = residual_norms(ctx, s, a, r);
+        if (rc) return rc;
+    }
     st->converged = conv;
     st->iterations = conv ? it + 1 : it;
     st->newton_iterations = 0;

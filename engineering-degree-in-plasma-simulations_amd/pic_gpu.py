@@ -23,6 +23,7 @@ TOL_REF_RMS = 0
 TOL_REL = 1
 SOLVE_AXIS_SWAP = 1
 SOLVE_EXTRAPOLATE = 2
+SOLVE_SOR_PLAIN = 4
 
 FIELD_PHI, FIELD_RHO, FIELD_EF, FIELD_NODE_VOL = 0, 1, 2, 3
 SFIELD_DEN, SFIELD_NSUM, SFIELD_NVSUM, SFIELD_NUU, SFIELD_NVV, SFIELD_NWW, SFIELD_MACRO_COUNT = range(7)

@@ -190,6 +190,12 @@ int pic_charge_density(pic_ctx* ctx);
  * first flagged solve, and the first after phi is uploaded, starts from phi.
  * The solve runs to the same tolerance either way. */
 #define PIC_SOLVE_EXTRAPOLATE 2
+/* Red-black SOR on the natural node layout, in place.  Without it, a grid
+ * without Neumann faces sweeps on a copy of phi, rho and the node types in
+ * red-black split layout (each colour of a row contiguous, so a colour's
+ * sweep writes whole lines); both give the same values bit for bit (the
+ * flag is for comparisons). */
+#define PIC_SOLVE_SOR_PLAIN 4
 typedef struct {
     int method;
     int tol_kind;

@@ -307,6 +307,36 @@ def test_sor_dirichlet_matches_oracle(pg, golden):
     assert np.max(np.abs(phi - ref)) <= 1e-5 * np.max(np.abs(exact))
 
 
+@pytest.mark.parametrize("shape,n0", [((67, 45, 37), 0.0), ((130, 33, 29), 1e10)])
+def test_split_sor_sweeps_equal_plain(pg, shape, n0):
+    """The sweeps on the red-black split layout (default without Neumann
+    faces) against the natural layout's (PIC_SOLVE_SOR_PLAIN): the same
+    potential bit for bit after 37 sweeps, on grids of odd and even ni, with
+    two objects and, in the second case, the Boltzmann term."""
+    ni, nj, nk = shape
+    L = [1e-2 * (ni - 1), 1e-2 * (nj - 1), 1e-2 * (nk - 1)]
+    rng = np.random.default_rng(ni)
+    rho = 1e-9 * rng.normal(size=ni * nj * nk)
+    phi0 = 0.1 * rng.normal(size=ni * nj * nk)
+    out = []
+    for flags in (0, pg.SOLVE_SOR_PLAIN):
+        c = pg.Context(ni, nj, nk, [0, 0, 0], L)
+        c.dirichlet_walls()
+        c.add_rectangle([0.3 * L[0], 0.4 * L[1], 0.5 * L[2]], [0.2 * L[0], 0.3 * L[1], 0.25 * L[2]], 1.5)
+        c.add_rectangle([0.8 * L[0], 0.7 * L[1], 0.2 * L[2]], [0.1 * L[0], 0.2 * L[1], 0.3 * L[2]], -2.0)
+        c.set_field(pg.FIELD_RHO, rho)
+        c.set_field(pg.FIELD_PHI, phi0)
+        st = c.poisson(pg.SOLVER_SOR_RB, tol=1e-300, tol_kind=pg.TOL_REL, max_it=37, omega=1.6,
+                       check_every=10, n0=n0, Te0=2.0, flags=flags)
+        assert not st["converged"] and st["iterations"] == 37
+        out.append((c.get_field(pg.FIELD_PHI), st["rel_residual"]))
+        c.close()
+    assert np.all(np.isfinite(out[0][0]))
+    assert np.array_equal(out[0][0], out[1][0])
+    # the residual's striped sums add in no fixed order
+    assert out[0][1] == pytest.approx(out[1][1], rel=1e-12)
+
+
 @pytest.mark.parametrize("method", ["SOLVER_SOR_RB", "SOLVER_PCG"])
 def test_extrapolated_warm_start(pg, golden, method):
     """PIC_SOLVE_EXTRAPOLATE on a charge density that varies linearly in time:
