@@ -53,7 +53,7 @@ ALG_BYTES = {
 ALG_BYTES_FCG = 80.0
 KERNEL_NAMES = {
     "push": "k_push<false,2>", "scatter": "k_push_scatter", "deposit": "k_dep_cells + k_dep_nodes",
-    "pcg": "k_pcg1<float> + k_pcg2<float>", "sor": "k_sor",
+    "pcg": "k_pcg1<float> + k_pcg2<float>", "sor": "k_sor_split (one launch per colour)",
 }
 
 
@@ -619,6 +619,8 @@ def main():
     ap.add_argument("--no-extras", action="store_true", help="skip the secondary rows (moments, "
                                                              "diagnostics, loader, Newton-Boltzmann)")
     ap.add_argument("--only-dsmc", action="store_true", help="print the DSMC section alone")
+    ap.add_argument("--only-discharge", action="store_true", help="print the discharge-loop section alone")
+    ap.add_argument("--only-c5", action="store_true", help="print the C5 section alone")
     ap.add_argument("--dsmc-neutrals", type=int, default=40_000_000)
     ap.add_argument("--dsmc-electrons", type=int, default=5_000_000)
     ap.add_argument("--dsmc-ions", type=int, default=5_000_000)
@@ -650,11 +652,11 @@ def main():
         # the reference's default is all cores but one (Config.cpp:68-75), of
         # the 16 host cores a one-GPU job may use here
         threads = max(1, min(16, os.cpu_count() or 1) - 1)
-        if not args.only_dsmc:
+        if not (args.only_dsmc or args.only_discharge or args.only_c5):
             cpu = cpu_baseline(threads, args.cpu_particles, args.cpu_steps)
             print("cpu baseline: %.3e pushes/s (%s, %d threads)" % (cpu["value"], cpu["kind"],
                                                                      threads), file=sys.stderr)
-        if not args.no_dsmc:
+        if not args.no_dsmc and not (args.only_discharge or args.only_c5):
             # 32 x 32 x 128 cells (an eighth of the workload's) at its densities
             f = 32 * 32 * 128 / (63 * 63 * 255)
             cpu_dsmc = cpu_baseline_dsmc(33, 129, int(args.dsmc_neutrals * f), int(args.dsmc_electrons * f),
@@ -662,7 +664,7 @@ def main():
             if cpu_dsmc:
                 print("cpu baseline dsmc: %.3e particles/s (reference apply, 1 thread)" % cpu_dsmc["value"],
                       file=sys.stderr)
-        if not args.only_dsmc and not args.no_extras:
+        if not (args.only_dsmc or args.only_discharge or args.only_c5) and not args.no_extras:
             cpu_newton = cpu_baseline_newton(64, threads)
 
     import pic_gpu as pg
@@ -671,6 +673,11 @@ def main():
         d = dsmc_bench(pg, args, cpu_dsmc)
         d["n_gpus"] = 1
         d["higher_is_better"] = True
+        print(json.dumps(d))
+        return 0
+    if args.only_discharge or args.only_c5:
+        d = discharge_bench(pg, args) if args.only_discharge else c5_bench(pg, args)
+        d["n_gpus"] = 1
         print(json.dumps(d))
         return 0
 
