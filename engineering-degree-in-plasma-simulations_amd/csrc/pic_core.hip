@@ -132,6 +132,8 @@ extern "C" int pic_create(const pic_grid_desc* gd, pic_ctx** out) {
     if (e == hipSuccess) e = hipMalloc(&ctx->phi, nn * sizeof(double));
     if (e == hipSuccess) e = hipMalloc(&ctx->rho, nn * sizeof(double));
     if (e == hipSuccess) e = hipMalloc(&ctx->ef, nn * sizeof(float4));
+    if (e == hipSuccess) e = hipMalloc(&ctx->emax, 4 * sizeof(float));
+    if (e == hipSuccess) e = hipMemsetAsync(ctx->emax, 0, 4 * sizeof(float), ctx->stream);
     if (e == hipSuccess) e = hipMalloc(&ctx->ntype, nn);
     if (e == hipSuccess) e = hipMalloc(&ctx->scratch, SCRATCH_DOUBLES * sizeof(double));
     if (e == hipSuccess) e = hipMalloc(&ctx->scan_tmp, ((size_t)g.ncells / 4096 + 2) * 4);
@@ -175,6 +177,7 @@ extern "C" void pic_destroy(pic_ctx* ctx) {
     (void)hipFree(ctx->sor_split);
     (void)hipFree(ctx->rho);
     (void)hipFree(ctx->ef);
+    (void)hipFree(ctx->emax);
     (void)hipFree(ctx->ntype);
     (void)hipFree(ctx->scratch);
     (void)hipFree(ctx->scan_tmp);
@@ -574,10 +577,14 @@ extern "C" int pic_particles_download(pic_ctx* ctx, int sid, double* aos7, size_
 // ------------------------------------------------------------------------
 // field transfer
 // ------------------------------------------------------------------------
-__global__ void k_ef_pack(GridC g, const double* __restrict__ e3, float4* __restrict__ ef) {
+__global__ void k_ef_pack(GridC g, const double* __restrict__ e3, float4* __restrict__ ef, float* __restrict__ emax) {
     long long u = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (u >= g.nn) return;
-    ef[u] = make_float4((float)e3[3 * u], (float)e3[3 * u + 1], (float)e3[3 * u + 2], 0.f);
+    float4 e = make_float4(0.f, 0.f, 0.f, 0.f);
+    if (u < g.nn) {
+        e = make_float4((float)e3[3 * u], (float)e3[3 * u + 1], (float)e3[3 * u + 2], 0.f);
+        ef[u] = e;
+    }
+    block_max3_to(fabsf(e.x), fabsf(e.y), fabsf(e.z), emax);
 }
 
 // FP64 E = -grad(phi) (PotentialSolver.cpp:354-408), for downloads/diagnostics
@@ -622,7 +629,8 @@ extern "C" int pic_field_upload(pic_ctx* ctx, int which, const double* src) {
         int rc = pic_ensure_work(ctx, nn * 24);
         if (rc) return rc;
         PIC_HIP(ctx, hipMemcpyAsync(ctx->work, src, nn * 24, hipMemcpyHostToDevice, ctx->stream));
-        k_ef_pack<<<grid_for(nn, 256), 256, 0, ctx->stream>>>(ctx->g, ctx->work, ctx->ef);
+        PIC_HIP(ctx, hipMemsetAsync(ctx->emax, 0, 4 * sizeof(float), ctx->stream));
+        k_ef_pack<<<grid_for(nn, 256), 256, 0, ctx->stream>>>(ctx->g, ctx->work, ctx->ef, ctx->emax);
         PIC_HIP(ctx, hipGetLastError());
     } else {
         return pic_fail(ctx, PIC_ERR_ARG, "field not uploadable");

@@ -240,6 +240,9 @@ struct pic_ctx {
     bool phi_prev_valid = false;
     double* rho = nullptr;
     float4* ef = nullptr;
+    // max over the nodes of |Ex|, |Ey|, |Ez| of ef (as float bits, for
+    // atomicMax), kept with every write of ef
+    float* emax = nullptr;
     uint8_t* ntype = nullptr;
     double* scratch = nullptr;   // reductions and solver scalars
     uint32_t* scan_tmp = nullptr; // block sums of the offset scan
@@ -301,6 +304,35 @@ int pic_sort_tail(pic_ctx* ctx, int sid);
 int pic_ensure_pin(pic_ctx* ctx, size_t bytes);
 // histogram + ranks of the current arrays (run-aggregated atomics)
 int pic_histogram(pic_ctx* ctx, int sid);
+// The workgroup's maxima of three non-negative floats into m[0..2] (as
+// unsigned bits, which order like the floats; a NaN orders above every
+// number).  Every thread of the workgroup (blockDim.x a multiple of 64, at
+// most 1024) calls this.
+__device__ inline void block_max3_to(float a0, float a1, float a2, float* m) {
+    __shared__ float red[16][3];
+    for (int o = 32; o > 0; o >>= 1) {
+        a0 = fmaxf(a0, __shfl_xor(a0, o, 64));
+        a1 = fmaxf(a1, __shfl_xor(a1, o, 64));
+        a2 = fmaxf(a2, __shfl_xor(a2, o, 64));
+    }
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    if (lane == 0) {
+        red[wv][0] = a0;
+        red[wv][1] = a1;
+        red[wv][2] = a2;
+    }
+    __syncthreads();
+    if (threadIdx.x < 3) {
+        float v = 0.f;
+        for (int w = 0; w < (int)(blockDim.x >> 6); w++) v = fmaxf(v, red[w][threadIdx.x]);
+        // an atomic only when the value raises the maximum: thousands of
+        // workgroups contend for the three words
+        unsigned* mp = reinterpret_cast<unsigned*>(m) + threadIdx.x;
+        const unsigned bits = __float_as_uint(v);
+        if (bits > __hip_atomic_load(mp, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) atomicMax(mp, bits);
+    }
+}
+
 // number density of a sorted species with its per-cell corner sums in
 // `corner` (8 * ncells floats): launches only
 void pic_density_launch(pic_ctx* ctx, int sid, float* corner);

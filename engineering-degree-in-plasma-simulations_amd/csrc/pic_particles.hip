@@ -93,8 +93,28 @@ struct PushArgs {
     float qmdt;    // dt * q / m   (Species.cpp:372)
     float dtx[3];  // dt / dx      (drift in cells)
     int kick;
+    // the field's per-component |E| maxima, for the histogram pass's test of
+    // particles that cannot leave their cell; null: every particle is pushed
+    const float* emax;
     SurfArgs s;
 };
+
+// A particle whose drift cannot take it out of its cell, whatever E it
+// gathers: for every axis |f' - f| <= (|v| + |qm dt| max|E|) dt/dx (with
+// margins for the FP32 rounding of the kick, the interpolation and the
+// drift) leaves f' strictly inside (0, 1).  Its new cell is its old one, so
+// the histogram pass needs no gather for it; the pass that writes pushes
+// every particle and finds the same cell.
+__device__ inline bool stays_in_cell(const float (&f)[3], const float (&v)[3], const float (&kick)[3],
+                                     const float (&dtx)[3]) {
+    bool in = true;
+#pragma unroll
+    for (int d = 0; d < 3; d++) {
+        const float m = (fabsf(v[d]) + kick[d]) * fabsf(dtx[d]) * 1.00001f + 1e-6f;
+        in = in && (f[d] > m) && (f[d] + m < 1.0f);  // false for NaN
+    }
+    return in;
+}
 
 // 32-bit node offsets (grids hold fewer than 2^31 nodes)
 __device__ inline float3 gather_e(const GridC& g, const float4* __restrict__ E, int i, int j, int k,
@@ -406,6 +426,12 @@ __global__ void __launch_bounds__(BLOCK)
     if (p0 == 0) d_n[1] = n;
     uint32_t c[ILP];
     float f[ILP][3], v[ILP][3];
+    // the largest kick per axis (histogram pass, no objects)
+    const bool bound = !WRITE && !SURF && a.emax != nullptr;
+    float kick[3] = {0.f, 0.f, 0.f};
+    if (bound && a.kick)
+#pragma unroll
+        for (int d = 0; d < 3; d++) kick[d] = fabsf(a.qmdt) * a.emax[d];
 #pragma unroll
     for (int q = 0; q < ILP; q++) {
         const size_t p = p0 + (size_t)q * BLOCK;
@@ -427,7 +453,9 @@ __global__ void __launch_bounds__(BLOCK)
         if (valid) {
             key = (uint32_t)g.ncells;
             if (c[q] != DEAD) {
-                const uint32_t nc = advance<SURF>(g, E, a, c[q], f[q], v[q], b, p);
+                const uint32_t nc = (bound && stays_in_cell(f[q], v[q], kick, a.dtx))
+                                        ? c[q]
+                                        : advance<SURF>(g, E, a, c[q], f[q], v[q], b, p);
                 key = cell_key(g, nc);
                 if (WRITE) {
                     if (nc != c[q]) b.cell[p] = nc;  // most particles stay in their cell
@@ -1176,6 +1204,8 @@ static PushArgs push_args(pic_ctx* ctx, Species& s, double dt) {
     a.qmdt = (float)(dt * s.charge / s.mass);
     for (int d = 0; d < 3; d++) a.dtx[d] = (float)(dt * ctx->g.inv_dx[d]);
     a.kick = s.charge != 0.0;
+    // without objects a particle that stays in its cell cannot be removed
+    a.emax = ctx->g.nrect ? nullptr : ctx->emax;
     SurfArgs& f = a.s;
     f = SurfArgs{};
     f.mode = ctx->g.nrect ? s.surf_mode : PIC_SURF_ABSORB;

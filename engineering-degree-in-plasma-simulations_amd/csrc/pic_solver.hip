@@ -129,16 +129,13 @@ __device__ inline T apply_A(uint8_t t, long long u, const Stencil& s, F v) {
 // ------------------------------------------------------------------------
 // E-field
 // ------------------------------------------------------------------------
-__global__ void __launch_bounds__(BLOCK)
-    k_efield(GridC g, const double* __restrict__ phi, float4* __restrict__ ef) {
-    long long u = (long long)blockIdx.x * BLOCK + threadIdx.x;
-    if (u >= g.nn) return;
+__device__ inline void efield_node(const GridC& g, const double* __restrict__ phi, float4* __restrict__ ef,
+                                   long long u, float (&e)[3]) {
     unsigned uu = (unsigned)u, row = uu / (unsigned)g.ni, kk = row / (unsigned)g.nj;
     int i = (int)(uu - row * (unsigned)g.ni), j = (int)(row - kk * (unsigned)g.nj), k = (int)kk;
     int ijk[3] = {i, j, k};
     int n[3] = {g.ni, g.nj, g.nk};
     long long st[3] = {1, g.ni, (long long)g.ni * g.nj};
-    float e[3];
 #pragma unroll
     for (int d = 0; d < 3; d++) {
         double inv2 = 1.0 / (2 * g.dx[d]);
@@ -154,6 +151,20 @@ __global__ void __launch_bounds__(BLOCK)
     }
     ef[u] = make_float4(e[0], e[1], e[2], 0.f);
 }
+
+// a few nodes per thread (grid-stride): one maximum reduction per workgroup
+__global__ void __launch_bounds__(BLOCK)
+    k_efield(GridC g, const double* __restrict__ phi, float4* __restrict__ ef, float* __restrict__ emax) {
+    float m[3] = {0.f, 0.f, 0.f};
+    for (long long u = (long long)blockIdx.x * BLOCK + threadIdx.x; u < g.nn; u += (long long)gridDim.x * BLOCK) {
+        float e[3];
+        efield_node(g, phi, ef, u, e);
+#pragma unroll
+        for (int d = 0; d < 3; d++) m[d] = fmaxf(m[d], fabsf(e[d]));
+    }
+    block_max3_to(m[0], m[1], m[2], emax);  // the push's bound on the kick
+}
+
 
 // warm start of a time loop's solve: phi = 2 phi - prev, prev = the old phi
 __global__ void __launch_bounds__(BLOCK)
@@ -403,7 +414,9 @@ __global__ void __launch_bounds__(BLOCK)
     }
 }
 
-// d_new = beta d_old - s (beta = gs[t]/gs[t-1], 0 at t = 0); z = J d_new
+// d_new = beta d_old - s (beta = gs[t]/gs[t-1], 0 at t = 0); z = J d_new,
+// by the recurrence J d_new = beta J d_old - J s: the stencil of s alone
+// (seven loads instead of fourteen), the previous z in place
 template <typename T>
 __global__ void __launch_bounds__(BLOCK)
     k_pcg1(Stencil s, const uint8_t* __restrict__ nt, const T* __restrict__ dold,
@@ -418,11 +431,15 @@ __global__ void __launch_bounds__(BLOCK)
     double acc = 0;
     long long end;
     for (long long u = chunk_begin(s.nn, end) + threadIdx.x; u < end; u += BLOCK) {
-        auto dn = [&](long long q) { return bt * dold[q] - sv[q]; };
         uint8_t ty = nt[u];
-        T du = dn(u);
-        T zu = apply_A<T>(ty, u, s, dn);
-        if (P) zu -= (T)P[u] * du;
+        const T su = sv[u];
+        T js = apply_A<T>(ty, u, s, [&](long long q) { return sv[q]; });
+        if (P) js -= (T)P[u] * su;
+        T du = -su, zu = -js;
+        if (t > 0) {
+            du += bt * dold[u];
+            zu += bt * z[u];
+        }
         dnew[u] = du;
         z[u] = zu;
         acc += (double)du * (double)zu;
@@ -522,8 +539,9 @@ extern "C" int pic_efield(pic_ctx* ctx) {
     if (!ctx) return PIC_ERR_ARG;
     {
         PicProfScope ps(ctx, PIC_K_EFIELD);
-        k_efield<<<grid_for((size_t)ctx->g.nn, BLOCK), BLOCK, 0, ctx->stream>>>(ctx->g, ctx->phi,
-                                                                                ctx->ef);
+        PIC_HIP(ctx, hipMemsetAsync(ctx->emax, 0, 4 * sizeof(float), ctx->stream));
+        k_efield<<<std::min(grid_for((size_t)ctx->g.nn, BLOCK), 2048u), BLOCK, 0, ctx->stream>>>(
+            ctx->g, ctx->phi, ctx->ef, ctx->emax);
     }
     PIC_HIP(ctx, hipGetLastError());
     return PIC_OK;
@@ -897,7 +915,8 @@ __global__ void __launch_bounds__(BLOCK)
 }
 
 // p_k = z_k + beta p_{k-1} (beta = -zq[k-1] / pq[k-1]; p_0 = z_0), q = A p_k on
-// regular rows; pq[k] += p_k.q.  z and p vanish on non-regular nodes, and
+// regular rows (by the recurrence A p_k = A z_k + beta A p_{k-1}); pq[k] +=
+// p_k.q.  z and p vanish on non-regular nodes, and
 // regular nodes are interior (Dirichlet walls), so q is the 7-point stencil.
 // Neighbour offsets are zeroed on non-regular nodes so that every load is in
 // bounds and the loop has no branches.
@@ -911,17 +930,23 @@ __global__ void __launch_bounds__(BLOCK)
         beta = den != 0.0 ? -sget(h.zq + (k - 1) * SSIZE) / den : 0.0;
     }
     const bool first = k == 0;
-    auto pn = [&](unsigned w) { return first ? z[w] : z[w] + beta * pold[w]; };
     double acc = 0;
     long long end;
     for (long long m = chunk_begin(s.nn, end) + threadIdx.x; m < end; m += BLOCK) {
         const unsigned u = (unsigned)m;
         const bool reg = nt[u] == NT_REG;
         const unsigned oi = reg ? 1u : 0u, oj = reg ? (unsigned)s.sj : 0u, ok = reg ? (unsigned)s.sk : 0u;
-        double pu = pn(u);
-        double qv = (pn(u - ok) + pn(u + ok)) * s.cz + (pn(u - oj) + pn(u + oj)) * s.cy +
-                    (pn(u - oi) + pn(u + oi)) * s.cx - s.twos * pu;
-        if (P) qv -= P[u] * pu;  // J = A - diag(P) (the Newton Jacobian)
+        // q = A p = A z + beta A p_old: the stencil of z alone (seven loads
+        // instead of fourteen), the previous q in place
+        const double zu = z[u];
+        double az = (z[u - ok] + z[u + ok]) * s.cz + (z[u - oj] + z[u + oj]) * s.cy +
+                    (z[u - oi] + z[u + oi]) * s.cx - s.twos * zu;
+        if (P) az -= P[u] * zu;  // J = A - diag(P) (the Newton Jacobian)
+        double pu = zu, qv = az;
+        if (!first) {
+            pu += beta * pold[u];
+            qv += beta * q[u];
+        }
         qv = reg ? qv : 0.0;
         pnew[u] = pu;
         q[u] = qv;
