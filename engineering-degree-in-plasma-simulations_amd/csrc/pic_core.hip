@@ -163,6 +163,10 @@ extern "C" void pic_destroy(pic_ctx* ctx) {
         (void)hipFree(s.den);
         (void)hipFree(s.mom);
         (void)hipFree(s.toff);
+        (void)hipFree(s.tidx);
+        (void)hipFree(s.boff);
+        (void)hipFree(s.toff_n);
+        (void)hipFree(s.tidx_n);
     }
     for (auto& m : ctx->mcc) {
         (void)hipFree(m.sig);
@@ -461,11 +465,22 @@ __global__ void k_upload(GridC g, SpeciesBuf b, const double* __restrict__ aos, 
     b.w[o] = (float)q[6];
 }
 
-__global__ void k_download(GridC g, SpeciesBuf b, double* __restrict__ aos, size_t n) {
-    size_t p = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= n) return;
-    uint32_t c = b.cell[p];
-    double* q = aos + 7 * p;
+// record p of the download: particle p, or for a species with an indexed
+// appended part (tidx) and p >= bulk its (p - bulk)-th index entry (the
+// order of the cells' pick pools)
+__global__ void k_download(GridC g, SpeciesBuf b, double* __restrict__ aos, size_t n,
+                           const uint32_t* __restrict__ tidx, size_t bulk, const uint32_t* __restrict__ tlive) {
+    const size_t r = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= n) return;
+    size_t p = r;
+    uint32_t c = DEAD;
+    if (!tidx || r < bulk) {
+        c = b.cell[p];
+    } else if (r - bulk < *tlive) {
+        p = tidx[r - bulk];
+        c = b.cell[p];
+    }
+    double* q = aos + 7 * r;
     if (c == DEAD) {
         for (int d = 0; d < 7; d++) q[d] = __builtin_nan("");
         return;
@@ -515,6 +530,7 @@ static int read_count(pic_ctx* ctx, Species& s, size_t* n) {
     PIC_HIP(ctx, hipStreamSynchronize(ctx->stream));
     *n = hn[0];
     s.n_upper = hn[0];
+    s.bulk_n = hn[3];  // the live count of the last whole sort
     return PIC_OK;
 }
 
@@ -549,11 +565,20 @@ extern "C" int pic_particles_download(pic_ctx* ctx, int sid, double* aos7, size_
     size_t n;
     rc = read_count(ctx, s, &n);
     if (rc) return rc;
+    // DSMC births behind a sorted part: indexed first, then listed in the
+    // index's order, which is the order the next DSMC step picks them in
+    if (s.tail_ok && !s.sorted && n > s.bulk_n) {
+        rc = pic_sort_tail(ctx, sid);
+        if (rc) return rc;
+    }
+    const bool perm = s.tail_ok && !s.sorted && s.two_part;
     std::vector<double> host(n * 7);
     if (n) {
         double* stage = nullptr;
         PIC_HIP(ctx, hipMalloc(&stage, n * 7 * sizeof(double)));
-        k_download<<<grid_for(n, 256), 256, 0, ctx->stream>>>(ctx->g, s.buf[s.cur], stage, n);
+        k_download<<<grid_for(n, 256), 256, 0, ctx->stream>>>(ctx->g, s.buf[s.cur], stage, n,
+                                                              perm ? s.tidx : nullptr, s.bulk_n,
+                                                              perm ? s.toff + ctx->g.ncells : nullptr);
         hipError_t e = hipGetLastError();
         if (e == hipSuccess)
             e = hipMemcpyAsync(host.data(), stage, n * 7 * sizeof(double), hipMemcpyDeviceToHost,

@@ -155,14 +155,18 @@ struct Species {
     bool hist_valid = false;     // rank/count match the current arrays
     bool sorted = false;         // arrays ordered by cell, off valid
     // DSMC births are appended at the tail of a sorted species; the next
-    // DSMC step may sort only them (pic_sort_tail) while nothing else has
+    // DSMC step may index only them (pic_sort_tail) while nothing else has
     // changed the particles (tail_ok).  two_part: the arrays are the sorted
-    // part [0, bulk_n) with offsets off, then a sorted appended part with
-    // offsets toff (relative to bulk_n).
+    // part [0, bulk_n) with offsets off, then an appended part whose
+    // positions tidx lists in cell order, with offsets toff into tidx; the
+    // index covers [bulk_n, t_end).  boff, toff_n, tidx_n: the next
+    // extension of the index.
     bool tail_ok = false;
     bool two_part = false;
-    size_t bulk_n = 0;
+    size_t bulk_n = 0, t_end = 0;
     uint32_t* toff = nullptr;
+    uint32_t* tidx = nullptr;
+    uint32_t *boff = nullptr, *toff_n = nullptr, *tidx_n = nullptr;
     double* den = nullptr;       // number density
     double* mom = nullptr;       // n, nvx, nvy, nvz, nuu, nvv, nww (7 * nn)
     uint32_t seq = 0;            // loader stream id
@@ -299,6 +303,7 @@ int pic_check_sid(pic_ctx* ctx, int sid);
 int pic_ensure_ntype(pic_ctx* ctx);
 int pic_ensure_work(pic_ctx* ctx, size_t bytes);
 int pic_sort_tail(pic_ctx* ctx, int sid);
+int pic_join_tail(pic_ctx* ctx, int sid);
 // ctx->pin holds at least `bytes` of pinned host memory (device-to-host
 // copies into it are asynchronous, unlike copies into pageable memory)
 int pic_ensure_pin(pic_ctx* ctx, size_t bytes);

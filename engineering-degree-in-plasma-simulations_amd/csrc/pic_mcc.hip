@@ -223,11 +223,16 @@ constexpr int POOL_OWN = 16;
 constexpr int SPILL_CHUNK = 32;
 constexpr uint32_t SPILL_CAP = 1u << 20;  // words (4 MiB) per MC_MEX object
 constexpr uint32_t CHUNK_NONE = 0xFFFFFFFFu;
+// The appended part's size, as a fraction of the sorted part, past which it
+// is joined with it: indexing costs grow with the appended part, a join
+// streams the whole species once, and picks through the index are scattered
+// reads.
+constexpr double JOIN_FRACTION = 0.75;
 
 template <bool LDS_TABLE>
 __global__ void __launch_bounds__(BLOCK)
     k_mcc(GridC g, MccArgs a, Sp N, const uint32_t* __restrict__ offN, const uint32_t* __restrict__ toffN,
-          uint32_t tbase, Sp I, Sp E, const uint32_t* __restrict__ offE, const double* __restrict__ table,
+          const uint32_t* __restrict__ tidxN, Sp I, Sp E, const uint32_t* __restrict__ offE, const double* __restrict__ table,
           const double* __restrict__ wsv, unsigned long long* __restrict__ part, uint32_t* __restrict__ spill) {
     // this workgroup's stripe: collisions, ionizations, groups, overflow, step maximum
     unsigned long long* stats = part + (blockIdx.x % NSTRIPE) * SSTRIDE;
@@ -252,22 +257,22 @@ __global__ void __launch_bounds__(BLOCK)
         uint32_t e0 = offE[c], e1 = offE[c + 1];
         uint32_t n0 = offN[c], n1 = offN[c + 1];
         // a cell's neutrals: its range of the sorted part, then (toffN) its
-        // range of the sorted appended part at tbase
+        // range of the appended part's cell-ordered index tidxN
         uint32_t t0 = 0, t1 = 0;
         if (toffN) {
-            t0 = tbase + toffN[c];
-            t1 = tbase + toffN[c + 1];
+            t0 = toffN[c];
+            t1 = toffN[c + 1];
         }
         // the cell's pick pool (Interactions.cpp:633-635, :698-700): its
-        // range of the sorted part (Nb), its range of the sorted appended
-        // part (Nt), then the neutrals split off in this call (nown, as the
+        // range of the sorted part (Nb), its entries of the appended part's
+        // index (Nt), then the neutrals split off in this call (nown, as the
         // reference appends their indices to the cell's list); removals take
         // the last entry into the removed slot (:719-721)
         int Nb = (int)(n1 - n0), Nt = (int)(t1 - t0), nown = 0;
         const int Ne = (int)(e1 - e0), Nn0 = Nb + Nt;
         auto loc_of = [&](int slot) -> uint32_t {
             if (slot < Nb) return n0 + (uint32_t)slot;
-            if (slot < Nb + Nt) return t0 + (uint32_t)(slot - Nb);
+            if (slot < Nb + Nt) return tidxN[t0 + (uint32_t)(slot - Nb)];
             const int k = slot - Nb - Nt;
             if (k < POOL_OWN) return own[k][threadIdx.x];
             // walk back from the last chunk to the one holding entry k
@@ -557,15 +562,18 @@ extern "C" int pic_mcc_apply(pic_ctx* ctx, int id, double dt, uint32_t seed, uin
     Mcc& m = ctx->mcc[id];
     Species &N = ctx->sp[m.sn], &I = ctx->sp[m.si], &E = ctx->sp[m.se];
     int rc = 0;
-    // the neutrals: only the births appended since their last sort are
-    // sorted while they stay a few (the neutrals move once per many steps),
-    // else the whole species
+    // the neutrals (they move once per many steps): the births appended
+    // since their last whole sort are indexed by cell, and joined with the
+    // sorted part once they pass JOIN_FRACTION of it; after anything else a
+    // whole counting sort
     if (!N.sorted) {
-        const size_t tail = N.n_upper > N.bulk_n ? N.n_upper - N.bulk_n : 0;
-        if (N.tail_ok && N.n_upper >= N.bulk_n && tail <= N.bulk_n / 4)
+        if (N.tail_ok && N.n_upper >= N.bulk_n) {
             rc = pic_sort_tail(ctx, m.sn);
-        else
+            if (!rc && (double)(N.n_upper - N.bulk_n) > JOIN_FRACTION * (double)N.bulk_n)
+                rc = pic_join_tail(ctx, m.sn);
+        } else {
             rc = pic_sort(ctx, m.sn);
+        }
     }
     if (!rc && !E.sorted) rc = pic_sort(ctx, m.se);
     if (rc) return rc;
@@ -598,11 +606,11 @@ extern "C" int pic_mcc_apply(pic_ctx* ctx, int id, double dt, uint32_t seed, uin
         PicProfScope ps(ctx, PIC_K_MCC);
         if (m.nsig <= SIG_LDS_MAX)
             k_mcc<true><<<grid_for((size_t)g.ncells, BLOCK), BLOCK, 0, ctx->stream>>>(
-                g, a, sn, N.off, two ? N.toff : nullptr, (uint32_t)N.bulk_n, si, se, E.off, m.sig, m.d_wsv,
+                g, a, sn, N.off, two ? N.toff : nullptr, N.tidx, si, se, E.off, m.sig, m.d_wsv,
                 m.d_part, m.d_spill);
         else
             k_mcc<false><<<grid_for((size_t)g.ncells, BLOCK), BLOCK, 0, ctx->stream>>>(
-                g, a, sn, N.off, two ? N.toff : nullptr, (uint32_t)N.bulk_n, si, se, E.off, m.sig, m.d_wsv,
+                g, a, sn, N.off, two ? N.toff : nullptr, N.tidx, si, se, E.off, m.sig, m.d_wsv,
                 m.d_part, m.d_spill);
         k_mcc_finish<<<1, 1, 0, ctx->stream>>>(m.d_wsv, m.d_part, m.d_stats, N.d_n, (uint32_t)N.cap,
                                                I.d_n, (uint32_t)I.cap, E.d_n, (uint32_t)E.cap);

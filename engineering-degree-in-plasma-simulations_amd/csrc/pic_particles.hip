@@ -901,18 +901,110 @@ __global__ void __launch_bounds__(BLOCK)
     }
 }
 
-// particles [start, start + cnt) of one buffer set to the other (all eight arrays)
-__global__ void __launch_bounds__(BLOCK) k_copy_part(SpeciesBuf src, SpeciesBuf dst, size_t start, size_t cnt) {
-    const size_t i = (size_t)blockIdx.x * BLOCK + threadIdx.x;
-    if (i >= cnt) return;
-    const size_t p = start + i;
-    dst.cell[p] = src.cell[p];
+// The appended part of a two-part species as an index: tidx[toff[cell] +
+// rank] = the particle's position, for [start, d_n[1]) (DEAD ones left out).
+__global__ void __launch_bounds__(BLOCK)
+    k_tail_index(GridC g, const uint32_t* __restrict__ cell, const uint32_t* __restrict__ rank,
+                 const uint32_t* __restrict__ toff, const uint32_t* __restrict__ d_n, uint32_t start,
+                 uint32_t* __restrict__ tidx) {
+    const size_t p = (size_t)start + (size_t)blockIdx.x * BLOCK + threadIdx.x;
+    if (p >= d_n[1]) return;
+    const uint32_t c = cell[p];
+    if (c == DEAD) return;
+    tidx[toff[cell_key(g, c)] + rank[p]] = (uint32_t)p;
+}
+
+// The index extended by the particles appended since it was built (their
+// per-cell offsets boff, ranks from k_hist): a cell's previous entries move
+// by boff[c] (toff_n[c] = toff[c] + boff[c]); the new particles follow them
+// (k_tail_index_more).  A workgroup takes BLOCK consecutive cells, whose
+// previous entries are one contiguous range: the threads stream it and find
+// each entry's cell by binary search over the cells' offsets in LDS.
+__global__ void __launch_bounds__(BLOCK)
+    k_tail_merge(const uint32_t* __restrict__ toff, const uint32_t* __restrict__ boff, int ncells,
+                 const uint32_t* __restrict__ tidx, uint32_t* __restrict__ toff_n, uint32_t* __restrict__ tidx_n) {
+    __shared__ uint32_t so[BLOCK + 1], sb[BLOCK];
+    const size_t c0 = (size_t)xcd_wg() * BLOCK, c = c0 + threadIdx.x;
+    if (c <= (size_t)ncells + 1) {
+        const uint32_t o = toff[c], b = boff[c];
+        toff_n[c] = o + b;
+        so[threadIdx.x] = o;
+        sb[threadIdx.x] = b;
+    }
+    if (c0 >= (size_t)ncells) return;  // (whole workgroup)
+    const int nc = (int)min((size_t)BLOCK, (size_t)ncells - c0);  // cells whose entries move
+    if (threadIdx.x == 0) so[nc] = toff[c0 + nc];
+    __syncthreads();
+    const uint32_t e1 = so[nc];
+    for (uint32_t j = so[0] + threadIdx.x; j < e1; j += BLOCK) {
+        int lo = 0, hi = nc - 1;  // the last cell k with so[k] <= j
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (so[mid] <= j)
+                lo = mid;
+            else
+                hi = mid - 1;
+        }
+        tidx_n[j + sb[lo]] = tidx[j];
+    }
+}
+__global__ void __launch_bounds__(BLOCK)
+    k_tail_index_more(GridC g, const uint32_t* __restrict__ cell, const uint32_t* __restrict__ rank,
+                      const uint32_t* __restrict__ toff, const uint32_t* __restrict__ boff,
+                      const uint32_t* __restrict__ d_n, uint32_t start, uint32_t* __restrict__ tidx_n) {
+    const size_t p = (size_t)start + (size_t)blockIdx.x * BLOCK + threadIdx.x;
+    if (p >= d_n[1]) return;
+    const uint32_t c = cell[p];
+    if (c == DEAD) return;
+    const uint32_t key = cell_key(g, c);
+    tidx_n[toff[key + 1] + boff[key] + rank[p]] = (uint32_t)p;
+}
+
+// The two parts joined into one cell-sorted array: a cell's particles of the
+// sorted part, then its appended ones in index order (the order of a cell's
+// pick pool).  The sorted part's particle i moves to i + toff[cell]; the
+// appended part's j-th indexed particle to off[cell + 1] + j.  Both reads
+// and writes stream (the appended part is gathered through the index).
+__device__ inline void copy_particle(const SpeciesBuf& src, size_t p, const SpeciesBuf& dst, size_t q,
+                                     uint32_t c) {
+    dst.cell[q] = c;
 #pragma unroll
     for (int d = 0; d < 3; d++) {
-        dst.f[d][p] = src.f[d][p];
-        dst.v[d][p] = src.v[d][p];
+        dst.f[d][q] = src.f[d][p];
+        dst.v[d][q] = src.v[d][p];
     }
-    dst.w[p] = src.w[p];
+    dst.w[q] = src.w[p];
+}
+__global__ void __launch_bounds__(BLOCK)
+    k_join_sorted(GridC g, SpeciesBuf src, SpeciesBuf dst, const uint32_t* __restrict__ toff, uint32_t nb) {
+    const size_t i = (size_t)xcd_wg() * BLOCK + threadIdx.x;
+    if (i >= nb) return;
+    const uint32_t c = src.cell[i];
+    if (c == DEAD) return;  // none in a sorted part
+    copy_particle(src, i, dst, i + toff[cell_key(g, c)], c);
+}
+__global__ void __launch_bounds__(BLOCK)
+    k_join_tail(GridC g, SpeciesBuf src, SpeciesBuf dst, const uint32_t* __restrict__ off,
+                const uint32_t* __restrict__ toff, const uint32_t* __restrict__ tidx) {
+    const size_t j = (size_t)xcd_wg() * BLOCK + threadIdx.x;
+    if (j >= toff[g.ncells]) return;
+    const uint32_t p = tidx[j];
+    const uint32_t c = src.cell[p];
+    copy_particle(src, p, dst, off[cell_key(g, c) + 1] + j, c);
+}
+// the joined offsets (off[c] + toff[c]) and live count
+__global__ void __launch_bounds__(BLOCK)
+    k_join_off(uint32_t* __restrict__ off, const uint32_t* __restrict__ toff, int ncells,
+               uint32_t* __restrict__ d_n) {
+    const size_t c = (size_t)blockIdx.x * BLOCK + threadIdx.x;
+    if (c > (size_t)ncells) return;
+    const uint32_t v = off[c] + toff[c];
+    off[c] = v;
+    if (c == (size_t)ncells) {
+        off[ncells + 1] = v;
+        d_n[0] = v;
+        d_n[3] = v;
+    }
 }
 
 struct RhoArgs {
@@ -1234,6 +1326,7 @@ static int after_emission(pic_ctx* ctx, const Species& s, const PushArgs& a) {
     PIC_HIP(ctx, hipMemcpyAsync(hn, t.d_n, sizeof(hn), hipMemcpyDeviceToHost, ctx->stream));
     PIC_HIP(ctx, hipStreamSynchronize(ctx->stream));
     if (hn[0] > t.cap) hn[0] = (uint32_t)t.cap;
+    t.bulk_n = hn[3];  // the emissions were appended behind the last whole sort's particles
     if ((size_t)hn[0] != t.n_upper) {
         t.n_upper = hn[0];
         t.sorted = false;
@@ -1347,20 +1440,28 @@ extern "C" int pic_sort(pic_ctx* ctx, int sid) {
     return PIC_OK;
 }
 
-// Sort only the particles appended since the last whole-species sort
-// (indices [bulk_n, n_upper), both exact: DSMC births, nothing removed), by
-// cell into toff, in place: the species is then a sorted part with offsets
-// off followed by a sorted appended part with offsets toff.
+// Index the particles appended since the last whole-species sort (indices
+// [bulk_n, n_upper), both exact: DSMC births, nothing removed) by cell: toff
+// holds the appended part's per-cell offsets, tidx their positions in cell
+// order.  The particles do not move; the species is then a sorted part with
+// offsets off and an indexed appended part (a cell's pick pool: its sorted
+// range, then its index entries).  An index that covers [bulk_n, t_end)
+// takes only [t_end, n_upper) in; the order inside a cell is the index's
+// (earlier appends first, then atomic order).
 int pic_sort_tail(pic_ctx* ctx, int sid) {
     Species& s = ctx->sp[sid];
     const GridC& g = ctx->g;
-    const size_t start = s.bulk_n, n = s.n_upper;
-    if (n < start) return pic_fail(ctx, PIC_ERR_STATE, "appended part before the sorted part");
-    if (n == start) {  // nothing appended (e.g. only ionizing collisions): the sorted part alone
-        s.two_part = false;
+    const size_t n = s.n_upper;
+    if (n < s.bulk_n) return pic_fail(ctx, PIC_ERR_STATE, "appended part before the sorted part");
+    const bool more = s.two_part && s.t_end >= s.bulk_n && s.t_end <= n;
+    const size_t start = more ? s.t_end : s.bulk_n;
+    if (n == start) {  // nothing appended since (e.g. only ionizing collisions)
+        if (!more) s.two_part = false;  // the sorted part alone
         return PIC_OK;
     }
-    if (!s.toff) PIC_HIP(ctx, hipMalloc(&s.toff, ((size_t)g.ncells + 2) * 4));
+    const size_t words = ((size_t)g.ncells + 2) * 4, iw = (s.cap + 63) / 64 * 64 * 4;
+    if (!s.toff) PIC_HIP(ctx, hipMalloc(&s.toff, words));
+    if (!s.tidx) PIC_HIP(ctx, hipMalloc(&s.tidx, iw));
     PIC_HIP(ctx, hipMemsetAsync(s.count, 0, ((size_t)g.ncells + 1) * 4, ctx->stream));
     const size_t cnt = n - start;
     {
@@ -1368,21 +1469,55 @@ int pic_sort_tail(pic_ctx* ctx, int sid) {
         k_hist<<<xcd_blocks1(cnt, BLOCK), BLOCK, 0, ctx->stream>>>(g, s.buf[s.cur], s.d_n, s.count,
                                                                           s.rank, (uint32_t)start);
     }
-    scan_counts(ctx, s, s.toff, false);
-    {
+    if (!more) {
+        scan_counts(ctx, s, s.toff, false);
         PicProfScope ps(ctx, PIC_K_SCATTER);
-        const size_t span = n - (start & ~(size_t)3);  // from the aligned group holding start
-        k_scatter<<<xcd_blocks1((span + SC_V - 1) / SC_V, BLOCK), BLOCK, 0, ctx->stream>>>(
-            g, s.buf[s.cur], s.buf[s.cur ^ 1], s.rank, s.toff, s.d_n, (uint32_t)s.cap, (uint32_t)start);
+        k_tail_index<<<grid_for(cnt, BLOCK), BLOCK, 0, ctx->stream>>>(g, s.buf[s.cur].cell, s.rank, s.toff,
+                                                                     s.d_n, (uint32_t)start, s.tidx);
+    } else {
+        if (!s.boff) PIC_HIP(ctx, hipMalloc(&s.boff, words));
+        if (!s.toff_n) PIC_HIP(ctx, hipMalloc(&s.toff_n, words));
+        if (!s.tidx_n) PIC_HIP(ctx, hipMalloc(&s.tidx_n, iw));
+        scan_counts(ctx, s, s.boff, false);
+        PicProfScope ps(ctx, PIC_K_SCATTER);
+        k_tail_merge<<<grid_for((size_t)g.ncells + 2, BLOCK), BLOCK, 0, ctx->stream>>>(
+            s.toff, s.boff, g.ncells, s.tidx, s.toff_n, s.tidx_n);
+        k_tail_index_more<<<grid_for(cnt, BLOCK), BLOCK, 0, ctx->stream>>>(
+            g, s.buf[s.cur].cell, s.rank, s.toff, s.boff, s.d_n, (uint32_t)start, s.tidx_n);
+        std::swap(s.toff, s.toff_n);
+        std::swap(s.tidx, s.tidx_n);
     }
     PIC_HIP(ctx, hipGetLastError());
-    {  // the sorted appended part back behind the sorted part
-        PicProfScope ps(ctx, PIC_K_SCATTER);
-        k_copy_part<<<grid_for(cnt, BLOCK), BLOCK, 0, ctx->stream>>>(s.buf[s.cur ^ 1], s.buf[s.cur], start, cnt);
-        PIC_HIP(ctx, hipGetLastError());
-    }
     s.two_part = true;
+    s.t_end = n;
     s.hist_valid = false;
+    return PIC_OK;
+}
+
+// A two-part species (pic_sort_tail just run) sorted whole by joining its
+// parts: both stream once, where a counting sort scatters the appended part
+// over the whole array.
+int pic_join_tail(pic_ctx* ctx, int sid) {
+    Species& s = ctx->sp[sid];
+    const GridC& g = ctx->g;
+    if (!s.two_part) return pic_fail(ctx, PIC_ERR_STATE, "no indexed appended part to join");
+    const size_t nb = s.bulk_n, nt = s.n_upper - s.bulk_n;
+    {
+        PicProfScope ps(ctx, PIC_K_SCATTER);
+        if (nb)
+            k_join_sorted<<<xcd_blocks1(nb, BLOCK), BLOCK, 0, ctx->stream>>>(g, s.buf[s.cur], s.buf[s.cur ^ 1],
+                                                                             s.toff, (uint32_t)nb);
+        k_join_tail<<<xcd_blocks1(nt ? nt : 1, BLOCK), BLOCK, 0, ctx->stream>>>(
+            g, s.buf[s.cur], s.buf[s.cur ^ 1], s.off, s.toff, s.tidx);
+        k_join_off<<<grid_for((size_t)g.ncells + 1, BLOCK), BLOCK, 0, ctx->stream>>>(s.off, s.toff, g.ncells,
+                                                                                    s.d_n);
+    }
+    PIC_HIP(ctx, hipGetLastError());
+    s.cur ^= 1;
+    s.sorted = true;
+    s.hist_valid = false;
+    s.two_part = false;
+    s.tail_ok = true;
     return PIC_OK;
 }
 

@@ -87,6 +87,75 @@ def test_electron_heavier_branch_matches_oracle(pg, sigma_table_path):
         wsv = wsv_o
 
 
+def test_appended_neutrals_indexed_then_joined_match_oracle(pg, sigma_table_path):
+    """Neutrals ten times heavier than the electrons: collisions split them
+    (Interactions.cpp:684-703) and the split neutrals are appended.  The
+    neutrals are sorted once; every later step indexes the appended part by
+    cell (a cell's pick pool lists it in array order) and joins it with the
+    sorted part once it outgrows a fraction of it.  Each step is compared
+    with the oracle under the same draws on the downloaded arrays, whose
+    scan order per cell is the pool order."""
+    n, dx = 9, 1e-4
+    L = dx * (n - 1)
+    c = pg.Context(n, n, n, [0, 0, 0], [L, L, L])
+    g = O.make_grid(n, n, n, [0, 0, 0], [L, L, L])
+    rng = np.random.default_rng(12)
+
+    def parts(count, vth, w):
+        p = np.zeros((count, 7))
+        p[:, :3] = rng.uniform(0.01 * L, 0.99 * L, (count, 3))
+        p[:, 3:6] = rng.normal(0, vth, (count, 3)).astype(np.float32)
+        p[:, 6] = w
+        return p.reshape(-1)
+
+    n0 = 30 * 512
+    # (the species' mpw0 as MC_MEX_Ionization requires, Interactions.cpp:479-489; the
+    # particles' own weights are uploaded)
+    sn = c.species_add(16 * O.AMU, 0.0, 100.0, 200000)
+    si = c.species_add(16 * O.AMU, O.QE, 1.0, 200000)
+    se = c.species_add(O.ME, -O.QE, 1.0, 200000)
+    c.upload(sn, parts(n0, 300.0, 10.0))
+    c.upload(se, parts(10 * 512, 3e6, 1.0))
+    sE, sV = O.load_sigma_table(sigma_table_path)
+    E_ion = 1313.9 * 1000 / 6.02214076e23
+    dt, wsv = 2e-2, 1.5e-12
+    mid = c.mcc_create(sn, si, se, E_ion, sE, sV, wsv_max_init=wsv)
+    prm = O.mcc_params(16 * O.AMU, O.ME, E_ion, sE, sV, dt, g.dx[0] * g.dx[1] * g.dx[2],
+                       pool_append=True)
+
+    def sorted_prefix(p):
+        q = p.reshape(-1, 7)
+        ijk = np.clip(np.floor(q[:, :3] / dx + 1e-8).astype(np.int64), 0, n - 2)
+        key = (ijk[:, 2] * (n - 1) + ijk[:, 1]) * (n - 1) + ijk[:, 0]
+        drop = np.nonzero(np.diff(key) < 0)[0]
+        return len(key) if len(drop) == 0 else int(drop[0]) + 1
+
+    c.sort(sn)
+    prefixes, counts = [], []
+    for step in range(6):
+        c.sort(se)  # the electrons' pool in array order too
+        neu, ele, ion = c.download(sn), c.download(se), c.download(si)
+        prefixes.append(sorted_prefix(neu))
+        counts.append(neu.size // 7)
+        st = c.mcc_apply(mid, dt, 17, step)
+        on, oi, oe, wsv_o, ost = O.mcc_apply(g, prm, wsv, O.rng_philox(17, step, TAG_MCC), neu,
+                                             ion, ele)
+        print("step", step, "neutrals", counts[-1], "sorted prefix", prefixes[-1], st,
+              ost.collisions, flush=True)
+        assert st["groups"] == ost.groups
+        assert st["collisions"] == ost.collisions > 1000
+        assert st["ionizations"] == ost.ionizations
+        assert st["pool_overflow"] == 0
+        assert st["wsv_max"] == pytest.approx(wsv_o, rel=1e-6)
+        for sid, ref in ((sn, on), (si, oi), (se, oe)):
+            _same_multiset(ref, c.download(sid), L)
+        wsv = wsv_o
+    # an appended part was indexed (the arrays unsorted past the sorted part)
+    # and later joined (the sorted part grew past the first sort's)
+    assert any(p < cnt for p, cnt in zip(prefixes[1:], counts[1:]))
+    assert max(prefixes) > 1.2 * n0, prefixes
+
+
 def _ensemble_stats(ele, ion_iz, coll):
     e = ele.reshape(-1, 7)
     ke = np.sum(e[:, 6] * np.sum(e[:, 3:6] ** 2, axis=1)) / np.sum(e[:, 6])
