@@ -185,6 +185,7 @@ extern "C" void pic_destroy(pic_ctx* ctx) {
     (void)hipFree(ctx->ntype);
     (void)hipFree(ctx->scratch);
     (void)hipFree(ctx->scan_tmp);
+    (void)hipFree(ctx->part);
     (void)hipFree(ctx->work);
     (void)hipFree(ctx->dep_corner);
     if (ctx->pin) (void)hipHostFree(ctx->pin);
@@ -521,6 +522,7 @@ extern "C" int pic_particles_upload(pic_ctx* ctx, int sid, const double* aos7, s
     s.sorted = false;
     s.two_part = false;
     s.tail_ok = false;
+    s.scrambled = true;  // the caller's order
     return PIC_OK;
 }
 

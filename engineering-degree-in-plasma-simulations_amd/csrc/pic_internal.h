@@ -167,6 +167,9 @@ struct Species {
     uint32_t* toff = nullptr;
     uint32_t* tidx = nullptr;
     uint32_t *boff = nullptr, *toff_n = nullptr, *tidx_n = nullptr;
+    // particles in random order (a load or an upload): the next sort
+    // partitions them by coarse bucket first
+    bool scrambled = false;
     double* den = nullptr;       // number density
     double* mom = nullptr;       // n, nvx, nvy, nvz, nuu, nvv, nww (7 * nn)
     uint32_t seq = 0;            // loader stream id
@@ -250,6 +253,8 @@ struct pic_ctx {
     uint8_t* ntype = nullptr;
     double* scratch = nullptr;   // reductions and solver scalars
     uint32_t* scan_tmp = nullptr; // block sums of the offset scan
+    uint32_t* part = nullptr;     // partition counts, offsets and scan sums of a scrambled sort
+    size_t part_words = 0;
     double* work = nullptr;      // solver vectors (allocated on demand)
     size_t work_bytes = 0;
     std::vector<pic::Species> sp;

@@ -647,6 +647,345 @@ __global__ void __launch_bounds__(BLOCK)
     }
 }
 
+// ---- sort of scrambled input (after a load or an upload) ----
+// A counting sort writes each particle to off[cell] + rank: on input in
+// random order every histogram atomic returns from a different line and
+// every store lands on its own cache line, eight arrays per particle.
+// Scrambled input is therefore sorted as an MSD radix sort on the cell key:
+//   * level 1 over the whole array, up to P1_B buckets: per tile of PART_TILE particles a bucket
+//     histogram (k_part_count), one scan over the bucket-major (bucket, tile)
+//     counts, and k_part_move, which orders each tile by bucket in LDS and
+//     writes every bucket's share as one contiguous run;
+//   * further levels inside each bucket of the previous one, splitting it
+//     into up to PART_B buckets (tiles never straddle a bucket: k_seg_tiles,
+//     k_seg_count, k_seg_scan, k_seg_move), until a bucket spans at most
+//     FINE_CELLS keys;
+//   * k_fine_sort: one workgroup per final bucket ranks its particles by
+//     cell with LDS atomics, writes the cells' offsets and moves the
+//     particles through LDS with coalesced loads and stores (a bucket larger
+//     than LDS takes global atomics and a scatter confined to its range).
+// Every pass streams: a tile's run per bucket is PART_TILE / P1_B or more
+// particles long.  The order inside a cell is that of the LDS atomics.
+constexpr int PART_TILE = 4096, P1_B = 128, PART_B = 64, PART_BLOCK = 256;
+constexpr int FINE_SHIFT = 9, FINE_CELLS = 1 << FINE_SHIFT;  // keys of a final bucket
+constexpr int FINE_CAP = 12288;  // particles a final bucket sorts in LDS
+
+__device__ inline uint32_t part_bucket(const GridC& g, uint32_t c, int shift) { return cell_key(g, c) >> shift; }
+
+// One tile [base, base + cnt) ordered by bucket ((key >> shift) - bsub, in
+// [0, nbk)) and written out: bucket b's run starts at start[b * stride].
+__device__ inline void part_tile(const GridC& g, const SpeciesBuf& src, const SpeciesBuf& dst, size_t base,
+                                 int cnt, int shift, uint32_t bsub, int nbk, const uint32_t* __restrict__ start,
+                                 size_t stride) {
+    __shared__ uint32_t lstart[P1_B];  // bucket counts, then the tile's local bucket starts
+    __shared__ uint32_t lbase[P1_B];   // global position of the bucket's run minus its local start
+    __shared__ uint16_t lpos[PART_TILE];   // tile particle q -> its position in the tile's bucket order
+    __shared__ uint32_t ldest[PART_TILE];  // bucket-order position r -> global position
+    __shared__ uint32_t stage[PART_TILE];  // one array at a time, in bucket order
+    uint16_t* lkey = reinterpret_cast<uint16_t*>(stage);  // (set-up only, aliasing stage)
+    uint16_t* lslot = lkey + PART_TILE;
+    for (int b = threadIdx.x; b < nbk; b += PART_BLOCK) lstart[b] = 0;
+    __syncthreads();
+    for (int q = threadIdx.x; q < cnt; q += PART_BLOCK) {
+        const uint32_t k = part_bucket(g, src.cell[base + q], shift) - bsub;
+        lkey[q] = (uint16_t)k;
+        lslot[q] = (uint16_t)atomicAdd(&lstart[k], 1u);
+    }
+    __syncthreads();
+    {
+        const uint32_t v = threadIdx.x < nbk ? lstart[threadIdx.x] : 0u;
+        const uint32_t ex = block_exclusive_scan(v, nullptr);
+        if (threadIdx.x < nbk) {
+            lstart[threadIdx.x] = ex;
+            lbase[threadIdx.x] = start[(size_t)threadIdx.x * stride] - ex;
+        }
+    }
+    __syncthreads();
+    for (int q = threadIdx.x; q < cnt; q += PART_BLOCK) {
+        const uint32_t k = lkey[q], r = lstart[k] + lslot[q];
+        lpos[q] = (uint16_t)r;
+        ldest[r] = lbase[k] + r;
+    }
+    __syncthreads();
+    const uint32_t* in[8] = {src.cell,
+                             reinterpret_cast<const uint32_t*>(src.f[0]),
+                             reinterpret_cast<const uint32_t*>(src.f[1]),
+                             reinterpret_cast<const uint32_t*>(src.f[2]),
+                             reinterpret_cast<const uint32_t*>(src.v[0]),
+                             reinterpret_cast<const uint32_t*>(src.v[1]),
+                             reinterpret_cast<const uint32_t*>(src.v[2]),
+                             reinterpret_cast<const uint32_t*>(src.w)};
+    uint32_t* out[8] = {dst.cell,
+                        reinterpret_cast<uint32_t*>(dst.f[0]),
+                        reinterpret_cast<uint32_t*>(dst.f[1]),
+                        reinterpret_cast<uint32_t*>(dst.f[2]),
+                        reinterpret_cast<uint32_t*>(dst.v[0]),
+                        reinterpret_cast<uint32_t*>(dst.v[1]),
+                        reinterpret_cast<uint32_t*>(dst.v[2]),
+                        reinterpret_cast<uint32_t*>(dst.w)};
+    // each thread's PER slots (q = r = threadIdx.x + e * PART_BLOCK) in
+    // registers; every array's loads issued before their LDS stores
+    constexpr int PER = PART_TILE / PART_BLOCK;
+    uint32_t mypos[PER], mydest[PER];
+#pragma unroll
+    for (int e = 0; e < PER; e++) {
+        const int q = threadIdx.x + e * PART_BLOCK;
+        mypos[e] = q < cnt ? lpos[q] : 0u;
+        mydest[e] = q < cnt ? ldest[q] : 0u;
+    }
+    // (the next array's loads are issued before this array's stores)
+    uint32_t v[PER];
+#pragma unroll
+    for (int e = 0; e < PER; e++) {
+        const int q = threadIdx.x + e * PART_BLOCK;
+        v[e] = q < cnt ? in[0][base + q] : 0u;
+    }
+#pragma unroll 1
+    for (int a = 0; a < 8; a++) {
+#pragma unroll
+        for (int e = 0; e < PER; e++)
+            if (threadIdx.x + e * PART_BLOCK < cnt) stage[mypos[e]] = v[e];
+        __syncthreads();
+        if (a < 7) {
+#pragma unroll
+            for (int e = 0; e < PER; e++) {
+                const int q = threadIdx.x + e * PART_BLOCK;
+                v[e] = q < cnt ? in[a + 1][base + q] : 0u;
+            }
+        }
+#pragma unroll
+        for (int e = 0; e < PER; e++) {
+            const int r = threadIdx.x + e * PART_BLOCK;
+            if (r < cnt) out[a][mydest[e]] = stage[r];
+        }
+        __syncthreads();
+    }
+}
+
+// level 1: tiles of the whole array [0, d_n[0])
+__global__ void __launch_bounds__(PART_BLOCK)
+    k_part_count(GridC g, const uint32_t* __restrict__ cell, const uint32_t* __restrict__ d_n, int shift,
+                 int nbk, uint32_t ntile, uint32_t* __restrict__ pcount) {
+    __shared__ uint32_t h[P1_B];
+    for (int b = threadIdx.x; b < nbk; b += PART_BLOCK) h[b] = 0;
+    __syncthreads();
+    const size_t n = d_n[0], base = (size_t)blockIdx.x * PART_TILE;
+    for (int q = threadIdx.x; q < PART_TILE; q += PART_BLOCK)
+        if (base + q < n) atomicAdd(&h[part_bucket(g, cell[base + q], shift)], 1u);
+    __syncthreads();
+    for (int b = threadIdx.x; b < nbk; b += PART_BLOCK) pcount[(size_t)b * ntile + blockIdx.x] = h[b];
+}
+__global__ void __launch_bounds__(PART_BLOCK)
+    k_part_move(GridC g, SpeciesBuf src, SpeciesBuf dst, const uint32_t* __restrict__ d_n, int shift, int nbk,
+                uint32_t ntile, const uint32_t* __restrict__ poff) {
+    const size_t n = d_n[0], base = (size_t)blockIdx.x * PART_TILE;
+    const int cnt = base < n ? (int)min((size_t)PART_TILE, n - base) : 0;
+    part_tile(g, src, dst, base, cnt, shift, 0u, nbk, poff + blockIdx.x, ntile);
+}
+// level-1 bucket bounds: bound[b] = poff[b * ntile] (b <= nbk)
+__global__ void k_part_bounds(const uint32_t* __restrict__ poff, int nbk, uint32_t ntile, uint32_t* __restrict__ bound) {
+    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b <= nbk) bound[b] = poff[(size_t)b * ntile];
+}
+
+// further levels: segment s = [bound[s], bound[s + 1]) holds keys with
+// key >> (shift + step) == s, split into 1 << step buckets (step <= 6);
+// its tiles are numbered from tbase[s]
+__global__ void __launch_bounds__(PART_BLOCK)
+    k_seg_tiles(const uint32_t* __restrict__ bound, int nseg, uint32_t* __restrict__ tbase) {
+    constexpr int PER = 64;  // segments per thread (nseg <= PER * PART_BLOCK)
+    uint32_t v[PER], sum = 0;
+    for (int e = 0; e < PER; e++) {
+        const int s = threadIdx.x * PER + e;
+        v[e] = s < nseg ? (bound[s + 1] - bound[s] + PART_TILE - 1) / PART_TILE : 0u;
+        sum += v[e];
+    }
+    uint32_t total;
+    uint32_t ex = block_exclusive_scan(sum, &total);
+    for (int e = 0; e < PER; e++) {
+        const int s = threadIdx.x * PER + e;
+        if (s < nseg) tbase[s] = ex;
+        ex += v[e];
+    }
+    if (threadIdx.x == 0) tbase[nseg] = total;
+}
+__device__ inline bool seg_tile(const uint32_t* __restrict__ bound, const uint32_t* __restrict__ tbase, int nseg,
+                                uint32_t u, int& s, size_t& base, int& cnt) {
+    if (u >= tbase[nseg]) return false;
+    int lo = 0, hi = nseg - 1;  // the last segment with tbase[s] <= u
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (tbase[mid] <= u)
+            lo = mid;
+        else
+            hi = mid - 1;
+    }
+    s = lo;
+    base = (size_t)bound[s] + (size_t)(u - tbase[s]) * PART_TILE;
+    cnt = (int)min((size_t)PART_TILE, (size_t)bound[s + 1] - base);
+    return true;
+}
+__global__ void __launch_bounds__(PART_BLOCK)
+    k_seg_count(GridC g, const uint32_t* __restrict__ cell, const uint32_t* __restrict__ bound,
+                const uint32_t* __restrict__ tbase, int nseg, int shift, int step, uint32_t* __restrict__ cnt2) {
+    __shared__ uint32_t h[PART_B];
+    int s, cnt;
+    size_t base;
+    if (!seg_tile(bound, tbase, nseg, blockIdx.x, s, base, cnt)) return;  // (whole workgroup)
+    for (int b = threadIdx.x; b < PART_B; b += PART_BLOCK) h[b] = 0;
+    __syncthreads();
+    const uint32_t bsub = (uint32_t)s << step;
+    for (int q = threadIdx.x; q < cnt; q += PART_BLOCK) atomicAdd(&h[part_bucket(g, cell[base + q], shift) - bsub], 1u);
+    __syncthreads();
+    for (int b = threadIdx.x; b < PART_B; b += PART_BLOCK) cnt2[(size_t)blockIdx.x * PART_B + b] = h[b];
+}
+// one workgroup of PART_B threads per segment: thread b scans bucket b's
+// counts over the segment's tiles (into run starts) and the buckets' totals
+// give the next level's bounds
+__global__ void __launch_bounds__(PART_B)
+    k_seg_scan(const uint32_t* __restrict__ bound, const uint32_t* __restrict__ tbase, int nseg, int step,
+               uint32_t* __restrict__ cnt2, uint32_t* __restrict__ nbound) {
+    const int s = blockIdx.x, b = threadIdx.x, nsub = 1 << step;
+    const uint32_t u0 = tbase[s], u1 = b < nsub ? tbase[s + 1] : tbase[s];  // lanes past nsub: none
+    uint32_t run = 0;
+    for (uint32_t u = u0; u < u1; u++) {
+        const uint32_t c = cnt2[(size_t)u * PART_B + b];
+        cnt2[(size_t)u * PART_B + b] = run;
+        run += c;
+    }
+    uint32_t incl = run;  // wave-wide inclusive scan of the totals (PART_B = 64 lanes)
+    for (int o = 1; o < PART_B; o <<= 1) {
+        const uint32_t t = __shfl_up(incl, o, PART_B);
+        if (b >= o) incl += t;
+    }
+    const uint32_t bst = bound[s] + incl - run;
+    for (uint32_t u = u0; u < u1; u++) cnt2[(size_t)u * PART_B + b] += bst;
+    if (b < nsub) nbound[((size_t)s << step) + b] = bst;
+    if (s == nseg - 1 && b == nsub - 1) nbound[(size_t)nseg << step] = bst + run;
+}
+__global__ void __launch_bounds__(PART_BLOCK)
+    k_seg_move(GridC g, SpeciesBuf src, SpeciesBuf dst, const uint32_t* __restrict__ bound,
+               const uint32_t* __restrict__ tbase, int nseg, int shift, int step, const uint32_t* __restrict__ cnt2) {
+    int s, cnt;
+    size_t base;
+    if (!seg_tile(bound, tbase, nseg, blockIdx.x, s, base, cnt)) return;  // (whole workgroup)
+    part_tile(g, src, dst, base, cnt, shift, (uint32_t)s << step, 1 << step, cnt2 + (size_t)blockIdx.x * PART_B, 1);
+}
+
+// final buckets: bucket f = [bound[f], bound[f + 1]) holds the keys
+// [f << shift, (f + 1) << shift), at most FINE_CELLS of them.  Writes off[]
+// for those cells (and the live count with the removed particles' key).
+__global__ void __launch_bounds__(PART_BLOCK)
+    k_fine_sort(GridC g, SpeciesBuf src, SpeciesBuf dst, const uint32_t* __restrict__ bound, int shift,
+                uint32_t* __restrict__ off, uint32_t* __restrict__ gcount, uint32_t* __restrict__ grank,
+                uint32_t* __restrict__ d_n) {
+    __shared__ uint32_t lcnt[FINE_CELLS];
+    __shared__ uint16_t lpos[FINE_CAP];
+    __shared__ uint32_t stage[FINE_CAP];
+    const uint32_t f = blockIdx.x, fs = bound[f], fe = bound[f + 1];
+    const uint32_t k0 = f << shift;
+    const uint32_t k1 = min((f + 1) << shift, (uint32_t)g.ncells + 1);  // keys of this bucket
+    if (k0 >= k1) return;  // (whole workgroup)
+    const int ncl = (int)(k1 - k0), size = (int)(fe - fs);
+    const bool lds = size <= FINE_CAP;
+    for (int k = threadIdx.x; k < ncl; k += PART_BLOCK) lcnt[k] = 0;
+    __syncthreads();
+    if (lds) {
+        for (int q = threadIdx.x; q < size; q += PART_BLOCK) {
+            const uint32_t k = cell_key(g, src.cell[fs + q]) - k0;
+            lpos[q] = (uint16_t)atomicAdd(&lcnt[k], 1u);
+        }
+    } else {  // ranks from global atomics (gcount zeroed), kept in grank
+        for (int q = threadIdx.x; q < size; q += PART_BLOCK) {
+            const uint32_t k = cell_key(g, src.cell[fs + q]) - k0;
+            grank[fs + q] = atomicAdd(&gcount[k0 + k], 1u);
+            atomicAdd(&lcnt[k], 1u);
+        }
+    }
+    __syncthreads();
+    {  // exclusive scan of the keys' counts, FINE_CELLS / PART_BLOCK per thread
+        constexpr int PER = FINE_CELLS / PART_BLOCK;
+        uint32_t v[PER], sum = 0;
+#pragma unroll
+        for (int e = 0; e < PER; e++) {
+            const int k = threadIdx.x * PER + e;
+            v[e] = k < ncl ? lcnt[k] : 0u;
+            sum += v[e];
+        }
+        uint32_t ex = block_exclusive_scan(sum, nullptr);
+#pragma unroll
+        for (int e = 0; e < PER; e++) {
+            const int k = threadIdx.x * PER + e;
+            if (k < ncl) {
+                lcnt[k] = ex;
+                const uint32_t key = k0 + (uint32_t)k;
+                off[key] = fs + ex;
+                if (key == (uint32_t)g.ncells) {  // the removed particles' bucket: the live count
+                    off[key + 1] = fe;
+                    d_n[0] = fs + ex;
+                    d_n[3] = fs + ex;
+                }
+            }
+            ex += v[e];
+        }
+    }
+    __syncthreads();
+    const uint32_t* in[8] = {src.cell,
+                             reinterpret_cast<const uint32_t*>(src.f[0]),
+                             reinterpret_cast<const uint32_t*>(src.f[1]),
+                             reinterpret_cast<const uint32_t*>(src.f[2]),
+                             reinterpret_cast<const uint32_t*>(src.v[0]),
+                             reinterpret_cast<const uint32_t*>(src.v[1]),
+                             reinterpret_cast<const uint32_t*>(src.v[2]),
+                             reinterpret_cast<const uint32_t*>(src.w)};
+    uint32_t* out[8] = {dst.cell,
+                        reinterpret_cast<uint32_t*>(dst.f[0]),
+                        reinterpret_cast<uint32_t*>(dst.f[1]),
+                        reinterpret_cast<uint32_t*>(dst.f[2]),
+                        reinterpret_cast<uint32_t*>(dst.v[0]),
+                        reinterpret_cast<uint32_t*>(dst.v[1]),
+                        reinterpret_cast<uint32_t*>(dst.v[2]),
+                        reinterpret_cast<uint32_t*>(dst.w)};
+    if (lds) {
+        // each thread's PER slots (q = threadIdx.x + e * PART_BLOCK): final
+        // positions in registers, every array's loads issued before their
+        // LDS stores
+        constexpr int PER = FINE_CAP / PART_BLOCK;
+        uint32_t mypos[PER];
+#pragma unroll
+        for (int e = 0; e < PER; e++) {
+            const int q = threadIdx.x + e * PART_BLOCK;
+            mypos[e] = q < size ? lcnt[cell_key(g, src.cell[fs + q]) - k0] + lpos[q] : 0u;
+        }
+#pragma unroll 1
+        for (int a = 0; a < 8; a++) {
+            uint32_t v[PER];
+#pragma unroll
+            for (int e = 0; e < PER; e++) {
+                const int q = threadIdx.x + e * PART_BLOCK;
+                v[e] = q < size ? in[a][fs + q] : 0u;
+            }
+#pragma unroll
+            for (int e = 0; e < PER; e++)
+                if (threadIdx.x + e * PART_BLOCK < size) stage[mypos[e]] = v[e];
+            __syncthreads();
+#pragma unroll
+            for (int e = 0; e < PER; e++) {
+                const int r = threadIdx.x + e * PART_BLOCK;
+                if (r < size) out[a][fs + r] = stage[r];
+            }
+            __syncthreads();
+        }
+    } else {
+        for (int q = threadIdx.x; q < size; q += PART_BLOCK) {
+            const uint32_t k = cell_key(g, src.cell[fs + q]) - k0;
+            const uint32_t d = fs + lcnt[k] + grank[fs + q];
+#pragma unroll
+            for (int a = 0; a < 8; a++) out[a][d] = in[a][fs + q];
+        }
+    }
+}
+
 // Second pass of pic_push_sort: every particle is pushed again from the
 // unchanged source arrays (the same arithmetic as the first pass, so the
 // same new cell) and written straight to its sorted position off[new cell] +
@@ -1415,24 +1754,125 @@ static void scan_counts(pic_ctx* ctx, Species& s, uint32_t* out, bool whole) {
                                                whole ? s.d_n : nullptr);
 }
 
+// pic_sort for scrambled input: the MSD passes above, ping-ponging between
+// the two buffer sets; the sorted particles, offsets and live count as a
+// counting sort leaves them (no ranks: the histogram is not valid after).
+constexpr size_t PART_MIN = 1u << 20;  // particles; fewer fit in cache anyway
+static int sort_partitioned(pic_ctx* ctx, Species& s) {
+    const GridC& g = ctx->g;
+    const size_t n = s.n_upper;
+    const uint32_t kmax = (uint32_t)g.ncells;  // the removed particles' key
+    int shift = 0;
+    while ((kmax >> shift) + 1 > (uint32_t)P1_B) shift++;
+    const int nbk = (int)(kmax >> shift) + 1;
+    const size_t ntile = (n + PART_TILE - 1) / PART_TILE, M = (size_t)nbk * ntile;
+    // segmented levels until a bucket spans at most FINE_CELLS keys and, at
+    // the mean density, holds two thirds of FINE_CAP particles; each level
+    // splits a bucket into up to PART_B (steps of at most 6 bits)
+    int fine = FINE_SHIFT;
+    while (fine > 0 && (double)n * (double)(1u << fine) > (2.0 / 3.0) * FINE_CAP * ((double)kmax + 1)) fine--;
+    int steps[8], levels = 0;
+    size_t nseg_max = nbk;
+    for (int sh = shift; sh > fine;) {
+        const int st = std::min(6, sh - fine);
+        steps[levels++] = st;
+        nseg_max <<= st;
+        sh -= st;
+    }
+    const size_t nbound = nseg_max + 1;
+    const size_t ntmax = ntile + nseg_max;  // tiles of any level
+    const size_t nsum = M / SCAN_TILE + 2;
+    const size_t words = M + (M + 1) + nsum + 2 * nbound + (nseg_max + 1) + ntmax * PART_B;
+    if (M >= (1ull << 31) || ntmax * PART_B >= (1ull << 32))
+        return pic_fail(ctx, PIC_ERR_CAPACITY, "too many particles to partition");
+    if (ctx->part_words < words) {
+        if (ctx->part) PIC_HIP(ctx, hipFree(ctx->part));
+        ctx->part = nullptr;
+        ctx->part_words = 0;
+        PIC_HIP(ctx, hipMalloc(&ctx->part, words * 4));
+        ctx->part_words = words;
+    }
+    uint32_t* pcount = ctx->part;
+    uint32_t* poff = pcount + M;  // M + 1
+    uint32_t* psum = poff + M + 1;
+    uint32_t* bnd[2] = {psum + nsum, psum + nsum + nbound};
+    uint32_t* tbase = bnd[1] + nbound;
+    uint32_t* cnt2 = tbase + nseg_max + 1;
+    int src = s.cur;
+    {
+        PicProfScope ps(ctx, PIC_K_SORT);
+        k_part_count<<<(unsigned)ntile, PART_BLOCK, 0, ctx->stream>>>(g, s.buf[src].cell, s.d_n, shift, nbk,
+                                                                     (uint32_t)ntile, pcount);
+        const int nb = (int)((M + SCAN_TILE - 1) / SCAN_TILE);
+        k_scan_reduce<<<nb, BLOCK, 0, ctx->stream>>>(pcount, (int)M, psum);
+        k_scan_top<<<1, BLOCK, 0, ctx->stream>>>(psum, nb);
+        k_scan_down<<<nb, BLOCK, 0, ctx->stream>>>(pcount, (int)M, psum, poff, -1, nullptr);
+        k_part_bounds<<<(unsigned)((nbk + 1 + 255) / 256), 256, 0, ctx->stream>>>(poff, nbk, (uint32_t)ntile, bnd[0]);
+    }
+    {
+        PicProfScope ps(ctx, PIC_K_SCATTER);
+        k_part_move<<<(unsigned)ntile, PART_BLOCK, 0, ctx->stream>>>(g, s.buf[src], s.buf[src ^ 1], s.d_n, shift,
+                                                                    nbk, (uint32_t)ntile, poff);
+    }
+    src ^= 1;
+    int nseg = nbk, cur_b = 0;
+    for (int l = 0; l < levels; l++) {
+        const int st = steps[l];
+        shift -= st;
+        const unsigned tiles = (unsigned)(ntile + (size_t)nseg);
+        {
+            PicProfScope ps(ctx, PIC_K_SORT);
+            k_seg_tiles<<<1, PART_BLOCK, 0, ctx->stream>>>(bnd[cur_b], nseg, tbase);
+            k_seg_count<<<tiles, PART_BLOCK, 0, ctx->stream>>>(g, s.buf[src].cell, bnd[cur_b], tbase, nseg, shift,
+                                                               st, cnt2);
+            k_seg_scan<<<(unsigned)nseg, PART_B, 0, ctx->stream>>>(bnd[cur_b], tbase, nseg, st, cnt2,
+                                                                   bnd[cur_b ^ 1]);
+        }
+        {
+            PicProfScope ps(ctx, PIC_K_SCATTER);
+            k_seg_move<<<tiles, PART_BLOCK, 0, ctx->stream>>>(g, s.buf[src], s.buf[src ^ 1], bnd[cur_b], tbase,
+                                                              nseg, shift, st, cnt2);
+        }
+        src ^= 1;
+        cur_b ^= 1;
+        nseg <<= st;
+    }
+    PIC_HIP(ctx, hipMemsetAsync(s.count, 0, ((size_t)g.ncells + 1) * 4, ctx->stream));
+    {
+        PicProfScope ps(ctx, PIC_K_SCATTER);
+        k_fine_sort<<<(unsigned)nseg, PART_BLOCK, 0, ctx->stream>>>(g, s.buf[src], s.buf[src ^ 1], bnd[cur_b], shift,
+                                                                    s.off, s.count, s.rank, s.d_n);
+    }
+    PIC_HIP(ctx, hipGetLastError());
+    s.cur = src ^ 1;
+    s.hist_valid = false;
+    return PIC_OK;
+}
+
 extern "C" int pic_sort(pic_ctx* ctx, int sid) {
     int rc = pic_check_sid(ctx, sid);
     if (rc) return rc;
     Species& s = ctx->sp[sid];
     const GridC& g = ctx->g;
-    if (!s.hist_valid) {
-        rc = pic_histogram(ctx, sid);
+    const size_t n = s.n_upper ? s.n_upper : 1;
+    if (s.scrambled && n >= PART_MIN) {
+        rc = sort_partitioned(ctx, s);
         if (rc) return rc;
+    } else {
+        if (!s.hist_valid) {
+            rc = pic_histogram(ctx, sid);
+            if (rc) return rc;
+        }
+        scan_counts(ctx, s, s.off, true);
+        {
+            PicProfScope ps(ctx, PIC_K_SCATTER);
+            k_scatter<<<xcd_blocks1((n + SC_V - 1) / SC_V, BLOCK), BLOCK, 0, ctx->stream>>>(
+                g, s.buf[s.cur], s.buf[s.cur ^ 1], s.rank, s.off, s.d_n, (uint32_t)s.cap, 0u);
+        }
+        PIC_HIP(ctx, hipGetLastError());
+        s.cur ^= 1;
     }
-    scan_counts(ctx, s, s.off, true);
-    {
-        PicProfScope ps(ctx, PIC_K_SCATTER);
-        size_t n = s.n_upper ? s.n_upper : 1;
-        k_scatter<<<xcd_blocks1((n + SC_V - 1) / SC_V, BLOCK), BLOCK, 0, ctx->stream>>>(
-            g, s.buf[s.cur], s.buf[s.cur ^ 1], s.rank, s.off, s.d_n, (uint32_t)s.cap, 0u);
-    }
-    PIC_HIP(ctx, hipGetLastError());
-    s.cur ^= 1;
+    s.scrambled = false;
     s.sorted = true;
     s.hist_valid = false;
     s.two_part = false;
@@ -1515,6 +1955,7 @@ int pic_join_tail(pic_ctx* ctx, int sid) {
     PIC_HIP(ctx, hipGetLastError());
     s.cur ^= 1;
     s.sorted = true;
+    s.scrambled = false;
     s.hist_valid = false;
     s.two_part = false;
     s.tail_ok = true;
@@ -1560,6 +2001,7 @@ extern "C" int pic_push_sort(pic_ctx* ctx, int sid, double dt) {
     PIC_HIP(ctx, hipGetLastError());
     s.cur ^= 1;
     s.sorted = true;
+    s.scrambled = false;
     s.hist_valid = false;
     s.two_part = false;
     s.tail_ok = true;
@@ -1688,6 +2130,7 @@ extern "C" int pic_load_box_thermal(pic_ctx* ctx, int sid, const double centre[3
     s.sorted = false;
     s.two_part = false;
     s.tail_ok = false;
+    s.scrambled = true;  // uniform positions in thread order
     if (n_loaded) *n_loaded = num_macro;
     return PIC_OK;
 }
