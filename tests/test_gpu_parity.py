@@ -193,10 +193,10 @@ def test_sort_and_density(pg):
 
 def test_scrambled_sort_keeps_every_particle(pg):
     """Particles in random order (an upload, a box load) of more than 2^20:
-    the sort partitions them by coarse bucket before the counting sort's
-    scatter.  The result is ordered by cell, holds the same particles bit
-    for bit, and its density agrees with that after a plain sort (the same
-    species sorted again takes the plain path)."""
+    the sort is an MSD radix sort (coarse buckets, then LDS-local sorts of
+    the final buckets).  The result is ordered by cell, holds the same
+    particles bit for bit, and its density agrees with that after a plain
+    sort (the same species sorted again takes the plain path)."""
     ni, nj, nk = 64, 48, 40
     x0, xm = np.array([0.0, 0.0, 0.0]), np.array([0.063, 0.047, 0.039])
     g = O.make_grid(ni, nj, nk, x0, xm)
@@ -221,6 +221,51 @@ def test_scrambled_sort_keeps_every_particle(pg):
         c.sort(sid)  # sorted already: the plain path (the order inside a cell may differ)
         c.deposit(sid, pg.DEPOSIT_DENSITY)
         assert np.max(np.abs(den - c.species_field(sid, pg.SFIELD_DEN))) <= REL32 * np.max(den)
+
+
+def test_scrambled_sort_hot_cells_and_removed(pg):
+    """The MSD sort's other paths: final buckets beyond the LDS capacity
+    (most particles in three cells: global ranks and a scatter within the
+    bucket), and removed particles (a push without a sort of scrambled
+    input leaves holes that the sort drops).  Checked against the same
+    particles taken through the counting sort."""
+    ni, nj, nk = 64, 48, 40
+    x0, xm = np.array([0.0, 0.0, 0.0]), np.array([0.063, 0.047, 0.039])
+    g = O.make_grid(ni, nj, nk, x0, xm)
+    c = pg.Context(ni, nj, nk, x0, xm)
+    rng = np.random.default_rng(23)
+    N = 1_300_000
+    p = _rows(_f32_particles(_uniform_particles(rng, g, N, 1e5, 2.5e3))).copy()
+    dx = (xm - x0) / (np.array([ni, nj, nk]) - 1)
+    hot = rng.integers(0, 3, int(0.6 * N))  # 60% of the particles in three cells
+    cells = np.array([[5, 7, 9], [30, 20, 11], [62, 46, 38]])
+    p[: len(hot), :3] = x0 + (cells[hot] + rng.random((len(hot), 3)) * 0.98 + 0.01) * dx
+    p = p[rng.permutation(N)].reshape(-1)
+    order = lambda a: a[np.lexsort(a.T[::-1])]
+    a = c.species_add(O.ME, -O.QE, 2.5e3, N + 1000)
+    c.upload(a, p)
+    before = _rows(c.download(a))
+    c.sort(a)
+    after = _rows(c.download(a))
+    assert np.all(np.diff(_cell_of(g, after)) >= 0)
+    assert np.array_equal(order(before), order(after))
+    # removed particles: fast particles pushed (no sort) out of the box
+    fast = _rows(p).copy()
+    fast[:, 3:6] = rng.normal(0, 2e6, (N, 3)).astype(np.float32)
+    s1 = c.species_add(O.ME, -O.QE, 2.5e3, N + 1000)
+    s2 = c.species_add(O.ME, -O.QE, 2.5e3, N + 1000)
+    dt = 5e-9  # about a cell per step: particles near the walls leave
+    for sid in (s1, s2):
+        c.upload(sid, fast.reshape(-1))
+    c.push(s1, dt)  # scrambled with holes -> MSD sort
+    c.sort(s1)
+    c.sort(s2)  # MSD sort, then a push and the counting sort
+    c.push(s2, dt)
+    c.sort(s2)
+    q1, q2 = _rows(c.download(s1)), _rows(c.download(s2))
+    assert len(q1) < N and len(q1) == c.count(s1)
+    assert np.all(np.diff(_cell_of(g, q1)) >= 0)
+    assert np.array_equal(order(q1), order(q2))
 
 
 def _cell_of(g, parts):
