@@ -81,7 +81,11 @@ int pic_species_add(pic_ctx* ctx, double mass, double charge, double mpw0, size_
  * particle set (positions outside the box or in an object are dropped at
  * the next sort, as addParticle would, Species.cpp:420-426). */
 int pic_particles_upload(pic_ctx* ctx, int sid, const double* aos7, size_t n);
-/* const std::vector<Particle>& getConstPartRef() (Species.cpp:820-822) */
+/* const std::vector<Particle>& getConstPartRef() (Species.cpp:820-822).
+ * Particles are listed in array order, except for a DSMC neutral species
+ * whose births since its last sort are kept behind the sorted part: those
+ * are listed cell by cell in the order the next pic_mcc_apply picks them
+ * (each cell's particles, scanned in the listed order, form its pick pool). */
 int pic_particles_download(pic_ctx* ctx, int sid, double* aos7, size_t capacity, size_t* n);
 /* size_t getNumParticles() (Species.cpp:43-45) */
 int pic_particles_count(pic_ctx* ctx, int sid, size_t* n);
@@ -263,10 +267,12 @@ int pic_mcc_create(pic_ctx* ctx, int sid_neutral, int sid_ion, int sid_electron,
                    const pic_mcc_desc* desc, const double* sig_E, const double* sig_v, int nsig,
                    int* mcc_id);
 /* Interaction::apply(dt) (Interactions.cpp:567-598 -> :600-762).  Sorts the
- * neutrals and electrons first when needed.  A cell's pick pool holds its
- * neutrals and, as in the reference, the neutrals split off earlier in the
- * same call (:698-700).  Draws: Philox4x32-10 keyed by (seed, step),
- * counter (cell, group). */
+ * electrons first when needed.  The neutrals' births of earlier calls stay
+ * behind their sorted part, indexed by cell (joined with it once they pass
+ * 0.75 of it; after any other change the neutrals are sorted whole).  A
+ * cell's pick pool holds its neutrals and, as in the reference, the
+ * neutrals split off earlier in the same call (:698-700).  Draws:
+ * Philox4x32-10 keyed by (seed, step), counter (cell, group). */
 int pic_mcc_apply(pic_ctx* ctx, int mcc_id, double dt, uint32_t seed, uint32_t step,
                   pic_mcc_stats* stats);
 
