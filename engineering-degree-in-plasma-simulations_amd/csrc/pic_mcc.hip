@@ -232,7 +232,8 @@ constexpr double JOIN_FRACTION = 0.75;
 template <bool LDS_TABLE>
 __global__ void __launch_bounds__(BLOCK)
     k_mcc(GridC g, MccArgs a, Sp N, const uint32_t* __restrict__ offN, const uint32_t* __restrict__ toffN,
-          const uint32_t* __restrict__ tidxN, Sp I, Sp E, const uint32_t* __restrict__ offE, const double* __restrict__ table,
+          const uint32_t* __restrict__ tidxN, Sp I, Sp E, const uint32_t* __restrict__ offE,
+          const uint32_t* __restrict__ toffE, const uint32_t* __restrict__ tidxE, const double* __restrict__ table,
           const double* __restrict__ wsv, unsigned long long* __restrict__ part, uint32_t* __restrict__ spill) {
     // this workgroup's stripe: collisions, ionizations, groups, overflow, step maximum
     unsigned long long* stats = part + (blockIdx.x % NSTRIPE) * SSTRIDE;
@@ -254,7 +255,13 @@ __global__ void __launch_bounds__(BLOCK)
     unsigned long long* spill_fill = part + 7;
     uint32_t chunk = CHUNK_NONE;  // this thread's last arena chunk
     if (c < g.ncells) {
-        uint32_t e0 = offE[c], e1 = offE[c + 1];
+        // a cell's electrons: its range of the sorted part, then (toffE) its
+        // entries of the appended part's index
+        uint32_t e0 = offE[c], e1 = offE[c + 1], te0 = 0, te1 = 0;
+        if (toffE) {
+            te0 = toffE[c];
+            te1 = toffE[c + 1];
+        }
         uint32_t n0 = offN[c], n1 = offN[c + 1];
         // a cell's neutrals: its range of the sorted part, then (toffN) its
         // range of the appended part's cell-ordered index tidxN
@@ -269,7 +276,7 @@ __global__ void __launch_bounds__(BLOCK)
         // reference appends their indices to the cell's list); removals take
         // the last entry into the removed slot (:719-721)
         int Nb = (int)(n1 - n0), Nt = (int)(t1 - t0), nown = 0;
-        const int Ne = (int)(e1 - e0), Nn0 = Nb + Nt;
+        const int Neb = (int)(e1 - e0), Ne = Neb + (int)(te1 - te0), Nn0 = Nb + Nt;
         auto loc_of = [&](int slot) -> uint32_t {
             if (slot < Nb) return n0 + (uint32_t)slot;
             if (slot < Nb + Nt) return tidxN[t0 + (uint32_t)(slot - Nb)];
@@ -298,7 +305,8 @@ __global__ void __launch_bounds__(BLOCK)
                 const int Nn = Nb + Nt + nown;
                 int ns = (int)(dr(0) * (double)Nn);
                 int es = (int)(dr(1) * (double)Ne);
-                const uint32_t pn = loc_of(ns), pe = e0 + (uint32_t)es;
+                const uint32_t pn = loc_of(ns);
+                const uint32_t pe = es < Neb ? e0 + (uint32_t)es : tidxE[te0 + (uint32_t)(es - Neb)];
                 const bool staged = (pn & REF_LDS) != 0;
                 Birth* nb = staged ? &births.b[pn & ~REF_LDS] : nullptr;
                 // both particles' weights and velocities requested together
@@ -575,9 +583,19 @@ extern "C" int pic_mcc_apply(pic_ctx* ctx, int id, double dt, uint32_t seed, uin
             rc = pic_sort(ctx, m.sn);
         }
     }
-    if (!rc && !E.sorted) rc = pic_sort(ctx, m.se);
+    // the electrons likewise (their births are appended; none is removed)
+    if (!rc && !E.sorted) {
+        if (E.tail_ok && E.n_upper >= E.bulk_n) {
+            rc = pic_sort_tail(ctx, m.se);
+            if (!rc && (double)(E.n_upper - E.bulk_n) > JOIN_FRACTION * (double)E.bulk_n)
+                rc = pic_join_tail(ctx, m.se);
+        } else {
+            rc = pic_sort(ctx, m.se);
+        }
+    }
     if (rc) return rc;
     const bool two = !N.sorted && N.two_part;
+    const bool twoE = !E.sorted && E.two_part;
     const GridC& g = ctx->g;
     MccArgs a;
     a.m_n = N.mass;
@@ -606,11 +624,13 @@ extern "C" int pic_mcc_apply(pic_ctx* ctx, int id, double dt, uint32_t seed, uin
         PicProfScope ps(ctx, PIC_K_MCC);
         if (m.nsig <= SIG_LDS_MAX)
             k_mcc<true><<<grid_for((size_t)g.ncells, BLOCK), BLOCK, 0, ctx->stream>>>(
-                g, a, sn, N.off, two ? N.toff : nullptr, N.tidx, si, se, E.off, m.sig, m.d_wsv,
+                g, a, sn, N.off, two ? N.toff : nullptr, N.tidx, si, se, E.off, twoE ? E.toff : nullptr, E.tidx,
+                m.sig, m.d_wsv,
                 m.d_part, m.d_spill);
         else
             k_mcc<false><<<grid_for((size_t)g.ncells, BLOCK), BLOCK, 0, ctx->stream>>>(
-                g, a, sn, N.off, two ? N.toff : nullptr, N.tidx, si, se, E.off, m.sig, m.d_wsv,
+                g, a, sn, N.off, two ? N.toff : nullptr, N.tidx, si, se, E.off, twoE ? E.toff : nullptr, E.tidx,
+                m.sig, m.d_wsv,
                 m.d_part, m.d_spill);
         k_mcc_finish<<<1, 1, 0, ctx->stream>>>(m.d_wsv, m.d_part, m.d_stats, N.d_n, (uint32_t)N.cap,
                                                I.d_n, (uint32_t)I.cap, E.d_n, (uint32_t)E.cap);
@@ -621,7 +641,7 @@ extern "C" int pic_mcc_apply(pic_ctx* ctx, int id, double dt, uint32_t seed, uin
     if (rc) return rc;
     unsigned long long* hs = reinterpret_cast<unsigned long long*>(ctx->pin);  // 6 counters
     double* hw = ctx->pin + 6;                                                 // 2 maxima
-    uint32_t* cn = reinterpret_cast<uint32_t*>(ctx->pin + 8);                 // 4 counts
+    uint32_t* cn = reinterpret_cast<uint32_t*>(ctx->pin + 8);                 // 5 counts
     PIC_HIP(ctx, hipMemcpyAsync(hs, m.d_stats, 6 * sizeof(unsigned long long), hipMemcpyDeviceToHost,
                                 ctx->stream));
     PIC_HIP(ctx, hipMemcpyAsync(hw, m.d_wsv, 2 * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
@@ -629,8 +649,10 @@ extern "C" int pic_mcc_apply(pic_ctx* ctx, int id, double dt, uint32_t seed, uin
     PIC_HIP(ctx, hipMemcpyAsync(&cn[1], I.d_n, 4, hipMemcpyDeviceToHost, ctx->stream));
     PIC_HIP(ctx, hipMemcpyAsync(&cn[2], E.d_n, 4, hipMemcpyDeviceToHost, ctx->stream));
     PIC_HIP(ctx, hipMemcpyAsync(&cn[3], N.d_n + 3, 4, hipMemcpyDeviceToHost, ctx->stream));
+    PIC_HIP(ctx, hipMemcpyAsync(&cn[4], E.d_n + 3, 4, hipMemcpyDeviceToHost, ctx->stream));
     PIC_HIP(ctx, pic_spin_wait(ctx->stream));
-    N.bulk_n = cn[3];  // the sorted part (the live count of the last whole sort)
+    N.bulk_n = cn[3];  // the sorted parts (the live counts of the last whole sorts)
+    E.bulk_n = cn[4];
     N.n_upper = cn[0];
     I.n_upper = cn[1];
     E.n_upper = cn[2];
@@ -639,12 +661,12 @@ extern "C" int pic_mcc_apply(pic_ctx* ctx, int id, double dt, uint32_t seed, uin
         N.sorted = I.sorted = E.sorted = false;
         N.hist_valid = I.hist_valid = E.hist_valid = false;
     }
-    // births only appended: the neutrals' next sort may take just those,
-    // unless a neutral was removed (a removed particle leaves the species
-    // only in a whole sort); ions and electrons are sorted whole
+    // births only appended: the neutrals' and electrons' next sorts may
+    // take just those, unless a neutral was removed (a removed particle
+    // leaves the species only in a whole sort); the ions are sorted whole
     if (hs[4]) N.tail_ok = false;
-    I.tail_ok = E.tail_ok = false;
-    I.two_part = E.two_part = false;
+    I.tail_ok = false;
+    I.two_part = false;
     if (stats) {
         stats->collisions = (long long)hs[0];
         stats->ionizations = (long long)hs[1];

@@ -89,12 +89,12 @@ def test_electron_heavier_branch_matches_oracle(pg, sigma_table_path):
 
 def test_appended_neutrals_indexed_then_joined_match_oracle(pg, sigma_table_path):
     """Neutrals ten times heavier than the electrons: collisions split them
-    (Interactions.cpp:684-703) and the split neutrals are appended.  The
-    neutrals are sorted once; every later step indexes the appended part by
-    cell (a cell's pick pool lists it in array order) and joins it with the
-    sorted part once it outgrows a fraction of it.  Each step is compared
-    with the oracle under the same draws on the downloaded arrays, whose
-    scan order per cell is the pool order."""
+    (Interactions.cpp:684-703) and the split neutrals are appended, as are
+    the ionizations' electrons.  Neutrals and electrons are sorted once;
+    every later step indexes their appended parts by cell and joins the
+    neutrals' with the sorted part once it outgrows a fraction of it.  Each
+    step is compared with the oracle under the same draws on the downloaded
+    arrays, which list the appended parts cell by cell in pick-pool order."""
     n, dx = 9, 1e-4
     L = dx * (n - 1)
     c = pg.Context(n, n, n, [0, 0, 0], [L, L, L])
@@ -131,9 +131,9 @@ def test_appended_neutrals_indexed_then_joined_match_oracle(pg, sigma_table_path
         return len(key) if len(drop) == 0 else int(drop[0]) + 1
 
     c.sort(sn)
+    c.sort(se)
     prefixes, counts = [], []
     for step in range(6):
-        c.sort(se)  # the electrons' pool in array order too
         neu, ele, ion = c.download(sn), c.download(se), c.download(si)
         prefixes.append(sorted_prefix(neu))
         counts.append(neu.size // 7)
