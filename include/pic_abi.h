@@ -266,13 +266,13 @@ typedef struct {
 int pic_mcc_create(pic_ctx* ctx, int sid_neutral, int sid_ion, int sid_electron,
                    const pic_mcc_desc* desc, const double* sig_E, const double* sig_v, int nsig,
                    int* mcc_id);
-/* Interaction::apply(dt) (Interactions.cpp:567-598 -> :600-762).  Sorts the
- * electrons first when needed.  The neutrals' births of earlier calls stay
- * behind their sorted part, indexed by cell (joined with it once they pass
- * 0.75 of it; after any other change the neutrals are sorted whole).  A
- * cell's pick pool holds its neutrals and, as in the reference, the
- * neutrals split off earlier in the same call (:698-700).  Draws:
- * Philox4x32-10 keyed by (seed, step), counter (cell, group). */
+/* Interaction::apply(dt) (Interactions.cpp:567-598 -> :600-762).  The
+ * neutrals' and electrons' births of earlier calls stay behind their sorted
+ * parts, indexed by cell (joined with them once they pass 0.75 of them;
+ * after any other change a species is sorted whole).  A cell's pick pool
+ * holds its neutrals and, as in the reference, the neutrals split off
+ * earlier in the same call (:698-700).  Draws: Philox4x32-10 keyed by
+ * (seed, step), counter (cell, group). */
 int pic_mcc_apply(pic_ctx* ctx, int mcc_id, double dt, uint32_t seed, uint32_t step,
                   pic_mcc_stats* stats);
 
