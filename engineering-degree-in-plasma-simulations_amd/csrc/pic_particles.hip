@@ -1973,6 +1973,13 @@ extern "C" int pic_push_sort(pic_ctx* ctx, int sid, double dt) {
     if (rc) return rc;
     Species& s = ctx->sp[sid];
     const GridC& g = ctx->g;
+    // random-order input (a load or an upload): sorted first, so that both
+    // passes see a wave's particles in a few cells (same result: the push is
+    // per particle and the output is sorted either way)
+    if (s.scrambled && s.n_upper >= PART_MIN) {
+        rc = pic_sort(ctx, sid);
+        if (rc) return rc;
+    }
     PushArgs a = push_args(ctx, s, dt);  // emission in the second pass only
     const size_t n = s.n_upper ? s.n_upper : 1;
     PIC_HIP(ctx, hipMemsetAsync(s.count, 0, ((size_t)g.ncells + 1) * 4, ctx->stream));

@@ -254,18 +254,22 @@ def test_scrambled_sort_hot_cells_and_removed(pg):
     fast[:, 3:6] = rng.normal(0, 2e6, (N, 3)).astype(np.float32)
     s1 = c.species_add(O.ME, -O.QE, 2.5e3, N + 1000)
     s2 = c.species_add(O.ME, -O.QE, 2.5e3, N + 1000)
+    s3 = c.species_add(O.ME, -O.QE, 2.5e3, N + 1000)
     dt = 5e-9  # about a cell per step: particles near the walls leave
-    for sid in (s1, s2):
+    for sid in (s1, s2, s3):
         c.upload(sid, fast.reshape(-1))
     c.push(s1, dt)  # scrambled with holes -> MSD sort
     c.sort(s1)
     c.sort(s2)  # MSD sort, then a push and the counting sort
     c.push(s2, dt)
     c.sort(s2)
-    q1, q2 = _rows(c.download(s1)), _rows(c.download(s2))
+    c.push_sort(s3, dt)  # scrambled: sorted before the two passes
+    q1, q2, q3 = _rows(c.download(s1)), _rows(c.download(s2)), _rows(c.download(s3))
     assert len(q1) < N and len(q1) == c.count(s1)
     assert np.all(np.diff(_cell_of(g, q1)) >= 0)
+    assert np.all(np.diff(_cell_of(g, q3)) >= 0)
     assert np.array_equal(order(q1), order(q2))
+    assert np.array_equal(order(q1), order(q3))
 
 
 def _cell_of(g, parts):
