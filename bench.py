@@ -158,10 +158,13 @@ def cpu_baseline_dsmc(nxy, nz, n_neu, n_ele, steps, dt):
 def dsmc_bench(pg, args, cpu):
     """The DSMC step of the discharge workload on one GPU: 256 x 64 x 64 nodes,
     O2 at 10 Pa / 300 K as variable-weight neutrals, electrons at 10 eV and
-    ions, 50M particles in all; a step is pic_mcc_apply (the counting sort of
-    neutrals and electrons, then one thread per cell running the NTC groups,
-    collisions, ionization and births), as the reference's apply sorts and
-    pairs.  Returns the section of the bench line."""
+    ions, 50M particles in all; a step is pic_mcc_apply (the neutrals' and
+    electrons' births of the last step indexed by cell, or a join or whole
+    sort, then one thread per cell running the NTC groups, collisions,
+    ionization and births), as the reference's apply sorts and pairs.  The
+    per-stage profile repeats the same steps on a second, identical set-up
+    (same seeds and step keys), so its stages cover the timed steps' work.
+    Returns the section of the bench line."""
     nxy, nz, dx = 64, 256, 1e-4
     L = np.array([dx * (nxy - 1), dx * (nxy - 1), dx * (nz - 1)])
     box = 0.98 * L
@@ -171,22 +174,26 @@ def dsmc_bench(pg, args, cpu):
     w_n = n_gas * vol / Nn
     w_e = w_n / 1000.0
     dt = args.dsmc_dt
-    ctx = pg.Context(nxy, nxy, nz, [0, 0, 0], L)
-    # elastic splits append ~5% of the neutrals per step: room for the
-    # warm-up, timed and profiled steps
-    sn = ctx.species_add(32 * AMU, 0.0, w_n, int(Nn * (1.0 + 0.06 * (args.dsmc_warmup + 2 * args.dsmc_steps + 2))))
-    si = ctx.species_add(32 * AMU, QE, w_e, Ni + Ne // 2 + 1024)
-    se = ctx.species_add(ME, -QE, w_e, Ne + Ne // 2 + 1024)
-    c = 0.5 * L
-    ctx.load_box_thermal(sn, c, box, n_gas, 300.0, seed=21, mode=0)
-    ctx.load_box_thermal(se, c, box, Ne * w_e / vol, 10.0 * EV_K, seed=22, mode=0)
-    ctx.load_box_thermal(si, c, box, Ni * w_e / vol, 300.0, seed=23, mode=0)
     t = np.loadtxt(SIGMA_TABLE)
-    mid = ctx.mcc_create(sn, si, se, 12.07 * QE, t[:, 0], t[:, 1])
-    step = 0
-    for _ in range(args.dsmc_warmup):  # also brings W sigma v_max to its working value
-        ctx.mcc_apply(mid, dt, 7, step)
-        step += 1
+
+    def setup():
+        ctx = pg.Context(nxy, nxy, nz, [0, 0, 0], L)
+        # elastic splits append ~5% of the neutrals per step: room for the
+        # warm-up and timed steps
+        sn = ctx.species_add(32 * AMU, 0.0, w_n, int(Nn * (1.0 + 0.06 * (args.dsmc_warmup + args.dsmc_steps + 2))))
+        si = ctx.species_add(32 * AMU, QE, w_e, Ni + Ne // 2 + 1024)
+        se = ctx.species_add(ME, -QE, w_e, Ne + Ne // 2 + 1024)
+        c = 0.5 * L
+        ctx.load_box_thermal(sn, c, box, n_gas, 300.0, seed=21, mode=0)
+        ctx.load_box_thermal(se, c, box, Ne * w_e / vol, 10.0 * EV_K, seed=22, mode=0)
+        ctx.load_box_thermal(si, c, box, Ni * w_e / vol, 300.0, seed=23, mode=0)
+        mid = ctx.mcc_create(sn, si, se, 12.07 * QE, t[:, 0], t[:, 1])
+        for step in range(args.dsmc_warmup):  # also brings W sigma v_max to its working value
+            ctx.mcc_apply(mid, dt, 7, step)
+        return ctx, sn, se, mid
+
+    ctx, sn, se, mid = setup()
+    step = args.dsmc_warmup
     K = args.dsmc_steps
     n0 = ctx.count(sn) + ctx.count(se)
     ctx.sync()
@@ -200,8 +207,11 @@ def dsmc_bench(pg, args, cpu):
     ctx.sync()
     T = time.perf_counter() - t0
     n1 = ctx.count(sn) + ctx.count(se)
-    # as many more steps with HIP events around every launch (the same
-    # mix of tail sorts and whole sorts as the timed steps), averaged
+    ctx.close()
+    # the same steps again on an identical set-up, with HIP events around
+    # every launch, averaged
+    ctx, sn, se, mid = setup()
+    step = args.dsmc_warmup
     ctx.profile(True)
     ctx.profile_reset()
     st = {"collisions": 0, "ionizations": 0, "groups": 0}
@@ -224,9 +234,8 @@ def dsmc_bench(pg, args, cpu):
     roof_mcc = {"bound": "hbm", "kernel": "k_mcc", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": ach / HBM_PEAK_GBS, "traffic": measured_traffic("k_mcc"), "alg_bytes_per_launch": alg,
                 "avg_launch_ms": mcc_ms}
-    # the sorts (the neutrals' appended births, or the whole species, and the
-    # electrons): their time only -- how many particles a sort moved is not
-    # visible here
+    # the sorts (the appended births' index, a join or a whole sort): their
+    # time only -- how many particles a sort moved is not visible here
     sc_ms, sc_n = prof["scatter"]
     so_ms = prof["sort"][0]
     out = {
