@@ -15,6 +15,7 @@
 // sequential rnd() calls, Interactions.cpp:656-678, :895-944).
 #include <algorithm>
 #include <cmath>
+#include <cstring>
 
 #include "pic_internal.h"
 
@@ -501,23 +502,38 @@ __global__ void __launch_bounds__(BLOCK)
 
 // the stripes summed into stats (and the step maximum into wsv[1]);
 // W_sigma_v_rel_max <- step maximum when any collision happened
-// (Interactions.cpp:751-756)
-__global__ void k_mcc_finish(double* wsv, const unsigned long long* part, unsigned long long* stats,
-                             uint32_t* caps_n, uint32_t cap_n, uint32_t* caps_i, uint32_t cap_i,
-                             uint32_t* caps_e, uint32_t cap_e) {
-    unsigned long long t[6] = {0, 0, 0, 0, 0, 0}, mb = 0;
-    for (int s = 0; s < NSTRIPE; s++) {
-        for (int q = 0; q < 4; q++) t[q] += part[s * SSTRIDE + q];
-        t[4] += part[s * SSTRIDE + 5];
-        t[5] += part[s * SSTRIDE + 6];
-        mb = part[s * SSTRIDE + 4] > mb ? part[s * SSTRIDE + 4] : mb;
+// (Interactions.cpp:751-756).  stats then holds all the host reads back
+// (MCC_OUT words: the six counters, wsv[0..1] as bits, the live counts of
+// the three species and the neutrals' and electrons' sorted parts), and the
+// stripes are cleared for the next call.
+constexpr int MCC_OUT = 13;
+__global__ void __launch_bounds__(256)
+    k_mcc_finish(double* wsv, unsigned long long* part, unsigned long long* stats, uint32_t* caps_n,
+                 uint32_t cap_n, uint32_t* caps_i, uint32_t cap_i, uint32_t* caps_e, uint32_t cap_e) {
+    if (threadIdx.x == 0) {
+        unsigned long long t[6] = {0, 0, 0, 0, 0, 0}, mb = 0;
+        for (int s = 0; s < NSTRIPE; s++) {
+            for (int q = 0; q < 4; q++) t[q] += part[s * SSTRIDE + q];
+            t[4] += part[s * SSTRIDE + 5];
+            t[5] += part[s * SSTRIDE + 6];
+            mb = part[s * SSTRIDE + 4] > mb ? part[s * SSTRIDE + 4] : mb;
+        }
+        for (int q = 0; q < 6; q++) stats[q] = t[q];
+        wsv[1] = __longlong_as_double((long long)mb);
+        if (t[0] > 0) wsv[0] = wsv[1];
+        if (caps_n[0] > cap_n) caps_n[0] = cap_n;
+        if (caps_i[0] > cap_i) caps_i[0] = cap_i;
+        if (caps_e[0] > cap_e) caps_e[0] = cap_e;
+        stats[6] = (unsigned long long)__double_as_longlong(wsv[0]);
+        stats[7] = (unsigned long long)__double_as_longlong(wsv[1]);
+        stats[8] = caps_n[0];
+        stats[9] = caps_i[0];
+        stats[10] = caps_e[0];
+        stats[11] = caps_n[3];
+        stats[12] = caps_e[3];
     }
-    for (int q = 0; q < 6; q++) stats[q] = t[q];
-    wsv[1] = __longlong_as_double((long long)mb);
-    if (stats[0] > 0) wsv[0] = wsv[1];
-    if (caps_n[0] > cap_n) caps_n[0] = cap_n;
-    if (caps_i[0] > cap_i) caps_i[0] = cap_i;
-    if (caps_e[0] > cap_e) caps_e[0] = cap_e;
+    __syncthreads();
+    for (int q = threadIdx.x; q < NSTRIPE * SSTRIDE; q += blockDim.x) part[q] = 0;
 }
 
 }  // namespace
@@ -546,8 +562,11 @@ extern "C" int pic_mcc_create(pic_ctx* ctx, int sn, int si, int se, const pic_mc
     m.nsig = nsig;
     PIC_HIP(ctx, hipMalloc(&m.sig, 2 * (size_t)nsig * sizeof(double)));
     PIC_HIP(ctx, hipMalloc(&m.d_wsv, 2 * sizeof(double)));
-    PIC_HIP(ctx, hipMalloc(&m.d_stats, 6 * sizeof(unsigned long long)));
+    PIC_HIP(ctx, hipMalloc(&m.d_stats, MCC_OUT * sizeof(unsigned long long)));
     PIC_HIP(ctx, hipMalloc(&m.d_part, (size_t)NSTRIPE * SSTRIDE * sizeof(unsigned long long)));
+    // (k_mcc_finish clears the stripes after each call)
+    PIC_HIP(ctx, hipMemsetAsync(m.d_part, 0, (size_t)NSTRIPE * SSTRIDE * sizeof(unsigned long long),
+                                ctx->stream));
     PIC_HIP(ctx, hipMalloc(&m.d_spill, (size_t)SPILL_CAP * sizeof(uint32_t)));
     PIC_HIP(ctx, hipMemcpyAsync(m.sig, sig_E, nsig * sizeof(double), hipMemcpyHostToDevice,
                                 ctx->stream));
@@ -618,8 +637,6 @@ extern "C" int pic_mcc_apply(pic_ctx* ctx, int id, double dt, uint32_t seed, uin
     Sp sn{N.buf[N.cur], N.d_n, (uint32_t)N.cap};
     Sp si{I.buf[I.cur], I.d_n, (uint32_t)I.cap};
     Sp se{E.buf[E.cur], E.d_n, (uint32_t)E.cap};
-    PIC_HIP(ctx, hipMemsetAsync(m.d_part, 0, (size_t)NSTRIPE * SSTRIDE * sizeof(unsigned long long),
-                                ctx->stream));
     {
         PicProfScope ps(ctx, PIC_K_MCC);
         if (m.nsig <= SIG_LDS_MAX)
@@ -632,30 +649,25 @@ extern "C" int pic_mcc_apply(pic_ctx* ctx, int id, double dt, uint32_t seed, uin
                 g, a, sn, N.off, two ? N.toff : nullptr, N.tidx, si, se, E.off, twoE ? E.toff : nullptr, E.tidx,
                 m.sig, m.d_wsv,
                 m.d_part, m.d_spill);
-        k_mcc_finish<<<1, 1, 0, ctx->stream>>>(m.d_wsv, m.d_part, m.d_stats, N.d_n, (uint32_t)N.cap,
-                                               I.d_n, (uint32_t)I.cap, E.d_n, (uint32_t)E.cap);
+        k_mcc_finish<<<1, 256, 0, ctx->stream>>>(m.d_wsv, m.d_part, m.d_stats, N.d_n, (uint32_t)N.cap,
+                                                 I.d_n, (uint32_t)I.cap, E.d_n, (uint32_t)E.cap);
     }
     PIC_HIP(ctx, hipGetLastError());
-    // the step's results through pinned memory (asynchronous copies), one wait
-    rc = pic_ensure_pin(ctx, 128);
+    // the step's results through pinned memory (one asynchronous copy), one wait
+    rc = pic_ensure_pin(ctx, MCC_OUT * sizeof(unsigned long long));
     if (rc) return rc;
-    unsigned long long* hs = reinterpret_cast<unsigned long long*>(ctx->pin);  // 6 counters
-    double* hw = ctx->pin + 6;                                                 // 2 maxima
-    uint32_t* cn = reinterpret_cast<uint32_t*>(ctx->pin + 8);                 // 5 counts
-    PIC_HIP(ctx, hipMemcpyAsync(hs, m.d_stats, 6 * sizeof(unsigned long long), hipMemcpyDeviceToHost,
+    unsigned long long* hs = reinterpret_cast<unsigned long long*>(ctx->pin);  // MCC_OUT words
+    PIC_HIP(ctx, hipMemcpyAsync(hs, m.d_stats, MCC_OUT * sizeof(unsigned long long), hipMemcpyDeviceToHost,
                                 ctx->stream));
-    PIC_HIP(ctx, hipMemcpyAsync(hw, m.d_wsv, 2 * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
-    PIC_HIP(ctx, hipMemcpyAsync(&cn[0], N.d_n, 4, hipMemcpyDeviceToHost, ctx->stream));
-    PIC_HIP(ctx, hipMemcpyAsync(&cn[1], I.d_n, 4, hipMemcpyDeviceToHost, ctx->stream));
-    PIC_HIP(ctx, hipMemcpyAsync(&cn[2], E.d_n, 4, hipMemcpyDeviceToHost, ctx->stream));
-    PIC_HIP(ctx, hipMemcpyAsync(&cn[3], N.d_n + 3, 4, hipMemcpyDeviceToHost, ctx->stream));
-    PIC_HIP(ctx, hipMemcpyAsync(&cn[4], E.d_n + 3, 4, hipMemcpyDeviceToHost, ctx->stream));
     PIC_HIP(ctx, pic_spin_wait(ctx->stream));
-    N.bulk_n = cn[3];  // the sorted parts (the live counts of the last whole sorts)
-    E.bulk_n = cn[4];
-    N.n_upper = cn[0];
-    I.n_upper = cn[1];
-    E.n_upper = cn[2];
+    double hw[2];
+    std::memcpy(&hw[0], &hs[6], sizeof(double));
+    std::memcpy(&hw[1], &hs[7], sizeof(double));
+    N.n_upper = (size_t)hs[8];
+    I.n_upper = (size_t)hs[9];
+    E.n_upper = (size_t)hs[10];
+    N.bulk_n = (size_t)hs[11];  // the sorted parts (the live counts of the last whole sorts)
+    E.bulk_n = (size_t)hs[12];
     m.wsv_host = hw[0];
     if (hs[0] || hs[1]) {
         N.sorted = I.sorted = E.sorted = false;
