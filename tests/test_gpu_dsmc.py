@@ -156,6 +156,62 @@ def test_appended_neutrals_indexed_then_joined_match_oracle(pg, sigma_table_path
     assert max(prefixes) > 1.2 * n0, prefixes
 
 
+def test_species_operations_after_indexed_births(pg, sigma_table_path):
+    """After DSMC steps leave births indexed behind the sorted parts, every
+    other operation takes the species as they are: the density of the
+    neutrals equals the oracle's of their downloaded particles, a merge of
+    the electrons keeps their weight, a push of the electrons sorts them,
+    and the next DSMC step still equals the oracle's under the same draws."""
+    n, dx = 9, 1e-4
+    L = dx * (n - 1)
+    c = pg.Context(n, n, n, [0, 0, 0], [L, L, L])
+    g = O.make_grid(n, n, n, [0, 0, 0], [L, L, L])
+    rng = np.random.default_rng(31)
+
+    def parts(count, vth, w):
+        p = np.zeros((count, 7))
+        p[:, :3] = rng.uniform(0.01 * L, 0.99 * L, (count, 3))
+        p[:, 3:6] = rng.normal(0, vth, (count, 3)).astype(np.float32)
+        p[:, 6] = w
+        return p.reshape(-1)
+
+    sn = c.species_add(16 * O.AMU, 0.0, 100.0, 200000)
+    si = c.species_add(16 * O.AMU, O.QE, 1.0, 200000)
+    se = c.species_add(O.ME, -O.QE, 1.0, 200000)
+    c.upload(sn, parts(30 * 512, 300.0, 10.0))
+    c.upload(se, parts(10 * 512, 3e6, 1.0))
+    sE, sV = O.load_sigma_table(sigma_table_path)
+    E_ion = 1313.9 * 1000 / 6.02214076e23
+    dt, wsv = 2e-2, 1.5e-12
+    mid = c.mcc_create(sn, si, se, E_ion, sE, sV, wsv_max_init=wsv)
+    prm = O.mcc_params(16 * O.AMU, O.ME, E_ion, sE, sV, dt, g.dx[0] * g.dx[1] * g.dx[2],
+                       pool_append=True)
+    c.sort(sn)
+    c.sort(se)
+    for step in range(2):
+        wsv = c.mcc_apply(mid, dt, 5, step)["wsv_max"]
+    # the neutrals' density from the arrays as they are
+    neu = c.download(sn)
+    c.deposit(sn, pg.DEPOSIT_DENSITY)
+    ref = O.number_density(g, neu)
+    assert np.max(np.abs(c.species_field(sn, pg.SFIELD_DEN) - ref)) <= 1e-5 * np.max(ref)
+    # a merge of the electrons keeps their weight
+    w0 = np.sum(_rows_sorted(c.download(se))[:, 6])
+    c.merge(se, seed=3)
+    assert np.sum(_rows_sorted(c.download(se))[:, 6]) == pytest.approx(w0, rel=1e-6)
+    # a push of the electrons (no field) leaves them sorted by cell
+    c.push_sort(se, 1e-13)
+    # the next DSMC step from these states equals the oracle's
+    neu, ele, ion = c.download(sn), c.download(se), c.download(si)
+    st = c.mcc_apply(mid, dt, 5, 2)
+    on, oi, oe, wsv_o, ost = O.mcc_apply(g, prm, wsv, O.rng_philox(5, 2, TAG_MCC), neu, ion, ele)
+    assert st["groups"] == ost.groups
+    assert st["collisions"] == ost.collisions > 100
+    assert st["ionizations"] == ost.ionizations
+    for sid, want in ((sn, on), (si, oi), (se, oe)):
+        _same_multiset(want, c.download(sid), L)
+
+
 def _ensemble_stats(ele, ion_iz, coll):
     e = ele.reshape(-1, 7)
     ke = np.sum(e[:, 6] * np.sum(e[:, 3:6] ** 2, axis=1)) / np.sum(e[:, 6])
