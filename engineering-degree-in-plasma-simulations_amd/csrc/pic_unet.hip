@@ -387,8 +387,12 @@ __global__ void __launch_bounds__(256, 2) k_conv3_rb(ConvArgs a) {  // two workg
     constexpr int NWZ = 4 / NWN;
     constexpr int CIN = 32 * KS, NC = CIN / 16;
     constexpr int SX = TX + 2, SY = RY + 2, SZ = RZ * NWZ + 2, NV = SX * SY * SZ;
-    constexpr int NT = NV * NC * 2, RT = (NT + 255) / 256;
-    __shared__ __attribute__((aligned(16))) uint4 tile[NT];  // [chunk][voxel][2]
+    constexpr int J = NC * 2;  // uint4 per voxel
+    // chunk stride padded by 64 bytes: the staging stores (a voxel's J uint4
+    // on consecutive lanes) fall on distinct banks
+    constexpr int NVP = NV + 2;
+    constexpr int NT = NV * J, RT = (NT + 255) / 256;
+    __shared__ __attribute__((aligned(16))) uint4 tile[NVP * J];  // [chunk][voxel (NVP)][2]
     const Dims d = a.d;
     const int ntx = d.nx / TX, nty = d.ny / RY;
     const int b = blockIdx.x;
@@ -405,24 +409,27 @@ __global__ void __launch_bounds__(256, 2) k_conv3_rb(ConvArgs a) {  // two workg
     for (int t = 0; t < 27; t++)
         bfr[t] = __builtin_bit_cast(bf8v, *reinterpret_cast<const uint4*>(wrow + t * CIN));
     {
+        // item e = voxel * J + j, j the 16-byte piece of the voxel's
+        // channels: consecutive lanes read consecutive 16 bytes
         uint4 rt[RT];
 #pragma unroll
         for (int q = 0; q < RT; q++) {
-            const int e = tid + 256 * q;  // e = (chunk * NV + vox) * 2 + half
+            const int e = tid + 256 * q;
             uint4 v = make_uint4(0, 0, 0, 0);
             if (e < NT) {
-                const int hf = e & 1, vox = (e >> 1) % NV, ch = (e >> 1) / NV;
+                const int vox = e / J, j = e % J;
                 const int sx = vox % SX, sy = (vox / SX) % SY, sz = vox / (SX * SY);
                 const int gx = x0 + sx - 1, gy = y0 + sy - 1, gz = z0 + sz - 1;
                 if (gx >= 0 && gx < d.nx && gy >= 0 && gy < d.ny && gz >= 0 && gz < d.nz)
-                    v = *reinterpret_cast<const uint4*>(a.in + (((long long)gz * d.ny + gy) * d.nx + gx) * CIN +
-                                                        16 * ch + 8 * hf);
+                    v = *reinterpret_cast<const uint4*>(a.in + (((long long)gz * d.ny + gy) * d.nx + gx) * CIN + 8 * j);
             }
             rt[q] = v;
         }
 #pragma unroll
-        for (int q = 0; q < RT; q++)
-            if (tid + 256 * q < NT) tile[tid + 256 * q] = rt[q];
+        for (int q = 0; q < RT; q++) {
+            const int e = tid + 256 * q;
+            if (e < NT) tile[((e % J >> 1) * NVP + e / J) * 2 + (e & 1)] = rt[q];
+        }
     }
     __syncthreads();
     f4v acc[RY][RZ];
@@ -448,7 +455,7 @@ __global__ void __launch_bounds__(256, 2) k_conv3_rb(ConvArgs a) {  // two workg
 #pragma unroll
                     for (int ox = 0; ox < 3; ox++) {
                         const int sv = ((wz * RZ + iz) * SY + iy) * SX + m + ox;
-                        const bf8v afr = __builtin_bit_cast(bf8v, tile[(c16 * NV + sv) * 2 + hf]);
+                        const bf8v afr = __builtin_bit_cast(bf8v, tile[(c16 * NVP + sv) * 2 + hf]);
 #pragma unroll
                         for (int dz = 0; dz < 3; dz++)
 #pragma unroll
@@ -466,7 +473,7 @@ __global__ void __launch_bounds__(256, 2) k_conv3_rb(ConvArgs a) {  // two workg
             auto afrag = [&](int q) {
                 const int ox = q % 3, iy = (q / 3) % (RY + 2), iz = q / (3 * (RY + 2));
                 const int sv = ((wz * RZ + iz) * SY + iy) * SX + m + ox;
-                return __builtin_bit_cast(bf8v, tile[(c16 * NV + sv) * 2 + hf]);
+                return __builtin_bit_cast(bf8v, tile[(c16 * NVP + sv) * 2 + hf]);
             };
             bf8v aq[PD];
 #pragma unroll
@@ -495,7 +502,7 @@ __global__ void __launch_bounds__(256, 2) k_conv3_rb(ConvArgs a) {  // two workg
     // then written with 16-byte stores (every (z, y) row of 16 voxels is
     // contiguous in memory).
     constexpr int CO = 16 * NWN, OZ = RZ * NWZ;
-    static_assert(OZ * RY * TX * CO * 2 <= NT * 16, "the output tile fits the staged input's LDS");
+    static_assert(OZ * RY * TX * CO * 2 <= NVP * J * 16, "the output tile fits the staged input's LDS");
     __syncthreads();  // every wave is done with the staged input
     bf16* ot = reinterpret_cast<bf16*>(tile);
     const int n = n0 + m;
@@ -565,14 +572,18 @@ __global__ void __launch_bounds__(256) k_conv3_small(ConvArgs a) {
     constexpr int TY = 2, TZ = 2, SX = TX + 2, SY = TY + 2, SZ = TZ + 2;
     constexpr int NV = SX * SY * SZ;             // staged voxels
     constexpr int NC = CIN / 16;                 // 16-channel chunks
-    constexpr int NT = NV * NC * 2, NW = 27 * 16 * NC * 2;  // uint4 to stage
+    constexpr int J = NC * 2;                    // uint4 per voxel or weight row
+    // chunk strides padded by 64 bytes: the staging stores (a voxel's or
+    // row's J uint4 on consecutive lanes) then fall on distinct banks
+    constexpr int NVP = NV + 2, WR = 27 * 16, WRP = WR + 2;
+    constexpr int NT = NV * J, NW = WR * J;      // uint4 to stage
     constexpr int RT = (NT + 255) / 256, RW = (NW + 255) / 256;
-    constexpr int LDS_ST = (NT + NW) * 16, LDS_RED = 4 * 4 * 64 * 16;
+    constexpr int LDS_ST = (NVP + WRP) * J * 16, LDS_RED = 4 * 4 * 64 * 16;
     constexpr int LDS = LDS_ST > LDS_RED ? LDS_ST : LDS_RED;
     constexpr int KS = CIN / 32;                 // K steps per tap
     __shared__ __attribute__((aligned(16))) unsigned char smem[LDS];
-    uint4* tile = reinterpret_cast<uint4*>(smem);  // [chunk][voxel][2]
-    uint4* wl = tile + NT;                         // [chunk][tap * 16 + n][2]
+    uint4* tile = reinterpret_cast<uint4*>(smem);  // [chunk][voxel (NVP)][2]
+    uint4* wl = tile + NVP * J;                    // [chunk][tap * 16 + n (WRP)][2]
     const Dims d = a.d;
     const int ntx = (d.nx + TX - 1) / TX, nty = (d.ny + TY - 1) / TY;
     const int b = blockIdx.x;
@@ -580,38 +591,44 @@ __global__ void __launch_bounds__(256) k_conv3_small(ConvArgs a) {
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
     const int g = lane >> 4, m = lane & 15;
     const int nb0 = blockIdx.y;
+    // staging: item e = (voxel or row) * J + j, j the 16-byte piece of its
+    // channels, so consecutive lanes read consecutive 16 bytes
     uint4 rt[RT], rw[RW];
 #pragma unroll
     for (int q = 0; q < RT; q++) {
-        const int e = tid + 256 * q;  // e = (chunk * NV + vox) * 2 + half
+        const int e = tid + 256 * q;
         uint4 v = make_uint4(0, 0, 0, 0);
         if (e < NT) {
-            const int hf = e & 1, vox = (e >> 1) % NV, ch = (e >> 1) / NV;
+            const int vox = e / J, j = e % J;
             const int sx = vox % SX, sy = (vox / SX) % SY, sz = vox / (SX * SY);
             const int gx = x0 + sx - 1, gy = y0 + sy - 1, gz = z0 + sz - 1;
             if (gx >= 0 && gx < d.nx && gy >= 0 && gy < d.ny && gz >= 0 && gz < d.nz)
-                v = *reinterpret_cast<const uint4*>(a.in + (((long long)gz * d.ny + gy) * d.nx + gx) * CIN +
-                                                    16 * ch + 8 * hf);
+                v = *reinterpret_cast<const uint4*>(a.in + (((long long)gz * d.ny + gy) * d.nx + gx) * CIN + 8 * j);
         }
         rt[q] = v;
     }
+    const bf16* wblk = a.w + (long long)nb0 * 16 * 27 * CIN;
 #pragma unroll
     for (int q = 0; q < RW; q++) {
-        const int e = tid + 256 * q;  // e = (chunk * 432 + tap * 16 + n16) * 2 + half
+        const int e = tid + 256 * q;
         uint4 v = make_uint4(0, 0, 0, 0);
         if (e < NW) {
-            const int hf = e & 1, row = (e >> 1) % 432, ch = (e >> 1) / 432;
-            const int t = row >> 4, n = nb0 * 16 + (row & 15);
-            v = *reinterpret_cast<const uint4*>(a.w + (long long)n * 27 * CIN + t * CIN + 16 * ch + 8 * hf);
+            const int row = e / J, j = e % J;  // row = tap * 16 + n
+            const int t = row >> 4, n = row & 15;
+            v = *reinterpret_cast<const uint4*>(wblk + (n * 27 + t) * CIN + 8 * j);
         }
         rw[q] = v;
     }
 #pragma unroll
-    for (int q = 0; q < RT; q++)
-        if (tid + 256 * q < NT) tile[tid + 256 * q] = rt[q];
+    for (int q = 0; q < RT; q++) {
+        const int e = tid + 256 * q;
+        if (e < NT) tile[((e % J >> 1) * NVP + e / J) * 2 + (e & 1)] = rt[q];
+    }
 #pragma unroll
-    for (int q = 0; q < RW; q++)
-        if (tid + 256 * q < NW) wl[tid + 256 * q] = rw[q];
+    for (int q = 0; q < RW; q++) {
+        const int e = tid + 256 * q;
+        if (e < NW) wl[((e % J >> 1) * WRP + e / J) * 2 + (e & 1)] = rw[q];
+    }
     __syncthreads();
     f4v acc[4];
 #pragma unroll
@@ -621,12 +638,12 @@ __global__ void __launch_bounds__(256) k_conv3_small(ConvArgs a) {
         const int t = st / KS, h = st - t * KS;
         const int ox = t % 3, oy = (t / 3) % 3, oz = t / 9;
         const int c16 = 2 * h + ch;
-        const bf8v bfr = __builtin_bit_cast(bf8v, wl[(c16 * 432 + t * 16 + m) * 2 + hf]);
+        const bf8v bfr = __builtin_bit_cast(bf8v, wl[(c16 * WRP + t * 16 + m) * 2 + hf]);
 #pragma unroll
         for (int r = 0; r < 4; r++) {
             const int ty = r % TY, tz = r / TY;
             const int sv = ((tz + oz) * SY + (ty + oy)) * SX + (m + ox);
-            const bf8v afr = __builtin_bit_cast(bf8v, tile[(c16 * NV + sv) * 2 + hf]);
+            const bf8v afr = __builtin_bit_cast(bf8v, tile[(c16 * NVP + sv) * 2 + hf]);
             acc[r] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(afr, bfr, acc[r], 0, 0, 0);
         }
     }
