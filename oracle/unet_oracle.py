@@ -210,13 +210,14 @@ def prolong_trilinear(y):
     return out
 
 
-def precondition(r, regular, h2, params, channels, coarsen):
+def precondition(r, regular, h2, params, channels, coarsen, post_sweeps=3):
     """z = M(r) of the learned solver on a Dirichlet-walled uniform grid;
     arrays (nz, ny, nx).  One red-black Gauss-Seidel sweep on A z = r from 0
     (A = 7-point Laplacian / h2 on regular nodes; red = (i+j+k) even), the
     network's coarse correction c of the remaining residual res = r - A z,
     added as z += omega c with the A-norm optimal omega = (c.res)/(c.A c),
-    one sweep in the reverse colour order.  Returns (z, omega c)."""
+    post_sweeps sweeps in the reverse colour order (black, red).  Returns
+    (z, omega c)."""
     nz, ny, nx = r.shape
     k, j, i = np.meshgrid(np.arange(nz), np.arange(ny), np.arange(nx), indexing="ij")
     par = (i + j + k) % 2
@@ -251,6 +252,7 @@ def precondition(r, regular, h2, params, channels, coarsen):
     omega = np.sum(c * res) / den if den != 0 else 0.0
     corr = omega * c
     z += corr
-    sweep(1)
-    sweep(0)
+    for _ in range(post_sweeps):
+        sweep(1)
+        sweep(0)
     return z, corr

@@ -84,8 +84,9 @@ def test_grid_transfers():
 
 
 def test_precondition_oracle_is_two_level_cycle():
-    """With a zero network the preconditioner is a symmetric Gauss-Seidel
-    sweep; its output then satisfies the red/black update equations."""
+    """With a zero network the preconditioner is Gauss-Seidel sweeps (one
+    red-black, then the closing black-red ones); its output then satisfies
+    the red update equations."""
     n = 8
     regular = np.zeros((n, n, n), bool)
     regular[1:-1, 1:-1, 1:-1] = True
@@ -101,6 +102,49 @@ def test_precondition_oracle_is_two_level_cycle():
     # the last half-sweep was red: red nodes are exact for the final black values
     assert np.allclose(z[red], (nb[red] - 1e-4 * r[red]) / 6.0)
     assert np.all(z[~regular] == 0)
+
+
+def test_precondition_oracle_closing_sweeps_match_node_loops():
+    """The oracle's sweeps (numpy slices) against Gauss-Seidel written as
+    node-by-node loops, with a zero network (no coarse correction): one
+    red-black sweep from 0, then post_sweeps black-red sweeps; and with many
+    closing sweeps M(r) converges to the solution of A z = r."""
+    n = 8
+    h2 = 1e-4
+    regular = np.zeros((n, n, n), bool)
+    regular[1:-1, 1:-1, 1:-1] = True
+    r = np.where(regular, np.random.default_rng(5).normal(size=(n, n, n)), 0.0)
+    ch = (16, 16)
+    p0 = np.zeros(U.n_params(ch))
+
+    def gs_loops(post):
+        z = np.zeros((n, n, n))
+
+        def half(colour):
+            for k in range(1, n - 1):
+                for j in range(1, n - 1):
+                    for i in range(1, n - 1):
+                        if (i + j + k) % 2 == colour:
+                            s = (z[k - 1, j, i] + z[k + 1, j, i] + z[k, j - 1, i] + z[k, j + 1, i] +
+                                 z[k, j, i - 1] + z[k, j, i + 1])
+                            z[k, j, i] = (s - h2 * r[k, j, i]) / 6.0
+
+        half(0)
+        half(1)
+        for _ in range(post):
+            half(1)
+            half(0)
+        return z
+
+    for post in (1, 3):
+        z, corr = U.precondition(r, regular, h2, p0, ch, 2, post_sweeps=post)
+        assert np.all(corr == 0)
+        assert np.allclose(z, gs_loops(post), rtol=1e-12, atol=1e-14 * np.abs(z).max())
+    z, _ = U.precondition(r, regular, h2, p0, ch, 2, post_sweeps=300)
+    Az = np.zeros_like(z)
+    Az[1:-1, 1:-1, 1:-1] = (z[:-2, 1:-1, 1:-1] + z[2:, 1:-1, 1:-1] + z[1:-1, :-2, 1:-1] + z[1:-1, 2:, 1:-1] +
+                            z[1:-1, 1:-1, :-2] + z[1:-1, 1:-1, 2:] - 6.0 * z[1:-1, 1:-1, 1:-1]) / h2
+    assert np.linalg.norm((Az - r)[regular]) <= 1e-10 * np.linalg.norm(r[regular])
 
 
 def _ctx(pg, n):
