@@ -882,13 +882,32 @@ struct FcgHist {
     double* rr;  // r_{k+1} . r_{k+1}
 };
 
-// 7-point neighbour sum of f at node u, weighted per axis; offsets zeroed on
-// non-regular nodes (the result is then discarded) keep every load in bounds
+// The neighbour offsets of node u clamped only at the array's ends: the full
+// offsets on every regular (interior) node, in bounds on every node, and
+// independent of the node type, so a kernel's neighbour loads are issued
+// together with its node-type load instead of after it (values on
+// non-regular nodes are discarded)
+struct NbrOff {
+    unsigned mi, pi, mj, pj, mk, pk;
+};
+__device__ inline NbrOff nbr_off(const Stencil& s, unsigned u) {
+    const unsigned n = (unsigned)s.nn, sj = (unsigned)s.sj, sk = (unsigned)s.sk;
+    NbrOff o;
+    o.mi = u >= 1u ? 1u : 0u;
+    o.pi = u + 1u < n ? 1u : 0u;
+    o.mj = u >= sj ? sj : 0u;
+    o.pj = u + sj < n ? sj : 0u;
+    o.mk = u >= sk ? sk : 0u;
+    o.pk = u + sk < n ? sk : 0u;
+    return o;
+}
+// 7-point neighbour sum of f at node u, weighted per axis, with nbr_off's
+// offsets (the full stencil on regular nodes)
 template <typename F>
-__device__ inline double nbr_sum(const Stencil& s, unsigned u, bool reg, F f) {
-    const unsigned oi = reg ? 1u : 0u, oj = reg ? (unsigned)s.sj : 0u, ok = reg ? (unsigned)s.sk : 0u;
-    return (f(u - oi) + f(u + oi)) * s.cx + (f(u - oj) + f(u + oj)) * s.cy +
-           (f(u - ok) + f(u + ok)) * s.cz;
+__device__ inline double nbr_sum_any(const Stencil& s, unsigned u, F f) {
+    const NbrOff o = nbr_off(s, u);
+    return (f(u - o.mi) + f(u + o.pi)) * s.cx + (f(u - o.mj) + f(u + o.pj)) * s.cy +
+           (f(u - o.mk) + f(u + o.pk)) * s.cz;
 }
 
 // r = b - A phi on regular rows (0 elsewhere); out += {r.r, b.b, #(phi != 0)}
@@ -904,7 +923,7 @@ __global__ void __launch_bounds__(BLOCK)
         const bool reg = nt[u] == NT_REG;  // regular nodes are interior (Dirichlet walls)
         const double pu = phi[u];
         const double b = -rho[u] * ieps;
-        const double rv = reg ? b - (nbr_sum(s, u, reg, ph) - s.twos * pu) : 0.0;
+        const double rv = reg ? b - (nbr_sum_any(s, u, ph) - s.twos * pu) : 0.0;
         if (reg) {
             bb += b * b;
             cold += pu != 0.0;
@@ -941,12 +960,12 @@ __global__ void __launch_bounds__(BLOCK)
     for (long long m = chunk_begin(s.nn, end) + threadIdx.x; m < end; m += BLOCK) {
         const unsigned u = (unsigned)m;
         const bool reg = nt[u] == NT_REG;
-        const unsigned oi = reg ? 1u : 0u, oj = reg ? (unsigned)s.sj : 0u, ok = reg ? (unsigned)s.sk : 0u;
+        const NbrOff o = nbr_off(s, u);
         // q = A p = A z + beta A p_old: the stencil of z alone (seven loads
         // instead of fourteen), the previous q in place
         const double zu = z[u];
-        double az = (z[u - ok] + z[u + ok]) * s.cz + (z[u - oj] + z[u + oj]) * s.cy +
-                    (z[u - oi] + z[u + oi]) * s.cx - s.twos * zu;
+        double az = (z[u - o.mk] + z[u + o.pk]) * s.cz + (z[u - o.mj] + z[u + o.pj]) * s.cy +
+                    (z[u - o.mi] + z[u + o.pi]) * s.cx - s.twos * zu;
         if (P) az -= P[u] * zu;  // J = A - diag(P) (the Newton Jacobian)
         double pu = zu, qv = az;
         if (!first) {
@@ -1011,7 +1030,7 @@ __global__ void __launch_bounds__(BLOCK)
         if (red)
             zu = -r[u] * id;
         else
-            zu = (-nbr_sum(s, u, reg, rr) - r[u]) * id;
+            zu = (-nbr_sum_any(s, u, rr) - r[u]) * id;
         z[u] = reg ? (float)zu : 0.f;
     };
     for_pairs(s, [&](unsigned u, bool second, bool red) {
@@ -1034,7 +1053,7 @@ __global__ void __launch_bounds__(BLOCK)
         double zu = z[u];
         if (red) {
             const bool reg = nt[u] == NT_REG;
-            double upd = (nbr_sum(s, u, reg, zv) - r[u]) * inv_diag(s, P, u);
+            double upd = (nbr_sum_any(s, u, zv) - r[u]) * inv_diag(s, P, u);
             if (reg) {
                 zu = upd;
                 z[u] = zu;
@@ -1071,7 +1090,7 @@ __global__ void __launch_bounds__(BLOCK)
         if (red)
             zu = zc(u);
         else
-            zu = (nbr_sum(s, u, reg, zc) - r[u]) * inv_diag(s, P, u);
+            zu = (nbr_sum_any(s, u, zc) - r[u]) * inv_diag(s, P, u);
         z[u] = reg ? zu : 0.0;
     };
     for_pairs(s, [&](unsigned u, bool second, bool red) {
@@ -1088,8 +1107,9 @@ __global__ void __launch_bounds__(BLOCK)
               double* __restrict__ z, int red_colour, const double* __restrict__ P) {
     auto zv = [&](unsigned w) { return z[w]; };
     auto node = [&](unsigned u, bool red) {
-        if (red != (red_colour != 0) || nt[u] != NT_REG) return;
-        z[u] = (nbr_sum(s, u, true, zv) - r[u]) * inv_diag(s, P, u);
+        if (red != (red_colour != 0)) return;
+        const double upd = (nbr_sum_any(s, u, zv) - r[u]) * inv_diag(s, P, u);
+        if (nt[u] == NT_REG) z[u] = upd;
     };
     for_pairs(s, [&](unsigned u, bool second, bool red) {
         node(u, red);
