@@ -946,7 +946,7 @@ __global__ void __launch_bounds__(BLOCK)
 // Neighbour offsets are zeroed on non-regular nodes so that every load is in
 // bounds and the loop has no branches.
 __global__ void __launch_bounds__(BLOCK)
-    k_fcg_pq(Stencil s, const uint8_t* __restrict__ nt, const double* __restrict__ z,
+    k_fcg_pq(Stencil s, const uint8_t* __restrict__ nt, const float* __restrict__ z,
              const double* __restrict__ pold, double* __restrict__ pnew, double* __restrict__ q,
              FcgHist h, int k, const double* __restrict__ P) {
     double beta = 0.0;
@@ -964,8 +964,8 @@ __global__ void __launch_bounds__(BLOCK)
         // q = A p = A z + beta A p_old: the stencil of z alone (seven loads
         // instead of fourteen), the previous q in place
         const double zu = z[u];
-        double az = (z[u - o.mk] + z[u + o.pk]) * s.cz + (z[u - o.mj] + z[u + o.pj]) * s.cy +
-                    (z[u - o.mi] + z[u + o.pi]) * s.cx - s.twos * zu;
+        double az = ((double)z[u - o.mk] + z[u + o.pk]) * s.cz + ((double)z[u - o.mj] + z[u + o.pj]) * s.cy +
+                    ((double)z[u - o.mi] + z[u + o.pi]) * s.cx - s.twos * zu;
         if (P) az -= P[u] * zu;  // J = A - diag(P) (the Newton Jacobian)
         double pu = zu, qv = az;
         if (!first) {
@@ -1005,7 +1005,8 @@ __global__ void __launch_bounds__(BLOCK)
 // red half-sweep gives z = -r / (2 sum 1/dx^2) on red regular nodes
 // ((i + j + k) even); the black half-sweep that follows reads only those red
 // values, so it is written directly in r.  Non-regular nodes get 0.  z is
-// FP32 (the preconditioner's internal vectors; FP64 arithmetic).
+// FP32 (the preconditioner's vectors are stored in FP32, its arithmetic is
+// FP64); r32 (optional) receives r in FP32 for the closing sweeps.
 // 1 / the diagonal of the operator's regular row: 2 sum 1/dx^2, plus P_u
 // for the Newton Jacobian J = A - diag(P)
 __device__ inline double inv_diag(const Stencil& s, const double* __restrict__ P, unsigned u) {
@@ -1015,7 +1016,7 @@ __device__ inline double inv_diag(const Stencil& s, const double* __restrict__ P
 __global__ void __launch_bounds__(BLOCK)
     k_presmooth(Stencil s, const uint8_t* __restrict__ nt, const double* __restrict__ r,
                 float* __restrict__ z, double* __restrict__ clear, int nclear,
-                const double* __restrict__ P) {
+                const double* __restrict__ P, float* __restrict__ r32) {
     // the coarse correction's striped sums start from zero (no memset node)
     for (int q = blockIdx.x * BLOCK + threadIdx.x; q < nclear; q += gridDim.x * BLOCK) clear[q] = 0.0;
     // r / diag on regular nodes, 0 elsewhere (a red value is -that)
@@ -1032,6 +1033,7 @@ __global__ void __launch_bounds__(BLOCK)
         else
             zu = (-nbr_sum_any(s, u, rr) - r[u]) * id;
         z[u] = reg ? (float)zu : 0.f;
+        if (r32) r32[u] = (float)r[u];
     };
     for_pairs(s, [&](unsigned u, bool second, bool red) {
         node(u, red);
@@ -1042,25 +1044,27 @@ __global__ void __launch_bounds__(BLOCK)
 // The closing red half-sweep of the preconditioner fused with FCG's dot
 // products: red regular nodes get the Gauss-Seidel update (their neighbours
 // are black and final); then zq += z.q (when q is given) and rz += z.r.
-// z vanishes on non-regular nodes.
+// z vanishes on non-regular nodes.  z and r in FP32 (r32: r rounded), the
+// sums in FP64 of the stored values.
 __global__ void __launch_bounds__(BLOCK)
-    k_gs_red_dots(Stencil s, const uint8_t* __restrict__ nt, const double* __restrict__ r,
-                  double* __restrict__ z, const double* __restrict__ q, double* __restrict__ zq,
+    k_gs_red_dots(Stencil s, const uint8_t* __restrict__ nt, const float* __restrict__ r,
+                  float* __restrict__ z, const double* __restrict__ q, double* __restrict__ zq,
                   double* __restrict__ rz, const double* __restrict__ P) {
     double a = 0, b = 0;
-    auto zv = [&](unsigned w) { return z[w]; };
+    auto zv = [&](unsigned w) { return (double)z[w]; };
     auto node = [&](unsigned u, bool red) {
         double zu = z[u];
         if (red) {
             const bool reg = nt[u] == NT_REG;
-            double upd = (nbr_sum_any(s, u, zv) - r[u]) * inv_diag(s, P, u);
+            double upd = (nbr_sum_any(s, u, zv) - (double)r[u]) * inv_diag(s, P, u);
             if (reg) {
-                zu = upd;
-                z[u] = zu;
+                const float zf = (float)upd;
+                z[u] = zf;
+                zu = zf;
             }
         }
         if (q) a += zu * q[u];
-        b += zu * r[u];
+        b += zu * (double)r[u];
     };
     for_pairs(s, [&](unsigned u, bool second, bool red) {
         node(u, red);
@@ -1076,11 +1080,12 @@ __global__ void __launch_bounds__(BLOCK)
 // the coarse correction fused with the next (black) half-sweep:
 // z = zin + omega c on red nodes, the Gauss-Seidel update from those values on
 // black regular nodes, 0 on non-regular nodes; omega = d[0] / d[1] (A-norm
-// optimal step along c).  zin and c (FP32) vanish on non-regular nodes.
+// optimal step along c).  zin and c (FP32) vanish on non-regular nodes; r
+// and z in FP32.
 __global__ void __launch_bounds__(BLOCK)
-    k_gs_corr(Stencil s, const uint8_t* __restrict__ nt, const double* __restrict__ r,
+    k_gs_corr(Stencil s, const uint8_t* __restrict__ nt, const float* __restrict__ r,
               const float* __restrict__ zin, const float* __restrict__ c,
-              const double* __restrict__ d, double* __restrict__ z, const double* __restrict__ P) {
+              const double* __restrict__ d, float* __restrict__ z, const double* __restrict__ P) {
     const double d1 = sget(d + SSIZE);
     const double om = d1 != 0.0 ? sget(d) / d1 : 0.0;
     auto zc = [&](unsigned w) { return (double)zin[w] + om * c[w]; };
@@ -1090,8 +1095,8 @@ __global__ void __launch_bounds__(BLOCK)
         if (red)
             zu = zc(u);
         else
-            zu = (nbr_sum_any(s, u, zc) - r[u]) * inv_diag(s, P, u);
-        z[u] = reg ? zu : 0.0;
+            zu = (nbr_sum_any(s, u, zc) - (double)r[u]) * inv_diag(s, P, u);
+        z[u] = reg ? (float)zu : 0.f;
     };
     for_pairs(s, [&](unsigned u, bool second, bool red) {
         node(u, red);
@@ -1103,18 +1108,24 @@ __global__ void __launch_bounds__(BLOCK)
 // (red: (i + j + k) even) get (sum of neighbours - r) / diag; their
 // neighbours have the other colour, so the update is race-free
 __global__ void __launch_bounds__(BLOCK)
-    k_gs_half(Stencil s, const uint8_t* __restrict__ nt, const double* __restrict__ r,
-              double* __restrict__ z, int red_colour, const double* __restrict__ P) {
-    auto zv = [&](unsigned w) { return z[w]; };
+    k_gs_half(Stencil s, const uint8_t* __restrict__ nt, const float* __restrict__ r,
+              float* __restrict__ z, int red_colour, const double* __restrict__ P) {
+    auto zv = [&](unsigned w) { return (double)z[w]; };
     auto node = [&](unsigned u, bool red) {
         if (red != (red_colour != 0)) return;
-        const double upd = (nbr_sum_any(s, u, zv) - r[u]) * inv_diag(s, P, u);
-        if (nt[u] == NT_REG) z[u] = upd;
+        const double upd = (nbr_sum_any(s, u, zv) - (double)r[u]) * inv_diag(s, P, u);
+        if (nt[u] == NT_REG) z[u] = (float)upd;
     };
     for_pairs(s, [&](unsigned u, bool second, bool red) {
         node(u, red);
         if (second) node(u + 1, !red);
     });
+}
+
+// b = a in FP64
+__global__ void __launch_bounds__(BLOCK) k_widen(long long n, const float* __restrict__ a, double* __restrict__ b) {
+    for (long long i = (long long)blockIdx.x * BLOCK + threadIdx.x; i < n; i += (long long)gridDim.x * BLOCK)
+        b[i] = (double)a[i];
 }
 
 }  // namespace
@@ -1123,34 +1134,36 @@ __global__ void __launch_bounds__(BLOCK)
 // the learned preconditioner's pre-smoother, also used for its training pairs
 int pic_learned_presmooth(pic_ctx* ctx, const double* r, float* z) {
     Stencil s = make_stencil(ctx->g, false);
-    k_presmooth<<<pair_blocks(s), BLOCK, 0, ctx->stream>>>(s, ctx->ntype, r, z, nullptr, 0, nullptr);
+    k_presmooth<<<pair_blocks(s), BLOCK, 0, ctx->stream>>>(s, ctx->ntype, r, z, nullptr, 0, nullptr, nullptr);
     PIC_HIP(ctx, hipGetLastError());
     return PIC_OK;
 }
 
-// z = M(r); tmp holds the three FP32 vectors zpre, c, res.  The closing
-// half-sweep also accumulates *zq += z.q (q may be null) and *rz += z.r;
-// both must be zeroed by the caller.
-static int learned_precond(pic_ctx* ctx, const Stencil& s, const double* rin, double* zout,
+// z = M(r) in FP32; tmp holds the four FP32 vectors zpre, c, res and r32
+// (r rounded, read by the closing sweeps).  The closing half-sweep also
+// accumulates *zq += z.q (q may be null) and *rz += z.r; both must be zeroed
+// by the caller.
+static int learned_precond(pic_ctx* ctx, const Stencil& s, const double* rin, float* zout,
                            float* tmp, const double* q, double* zq, double* rz,
                            const double* P = nullptr) {
     unsigned nb = pair_blocks(s);
     float* zpre = tmp;
     float* c = tmp + s.nn;
     float* res = c + s.nn;
+    float* r32 = res + s.nn;
     double* ws = ctx->scratch + LEARNED_WS;  // the coarse correction's striped sums
     double* dots = ws + 2 * SSIZE;
-    k_presmooth<<<nb, BLOCK, 0, ctx->stream>>>(s, ctx->ntype, rin, zpre, ws, 4 * SSIZE, P);
+    k_presmooth<<<nb, BLOCK, 0, ctx->stream>>>(s, ctx->ntype, rin, zpre, ws, 4 * SSIZE, P, r32);
     int e = pic_unet_coarse_correct(ctx, rin, zpre, c, res, ws, P);
     if (e) return e;
     // the closing sweeps: the first black half in k_gs_corr, the last red
     // half in k_gs_red_dots, the half-sweeps between them in place
-    k_gs_corr<<<nb, BLOCK, 0, ctx->stream>>>(s, ctx->ntype, rin, zpre, c, dots, zout, P);
+    k_gs_corr<<<nb, BLOCK, 0, ctx->stream>>>(s, ctx->ntype, r32, zpre, c, dots, zout, P);
     for (int e = 1; e < LEARNED_POST_SWEEPS; e++) {
-        k_gs_half<<<nb, BLOCK, 0, ctx->stream>>>(s, ctx->ntype, rin, zout, 1, P);
-        k_gs_half<<<nb, BLOCK, 0, ctx->stream>>>(s, ctx->ntype, rin, zout, 0, P);
+        k_gs_half<<<nb, BLOCK, 0, ctx->stream>>>(s, ctx->ntype, r32, zout, 1, P);
+        k_gs_half<<<nb, BLOCK, 0, ctx->stream>>>(s, ctx->ntype, r32, zout, 0, P);
     }
-    k_gs_red_dots<<<nb, BLOCK, 0, ctx->stream>>>(s, ctx->ntype, rin, zout, q, zq, rz, P);
+    k_gs_red_dots<<<nb, BLOCK, 0, ctx->stream>>>(s, ctx->ntype, r32, zout, q, zq, rz, P);
     PIC_HIP(ctx, hipGetLastError());
     return PIC_OK;
 }
@@ -1174,14 +1187,16 @@ extern "C" int pic_learned_precondition(pic_ctx* ctx, const double* r, double* z
     rc = pic_ensure_work(ctx, 5 * nn * sizeof(double));
     if (rc) return rc;
     double* rd = ctx->work;
-    double* zd = rd + nn;
+    float* z32 = reinterpret_cast<float*>(rd + nn);
     double* rz = ctx->scratch + LEARNED_WS + 4 * SSIZE;  // striped
     PIC_HIP(ctx, hipMemcpyAsync(rd, r, nn * 8, hipMemcpyHostToDevice, ctx->stream));
     PIC_HIP(ctx, hipMemsetAsync(rz, 0, SSIZE * sizeof(double), ctx->stream));
-    rc = learned_precond(ctx, make_stencil(ctx->g, false), rd, zd, reinterpret_cast<float*>(zd + nn),
+    rc = learned_precond(ctx, make_stencil(ctx->g, false), rd, z32, reinterpret_cast<float*>(rd + 2 * nn),
                          nullptr, nullptr, rz);
     if (rc) return rc;
-    PIC_HIP(ctx, hipMemcpyAsync(z, zd, nn * 8, hipMemcpyDeviceToHost, ctx->stream));
+    k_widen<<<grid_for(nn, BLOCK), BLOCK, 0, ctx->stream>>>((long long)nn, z32, rd);  // r is spent
+    PIC_HIP(ctx, hipGetLastError());
+    PIC_HIP(ctx, hipMemcpyAsync(z, rd, nn * 8, hipMemcpyDeviceToHost, ctx->stream));
     PIC_HIP(ctx, pic_spin_wait(ctx->stream));
     return PIC_OK;
 }
@@ -1212,11 +1227,11 @@ static int solve_learned(pic_ctx* ctx, const pic_solver_params* p, pic_solver_st
     rc = pic_ensure_work(ctx, 8 * nn * sizeof(double) + 4 * hslots * sizeof(double));
     if (rc) return rc;
     double* r = ctx->work;
-    double* z = r + nn;
-    double* p0 = z + nn;
+    float* z = reinterpret_cast<float*>(r + nn);  // the preconditioned residual, FP32
+    double* p0 = r + 2 * nn;
     double* p1 = p0 + nn;
     double* q = p1 + nn;
-    float* ftmp = reinterpret_cast<float*>(q + nn);  // three FP32 vectors: 1.5 nn doubles
+    float* ftmp = reinterpret_cast<float*>(q + nn);  // four FP32 vectors: 2 nn doubles
     FcgHist h;
     h.rz = q + 3 * nn;  // past the FP32 vectors, inside the 8 nn doubles
     h.pq = h.rz + hslots;
@@ -1224,7 +1239,7 @@ static int solve_learned(pic_ctx* ctx, const pic_solver_params* p, pic_solver_st
     h.rr = h.zq + hslots;
     double* sc = ctx->scratch + LEARNED_WS + 5 * SSIZE;  // three striped scalars
     unsigned nb = stencil_blocks(s.nn), nbr = stencil_blocks_red(s.nn);
-    auto precond = [&](const double* rin, double* zout, const double* qv, double* zq, double* rz) {
+    auto precond = [&](const double* rin, float* zout, const double* qv, double* zq, double* rz) {
         return learned_precond(ctx, s, rin, zout, ftmp, qv, zq, rz);
     };
     // pinned host copies: [0, 3 S) initial sums, [3 S, 6 S) final FP64
@@ -1368,13 +1383,13 @@ static int solve_newton_learned(pic_ctx* ctx, const pic_solver_params* p, pic_so
     rc = pic_ensure_work(ctx, 9 * nn * sizeof(double) + 4 * hslots * sizeof(double));
     if (rc) return rc;
     double* r = ctx->work;
-    double* z = r + nn;
-    double* p0 = z + nn;
+    float* z = reinterpret_cast<float*>(r + nn);  // the preconditioned residual, FP32
+    double* p0 = r + 2 * nn;
     double* p1 = p0 + nn;
     double* q = p1 + nn;
     double* y = q + nn;
     double* P = y + nn;
-    float* ftmp = reinterpret_cast<float*>(P + nn);  // 1.5 nn doubles
+    float* ftmp = reinterpret_cast<float*>(P + nn);  // four FP32 vectors: 2 nn doubles
     FcgHist h;
     h.rz = P + nn + 2 * nn;
     h.pq = h.rz + hslots;
