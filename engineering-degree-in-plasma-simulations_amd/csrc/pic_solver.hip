@@ -863,8 +863,9 @@ static int solve_pcg(pic_ctx* ctx, const pic_solver_params* p, pic_solver_stats*
 // residual, LEARNED_POST_SWEEPS sweeps in the reverse colour order (black,
 // red).  The network sees only what the first sweep leaves, so the closing
 // sweeps are free to choose: three take the 128^3 time loop's solves from 5
-// to 3 iterations and cold solves from 11 to 7 (one sweep costs two
-// half-sweeps of ~15 us, an iteration ~275 us).
+// to 3 iterations and cold solves from 11 to 7 (a sweep costs ~17 us, an
+// iteration ~305 us).  The closing sweeps and the output z are stored in
+// FP32 (FP64 arithmetic).
 // ------------------------------------------------------------------------
 bool pic_unet_ready(pic_ctx* ctx);
 int pic_unet_coarse_correct(pic_ctx* ctx, const double* t, const float* z, float* c, float* res,
