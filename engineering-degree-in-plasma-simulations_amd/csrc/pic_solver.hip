@@ -858,14 +858,13 @@ static int solve_pcg(pic_ctx* ctx, const pic_solver_params* p, pic_solver_stats*
 // few iterations.  Plain PCG is the fallback and the final residual is
 // recomputed in FP64.
 //
-// Preconditioner M(r): one red-black Gauss-Seidel sweep on A z = r from
-// z = 0 (red, black), the network's coarse correction of the remaining
-// residual, LEARNED_POST_SWEEPS sweeps in the reverse colour order (black,
-// red).  The network sees only what the first sweep leaves, so the closing
-// sweeps are free to choose: three take the 128^3 time loop's solves from 5
-// to 3 iterations and cold solves from 11 to 7 (a sweep costs ~17 us, an
-// iteration ~305 us).  The closing sweeps and the output z are stored in
-// FP32 (FP64 arithmetic).
+// Preconditioner M(r): LEARNED_PRE_SWEEPS red-black Gauss-Seidel sweeps on
+// A z = r from z = 0 (red, black), the network's coarse correction of the
+// remaining residual, LEARNED_POST_SWEEPS sweeps in the reverse colour order
+// (black, red).  A sweep costs ~17 us at 128^3, an iteration ~290 us: three
+// sweeps on each side take the 128^3 time loop's solves from 5 iterations
+// (one sweep each side) to 2 and cold solves from 11 to 6.  The sweeps after
+// the first and the output z are stored in FP32 (FP64 arithmetic).
 // ------------------------------------------------------------------------
 bool pic_unet_ready(pic_ctx* ctx);
 int pic_unet_coarse_correct(pic_ctx* ctx, const double* t, const float* z, float* c, float* res,
@@ -873,7 +872,7 @@ int pic_unet_coarse_correct(pic_ctx* ctx, const double* t, const float* z, float
 
 namespace {
 
-constexpr int LEARNED_POST_SWEEPS = 3;
+constexpr int LEARNED_PRE_SWEEPS = 3, LEARNED_POST_SWEEPS = 3;
 
 // per-iteration striped scalars (slot k at base + k * SSIZE)
 struct FcgHist {
@@ -1107,9 +1106,11 @@ __global__ void __launch_bounds__(BLOCK)
 
 // one Gauss-Seidel half-sweep in place: the regular nodes of one colour
 // (red: (i + j + k) even) get (sum of neighbours - r) / diag; their
-// neighbours have the other colour, so the update is race-free
+// neighbours have the other colour, so the update is race-free.  r in FP32
+// (the solver's rounded copy) or FP64.
+template <typename RT>
 __global__ void __launch_bounds__(BLOCK)
-    k_gs_half(Stencil s, const uint8_t* __restrict__ nt, const float* __restrict__ r,
+    k_gs_half(Stencil s, const uint8_t* __restrict__ nt, const RT* __restrict__ r,
               float* __restrict__ z, int red_colour, const double* __restrict__ P) {
     auto zv = [&](unsigned w) { return (double)z[w]; };
     auto node = [&](unsigned u, bool red) {
@@ -1131,11 +1132,17 @@ __global__ void __launch_bounds__(BLOCK) k_widen(long long n, const float* __res
 
 }  // namespace
 
-// z = one red-black Gauss-Seidel sweep on A z = r from z = 0 (device vectors):
-// the learned preconditioner's pre-smoother, also used for its training pairs
+// z = LEARNED_PRE_SWEEPS red-black Gauss-Seidel sweeps on A z = r from z = 0
+// (device vectors): the learned preconditioner's pre-smoother, also used for
+// its training pairs
 int pic_learned_presmooth(pic_ctx* ctx, const double* r, float* z) {
     Stencil s = make_stencil(ctx->g, false);
-    k_presmooth<<<pair_blocks(s), BLOCK, 0, ctx->stream>>>(s, ctx->ntype, r, z, nullptr, 0, nullptr, nullptr);
+    unsigned nb = pair_blocks(s);
+    k_presmooth<<<nb, BLOCK, 0, ctx->stream>>>(s, ctx->ntype, r, z, nullptr, 0, nullptr, nullptr);
+    for (int e = 1; e < LEARNED_PRE_SWEEPS; e++) {
+        k_gs_half<double><<<nb, BLOCK, 0, ctx->stream>>>(s, ctx->ntype, r, z, 1, nullptr);
+        k_gs_half<double><<<nb, BLOCK, 0, ctx->stream>>>(s, ctx->ntype, r, z, 0, nullptr);
+    }
     PIC_HIP(ctx, hipGetLastError());
     return PIC_OK;
 }
@@ -1155,14 +1162,18 @@ static int learned_precond(pic_ctx* ctx, const Stencil& s, const double* rin, fl
     double* ws = ctx->scratch + LEARNED_WS;  // the coarse correction's striped sums
     double* dots = ws + 2 * SSIZE;
     k_presmooth<<<nb, BLOCK, 0, ctx->stream>>>(s, ctx->ntype, rin, zpre, ws, 4 * SSIZE, P, r32);
+    for (int e = 1; e < LEARNED_PRE_SWEEPS; e++) {
+        k_gs_half<float><<<nb, BLOCK, 0, ctx->stream>>>(s, ctx->ntype, r32, zpre, 1, P);
+        k_gs_half<float><<<nb, BLOCK, 0, ctx->stream>>>(s, ctx->ntype, r32, zpre, 0, P);
+    }
     int e = pic_unet_coarse_correct(ctx, rin, zpre, c, res, ws, P);
     if (e) return e;
     // the closing sweeps: the first black half in k_gs_corr, the last red
     // half in k_gs_red_dots, the half-sweeps between them in place
     k_gs_corr<<<nb, BLOCK, 0, ctx->stream>>>(s, ctx->ntype, r32, zpre, c, dots, zout, P);
     for (int e = 1; e < LEARNED_POST_SWEEPS; e++) {
-        k_gs_half<<<nb, BLOCK, 0, ctx->stream>>>(s, ctx->ntype, r32, zout, 1, P);
-        k_gs_half<<<nb, BLOCK, 0, ctx->stream>>>(s, ctx->ntype, r32, zout, 0, P);
+        k_gs_half<float><<<nb, BLOCK, 0, ctx->stream>>>(s, ctx->ntype, r32, zout, 1, P);
+        k_gs_half<float><<<nb, BLOCK, 0, ctx->stream>>>(s, ctx->ntype, r32, zout, 0, P);
     }
     k_gs_red_dots<<<nb, BLOCK, 0, ctx->stream>>>(s, ctx->ntype, r32, zout, q, zq, rz, P);
     PIC_HIP(ctx, hipGetLastError());

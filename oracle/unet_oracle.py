@@ -210,14 +210,14 @@ def prolong_trilinear(y):
     return out
 
 
-def precondition(r, regular, h2, params, channels, coarsen, post_sweeps=3):
+def precondition(r, regular, h2, params, channels, coarsen, pre_sweeps=3, post_sweeps=3):
     """z = M(r) of the learned solver on a Dirichlet-walled uniform grid;
-    arrays (nz, ny, nx).  One red-black Gauss-Seidel sweep on A z = r from 0
-    (A = 7-point Laplacian / h2 on regular nodes; red = (i+j+k) even), the
-    network's coarse correction c of the remaining residual res = r - A z,
-    added as z += omega c with the A-norm optimal omega = (c.res)/(c.A c),
-    post_sweeps sweeps in the reverse colour order (black, red).  Returns
-    (z, omega c)."""
+    arrays (nz, ny, nx).  pre_sweeps red-black Gauss-Seidel sweeps on
+    A z = r from 0 (A = 7-point Laplacian / h2 on regular nodes; red =
+    (i+j+k) even), the network's coarse correction c of the remaining
+    residual res = r - A z, added as z += omega c with the A-norm optimal
+    omega = (c.res)/(c.A c), post_sweeps sweeps in the reverse colour order
+    (black, red).  Returns (z, omega c)."""
     nz, ny, nx = r.shape
     k, j, i = np.meshgrid(np.arange(nz), np.arange(ny), np.arange(nx), indexing="ij")
     par = (i + j + k) % 2
@@ -233,8 +233,9 @@ def precondition(r, regular, h2, params, channels, coarsen, post_sweeps=3):
         m = regular & (par == color)
         z[m] = (nbr_sum(z)[m] - h2 * r[m]) / 6.0
 
-    sweep(0)
-    sweep(1)
+    for _ in range(pre_sweeps):
+        sweep(0)
+        sweep(1)
     Az = (nbr_sum(z) - 6.0 * z) / h2
     res = np.where(regular, r - Az, 0.0)
     if coarsen == 2:

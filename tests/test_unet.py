@@ -106,9 +106,9 @@ def test_precondition_oracle_is_two_level_cycle():
 
 def test_precondition_oracle_closing_sweeps_match_node_loops():
     """The oracle's sweeps (numpy slices) against Gauss-Seidel written as
-    node-by-node loops, with a zero network (no coarse correction): one
-    red-black sweep from 0, then post_sweeps black-red sweeps; and with many
-    closing sweeps M(r) converges to the solution of A z = r."""
+    node-by-node loops, with a zero network (no coarse correction):
+    pre_sweeps red-black sweeps from 0, then post_sweeps black-red sweeps;
+    and with many closing sweeps M(r) converges to the solution of A z = r."""
     n = 8
     h2 = 1e-4
     regular = np.zeros((n, n, n), bool)
@@ -117,7 +117,7 @@ def test_precondition_oracle_closing_sweeps_match_node_loops():
     ch = (16, 16)
     p0 = np.zeros(U.n_params(ch))
 
-    def gs_loops(post):
+    def gs_loops(pre, post):
         z = np.zeros((n, n, n))
 
         def half(colour):
@@ -129,17 +129,18 @@ def test_precondition_oracle_closing_sweeps_match_node_loops():
                                  z[k, j, i - 1] + z[k, j, i + 1])
                             z[k, j, i] = (s - h2 * r[k, j, i]) / 6.0
 
-        half(0)
-        half(1)
+        for _ in range(pre):
+            half(0)
+            half(1)
         for _ in range(post):
             half(1)
             half(0)
         return z
 
-    for post in (1, 3):
-        z, corr = U.precondition(r, regular, h2, p0, ch, 2, post_sweeps=post)
+    for pre, post in ((1, 1), (3, 3), (2, 1)):
+        z, corr = U.precondition(r, regular, h2, p0, ch, 2, pre_sweeps=pre, post_sweeps=post)
         assert np.all(corr == 0)
-        assert np.allclose(z, gs_loops(post), rtol=1e-12, atol=1e-14 * np.abs(z).max())
+        assert np.allclose(z, gs_loops(pre, post), rtol=1e-12, atol=1e-14 * np.abs(z).max())
     z, _ = U.precondition(r, regular, h2, p0, ch, 2, post_sweeps=300)
     Az = np.zeros_like(z)
     Az[1:-1, 1:-1, 1:-1] = (z[:-2, 1:-1, 1:-1] + z[2:, 1:-1, 1:-1] + z[1:-1, :-2, 1:-1] + z[1:-1, 2:, 1:-1] +
