@@ -366,8 +366,10 @@ int pic_unet_adam_step(pic_ctx* ctx, const float* grad, float lr);
 /* generate pairs with the plain solver on the device and train (Adam) */
 int pic_unet_train(pic_ctx* ctx, const pic_unet_train_desc* desc, pic_unet_train_stats* stats);
 /* z = M(r): the learned solver's preconditioner alone on host vectors of one
- * double per node (a red-black Gauss-Seidel sweep from z = 0, the network's
- * coarse correction of the remaining residual, a reverse-order sweep). */
+ * double per node (three red-black Gauss-Seidel sweeps from z = 0, the
+ * network's coarse correction of the remaining residual, three reverse-order
+ * sweeps; the sweeps after the first and z are computed in FP64 and stored in
+ * FP32). */
 int pic_learned_precondition(pic_ctx* ctx, const double* r, double* z);
 
 /* ---- profiling: HIP events around every launch of the context ---- */
