@@ -318,10 +318,10 @@ def _blob_density(n, seed=12345):
 
 def test_c4_shipped_network_cold_solve_128cubed(pg):
     """The shipped 128^3 weights: a blob density (not among the training
-    pairs) solved cold to a 1e-6 relative residual in at most 16 flexible-CG
-    iterations, checked by the oracle; the same cycle with a zero network
-    needs at least twice as many.  A short on-device training run at 128^3
-    cuts its loss by 40 %."""
+    pairs) solved cold to a 1e-6 relative residual in at most 10 flexible-CG
+    iterations (8 measured), checked by the oracle; the same cycle with a
+    zero network needs at least twice as many (93 measured).  A short
+    on-device training run at 128^3 cuts its loss by 40 % (5.8x measured)."""
     n = 128
     c, g, L = _box(pg, n)
     meta = c.unet_load(os.path.join(PKG, "data", "unet_128.f32"))
@@ -344,7 +344,7 @@ def test_c4_shipped_network_cold_solve_128cubed(pg):
     s_zero = solve()
     print("C4: trained %d FCG iterations (%.2f ms), zero network %d" % (
         s_trained["iterations"], 1e3 * s_trained["seconds"], s_zero["iterations"]), flush=True)
-    assert s_trained["iterations"] <= 16
+    assert s_trained["iterations"] <= 10
     assert s_zero["iterations"] >= 2 * s_trained["iterations"]
     c2 = pg.Context(n, n, n, [0, 0, 0], [L, L, L])
     c2.dirichlet_walls()
