@@ -1124,6 +1124,55 @@ __global__ void __launch_bounds__(BLOCK)
     });
 }
 
+// k_gs_half<float> on four consecutive nodes of an i-line per thread (ni a
+// multiple of 4): the z rows around the group come in 16-byte loads (the
+// i-neighbours at the group's ends as single words), r and the node types
+// likewise, and the group goes back in one 16-byte store with the other
+// colour's two values unchanged.  The same arithmetic as k_gs_half (equal
+// bit for bit); every value an update reads has the other colour, which no
+// thread writes.
+__global__ void __launch_bounds__(BLOCK)
+    k_gs_half4(Stencil s, const uint8_t* __restrict__ nt, const float* __restrict__ r,
+               float* __restrict__ z, int red_colour, const double* __restrict__ P) {
+    const long long ng = s.nn / 4;
+    const unsigned n = (unsigned)s.nn, sj = (unsigned)s.sj, sk = (unsigned)s.sk;
+    const unsigned want = red_colour ? 0u : 1u;  // parity (i + j + k) & 1 of the updated nodes
+    long long end;
+    for (long long m = chunk_begin(ng, end) + threadIdx.x; m < end; m += BLOCK) {
+        const unsigned u0 = 4u * (unsigned)m;
+        const unsigned row = u0 / (unsigned)s.ni;
+        const unsigned i0 = u0 - row * (unsigned)s.ni;
+        const unsigned k = row / (unsigned)s.nj, j = row - k * (unsigned)s.nj;
+        const unsigned p0 = (i0 + j + k) & 1u;
+        // in-bounds addresses everywhere; only regular (interior) nodes are
+        // updated, and their neighbours are the unclamped ones
+        const float4 zc = *reinterpret_cast<const float4*>(z + u0);
+        const float zl = z[u0 >= 1u ? u0 - 1u : u0];
+        const float zr = z[u0 + 4u < n ? u0 + 4u : u0];
+        const float4 ym = *reinterpret_cast<const float4*>(z + (u0 >= sj ? u0 - sj : u0));
+        const float4 yp = *reinterpret_cast<const float4*>(z + (u0 + sj < n ? u0 + sj : u0));
+        const float4 zm = *reinterpret_cast<const float4*>(z + (u0 >= sk ? u0 - sk : u0));
+        const float4 zp = *reinterpret_cast<const float4*>(z + (u0 + sk < n ? u0 + sk : u0));
+        const float4 r4 = *reinterpret_cast<const float4*>(r + u0);
+        const uint32_t t4 = *reinterpret_cast<const uint32_t*>(nt + u0);
+        const float c[4] = {zc.x, zc.y, zc.z, zc.w};
+        const float xm[4] = {zl, zc.x, zc.y, zc.z}, xp[4] = {zc.y, zc.z, zc.w, zr};
+        const float ymv[4] = {ym.x, ym.y, ym.z, ym.w}, ypv[4] = {yp.x, yp.y, yp.z, yp.w};
+        const float zmv[4] = {zm.x, zm.y, zm.z, zm.w}, zpv[4] = {zp.x, zp.y, zp.z, zp.w};
+        const float rv[4] = {r4.x, r4.y, r4.z, r4.w};
+        float out[4];
+#pragma unroll
+        for (int a = 0; a < 4; a++) {
+            out[a] = c[a];
+            if (((p0 + (unsigned)a) & 1u) != want || ((t4 >> (8 * a)) & 0xFFu) != (uint32_t)NT_REG) continue;
+            const double sum = ((double)xm[a] + (double)xp[a]) * s.cx + ((double)ymv[a] + (double)ypv[a]) * s.cy +
+                               ((double)zmv[a] + (double)zpv[a]) * s.cz;
+            out[a] = (float)((sum - (double)rv[a]) * inv_diag(s, P, u0 + (unsigned)a));
+        }
+        *reinterpret_cast<float4*>(z + u0) = make_float4(out[0], out[1], out[2], out[3]);
+    }
+}
+
 // b = a in FP64
 __global__ void __launch_bounds__(BLOCK) k_widen(long long n, const float* __restrict__ a, double* __restrict__ b) {
     for (long long i = (long long)blockIdx.x * BLOCK + threadIdx.x; i < n; i += (long long)gridDim.x * BLOCK)
@@ -1131,6 +1180,19 @@ __global__ void __launch_bounds__(BLOCK) k_widen(long long n, const float* __res
 }
 
 }  // namespace
+
+// one half-sweep of the learned cycle on FP32 vectors: the four-node kernel
+// where i-lines are a multiple of 4 long, else the per-pair one
+static void gs_half32(pic_ctx* ctx, const Stencil& s, const float* r32, float* z, int red_colour,
+                      const double* P) {
+    if (s.ni % 4 == 0) {
+        long long b = (s.nn / 4 + BLOCK - 1) / BLOCK;
+        b = (b + NXCD - 1) / NXCD * NXCD;
+        k_gs_half4<<<(unsigned)b, BLOCK, 0, ctx->stream>>>(s, ctx->ntype, r32, z, red_colour, P);
+    } else {
+        k_gs_half<float><<<pair_blocks(s), BLOCK, 0, ctx->stream>>>(s, ctx->ntype, r32, z, red_colour, P);
+    }
+}
 
 // z = LEARNED_PRE_SWEEPS red-black Gauss-Seidel sweeps on A z = r from z = 0
 // (device vectors): the learned preconditioner's pre-smoother, also used for
@@ -1163,8 +1225,8 @@ static int learned_precond(pic_ctx* ctx, const Stencil& s, const double* rin, fl
     double* dots = ws + 2 * SSIZE;
     k_presmooth<<<nb, BLOCK, 0, ctx->stream>>>(s, ctx->ntype, rin, zpre, ws, 4 * SSIZE, P, r32);
     for (int e = 1; e < LEARNED_PRE_SWEEPS; e++) {
-        k_gs_half<float><<<nb, BLOCK, 0, ctx->stream>>>(s, ctx->ntype, r32, zpre, 1, P);
-        k_gs_half<float><<<nb, BLOCK, 0, ctx->stream>>>(s, ctx->ntype, r32, zpre, 0, P);
+        gs_half32(ctx, s, r32, zpre, 1, P);
+        gs_half32(ctx, s, r32, zpre, 0, P);
     }
     int e = pic_unet_coarse_correct(ctx, rin, zpre, c, res, ws, P);
     if (e) return e;
@@ -1172,8 +1234,8 @@ static int learned_precond(pic_ctx* ctx, const Stencil& s, const double* rin, fl
     // half in k_gs_red_dots, the half-sweeps between them in place
     k_gs_corr<<<nb, BLOCK, 0, ctx->stream>>>(s, ctx->ntype, r32, zpre, c, dots, zout, P);
     for (int e = 1; e < LEARNED_POST_SWEEPS; e++) {
-        k_gs_half<float><<<nb, BLOCK, 0, ctx->stream>>>(s, ctx->ntype, r32, zout, 1, P);
-        k_gs_half<float><<<nb, BLOCK, 0, ctx->stream>>>(s, ctx->ntype, r32, zout, 0, P);
+        gs_half32(ctx, s, r32, zout, 1, P);
+        gs_half32(ctx, s, r32, zout, 0, P);
     }
     k_gs_red_dots<<<nb, BLOCK, 0, ctx->stream>>>(s, ctx->ntype, r32, zout, q, zq, rz, P);
     PIC_HIP(ctx, hipGetLastError());
