@@ -867,7 +867,7 @@ static int solve_pcg(pic_ctx* ctx, const pic_solver_params* p, pic_solver_stats*
 // the first and the output z are stored in FP32 (FP64 arithmetic).
 // ------------------------------------------------------------------------
 bool pic_unet_ready(pic_ctx* ctx);
-int pic_unet_coarse_correct(pic_ctx* ctx, const double* t, const float* z, float* c, float* res,
+int pic_unet_coarse_correct(pic_ctx* ctx, const float* t, const float* z, float* c, float* res,
                             double* ws, const double* P);
 
 namespace {
@@ -1173,6 +1173,138 @@ __global__ void __launch_bounds__(BLOCK)
     }
 }
 
+// The four-node forms of k_gs_corr and k_gs_red_dots (ni a multiple of 4):
+// a group of four consecutive nodes of an i-line per thread, its
+// neighbour rows in 16-byte loads, one 16-byte store; the same arithmetic
+// (z equal bit for bit; the dot products sum in another order).
+struct Group4 {
+    unsigned u0, p0;  // first node, its parity (i + j + k) & 1
+    unsigned xm, xp, ym, yp, zm, zp;  // clamped addresses: u0 - 1, u0 + 4, u0 -+ sj, u0 -+ sk
+};
+__device__ inline Group4 group4(const Stencil& s, long long m) {
+    Group4 g;
+    const unsigned n = (unsigned)s.nn, sj = (unsigned)s.sj, sk = (unsigned)s.sk;
+    g.u0 = 4u * (unsigned)m;
+    const unsigned row = g.u0 / (unsigned)s.ni;
+    const unsigned i0 = g.u0 - row * (unsigned)s.ni;
+    const unsigned k = row / (unsigned)s.nj, j = row - k * (unsigned)s.nj;
+    g.p0 = (i0 + j + k) & 1u;
+    g.xm = g.u0 >= 1u ? g.u0 - 1u : g.u0;
+    g.xp = g.u0 + 4u < n ? g.u0 + 4u : g.u0;
+    g.ym = g.u0 >= sj ? g.u0 - sj : g.u0;
+    g.yp = g.u0 + sj < n ? g.u0 + sj : g.u0;
+    g.zm = g.u0 >= sk ? g.u0 - sk : g.u0;
+    g.zp = g.u0 + sk < n ? g.u0 + sk : g.u0;
+    return g;
+}
+// a vector's values around a group: [a] = at u0 + a, xm/xp the i-neighbours
+// of node a, ym/yp/zm/zp the rows on either side
+struct Around4 {
+    float c[4], xm[4], xp[4], ym[4], yp[4], zm[4], zp[4];
+};
+__device__ inline void load_around4(const float* __restrict__ v, const Group4& g, Around4& a) {
+    const float4 c = *reinterpret_cast<const float4*>(v + g.u0);
+    const float l = v[g.xm], r = v[g.xp];
+    const float4 ym = *reinterpret_cast<const float4*>(v + g.ym);
+    const float4 yp = *reinterpret_cast<const float4*>(v + g.yp);
+    const float4 zm = *reinterpret_cast<const float4*>(v + g.zm);
+    const float4 zp = *reinterpret_cast<const float4*>(v + g.zp);
+    const float cc[4] = {c.x, c.y, c.z, c.w};
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+        a.c[q] = cc[q];
+        a.xm[q] = q == 0 ? l : cc[q - 1];
+        a.xp[q] = q == 3 ? r : cc[q + 1];
+    }
+    a.ym[0] = ym.x; a.ym[1] = ym.y; a.ym[2] = ym.z; a.ym[3] = ym.w;
+    a.yp[0] = yp.x; a.yp[1] = yp.y; a.yp[2] = yp.z; a.yp[3] = yp.w;
+    a.zm[0] = zm.x; a.zm[1] = zm.y; a.zm[2] = zm.z; a.zm[3] = zm.w;
+    a.zp[0] = zp.x; a.zp[1] = zp.y; a.zp[2] = zp.z; a.zp[3] = zp.w;
+}
+unsigned group4_blocks(const Stencil& s) {
+    long long b = (s.nn / 4 + BLOCK - 1) / BLOCK;
+    return (unsigned)((b + NXCD - 1) / NXCD * NXCD);
+}
+
+__global__ void __launch_bounds__(BLOCK)
+    k_gs_corr4(Stencil s, const uint8_t* __restrict__ nt, const float* __restrict__ r,
+               const float* __restrict__ zin, const float* __restrict__ c,
+               const double* __restrict__ d, float* __restrict__ z, const double* __restrict__ P) {
+    const double d1 = sget(d + SSIZE);
+    const double om = d1 != 0.0 ? sget(d) / d1 : 0.0;
+    long long end;
+    for (long long m = chunk_begin(s.nn / 4, end) + threadIdx.x; m < end; m += BLOCK) {
+        const Group4 g = group4(s, m);
+        Around4 zi, cc;
+        load_around4(zin, g, zi);
+        load_around4(c, g, cc);
+        const float4 r4 = *reinterpret_cast<const float4*>(r + g.u0);
+        const uint32_t t4 = *reinterpret_cast<const uint32_t*>(nt + g.u0);
+        const float rv[4] = {r4.x, r4.y, r4.z, r4.w};
+        auto zc = [&](float a, float b) { return (double)a + om * b; };
+        float out[4];
+#pragma unroll
+        for (int a = 0; a < 4; a++) {
+            const bool red = ((g.p0 + (unsigned)a) & 1u) == 0u;
+            const bool reg = ((t4 >> (8 * a)) & 0xFFu) == (uint32_t)NT_REG;
+            double zu;
+            if (red)
+                zu = zc(zi.c[a], cc.c[a]);
+            else
+                zu = ((zc(zi.xm[a], cc.xm[a]) + zc(zi.xp[a], cc.xp[a])) * s.cx +
+                      (zc(zi.ym[a], cc.ym[a]) + zc(zi.yp[a], cc.yp[a])) * s.cy +
+                      (zc(zi.zm[a], cc.zm[a]) + zc(zi.zp[a], cc.zp[a])) * s.cz - (double)rv[a]) *
+                     inv_diag(s, P, g.u0 + (unsigned)a);
+            out[a] = reg ? (float)zu : 0.f;
+        }
+        *reinterpret_cast<float4*>(z + g.u0) = make_float4(out[0], out[1], out[2], out[3]);
+    }
+}
+
+__global__ void __launch_bounds__(BLOCK)
+    k_gs_red_dots4(Stencil s, const uint8_t* __restrict__ nt, const float* __restrict__ r,
+                   float* __restrict__ z, const double* __restrict__ q, double* __restrict__ zq,
+                   double* __restrict__ rz, const double* __restrict__ P) {
+    double acc_a = 0, acc_b = 0;
+    long long end;
+    for (long long m = chunk_begin(s.nn / 4, end) + threadIdx.x; m < end; m += BLOCK) {
+        const Group4 g = group4(s, m);
+        Around4 zz;
+        load_around4(z, g, zz);
+        const float4 r4 = *reinterpret_cast<const float4*>(r + g.u0);
+        const uint32_t t4 = *reinterpret_cast<const uint32_t*>(nt + g.u0);
+        const float rv[4] = {r4.x, r4.y, r4.z, r4.w};
+        double qv[4] = {0, 0, 0, 0};
+        if (q) {
+            const double2 q0 = *reinterpret_cast<const double2*>(q + g.u0);
+            const double2 q1 = *reinterpret_cast<const double2*>(q + g.u0 + 2);
+            qv[0] = q0.x; qv[1] = q0.y; qv[2] = q1.x; qv[3] = q1.y;
+        }
+        float out[4];
+#pragma unroll
+        for (int a = 0; a < 4; a++) {
+            out[a] = zz.c[a];
+            const bool red = ((g.p0 + (unsigned)a) & 1u) == 0u;
+            const bool reg = ((t4 >> (8 * a)) & 0xFFu) == (uint32_t)NT_REG;
+            if (red && reg) {
+                const double sum = ((double)zz.xm[a] + (double)zz.xp[a]) * s.cx +
+                                   ((double)zz.ym[a] + (double)zz.yp[a]) * s.cy +
+                                   ((double)zz.zm[a] + (double)zz.zp[a]) * s.cz;
+                out[a] = (float)((sum - (double)rv[a]) * inv_diag(s, P, g.u0 + (unsigned)a));
+            }
+            const double zu = out[a];
+            if (q) acc_a += zu * qv[a];
+            acc_b += zu * (double)rv[a];
+        }
+        *reinterpret_cast<float4*>(z + g.u0) = make_float4(out[0], out[1], out[2], out[3]);
+    }
+    double ta = block_sum(acc_a), tb = block_sum(acc_b);
+    if (threadIdx.x == 0) {
+        if (zq) sadd(zq, ta);
+        sadd(rz, tb);
+    }
+}
+
 // b = a in FP64
 __global__ void __launch_bounds__(BLOCK) k_widen(long long n, const float* __restrict__ a, double* __restrict__ b) {
     for (long long i = (long long)blockIdx.x * BLOCK + threadIdx.x; i < n; i += (long long)gridDim.x * BLOCK)
@@ -1186,9 +1318,7 @@ __global__ void __launch_bounds__(BLOCK) k_widen(long long n, const float* __res
 static void gs_half32(pic_ctx* ctx, const Stencil& s, const float* r32, float* z, int red_colour,
                       const double* P) {
     if (s.ni % 4 == 0) {
-        long long b = (s.nn / 4 + BLOCK - 1) / BLOCK;
-        b = (b + NXCD - 1) / NXCD * NXCD;
-        k_gs_half4<<<(unsigned)b, BLOCK, 0, ctx->stream>>>(s, ctx->ntype, r32, z, red_colour, P);
+        k_gs_half4<<<group4_blocks(s), BLOCK, 0, ctx->stream>>>(s, ctx->ntype, r32, z, red_colour, P);
     } else {
         k_gs_half<float><<<pair_blocks(s), BLOCK, 0, ctx->stream>>>(s, ctx->ntype, r32, z, red_colour, P);
     }
@@ -1228,16 +1358,23 @@ static int learned_precond(pic_ctx* ctx, const Stencil& s, const double* rin, fl
         gs_half32(ctx, s, r32, zpre, 1, P);
         gs_half32(ctx, s, r32, zpre, 0, P);
     }
-    int e = pic_unet_coarse_correct(ctx, rin, zpre, c, res, ws, P);
+    int e = pic_unet_coarse_correct(ctx, r32, zpre, c, res, ws, P);
     if (e) return e;
     // the closing sweeps: the first black half in k_gs_corr, the last red
     // half in k_gs_red_dots, the half-sweeps between them in place
-    k_gs_corr<<<nb, BLOCK, 0, ctx->stream>>>(s, ctx->ntype, r32, zpre, c, dots, zout, P);
+    const bool v4 = s.ni % 4 == 0;  // the four-node kernels
+    if (v4)
+        k_gs_corr4<<<group4_blocks(s), BLOCK, 0, ctx->stream>>>(s, ctx->ntype, r32, zpre, c, dots, zout, P);
+    else
+        k_gs_corr<<<nb, BLOCK, 0, ctx->stream>>>(s, ctx->ntype, r32, zpre, c, dots, zout, P);
     for (int e = 1; e < LEARNED_POST_SWEEPS; e++) {
         gs_half32(ctx, s, r32, zout, 1, P);
         gs_half32(ctx, s, r32, zout, 0, P);
     }
-    k_gs_red_dots<<<nb, BLOCK, 0, ctx->stream>>>(s, ctx->ntype, r32, zout, q, zq, rz, P);
+    if (v4)
+        k_gs_red_dots4<<<group4_blocks(s), BLOCK, 0, ctx->stream>>>(s, ctx->ntype, r32, zout, q, zq, rz, P);
+    else
+        k_gs_red_dots<<<nb, BLOCK, 0, ctx->stream>>>(s, ctx->ntype, r32, zout, q, zq, rz, P);
     PIC_HIP(ctx, hipGetLastError());
     return PIC_OK;
 }

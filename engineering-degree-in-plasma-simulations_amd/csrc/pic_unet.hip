@@ -1428,10 +1428,10 @@ __global__ void __launch_bounds__(256) k_scale_f(long long n, float f, float* __
 // non-regular nodes.  CF 2 loads z once per aggregate (the 2x2x2 cube and
 // its 24 face neighbours, coordinates clamped into the grid; values of
 // non-regular nodes are discarded), so the loop has no branches.
-template <int CF>
+template <int CF, typename TT = double>
 __global__ void __launch_bounds__(256)
     k_res_restrict(Dims df, Dims dc, double h2, const uint8_t* __restrict__ nt,
-                   const double* __restrict__ t, const float* __restrict__ z, float* __restrict__ x2,
+                   const TT* __restrict__ t, const float* __restrict__ z, float* __restrict__ x2,
                    double* __restrict__ sums, float* __restrict__ res, const double* __restrict__ P) {
     double s2 = 0, cnt = 0;
     const unsigned sj = df.nx, sk = (unsigned)df.nx * df.ny;
@@ -1461,7 +1461,7 @@ __global__ void __launch_bounds__(256)
                              zat(a, b, c - 1) + zat(a, b, c + 1) - 6.0 * zat(a, b, c)) *
                             ih2;
                 if (P) az -= P[u] * zat(a, b, c);  // J z, J = A - diag(P)
-                double rv = reg ? t[u] - az : 0.0;
+                double rv = reg ? (double)t[u] - az : 0.0;
                 acc += rv;
                 nreg += reg;
                 if (res) res[u] = (float)rv;
@@ -1477,7 +1477,7 @@ __global__ void __launch_bounds__(256)
                              6.0 * z[u]) *
                             ih2;
                 if (P) az -= P[u] * (double)z[u];
-                double rv = reg ? t[u] - az : 0.0;
+                double rv = reg ? (double)t[u] - az : 0.0;
                 acc += rv;
                 nreg += reg;
                 if (res) res[u] = (float)rv;
@@ -2329,7 +2329,7 @@ bool pic_unet_ready(pic_ctx* ctx) {
 // caller takes the A-norm optimal step omega = (c.res) / (c.A c) for
 // z += omega c.  Only for Dirichlet-walled uniform grids (every regular node
 // is interior).
-int pic_unet_coarse_correct(pic_ctx* ctx, const double* t, const float* z, float* c, float* res,
+int pic_unet_coarse_correct(pic_ctx* ctx, const float* t, const float* z, float* c, float* res,
                             double* ws, const double* P) {
     UNet* u = get_unet(ctx);
     if (!u) return pic_fail(ctx, PIC_ERR_STATE, "no U-Net");
@@ -2341,9 +2341,9 @@ int pic_unet_coarse_correct(pic_ctx* ctx, const double* t, const float* z, float
     double* dots = ws + 2 * SSIZE;
     unsigned nbc = xcd_blocks1(dc.v, 256), nbf = xcd_blocks(df.v, 256);
     if (u->coarsen == 2)
-        k_res_restrict<2><<<nbc, 256, 0, ctx->stream>>>(df, dc, h2, ctx->ntype, t, z, u->x, sums, res, P);
+        k_res_restrict<2, float><<<nbc, 256, 0, ctx->stream>>>(df, dc, h2, ctx->ntype, t, z, u->x, sums, res, P);
     else
-        k_res_restrict<1><<<nbc, 256, 0, ctx->stream>>>(df, dc, h2, ctx->ntype, t, z, u->x, sums, res, P);
+        k_res_restrict<1, float><<<nbc, 256, 0, ctx->stream>>>(df, dc, h2, ctx->ntype, t, z, u->x, sums, res, P);
     int rc = unet_forward(ctx, u, u->x, sums);
     if (rc) return rc;
     if (u->coarsen == 2) {
