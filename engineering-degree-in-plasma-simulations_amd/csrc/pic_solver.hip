@@ -1305,6 +1305,146 @@ __global__ void __launch_bounds__(BLOCK)
     }
 }
 
+// The four-node form of k_presmooth without the Jacobian's diagonal (the
+// plain operator): r and the node types of the group's neighbour rows in
+// 16-byte loads, z and r32 of the group in 16-byte stores; the same
+// arithmetic (equal bit for bit).
+__device__ inline void load_around4d(const double* __restrict__ v, const Group4& g, double (&c)[4],
+                                     double (&xm)[4], double (&xp)[4], double (&ym)[4], double (&yp)[4],
+                                     double (&zm)[4], double (&zp)[4]) {
+    auto row = [&](unsigned a, double (&o)[4]) {
+        const double2 h0 = *reinterpret_cast<const double2*>(v + a);
+        const double2 h1 = *reinterpret_cast<const double2*>(v + a + 2);
+        o[0] = h0.x; o[1] = h0.y; o[2] = h1.x; o[3] = h1.y;
+    };
+    row(g.u0, c);
+    row(g.ym, ym);
+    row(g.yp, yp);
+    row(g.zm, zm);
+    row(g.zp, zp);
+    const double l = v[g.xm], r = v[g.xp];
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+        xm[q] = q == 0 ? l : c[q - 1];
+        xp[q] = q == 3 ? r : c[q + 1];
+    }
+}
+__device__ inline void load_nt4(const uint8_t* __restrict__ nt, const Group4& g, uint32_t (&c)[4],
+                                uint32_t (&xm)[4], uint32_t (&xp)[4], uint32_t (&ym)[4], uint32_t (&yp)[4],
+                                uint32_t (&zm)[4], uint32_t (&zp)[4]) {
+    auto row = [&](unsigned a, uint32_t (&o)[4]) {
+        const uint32_t w = *reinterpret_cast<const uint32_t*>(nt + a);
+#pragma unroll
+        for (int q = 0; q < 4; q++) o[q] = (w >> (8 * q)) & 0xFFu;
+    };
+    row(g.u0, c);
+    row(g.ym, ym);
+    row(g.yp, yp);
+    row(g.zm, zm);
+    row(g.zp, zp);
+    const uint32_t l = nt[g.xm], r = nt[g.xp];
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+        xm[q] = q == 0 ? l : c[q - 1];
+        xp[q] = q == 3 ? r : c[q + 1];
+    }
+}
+
+__global__ void __launch_bounds__(BLOCK)
+    k_presmooth4(Stencil s, const uint8_t* __restrict__ nt, const double* __restrict__ r,
+                 float* __restrict__ z, double* __restrict__ clear, int nclear, float* __restrict__ r32) {
+    for (int q = blockIdx.x * BLOCK + threadIdx.x; q < nclear; q += gridDim.x * BLOCK) clear[q] = 0.0;
+    long long end;
+    for (long long m = chunk_begin(s.nn / 4, end) + threadIdx.x; m < end; m += BLOCK) {
+        const Group4 g = group4(s, m);
+        double rc[4], rxm[4], rxp[4], rym[4], ryp[4], rzm[4], rzp[4];
+        uint32_t tc[4], txm[4], txp[4], tym[4], typ[4], tzm[4], tzp[4];
+        load_around4d(r, g, rc, rxm, rxp, rym, ryp, rzm, rzp);
+        load_nt4(nt, g, tc, txm, txp, tym, typ, tzm, tzp);
+        // r / diag on regular nodes, 0 elsewhere
+        auto rr = [&](double v, uint32_t t) {
+            const double rv = v * s.itwos;
+            return t == (uint32_t)NT_REG ? rv : 0.0;
+        };
+        float out[4], o32[4];
+#pragma unroll
+        for (int a = 0; a < 4; a++) {
+            const bool red = ((g.p0 + (unsigned)a) & 1u) == 0u;
+            double zu;
+            if (red)
+                zu = -rc[a] * s.itwos;
+            else
+                zu = (-((rr(rxm[a], txm[a]) + rr(rxp[a], txp[a])) * s.cx +
+                        (rr(rym[a], tym[a]) + rr(ryp[a], typ[a])) * s.cy +
+                        (rr(rzm[a], tzm[a]) + rr(rzp[a], tzp[a])) * s.cz) -
+                      rc[a]) *
+                     s.itwos;
+            out[a] = tc[a] == (uint32_t)NT_REG ? (float)zu : 0.f;
+            o32[a] = (float)rc[a];
+        }
+        *reinterpret_cast<float4*>(z + g.u0) = make_float4(out[0], out[1], out[2], out[3]);
+        *reinterpret_cast<float4*>(r32 + g.u0) = make_float4(o32[0], o32[1], o32[2], o32[3]);
+    }
+}
+
+// The four-node form of k_fcg_pq (ni a multiple of 4): z's neighbour rows,
+// p_old, q and P in 16-byte loads, p_new and q in 16-byte stores; the same
+// arithmetic per node (p and q equal bit for bit; pq sums in another order).
+__global__ void __launch_bounds__(BLOCK)
+    k_fcg_pq4(Stencil s, const uint8_t* __restrict__ nt, const float* __restrict__ z,
+              const double* __restrict__ pold, double* __restrict__ pnew, double* __restrict__ q,
+              FcgHist h, int k, const double* __restrict__ P) {
+    double beta = 0.0;
+    if (k > 0) {
+        double den = sget(h.pq + (k - 1) * SSIZE);
+        beta = den != 0.0 ? -sget(h.zq + (k - 1) * SSIZE) / den : 0.0;
+    }
+    const bool first = k == 0;
+    double acc = 0;
+    auto ld4 = [&](const double* v, unsigned a, double (&o)[4]) {
+        const double2 h0 = *reinterpret_cast<const double2*>(v + a);
+        const double2 h1 = *reinterpret_cast<const double2*>(v + a + 2);
+        o[0] = h0.x; o[1] = h0.y; o[2] = h1.x; o[3] = h1.y;
+    };
+    long long end;
+    for (long long m = chunk_begin(s.nn / 4, end) + threadIdx.x; m < end; m += BLOCK) {
+        const Group4 g = group4(s, m);
+        Around4 zz;
+        load_around4(z, g, zz);
+        const uint32_t t4 = *reinterpret_cast<const uint32_t*>(nt + g.u0);
+        double po[4] = {0, 0, 0, 0}, qo[4] = {0, 0, 0, 0}, pv[4] = {0, 0, 0, 0};
+        if (!first) {
+            ld4(pold, g.u0, po);
+            ld4(q, g.u0, qo);
+        }
+        if (P) ld4(P, g.u0, pv);
+        double pn[4], qn[4];
+#pragma unroll
+        for (int a = 0; a < 4; a++) {
+            const bool reg = ((t4 >> (8 * a)) & 0xFFu) == (uint32_t)NT_REG;
+            const double zu = zz.c[a];
+            double az = ((double)zz.zm[a] + zz.zp[a]) * s.cz + ((double)zz.ym[a] + zz.yp[a]) * s.cy +
+                        ((double)zz.xm[a] + zz.xp[a]) * s.cx - s.twos * zu;
+            if (P) az -= pv[a] * zu;
+            double pu = zu, qv = az;
+            if (!first) {
+                pu += beta * po[a];
+                qv += beta * qo[a];
+            }
+            qv = reg ? qv : 0.0;
+            pn[a] = pu;
+            qn[a] = qv;
+            acc += pu * qv;
+        }
+        *reinterpret_cast<double2*>(pnew + g.u0) = make_double2(pn[0], pn[1]);
+        *reinterpret_cast<double2*>(pnew + g.u0 + 2) = make_double2(pn[2], pn[3]);
+        *reinterpret_cast<double2*>(q + g.u0) = make_double2(qn[0], qn[1]);
+        *reinterpret_cast<double2*>(q + g.u0 + 2) = make_double2(qn[2], qn[3]);
+    }
+    double a = block_sum(acc);
+    if (threadIdx.x == 0) sadd(h.pq + k * SSIZE, a);
+}
+
 // b = a in FP64
 __global__ void __launch_bounds__(BLOCK) k_widen(long long n, const float* __restrict__ a, double* __restrict__ b) {
     for (long long i = (long long)blockIdx.x * BLOCK + threadIdx.x; i < n; i += (long long)gridDim.x * BLOCK)
@@ -1353,7 +1493,11 @@ static int learned_precond(pic_ctx* ctx, const Stencil& s, const double* rin, fl
     float* r32 = res + s.nn;
     double* ws = ctx->scratch + LEARNED_WS;  // the coarse correction's striped sums
     double* dots = ws + 2 * SSIZE;
-    k_presmooth<<<nb, BLOCK, 0, ctx->stream>>>(s, ctx->ntype, rin, zpre, ws, 4 * SSIZE, P, r32);
+    const bool v4 = s.ni % 4 == 0;  // the four-node kernels
+    if (v4 && !P)
+        k_presmooth4<<<group4_blocks(s), BLOCK, 0, ctx->stream>>>(s, ctx->ntype, rin, zpre, ws, 4 * SSIZE, r32);
+    else
+        k_presmooth<<<nb, BLOCK, 0, ctx->stream>>>(s, ctx->ntype, rin, zpre, ws, 4 * SSIZE, P, r32);
     for (int e = 1; e < LEARNED_PRE_SWEEPS; e++) {
         gs_half32(ctx, s, r32, zpre, 1, P);
         gs_half32(ctx, s, r32, zpre, 0, P);
@@ -1362,7 +1506,6 @@ static int learned_precond(pic_ctx* ctx, const Stencil& s, const double* rin, fl
     if (e) return e;
     // the closing sweeps: the first black half in k_gs_corr, the last red
     // half in k_gs_red_dots, the half-sweeps between them in place
-    const bool v4 = s.ni % 4 == 0;  // the four-node kernels
     if (v4)
         k_gs_corr4<<<group4_blocks(s), BLOCK, 0, ctx->stream>>>(s, ctx->ntype, r32, zpre, c, dots, zout, P);
     else
@@ -1499,7 +1642,11 @@ static int solve_learned(pic_ctx* ctx, const pic_solver_params* p, pic_solver_st
         for (int k = 0; k < max_fcg; k++) {
             {
                 PicProfScope ps(ctx, PIC_K_PCG);
-                k_fcg_pq<<<nbr, BLOCK, 0, ctx->stream>>>(s, ctx->ntype, z, pold, pnew, q, h, k, nullptr);
+                if (s.ni % 4 == 0)
+                    k_fcg_pq4<<<group4_blocks(s), BLOCK, 0, ctx->stream>>>(s, ctx->ntype, z, pold, pnew, q, h, k,
+                                                                          nullptr);
+                else
+                    k_fcg_pq<<<nbr, BLOCK, 0, ctx->stream>>>(s, ctx->ntype, z, pold, pnew, q, h, k, nullptr);
                 k_fcg_update<<<nb, BLOCK, 0, ctx->stream>>>(s.nn, h, k, ctx->phi, pnew, r, q);
             }
             PIC_HIP(ctx, hipGetLastError());
@@ -1649,7 +1796,10 @@ static int solve_newton_learned(pic_ctx* ctx, const pic_solver_params* p, pic_so
         for (int k = 0; k < max_fcg; k++) {
             {
                 PicProfScope ps(ctx, PIC_K_PCG);
-                k_fcg_pq<<<nbr, BLOCK, 0, ctx->stream>>>(s, ctx->ntype, z, pold, pnew, q, h, k, P);
+                if (s.ni % 4 == 0)
+                    k_fcg_pq4<<<group4_blocks(s), BLOCK, 0, ctx->stream>>>(s, ctx->ntype, z, pold, pnew, q, h, k, P);
+                else
+                    k_fcg_pq<<<nbr, BLOCK, 0, ctx->stream>>>(s, ctx->ntype, z, pold, pnew, q, h, k, P);
                 k_fcg_update<<<nb, BLOCK, 0, ctx->stream>>>(s.nn, h, k, y, pnew, r, q);
             }
             PIC_HIP(ctx, hipGetLastError());
