@@ -1499,6 +1499,91 @@ __global__ void __launch_bounds__(256)
 }
 
 
+// k_res_restrict<2, float> with even fine extents (the fine grid exactly twice
+// the network grid): each x-row of an aggregate's neighbourhood comes in
+// 8-byte pieces (z at 2cx, 2cx+1 and the pieces on either side for the
+// i-neighbours; r, the node types, P and res per row likewise); the same
+// arithmetic per node (equal bit for bit).
+__global__ void __launch_bounds__(256)
+    k_res_restrict2v(Dims df, Dims dc, double h2, const uint8_t* __restrict__ nt,
+                     const float* __restrict__ t, const float* __restrict__ z, float* __restrict__ x2,
+                     double* __restrict__ sums, float* __restrict__ res, const double* __restrict__ P) {
+    double s2 = 0, cnt = 0;
+    const unsigned sj = df.nx, sk = (unsigned)df.nx * df.ny;
+    const double ih2 = 1.0 / h2;
+    long long end;
+    for (long long v = xcd_chunk(dc.v, 256, end) + threadIdx.x; v < end; v += 256) {
+        int cx, cy, cz;
+        vox_xyz(dc, v, cx, cy, cz);
+        const unsigned X1 = 2u * (unsigned)cx;
+        unsigned Y[4], Z[4];
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            Y[q] = (unsigned)min(max(2 * cy - 1 + q, 0), df.ny - 1) * sj;
+            Z[q] = (unsigned)min(max(2 * cz - 1 + q, 0), df.nz - 1) * sk;
+        }
+        auto pair = [&](const float* a, unsigned o) { return *reinterpret_cast<const float2*>(a + o); };
+        // V[c][b][a]: z at fine (X[a], Y[b], Z[c]) with X = clamp(2cx - 1 + a)
+        float V[4][4][4];
+#pragma unroll
+        for (int c = 0; c < 4; c++)
+#pragma unroll
+            for (int b = 0; b < 4; b++) {
+                const bool full = (b == 1 || b == 2) && (c == 1 || c == 2);
+                const bool side = !full && ((b == 1 || b == 2) || (c == 1 || c == 2)) &&
+                                  !((b == 0 || b == 3) && (c == 0 || c == 3));
+                if (!full && !side) continue;
+                const unsigned row = Z[c] + Y[b];
+                const float2 m = pair(z, row + X1);
+                V[c][b][1] = m.x;
+                V[c][b][2] = m.y;
+                if (full) {
+                    V[c][b][0] = X1 >= 2u ? pair(z, row + X1 - 2u).y : m.x;
+                    V[c][b][3] = X1 + 2u < (unsigned)df.nx ? pair(z, row + X1 + 2u).x : m.y;
+                }
+            }
+        double acc = 0;
+        int nreg = 0;
+#pragma unroll
+        for (int c = 1; c <= 2; c++)
+#pragma unroll
+            for (int b = 1; b <= 2; b++) {
+                const unsigned row = Z[c] + Y[b];
+                const float2 tv = pair(t, row + X1);
+                const unsigned short t2 = *reinterpret_cast<const unsigned short*>(nt + row + X1);
+                double2 pv = make_double2(0.0, 0.0);
+                if (P) pv = *reinterpret_cast<const double2*>(P + row + X1);
+                float ro[2];
+#pragma unroll
+                for (int a = 1; a <= 2; a++) {
+                    auto zat = [&](int aa, int bb, int cc) { return (double)V[cc][bb][aa]; };
+                    const bool reg = ((t2 >> (8 * (a - 1))) & 0xFFu) == (unsigned)NT_REG;
+                    double az = (zat(a - 1, b, c) + zat(a + 1, b, c) + zat(a, b - 1, c) + zat(a, b + 1, c) +
+                                 zat(a, b, c - 1) + zat(a, b, c + 1) - 6.0 * zat(a, b, c)) *
+                                ih2;
+                    if (P) az -= (a == 1 ? pv.x : pv.y) * zat(a, b, c);
+                    const double rv = reg ? (double)(a == 1 ? tv.x : tv.y) - az : 0.0;
+                    acc += rv;
+                    nreg += reg;
+                    ro[a - 1] = (float)rv;
+                }
+                if (res) *reinterpret_cast<float2*>(res + row + X1) = make_float2(ro[0], ro[1]);
+            }
+        float xv = nreg ? (float)(-h2 * acc / nreg) : 0.f;
+        x2[2 * v] = xv;
+        x2[2 * v + 1] = 1.f - (float)nreg / 8.0f;
+        if (nreg) {
+            s2 += (double)xv * xv;
+            cnt += 1;
+        }
+    }
+    block_sum2(s2, cnt);
+    if (threadIdx.x == 0) {
+        sadd(sums, s2);
+        sadd(sums + SSIZE, cnt);
+    }
+}
+
 // (P y) at node (i, j, k): identity (CF 1) or trilinear from aggregate
 // centres (CF 2: per axis weight 3/4 on the own aggregate, 1/4 on the
 // neighbour on the node's side, the own one again at the grid's ends)
@@ -2341,7 +2426,11 @@ int pic_unet_coarse_correct(pic_ctx* ctx, const float* t, const float* z, float*
     double* dots = ws + 2 * SSIZE;
     unsigned nbc = xcd_blocks1(dc.v, 256), nbf = xcd_blocks(df.v, 256);
     if (u->coarsen == 2)
-        k_res_restrict<2, float><<<nbc, 256, 0, ctx->stream>>>(df, dc, h2, ctx->ntype, t, z, u->x, sums, res, P);
+        if (g.ni == 2 * dc.nx && g.nj == 2 * dc.ny && g.nk == 2 * dc.nz)
+            k_res_restrict2v<<<nbc, 256, 0, ctx->stream>>>(df, dc, h2, ctx->ntype, t, z, u->x, sums, res, P);
+        else
+            k_res_restrict<2, float><<<nbc, 256, 0, ctx->stream>>>(df, dc, h2, ctx->ntype, t, z, u->x, sums, res,
+                                                                   P);
     else
         k_res_restrict<1, float><<<nbc, 256, 0, ctx->stream>>>(df, dc, h2, ctx->ntype, t, z, u->x, sums, res, P);
     int rc = unet_forward(ctx, u, u->x, sums);
