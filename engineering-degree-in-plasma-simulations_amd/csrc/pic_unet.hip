@@ -1667,6 +1667,10 @@ __global__ void __launch_bounds__(256)
 // aggregate values the thread gets c at fine offsets {0,1,2}^3: its own 8
 // nodes and the forward neighbours of the edge form of c.A c, which with c
 // zero off the regular nodes is -(1/h^2) sum over grid edges (c_u - c_v)^2.
+// EVEN (the fine grid exactly twice the network grid): the node types, res,
+// P and c of each fine x-row of the aggregate in 2- and 8-byte pieces; the
+// same arithmetic (equal bit for bit).
+template <bool EVEN>
 __global__ void __launch_bounds__(256)
     k_prolong_dots_agg(Dims df, Dims dc, double ih2, const uint8_t* __restrict__ nt,
                        const float* __restrict__ y, const float* __restrict__ res,
@@ -1711,7 +1715,17 @@ __global__ void __launch_bounds__(256)
 #pragma unroll
         for (int cc = 0; cc < 3; cc++)
 #pragma unroll
-            for (int b = 0; b < 3; b++)
+            for (int b = 0; b < 3; b++) {
+                if (b == 2 && cc == 2) continue;  // not needed
+                // EVEN: the row's node types at x offsets 0, 1 (one 2-byte load) and 2
+                const int fy = fy0 + b, fz = fz0 + cc;
+                const bool row_in = fy < df.ny && fz < df.nz;
+                const unsigned rowa = (unsigned)fz * sk + (unsigned)fy * sj + (unsigned)fx0;
+                unsigned n01 = 0xFFFFu, n2 = 0xFFu;
+                if (EVEN && row_in) {
+                    n01 = *reinterpret_cast<const unsigned short*>(nt + rowa);
+                    if (b < 2 && cc < 2 && fx0 + 2 < df.nx) n2 = nt[rowa + 2];
+                }
 #pragma unroll
                 for (int a = 0; a < 3; a++) {
                     const int nfwd = (a == 2) + (b == 2) + (cc == 2);
@@ -1720,28 +1734,48 @@ __global__ void __launch_bounds__(256)
                     if (cc == 0) w = 0.75f * B[1][b][a] + 0.25f * B[0][b][a];
                     else if (cc == 1) w = 0.75f * B[1][b][a] + 0.25f * B[2][b][a];
                     else w = 0.75f * B[2][b][a] + 0.25f * B[1][b][a];
-                    const int fx = fx0 + a, fy = fy0 + b, fz = fz0 + cc;
+                    const int fx = fx0 + a;
                     bool reg = false;
-                    if (fx < df.nx && fy < df.ny && fz < df.nz)
-                        reg = nt[(unsigned)fz * sk + (unsigned)fy * sj + (unsigned)fx] == NT_REG;
+                    if (EVEN) {
+                        const unsigned t = a < 2 ? (n01 >> (8 * a)) & 0xFFu : n2;
+                        reg = t == (unsigned)NT_REG;
+                    } else if (fx < df.nx && row_in) {
+                        reg = nt[rowa + (unsigned)a] == NT_REG;
+                    }
                     C[cc][b][a] = reg ? (double)w : 0.0;
                 }
+            }
         double e = 0.0;
 #pragma unroll
-        for (int q = 0; q < 8; q++) {
-            const int a = q & 1, b = (q >> 1) & 1, cc = q >> 2;
-            const int fx = fx0 + a, fy = fy0 + b, fz = fz0 + cc;
-            if (fx >= df.nx || fy >= df.ny || fz >= df.nz) continue;
-            const unsigned u = (unsigned)fz * sk + (unsigned)fy * sj + (unsigned)fx;
-            const double cu = C[cc][b][a];
-            c[u] = (float)cu;  // exact: interpolated in FP32
-            const double rv = res[u];
-            acc0 += cu != 0.0 ? cu * rv : 0.0;  // res is defined on regular nodes only
-            if (P && cu != 0.0) accp += P[u] * cu * cu;
-            if (fx + 1 < df.nx) e += (cu - C[cc][b][a + 1]) * (cu - C[cc][b][a + 1]);
-            if (fy + 1 < df.ny) e += (cu - C[cc][b + 1][a]) * (cu - C[cc][b + 1][a]);
-            if (fz + 1 < df.nz) e += (cu - C[cc + 1][b][a]) * (cu - C[cc + 1][b][a]);
-        }
+        for (int cc = 0; cc < 2; cc++)
+#pragma unroll
+            for (int b = 0; b < 2; b++) {
+                const int fy = fy0 + b, fz = fz0 + cc;
+                const unsigned rowa = (unsigned)fz * sk + (unsigned)fy * sj + (unsigned)fx0;
+                float2 r2 = make_float2(0.f, 0.f);
+                double2 p2 = make_double2(0.0, 0.0);
+                if (EVEN) {
+                    r2 = *reinterpret_cast<const float2*>(res + rowa);
+                    if (P) p2 = *reinterpret_cast<const double2*>(P + rowa);
+                }
+                float co[2];
+#pragma unroll
+                for (int a = 0; a < 2; a++) {
+                    const int fx = fx0 + a;
+                    if (!EVEN && (fx >= df.nx || fy >= df.ny || fz >= df.nz)) continue;
+                    const unsigned u = rowa + (unsigned)a;
+                    const double cu = C[cc][b][a];
+                    co[a] = (float)cu;  // exact: interpolated in FP32
+                    if (!EVEN) c[u] = co[a];
+                    const double rv = EVEN ? (double)(a == 0 ? r2.x : r2.y) : (double)res[u];
+                    acc0 += cu != 0.0 ? cu * rv : 0.0;  // res is defined on regular nodes only
+                    if (P && cu != 0.0) accp += (EVEN ? (a == 0 ? p2.x : p2.y) : P[u]) * cu * cu;
+                    if (fx + 1 < df.nx) e += (cu - C[cc][b][a + 1]) * (cu - C[cc][b][a + 1]);
+                    if (fy + 1 < df.ny) e += (cu - C[cc][b + 1][a]) * (cu - C[cc][b + 1][a]);
+                    if (fz + 1 < df.nz) e += (cu - C[cc + 1][b][a]) * (cu - C[cc + 1][b][a]);
+                }
+                if (EVEN) *reinterpret_cast<float2*>(c + rowa) = make_float2(co[0], co[1]);
+            }
         acc1 -= e * ih2;
     }
     acc1 -= accp;  // c.J c = c.A c - sum P c^2 (the Newton Jacobian)
@@ -2437,8 +2471,12 @@ int pic_unet_coarse_correct(pic_ctx* ctx, const float* t, const float* z, float*
     if (rc) return rc;
     if (u->coarsen == 2) {
         long long na = (long long)((g.ni + 1) / 2) * ((g.nj + 1) / 2) * ((g.nk + 1) / 2);
-        k_prolong_dots_agg<<<xcd_blocks1(na, 256), 256, 0, ctx->stream>>>(df, dc, 1.0 / h2, ctx->ntype,
-                                                                          u->y, res, c, dots, P);
+        if (g.ni == 2 * dc.nx && g.nj == 2 * dc.ny && g.nk == 2 * dc.nz)
+            k_prolong_dots_agg<true><<<xcd_blocks1(na, 256), 256, 0, ctx->stream>>>(df, dc, 1.0 / h2, ctx->ntype,
+                                                                                    u->y, res, c, dots, P);
+        else
+            k_prolong_dots_agg<false><<<xcd_blocks1(na, 256), 256, 0, ctx->stream>>>(
+                df, dc, 1.0 / h2, ctx->ntype, u->y, res, c, dots, P);
     } else
         k_prolong_dots<1><<<nbf, 256, 0, ctx->stream>>>(df, dc, 1.0 / h2, ctx->ntype, u->y, res, c, dots,
                                                         P);
