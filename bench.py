@@ -48,9 +48,9 @@ ALG_BYTES = {
     "pcg": 44.0,       # FP32 inner PCG: per node and iteration (88 B in FP64)
     "sor": 24.0,       # FP64 phi r/w + rho read per node and sweep
 }
-# the learned solver's flexible-CG vector updates per node and iteration (FP64):
-# p, q from z and the previous p (32 B), then x, r updated from p, q (48 B)
-ALG_BYTES_FCG = 80.0
+# the learned solver's flexible-CG vector updates per node and iteration: p, q
+# (FP64) from z (FP32) and the previous p (28 B), then x, r updated from p, q (48 B)
+ALG_BYTES_FCG = 76.0
 KERNEL_NAMES = {
     "push": "k_push<false,2>", "scatter": "k_push_scatter", "deposit": "k_dep_cells + k_dep_nodes",
     "pcg": "k_pcg1<float> + k_pcg2<float>", "sor": "k_sor_split (one launch per colour)",
