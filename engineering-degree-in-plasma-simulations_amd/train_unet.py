@@ -6,8 +6,9 @@
 
 Each round generates fresh training pairs on the device: a residual t
 (random Gaussian blobs plus white noise, the particle-noise component of a
-PIC charge density), what one red-black Gauss-Seidel sweep leaves of it
-(restricted to the network grid: the input) and the exact error of that
+PIC charge density), what the cycle's pre-smoothing (three red-black
+Gauss-Seidel sweeps) leaves of it (restricted to the network grid: the
+input) and the exact error of that
 remainder from the plain red-black SOR solver (restricted: the target); then
 Adam runs over them.  The weights (FP32, the
 parameter order of pic_unet_get_params) go to data/unet_<nodes>.f32 with a
