@@ -321,7 +321,7 @@ def test_c4_shipped_network_cold_solve_128cubed(pg):
     pairs) solved cold to a 1e-6 relative residual in at most 10 flexible-CG
     iterations (8 measured), checked by the oracle; the same cycle with a
     zero network needs at least twice as many (93 measured).  A short
-    on-device training run at 128^3 cuts its loss by 40 % (5.8x measured)."""
+    on-device training run at 128^3 cuts its loss by 40 % (16x measured)."""
     n = 128
     c, g, L = _box(pg, n)
     meta = c.unet_load(os.path.join(PKG, "data", "unet_128.f32"))
